@@ -1,0 +1,32 @@
+"""Loading the golden vectors of tests/golden/ (written from the reference by tests/golden/make_golden.py)."""
+import json
+import os
+
+import numpy as np
+import torch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+CASES = ["case_a_shared64", "case_a_percol64", "case_b_jitter64", "case_c_rank0", "case_e_tall"]
+
+
+def load(name):
+    data = np.load(os.path.join(GOLDEN, name + ".npz"))
+    config = json.loads(str(data["config_json"]))
+    state = {k[2:]: torch.from_numpy(np.array(data[k])) for k in data.files if k.startswith("w:")}
+    return data, config, state
+
+
+def regenerate_state(config, seed, mean_x, std_x, mean_y, std_y):
+    """The weights of a case stored without them: ffno_amd.synthetic's generator, as make_golden.py used it."""
+    from ffno_amd import synthetic
+    model = synthetic.random_model(seed=seed, **config)
+    model.set_normalization(mean_x, std_x, mean_y, std_y)
+    return {k: v.detach().clone() for k, v in model.state_dict().items()}
+
+
+def oracle_config(config):
+    from oracle import ffno_oracle
+    keys = ("width", "n_layers", "hidden_spectral", "hidden_vertical", "freq_mix_rank", "z_reference_length",
+            "z_min_wavelength", "z_feature_wavelength", "species", "in_channels", "out_channels")
+    return ffno_oracle.model_config(**{k: config[k] for k in keys if k in config})

@@ -1,0 +1,234 @@
+"""The CPU oracle (oracle/ffno_oracle.py) pinned to the reference: golden vectors written from the reference's own
+code, and closed forms for the parts that have them. Tolerances are stated per check."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from golden_cases import CASES, GOLDEN, load, oracle_config, regenerate_state
+from oracle import ffno_oracle as O
+
+
+def _run(name, per_column=False):
+    data, config, state = load(name)
+    cfg = oracle_config(config)
+    sd = O.to_dtype(state, torch.float64)
+    x = torch.from_numpy(data["x"]).double()
+    z = torch.from_numpy(data["z"]).double()
+    ct = torch.from_numpy(data["cotangent"]).double()
+    y, xb, zb = O.vjp(sd, cfg, x, z, 48e3, 48e3, ct, per_column)
+    return data, y, xb, zb
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_golden_forward_and_gradients(name):
+    """float64 oracle vs float64 reference: forward to 1e-10 of its range, x- and z-gradients to 1e-8 relative."""
+    data, y, xb, zb = _run(name)
+    assert _rel(y.numpy(), data["y"]) < 1e-10
+    assert _rel(xb.numpy(), data["xbar"]) < 1e-8
+    assert _rel(zb.numpy(), data["zbar"]) < 1e-8
+
+
+def test_golden_float32():
+    """float32 oracle vs float32 reference, same weights and inputs: 2e-5 of the output range."""
+    data, config, state = load("case_a_shared32")
+    cfg = oracle_config(config)
+    y = O.forward(O.to_dtype(state, torch.float32), cfg, torch.from_numpy(data["x"]), torch.from_numpy(data["z"]),
+                  48e3, 48e3)
+    assert _rel(y.numpy(), data["y"]) < 2e-5
+
+
+def test_golden_submodules():
+    """The horizontal and vertical branches and block 0 of case A, against the reference's module outputs (stored
+    as float32): 1e-5 of their range."""
+    data, config, state = load("case_a_shared64")
+    cfg = oracle_config(config)
+    sd = O.to_dtype(state, torch.float64)
+    x = torch.from_numpy(data["x"]).double()
+    z = torch.from_numpy(data["z"]).double()
+    w = O.depth_weights(z)
+    h = O.gelu(O.weighted_group_norm(O.conv1x1(x, sd["lift.0.weight"]), w, sd["lift.1.weight"], sd["lift.1.bias"],
+                                     cfg["groups"]))
+    spec = O.horizontal_branch(sd, "blocks.0.spec.", h, 48e3, 48e3, cfg)
+    vert = O.vertical_branch(sd, "blocks.0.vertical.", h, z, cfg)
+    blk = O.block(sd, 0, h, z, 48e3, 48e3, cfg)
+    assert _rel(spec.numpy(), data["sub:blocks.0.spec"]) < 1e-5
+    assert _rel(vert.numpy(), data["sub:blocks.0.vertical"]) < 1e-5
+    assert _rel(blk.numpy(), data["sub:blocks.0"]) < 1e-5
+
+
+def test_golden_cfg0_workload():
+    """The CPU workload (width 32, 4 layers, 64x32x32, 12 vertical modes): weights and inputs regenerate
+    bit-for-bit (sha256 of the state), and the oracle matches the stored subsample and the full-tensor sums."""
+    from ffno_amd import synthetic
+    data = np.load(os.path.join(GOLDEN, "case_f_cfg0.npz"))
+    config = json.loads(str(data["config_json"]))
+    x, z, mean_x, std_x = synthetic.atmosphere(1, 64, 32, 32, seed=0)
+    state = regenerate_state(config, 0, mean_x, std_x, np.zeros(12), np.ones(12))
+    import hashlib
+    h = hashlib.sha256()
+    for k in sorted(state):
+        h.update(k.encode())
+        h.update(np.ascontiguousarray(state[k].double().numpy()).tobytes())
+    assert h.hexdigest() == str(data["state_sha256"])
+    cfg = oracle_config(config)
+    sd = O.to_dtype(state, torch.float64)
+    y, xb, zb = O.vjp(sd, cfg, x.double(), z.double(), 48e3, 48e3,
+                      synthetic.cotangent((1, 12, 64, 32, 32), 0).double())
+    sub = (slice(None), slice(None), slice(None, None, 4), slice(None, None, 4), slice(None, None, 4))
+    assert _rel(y.numpy()[sub], data["y"]) < 1e-10
+    assert abs(float(y.sum()) - float(data["y:sum"])) < 1e-8 * float(data["y:abssum"])
+    assert _rel(xb.numpy()[sub], data["xbar"]) < 1e-8
+    assert abs(float(zb.sum()) - float(data["zbar:sum"])) < 1e-8 * float(data["zbar:abssum"])
+
+
+def test_planted_mistake_is_caught():
+    """A plausible slip in the oracle (the imaginary spectral weight transposed) fails the golden comparison."""
+    data, config, state = load("case_a_shared64")
+    cfg = oracle_config(config)
+    sd = O.to_dtype(state, torch.float64)
+    for k in list(sd):
+        if k.endswith("spec.weight_imag"):
+            sd[k] = sd[k].t().contiguous()
+    y = O.forward(sd, cfg, torch.from_numpy(data["x"]).double(), torch.from_numpy(data["z"]).double(), 48e3, 48e3)
+    assert _rel(y.numpy(), data["y"]) > 1e-4
+
+
+def test_atom_tables_are_the_references():
+    from ffno_amd.atoms import ATOMS
+    with open(os.path.join(GOLDEN, "atoms.json")) as f:
+        ref = json.load(f)
+    for a, entry in ref.items():
+        assert ATOMS[a]["levels"] == entry["levels"]
+        assert abs(ATOMS[a]["log10_particle_mass"] - entry["log10_particle_mass"]) < 1e-12
+        assert ATOMS[a]["multi3d_lte"] == entry["multi3d_lte"]
+
+
+# ------------------------------------------------------------------------------------------------------ closed forms
+def _stretched(depth):
+    s = torch.linspace(0.0, 1.0, depth, dtype=torch.float64)
+    return (-1.0 + 15.0 * (0.4 * s + 0.6 * s ** 2)).flip(0)
+
+
+def test_filon_weights_closed_form():
+    """Exact integral of a linear profile at every k; trapezoid weights at k = 0; flip symmetry (the properties of
+    tests/test_height_operator.py:122-137)."""
+    z = _stretched(30)
+    k = torch.tensor([0.0, 0.7, 5.0, 40.0], dtype=torch.float64)
+    q_re, q_im = O.height_fourier_weights(z, k)
+    trap = O.trapezoid_weights(z[None, None, :, None, None])[0, 0, :, 0, 0]
+    assert (q_re[:, 0] - trap).abs().max() < 1e-12
+    assert q_im[:, 0].abs().max() < 1e-12
+    x = 2.0 - 0.3 * z
+    lo, hi = float(z.min()), float(z.max())
+    for m, km in enumerate(k[1:].tolist(), start=1):
+        f = lambda zz: np.exp(-1j * km * zz) * ((2.0 - 0.3 * zz) / (-1j * km) - (-0.3) / (-1j * km) ** 2)
+        exact = f(hi) - f(lo)
+        got = complex(float((x * q_re[:, m]).sum()), float((x * q_im[:, m]).sum()))
+        assert abs(got - exact) < 1e-9 * max(1.0, abs(exact))
+    fr, fi = O.height_fourier_weights(z.flip(0), k)
+    assert (fr.flip(0) - q_re).abs().max() < 1e-12
+    assert (fi.flip(0) - q_im).abs().max() < 1e-12
+
+
+def test_weighted_group_norm_is_group_norm_at_unit_weights():
+    torch.manual_seed(2)
+    x = torch.randn(2, 16, 7, 3, 4, dtype=torch.float64)
+    weight, bias = 1 + 0.1 * torch.randn(16, dtype=torch.float64), 0.1 * torch.randn(16, dtype=torch.float64)
+    plain = torch.nn.functional.group_norm(x, 4, weight, bias, 1e-5)
+    got = O.weighted_group_norm(x, torch.ones(2, 1, 7, 3, 4, dtype=torch.float64), weight, bias, 4)
+    assert (got - plain).abs().max() < 1e-12
+
+
+def test_box_and_taper():
+    """L = 30 and 61 modes for a 15 Mm column at (L_ref 20, lambda_min 0.5); taper 1 for wavelengths >= 3 dz and
+    < 1e-6 below 2 dz (tests/test_height_operator.py:95-102, :148-157)."""
+    cfg = O.model_config(width=4, z_reference_length=20.0, z_min_wavelength=0.5, species=[dict(levels=1,
+                         log10_particle_mass=0.0, multi3d_lte=dict(g=[1.0], ev=[0.0], ion=[1]))])
+    z = torch.linspace(14.0, -1.0, 50)[None, :, None, None].expand(1, 50, 3, 2)
+    length, k, factor = O.vertical_box(z.unsqueeze(1), cfg, torch.float32)
+    assert abs(length - 30.0) < 1e-4 and k.numel() == 61 and float(k.max()) <= 2 * math.pi / 0.5 + 1e-5
+    assert float(factor[0]) == 1.0 and bool((factor[1:] == 2).all())
+    kk = 2 * math.pi * torch.arange(1001, dtype=torch.float64) / 40.0
+    for dz in (0.012, 0.02, 0.025, 0.164):
+        col = torch.arange(0, 14.0, dz, dtype=torch.float64)
+        taper, wl = O.column_taper(col, kk), 2 * np.pi / kk[1:]
+        assert bool((taper[1:][wl >= 3 * dz * 1.001] == 1).all())
+        assert bool((taper[1:][wl <= 2 * dz] < 1e-6).all())
+        assert float(taper[0]) == 1.0
+
+
+def _calc_lte_fractions(t, ne_cm3, atom):
+    """MULTI3D calc_lte_populations, written independently in numpy float64: n*_i / n_total per level, from the
+    Saha-Boltzmann weights with Debye lowering of the ionisation energies."""
+    ee, hh, em, bk, pi = 1.602189e-12, 6.626176e-27, 9.109534e-28, 1.380662e-16, 3.14159265359
+    g, ev, ion = atom["g"], atom["ev"], atom["ion"]
+    nd = {i: i * (i - 1) // 2 for i in range(1, 21)}        # Debye multiple of stage i: 0, 1, 3, 6, ...
+    # MULTI3D's loop subtracts ndxi(ion_1) while zeroing it: stages above ion_1 keep their value
+    shifted = {i: (nd[i] - nd[ion[0]] if i <= ion[0] else nd[i]) for i in nd}
+    ccon = 0.5 * (hh / math.sqrt(2.0 * pi * em) / math.sqrt(bk)) ** 3
+    conl = np.log(ccon * ne_cm3) - 1.5 * np.log(t)
+    dxi = 4.98e-4 * 5040.0 / t * np.sqrt(ne_cm3 * bk * t)
+    logw = []
+    for i in range(len(g)):
+        e = ev[i] - dxi * shifted[ion[i]]
+        lw = math.log(g[i] / g[0]) - (ee / bk) / t * e - (ion[i] - ion[0]) * conl
+        logw.append(lw)
+    logw = np.stack(logw)
+    logw -= logw.max(axis=0, keepdims=True)
+    w = np.exp(logw)
+    return w / w.sum(axis=0, keepdims=True)
+
+
+def test_lte_weights_match_calc_lte_populations():
+    """delta = 0 gives MULTI3D's LTE fractions (tests/test_departure_coefficients.py:33-54, 94-109): 2e-12 dex where
+    the fraction exceeds 1e-200."""
+    from ffno_amd.atoms import ATOMS
+    rng = np.random.default_rng(4)
+    log10_t = np.log10(2000.0) + (np.log10(6e4) - np.log10(2000.0)) * rng.random(500)
+    log10_ne = 14.0 + 9.0 * rng.random(500)
+    for a in ("H", "CA"):
+        atom = ATOMS[a]["multi3d_lte"]
+        tables = O.lte_tables([dict(levels=6, multi3d_lte=atom)])
+        lw = O.lte_log_weights(torch.from_numpy(log10_t)[None], torch.from_numpy(log10_ne)[None], tables)[0]
+        got = torch.log_softmax(lw, dim=0).numpy() / math.log(10)
+        expected = _calc_lte_fractions(10.0 ** log10_t, 10.0 ** (log10_ne - 6.0), atom)
+        usable = expected > 1e-200
+        assert np.abs(got - np.log10(np.where(usable, expected, 1.0)))[usable].max() < 2e-12
+    # Ca III is lowered by 3 dxi, not 2 (tests/test_departure_coefficients.py:111-115)
+    assert O.lte_tables([dict(levels=6, multi3d_lte=ATOMS["CA"]["multi3d_lte"])])[2] == [0.0] * 5 + [3.0]
+    assert O.lte_tables([dict(levels=6, multi3d_lte=ATOMS["H"]["multi3d_lte"])])[2] == [0.0] * 5 + [1.0]
+
+
+def test_head_reproduces_any_target():
+    """Any levels whose total lies within the bound are reproduced (tests/test_level_fraction_head.py:53-73), 1e-9."""
+    from ffno_amd.atoms import species
+    cfg = O.model_config(width=4, species=species(["H", "CA"]))
+    torch.manual_seed(11)
+    x = torch.randn(1, 6, 5, 4, 3, dtype=torch.float64)
+    mx = torch.tensor([3.8, 0.0, 0.0, 0.0, 16.0, -7.0], dtype=torch.float64)
+    sx = torch.tensor([0.4, 5.0, 5.0, 5.0, 2.0, 2.5], dtype=torch.float64)
+    sd = dict(mean_X=mx, std_X=sx, mean_Y=torch.zeros(12, dtype=torch.float64),
+              std_Y=torch.ones(12, dtype=torch.float64),
+              log10_particle_mass=torch.tensor([s["log10_particle_mass"] for s in cfg["species"]],
+                                               dtype=torch.float64))
+    lte = O.lte_log_weights(x[:, 0] * sx[0] + mx[0], x[:, 4] * sx[4] + mx[4], O.lte_tables(cfg["species"]))
+    head = torch.empty(1, 14, 5, 4, 3, dtype=torch.float64)
+    target = torch.empty(1, 12, 5, 4, 3, dtype=torch.float64)
+    for sp in (0, 1):
+        ratios = torch.randn(1, 6, 5, 4, 3, dtype=torch.float64) * 6.0
+        frac = ratios - torch.logsumexp(ratios * math.log(10.0), 1, keepdim=True) / math.log(10.0)
+        s = torch.tensor([-3.16, -0.5, 0.0, 1.93, 5.4], dtype=torch.float64)[torch.randint(0, 5, (1, 5, 4, 3))]
+        ref = x[:, 5] * sx[5] + mx[5] - sd["log10_particle_mass"][sp]
+        target[:, 6 * sp:6 * sp + 6] = ref[:, None] + s[:, None] + frac
+        head[:, 7 * sp:7 * sp + 6] = ratios * math.log(10.0) - lte[:, 6 * sp:6 * sp + 6]
+        head[:, 7 * sp + 6] = 6.0 * torch.atanh(s / 6.0)
+    assert (O.head_output(sd, head, x, cfg) - target).abs().max() < 1e-9
