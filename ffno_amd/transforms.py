@@ -1,0 +1,121 @@
+"""Transform matrices of the spectral branches, built once per plan in float64 on the host and uploaded in split-bf16.
+
+Horizontal (models/ffno_model.py:441, :448): torch.fft.rfft2 over (H, W), then irfft2 with s=(H, W), written as four
+real matrices acting on spectra held as separate real/imaginary planes:
+  W_fwd  [2Wf x W]    rows kx: cos(2 pi kx w / W); rows Wf+kx: -sin(...)             (rfft along W)
+  H_fwd  [2H x 2H]    complex DFT along H, exp(-2 pi i ky h / H)
+  H_inv  [2H x 2H]    complex inverse DFT along H with 1/H
+  W_inv  [W x 2Wf]    the c2r inverse along W with 1/W: bin 0 (and W/2 for even W) count once and contribute only
+                      their real part, every other bin counts twice -- the Hermitian convention irfft applies to a
+                      spectrum that is not Hermitian (it is not, after the per-mode mix).
+Vertical (models/ffno_model.py:37-64, :239-259, :270-276, :301-305) on one shared column grid:
+  V_fwd  [2M x D]     Filon weights q_re, q_im of the column's (shifted) heights
+  V_inv  [D x 2M]     f_m cos(k_m z_d) / L and -f_m sin(k_m z_d) / L, f = mode factor * taper
+"""
+import math
+
+import numpy as np
+import torch
+
+
+def round_up(n, m):
+    return (n + m - 1) // m * m
+
+
+def horizontal_matrices(H, W):
+    wf = W // 2 + 1
+    w = np.arange(W)
+    kx = np.arange(wf)
+    th = 2 * np.pi * np.outer(kx, w) / W                       # [Wf, W]
+    w_fwd = np.concatenate([np.cos(th), -np.sin(th)], 0)       # [2Wf, W]
+    h = np.arange(H)
+    ph = 2 * np.pi * np.outer(h, h) / H                        # [ky, h] (symmetric)
+    c, s = np.cos(ph), np.sin(ph)
+    h_fwd = np.block([[c, s], [-s, c]])                        # re: c xr + s xi ; im: -s xr + c xi
+    h_inv = np.block([[c, -s], [s, c]]) / H                    # re: c zr - s zi ; im: s zr + c zi
+    weight = np.full(wf, 2.0)
+    weight[0] = 1.0
+    if W % 2 == 0:
+        weight[-1] = 1.0
+    thi = 2 * np.pi * np.outer(w, kx) / W                      # [W, Wf]
+    w_inv = np.concatenate([weight * np.cos(thi), -weight * np.sin(thi)], 1) / W   # [W, 2Wf]
+    return dict(w_fwd=w_fwd, h_fwd=h_fwd, h_inv=h_inv, w_inv=w_inv)
+
+
+def filon_weights(z, k):
+    """Filon quadrature weights (q_re, q_im) [D, M], float64: sum_j x_j q_jm = int x(z) exp(-i k_m z) dz for x linear
+    between the levels z [D] (any order). Same quadrature and small-angle series (|theta| < 1e-2) as
+    models/ffno_model.py:37-64."""
+    z = np.asarray(z, np.float64)
+    k = np.asarray(k, np.float64)
+    z0, z1 = z[:-1, None], z[1:, None]
+    low = np.minimum(z0, z1)
+    h = np.abs(z1 - z0)
+    th = h * k[None, :]
+    small = np.abs(th) < 1e-2
+    tt = np.where(small, 1.0, th)
+    c, s = np.cos(tt), np.sin(tt)
+    up_re = np.where(small, 0.5 - th ** 2 / 8, (c + tt * s - 1) / tt ** 2)
+    up_im = np.where(small, -th / 3 + th ** 3 / 30, (tt * c - s) / tt ** 2)
+    lo_re = np.where(small, 0.5 - th ** 2 / 24, s / tt - up_re)
+    lo_im = np.where(small, -th / 6 + th ** 3 / 120, (c - 1) / tt - up_im)
+    ph = low * k[None, :]
+    e_re, e_im = h * np.cos(ph), -h * np.sin(ph)
+    a_re, a_im = e_re * lo_re - e_im * lo_im, e_re * lo_im + e_im * lo_re
+    b_re, b_im = e_re * up_re - e_im * up_im, e_re * up_im + e_im * up_re
+    asc = z0 <= z1
+    q_re = np.zeros((len(z), len(k)))
+    q_im = np.zeros((len(z), len(k)))
+    q_re[:-1] += np.where(asc, a_re, b_re)
+    q_re[1:] += np.where(asc, b_re, a_re)
+    q_im[:-1] += np.where(asc, a_im, b_im)
+    q_im[1:] += np.where(asc, b_im, a_im)
+    return q_re, q_im
+
+
+def vertical_box(span, reference_length, min_wavelength, modes):
+    """(L, k [M] as the reference's float32 values in float64, mode factors [M]) (models/ffno_model.py:248-259,
+    :226-229): L = reference_length unless twice the tallest column is longer."""
+    if 2 * span <= reference_length:
+        k = (2 * math.pi * torch.arange(modes, dtype=torch.float32) / reference_length).double().numpy()
+        length = reference_length
+    else:
+        length = 2 * span
+        modes = int(math.floor(length / min_wavelength)) + 1
+        k = (2 * math.pi * torch.arange(modes, dtype=torch.float32) / length).double().numpy()
+    factor = np.full(modes, 2.0)
+    factor[0] = 1.0
+    return length, k, factor
+
+
+def column_taper(column, k):
+    """Half-cosine taper [M] from 2 pi / (3 dz) to pi / dz, dz the lower median of |spacing|
+    (models/ffno_model.py:239-246; torch.median returns the lower of the two middle values)."""
+    d = np.sort(np.abs(np.diff(np.asarray(column, np.float64))))
+    dz = d[(len(d) - 1) // 2]
+    k_nyq = math.pi / dz
+    u = np.clip((k - 2 * k_nyq / 3) / (k_nyq / 3), 0.0, 1.0)
+    return 0.5 * (1 + np.cos(math.pi * u))
+
+
+def vertical_matrices(column, k, factor, length):
+    """V_fwd [2M, D] and V_inv [D, 2M] for the shifted column heights [D] (float64)."""
+    q_re, q_im = filon_weights(column, k)
+    v_fwd = np.concatenate([q_re.T, q_im.T], 0)
+    f = factor * column_taper(column, k)
+    ph = np.outer(np.asarray(column, np.float64), k)
+    v_inv = np.concatenate([f * np.cos(ph), -f * np.sin(ph)], 1) / length
+    return v_fwd, v_inv
+
+
+def split_bf16(a, device):
+    """(hi, lo) bf16 [m_pad, k_pad] of a float64 matrix [m, k]: hi = bf16(a), lo = bf16(a - hi), zero padded."""
+    a = np.asarray(a, np.float64)
+    m, k = a.shape
+    mp, kp = round_up(m, 16), round_up(k, 32)
+    full = np.zeros((mp, kp))
+    full[:m, :k] = a
+    t = torch.from_numpy(full)
+    hi = t.to(torch.bfloat16)
+    lo = (t - hi.double()).to(torch.bfloat16)
+    return (hi.view(torch.int16).to(device), lo.view(torch.int16).to(device), m, k, mp, kp)

@@ -1,0 +1,148 @@
+/* C ABI of the FFNO3D HIP library (libffno_hip.so), gfx950.
+ *
+ * The reference has no native code: its FFNO3D forward (models/ffno_model.py:800-832) is a chain of stock torch
+ * ops. Each entry point below launches one hand-written kernel that replaces a span of that chain; the comment on
+ * each names the reference lines it stands for. Host code (ffno_amd/engine.py) owns every device buffer, decides the
+ * launch order and passes plain device pointers, sizes and a hipStream_t. Nothing here allocates device memory.
+ *
+ * Layouts (all float32 unless noted):
+ *   activations    channel-last [B, D, H, W, C]  ("voxel rows" of C channels)
+ *   model inputs   channel-first [B, 6, D, H, W], outputs channel-first [B, Cout, D, H, W] (the reference's)
+ *   packed weights bf16 (uint16_t) [N_pad][K_pad], K permuted within every 32-block so that one 16-byte load is
+ *                  one MFMA operand (see ffno_amd/engine.py: pack_mlp_weight)
+ *   twiddles       bf16 pairs (hi, lo) [M_pad][K_pad], the split-bf16 form of a float64 matrix
+ *   GroupNorm      partial statistics [B][nwg][G][3] = (weight, mean, M2); finalized stats [B][G][2] = (mean, rstd);
+ *                  per-channel affine [B][C] scale and [B][C] shift
+ *
+ * Every call returns 0 on success or a negative code; ffno_last_error() gives the message (thread-local).
+ */
+#ifndef FFNO_HIP_H
+#define FFNO_HIP_H
+
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+const char* ffno_last_error(void);
+int ffno_version(void);
+
+/* Voxel-row sizes shared by the pointwise kernels. */
+typedef struct {
+  int B;              /* samples */
+  int64_t voxels;     /* D*H*W per sample */
+  int C;              /* width: 16, 32, 64, 96 or 128 */
+  int nwg;            /* workgroups per sample (grid = nwg x B) */
+} FfnoRows;
+
+/* GroupNorm statistics output of a producer kernel (NULL partials: no statistics). */
+typedef struct {
+  const float* depth_w;   /* [B, D*H*W] depth_weights (models/ffno_model.py:67-74) */
+  float* partials;        /* [B][nwg][8][3] */
+} FfnoStatsOut;
+
+/* "base + scale * act(A * t + Bshift)" input assembly from a finalized GroupNorm (affine per (b, c)). */
+typedef struct {
+  const float* base;      /* [N, C] or NULL */
+  const float* t;         /* [N, C] the normalised tensor's input */
+  const float* aff_a;     /* [B][C] */
+  const float* aff_b;     /* [B][C] */
+  const float* scale;     /* device scalar or NULL (1) */
+  int act_gelu;           /* 1: GELU after the affine, 0: none */
+} FfnoAssemble;
+
+/* Lift: Conv3d 6->C then GroupNorm statistics (models/ffno_model.py:686-690, :782-783). */
+int ffno_lift(FfnoRows rows, const float* x_in, const float* w_lift /* [C][6] */, float* out, FfnoStatsOut stats,
+              hipStream_t stream);
+
+/* GroupNorm finalize: merge partial statistics, produce (mean, rstd) and the per-channel affine
+ * (models/ffno_model.py:77-95, eps 1e-5). gamma/beta [C]. */
+int ffno_gn_finalize(int B, int nwg, int C, const float* partials, const float* gamma, const float* beta,
+                     float* stats, float* aff_a, float* aff_b, hipStream_t stream);
+
+/* Block entry (models/ffno_model.py:536-542, :266, :439-440): x = assemble(...); writes x, then both input gates:
+ * xs = x * sigmoid(G2 gelu(G1 x)), xv = x * sigmoid(V2 gelu(V1 x)) + z_proj * z. */
+int ffno_block_in(FfnoRows rows, FfnoAssemble in, float* x_out, const uint16_t* g1, const uint16_t* g2,
+                  const uint16_t* v1, const uint16_t* v2, const float* z_proj /* [C] */, const float* z /* [B,DHW] */,
+                  float* xs, float* xv, hipStream_t stream);
+
+/* Two-layer pointwise MLP y = W2 gelu(W1 u) with GroupNorm statistics of y: the post networks of both branches
+ * (models/ffno_model.py:221-225, :369-373, :309, :449). */
+int ffno_post(FfnoRows rows, const float* u, const uint16_t* w1, const uint16_t* w2, float* y, FfnoStatsOut stats,
+              hipStream_t stream);
+
+/* Branch fusion (models/ffno_model.py:524-534, :541-543): s = gelu(GN(sp)), v = gelu(GN(vp)),
+ * gates = sigmoid(BG2 gelu(BG1 [x,s,v]) + b), fp = F2 gelu(F1 [s*gs, v*gv]) + 0.5 (s*gs + v*gv), statistics of fp. */
+int ffno_fuse(FfnoRows rows, const float* x, const float* sp, const float* vp, const float* s_a, const float* s_b,
+              const float* v_a, const float* v_b, const uint16_t* bg1, const uint16_t* bg2, const float* bg2_bias,
+              const uint16_t* f1, const uint16_t* f2, float* fp, FfnoStatsOut stats, hipStream_t stream);
+
+/* Residual pointwise path (models/ffno_model.py:545-551): x_out = assemble(...); y = W2 gelu(W1 x_out); statistics
+ * of y. Used for pw (assemble = x + res_fused gelu(GN(fp))) and mlp (assemble = x1 + res_pw GN(pp)). */
+int ffno_resmlp(FfnoRows rows, FfnoAssemble in, float* x_out, const uint16_t* w1, const uint16_t* w2, float* y,
+                FfnoStatsOut stats, hipStream_t stream);
+
+/* Head (models/ffno_model.py:790-798, :730-749, :116-141): x = assemble(...); head = P2 gelu(P1 x) + bias;
+ * per species log10 n = log10 rho - log10 m + 6 tanh(u/6) + log_softmax(head + ln f*(T, ne)) / ln 10, normalised
+ * with mean_Y, std_Y. Writes y channel-first [B, Cout, D, H, W]. */
+typedef struct {
+  int n_species;
+  int levels[4];
+  float log10_mass[4];
+  float log_g[32], ev[32], ndxi[32], dion[32];   /* per output level, concatenated over species */
+  float mean_x[6], std_x[6];
+  float mean_y[32], std_y[32];
+  int t_channel, ne_channel, rho_channel;
+  float total_bound;
+  float ln_ccon;      /* ln(0.5 (h / sqrt(2 pi m_e) / sqrt(k))^3), cgs (models/ffno_model.py:113) */
+  float ln_k;         /* ln k_B, cgs */
+  float e_over_k;     /* eV / k_B [K] */
+  float dxi_coef;     /* 4.98e-4 * 5040 */
+} FfnoHeadDesc;
+
+int ffno_head(FfnoRows rows, FfnoAssemble in, const uint16_t* p1, const uint16_t* p2, const float* p2_bias,
+              const float* x_in /* [B,6,DHW] */, const FfnoHeadDesc* desc, float* y, hipStream_t stream);
+
+/* Split-bf16 ("bf16x3") GEMM of a small precomputed real matrix A [m x k] against batched data X [k x n]:
+ * Y[b][m][n] = sum_k A[m][k] X[b][k][n]. It carries every DFT (rfft/ifft/irfft along W and H) and the vertical
+ * Filon transform and its inverse (models/ffno_model.py:441, :448, :272-275, :303-305). Row offsets come from
+ * two-level index maps: offset(i) = (i / split) * hi + (i % split) * lo. */
+typedef struct {
+  const uint16_t* a_hi;
+  const uint16_t* a_lo;
+  int m, k, n;            /* n % 4 == 0 */
+  int m_pad, k_pad;       /* multiples of 16 and 32 */
+  int batch;
+  const float* x;
+  float* y;
+  int64_t b_split, x_b_hi, x_b_lo, y_b_hi, y_b_lo;
+  int64_t k_split, x_k_hi, x_k_lo;
+  int64_t m_split, y_m_hi, y_m_lo;
+  int accumulate;         /* 1: Y += A X */
+} FfnoTwGemm;
+
+int ffno_tw_gemm(const FfnoTwGemm* g, hipStream_t stream);
+
+/* Per-mode complex channel mix of a spectrum held as separate real and imaginary planes, in place
+ * (models/ffno_model.py:394-429 for the horizontal branch, :286-302 for the vertical one):
+ * Z = g(k) (Y (W_r + i W_i)) + U(k) (V(k) Y), times an optional per-mode factor. */
+typedef struct {
+  float* re;
+  float* im;
+  int C, rows, modes, rank;
+  int64_t mode_stride, r_split, r_hi, r_lo;
+  const uint16_t* wc;     /* packed [2C][2C] */
+  const float* gate;      /* [modes][2] */
+  const uint16_t* v_p;    /* [modes][16][KS*32] or NULL */
+  const uint16_t* u_p;    /* [modes][C][32] or NULL */
+  const float* factor;    /* [modes] or NULL */
+} FfnoMix;
+
+int ffno_mix(const FfnoMix* m, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

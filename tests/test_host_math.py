@@ -1,0 +1,127 @@
+"""Host-side pieces of the HIP engine, checked on the CPU: the transform matrices against numpy.fft and the oracle,
+the weight packing against the kernels' lane-layout contraction, the model's state_dict layout, and the C ABI's
+exported symbols."""
+import ctypes
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+from ffno_amd import transforms as T
+from ffno_amd.engine import pack_mlp_weight
+
+
+@pytest.mark.parametrize("H,W", [(8, 12), (11, 9), (6, 5), (64, 64)])
+def test_horizontal_matrices_are_rfft2_and_irfft2(H, W):
+    """rfft2 then irfft2(s=(H, W)) as the four real matrices, on data and on a NON-Hermitian spectrum (as the mix
+    leaves it): 1e-12 of numpy.fft's result."""
+    rng = np.random.default_rng(0)
+    m = T.horizontal_matrices(H, W)
+    wf = W // 2 + 1
+    x = rng.standard_normal((H, W))
+    s1 = m["w_fwd"] @ x.T                                   # [2Wf, H]: rows (part, kx), columns h
+    re, im = s1[:wf].T, s1[wf:].T                           # [H, Wf]
+    s2 = m["h_fwd"] @ np.concatenate([re, im], 0)           # rows (part, ky)
+    ref = np.fft.rfft2(x)
+    got = s2[:H] + 1j * s2[H:]
+    assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
+    z = rng.standard_normal((H, wf)) + 1j * rng.standard_normal((H, wf))   # not Hermitian
+    v = m["h_inv"] @ np.concatenate([z.real, z.imag], 0)    # [2H, Wf]
+    y = (m["w_inv"] @ np.concatenate([v[:H].T, v[H:].T], 0)).T     # [H, W]
+    yref = np.fft.irfft2(z, s=(H, W))
+    assert np.abs(y - yref).max() < 1e-12 * np.abs(yref).max()
+    # torch's irfft2 (what the reference calls) agrees on the same spectrum
+    yt = torch.fft.irfft2(torch.from_numpy(z), s=(H, W)).numpy()
+    assert np.abs(y - yt).max() < 1e-12 * np.abs(yref).max()
+
+
+def test_filon_matrices_match_oracle():
+    from oracle import ffno_oracle as O
+    z = np.linspace(15.0, 0.0, 37) ** 1.1 - 1.0
+    z = z - z.min()
+    k = 2 * np.pi * np.arange(40) / 40.0
+    q_re, q_im = T.filon_weights(z, k)
+    o_re, o_im = O.height_fourier_weights(torch.from_numpy(z), torch.from_numpy(k))
+    assert np.abs(q_re - o_re.numpy()).max() < 1e-13
+    assert np.abs(q_im - o_im.numpy()).max() < 1e-13
+    taper = T.column_taper(z, k)
+    assert np.abs(taper - O.column_taper(torch.from_numpy(z), torch.from_numpy(k)).numpy()).max() < 1e-13
+
+
+def test_split_bf16_recovers_float32_accuracy():
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal((40, 70))
+    hi, lo, m, k, mp, kp = T.split_bf16(a, "cpu")
+    back = hi.view(torch.bfloat16).double() + lo.view(torch.bfloat16).double()
+    assert (mp, kp) == (48, 96)
+    assert np.abs(back.numpy()[:m, :k] - a).max() < 2e-5 * np.abs(a).max()
+    assert float(back[m:].abs().max()) == 0.0 and float(back[:, k:].abs().max()) == 0.0
+
+
+def _kernel_contraction(wp, act):
+    """What the kernels compute from a packed weight: output o = sum over k-steps s, lane groups q and slots j of
+    wp[o, 32s + 8q + j] * act[feature(s, q, j)], where slot j < 4 is feature 32s + 4q + j (tile 2s) and slot
+    j >= 4 is feature 32s + 16 + 4q + j - 4 (tile 2s + 1)."""
+    npad, kp = wp.shape
+    out = np.zeros(npad)
+    for s in range(kp // 32):
+        for q in range(4):
+            for j in range(8):
+                f = 32 * s + 4 * q + j if j < 4 else 32 * s + 16 + 4 * q + (j - 4)
+                out += wp[:, 32 * s + 8 * q + j] * act[f]
+    return out
+
+
+@pytest.mark.parametrize("N,segments", [(32, [16]), (64, [64]), (96, [96, 96, 96]), (14, [128]), (48, [16, 16])])
+def test_packing_matches_kernel_contraction(N, segments):
+    rng = np.random.default_rng(2)
+    K = sum(segments)
+    w = torch.from_numpy(rng.standard_normal((N, K))).float()
+    wp = pack_mlp_weight(w, segments).view(torch.bfloat16).double().numpy()
+    act_parts = [rng.standard_normal(s) for s in segments]
+    act = np.concatenate([np.pad(a, (0, T.round_up(len(a), 32) - len(a))) for a in act_parts])
+    got = _kernel_contraction(wp, act)[:N]
+    wb = w.to(torch.bfloat16).double().numpy()
+    expected = wb @ np.concatenate(act_parts)
+    assert np.abs(got - expected).max() < 1e-10 * max(1.0, np.abs(expected).max())
+    assert not np.any(_kernel_contraction(wp, act)[N:])
+
+
+def test_state_dict_is_the_references():
+    """ffno_amd.model.FFNO3D has exactly the keys and shapes of the reference checkpoint (golden case A was loaded
+    into the reference with strict=True)."""
+    from golden_cases import load
+    from ffno_amd.model import FFNO3D
+    data, config, state = load("case_a_shared64")
+    model = FFNO3D(**config)
+    ours = model.state_dict()
+    assert sorted(ours) == sorted(state)
+    for k in ours:
+        assert tuple(ours[k].shape) == tuple(state[k].shape), k
+    model.load_state_dict(state, strict=True)
+
+
+def test_c_abi_exports_every_declared_symbol():
+    """libffno_hip.so loads (no GPU needed) and exports every function include/ffno_hip.h declares."""
+    from ffno_amd import _lib
+    header = open(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include",
+                               "ffno_hip.h")).read()
+    names = set(re.findall(r"^\s*(?:const char\*|int)\s+(ffno_\w+)\s*\(", header, re.M))
+    assert len(names) >= 10
+    handle = ctypes.CDLL(_lib.LIB_PATH)
+    for n in sorted(names):
+        assert hasattr(handle, n), n
+    assert set(_lib.SIGNATURES) | {"ffno_last_error", "ffno_version"} >= names
+
+
+def test_cpu_tensors_are_refused():
+    """No CPU path: the module raises on CPU tensors instead of computing something else."""
+    from ffno_amd import synthetic
+    from ffno_amd.atoms import species
+    model = synthetic.random_model(0, width=16, n_layers=1, hidden_spectral=8, hidden_vertical=8, freq_mix_rank=2,
+                                   out_channels=6, species=species(["H"]))
+    with pytest.raises(RuntimeError):
+        model(torch.zeros(1, 6, 4, 4, 4), torch.zeros(1, 4, 4, 4), 48e3, 48e3)
