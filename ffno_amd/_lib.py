@@ -47,6 +47,11 @@ class FfnoMix(ctypes.Structure):
                 ("factor", c_ptr)]
 
 
+class FfnoGnBack(ctypes.Structure):
+    _fields_ = [("t", c_ptr), ("stats", c_ptr), ("gamma", c_ptr), ("aff_a", c_ptr), ("aff_b", c_ptr),
+                ("scale", c_ptr), ("act_gelu", ctypes.c_int), ("sums", c_ptr), ("depth_w", c_ptr)]
+
+
 # entry point -> argument types (every one returns int)
 SIGNATURES = {
     "ffno_lift": [FfnoRows, c_ptr, c_ptr, c_ptr, FfnoStatsOut, c_ptr],
@@ -56,9 +61,17 @@ SIGNATURES = {
     "ffno_fuse": [FfnoRows, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr,
                   FfnoStatsOut, c_ptr],
     "ffno_resmlp": [FfnoRows, FfnoAssemble, c_ptr, c_ptr, c_ptr, c_ptr, FfnoStatsOut, c_ptr],
-    "ffno_head": [FfnoRows, FfnoAssemble, c_ptr, c_ptr, c_ptr, c_ptr, ctypes.POINTER(FfnoHeadDesc), c_ptr, c_ptr],
+    "ffno_head": [FfnoRows, FfnoAssemble, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, ctypes.POINTER(FfnoHeadDesc), c_ptr,
+                  c_ptr],
     "ffno_tw_gemm": [ctypes.POINTER(FfnoTwGemm), c_ptr],
     "ffno_mix": [ctypes.POINTER(FfnoMix), c_ptr],
+    "ffno_gn_grad_stats": [FfnoRows, FfnoGnBack, c_ptr, c_ptr, c_ptr],
+    "ffno_gn_grad_finalize": [ctypes.c_int, ctypes.c_int, c_ptr, c_ptr, c_ptr],
+    "ffno_mlp_bwd": [FfnoRows, c_ptr, FfnoGnBack, c_ptr, c_ptr, c_ptr, c_ptr, ctypes.c_int, c_ptr, c_ptr],
+    "ffno_fuse_bwd": [FfnoRows, c_ptr, FfnoGnBack] + [c_ptr] * 18 + [c_ptr],
+    "ffno_gates_bwd": [FfnoRows] + [c_ptr] * 15 + [c_ptr],
+    "ffno_head_bwd": [FfnoRows] + [c_ptr] * 7 + [ctypes.POINTER(FfnoHeadDesc)] + [c_ptr] * 3 + [c_ptr],
+    "ffno_lift_bwd": [FfnoRows, c_ptr, FfnoGnBack, c_ptr, c_ptr, c_ptr],
 }
 
 _lib = None

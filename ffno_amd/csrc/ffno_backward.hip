@@ -1,0 +1,613 @@
+// FFNO3D adjoint kernels for gfx950: the input VJP of every forward kernel, with the forward recomputed from saved
+// block inputs. Same lane layout and split-bf16 MFMA arithmetic as the forward (ffno_device.h).
+#include "ffno_device.h"
+
+namespace ffno {
+namespace {
+
+// ---------------------------------------------------------------------------------------------------------------
+// GroupNorm backward
+// ---------------------------------------------------------------------------------------------------------------
+// g_hat and x_hat of one lane's 4 channels of tile t
+template <int CT>
+__device__ __forceinline__ void gn_ghat(const FfnoGnBack& gb, float scale, int64_t row, int b, bool ok, int q, int t,
+                                        f32x4 ub, f32x4& gh, f32x4& xh) {
+  constexpr int C = CT * 16, CPG = C / G;
+  const int c0 = 16 * t + 4 * q;
+  const f32x4 tv = ld4(gb.t + row * C + c0, ok);
+  const f32x4 gm = *(const f32x4*)(gb.gamma + c0);
+  f32x4 act = f32x4{1.f, 1.f, 1.f, 1.f};
+  if (gb.act_gelu) {
+    const f32x4 a = *(const f32x4*)(gb.aff_a + b * C + c0);
+    const f32x4 s = *(const f32x4*)(gb.aff_b + b * C + c0);
+    act = gelu_grad4(tv * a + s);
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const float* st = gb.stats + (b * G + (c0 + j) / CPG) * 3;
+    xh[j] = ok ? (tv[j] - st[0]) * st[1] : 0.f;
+    gh[j] = ok ? scale * ub[j] * gm[j] * act[j] : 0.f;
+  }
+}
+
+// t_bar = rstd * (g_hat - (w / W) * (S1 + x_hat * S2))
+template <int CT>
+__device__ __forceinline__ void gn_back(const FfnoGnBack& gb, float scale, int64_t row, int b, bool ok, int q,
+                                        const f32x4 (&ub)[CT], f32x4 (&tb)[CT]) {
+  constexpr int C = CT * 16, CPG = C / G;
+  const float w = ok ? gb.depth_w[row] : 0.f;
+#pragma unroll
+  for (int t = 0; t < CT; ++t) {
+    f32x4 gh, xh;
+    gn_ghat<CT>(gb, scale, row, b, ok, q, t, ub[t], gh, xh);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int g = (16 * t + 4 * q + j) / CPG;
+      const float* st = gb.stats + (b * G + g) * 3;
+      const float* sm = gb.sums + (b * G + g) * 2;
+      tb[t][j] = ok ? st[1] * (gh[j] - (w / st[2]) * (sm[0] + xh[j] * sm[1])) : 0.f;
+    }
+  }
+}
+
+template <int CT, int NW>
+__global__ __launch_bounds__(NW * 64) void k_gn_grad_stats(FfnoRows rows, FfnoGnBack gb, const float* __restrict__ ubar,
+                                                           float* __restrict__ partials) {
+  constexpr int C = CT * 16, CPG = C / G;
+  __shared__ float red[NW][4][CT * 4][2];
+  Lane L;
+  const int b = blockIdx.y;
+  const float scale = gb.scale ? *gb.scale : 1.f;
+  const int64_t nt = (rows.voxels + 15) / 16;
+  float s1[CT * 4], s2[CT * 4];
+#pragma unroll
+  for (int i = 0; i < CT * 4; ++i) { s1[i] = 0.f; s2[i] = 0.f; }
+  for (int64_t tile = (int64_t)blockIdx.x * NW + L.wave; tile < nt; tile += (int64_t)gridDim.x * NW) {
+    const int64_t vox = tile * 16 + L.v;
+    const bool ok = vox < rows.voxels;
+    const int64_t row = (int64_t)b * rows.voxels + vox;
+#pragma unroll
+    for (int t = 0; t < CT; ++t) {
+      const f32x4 ub = ld4(ubar + row * C + 16 * t + 4 * L.q, ok);
+      f32x4 gh, xh;
+      gn_ghat<CT>(gb, scale, row, b, ok, L.q, t, ub, gh, xh);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        s1[t * 4 + j] += gh[j];
+        s2[t * 4 + j] += gh[j] * xh[j];
+      }
+    }
+  }
+  // sum over the 16 voxel lanes, then over waves and lane groups per channel group
+#pragma unroll
+  for (int i = 0; i < CT * 4; ++i)
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) {
+      s1[i] += __shfl_xor(s1[i], off);
+      s2[i] += __shfl_xor(s2[i], off);
+    }
+  if (L.v == 0)
+#pragma unroll
+    for (int i = 0; i < CT * 4; ++i) {
+      red[L.wave][L.q][i][0] = s1[i];
+      red[L.wave][L.q][i][1] = s2[i];
+    }
+  __syncthreads();
+  if (threadIdx.x < G) {
+    const int g = threadIdx.x;
+    float a = 0.f, c = 0.f;
+    for (int w = 0; w < NW; ++w)
+      for (int qq = 0; qq < 4; ++qq)
+        for (int i = 0; i < CT * 4; ++i) {
+          const int ch = 16 * (i / 4) + 4 * qq + (i % 4);
+          if (ch / CPG != g) continue;
+          a += red[w][qq][i][0];
+          c += red[w][qq][i][1];
+        }
+    float* p = partials + (((int64_t)b * gridDim.x + blockIdx.x) * G + g) * 2;
+    p[0] = a;
+    p[1] = c;
+  }
+}
+
+__global__ void k_gn_grad_finalize(int nwg, const float* __restrict__ partials, float* __restrict__ sums) {
+  const int b = blockIdx.x, g = threadIdx.x;
+  if (g >= G) return;
+  float a = 0.f, c = 0.f;
+  for (int i = 0; i < nwg; ++i) {
+    const float* p = partials + (((int64_t)b * nwg + i) * G + g) * 2;
+    a += p[0];
+    c += p[1];
+  }
+  sums[(b * G + g) * 2 + 0] = a;
+  sums[(b * G + g) * 2 + 1] = c;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// MLP behind a GroupNorm, with or without the residual path
+// ---------------------------------------------------------------------------------------------------------------
+template <int CT, int NW>
+__global__ __launch_bounds__(NW * 64) void k_mlp_bwd(FfnoRows rows, const float* __restrict__ ubar, FfnoGnBack gb,
+                                                     const float* __restrict__ xin, const uint16_t* w1,
+                                                     const uint16_t* w2t, const uint16_t* w1t, int residual,
+                                                     float* __restrict__ xbar) {
+  constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32;
+  extern __shared__ __align__(16) unsigned char smem[];
+  bf16* lds = reinterpret_cast<bf16*>(smem);
+  const Wm W1 = stage_w(lds, w1, 2 * C * KP), W2T = stage_w(lds, w2t, 2 * C * KP), W1T = stage_w(lds, w1t, C * 2 * C);
+  __syncthreads();
+  Lane L;
+  const int b = blockIdx.y;
+  const float scale = gb.scale ? *gb.scale : 1.f;
+  const int64_t nt = (rows.voxels + 15) / 16;
+  for (int64_t tile = (int64_t)blockIdx.x * NW + L.wave; tile < nt; tile += (int64_t)gridDim.x * NW) {
+    const int64_t vox = tile * 16 + L.v;
+    const bool ok = vox < rows.voxels;
+    const int64_t row = (int64_t)b * rows.voxels + vox;
+    f32x4 ub[CT], acc[CT];
+    load_act<CT>(ubar, row, ok, L.q, ub);
+    Frag of[KS], xf[KS];
+    {
+      f32x4 tb[CT];
+      gn_back<CT>(gb, scale, row, b, ok, L.q, ub, tb);
+      to_frags<CT>(tb, of);
+      f32x4 x[CT];
+      load_act<CT>(xin, row, ok, L.q, x);
+      to_frags<CT>(x, xf);
+    }
+#pragma unroll
+    for (int t = 0; t < CT; ++t) acc[t] = residual ? ub[t] : zero4();
+    mlp2_bwd<KS, 2 * CT, KS, CT>(W1, W2T, W1T, xf, of, acc, L.v, L.q);
+    store_act<CT>(xbar, row, ok, L.q, acc);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// branch fusion adjoint (weights read from global memory: seven matrices do not fit LDS together)
+// ---------------------------------------------------------------------------------------------------------------
+template <int CT, int NW>
+__global__ __launch_bounds__(NW * 64) void k_fuse_bwd(FfnoRows rows, const float* __restrict__ x1bar, FfnoGnBack gf,
+                                                      const float* __restrict__ hin, const float* __restrict__ sp,
+                                                      const float* __restrict__ vp, const float* __restrict__ s_a,
+                                                      const float* __restrict__ s_b, const float* __restrict__ v_a,
+                                                      const float* __restrict__ v_b, const uint16_t* bg1,
+                                                      const uint16_t* bg2, const float* __restrict__ bg2_bias,
+                                                      const uint16_t* f1, const uint16_t* f2t, const uint16_t* f1t,
+                                                      const uint16_t* bg2t, const uint16_t* bg1t,
+                                                      float* __restrict__ hbar, float* __restrict__ sbar,
+                                                      float* __restrict__ vbar) {
+  constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32;
+  const Wm B1 = global_w(bg1, C * 3 * KP), B2 = global_w(bg2, 2 * C * KP);
+  const Wm F1 = global_w(f1, 2 * C * 2 * KP), F2T = global_w(f2t, 2 * C * KP), F1T = global_w(f1t, 2 * C * 2 * C);
+  const Wm B2T = global_w(bg2t, C * 2 * C), B1T = global_w(bg1t, 3 * C * KP);
+  Lane L;
+  const int b = blockIdx.y;
+  const float scale = gf.scale ? *gf.scale : 1.f;
+  const int64_t nt = (rows.voxels + 15) / 16;
+  for (int64_t tile = (int64_t)blockIdx.x * NW + L.wave; tile < nt; tile += (int64_t)gridDim.x * NW) {
+    const int64_t vox = tile * 16 + L.v;
+    const bool ok = vox < rows.voxels;
+    const int64_t row = (int64_t)b * rows.voxels + vox;
+    f32x4 h[CT], s[CT], v[CT], gt[2 * CT];
+    load_act<CT>(hin, row, ok, L.q, h);
+    load_act<CT>(sp, row, ok, L.q, s);
+    load_act<CT>(vp, row, ok, L.q, v);
+#pragma unroll
+    for (int t = 0; t < CT; ++t) {
+      const int c0 = b * C + 16 * t + 4 * L.q;
+      s[t] = ok ? gelu4(s[t] * *(const f32x4*)(s_a + c0) + *(const f32x4*)(s_b + c0)) : zero4();
+      v[t] = ok ? gelu4(v[t] * *(const f32x4*)(v_a + c0) + *(const f32x4*)(v_b + c0)) : zero4();
+    }
+    {  // gates (models/ffno_model.py:525-526)
+      Frag cat[3 * KS];
+      to_frags<CT>(h, cat);
+      to_frags<CT>(s, cat + KS);
+      to_frags<CT>(v, cat + 2 * KS);
+      mlp2<3 * KS, CT, 2 * CT>(B1, B2, cat, gt, L.v, L.q);
+#pragma unroll
+      for (int t = 0; t < 2 * CT; ++t) gt[t] = sigm4(gt[t] + *(const f32x4*)(bg2_bias + 16 * t + 4 * L.q));
+    }
+    // fp_bar from the GroupNorm (GELU after it) and the fusion MLP's adjoint (models/ffno_model.py:531-533)
+    f32x4 mb[2 * CT];
+    {
+      f32x4 x1b[CT], fb[CT];
+      load_act<CT>(x1bar, row, ok, L.q, x1b);
+      gn_back<CT>(gf, scale, row, b, ok, L.q, x1b, fb);
+      Frag fbf[KS], mm[2 * KS];
+#pragma unroll
+      for (int t = 0; t < CT; ++t) {
+        mb[t] = 0.5f * fb[t];
+        mb[CT + t] = 0.5f * fb[t];
+      }
+      to_frags<CT>(fb, fbf);
+      f32x4 sm[CT], vm[CT];
+#pragma unroll
+      for (int t = 0; t < CT; ++t) {
+        sm[t] = s[t] * gt[t];
+        vm[t] = v[t] * gt[CT + t];
+      }
+      to_frags<CT>(sm, mm);
+      to_frags<CT>(vm, mm + KS);
+      mlp2_bwd<2 * KS, 2 * CT, KS, 2 * CT>(F1, F2T, F1T, mm, fbf, mb, L.v, L.q);
+    }
+    // through the gates: s_bar = sm_bar * gs, gate pre-activation gradient = sm_bar * s * gs (1 - gs), same for v
+    f32x4 gp[2 * CT];
+#pragma unroll
+    for (int t = 0; t < CT; ++t) {
+      gp[t] = mb[t] * s[t] * gt[t] * (1.f - gt[t]);
+      gp[CT + t] = mb[CT + t] * v[t] * gt[CT + t] * (1.f - gt[CT + t]);
+      s[t] = mb[t] * gt[t];
+      v[t] = mb[CT + t] * gt[CT + t];
+    }
+    f32x4 cb[3 * CT];
+#pragma unroll
+    for (int t = 0; t < 3 * CT; ++t) cb[t] = zero4();
+    {
+      Frag gpf[CT], cat[3 * KS];
+      to_frags<2 * CT>(gp, gpf);
+      f32x4 sv[CT];
+      load_act<CT>(sp, row, ok, L.q, sv);
+      to_frags<CT>(h, cat);
+#pragma unroll
+      for (int t = 0; t < CT; ++t) {
+        const int c0 = b * C + 16 * t + 4 * L.q;
+        sv[t] = ok ? gelu4(sv[t] * *(const f32x4*)(s_a + c0) + *(const f32x4*)(s_b + c0)) : zero4();
+      }
+      to_frags<CT>(sv, cat + KS);
+      load_act<CT>(vp, row, ok, L.q, sv);
+#pragma unroll
+      for (int t = 0; t < CT; ++t) {
+        const int c0 = b * C + 16 * t + 4 * L.q;
+        sv[t] = ok ? gelu4(sv[t] * *(const f32x4*)(v_a + c0) + *(const f32x4*)(v_b + c0)) : zero4();
+      }
+      to_frags<CT>(sv, cat + 2 * KS);
+      mlp2_bwd<3 * KS, CT, CT, 3 * CT>(B1, B2T, B1T, cat, gpf, cb, L.v, L.q);
+    }
+    f32x4 x1b[CT];
+    load_act<CT>(x1bar, row, ok, L.q, x1b);
+#pragma unroll
+    for (int t = 0; t < CT; ++t) {
+      x1b[t] += cb[t];
+      s[t] += cb[CT + t];
+      v[t] += cb[2 * CT + t];
+    }
+    store_act<CT>(hbar, row, ok, L.q, x1b);
+    store_act<CT>(sbar, row, ok, L.q, s);
+    store_act<CT>(vbar, row, ok, L.q, v);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// input-gate adjoint
+// ---------------------------------------------------------------------------------------------------------------
+// one gate xo = h * sigmoid(A2 gelu(A1 h)): hb += xb * sig + A1^T (gelu'(A1 h) * A2^T (xb * h * sig (1 - sig)))
+template <int CT>
+__device__ __forceinline__ void gate_bwd(Wm A1, Wm A2, Wm A2T, Wm A1T, const Frag* hf, const f32x4 (&h)[CT],
+                                         const f32x4 (&xb)[CT], f32x4 (&hb)[CT], int v, int q) {
+  constexpr int KS = ks_of(CT);
+  f32x4 sg[CT], og[CT];
+  mlp2<KS, CT, CT>(A1, A2, hf, sg, v, q);
+#pragma unroll
+  for (int t = 0; t < CT; ++t) {
+    sg[t] = sigm4(sg[t]);
+    hb[t] += xb[t] * sg[t];
+    og[t] = xb[t] * h[t] * sg[t] * (1.f - sg[t]);
+  }
+  Frag ogf[KS];
+  to_frags<CT>(og, ogf);
+  mlp2_bwd<KS, CT, KS, CT>(A1, A2T, A1T, hf, ogf, hb, v, q);
+}
+
+template <int CT, int NW, bool LW>
+__global__ __launch_bounds__(NW * 64) void k_gates_bwd(FfnoRows rows, const float* __restrict__ hin,
+                                                       const float* __restrict__ xsbar, const float* __restrict__ xvbar,
+                                                       const float* __restrict__ hbar_in, const uint16_t* g1,
+                                                       const uint16_t* g2, const uint16_t* g2t, const uint16_t* g1t,
+                                                       const uint16_t* v1, const uint16_t* v2, const uint16_t* v2t,
+                                                       const uint16_t* v1t, const float* __restrict__ z_proj,
+                                                       float* __restrict__ hbar_out, float* __restrict__ zbar) {
+  constexpr int C = CT * 16, KS = ks_of(CT), SZ = C * KS * 32;
+  extern __shared__ __align__(16) unsigned char smem[];
+  bf16* lds = reinterpret_cast<bf16*>(smem);
+  const Wm G1 = LW ? stage_w(lds, g1, SZ) : global_w(g1, SZ), G2 = LW ? stage_w(lds, g2, SZ) : global_w(g2, SZ);
+  const Wm G2T = LW ? stage_w(lds, g2t, SZ) : global_w(g2t, SZ), G1T = LW ? stage_w(lds, g1t, SZ) : global_w(g1t, SZ);
+  const Wm V1 = LW ? stage_w(lds, v1, SZ) : global_w(v1, SZ), V2 = LW ? stage_w(lds, v2, SZ) : global_w(v2, SZ);
+  const Wm V2T = LW ? stage_w(lds, v2t, SZ) : global_w(v2t, SZ), V1T = LW ? stage_w(lds, v1t, SZ) : global_w(v1t, SZ);
+  __syncthreads();
+  Lane L;
+  const int b = blockIdx.y;
+  const int64_t nt = (rows.voxels + 15) / 16;
+  for (int64_t tile = (int64_t)blockIdx.x * NW + L.wave; tile < nt; tile += (int64_t)gridDim.x * NW) {
+    const int64_t vox = tile * 16 + L.v;
+    const bool ok = vox < rows.voxels;
+    const int64_t row = (int64_t)b * rows.voxels + vox;
+    f32x4 h[CT], hb[CT], xb[CT];
+    load_act<CT>(hin, row, ok, L.q, h);
+    load_act<CT>(hbar_in, row, ok, L.q, hb);
+    Frag hf[KS];
+    to_frags<CT>(h, hf);
+    load_act<CT>(xsbar, row, ok, L.q, xb);
+    gate_bwd<CT>(G1, G2, G2T, G1T, hf, h, xb, hb, L.v, L.q);
+    load_act<CT>(xvbar, row, ok, L.q, xb);
+    gate_bwd<CT>(V1, V2, V2T, V1T, hf, h, xb, hb, L.v, L.q);
+    float zs = 0.f;
+#pragma unroll
+    for (int t = 0; t < CT; ++t) {
+      const f32x4 pr = xb[t] * *(const f32x4*)(z_proj + 16 * t + 4 * L.q);
+      zs += pr[0] + pr[1] + pr[2] + pr[3];
+    }
+    store_act<CT>(hbar_out, row, ok, L.q, hb);
+    zs += __shfl_xor(zs, 16);
+    zs += __shfl_xor(zs, 32);
+    if (zbar && ok && L.q == 0) zbar[row] += zs;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// head adjoint
+// ---------------------------------------------------------------------------------------------------------------
+constexpr float LN10 = 2.302585092994046f;
+
+template <int CT, int NW, int NTH>
+__global__ __launch_bounds__(NW * 64) void k_head_bwd(FfnoRows rows, const float* __restrict__ hin, const uint16_t* p1,
+                                                      const uint16_t* p2, const uint16_t* p2t, const uint16_t* p1t,
+                                                      const float* __restrict__ p2_bias, const float* __restrict__ x_in,
+                                                      FfnoHeadDesc hd, const float* __restrict__ ybar,
+                                                      float* __restrict__ hbar, float* __restrict__ xbar_in) {
+  constexpr int C = CT * 16, KS1 = ks_of(CT);
+  extern __shared__ __align__(16) unsigned char smem[];
+  bf16* lds = reinterpret_cast<bf16*>(smem);
+  const Wm P1 = stage_w(lds, p1, 2 * C * KS1 * 32), P2 = stage_w(lds, p2, NTH * 16 * 2 * C);
+  const Wm P2T = stage_w(lds, p2t, 2 * C * 32), P1T = stage_w(lds, p1t, C * 2 * C);
+  float* hs = reinterpret_cast<float*>(lds);
+  __syncthreads();
+  Lane L;
+  const int b = blockIdx.y;
+  const int64_t nt = (rows.voxels + 15) / 16;
+  volatile float* mine = hs + L.wave * 16 * NTH * 16;
+  int n_out = 0;
+  for (int s = 0; s < hd.n_species; ++s) n_out += hd.levels[s];
+  for (int64_t tile = (int64_t)blockIdx.x * NW + L.wave; tile < nt; tile += (int64_t)gridDim.x * NW) {
+    const int64_t vox = tile * 16 + L.v;
+    const bool ok = vox < rows.voxels;
+    const int64_t row = (int64_t)b * rows.voxels + vox;
+    f32x4 x[CT];
+    load_act<CT>(hin, row, ok, L.q, x);
+    Frag xf[KS1];
+    to_frags<CT>(x, xf);
+    f32x4 o[NTH];
+    mlp2<KS1, 2 * CT, NTH>(P1, P2, xf, o, L.v, L.q);
+#pragma unroll
+    for (int t = 0; t < NTH; ++t) {
+      const f32x4 hvv = o[t] + *(const f32x4*)(p2_bias + 16 * t + 4 * L.q);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) mine[L.v * (NTH * 16) + 16 * t + 4 * L.q + j] = hvv[j];
+    }
+    if (L.q == 0) {
+      // one lane per voxel: the level-fraction head and its adjoint (same lockstep/volatile ordering as k_head)
+      volatile float* hv = mine + L.v * (NTH * 16);
+      float gt_ = 0.f, gne = 0.f, grho = 0.f;
+      if (ok) {
+        auto feat = [&](int c) { return x_in[((int64_t)b * 6 + c) * rows.voxels + vox] * hd.std_x[c] + hd.mean_x[c]; };
+        const float lt = feat(hd.t_channel) * LN10;
+        const float lne = (feat(hd.ne_channel) - 6.0f) * LN10;
+        const float inv_t = __expf(-lt);
+        const float conl = hd.ln_ccon + lne - 1.5f * lt;
+        const float dxi = hd.dxi_coef * __expf(0.5f * (lne + hd.ln_k - lt));
+        int c = 0, k = 0;
+        for (int s = 0; s < hd.n_species; ++s) {
+          const int nl = hd.levels[s];
+          float logit[15], a[15];
+          float mx = -3.0e38f, asum = 0.f;
+          for (int i = 0; i < nl; ++i) {
+            const int li = k + i;
+            logit[i] = hv[c + i] + hd.log_g[li] - hd.e_over_k * inv_t * (hd.ev[li] - dxi * hd.ndxi[li]) -
+                       hd.dion[li] * conl;
+            mx = fmaxf(mx, logit[i]);
+            a[i] = ybar[((int64_t)b * n_out + li) * rows.voxels + vox] / hd.std_y[li];
+            asum += a[i];
+          }
+          float se = 0.f;
+          for (int i = 0; i < nl; ++i) se += __expf(logit[i] - mx);
+          const float lse = mx + __logf(se);
+          for (int i = 0; i < nl; ++i) {
+            const int li = k + i;
+            const float p = __expf(logit[i] - lse);
+            const float lb = (a[i] - p * asum) / LN10;          // gradient of the logit = head + ln f*
+            hv[c + i] = lb;
+            // d ln f*_i / d ln T and d ln ne (MULTI3D weights, models/ffno_model.py:131-141)
+            gt_ += lb * (hd.e_over_k * inv_t * (hd.ev[li] - 1.5f * dxi * hd.ndxi[li]) + 1.5f * hd.dion[li]);
+            gne += lb * (0.5f * hd.e_over_k * inv_t * dxi * hd.ndxi[li] - hd.dion[li]);
+          }
+          const float th = tanhf(hv[c + nl] / hd.total_bound);
+          hv[c + nl] = asum * (1.f - th * th);
+          grho += asum;
+          c += nl + 1;
+          k += nl;
+        }
+        for (int i = c; i < NTH * 16; ++i) hv[i] = 0.f;
+      } else {
+        for (int i = 0; i < NTH * 16; ++i) hv[i] = 0.f;
+      }
+      if (ok) {
+        for (int ch = 0; ch < 6; ++ch) {
+          float g = 0.f;
+          if (ch == hd.t_channel) g += gt_ * LN10 * hd.std_x[ch];
+          if (ch == hd.ne_channel) g += gne * LN10 * hd.std_x[ch];
+          if (ch == hd.rho_channel) g += grho * hd.std_x[ch];
+          xbar_in[((int64_t)b * 6 + ch) * rows.voxels + vox] = g;
+        }
+      }
+    }
+    f32x4 hb[NTH];
+#pragma unroll
+    for (int t = 0; t < NTH; ++t)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) hb[t][j] = mine[L.v * (NTH * 16) + 16 * t + 4 * L.q + j];
+    Frag hbf[1];
+    hbf[0] = split8(hb[0], NTH > 1 ? hb[NTH - 1] : zero4());
+    f32x4 acc[CT];
+#pragma unroll
+    for (int t = 0; t < CT; ++t) acc[t] = zero4();
+    mlp2_bwd<KS1, 2 * CT, 1, CT>(P1, P2T, P1T, xf, hbf, acc, L.v, L.q);
+    store_act<CT>(hbar, row, ok, L.q, acc);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// lift adjoint
+// ---------------------------------------------------------------------------------------------------------------
+template <int CT, int NW>
+__global__ __launch_bounds__(NW * 64) void k_lift_bwd(FfnoRows rows, const float* __restrict__ ubar, FfnoGnBack gb,
+                                                      const float* __restrict__ w_lift, float* __restrict__ xbar_in) {
+  constexpr int C = CT * 16;
+  __shared__ float wl[C * 6];
+  for (int i = threadIdx.x; i < C * 6; i += blockDim.x) wl[i] = w_lift[i];
+  __syncthreads();
+  Lane L;
+  const int b = blockIdx.y;
+  const float scale = gb.scale ? *gb.scale : 1.f;
+  const int64_t nt = (rows.voxels + 15) / 16;
+  for (int64_t tile = (int64_t)blockIdx.x * NW + L.wave; tile < nt; tile += (int64_t)gridDim.x * NW) {
+    const int64_t vox = tile * 16 + L.v;
+    const bool ok = vox < rows.voxels;
+    const int64_t row = (int64_t)b * rows.voxels + vox;
+    f32x4 ub[CT], tb[CT];
+    load_act<CT>(ubar, row, ok, L.q, ub);
+    gn_back<CT>(gb, scale, row, b, ok, L.q, ub, tb);
+    float acc[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      float a = 0.f;
+#pragma unroll
+      for (int t = 0; t < CT; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) a += wl[(16 * t + 4 * L.q + j) * 6 + c] * tb[t][j];
+      a += __shfl_xor(a, 16);
+      a += __shfl_xor(a, 32);
+      acc[c] = a;
+    }
+    if (ok && L.q == 0)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) xbar_in[((int64_t)b * 6 + c) * rows.voxels + vox] += acc[c];
+  }
+}
+
+}  // namespace
+}  // namespace ffno
+
+using namespace ffno;
+
+namespace {
+bool gb_ok(const FfnoGnBack& g) { return g.t && g.stats && g.gamma && g.depth_w && (!g.act_gelu || (g.aff_a && g.aff_b)); }
+}  // namespace
+
+extern "C" {
+
+int ffno_gn_grad_stats(FfnoRows rows, FfnoGnBack gn, const float* ubar, float* partials, hipStream_t stream) {
+  if (!rows_ok(rows) || !gb_ok(gn) || !ubar || !partials) return fail("ffno_gn_grad_stats: bad arguments");
+  dim3 grid(rows.nwg, rows.B);
+  FFNO_DISPATCH_CT(rows.C, (k_gn_grad_stats<CT, NWV><<<grid, NWV * 64, 0, stream>>>(rows, gn, ubar, partials)));
+  return check_launch("ffno_gn_grad_stats");
+}
+
+int ffno_gn_grad_finalize(int B, int nwg, const float* partials, float* sums, hipStream_t stream) {
+  if (B <= 0 || nwg <= 0 || !partials || !sums) return fail("ffno_gn_grad_finalize: bad arguments");
+  k_gn_grad_finalize<<<B, 64, 0, stream>>>(nwg, partials, sums);
+  return check_launch("ffno_gn_grad_finalize");
+}
+
+int ffno_mlp_bwd(FfnoRows rows, const float* ubar, FfnoGnBack gn, const float* x, const uint16_t* w1,
+                 const uint16_t* w2t, const uint16_t* w1t, int residual, float* xbar, hipStream_t stream) {
+  if (!rows_ok(rows) || !gb_ok(gn) || !gn.sums || !ubar || !x || !w1 || !w2t || !w1t || !xbar)
+    return fail("ffno_mlp_bwd: bad arguments");
+  dim3 grid(rows.nwg, rows.B);
+  FFNO_DISPATCH_CT(rows.C, {
+    constexpr int C = CT * 16, KP = ks_of(CT) * 32;
+    const size_t lds = ((size_t)2 * C * KP * 2 + (size_t)C * 2 * C) * 2 * 2;
+    auto k = k_mlp_bwd<CT, NWV>;
+    if (set_lds(k, lds)) return -1;
+    k<<<grid, NWV * 64, lds, stream>>>(rows, ubar, gn, x, w1, w2t, w1t, residual, xbar);
+  });
+  return check_launch("ffno_mlp_bwd");
+}
+
+int ffno_fuse_bwd(FfnoRows rows, const float* x1bar, FfnoGnBack gnf, const float* h, const float* sp, const float* vp,
+                  const float* s_a, const float* s_b, const float* v_a, const float* v_b, const uint16_t* bg1,
+                  const uint16_t* bg2, const float* bg2_bias, const uint16_t* f1, const uint16_t* f2t,
+                  const uint16_t* f1t, const uint16_t* bg2t, const uint16_t* bg1t, float* hbar, float* sbar,
+                  float* vbar, hipStream_t stream) {
+  if (!rows_ok(rows) || !gb_ok(gnf) || !gnf.sums || !x1bar || !h || !sp || !vp || !s_a || !s_b || !v_a || !v_b ||
+      !bg1 || !bg2 || !bg2_bias || !f1 || !f2t || !f1t || !bg2t || !bg1t || !hbar || !sbar || !vbar)
+    return fail("ffno_fuse_bwd: bad arguments");
+  dim3 grid(rows.nwg, rows.B);
+  FFNO_DISPATCH_CT(rows.C, (k_fuse_bwd<CT, NWV><<<grid, NWV * 64, 0, stream>>>(
+                               rows, x1bar, gnf, h, sp, vp, s_a, s_b, v_a, v_b, bg1, bg2, bg2_bias, f1, f2t, f1t, bg2t,
+                               bg1t, hbar, sbar, vbar)));
+  return check_launch("ffno_fuse_bwd");
+}
+
+int ffno_gates_bwd(FfnoRows rows, const float* h, const float* xsbar, const float* xvbar, const float* hbar_in,
+                   const uint16_t* g1, const uint16_t* g2, const uint16_t* g2t, const uint16_t* g1t,
+                   const uint16_t* v1, const uint16_t* v2, const uint16_t* v2t, const uint16_t* v1t,
+                   const float* z_proj, float* hbar_out, float* zbar, hipStream_t stream) {
+  if (!rows_ok(rows) || !h || !xsbar || !xvbar || !hbar_in || !g1 || !g2 || !g2t || !g1t || !v1 || !v2 || !v2t ||
+      !v1t || !z_proj || !hbar_out)
+    return fail("ffno_gates_bwd: bad arguments");
+  dim3 grid(rows.nwg, rows.B);
+  FFNO_DISPATCH_CT(rows.C, {
+    const size_t lds = 8 * (size_t)(CT * 16) * ks_of(CT) * 32 * 2 * 2;
+    if (lds <= 160 * 1024) {
+      auto k = k_gates_bwd<CT, NWV, true>;
+      if (set_lds(k, lds)) return -1;
+      k<<<grid, NWV * 64, lds, stream>>>(rows, h, xsbar, xvbar, hbar_in, g1, g2, g2t, g1t, v1, v2, v2t, v1t, z_proj,
+                                         hbar_out, zbar);
+    } else {
+      k_gates_bwd<CT, NWV, false><<<grid, NWV * 64, 0, stream>>>(rows, h, xsbar, xvbar, hbar_in, g1, g2, g2t, g1t, v1,
+                                                                 v2, v2t, v1t, z_proj, hbar_out, zbar);
+    }
+  });
+  return check_launch("ffno_gates_bwd");
+}
+
+int ffno_head_bwd(FfnoRows rows, const float* h, const uint16_t* p1, const uint16_t* p2, const uint16_t* p2t,
+                  const uint16_t* p1t, const float* p2_bias, const float* x_in, const FfnoHeadDesc* desc,
+                  const float* ybar, float* hbar, float* xbar_in, hipStream_t stream) {
+  if (!rows_ok(rows) || !h || !p1 || !p2 || !p2t || !p1t || !p2_bias || !x_in || !desc || !ybar || !hbar || !xbar_in)
+    return fail("ffno_head_bwd: bad arguments");
+  int n_head = 0;
+  if (desc->n_species < 1 || desc->n_species > 4) return fail("ffno_head_bwd: 1 to 4 species");
+  for (int s = 0; s < desc->n_species; ++s) {
+    if (desc->levels[s] < 1 || desc->levels[s] > 15) return fail("ffno_head_bwd: 1 to 15 levels per species");
+    n_head += desc->levels[s] + 1;
+  }
+  if (n_head > 32) return fail("ffno_head_bwd: at most 32 head channels");
+  const int nth = (n_head + 15) / 16;
+  dim3 grid(rows.nwg, rows.B);
+  FFNO_DISPATCH_CT(rows.C, {
+    constexpr int C = CT * 16;
+    const size_t w_bytes = ((size_t)2 * C * ks_of(CT) * 32 + (size_t)nth * 16 * 2 * C + (size_t)2 * C * 32 +
+                            (size_t)C * 2 * C) * 2 * 2;
+    const size_t lds = w_bytes + (size_t)NWV * 16 * nth * 16 * sizeof(float);
+    if (nth == 1) {
+      auto k = k_head_bwd<CT, NWV, 1>;
+      if (set_lds(k, lds)) return -1;
+      k<<<grid, NWV * 64, lds, stream>>>(rows, h, p1, p2, p2t, p1t, p2_bias, x_in, *desc, ybar, hbar, xbar_in);
+    } else {
+      auto k = k_head_bwd<CT, NWV, 2>;
+      if (set_lds(k, lds)) return -1;
+      k<<<grid, NWV * 64, lds, stream>>>(rows, h, p1, p2, p2t, p1t, p2_bias, x_in, *desc, ybar, hbar, xbar_in);
+    }
+  });
+  return check_launch("ffno_head_bwd");
+}
+
+int ffno_lift_bwd(FfnoRows rows, const float* ubar, FfnoGnBack gn, const float* w_lift, float* xbar_in,
+                  hipStream_t stream) {
+  if (!rows_ok(rows) || !gb_ok(gn) || !gn.sums || !ubar || !w_lift || !xbar_in) return fail("ffno_lift_bwd: bad arguments");
+  dim3 grid(rows.nwg, rows.B);
+  FFNO_DISPATCH_CT(rows.C, (k_lift_bwd<CT, NWV><<<grid, NWV * 64, 0, stream>>>(rows, ubar, gn, w_lift, xbar_in)));
+  return check_launch("ffno_lift_bwd");
+}
+
+}  // extern "C"

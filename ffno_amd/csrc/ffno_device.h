@@ -1,0 +1,360 @@
+// Device helpers shared by the FFNO3D forward and adjoint kernels (gfx950).
+//
+// MFMA conventions (v_mfma_f32_16x16x32_bf16, 64 lanes; lane l: v = l & 15, q = l >> 4):
+//   A operand (16 x 32): lane l supplies row v and 8 k-values; B operand (32 x 16): lane l supplies column v and the
+//   same 8 k-slots; D (16 x 16 f32): lane l holds rows 4q..4q+3 of column v.
+// Pointwise layers are computed transposed, out^T = W * in^T: rows are output features, columns are 16 voxels.
+// A lane then holds, for every 16-feature tile t, features 16t+4q..16t+4q+3 of voxel v (the "lane layout"), both for
+// what it loads and for what an MFMA leaves in it. Two such float4s (tiles 2s, 2s+1) make the B operand of k-step s;
+// the weights are packed on the host with the matching k permutation, so the hidden activations of a layer feed the
+// next layer straight from registers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include "ffno_hip.h"
+
+namespace ffno {
+
+
+typedef __bf16 bf16;
+typedef bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int G = 8;               // GroupNorm groups for every supported width (models/ffno_model.py:21-25)
+
+__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ f32x4 zero4() { return f32x4{0.f, 0.f, 0.f, 0.f}; }
+
+__device__ __forceinline__ bf16x8 pack8(f32x4 a, f32x4 b) {
+  bf16x8 r;
+  r[0] = (bf16)a[0]; r[1] = (bf16)a[1]; r[2] = (bf16)a[2]; r[3] = (bf16)a[3];
+  r[4] = (bf16)b[0]; r[5] = (bf16)b[1]; r[6] = (bf16)b[2]; r[7] = (bf16)b[3];
+  return r;
+}
+
+__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+__device__ __forceinline__ float sigm(float x) { return 1.0f / (1.0f + __expf(-x)); }
+__device__ __forceinline__ f32x4 gelu4(f32x4 x) { return f32x4{gelu(x[0]), gelu(x[1]), gelu(x[2]), gelu(x[3])}; }
+__device__ __forceinline__ f32x4 sigm4(f32x4 x) { return f32x4{sigm(x[0]), sigm(x[1]), sigm(x[2]), sigm(x[3])}; }
+
+__device__ __forceinline__ f32x4 ld4(const float* p, bool ok) { return ok ? *(const f32x4*)p : zero4(); }
+__device__ __forceinline__ void st4(float* p, f32x4 v, bool ok) { if (ok) *(f32x4*)p = v; }
+
+constexpr int ks_of(int ntiles) { return (ntiles + 1) / 2; }   // k-steps of 32 for ntiles 16-feature tiles
+
+// split-bf16 operands: x = hi + lo with hi = bf16(x), lo = bf16(x - hi); products hi*hi + hi*lo + lo*hi keep about
+// 16 mantissa bits, so every MFMA layer is accurate to ~1e-5 relative with float32 accumulation
+struct Frag {
+  bf16x8 h, l;
+};
+
+__device__ __forceinline__ Frag split8(f32x4 a, f32x4 b) {
+  Frag f;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const bf16 ha = (bf16)a[j], hb = (bf16)b[j];
+    f.h[j] = ha;
+    f.h[4 + j] = hb;
+    f.l[j] = (bf16)(a[j] - (float)ha);
+    f.l[4 + j] = (bf16)(b[j] - (float)hb);
+  }
+  return f;
+}
+
+__device__ __forceinline__ f32x4 mfma3(bf16x8 ah, bf16x8 al, const Frag& b, f32x4 c) {
+  c = mfma(ah, b.h, c);
+  c = mfma(ah, b.l, c);
+  return mfma(al, b.h, c);
+}
+
+template <int NT>
+__device__ __forceinline__ void to_frags(const f32x4 (&a)[NT], Frag* f) {
+#pragma unroll
+  for (int s = 0; s < ks_of(NT); ++s) f[s] = split8(a[2 * s], (2 * s + 1 < NT) ? a[2 * s + 1] : zero4());
+}
+
+// A weight matrix [rows][K] in kernel form (k permuted per 32-block on the host), hi and lo halves; the pointers are
+// LDS or global memory
+struct Wm {
+  const bf16* h;
+  const bf16* l;
+};
+
+// one 16-row output tile t of W * in^T over KS k-steps
+template <int KS>
+__device__ __forceinline__ f32x4 tile(Wm W, int t, const Frag* b, int v, int q) {
+  f32x4 c = zero4();
+#pragma unroll
+  for (int s = 0; s < KS; ++s) {
+    const int o = (16 * t + v) * (KS * 32) + 32 * s + 8 * q;
+    c = mfma3(*(const bf16x8*)(W.h + o), *(const bf16x8*)(W.l + o), b[s], c);
+  }
+  return c;
+}
+
+// out^T[NT tiles] = W[NT*16 rows][KS*32] * in^T
+template <int NT, int KS>
+__device__ __forceinline__ void gemm(Wm W, const Frag* b, f32x4 (&acc)[NT], int v, int q) {
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = tile<KS>(W, t, b, v, q);
+}
+
+// d/dx of the exact GELU: Phi(x) + x phi(x)
+__device__ __forceinline__ float gelu_grad(float x) {
+  return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * 0.3989422804014327f * __expf(-0.5f * x * x);
+}
+__device__ __forceinline__ f32x4 gelu_grad4(f32x4 x) {
+  return f32x4{gelu_grad(x[0]), gelu_grad(x[1]), gelu_grad(x[2]), gelu_grad(x[3])};
+}
+
+// Adjoint of out = W2 gelu(W1 in) with respect to in, accumulated into acc (NTI tiles of in):
+// acc += W1^T (gelu'(W1 in) * (W2^T og)). W2T and W1T are the transposed weights in kernel form; the hidden
+// pre-activations are recomputed two tiles at a time, as in mlp2.
+template <int KSI, int NH, int KSO, int NTI>
+__device__ __forceinline__ void mlp2_bwd(Wm W1, Wm W2T, Wm W1T, const Frag* in, const Frag* og, f32x4 (&acc)[NTI],
+                                         int v, int q) {
+  constexpr int KSH = ks_of(NH);
+#pragma unroll 1
+  for (int s = 0; s < KSH; ++s) {
+    f32x4 d0 = tile<KSO>(W2T, 2 * s, og, v, q) * gelu_grad4(tile<KSI>(W1, 2 * s, in, v, q));
+    f32x4 d1 = zero4();
+    if (2 * s + 1 < NH) d1 = tile<KSO>(W2T, 2 * s + 1, og, v, q) * gelu_grad4(tile<KSI>(W1, 2 * s + 1, in, v, q));
+    const Frag df = split8(d0, d1);
+#pragma unroll
+    for (int t = 0; t < NTI; ++t) {
+      const int o = (16 * t + v) * (KSH * 32) + 32 * s + 8 * q;
+      acc[t] = mfma3(*(const bf16x8*)(W1T.h + o), *(const bf16x8*)(W1T.l + o), df, acc[t]);
+    }
+  }
+}
+
+struct GeluAct {
+  __device__ __forceinline__ f32x4 operator()(f32x4 x) const { return gelu4(x); }
+};
+
+// out = W2 * gelu(W1 * in): the NH hidden tiles are produced two at a time and folded straight into the NT2
+// output accumulators, so the hidden layer is never held whole
+template <int KS1, int NH, int NT2>
+__device__ __forceinline__ void mlp2(Wm W1, Wm W2, const Frag* in, f32x4 (&out)[NT2], int v, int q) {
+  constexpr int KS2 = ks_of(NH);
+#pragma unroll
+  for (int t = 0; t < NT2; ++t) out[t] = zero4();
+  // a rolled loop: unrolled, the scheduler hoists every weight fragment of the layer into registers and spills
+#pragma unroll 1
+  for (int s = 0; s < KS2; ++s) {
+    const f32x4 h0 = gelu4(tile<KS1>(W1, 2 * s, in, v, q));
+    const f32x4 h1 = (2 * s + 1 < NH) ? gelu4(tile<KS1>(W1, 2 * s + 1, in, v, q)) : zero4();
+    const Frag hf = split8(h0, h1);
+#pragma unroll
+    for (int t = 0; t < NT2; ++t) {
+      const int o = (16 * t + v) * (KS2 * 32) + 32 * s + 8 * q;
+      out[t] = mfma3(*(const bf16x8*)(W2.h + o), *(const bf16x8*)(W2.l + o), hf, out[t]);
+    }
+  }
+}
+
+__device__ __forceinline__ void stage(bf16* dst, const uint16_t* src, int n) {   // n bf16, multiple of 8
+  const uint4* s = reinterpret_cast<const uint4*>(src);
+  uint4* d = reinterpret_cast<uint4*>(dst);
+  for (int i = threadIdx.x; i < n / 8; i += blockDim.x) d[i] = s[i];
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// GroupNorm statistics, accumulated per lane with a shift, merged (Chan) across lanes, waves and workgroups.
+// ---------------------------------------------------------------------------------------------------------------
+__device__ __forceinline__ void chan(float& W, float& m, float& M2, float W2, float m2, float M22) {
+  float Wt = W + W2;
+  if (Wt <= 0.f) return;
+  float d = m2 - m;
+  m += d * (W2 / Wt);
+  M2 += M22 + d * d * (W * (W2 / Wt));
+  W = Wt;
+}
+
+template <int CT>
+struct GnAcc {
+  static constexpr int CPG = CT * 16 / G;          // channels per group: 2, 4, 8, 12, 16
+  static constexpr int SUB = CPG >= 4 ? 1 : 2;     // sub-chunks per 4-feature chunk
+  static constexpr int NF = 4 / SUB;               // features per sub-chunk
+  float s1[CT * SUB], s2[CT * SUB], k[CT * SUB];
+  float wsum;
+  bool seen;
+
+  __device__ void init() {
+#pragma unroll
+    for (int i = 0; i < CT * SUB; ++i) { s1[i] = 0.f; s2[i] = 0.f; k[i] = 0.f; }
+    wsum = 0.f;
+    seen = false;
+  }
+  __device__ __forceinline__ void add(const f32x4 (&y)[CT], float w, bool valid) {
+    if (!valid) return;
+    if (!seen) {
+#pragma unroll
+      for (int t = 0; t < CT; ++t)
+#pragma unroll
+        for (int u = 0; u < SUB; ++u) {
+          float a = 0.f;
+#pragma unroll
+          for (int j = 0; j < NF; ++j) a += y[t][u * NF + j];
+          k[t * SUB + u] = a / NF;
+        }
+      seen = true;
+    }
+    wsum += w;
+#pragma unroll
+    for (int t = 0; t < CT; ++t)
+#pragma unroll
+      for (int u = 0; u < SUB; ++u) {
+        float a = 0.f, b = 0.f;
+#pragma unroll
+        for (int j = 0; j < NF; ++j) {
+          float d = y[t][u * NF + j] - k[t * SUB + u];
+          a += d;
+          b += d * d;
+        }
+        s1[t * SUB + u] += w * a;
+        s2[t * SUB + u] += w * b;
+      }
+  }
+  // merge everything the workgroup saw into partials[g][3]; lds needs NW*4*CT*SUB*3 floats
+  template <int NW>
+  __device__ void flush(float* lds, float* partials, int v, int q, int wave) {
+#pragma unroll
+    for (int i = 0; i < CT * SUB; ++i) {
+      float W = NF * wsum;
+      float m = W > 0.f ? k[i] + s1[i] / W : 0.f;
+      float M2 = W > 0.f ? fmaxf(s2[i] - s1[i] * s1[i] / W, 0.f) : 0.f;
+#pragma unroll
+      for (int off = 1; off < 16; off <<= 1) {
+        float W2 = __shfl_xor(W, off), m2 = __shfl_xor(m, off), M22 = __shfl_xor(M2, off);
+        chan(W, m, M2, W2, m2, M22);
+      }
+      if (v == 0) {
+        float* e = lds + ((wave * 4 + q) * (CT * SUB) + i) * 3;
+        e[0] = W; e[1] = m; e[2] = M2;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < G) {
+      const int g = threadIdx.x;
+      float W = 0.f, m = 0.f, M2 = 0.f;
+      for (int w = 0; w < NW; ++w)
+        for (int qq = 0; qq < 4; ++qq)
+          for (int i = 0; i < CT * SUB; ++i) {
+            int t = i / SUB, u = i % SUB;
+            int grp = (16 * t + 4 * qq + u * NF) / CPG;
+            if (grp != g) continue;
+            const float* e = lds + ((w * 4 + qq) * (CT * SUB) + i) * 3;
+            chan(W, m, M2, e[0], e[1], e[2]);
+          }
+      partials[g * 3 + 0] = W;
+      partials[g * 3 + 1] = m;
+      partials[g * 3 + 2] = M2;
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------------------------------------------
+// shared pieces of the voxel-row kernels
+// ---------------------------------------------------------------------------------------------------------------
+struct Lane {
+  int wave, lane, v, q;
+  __device__ Lane() {
+    wave = threadIdx.x >> 6;
+    lane = threadIdx.x & 63;
+    v = lane & 15;
+    q = lane >> 4;
+  }
+};
+
+template <int CT>
+__device__ __forceinline__ void load_act(const float* p, int64_t row, bool ok, int q, f32x4 (&x)[CT]) {
+#pragma unroll
+  for (int t = 0; t < CT; ++t) x[t] = ld4(p + row * (CT * 16) + 16 * t + 4 * q, ok);
+}
+
+template <int CT>
+__device__ __forceinline__ void store_act(float* p, int64_t row, bool ok, int q, const f32x4 (&x)[CT]) {
+#pragma unroll
+  for (int t = 0; t < CT; ++t) st4(p + row * (CT * 16) + 16 * t + 4 * q, x[t], ok);
+}
+
+template <int CT>
+__device__ __forceinline__ void assemble(const FfnoAssemble& in, float scale, int64_t row, int b, bool ok, int q,
+                                         f32x4 (&x)[CT]) {
+  constexpr int C = CT * 16;
+#pragma unroll
+  for (int t = 0; t < CT; ++t) {
+    const int c0 = 16 * t + 4 * q;
+    f32x4 tv = ld4(in.t + row * C + c0, ok);
+    f32x4 a = *(const f32x4*)(in.aff_a + b * C + c0);
+    f32x4 s = *(const f32x4*)(in.aff_b + b * C + c0);
+    f32x4 u = tv * a + s;
+    if (in.act_gelu) u = gelu4(u);
+    f32x4 base = in.base ? ld4(in.base + row * C + c0, ok) : zero4();
+    x[t] = ok ? base + scale * u : zero4();
+  }
+}
+
+template <int CT>
+__device__ __forceinline__ void gelu_tiles(f32x4 (&x)[CT]) {
+#pragma unroll
+  for (int t = 0; t < CT; ++t) x[t] = gelu4(x[t]);
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// weights: packed [2][rows][K] (hi then lo) in global memory; staged into LDS when the kernel's set fits
+// ---------------------------------------------------------------------------------------------------------------
+__device__ __forceinline__ Wm stage_w(bf16*& lds, const uint16_t* g, int n) {   // n = rows * K of one half
+  stage(lds, g, 2 * n);
+  Wm w{lds, lds + n};
+  lds += 2 * n;
+  return w;
+}
+
+__device__ __forceinline__ Wm global_w(const uint16_t* g, int n) {
+  return Wm{reinterpret_cast<const bf16*>(g), reinterpret_cast<const bf16*>(g) + n};
+}
+
+__device__ __forceinline__ int64_t imap(int64_t i, int64_t split, int64_t hi, int64_t lo) {
+  return (i / split) * hi + (i % split) * lo;
+}
+
+
+// error reporting of the C ABI (ffno_forward.hip): a thread-local message and a negative return code
+int fail(const char* msg);
+int check_launch(const char* what);
+
+template <typename K>
+int set_lds(K kernel, size_t bytes) {
+  if (bytes > 160 * 1024) return fail("kernel needs more LDS than a CU has");
+  if (bytes > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return fail("hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed");
+  }
+  return 0;
+}
+
+inline bool rows_ok(const FfnoRows& r) { return r.B > 0 && r.voxels > 0 && r.nwg > 0 && r.C % 16 == 0; }
+
+constexpr int NWV = 4;   // waves per workgroup of the voxel-row kernels
+
+#define FFNO_DISPATCH_CT(C, ...)                          \
+  switch ((C) / 16) {                                     \
+    case 1: { constexpr int CT = 1; __VA_ARGS__; break; } \
+    case 2: { constexpr int CT = 2; __VA_ARGS__; break; } \
+    case 4: { constexpr int CT = 4; __VA_ARGS__; break; } \
+    case 6: { constexpr int CT = 6; __VA_ARGS__; break; } \
+    case 8: { constexpr int CT = 8; __VA_ARGS__; break; } \
+    default: return fail("unsupported width (16, 32, 64, 96 or 128)"); \
+  }
+
+template <int CT>
+constexpr size_t red_bytes() { return (size_t)NWV * 4 * CT * GnAcc<CT>::SUB * 3 * sizeof(float); }
+
+}  // namespace ffno

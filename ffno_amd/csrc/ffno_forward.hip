@@ -1,219 +1,27 @@
-// FFNO3D forward kernels for gfx950 (MI355X): register-chained bf16 MFMA MLPs with fused GroupNorm statistics,
-// a split-bf16 twiddle GEMM for the DFTs and the vertical Filon transform, and the per-mode spectral channel mix.
-//
-// MFMA conventions (v_mfma_f32_16x16x32_bf16, 64 lanes; lane l: v = l & 15, q = l >> 4):
-//   A operand (16 x 32): lane l supplies row v and 8 k-values; B operand (32 x 16): lane l supplies column v and the
-//   same 8 k-slots; D (16 x 16 f32): lane l holds rows 4q..4q+3 of column v.
-// Pointwise layers are computed transposed, out^T = W * in^T: rows are output features, columns are 16 voxels.
-// A lane then holds, for every 16-feature tile t, features 16t+4q..16t+4q+3 of voxel v (the "lane layout"), both for
-// what it loads and for what an MFMA leaves in it. Two such float4s (tiles 2s, 2s+1) make the B operand of k-step s;
-// the weights are packed on the host with the matching k permutation, so the hidden activations of a layer feed the
-// next layer straight from registers.
-#include <hip/hip_runtime.h>
-#include <stdint.h>
-#include <stdio.h>
-#include "ffno_hip.h"
+// FFNO3D forward kernels for gfx950 (MI355X): register-chained split-bf16 MFMA MLPs with fused GroupNorm
+// statistics, the split-bf16 twiddle GEMM for the DFTs and the vertical Filon transform, and the per-mode
+// spectral channel mix. Conventions are in ffno_device.h.
+#include "ffno_device.h"
+
+namespace ffno {
+
+thread_local char g_err[512] = "";
+
+int fail(const char* msg) {
+  snprintf(g_err, sizeof(g_err), "%s", msg);
+  return -1;
+}
+
+int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    snprintf(g_err, sizeof(g_err), "%s: %s", what, hipGetErrorString(e));
+    return -2;
+  }
+  return 0;
+}
 
 namespace {
-
-typedef __bf16 bf16;
-typedef bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-
-constexpr int G = 8;               // GroupNorm groups for every supported width (models/ffno_model.py:21-25)
-
-__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
-  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
-}
-
-__device__ __forceinline__ f32x4 zero4() { return f32x4{0.f, 0.f, 0.f, 0.f}; }
-
-__device__ __forceinline__ bf16x8 pack8(f32x4 a, f32x4 b) {
-  bf16x8 r;
-  r[0] = (bf16)a[0]; r[1] = (bf16)a[1]; r[2] = (bf16)a[2]; r[3] = (bf16)a[3];
-  r[4] = (bf16)b[0]; r[5] = (bf16)b[1]; r[6] = (bf16)b[2]; r[7] = (bf16)b[3];
-  return r;
-}
-
-__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
-__device__ __forceinline__ float sigm(float x) { return 1.0f / (1.0f + __expf(-x)); }
-__device__ __forceinline__ f32x4 gelu4(f32x4 x) { return f32x4{gelu(x[0]), gelu(x[1]), gelu(x[2]), gelu(x[3])}; }
-__device__ __forceinline__ f32x4 sigm4(f32x4 x) { return f32x4{sigm(x[0]), sigm(x[1]), sigm(x[2]), sigm(x[3])}; }
-
-__device__ __forceinline__ f32x4 ld4(const float* p, bool ok) { return ok ? *(const f32x4*)p : zero4(); }
-__device__ __forceinline__ void st4(float* p, f32x4 v, bool ok) { if (ok) *(f32x4*)p = v; }
-
-constexpr int ks_of(int ntiles) { return (ntiles + 1) / 2; }   // k-steps of 32 for ntiles 16-feature tiles
-
-template <int NT>
-__device__ __forceinline__ void to_frags(const f32x4 (&a)[NT], bf16x8* f) {
-#pragma unroll
-  for (int s = 0; s < ks_of(NT); ++s) f[s] = pack8(a[2 * s], (2 * s + 1 < NT) ? a[2 * s + 1] : zero4());
-}
-
-// out^T[NT tiles] = W[NT*16 rows][KS*32] * in^T, W in LDS (row-major, K permuted per 32-block on the host)
-template <int NT, int KS>
-__device__ __forceinline__ void gemm(const bf16* W, const bf16x8* b, f32x4 (&acc)[NT], int v, int q) {
-#pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    f32x4 c = zero4();
-#pragma unroll
-    for (int s = 0; s < KS; ++s) {
-      bf16x8 a = *(const bf16x8*)(W + (16 * t + v) * (KS * 32) + 32 * s + 8 * q);
-      c = mfma(a, b[s], c);
-    }
-    acc[t] = c;
-  }
-}
-
-__device__ __forceinline__ void stage(bf16* dst, const uint16_t* src, int n) {   // n bf16, multiple of 8
-  const uint4* s = reinterpret_cast<const uint4*>(src);
-  uint4* d = reinterpret_cast<uint4*>(dst);
-  for (int i = threadIdx.x; i < n / 8; i += blockDim.x) d[i] = s[i];
-}
-
-// ---------------------------------------------------------------------------------------------------------------
-// GroupNorm statistics, accumulated per lane with a shift, merged (Chan) across lanes, waves and workgroups.
-// ---------------------------------------------------------------------------------------------------------------
-__device__ __forceinline__ void chan(float& W, float& m, float& M2, float W2, float m2, float M22) {
-  float Wt = W + W2;
-  if (Wt <= 0.f) return;
-  float d = m2 - m;
-  m += d * (W2 / Wt);
-  M2 += M22 + d * d * (W * (W2 / Wt));
-  W = Wt;
-}
-
-template <int CT>
-struct GnAcc {
-  static constexpr int CPG = CT * 16 / G;          // channels per group: 2, 4, 8, 12, 16
-  static constexpr int SUB = CPG >= 4 ? 1 : 2;     // sub-chunks per 4-feature chunk
-  static constexpr int NF = 4 / SUB;               // features per sub-chunk
-  float s1[CT * SUB], s2[CT * SUB], k[CT * SUB];
-  float wsum;
-  bool seen;
-
-  __device__ void init() {
-#pragma unroll
-    for (int i = 0; i < CT * SUB; ++i) { s1[i] = 0.f; s2[i] = 0.f; k[i] = 0.f; }
-    wsum = 0.f;
-    seen = false;
-  }
-  __device__ __forceinline__ void add(const f32x4 (&y)[CT], float w, bool valid) {
-    if (!valid) return;
-    if (!seen) {
-#pragma unroll
-      for (int t = 0; t < CT; ++t)
-#pragma unroll
-        for (int u = 0; u < SUB; ++u) {
-          float a = 0.f;
-#pragma unroll
-          for (int j = 0; j < NF; ++j) a += y[t][u * NF + j];
-          k[t * SUB + u] = a / NF;
-        }
-      seen = true;
-    }
-    wsum += w;
-#pragma unroll
-    for (int t = 0; t < CT; ++t)
-#pragma unroll
-      for (int u = 0; u < SUB; ++u) {
-        float a = 0.f, b = 0.f;
-#pragma unroll
-        for (int j = 0; j < NF; ++j) {
-          float d = y[t][u * NF + j] - k[t * SUB + u];
-          a += d;
-          b += d * d;
-        }
-        s1[t * SUB + u] += w * a;
-        s2[t * SUB + u] += w * b;
-      }
-  }
-  // merge everything the workgroup saw into partials[g][3]; lds needs NW*4*CT*SUB*3 floats
-  template <int NW>
-  __device__ void flush(float* lds, float* partials, int v, int q, int wave) {
-#pragma unroll
-    for (int i = 0; i < CT * SUB; ++i) {
-      float W = NF * wsum;
-      float m = W > 0.f ? k[i] + s1[i] / W : 0.f;
-      float M2 = W > 0.f ? fmaxf(s2[i] - s1[i] * s1[i] / W, 0.f) : 0.f;
-#pragma unroll
-      for (int off = 1; off < 16; off <<= 1) {
-        float W2 = __shfl_xor(W, off), m2 = __shfl_xor(m, off), M22 = __shfl_xor(M2, off);
-        chan(W, m, M2, W2, m2, M22);
-      }
-      if (v == 0) {
-        float* e = lds + ((wave * 4 + q) * (CT * SUB) + i) * 3;
-        e[0] = W; e[1] = m; e[2] = M2;
-      }
-    }
-    __syncthreads();
-    if (threadIdx.x < G) {
-      const int g = threadIdx.x;
-      float W = 0.f, m = 0.f, M2 = 0.f;
-      for (int w = 0; w < NW; ++w)
-        for (int qq = 0; qq < 4; ++qq)
-          for (int i = 0; i < CT * SUB; ++i) {
-            int t = i / SUB, u = i % SUB;
-            int grp = (16 * t + 4 * qq + u * NF) / CPG;
-            if (grp != g) continue;
-            const float* e = lds + ((w * 4 + qq) * (CT * SUB) + i) * 3;
-            chan(W, m, M2, e[0], e[1], e[2]);
-          }
-      partials[g * 3 + 0] = W;
-      partials[g * 3 + 1] = m;
-      partials[g * 3 + 2] = M2;
-    }
-  }
-};
-
-// ---------------------------------------------------------------------------------------------------------------
-// shared pieces of the voxel-row kernels
-// ---------------------------------------------------------------------------------------------------------------
-struct Lane {
-  int wave, lane, v, q;
-  __device__ Lane() {
-    wave = threadIdx.x >> 6;
-    lane = threadIdx.x & 63;
-    v = lane & 15;
-    q = lane >> 4;
-  }
-};
-
-template <int CT>
-__device__ __forceinline__ void load_act(const float* p, int64_t row, bool ok, int q, f32x4 (&x)[CT]) {
-#pragma unroll
-  for (int t = 0; t < CT; ++t) x[t] = ld4(p + row * (CT * 16) + 16 * t + 4 * q, ok);
-}
-
-template <int CT>
-__device__ __forceinline__ void store_act(float* p, int64_t row, bool ok, int q, const f32x4 (&x)[CT]) {
-#pragma unroll
-  for (int t = 0; t < CT; ++t) st4(p + row * (CT * 16) + 16 * t + 4 * q, x[t], ok);
-}
-
-template <int CT>
-__device__ __forceinline__ void assemble(const FfnoAssemble& in, float scale, int64_t row, int b, bool ok, int q,
-                                         f32x4 (&x)[CT]) {
-  constexpr int C = CT * 16;
-#pragma unroll
-  for (int t = 0; t < CT; ++t) {
-    const int c0 = 16 * t + 4 * q;
-    f32x4 tv = ld4(in.t + row * C + c0, ok);
-    f32x4 a = *(const f32x4*)(in.aff_a + b * C + c0);
-    f32x4 s = *(const f32x4*)(in.aff_b + b * C + c0);
-    f32x4 u = tv * a + s;
-    if (in.act_gelu) u = gelu4(u);
-    f32x4 base = in.base ? ld4(in.base + row * C + c0, ok) : zero4();
-    x[t] = ok ? base + scale * u : zero4();
-  }
-}
-
-template <int CT>
-__device__ __forceinline__ void gelu_tiles(f32x4 (&x)[CT]) {
-#pragma unroll
-  for (int t = 0; t < CT; ++t) x[t] = gelu4(x[t]);
-}
 
 // ---------------------------------------------------------------------------------------------------------------
 // lift: Conv3d 6 -> C (fp32) + GroupNorm statistics
@@ -277,8 +85,9 @@ __global__ void k_gn_finalize(int nwg, int C, const float* __restrict__ partials
     mean_s[g] = m;
     rstd_s[g] = r;
     if (stats) {
-      stats[(b * G + g) * 2 + 0] = m;
-      stats[(b * G + g) * 2 + 1] = r;
+      stats[(b * G + g) * 3 + 0] = m;
+      stats[(b * G + g) * 3 + 1] = r;
+      stats[(b * G + g) * 3 + 2] = W;
     }
   }
   __syncthreads();
@@ -300,14 +109,11 @@ __global__ __launch_bounds__(NW * 64) void k_block_in(FfnoRows rows, FfnoAssembl
                                                       const uint16_t* v2, const float* __restrict__ z_proj,
                                                       const float* __restrict__ z, float* __restrict__ xs,
                                                       float* __restrict__ xv) {
-  constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32;
+  constexpr int C = CT * 16, KS = ks_of(CT), SZ = C * KS * 32;
   extern __shared__ __align__(16) unsigned char smem[];
-  bf16* W = reinterpret_cast<bf16*>(smem);
-  constexpr int SZ = C * KP;
-  stage(W, g1, SZ);
-  stage(W + SZ, g2, SZ);
-  stage(W + 2 * SZ, v1, SZ);
-  stage(W + 3 * SZ, v2, SZ);
+  bf16* lds = reinterpret_cast<bf16*>(smem);
+  const Wm G1 = stage_w(lds, g1, SZ), G2 = stage_w(lds, g2, SZ);
+  const Wm V1 = stage_w(lds, v1, SZ), V2 = stage_w(lds, v2, SZ);
   __syncthreads();
   Lane L;
   const int b = blockIdx.y;
@@ -320,27 +126,20 @@ __global__ __launch_bounds__(NW * 64) void k_block_in(FfnoRows rows, FfnoAssembl
     f32x4 x[CT];
     assemble<CT>(in, scale, row, b, ok, L.q, x);
     if (x_out) store_act<CT>(x_out, row, ok, L.q, x);
-    bf16x8 xf[KS];
+    Frag xf[KS];
     to_frags<CT>(x, xf);
-    f32x4 h[CT], g[CT];
-    bf16x8 hf[KS];
+    f32x4 g[CT];
     // horizontal gate (models/ffno_model.py:439-440)
-    gemm<CT, KS>(W, xf, h, L.v, L.q);
-    gelu_tiles<CT>(h);
-    to_frags<CT>(h, hf);
-    gemm<CT, KS>(W + SZ, hf, g, L.v, L.q);
+    mlp2<KS, CT, CT>(G1, G2, xf, g, L.v, L.q);
 #pragma unroll
     for (int t = 0; t < CT; ++t) g[t] = x[t] * sigm4(g[t]);
     store_act<CT>(xs, row, ok, L.q, g);
     // vertical gate and height projection (models/ffno_model.py:266)
-    gemm<CT, KS>(W + 2 * SZ, xf, h, L.v, L.q);
-    gelu_tiles<CT>(h);
-    to_frags<CT>(h, hf);
-    gemm<CT, KS>(W + 3 * SZ, hf, g, L.v, L.q);
+    mlp2<KS, CT, CT>(V1, V2, xf, g, L.v, L.q);
     const float zv = ok ? z[row] : 0.f;
 #pragma unroll
     for (int t = 0; t < CT; ++t) {
-      f32x4 zp = *(const f32x4*)(z_proj + 16 * t + 4 * L.q);
+      const f32x4 zp = *(const f32x4*)(z_proj + 16 * t + 4 * L.q);
       g[t] = x[t] * sigm4(g[t]) + zp * zv;
     }
     store_act<CT>(xv, row, ok, L.q, g);
@@ -353,13 +152,11 @@ __global__ __launch_bounds__(NW * 64) void k_block_in(FfnoRows rows, FfnoAssembl
 template <int CT, int NW>
 __global__ __launch_bounds__(NW * 64) void k_post(FfnoRows rows, const float* __restrict__ u, const uint16_t* w1,
                                                   const uint16_t* w2, float* __restrict__ y, FfnoStatsOut st) {
-  constexpr int C = CT * 16, KS1 = ks_of(CT), KS2 = ks_of(2 * CT);
+  constexpr int C = CT * 16, KS1 = ks_of(CT);
   extern __shared__ __align__(16) unsigned char smem[];
-  bf16* W1 = reinterpret_cast<bf16*>(smem);
-  bf16* W2 = W1 + 2 * C * KS1 * 32;
-  float* red = reinterpret_cast<float*>(W2 + C * KS2 * 32);
-  stage(W1, w1, 2 * C * KS1 * 32);
-  stage(W2, w2, C * KS2 * 32);
+  bf16* lds = reinterpret_cast<bf16*>(smem);
+  const Wm W1 = stage_w(lds, w1, 2 * C * KS1 * 32), W2 = stage_w(lds, w2, C * 2 * C);
+  float* red = reinterpret_cast<float*>(lds);
   __syncthreads();
   Lane L;
   const int b = blockIdx.y;
@@ -372,15 +169,10 @@ __global__ __launch_bounds__(NW * 64) void k_post(FfnoRows rows, const float* __
     const int64_t row = (int64_t)b * rows.voxels + vox;
     f32x4 x[CT];
     load_act<CT>(u, row, ok, L.q, x);
-    bf16x8 xf[KS1];
+    Frag xf[KS1];
     to_frags<CT>(x, xf);
-    f32x4 h[2 * CT];
-    gemm<2 * CT, KS1>(W1, xf, h, L.v, L.q);
-    gelu_tiles<2 * CT>(h);
-    bf16x8 hf[KS2];
-    to_frags<2 * CT>(h, hf);
     f32x4 o[CT];
-    gemm<CT, KS2>(W2, hf, o, L.v, L.q);
+    mlp2<KS1, 2 * CT, CT>(W1, W2, xf, o, L.v, L.q);
     store_act<CT>(y, row, ok, L.q, o);
     if (st.partials) acc.add(o, ok ? st.depth_w[row] : 0.f, ok);
   }
@@ -389,9 +181,9 @@ __global__ __launch_bounds__(NW * 64) void k_post(FfnoRows rows, const float* __
 }
 
 // ---------------------------------------------------------------------------------------------------------------
-// branch fusion
+// branch fusion; LW: weights in LDS (all but the lo half of the first branch-gate layer), else all from global
 // ---------------------------------------------------------------------------------------------------------------
-template <int CT, int NW>
+template <int CT, int NW, bool LW>
 __global__ __launch_bounds__(NW * 64) void k_fuse(FfnoRows rows, const float* __restrict__ xin,
                                                   const float* __restrict__ sp, const float* __restrict__ vp,
                                                   const float* __restrict__ s_a, const float* __restrict__ s_b,
@@ -399,17 +191,25 @@ __global__ __launch_bounds__(NW * 64) void k_fuse(FfnoRows rows, const float* __
                                                   const uint16_t* bg1, const uint16_t* bg2,
                                                   const float* __restrict__ bg2_bias, const uint16_t* f1,
                                                   const uint16_t* f2, float* __restrict__ fp, FfnoStatsOut st) {
-  constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32, KS2 = ks_of(2 * CT);
+  constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32;
+  constexpr int N1 = C * 3 * KP, N2 = 2 * C * KP, N3 = 2 * C * 2 * KP, N4 = C * 2 * C;
   extern __shared__ __align__(16) unsigned char smem[];
-  bf16* B1 = reinterpret_cast<bf16*>(smem);          // [C][3KP]
-  bf16* B2 = B1 + C * 3 * KP;                        // [2C][KP]
-  bf16* F1 = B2 + 2 * C * KP;                        // [2C][2KP]
-  bf16* F2 = F1 + 2 * C * 2 * KP;                    // [C][KS2*32]
-  float* red = reinterpret_cast<float*>(F2 + C * KS2 * 32);
-  stage(B1, bg1, C * 3 * KP);
-  stage(B2, bg2, 2 * C * KP);
-  stage(F1, f1, 2 * C * 2 * KP);
-  stage(F2, f2, C * KS2 * 32);
+  bf16* lds = reinterpret_cast<bf16*>(smem);
+  Wm B1, B2, F1, F2;
+  if (LW) {
+    stage(lds, bg1, N1);                             // hi half only
+    B1 = Wm{lds, reinterpret_cast<const bf16*>(bg1) + N1};
+    lds += N1;
+    B2 = stage_w(lds, bg2, N2);
+    F1 = stage_w(lds, f1, N3);
+    F2 = stage_w(lds, f2, N4);
+  } else {
+    B1 = global_w(bg1, N1);
+    B2 = global_w(bg2, N2);
+    F1 = global_w(f1, N3);
+    F2 = global_w(f2, N4);
+  }
+  float* red = reinterpret_cast<float*>(lds);
   __syncthreads();
   Lane L;
   const int b = blockIdx.y;
@@ -420,28 +220,25 @@ __global__ __launch_bounds__(NW * 64) void k_fuse(FfnoRows rows, const float* __
     const int64_t vox = tile * 16 + L.v;
     const bool ok = vox < rows.voxels;
     const int64_t row = (int64_t)b * rows.voxels + vox;
-    f32x4 x[CT], s[CT], v[CT];
-    load_act<CT>(xin, row, ok, L.q, x);
+    f32x4 s[CT], v[CT];
+    Frag cat[3 * KS];
+    {
+      f32x4 x[CT];
+      load_act<CT>(xin, row, ok, L.q, x);
+      to_frags<CT>(x, cat);
+    }
     load_act<CT>(sp, row, ok, L.q, s);
     load_act<CT>(vp, row, ok, L.q, v);
 #pragma unroll
     for (int t = 0; t < CT; ++t) {
       const int c0 = b * C + 16 * t + 4 * L.q;
-      s[t] = gelu4(s[t] * *(const f32x4*)(s_a + c0) + *(const f32x4*)(s_b + c0));
-      v[t] = gelu4(v[t] * *(const f32x4*)(v_a + c0) + *(const f32x4*)(v_b + c0));
-      if (!ok) { s[t] = zero4(); v[t] = zero4(); }
+      s[t] = ok ? gelu4(s[t] * *(const f32x4*)(s_a + c0) + *(const f32x4*)(s_b + c0)) : zero4();
+      v[t] = ok ? gelu4(v[t] * *(const f32x4*)(v_a + c0) + *(const f32x4*)(v_b + c0)) : zero4();
     }
-    bf16x8 cat[3 * KS];
-    to_frags<CT>(x, cat);
     to_frags<CT>(s, cat + KS);
     to_frags<CT>(v, cat + 2 * KS);
-    f32x4 h[CT];
-    gemm<CT, 3 * KS>(B1, cat, h, L.v, L.q);
-    gelu_tiles<CT>(h);
-    bf16x8 hf[KS];
-    to_frags<CT>(h, hf);
     f32x4 g[2 * CT];
-    gemm<2 * CT, KS>(B2, hf, g, L.v, L.q);
+    mlp2<3 * KS, CT, 2 * CT>(B1, B2, cat, g, L.v, L.q);
 #pragma unroll
     for (int t = 0; t < CT; ++t) {
       s[t] = s[t] * sigm4(g[t] + *(const f32x4*)(bg2_bias + 16 * t + 4 * L.q));
@@ -449,12 +246,8 @@ __global__ __launch_bounds__(NW * 64) void k_fuse(FfnoRows rows, const float* __
     }
     to_frags<CT>(s, cat);
     to_frags<CT>(v, cat + KS);
-    gemm<2 * CT, 2 * KS>(F1, cat, g, L.v, L.q);
-    gelu_tiles<2 * CT>(g);
-    bf16x8 gf[KS2];
-    to_frags<2 * CT>(g, gf);
     f32x4 o[CT];
-    gemm<CT, KS2>(F2, gf, o, L.v, L.q);
+    mlp2<2 * KS, 2 * CT, CT>(F1, F2, cat, o, L.v, L.q);
 #pragma unroll
     for (int t = 0; t < CT; ++t) o[t] = o[t] + 0.5f * (s[t] + v[t]);
     store_act<CT>(fp, row, ok, L.q, o);
@@ -471,13 +264,11 @@ template <int CT, int NW>
 __global__ __launch_bounds__(NW * 64) void k_resmlp(FfnoRows rows, FfnoAssemble in, float* __restrict__ x_out,
                                                     const uint16_t* w1, const uint16_t* w2, float* __restrict__ y,
                                                     FfnoStatsOut st) {
-  constexpr int C = CT * 16, KS1 = ks_of(CT), KS2 = ks_of(2 * CT);
+  constexpr int C = CT * 16, KS1 = ks_of(CT);
   extern __shared__ __align__(16) unsigned char smem[];
-  bf16* W1 = reinterpret_cast<bf16*>(smem);
-  bf16* W2 = W1 + 2 * C * KS1 * 32;
-  float* red = reinterpret_cast<float*>(W2 + C * KS2 * 32);
-  stage(W1, w1, 2 * C * KS1 * 32);
-  stage(W2, w2, C * KS2 * 32);
+  bf16* lds = reinterpret_cast<bf16*>(smem);
+  const Wm W1 = stage_w(lds, w1, 2 * C * KS1 * 32), W2 = stage_w(lds, w2, C * 2 * C);
+  float* red = reinterpret_cast<float*>(lds);
   __syncthreads();
   Lane L;
   const int b = blockIdx.y;
@@ -492,15 +283,10 @@ __global__ __launch_bounds__(NW * 64) void k_resmlp(FfnoRows rows, FfnoAssemble 
     f32x4 x[CT];
     assemble<CT>(in, scale, row, b, ok, L.q, x);
     if (x_out) store_act<CT>(x_out, row, ok, L.q, x);
-    bf16x8 xf[KS1];
+    Frag xf[KS1];
     to_frags<CT>(x, xf);
-    f32x4 h[2 * CT];
-    gemm<2 * CT, KS1>(W1, xf, h, L.v, L.q);
-    gelu_tiles<2 * CT>(h);
-    bf16x8 hf[KS2];
-    to_frags<2 * CT>(h, hf);
     f32x4 o[CT];
-    gemm<CT, KS2>(W2, hf, o, L.v, L.q);
+    mlp2<KS1, 2 * CT, CT>(W1, W2, xf, o, L.v, L.q);
     store_act<CT>(y, row, ok, L.q, o);
     if (st.partials) acc.add(o, ok ? st.depth_w[row] : 0.f, ok);
   }
@@ -514,17 +300,16 @@ __global__ __launch_bounds__(NW * 64) void k_resmlp(FfnoRows rows, FfnoAssemble 
 constexpr float LN10 = 2.302585092994046f;
 
 template <int CT, int NW, int NTH>
-__global__ __launch_bounds__(NW * 64) void k_head(FfnoRows rows, FfnoAssemble in, const uint16_t* p1,
+__global__ __launch_bounds__(NW * 64) void k_head(FfnoRows rows, FfnoAssemble in, float* __restrict__ x_out,
+                                                  const uint16_t* p1,
                                                   const uint16_t* p2, const float* __restrict__ p2_bias,
                                                   const float* __restrict__ x_in, FfnoHeadDesc hd,
                                                   float* __restrict__ y) {
-  constexpr int C = CT * 16, KS1 = ks_of(CT), KS2 = ks_of(2 * CT);
+  constexpr int C = CT * 16, KS1 = ks_of(CT);
   extern __shared__ __align__(16) unsigned char smem[];
-  bf16* P1 = reinterpret_cast<bf16*>(smem);
-  bf16* P2 = P1 + 2 * C * KS1 * 32;
-  float* hs = reinterpret_cast<float*>(P2 + NTH * 16 * KS2 * 32);   // [NW][16 voxels][NTH*16]
-  stage(P1, p1, 2 * C * KS1 * 32);
-  stage(P2, p2, NTH * 16 * KS2 * 32);
+  bf16* lds = reinterpret_cast<bf16*>(smem);
+  const Wm P1 = stage_w(lds, p1, 2 * C * KS1 * 32), P2 = stage_w(lds, p2, NTH * 16 * 2 * C);
+  float* hs = reinterpret_cast<float*>(lds);   // [NW][16 voxels][NTH*16]
   __syncthreads();
   Lane L;
   const int b = blockIdx.y;
@@ -539,19 +324,14 @@ __global__ __launch_bounds__(NW * 64) void k_head(FfnoRows rows, FfnoAssemble in
     const int64_t row = (int64_t)b * rows.voxels + vox;
     f32x4 x[CT];
     assemble<CT>(in, scale, row, b, ok, L.q, x);
-    bf16x8 xf[KS1];
+    if (x_out) store_act<CT>(x_out, row, ok, L.q, x);
+    Frag xf[KS1];
     to_frags<CT>(x, xf);
-    f32x4 h[2 * CT];
-    gemm<2 * CT, KS1>(P1, xf, h, L.v, L.q);
-    gelu_tiles<2 * CT>(h);
-    bf16x8 hf[KS2];
-    to_frags<2 * CT>(h, hf);
     f32x4 o[NTH];
-    gemm<NTH, KS2>(P2, hf, o, L.v, L.q);
+    mlp2<KS1, 2 * CT, NTH>(P1, P2, xf, o, L.v, L.q);
 #pragma unroll
     for (int t = 0; t < NTH; ++t) {
-      f32x4 bb = *(const f32x4*)(p2_bias + 16 * t + 4 * L.q);
-      const f32x4 hvv = o[t] + bb;
+      const f32x4 hvv = o[t] + *(const f32x4*)(p2_bias + 16 * t + 4 * L.q);
 #pragma unroll
       for (int j = 0; j < 4; ++j) mine[L.v * (NTH * 16) + 16 * t + 4 * L.q + j] = hvv[j];
     }
@@ -598,10 +378,6 @@ __global__ __launch_bounds__(NW * 64) void k_head(FfnoRows rows, FfnoAssemble in
 // ---------------------------------------------------------------------------------------------------------------
 // split-bf16 twiddle GEMM
 // ---------------------------------------------------------------------------------------------------------------
-__device__ __forceinline__ int64_t imap(int64_t i, int64_t split, int64_t hi, int64_t lo) {
-  return (i / split) * hi + (i % split) * lo;
-}
-
 __global__ __launch_bounds__(256) void k_tw_gemm(FfnoTwGemm g) {
   Lane L;
   const int64_t ntn = (g.n + 63) / 64;                 // 64-column tasks per batch item
@@ -672,27 +448,21 @@ __global__ __launch_bounds__(256) void k_tw_gemm(FfnoTwGemm g) {
 // ---------------------------------------------------------------------------------------------------------------
 // per-mode complex channel mix, in place
 // ---------------------------------------------------------------------------------------------------------------
-template <int CT>
+template <int CT, bool LW>
 __global__ __launch_bounds__(256) void k_mix(FfnoMix mx) {
-  constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32;
+  constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32, NWC = 2 * C * 2 * KP;
   extern __shared__ __align__(16) unsigned char smem[];
-  bf16* Wc = reinterpret_cast<bf16*>(smem);             // [2C][2KP]
-  stage(Wc, mx.wc, 2 * C * 2 * KP);
+  bf16* lds = reinterpret_cast<bf16*>(smem);
+  const Wm Wc = LW ? stage_w(lds, mx.wc, NWC) : global_w(mx.wc, NWC);
   __syncthreads();
   Lane L;
   const int rtiles = (mx.rows + 15) / 16;
   for (int mode = blockIdx.x * 4 + L.wave; mode < mx.modes; mode += gridDim.x * 4) {
     const float gr = mx.gate[2 * mode], gi = mx.gate[2 * mode + 1];
     const float fac = mx.factor ? mx.factor[mode] : 1.f;
-    bf16x8 va[KS], ua[CT];
-    if (mx.rank) {
-      const uint16_t* vp = mx.v_p + (int64_t)mode * 16 * KP;
-#pragma unroll
-      for (int s = 0; s < KS; ++s) va[s] = *(const bf16x8*)(vp + L.v * KP + 32 * s + 8 * L.q);
-      const uint16_t* up = mx.u_p + (int64_t)mode * C * 32;
-#pragma unroll
-      for (int t = 0; t < CT; ++t) ua[t] = *(const bf16x8*)(up + (16 * t + L.v) * 32 + 8 * L.q);
-    }
+    // per-mode low-rank tables: V [16][KP] and U [C][32], hi and lo halves of each mode's block
+    const Wm V = global_w(mx.v_p ? mx.v_p + (int64_t)mode * 2 * 16 * KP : mx.wc, 16 * KP);
+    const Wm U = global_w(mx.u_p ? mx.u_p + (int64_t)mode * 2 * C * 32 : mx.wc, C * 32);
     for (int rt = 0; rt < rtiles; ++rt) {
       const int r = rt * 16 + L.v;
       const bool ok = r < mx.rows;
@@ -703,29 +473,26 @@ __global__ __launch_bounds__(256) void k_mix(FfnoMix mx) {
         yr[t] = ld4(mx.re + base + 16 * t + 4 * L.q, ok);
         yi[t] = ld4(mx.im + base + 16 * t + 4 * L.q, ok);
       }
-      bf16x8 cat[2 * KS];
+      Frag cat[2 * KS];
       to_frags<CT>(yr, cat);
       to_frags<CT>(yi, cat + KS);
-      f32x4 a[2 * CT];
-      gemm<2 * CT, 2 * KS>(Wc, cat, a, L.v, L.q);
       f32x4 zr[CT], zi[CT];
 #pragma unroll
       for (int t = 0; t < CT; ++t) {
-        zr[t] = gr * a[t] - gi * a[CT + t];
-        zi[t] = gi * a[t] + gr * a[CT + t];
+        const f32x4 ar = tile<2 * KS>(Wc, t, cat, L.v, L.q);
+        const f32x4 ai = tile<2 * KS>(Wc, CT + t, cat, L.v, L.q);
+        zr[t] = gr * ar - gi * ai;
+        zi[t] = gi * ar + gr * ai;
+        __builtin_amdgcn_sched_barrier(0);   // keep the next tile's weight loads from being hoisted here
       }
       if (mx.rank) {
-        f32x4 lr = zero4(), li = zero4();
-#pragma unroll
-        for (int s = 0; s < KS; ++s) {
-          lr = mfma(va[s], cat[s], lr);
-          li = mfma(va[s], cat[KS + s], li);
-        }
-        const bf16x8 lrf = pack8(lr, zero4()), lif = pack8(li, zero4());
+        const f32x4 lr = tile<KS>(V, 0, cat, L.v, L.q);
+        const f32x4 li = tile<KS>(V, 0, cat + KS, L.v, L.q);
+        const Frag lrf = split8(lr, zero4()), lif = split8(li, zero4());
 #pragma unroll
         for (int t = 0; t < CT; ++t) {
-          zr[t] = mfma(ua[t], lrf, zr[t]);
-          zi[t] = mfma(ua[t], lif, zi[t]);
+          zr[t] += tile<1>(U, t, &lrf, L.v, L.q);
+          zi[t] += tile<1>(U, t, &lif, L.v, L.q);
         }
       }
 #pragma unroll
@@ -737,54 +504,10 @@ __global__ __launch_bounds__(256) void k_mix(FfnoMix mx) {
   }
 }
 
-// ---------------------------------------------------------------------------------------------------------------
-// host side
-// ---------------------------------------------------------------------------------------------------------------
-thread_local char g_err[512] = "";
-
-int fail(const char* msg) {
-  snprintf(g_err, sizeof(g_err), "%s", msg);
-  return -1;
-}
-
-int check_launch(const char* what) {
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) {
-    snprintf(g_err, sizeof(g_err), "%s: %s", what, hipGetErrorString(e));
-    return -2;
-  }
-  return 0;
-}
-
-template <typename K>
-int set_lds(K kernel, size_t bytes) {
-  if (bytes > 160 * 1024) return fail("kernel needs more LDS than a CU has");
-  if (bytes > 64 * 1024) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    if (e != hipSuccess) return fail("hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed");
-  }
-  return 0;
-}
-
-bool rows_ok(const FfnoRows& r) { return r.B > 0 && r.voxels > 0 && r.nwg > 0 && r.C % 16 == 0; }
-
-constexpr int NWV = 4;   // waves per workgroup of the voxel-row kernels
-
-#define FFNO_DISPATCH_CT(C, ...)                          \
-  switch ((C) / 16) {                                     \
-    case 1: { constexpr int CT = 1; __VA_ARGS__; break; } \
-    case 2: { constexpr int CT = 2; __VA_ARGS__; break; } \
-    case 4: { constexpr int CT = 4; __VA_ARGS__; break; } \
-    case 6: { constexpr int CT = 6; __VA_ARGS__; break; } \
-    case 8: { constexpr int CT = 8; __VA_ARGS__; break; } \
-    default: return fail("unsupported width (16, 32, 64, 96 or 128)"); \
-  }
-
-template <int CT>
-constexpr size_t red_bytes() { return (size_t)NWV * 4 * CT * GnAcc<CT>::SUB * 3 * sizeof(float); }
-
 }  // namespace
+}  // namespace ffno
+
+using namespace ffno;
 
 extern "C" {
 
@@ -814,7 +537,7 @@ int ffno_block_in(FfnoRows rows, FfnoAssemble in, float* x_out, const uint16_t* 
     return fail("ffno_block_in: bad arguments");
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {
-    const size_t lds = 4 * (size_t)(CT * 16) * ks_of(CT) * 32 * 2;
+    const size_t lds = 4 * (size_t)(CT * 16) * ks_of(CT) * 32 * 2 * 2;
     auto k = k_block_in<CT, NWV>;
     if (set_lds(k, lds)) return -1;
     k<<<grid, NWV * 64, lds, stream>>>(rows, in, x_out, g1, g2, v1, v2, z_proj, z, xs, xv);
@@ -827,7 +550,7 @@ int ffno_post(FfnoRows rows, const float* u, const uint16_t* w1, const uint16_t*
   if (!rows_ok(rows) || !u || !w1 || !w2 || !y) return fail("ffno_post: bad arguments");
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {
-    const size_t lds = ((size_t)2 * CT * 16 * ks_of(CT) * 32 + (size_t)CT * 16 * ks_of(2 * CT) * 32) * 2 +
+    const size_t lds = ((size_t)2 * CT * 16 * ks_of(CT) * 32 + (size_t)CT * 16 * 2 * CT * 16) * 2 * 2 +
                        red_bytes<CT>();
     auto k = k_post<CT, NWV>;
     if (set_lds(k, lds)) return -1;
@@ -845,11 +568,17 @@ int ffno_fuse(FfnoRows rows, const float* x, const float* sp, const float* vp, c
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {
     constexpr int C = CT * 16, KP = ks_of(CT) * 32;
-    const size_t lds = ((size_t)C * 3 * KP + 2 * C * KP + 2 * C * 2 * KP + (size_t)C * ks_of(2 * CT) * 32) * 2 +
+    const size_t lds = ((size_t)C * 3 * KP + 2 * ((size_t)2 * C * KP + 2 * C * 2 * KP + (size_t)C * 2 * C)) * 2 +
                        red_bytes<CT>();
-    auto k = k_fuse<CT, NWV>;
-    if (set_lds(k, lds)) return -1;
-    k<<<grid, NWV * 64, lds, stream>>>(rows, x, sp, vp, s_a, s_b, v_a, v_b, bg1, bg2, bg2_bias, f1, f2, fp, stats);
+    if (lds <= 160 * 1024) {
+      auto k = k_fuse<CT, NWV, true>;
+      if (set_lds(k, lds)) return -1;
+      k<<<grid, NWV * 64, lds, stream>>>(rows, x, sp, vp, s_a, s_b, v_a, v_b, bg1, bg2, bg2_bias, f1, f2, fp, stats);
+    } else {
+      auto k = k_fuse<CT, NWV, false>;
+      k<<<grid, NWV * 64, red_bytes<CT>(), stream>>>(rows, x, sp, vp, s_a, s_b, v_a, v_b, bg1, bg2, bg2_bias, f1, f2,
+                                                      fp, stats);
+    }
   });
   return check_launch("ffno_fuse");
 }
@@ -859,7 +588,7 @@ int ffno_resmlp(FfnoRows rows, FfnoAssemble in, float* x_out, const uint16_t* w1
   if (!rows_ok(rows) || !in.t || !in.aff_a || !in.aff_b || !w1 || !w2 || !y) return fail("ffno_resmlp: bad arguments");
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {
-    const size_t lds = ((size_t)2 * CT * 16 * ks_of(CT) * 32 + (size_t)CT * 16 * ks_of(2 * CT) * 32) * 2 +
+    const size_t lds = ((size_t)2 * CT * 16 * ks_of(CT) * 32 + (size_t)CT * 16 * 2 * CT * 16) * 2 * 2 +
                        red_bytes<CT>();
     auto k = k_resmlp<CT, NWV>;
     if (set_lds(k, lds)) return -1;
@@ -868,8 +597,8 @@ int ffno_resmlp(FfnoRows rows, FfnoAssemble in, float* x_out, const uint16_t* w1
   return check_launch("ffno_resmlp");
 }
 
-int ffno_head(FfnoRows rows, FfnoAssemble in, const uint16_t* p1, const uint16_t* p2, const float* p2_bias,
-              const float* x_in, const FfnoHeadDesc* desc, float* y, hipStream_t stream) {
+int ffno_head(FfnoRows rows, FfnoAssemble in, float* x_out, const uint16_t* p1, const uint16_t* p2,
+              const float* p2_bias, const float* x_in, const FfnoHeadDesc* desc, float* y, hipStream_t stream) {
   if (!rows_ok(rows) || !in.t || !p1 || !p2 || !p2_bias || !x_in || !desc || !y) return fail("ffno_head: bad arguments");
   int n_head = 0, n_out = 0;
   if (desc->n_species < 1 || desc->n_species > 4) return fail("ffno_head: 1 to 4 species");
@@ -882,16 +611,16 @@ int ffno_head(FfnoRows rows, FfnoAssemble in, const uint16_t* p1, const uint16_t
   dim3 grid(rows.nwg, rows.B);
   const int nth = (n_head + 15) / 16;
   FFNO_DISPATCH_CT(rows.C, {
-    const size_t w_bytes = ((size_t)2 * CT * 16 * ks_of(CT) * 32 + (size_t)nth * 16 * ks_of(2 * CT) * 32) * 2;
+    const size_t w_bytes = ((size_t)2 * CT * 16 * ks_of(CT) * 32 + (size_t)nth * 16 * 2 * CT * 16) * 2 * 2;
     const size_t lds = w_bytes + (size_t)NWV * 16 * nth * 16 * sizeof(float);
     if (nth == 1) {
       auto k = k_head<CT, NWV, 1>;
       if (set_lds(k, lds)) return -1;
-      k<<<grid, NWV * 64, lds, stream>>>(rows, in, p1, p2, p2_bias, x_in, *desc, y);
+      k<<<grid, NWV * 64, lds, stream>>>(rows, in, x_out, p1, p2, p2_bias, x_in, *desc, y);
     } else {
       auto k = k_head<CT, NWV, 2>;
       if (set_lds(k, lds)) return -1;
-      k<<<grid, NWV * 64, lds, stream>>>(rows, in, p1, p2, p2_bias, x_in, *desc, y);
+      k<<<grid, NWV * 64, lds, stream>>>(rows, in, x_out, p1, p2, p2_bias, x_in, *desc, y);
     }
   });
   return check_launch("ffno_head");
@@ -916,10 +645,14 @@ int ffno_mix(const FfnoMix* m, hipStream_t stream) {
   const int blocks = (m->modes + 3) / 4;
   const int grid = blocks < 4096 ? blocks : 4096;
   FFNO_DISPATCH_CT(m->C, {
-    const size_t lds = (size_t)2 * CT * 16 * 2 * ks_of(CT) * 32 * 2;
-    auto k = k_mix<CT>;
-    if (set_lds(k, lds)) return -1;
-    k<<<grid, 256, lds, stream>>>(*m);
+    const size_t lds = (size_t)2 * CT * 16 * 2 * ks_of(CT) * 32 * 2 * 2;
+    if (lds <= 160 * 1024) {
+      auto k = k_mix<CT, true>;
+      if (set_lds(k, lds)) return -1;
+      k<<<grid, 256, lds, stream>>>(*m);
+    } else {
+      k_mix<CT, false><<<grid, 256, 0, stream>>>(*m);
+    }
   });
   return check_launch("ffno_mix");
 }

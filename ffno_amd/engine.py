@@ -30,10 +30,11 @@ def _p(t):
 
 
 def pack_mlp_weight(w, segments=None, n_pad=None):
-    """bf16 [.., N_pad, K_pad] for the MFMA kernels from a float weight [.., N, K] (PyTorch [out, in]): each input
-    segment padded to a multiple of 32, k permuted within every 32-block as the kernels' lane layout wants
-    (k = 32s + 16h + 4q + j is stored at 32s + 8q + 4h + j), N padded to a multiple of 16."""
-    w = w.float()
+    """Split-bf16 [.., 2, N_pad, K_pad] (hi, lo) for the MFMA kernels from a float weight [.., N, K] (PyTorch
+    [out, in]): each input segment padded to a multiple of 32, k permuted within every 32-block as the kernels' lane
+    layout wants (k = 32s + 16h + 4q + j is stored at 32s + 8q + 4h + j), N padded to a multiple of 16;
+    hi = bf16(w), lo = bf16(w - hi)."""
+    w = w.double()
     lead = w.shape[:-2]
     N, K = w.shape[-2:]
     segs = segments or [K]
@@ -48,7 +49,9 @@ def pack_mlp_weight(w, segments=None, n_pad=None):
     wp = F.pad(wp, (0, 0, 0, npad - N))
     kp = wp.shape[-1]
     wp = wp.reshape(*lead, npad, kp // 32, 2, 4, 4).transpose(-3, -2).reshape(*lead, npad, kp)
-    return wp.to(torch.bfloat16).contiguous().view(torch.int16)
+    hi = wp.to(torch.bfloat16)
+    lo = (wp - hi.double()).to(torch.bfloat16)
+    return torch.stack([hi, lo], -3).contiguous().view(torch.int16)
 
 
 def _conv_w(p):
@@ -80,6 +83,9 @@ class PackedWeights:
         C = module.width
         sd = {k: v.detach().to(device) for k, v in module.state_dict().items()}
         self.sd = sd
+        self.C = C
+        self._adj = None
+        self.wc_real = []
         self.lift = _conv_w(sd["lift.0.weight"]).float().contiguous()          # [C, 6]
         self.lift_gn = (sd["lift.1.weight"].float().contiguous(), sd["lift.1.bias"].float().contiguous())
         self.blocks = []
@@ -92,8 +98,12 @@ class PackedWeights:
                 b[br + ".p1"] = pack_mlp_weight(_conv_w(sd[p + br + ".post.0.weight"]))
                 b[br + ".p2"] = pack_mlp_weight(_conv_w(sd[p + br + ".post.2.weight"]))
                 wr, wi = sd[p + br + ".weight_real"].float(), sd[p + br + ".weight_imag"].float()   # [in, out]
+                # [Ar; Ai] = [[Wr^T, -Wi^T], [Wi^T, Wr^T]] [Yr; Yi]: the complex product Y (Wr + i Wi) over channels
                 wc = torch.cat([torch.cat([wr.t(), -wi.t()], 1), torch.cat([wi.t(), wr.t()], 1)], 0)
                 b[br + ".wc"] = pack_mlp_weight(wc, segments=[C, C])
+                if br == "spec":
+                    self.wc_real.append({})
+                self.wc_real[-1][br] = wc
             b["z_proj"] = sd[p + "vertical.z_proj.weight"].reshape(C).float().contiguous()
             b["bg1"] = pack_mlp_weight(_conv_w(sd[p + "branch_gate.0.weight"]), segments=[C, C, C])
             b["bg2"] = pack_mlp_weight(_conv_w(sd[p + "branch_gate.2.weight"]))
@@ -114,6 +124,34 @@ class PackedWeights:
         nh = sd["proj2.weight"].shape[0]
         self.p2_bias = F.pad(sd["proj2.bias"].float(), (0, T.round_up(nh, 16) - nh)).contiguous()
         self.head = self._head_desc(module, sd)
+
+    def adjoint(self):
+        """Transposed weights for the adjoint kernels, packed on first use."""
+        if getattr(self, "_adj", None) is None:
+            sd = self.sd
+            C = self.C
+            blocks = []
+            for i, b in enumerate(self.blocks):
+                p = f"blocks.{i}."
+                t = {}
+                for br in ("spec", "vertical"):
+                    t[br + ".g2t"] = pack_mlp_weight(_conv_w(sd[p + br + ".input_gate.2.weight"]).t())
+                    t[br + ".g1t"] = pack_mlp_weight(_conv_w(sd[p + br + ".input_gate.0.weight"]).t())
+                    t[br + ".p2t"] = pack_mlp_weight(_conv_w(sd[p + br + ".post.2.weight"]).t())
+                    t[br + ".p1t"] = pack_mlp_weight(_conv_w(sd[p + br + ".post.0.weight"]).t())
+                    t[br + ".wc"] = pack_mlp_weight(self.wc_real[i][br].t(), segments=[C, C])
+                t["bg2t"] = pack_mlp_weight(_conv_w(sd[p + "branch_gate.2.weight"]).t())
+                t["bg1t"] = pack_mlp_weight(_conv_w(sd[p + "branch_gate.0.weight"]).t())
+                t["f2t"] = pack_mlp_weight(_conv_w(sd[p + "fuse.2.weight"]).t())
+                t["f1t"] = pack_mlp_weight(_conv_w(sd[p + "fuse.0.weight"]).t())
+                t["pw2t"] = pack_mlp_weight(_conv_w(sd[p + "pw.2.weight"]).t())
+                t["pw1t"] = pack_mlp_weight(_conv_w(sd[p + "pw.0.weight"]).t())
+                t["m2t"] = pack_mlp_weight(_conv_w(sd[p + "mlp.net.3.weight"]).t())
+                t["m1t"] = pack_mlp_weight(_conv_w(sd[p + "mlp.net.0.weight"]).t())
+                blocks.append(t)
+            self._adj = dict(blocks=blocks, p2t=pack_mlp_weight(_conv_w(sd["proj2.weight"]).t()),
+                             p1t=pack_mlp_weight(_conv_w(sd["proj1.weight"]).t()))
+        return self._adj
 
     @staticmethod
     def _head_desc(module, sd):
@@ -177,6 +215,10 @@ class Plan:
         self.tw = {name: T.split_bf16(a, dev) for name, a in
                    (("w_fwd", hm["w_fwd"]), ("h_fwd", hm["h_fwd"]), ("h_inv", hm["h_inv"]), ("w_inv", hm["w_inv"]),
                     ("v_fwd", v_fwd), ("v_inv", v_inv))}
+        # the adjoint chain runs the same GEMMs with the transposed matrices in reverse order
+        self._mats = dict(w_fwd=hm["w_fwd"], h_fwd=hm["h_fwd"], h_inv=hm["h_inv"], w_inv=hm["w_inv"], v_fwd=v_fwd,
+                          v_inv=v_inv)
+        self._tw_adj = None
         # per-mode gates and low-rank tables (models/ffno_model.py:375-392, :443-447, :231-237, :285-296), float32
         sd = engine.weights.sd
         ky = torch.fft.fftfreq(H, d=float(dy), device=dev)
@@ -194,16 +236,36 @@ class Plan:
             for br, feats in (("spec", hfeat), ("vertical", vfeat)):
                 prefix = f"blocks.{i}.{br}."
                 gate = _freq_mlp(prefix, sd, feats).float().contiguous()
-                e = dict(gate=gate, v=None, u=None)
+                e = dict(gate=gate, v=None, u=None, u_raw=None, v_raw=None)
                 if r:
                     u, v = _freq_lowrank(prefix, sd, feats, C, r)
-                    e["v"] = pack_mlp_weight(v, n_pad=16)                      # [modes, 16, KS*32]
-                    e["u"] = pack_mlp_weight(u)                                # [modes, C, 32]
+                    e["v"] = pack_mlp_weight(v, n_pad=16)                      # [modes, 2, 16, KS*32]
+                    e["u"] = pack_mlp_weight(u)                                # [modes, 2, C, 32]
+                    e["u_raw"], e["v_raw"] = u, v
                 entry[br] = e
             self.mix.append(entry)
         nt = (self.voxels + 15) // 16
         self.nwg = max(1, min((nt + NW - 1) // NW, max(1, 2048 // B)))
         self.rows = FfnoRows(B, self.voxels, C, self.nwg)
+        self.device = dev
+
+    def adjoint_tables(self):
+        """Transposed transform matrices and adjoint mix tables: conj(g), and the low-rank roles swapped
+        (V' = U^T, U' = V^T), so that the same kernels apply the transpose of every forward stage."""
+        if self._tw_adj is None:
+            m = self._mats
+            self._tw_adj = {name + "_T": T.split_bf16(np.ascontiguousarray(a.T), self.device) for name, a in m.items()}
+            for entry in self.mix:
+                for e in entry.values():
+                    g = e["gate"].clone()
+                    g[:, 1] = -g[:, 1]
+                    e["gate_adj"] = g.contiguous()
+                    if e["u_raw"] is not None:
+                        e["v_adj"] = pack_mlp_weight(e["u_raw"].transpose(-1, -2), n_pad=16)
+                        e["u_adj"] = pack_mlp_weight(e["v_raw"].transpose(-1, -2))
+                    else:
+                        e["v_adj"] = e["u_adj"] = None
+        return self._tw_adj
 
 
 class Engine:
@@ -250,7 +312,7 @@ class Engine:
     def _gn(self, plan, partials, gn, dev):
         a = torch.empty(plan.B, plan.C, device=dev)
         b = torch.empty(plan.B, plan.C, device=dev)
-        stats = torch.empty(plan.B, GROUPS, 2, device=dev)
+        stats = torch.empty(plan.B, GROUPS, 3, device=dev)
         _lib.call("ffno_gn_finalize", plan.B, plan.nwg, plan.C, _p(partials), _p(gn[0]), _p(gn[1]), _p(stats),
                   _p(a), _p(b), self._stream())
         return a, b, stats
@@ -268,53 +330,121 @@ class Engine:
         g.accumulate = accumulate
         _lib.call("ffno_tw_gemm", ctypes.byref(g), self._stream())
 
-    def _mix(self, plan, re, im, rows, modes, mode_stride, rmap, wc, e):
+    def _mix(self, plan, re, im, rows, modes, mode_stride, rmap, wc, e, adjoint=False):
         mx = FfnoMix()
         mx.re, mx.im = _p(re), _p(im)
         mx.C, mx.rows, mx.modes, mx.rank = plan.C, rows, modes, plan.rank
         mx.mode_stride = mode_stride
         mx.r_split, mx.r_hi, mx.r_lo = rmap
-        mx.wc, mx.gate = _p(wc), _p(e["gate"])
-        mx.v_p, mx.u_p = _p(e["v"]), _p(e["u"])
+        sfx = "_adj" if adjoint else ""
+        mx.wc, mx.gate = _p(wc), _p(e["gate" + sfx])
+        mx.v_p, mx.u_p = _p(e["v" + sfx]), _p(e["u" + sfx])
         mx.factor = ctypes.c_void_p(0)
         _lib.call("ffno_mix", ctypes.byref(mx), self._stream())
 
-    def horizontal(self, plan, xs, s1, s2, out, i):
-        """rfft2 -> mix -> irfft2 of xs [B,D,H,W,C] into out (models/ffno_model.py:441-448)."""
+    def horizontal(self, plan, xs, s1, s2, out, i, adjoint=False):
+        """rfft2 -> mix -> irfft2 of xs [B,D,H,W,C] into out (models/ffno_model.py:441-448); with adjoint, the
+        transpose of that linear map (W_fwd^T H_fwd^T Mix^T H_inv^T W_inv^T)."""
         B, D, H, W, C, wf = plan.B, plan.D, plan.H, plan.W, plan.C, plan.wf
         plane = 2 * H * wf * C
         big = 1 << 40
-        # rfft along W: one GEMM per voxel row (b, d, h)
-        self._tw(plan.tw["w_fwd"], xs, s1, B * D * H, C, (H, H * W * C, W * C), (plane, wf * C),
-                 (big, 0, C), (wf, H * wf * C, C))
-        # DFT along H: one GEMM per plane (b, d) over (part, h) rows, columns (kx, c)
-        self._tw(plan.tw["h_fwd"], s1, s2, B * D, wf * C, (big, 0, plane), (0, plane),
-                 (H, H * wf * C, wf * C), (H, H * wf * C, wf * C))
-        e = plan.mix[i]["spec"]
-        self._mix(plan, s2, s2[H * wf * C:], B * D, H * wf, C, (big, 0, plane), self.weights.blocks[i]["spec.wc"], e)
-        self._tw(plan.tw["h_inv"], s2, s1, B * D, wf * C, (big, 0, plane), (0, plane),
-                 (H, H * wf * C, wf * C), (H, H * wf * C, wf * C))
-        # c2r along W
-        self._tw(plan.tw["w_inv"], s1, out, B * D * H, C, (H, plane, wf * C), (H * W * C, W * C),
-                 (wf, H * wf * C, C), (big, 0, C))
+        if adjoint:
+            ta = plan.adjoint_tables()
+            m1, m2, m3, m4 = ta["w_inv_T"], ta["h_inv_T"], ta["h_fwd_T"], ta["w_fwd_T"]
+            wc = self.weights.adjoint()["blocks"][i]["spec.wc"]
+        else:
+            m1, m2, m3, m4 = plan.tw["w_fwd"], plan.tw["h_fwd"], plan.tw["h_inv"], plan.tw["w_inv"]
+            wc = self.weights.blocks[i]["spec.wc"]
+        # along W, real -> half spectrum: one GEMM per voxel row (b, d, h)
+        self._tw(m1, xs, s1, B * D * H, C, (H, H * W * C, W * C), (plane, wf * C), (big, 0, C), (wf, H * wf * C, C))
+        # along H, complex: one GEMM per plane (b, d) over (part, h) rows, columns (kx, c)
+        self._tw(m2, s1, s2, B * D, wf * C, (big, 0, plane), (0, plane), (H, H * wf * C, wf * C),
+                 (H, H * wf * C, wf * C))
+        self._mix(plan, s2, s2[H * wf * C:], B * D, H * wf, C, (big, 0, plane), wc, plan.mix[i]["spec"], adjoint)
+        self._tw(m3, s2, s1, B * D, wf * C, (big, 0, plane), (0, plane), (H, H * wf * C, wf * C),
+                 (H, H * wf * C, wf * C))
+        # half spectrum -> real along W
+        self._tw(m4, s1, out, B * D * H, C, (H, plane, wf * C), (H * W * C, W * C), (wf, H * wf * C, C), (big, 0, C))
 
-    def vertical(self, plan, xv, v1, out, i):
-        """Filon transform along D -> mix -> synthesis at the levels, /L folded in (models/ffno_model.py:270-305)."""
+    def vertical(self, plan, xv, v1, out, i, adjoint=False):
+        """Filon transform along D -> mix -> synthesis at the levels, /L folded in (models/ffno_model.py:270-305);
+        with adjoint, its transpose."""
         B, D, C, M = plan.B, plan.D, plan.C, plan.modes
         hwc = plan.H * plan.W * C
         big = 1 << 40
-        self._tw(plan.tw["v_fwd"], xv, v1, B, hwc, (big, 0, D * hwc), (0, 2 * M * hwc),
-                 (big, 0, hwc), (M, M * hwc, hwc))
-        e = plan.mix[i]["vertical"]
+        if adjoint:
+            ta = plan.adjoint_tables()
+            m1, m2 = ta["v_inv_T"], ta["v_fwd_T"]
+            wc = self.weights.adjoint()["blocks"][i]["vertical.wc"]
+        else:
+            m1, m2 = plan.tw["v_fwd"], plan.tw["v_inv"]
+            wc = self.weights.blocks[i]["vertical.wc"]
+        self._tw(m1, xv, v1, B, hwc, (big, 0, D * hwc), (0, 2 * M * hwc), (big, 0, hwc), (M, M * hwc, hwc))
         hw = plan.H * plan.W
-        self._mix(plan, v1, v1[M * hwc:], B * hw, M, hwc, (hw, 2 * M * hwc, C), self.weights.blocks[i]["vertical.wc"],
-                  e)
-        self._tw(plan.tw["v_inv"], v1, out, B, hwc, (big, 0, 2 * M * hwc), (0, D * hwc),
-                 (M, M * hwc, hwc), (big, 0, hwc))
+        self._mix(plan, v1, v1[M * hwc:], B * hw, M, hwc, (hw, 2 * M * hwc, C), wc, plan.mix[i]["vertical"], adjoint)
+        self._tw(m2, v1, out, B, hwc, (big, 0, 2 * M * hwc), (0, D * hwc), (M, M * hwc, hwc), (big, 0, hwc))
+
+    def _buffers(self, plan, dev, extra=()):
+        B, D, H, W, C = plan.B, plan.D, plan.H, plan.W, plan.C
+        N = B * plan.voxels
+        f32 = dict(device=dev, dtype=torch.float32)
+        names = ("xs", "xv", "sp", "vp", "fp", "pp", "x1", "x2") + tuple(extra)
+        bufs = {n: torch.empty(N, C, **f32) for n in names}
+        bufs["s1"] = torch.empty(B * D * 2 * H * plan.wf * C, **f32)
+        bufs["s2"] = torch.empty_like(bufs["s1"])
+        bufs["v1"] = torch.empty(B * 2 * plan.modes * H * W * C, **f32)
+        bufs["part"] = torch.empty(B, plan.nwg, GROUPS, 3, **f32)
+        return bufs
+
+    def _block(self, plan, i, prev, xin, bf, ys=None, yv=None):
+        """One block's forward from the assembled input `prev`; writes the block input to xin (unless None) and
+        every intermediate into bf. ys/yv: where the branch transforms' outputs go (default: over xs/xv).
+        Returns the GroupNorm affines and statistics of the five norms."""
+        dev = bf["xs"].device
+        blk = self.weights.blocks[i]
+        st = self._stream()
+        rows = plan.rows
+        stats = FfnoStatsOut(_p(plan.depth_w), _p(bf["part"]))
+        xs, xv = bf["xs"], bf["xv"]
+        ys = xs if ys is None else ys
+        yv = xv if yv is None else yv
+        _lib.call("ffno_block_in", rows, prev, _p(xin), _p(blk["spec.g1"]), _p(blk["spec.g2"]),
+                  _p(blk["vertical.g1"]), _p(blk["vertical.g2"]), _p(blk["z_proj"]), _p(plan.z), _p(xs), _p(xv), st)
+        gn = {}
+        self.horizontal(plan, xs, bf["s1"], bf["s2"], ys, i)
+        _lib.call("ffno_post", rows, _p(ys), _p(blk["spec.p1"]), _p(blk["spec.p2"]), _p(bf["sp"]), stats, st)
+        gn["s"] = self._gn(plan, bf["part"], blk["norm_spec"], dev)
+        self.vertical(plan, xv, bf["v1"], yv, i)
+        _lib.call("ffno_post", rows, _p(yv), _p(blk["vertical.p1"]), _p(blk["vertical.p2"]), _p(bf["vp"]), stats, st)
+        gn["v"] = self._gn(plan, bf["part"], blk["norm_vert"], dev)
+        h = xin if xin is not None else None
+        return gn, h
+
+    def _block_tail(self, plan, i, h, bf, gn):
+        dev = bf["xs"].device
+        blk = self.weights.blocks[i]
+        st = self._stream()
+        rows = plan.rows
+        stats = FfnoStatsOut(_p(plan.depth_w), _p(bf["part"]))
+        sa, sb, _ = gn["s"]
+        va, vb, _ = gn["v"]
+        _lib.call("ffno_fuse", rows, _p(h), _p(bf["sp"]), _p(bf["vp"]), _p(sa), _p(sb), _p(va), _p(vb), _p(blk["bg1"]),
+                  _p(blk["bg2"]), _p(blk["bg2_bias"]), _p(blk["f1"]), _p(blk["f2"]), _p(bf["fp"]), stats, st)
+        gn["f"] = self._gn(plan, bf["part"], blk["norm_fuse"], dev)
+        fa, fb, _ = gn["f"]
+        _lib.call("ffno_resmlp", rows, FfnoAssemble(_p(h), _p(bf["fp"]), _p(fa), _p(fb), _p(blk["res_fused"]), 1),
+                  _p(bf["x1"]), _p(blk["pw1"]), _p(blk["pw2"]), _p(bf["pp"]), stats, st)
+        gn["p"] = self._gn(plan, bf["part"], blk["norm_pw"], dev)
+        pa, pb, _ = gn["p"]
+        _lib.call("ffno_resmlp", rows, FfnoAssemble(_p(bf["x1"]), _p(bf["pp"]), _p(pa), _p(pb), _p(blk["res_pw"]), 0),
+                  _p(bf["x2"]), _p(blk["m1"]), _p(blk["m2"]), _p(bf["mp"]), stats, st)
+        gn["m"] = self._gn(plan, bf["part"], blk["norm_mlp"], dev)
+        ma, mb, _ = gn["m"]
+        return FfnoAssemble(_p(bf["x2"]), _p(bf["mp"]), _p(ma), _p(mb), _p(blk["res_mlp"]), 0)
 
     def forward(self, x, z, dx, dy, keep=None):
         """Normalised log10 populations [B, Cout, D, H, W] from x [B, 6, D, H, W] and z_scale [B, D, H, W] (Mm).
-        keep: a list that receives every block input (channel-last) for the adjoint."""
+        keep: a list that receives every block input and the head input (channel-last) for the adjoint."""
         if not x.is_cuda:
             raise RuntimeError("the FFNO3D HIP engine runs on the GPU only; move the inputs to a cuda device")
         dev = x.device
@@ -330,53 +460,123 @@ class Engine:
         x = x.contiguous().float()
         st = self._stream()
         f32 = dict(device=dev, dtype=torch.float32)
-        part = torch.empty(B, plan.nwg, GROUPS, 3, **f32)
-        stats = FfnoStatsOut(_p(plan.depth_w), _p(part))
+        bf = self._buffers(plan, dev, extra=("mp",))
+        stats = FfnoStatsOut(_p(plan.depth_w), _p(bf["part"]))
         rows = plan.rows
         t0 = torch.empty(N, C, **f32)
         _lib.call("ffno_lift", rows, _p(x), _p(Wt.lift), _p(t0), stats, st)
-        a, b, _ = self._gn(plan, part, Wt.lift_gn, dev)
+        a, b, _ = self._gn(plan, bf["part"], Wt.lift_gn, dev)
         prev = FfnoAssemble(None, _p(t0), _p(a), _p(b), None, 1)
-        wf, M = plan.wf, plan.modes
-        xs = torch.empty(N, C, **f32)
-        xv = torch.empty(N, C, **f32)
-        s1 = torch.empty(B * D * 2 * H * wf * C, **f32)
-        s2 = torch.empty_like(s1)
-        v1 = torch.empty(B * 2 * M * H * W * C, **f32)
-        sp = torch.empty(N, C, **f32)
-        vp = torch.empty(N, C, **f32)
-        fp = torch.empty(N, C, **f32)
-        pp = torch.empty(N, C, **f32)
-        x1 = torch.empty(N, C, **f32)
-        x2 = torch.empty(N, C, **f32)
         xin_buf = torch.empty(N, C, **f32) if keep is None else None
-        for i, blk in enumerate(Wt.blocks):
-            # the block input never aliases x2, which the assembly reads (block i-1's x2 and mlp output)
+        for i in range(len(Wt.blocks)):
+            # the block input never aliases x2 or mp, which the assembly reads
             xin = torch.empty(N, C, **f32) if keep is not None else xin_buf
-            _lib.call("ffno_block_in", rows, prev, _p(xin), _p(blk["spec.g1"]), _p(blk["spec.g2"]),
-                      _p(blk["vertical.g1"]), _p(blk["vertical.g2"]), _p(blk["z_proj"]), _p(plan.z), _p(xs), _p(xv),
-                      st)
+            gn, _ = self._block(plan, i, prev, xin, bf)
             if keep is not None:
                 keep.append(xin)
-            self.horizontal(plan, xs, s1, s2, xs, i)
-            _lib.call("ffno_post", rows, _p(xs), _p(blk["spec.p1"]), _p(blk["spec.p2"]), _p(sp), stats, st)
-            sa, sb, _ = self._gn(plan, part, blk["norm_spec"], dev)
-            self.vertical(plan, xv, v1, xv, i)
-            _lib.call("ffno_post", rows, _p(xv), _p(blk["vertical.p1"]), _p(blk["vertical.p2"]), _p(vp), stats, st)
-            va, vb, _ = self._gn(plan, part, blk["norm_vert"], dev)
-            _lib.call("ffno_fuse", rows, _p(xin), _p(sp), _p(vp), _p(sa), _p(sb), _p(va), _p(vb), _p(blk["bg1"]),
-                      _p(blk["bg2"]), _p(blk["bg2_bias"]), _p(blk["f1"]), _p(blk["f2"]), _p(fp), stats, st)
-            fa, fb, _ = self._gn(plan, part, blk["norm_fuse"], dev)
-            _lib.call("ffno_resmlp", rows, FfnoAssemble(_p(xin), _p(fp), _p(fa), _p(fb), _p(blk["res_fused"]), 1),
-                      _p(x1), _p(blk["pw1"]), _p(blk["pw2"]), _p(pp), stats, st)
-            pa, pb, _ = self._gn(plan, part, blk["norm_pw"], dev)
-            mp = fp
-            _lib.call("ffno_resmlp", rows, FfnoAssemble(_p(x1), _p(pp), _p(pa), _p(pb), _p(blk["res_pw"]), 0),
-                      _p(x2), _p(blk["m1"]), _p(blk["m2"]), _p(mp), stats, st)
-            ma, mb, _ = self._gn(plan, part, blk["norm_mlp"], dev)
-            prev = FfnoAssemble(_p(x2), _p(mp), _p(ma), _p(mb), _p(blk["res_mlp"]), 0)
+            prev = self._block_tail(plan, i, xin, bf, gn)
         out_ch = self.module.out_channels
         y = torch.empty(B, out_ch, D, H, W, **f32)
-        _lib.call("ffno_head", rows, prev, _p(Wt.p1), _p(Wt.p2), _p(Wt.p2_bias), _p(x), ctypes.byref(Wt.head), _p(y),
-                  st)
+        h_last = torch.empty(N, C, **f32) if keep is not None else None
+        _lib.call("ffno_head", rows, prev, _p(h_last), _p(Wt.p1), _p(Wt.p2), _p(Wt.p2_bias), _p(x),
+                  ctypes.byref(Wt.head), _p(y), st)
+        if keep is not None:
+            keep.append(h_last)
         return y
+
+    # ------------------------------------------------------------------------------------------------ adjoint
+    def _gn_sums(self, plan, gb, ubar, part2):
+        sums = torch.empty(plan.B, GROUPS, 2, device=ubar.device)
+        _lib.call("ffno_gn_grad_stats", plan.rows, gb, _p(ubar), _p(part2), self._stream())
+        _lib.call("ffno_gn_grad_finalize", plan.B, plan.nwg, _p(part2), _p(sums), self._stream())
+        gb.sums = _p(sums)
+        return sums
+
+    def _gnback(self, plan, t, gnrec, norm, scale, act):
+        a, b, stats = gnrec
+        gb = _lib.FfnoGnBack()
+        gb.t, gb.stats, gb.gamma, gb.aff_a, gb.aff_b = _p(t), _p(stats), _p(norm[0]), _p(a), _p(b)
+        gb.scale = _p(scale)
+        gb.act_gelu = act
+        gb.sums = ctypes.c_void_p(0)
+        gb.depth_w = _p(plan.depth_w)
+        return gb
+
+    def vjp(self, x, z, dx, dy, ybar, keep):
+        """x_bar [B, 6, D, H, W] of <ybar, forward(x, z)>: the hand-written adjoint, block by block in reverse,
+        each block recomputed from its saved input (the reference's non-reentrant checkpointing,
+        models/ffno_model.py:751-780). The gradient with respect to the heights is not computed (None)."""
+        dev = x.device
+        self.sync(dev)
+        Wt = self.weights
+        A = Wt.adjoint()
+        B, _, D, H, W = x.shape
+        z = z.reshape(B, D, H, W)
+        plan = self.plan(B, D, H, W, z, dx, dy)
+        plan.adjoint_tables()
+        C = plan.C
+        N = B * plan.voxels
+        x = x.contiguous().float()
+        ybar = ybar.contiguous().float()
+        st = self._stream()
+        rows = plan.rows
+        f32 = dict(device=dev, dtype=torch.float32)
+        bf = self._buffers(plan, dev, extra=("mp", "ys", "yv", "gx2", "gx1", "gs", "gv", "gh"))
+        part2 = torch.empty(B, plan.nwg, GROUPS, 2, **f32)
+        ones = torch.ones(B, C, **f32)
+        zeros = torch.zeros(B, C, **f32)
+        xbar = torch.empty(B, 6, D, H, W, **f32)
+        hbar = torch.empty(N, C, **f32)
+        _lib.call("ffno_head_bwd", rows, _p(keep[-1]), _p(Wt.p1), _p(Wt.p2), _p(A["p2t"]), _p(A["p1t"]), _p(Wt.p2_bias),
+                  _p(x), ctypes.byref(Wt.head), _p(ybar), _p(hbar), _p(xbar), st)
+        for i in reversed(range(len(Wt.blocks))):
+            h = keep[i]
+            blk, adj = Wt.blocks[i], A["blocks"][i]
+            # recompute the block, keeping the branch transforms' outputs for the post networks' adjoints
+            gn, _ = self._block(plan, i, FfnoAssemble(None, _p(h), _p(ones), _p(zeros), None, 0), None, bf,
+                                ys=bf["ys"], yv=bf["yv"])
+            self._block_tail(plan, i, h, bf, gn)
+            # mlp: h' = x2 + res_mlp GN(mp)
+            gb = self._gnback(plan, bf["mp"], gn["m"], blk["norm_mlp"], blk["res_mlp"], 0)
+            self._gn_sums(plan, gb, hbar, part2)
+            _lib.call("ffno_mlp_bwd", rows, _p(hbar), gb, _p(bf["x2"]), _p(blk["m1"]), _p(adj["m2t"]), _p(adj["m1t"]),
+                      1, _p(bf["gx2"]), st)
+            # pw: x2 = x1 + res_pw GN(pp)
+            gb = self._gnback(plan, bf["pp"], gn["p"], blk["norm_pw"], blk["res_pw"], 0)
+            self._gn_sums(plan, gb, bf["gx2"], part2)
+            _lib.call("ffno_mlp_bwd", rows, _p(bf["gx2"]), gb, _p(bf["x1"]), _p(blk["pw1"]), _p(adj["pw2t"]),
+                      _p(adj["pw1t"]), 1, _p(bf["gx1"]), st)
+            # fusion: x1 = h + res_fused gelu(GN(fp))
+            gb = self._gnback(plan, bf["fp"], gn["f"], blk["norm_fuse"], blk["res_fused"], 1)
+            self._gn_sums(plan, gb, bf["gx1"], part2)
+            sa, sb_, _ = gn["s"]
+            va, vb, _ = gn["v"]
+            _lib.call("ffno_fuse_bwd", rows, _p(bf["gx1"]), gb, _p(h), _p(bf["sp"]), _p(bf["vp"]), _p(sa), _p(sb_),
+                      _p(va), _p(vb), _p(blk["bg1"]), _p(blk["bg2"]), _p(blk["bg2_bias"]), _p(blk["f1"]),
+                      _p(adj["f2t"]), _p(adj["f1t"]), _p(adj["bg2t"]), _p(adj["bg1t"]), _p(bf["gh"]), _p(bf["gs"]),
+                      _p(bf["gv"]), st)
+            # horizontal branch: spec = gelu(GN_s(post(Spec(xs))))
+            gb = self._gnback(plan, bf["sp"], gn["s"], blk["norm_spec"], None, 1)
+            self._gn_sums(plan, gb, bf["gs"], part2)
+            _lib.call("ffno_mlp_bwd", rows, _p(bf["gs"]), gb, _p(bf["ys"]), _p(blk["spec.p1"]), _p(adj["spec.p2t"]),
+                      _p(adj["spec.p1t"]), 0, _p(bf["gx2"]), st)
+            self.horizontal(plan, bf["gx2"], bf["s1"], bf["s2"], bf["xs"], i, adjoint=True)
+            # vertical branch
+            gb = self._gnback(plan, bf["vp"], gn["v"], blk["norm_vert"], None, 1)
+            self._gn_sums(plan, gb, bf["gv"], part2)
+            _lib.call("ffno_mlp_bwd", rows, _p(bf["gv"]), gb, _p(bf["yv"]), _p(blk["vertical.p1"]),
+                      _p(adj["vertical.p2t"]), _p(adj["vertical.p1t"]), 0, _p(bf["gx1"]), st)
+            self.vertical(plan, bf["gx1"], bf["v1"], bf["xv"], i, adjoint=True)
+            # input gates
+            _lib.call("ffno_gates_bwd", rows, _p(h), _p(bf["xs"]), _p(bf["xv"]), _p(bf["gh"]), _p(blk["spec.g1"]),
+                      _p(blk["spec.g2"]), _p(adj["spec.g2t"]), _p(adj["spec.g1t"]), _p(blk["vertical.g1"]),
+                      _p(blk["vertical.g2"]), _p(adj["vertical.g2t"]), _p(adj["vertical.g1t"]), _p(blk["z_proj"]),
+                      _p(hbar), None, st)
+        # lift: h0 = gelu(GN(W_lift x))
+        t0 = torch.empty(N, C, **f32)
+        _lib.call("ffno_lift", rows, _p(x), _p(Wt.lift), _p(t0), FfnoStatsOut(_p(plan.depth_w), _p(bf["part"])), st)
+        g0 = self._gn(plan, bf["part"], Wt.lift_gn, dev)
+        gb = self._gnback(plan, t0, g0, Wt.lift_gn, None, 1)
+        self._gn_sums(plan, gb, hbar, part2)
+        _lib.call("ffno_lift_bwd", rows, _p(hbar), gb, _p(Wt.lift), _p(xbar), st)
+        return xbar, None

@@ -11,7 +11,7 @@
  *   packed weights bf16 (uint16_t) [N_pad][K_pad], K permuted within every 32-block so that one 16-byte load is
  *                  one MFMA operand (see ffno_amd/engine.py: pack_mlp_weight)
  *   twiddles       bf16 pairs (hi, lo) [M_pad][K_pad], the split-bf16 form of a float64 matrix
- *   GroupNorm      partial statistics [B][nwg][G][3] = (weight, mean, M2); finalized stats [B][G][2] = (mean, rstd);
+ *   GroupNorm      partial statistics [B][nwg][G][3] = (weight, mean, M2); finalized stats [B][G][3] = (mean, rstd, W);
  *                  per-channel affine [B][C] scale and [B][C] shift
  *
  * Every call returns 0 on success or a negative code; ffno_last_error() gives the message (thread-local).
@@ -57,8 +57,8 @@ typedef struct {
 int ffno_lift(FfnoRows rows, const float* x_in, const float* w_lift /* [C][6] */, float* out, FfnoStatsOut stats,
               hipStream_t stream);
 
-/* GroupNorm finalize: merge partial statistics, produce (mean, rstd) and the per-channel affine
- * (models/ffno_model.py:77-95, eps 1e-5). gamma/beta [C]. */
+/* GroupNorm finalize: merge partial statistics, produce stats [B][8][3] = (mean, rstd, total weight) and the
+ * per-channel affine (models/ffno_model.py:77-95, eps 1e-5). gamma/beta [C]. */
 int ffno_gn_finalize(int B, int nwg, int C, const float* partials, const float* gamma, const float* beta,
                      float* stats, float* aff_a, float* aff_b, hipStream_t stream);
 
@@ -102,8 +102,9 @@ typedef struct {
   float dxi_coef;     /* 4.98e-4 * 5040 */
 } FfnoHeadDesc;
 
-int ffno_head(FfnoRows rows, FfnoAssemble in, const uint16_t* p1, const uint16_t* p2, const float* p2_bias,
-              const float* x_in /* [B,6,DHW] */, const FfnoHeadDesc* desc, float* y, hipStream_t stream);
+int ffno_head(FfnoRows rows, FfnoAssemble in, float* x_out, const uint16_t* p1, const uint16_t* p2,
+              const float* p2_bias, const float* x_in /* [B,6,DHW] */, const FfnoHeadDesc* desc, float* y,
+              hipStream_t stream);
 
 /* Split-bf16 ("bf16x3") GEMM of a small precomputed real matrix A [m x k] against batched data X [k x n]:
  * Y[b][m][n] = sum_k A[m][k] X[b][k][n]. It carries every DFT (rfft/ifft/irfft along W and H) and the vertical
@@ -141,6 +142,66 @@ typedef struct {
 } FfnoMix;
 
 int ffno_mix(const FfnoMix* m, hipStream_t stream);
+
+/* ------------------------------------------------------------------------------------------------------------
+ * Adjoint (input VJP). The reference differentiates its forward with torch autograd through non-reentrant
+ * checkpoints (models/ffno_model.py:751-798; ffno_fsdp_runtime.py:306-348). Here each block is recomputed from its
+ * saved input and differentiated by the kernels below; they return input gradients only.
+ * ------------------------------------------------------------------------------------------------------------ */
+
+/* One depth-weighted GroupNorm seen from its output gradient u_bar: g_hat = scale * u_bar * gamma (* GELU'(n) when
+ * the GN output went through GELU, n = a*t + b the forward affine), x_hat = (t - mean) * rstd. The backward needs
+ * S1 = sum g_hat and S2 = sum g_hat * x_hat per (sample, group), then
+ * t_bar = rstd * (g_hat - (w / W) * (S1 + x_hat * S2)), w the voxel's depth weight and W the group's total weight. */
+typedef struct {
+  const float* t;         /* GN input [N, C] */
+  const float* stats;     /* [B][8][3] (mean, rstd, W) from ffno_gn_finalize */
+  const float* gamma;     /* [C] */
+  const float* aff_a;     /* [B][C] forward affine (used when act_gelu) */
+  const float* aff_b;
+  const float* scale;     /* device scalar or NULL (1) */
+  int act_gelu;
+  const float* sums;      /* [B][8][2] (S1, S2) from ffno_gn_grad_finalize; unused by ffno_gn_grad_stats */
+  const float* depth_w;   /* [B, D*H*W] */
+} FfnoGnBack;
+
+/* S1, S2 partial sums of one GroupNorm's backward: partials [B][nwg][8][2]. */
+int ffno_gn_grad_stats(FfnoRows rows, FfnoGnBack gn, const float* ubar, float* partials, hipStream_t stream);
+/* partials [B][nwg][8][2] -> sums [B][8][2] (fixed order, deterministic) */
+int ffno_gn_grad_finalize(int B, int nwg, const float* partials, float* sums, hipStream_t stream);
+
+/* Adjoint of y = W2 gelu(W1 x) behind a GroupNorm: t_bar = GN backward of u_bar, then
+ * x_bar = [u_bar if residual] + W1^T (GELU'(W1 x) * W2^T t_bar) (models/ffno_model.py:547-551 for pw and mlp with
+ * residual; :221-225, :369-373 for the branch post networks without). w2t, w1t are the transposed weights packed. */
+int ffno_mlp_bwd(FfnoRows rows, const float* ubar, FfnoGnBack gn, const float* x, const uint16_t* w1,
+                 const uint16_t* w2t, const uint16_t* w1t, int residual, float* xbar, hipStream_t stream);
+
+/* Adjoint of the branch fusion (models/ffno_model.py:524-545): from x1_bar (gradient of x1 = x + res_fused * f)
+ * to the gradients of the block input (including x1_bar's residual path), of s = gelu(GN_s(sp)) and of
+ * v = gelu(GN_v(vp)); gates and hidden layers are recomputed. */
+int ffno_fuse_bwd(FfnoRows rows, const float* x1bar, FfnoGnBack gnf, const float* h, const float* sp, const float* vp,
+                  const float* s_a, const float* s_b, const float* v_a, const float* v_b, const uint16_t* bg1,
+                  const uint16_t* bg2, const float* bg2_bias, const uint16_t* f1, const uint16_t* f2t,
+                  const uint16_t* f1t, const uint16_t* bg2t, const uint16_t* bg1t, float* hbar, float* sbar,
+                  float* vbar, hipStream_t stream);
+
+/* Adjoint of both input gates (models/ffno_model.py:266, :439-440): hbar_out = hbar_in + d(xs)/dh^T xs_bar +
+ * d(xv)/dh^T xv_bar; zbar (optional, accumulated) += sum_c xv_bar * z_proj. */
+int ffno_gates_bwd(FfnoRows rows, const float* h, const float* xsbar, const float* xvbar, const float* hbar_in,
+                   const uint16_t* g1, const uint16_t* g2, const uint16_t* g2t, const uint16_t* g1t,
+                   const uint16_t* v1, const uint16_t* v2, const uint16_t* v2t, const uint16_t* v1t,
+                   const float* z_proj, float* hbar_out, float* zbar, hipStream_t stream);
+
+/* Adjoint of the head (models/ffno_model.py:730-749, :790-798) from the output gradient ybar [B, Cout, DHW]:
+ * writes hbar [N, C] and the direct input gradient xbar_in [B, 6, DHW] (T, ne and rho through the LTE weights
+ * and the total; the other channels 0). */
+int ffno_head_bwd(FfnoRows rows, const float* h, const uint16_t* p1, const uint16_t* p2, const uint16_t* p2t,
+                  const uint16_t* p1t, const float* p2_bias, const float* x_in, const FfnoHeadDesc* desc,
+                  const float* ybar, float* hbar, float* xbar_in, hipStream_t stream);
+
+/* Adjoint of the lift (models/ffno_model.py:782-783): xbar_in += W_lift^T (GN backward of ubar). */
+int ffno_lift_bwd(FfnoRows rows, const float* ubar, FfnoGnBack gn, const float* w_lift, float* xbar_in,
+                  hipStream_t stream);
 
 #ifdef __cplusplus
 }

@@ -67,8 +67,8 @@ def _small_model(width=64, rank=8, layers=1, species=("H", "CA")):
 
 
 def test_post_mlp_and_groupnorm_statistics():
-    """ffno_post (C -> 2C -> C, bf16 MFMA) against float64 on bf16-rounded operands (2e-3 of range), and its fused
-    weighted GroupNorm statistics against a float64 weighted mean/variance (1e-4 relative)."""
+    """ffno_post (C -> 2C -> C, split-bf16 MFMA) against float64 (5e-5 of range), and its fused weighted GroupNorm
+    statistics against a float64 weighted mean/variance (1e-4 relative)."""
     from ffno_amd import _lib
     from ffno_amd._lib import FfnoRows, FfnoStatsOut
     from ffno_amd.engine import pack_mlp_weight
@@ -82,8 +82,9 @@ def test_post_mlp_and_groupnorm_statistics():
         nwg = 7
         part = torch.empty(B, nwg, 8, 3, device=DEV)
         y = torch.empty_like(u)
-        _lib.call("ffno_post", FfnoRows(B, vox, C, nwg), _p(u), _p(pack_mlp_weight(w1)), _p(pack_mlp_weight(w2)),
-                  _p(y), FfnoStatsOut(_p(dw), _p(part)), _stream())
+        p1, p2 = pack_mlp_weight(w1), pack_mlp_weight(w2)          # held: the kernel reads them after the call
+        _lib.call("ffno_post", FfnoRows(B, vox, C, nwg), _p(u), _p(p1), _p(p2), _p(y), FfnoStatsOut(_p(dw), _p(part)),
+                  _stream())
         gamma = torch.ones(C, device=DEV)
         beta = torch.zeros(C, device=DEV)
         a = torch.empty(B, C, device=DEV)
@@ -91,11 +92,10 @@ def test_post_mlp_and_groupnorm_statistics():
         stats = torch.empty(B, 8, 2, device=DEV)
         _lib.call("ffno_gn_finalize", B, nwg, C, _p(part), _p(gamma), _p(beta), _p(stats), _p(a), _p(b), _stream())
         torch.cuda.synchronize()
-        h = _gelu(_bf(u) @ _bf(w1).t())
-        ref = _bf(h) @ _bf(w2).t()
+        ref = _gelu(u.double() @ w1.double().t()) @ w2.double().t()
         err = (y.double() - ref).abs().max() / ref.abs().max()
         print("post", C, "rel err", float(err))
-        assert err < 2e-3
+        assert err < 5e-5
         yy = y.double().reshape(B, vox, 8, C // 8)
         ww = dw.double()[:, :, None, None].expand_as(yy)
         mean = (ww * yy).sum((1, 3)) / ww.sum((1, 3))
@@ -107,7 +107,7 @@ def test_post_mlp_and_groupnorm_statistics():
 def test_spectral_pipelines_match_fft():
     """The horizontal rfft2 -> per-mode mix -> irfft2 and the vertical Filon -> mix -> synthesis, each as the
     engine runs them, against float64 torch.fft / einsum statements with the plan's own gates and low-rank tables:
-    1e-2 of range (the channel mix itself is bf16)."""
+    1e-4 of range (split-bf16 transforms and mix)."""
     from ffno_amd.engine import Engine
     model = _small_model()
     eng = Engine(model)
@@ -147,7 +147,7 @@ def test_spectral_pipelines_match_fft():
     torch.cuda.synchronize()
     err = (out.double().cpu() - ref).abs().max() / ref.abs().max()
     print("horizontal rel err", float(err))
-    assert err < 1e-2
+    assert err < 1e-4
     # vertical
     M = plan.modes
     xv = torch.randn(B, D, H, W, C, device=DEV)
@@ -188,4 +188,4 @@ def test_spectral_pipelines_match_fft():
     ref = yv.permute(0, 2, 3, 4, 1)
     err = (outv.double().cpu() - ref).abs().max() / ref.abs().max()
     print("vertical rel err", float(err))
-    assert err < 1e-2
+    assert err < 1e-4

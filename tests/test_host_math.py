@@ -80,13 +80,14 @@ def test_packing_matches_kernel_contraction(N, segments):
     rng = np.random.default_rng(2)
     K = sum(segments)
     w = torch.from_numpy(rng.standard_normal((N, K))).float()
-    wp = pack_mlp_weight(w, segments).view(torch.bfloat16).double().numpy()
+    packed = pack_mlp_weight(w, segments).view(torch.bfloat16).double()
+    assert packed.shape[0] == 2
+    wp = (packed[0] + packed[1]).numpy()                       # hi + lo
     act_parts = [rng.standard_normal(s) for s in segments]
     act = np.concatenate([np.pad(a, (0, T.round_up(len(a), 32) - len(a))) for a in act_parts])
     got = _kernel_contraction(wp, act)[:N]
-    wb = w.to(torch.bfloat16).double().numpy()
-    expected = wb @ np.concatenate(act_parts)
-    assert np.abs(got - expected).max() < 1e-10 * max(1.0, np.abs(expected).max())
+    expected = w.double().numpy() @ np.concatenate(act_parts)
+    assert np.abs(got - expected).max() < 2e-5 * max(1.0, np.abs(expected).max())
     assert not np.any(_kernel_contraction(wp, act)[N:])
 
 
