@@ -1,0 +1,202 @@
+"""Benchmark: FFNO3D forward + input VJP throughput (Mvoxels/s) of the HIP engine on one MI355X.
+
+Workload (one "step"): one forward and one input VJP (cotangent fixed) of FFNO3D at the production shape -- B=8
+patches, D=128, H=W=64, width 64, vertical modes 32 (z_min_wavelength 1.28 at z_reference_length 40), 8 layers, rank 8,
+hidden 128, species H + Ca II -- on a synthetic stratified atmosphere (seed 0) with seeded weights (seed 0). Inputs are
+resident in HBM before timing.
+
+Beside it, in the same run: the plain FP32 GPU path (stock torch ops, ffno_amd.reference_torch) on the same inputs
+and weights (its speed and the HIP path's relative error in log10 n against it), and the CPU oracle in float32 on a
+bounded sample (the CPU workload: B=1, D=64, 32 x 32, width 32, 4 layers) timed on this host's cores.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = dict(B=8, D=128, H=64, W=64, width=64, n_layers=8, z_min_wavelength=1.28, hidden=128, rank=8)
+CPU_WORKLOAD = dict(B=1, D=64, H=32, W=32, width=32, n_layers=4, z_min_wavelength=3.5, hidden=128, rank=8)
+
+
+def model_config(w):
+    from ffno_amd.atoms import species
+    return dict(in_channels=6, out_channels=12, width=w["width"], n_layers=w["n_layers"], hidden_spectral=w["hidden"],
+                hidden_vertical=w["hidden"], freq_mix_rank=w["rank"], z_reference_length=40.0,
+                z_min_wavelength=w["z_min_wavelength"], z_feature_wavelength=0.3, species=species(["H", "CA"]))
+
+
+def make(w, device):
+    import numpy as np
+    from ffno_amd import synthetic
+    x, z, mx, sx = synthetic.atmosphere(w["B"], w["D"], w["H"], w["W"], seed=0)
+    model = synthetic.random_model(0, **model_config(w))
+    model.set_normalization(mx, sx, np.zeros(12), np.ones(12))
+    ct = synthetic.cotangent((w["B"], 12, w["D"], w["H"], w["W"]), 0)
+    return model.to(device), x.to(device), z.to(device), ct.to(device)
+
+
+# Algorithmic HBM bytes per voxel of one launch, by kernel (float32 activations of C channels read or written).
+def kernel_bytes_per_voxel(name, C):
+    units = {"k_fuse_bwd": 8, "k_mlp_bwd": 4, "k_gates_bwd": 5, "k_fuse": 4, "k_resmlp": 4, "k_block_in": 5,
+             "k_post": 2, "k_head_bwd": 1, "k_gn_grad_stats": 2}
+    return units[name] * C * 4
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-baselines", action="store_true", help="skip the plain FP32 GPU and CPU legs")
+    ap.add_argument("--kernel", default="k_fuse_bwd", help="kernel whose launches are timed for the roofline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported (requests shard whole across GPUs "
+              f"with no exchange; run one process per GPU)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from ffno_amd import _lib
+
+    dev = torch.device("cuda:0")
+    w = WORKLOAD
+    voxels = w["B"] * w["D"] * w["H"] * w["W"]
+    model, x, z, ct = make(w, dev)
+
+    def step():
+        xg = x.detach().requires_grad_(True)
+        y = model(xg, z, 48e3, 48e3)
+        (xb,) = torch.autograd.grad(y, xg, grad_outputs=ct)
+        return y, xb
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    start.record()
+    for _ in range(args.steps):
+        y, xb = step()
+    end.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = start.elapsed_time(end) / args.steps
+    value = voxels / (ms * 1e-3) / 1e6
+
+    # dominant-kernel timing: HIP events around every launch of one kernel during one more step, recorded on the
+    # stream the launches go to (torch's current stream, which the engine passes to every entry point)
+    name = args.kernel
+    entry = {"k_fuse_bwd": "ffno_fuse_bwd", "k_mlp_bwd": "ffno_mlp_bwd", "k_gates_bwd": "ffno_gates_bwd",
+             "k_fuse": "ffno_fuse", "k_resmlp": "ffno_resmlp", "k_block_in": "ffno_block_in", "k_post": "ffno_post",
+             "k_gn_grad_stats": "ffno_gn_grad_stats"}[name]
+    events = []
+    real_call = _lib.call
+
+    def timed_call(n, *a):
+        if n != entry:
+            return real_call(n, *a)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        real_call(n, *a)
+        e1.record()
+        events.append((e0, e1))
+
+    _lib.call = timed_call
+    try:
+        step()
+        torch.cuda.synchronize()
+    finally:
+        _lib.call = real_call
+    kms = sum(a.elapsed_time(b) for a, b in events) / max(len(events), 1)
+    kbytes = kernel_bytes_per_voxel(name, w["width"]) * voxels
+    achieved = kbytes / (kms * 1e-3) / 1e9
+
+    out = {
+        "metric": "FFNO3D fwd+VJP Mvoxels/s vs plain FP32 (rel err<=1e-3); service p50/p99 ms",
+        "value": round(value, 3),
+        "unit": "Mvoxels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "wall_s_timed": round(wall, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 activations, split-bf16 MFMA (fp32 accumulate)",
+        "data": "synthetic (stratified random-field atmosphere, seed 0; seeded random weights)",
+        "config": {"workload": "FFNO3D forward + input VJP, B=8 D=128 H=W=64 width=64 modes(D)=32 layers=8",
+                   "global_batch": w["B"], "voxels_per_step": voxels, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": name, "launches": len(events), "avg_ms": round(kms, 4),
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+    }
+
+    if not args.no_baselines:
+        from ffno_amd.reference_torch import PlainFFNO3D
+        plain = PlainFFNO3D(model).to(dev)
+
+        def plain_step():
+            xg = x.detach().requires_grad_(True)
+            yp = plain(xg, z, 48e3, 48e3)
+            (xbp,) = torch.autograd.grad(yp, xg, grad_outputs=ct)
+            return yp, xbp
+
+        yp, xbp = plain_step()
+        torch.cuda.synchronize()
+        s0 = time.perf_counter()
+        yp, xbp = plain_step()
+        torch.cuda.synchronize()
+        pms = (time.perf_counter() - s0) * 1e3
+        rel = float(((y.detach() - yp.detach()).abs() / yp.detach().abs().clamp_min(1.0)).max())
+        grel = float((xb - xbp).norm() / xbp.norm())
+        del yp, xbp
+        out["plain_fp32_gpu"] = {"value": round(voxels / (pms * 1e-3) / 1e6, 3), "unit": "Mvoxels/s",
+                                 "ms_per_step": round(pms, 1), "speedup": round(pms / ms, 2),
+                                 "rel_err_log10n": rel, "grad_rel_err": grel}
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline():
+    """The float32 CPU oracle (oracle/ffno_oracle.py) on the CPU workload, forward + input VJP, all host cores."""
+    import torch
+    from ffno_amd import synthetic
+    from oracle import ffno_oracle as O
+    import numpy as np
+    cores = len(os.sched_getaffinity(0))
+    torch.set_num_threads(cores)
+    w = CPU_WORKLOAD
+    cfg = model_config(w)
+    x, z, mx, sx = synthetic.atmosphere(w["B"], w["D"], w["H"], w["W"], seed=0)
+    model = synthetic.random_model(0, **cfg)
+    model.set_normalization(mx, sx, np.zeros(12), np.ones(12))
+    sd = O.to_dtype({k: v.detach() for k, v in model.state_dict().items()}, torch.float32)
+    ocfg = O.model_config(**{k: cfg[k] for k in ("width", "n_layers", "hidden_spectral", "hidden_vertical",
+                                                 "freq_mix_rank", "z_reference_length", "z_min_wavelength",
+                                                 "z_feature_wavelength", "species", "out_channels")})
+    ct = synthetic.cotangent((w["B"], 12, w["D"], w["H"], w["W"]), 0)
+    O.vjp(sd, ocfg, x, z, 48e3, 48e3, ct)
+    times = []
+    budget = time.perf_counter() + 20.0
+    while len(times) < 3 and (not times or time.perf_counter() < budget):
+        t0 = time.perf_counter()
+        O.vjp(sd, ocfg, x, z, 48e3, 48e3, ct)
+        times.append(time.perf_counter() - t0)
+    t = sorted(times)[len(times) // 2]
+    voxels = w["B"] * w["D"] * w["H"] * w["W"]
+    return {"value": round(voxels / t / 1e6, 4), "unit": "Mvoxels/s", "cores": cores, "kind": "port",
+            "sample": f"float32 oracle forward + input VJP, B=1 D=64 H=W=32 width 32, 4 layers, 12 vertical modes; "
+                      f"median of {len(times)} after 1 warm-up, {t:.2f} s each"}
+
+
+if __name__ == "__main__":
+    main()

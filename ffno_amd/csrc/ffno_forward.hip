@@ -448,6 +448,8 @@ __global__ __launch_bounds__(256) void k_tw_gemm(FfnoTwGemm g) {
 // ---------------------------------------------------------------------------------------------------------------
 // per-mode complex channel mix, in place
 // ---------------------------------------------------------------------------------------------------------------
+constexpr int MIX_TILES = 8;   // row tiles of 16 per work item of the mix
+
 template <int CT, bool LW>
 __global__ __launch_bounds__(256) void k_mix(FfnoMix mx) {
   constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32, NWC = 2 * C * 2 * KP;
@@ -456,14 +458,20 @@ __global__ __launch_bounds__(256) void k_mix(FfnoMix mx) {
   const Wm Wc = LW ? stage_w(lds, mx.wc, NWC) : global_w(mx.wc, NWC);
   __syncthreads();
   Lane L;
+  // work items are (mode, chunk of MIX_TILES row tiles), mode-major so that neighbouring waves share a mode's tables
   const int rtiles = (mx.rows + 15) / 16;
-  for (int mode = blockIdx.x * 4 + L.wave; mode < mx.modes; mode += gridDim.x * 4) {
+  const int nchunk = (rtiles + MIX_TILES - 1) / MIX_TILES;
+  const int64_t items = (int64_t)mx.modes * nchunk;
+  for (int64_t item = (int64_t)blockIdx.x * 4 + L.wave; item < items; item += (int64_t)gridDim.x * 4) {
+    const int mode = (int)(item / nchunk);
+    const int rt0 = (int)(item % nchunk) * MIX_TILES;
+    const int rt1 = rt0 + MIX_TILES < rtiles ? rt0 + MIX_TILES : rtiles;
     const float gr = mx.gate[2 * mode], gi = mx.gate[2 * mode + 1];
     const float fac = mx.factor ? mx.factor[mode] : 1.f;
     // per-mode low-rank tables: V [16][KP] and U [C][32], hi and lo halves of each mode's block
     const Wm V = global_w(mx.v_p ? mx.v_p + (int64_t)mode * 2 * 16 * KP : mx.wc, 16 * KP);
     const Wm U = global_w(mx.u_p ? mx.u_p + (int64_t)mode * 2 * C * 32 : mx.wc, C * 32);
-    for (int rt = 0; rt < rtiles; ++rt) {
+    for (int rt = rt0; rt < rt1; ++rt) {
       const int r = rt * 16 + L.v;
       const bool ok = r < mx.rows;
       const int64_t base = imap(ok ? r : 0, mx.r_split, mx.r_hi, mx.r_lo) + (int64_t)mode * mx.mode_stride;
@@ -642,8 +650,9 @@ int ffno_mix(const FfnoMix* m, hipStream_t stream) {
   if (!m || !m->re || !m->im || !m->wc || !m->gate) return fail("ffno_mix: null pointer");
   if (m->rank && (!m->v_p || !m->u_p || m->rank > 16)) return fail("ffno_mix: low-rank tables missing or rank > 16");
   if (m->rows <= 0 || m->modes <= 0 || m->r_split <= 0) return fail("ffno_mix: bad sizes");
-  const int blocks = (m->modes + 3) / 4;
-  const int grid = blocks < 4096 ? blocks : 4096;
+  const int64_t items = (int64_t)m->modes * (((m->rows + 15) / 16 + MIX_TILES - 1) / MIX_TILES);
+  const int64_t blocks = (items + 3) / 4;
+  const int grid = (int)(blocks < 2048 ? blocks : 2048);
   FFNO_DISPATCH_CT(m->C, {
     const size_t lds = (size_t)2 * CT * 16 * 2 * ks_of(CT) * 32 * 2 * 2;
     if (lds <= 160 * 1024) {

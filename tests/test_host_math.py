@@ -126,3 +126,21 @@ def test_cpu_tensors_are_refused():
                                    out_channels=6, species=species(["H"]))
     with pytest.raises(RuntimeError):
         model(torch.zeros(1, 6, 4, 4, 4), torch.zeros(1, 4, 4, 4), 48e3, 48e3)
+
+
+def test_plain_torch_baseline_matches_oracle():
+    """The plain FP32 comparison path (ffno_amd.reference_torch) equals the float64 oracle on golden case A to
+    float32 precision (1e-4 of range), so the bench's baseline computes the same network."""
+    from golden_cases import load, oracle_config
+    from oracle import ffno_oracle as O
+    from ffno_amd.model import FFNO3D
+    from ffno_amd.reference_torch import PlainFFNO3D
+    data, config, state = load("case_a_shared64")
+    model = FFNO3D(**config)
+    model.load_state_dict(state, strict=True)
+    plain = PlainFFNO3D(model)
+    x = torch.from_numpy(data["x"]).float()
+    z = torch.from_numpy(data["z"]).float()
+    with torch.no_grad():
+        y = plain(x, z, 48e3, 48e3)
+    assert np.abs(y.double().numpy() - data["y"]).max() / np.abs(data["y"]).max() < 1e-4
