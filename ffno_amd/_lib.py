@@ -49,7 +49,7 @@ class FfnoMix(ctypes.Structure):
 
 class FfnoGnBack(ctypes.Structure):
     _fields_ = [("t", c_ptr), ("stats", c_ptr), ("gamma", c_ptr), ("aff_a", c_ptr), ("aff_b", c_ptr),
-                ("scale", c_ptr), ("act_gelu", ctypes.c_int), ("sums", c_ptr), ("depth_w", c_ptr)]
+                ("scale", c_ptr), ("act_gelu", ctypes.c_int), ("sums", c_ptr), ("depth_w", c_ptr), ("wbar", c_ptr)]
 
 
 # entry point -> argument types (every one returns int)
@@ -72,6 +72,7 @@ SIGNATURES = {
     "ffno_gates_bwd": [FfnoRows] + [c_ptr] * 15 + [c_ptr],
     "ffno_head_bwd": [FfnoRows] + [c_ptr] * 7 + [ctypes.POINTER(FfnoHeadDesc)] + [c_ptr] * 3 + [c_ptr],
     "ffno_lift_bwd": [FfnoRows, c_ptr, FfnoGnBack, c_ptr, c_ptr, c_ptr],
+    "ffno_dot_reduce": [FfnoRows, ctypes.c_int, c_ptr, ctypes.c_int, c_ptr, c_ptr, c_ptr],
 }
 
 _lib = None

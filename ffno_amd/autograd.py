@@ -1,6 +1,8 @@
 """torch.autograd.Function around the HIP engine: forward runs the fused kernels; backward is the hand-written
-adjoint (per-block recomputation from the saved block inputs), returning x_bar and z_bar as the reference's
-autograd would (models/ffno_model.py:751-780 is the reference's checkpointed backward)."""
+adjoint (per-block recomputation from the saved block inputs), returning x_bar and z_bar (models/ffno_model.py:751-780
+is the reference's checkpointed backward). z_bar is the per-column gradient of every column's own heights; on a shared
+grid the reference's autograd instead sends the vertical transform's height sensitivity of all columns to the first
+column's heights (models/ffno_model.py:270-276), so the two agree where the reference takes its per-column path."""
 import torch
 
 
@@ -21,9 +23,9 @@ class FFNOFunction(torch.autograd.Function):
     def backward(ctx, ybar):
         x, z = ctx.saved_tensors
         engine = ctx.module.engine()
-        xbar, zbar = engine.vjp(x, z, ctx.dx, ctx.dy, ybar.contiguous(), ctx.keep)
+        xbar, zbar = engine.vjp(x, z, ctx.dx, ctx.dy, ybar.contiguous(), ctx.keep, want_z=ctx.needs_input_grad[1])
         ctx.keep = None
-        return xbar, zbar, None, None, None
+        return xbar if ctx.needs_input_grad[0] else None, zbar, None, None, None
 
 
 def _spacing(v):

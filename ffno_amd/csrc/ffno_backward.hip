@@ -30,12 +30,14 @@ __device__ __forceinline__ void gn_ghat(const FfnoGnBack& gb, float scale, int64
   }
 }
 
-// t_bar = rstd * (g_hat - (w / W) * (S1 + x_hat * S2))
+// t_bar = rstd * (g_hat - (w / W) * (S1 + x_hat * S2)); with gb.wbar, also the depth-weight gradient of this voxel,
+// dL/dw = -(1/W) sum_c (S1 x_hat + S2 (x_hat^2 - rstd^2 var) / 2) with rstd^2 var = 1 - eps rstd^2, accumulated
 template <int CT>
 __device__ __forceinline__ void gn_back(const FfnoGnBack& gb, float scale, int64_t row, int b, bool ok, int q,
                                         const f32x4 (&ub)[CT], f32x4 (&tb)[CT]) {
   constexpr int C = CT * 16, CPG = C / G;
   const float w = ok ? gb.depth_w[row] : 0.f;
+  float wb = 0.f;
 #pragma unroll
   for (int t = 0; t < CT; ++t) {
     f32x4 gh, xh;
@@ -46,7 +48,14 @@ __device__ __forceinline__ void gn_back(const FfnoGnBack& gb, float scale, int64
       const float* st = gb.stats + (b * G + g) * 3;
       const float* sm = gb.sums + (b * G + g) * 2;
       tb[t][j] = ok ? st[1] * (gh[j] - (w / st[2]) * (sm[0] + xh[j] * sm[1])) : 0.f;
+      const float r2var = 1.f - 1e-5f * st[1] * st[1];
+      wb -= (sm[0] * xh[j] + 0.5f * sm[1] * (xh[j] * xh[j] - r2var)) / st[2];
     }
+  }
+  if (gb.wbar) {
+    wb += __shfl_xor(wb, 16);
+    wb += __shfl_xor(wb, 32);
+    if (ok && q == 0) gb.wbar[row] += wb;
   }
 }
 
@@ -493,6 +502,36 @@ __global__ __launch_bounds__(NW * 64) void k_lift_bwd(FfnoRows rows, const float
   }
 }
 
+// ---------------------------------------------------------------------------------------------------------------
+// channel contraction for the height gradient
+// ---------------------------------------------------------------------------------------------------------------
+template <int CT>
+__global__ __launch_bounds__(256) void k_dot_reduce(FfnoRows rows, int D, const float* __restrict__ a, int delta,
+                                                    const float* __restrict__ r, float* __restrict__ zbar) {
+  constexpr int C = CT * 16;
+  Lane L;
+  const int b = blockIdx.y;
+  const int64_t hw = rows.voxels / D;
+  const int64_t nt = (rows.voxels + 15) / 16;
+  for (int64_t tile = (int64_t)blockIdx.x * 4 + L.wave; tile < nt; tile += (int64_t)gridDim.x * 4) {
+    const int64_t vox = tile * 16 + L.v;
+    const int64_t d = vox / hw;
+    const bool ok = vox < rows.voxels && d + delta >= 0 && d + delta < D;
+    const int64_t row = (int64_t)b * rows.voxels + vox;
+    float s = 0.f;
+#pragma unroll
+    for (int t = 0; t < CT; ++t) {
+      const f32x4 av = ld4(a + (row + delta * hw) * C + 16 * t + 4 * L.q, ok);
+      const f32x4 rv = ld4(r + row * C + 16 * t + 4 * L.q, ok);
+      const f32x4 pr = av * rv;
+      s += pr[0] + pr[1] + pr[2] + pr[3];
+    }
+    s += __shfl_xor(s, 16);
+    s += __shfl_xor(s, 32);
+    if (ok && L.q == 0) zbar[row] += s;
+  }
+}
+
 }  // namespace
 }  // namespace ffno
 
@@ -600,6 +639,15 @@ int ffno_head_bwd(FfnoRows rows, const float* h, const uint16_t* p1, const uint1
     }
   });
   return check_launch("ffno_head_bwd");
+}
+
+int ffno_dot_reduce(FfnoRows rows, int D, const float* a, int delta, const float* r, float* zbar, hipStream_t stream) {
+  if (!rows_ok(rows) || D <= 0 || rows.voxels % D || !a || !r || !zbar) return fail("ffno_dot_reduce: bad arguments");
+  const int64_t tiles = (rows.voxels + 15) / 16;
+  const int64_t bx = (tiles + 3) / 4;
+  dim3 grid((unsigned)(bx < 1024 ? bx : 1024), rows.B);
+  FFNO_DISPATCH_CT(rows.C, (k_dot_reduce<CT><<<grid, 256, 0, stream>>>(rows, D, a, delta, r, zbar)));
+  return check_launch("ffno_dot_reduce");
 }
 
 int ffno_lift_bwd(FfnoRows rows, const float* ubar, FfnoGnBack gn, const float* w_lift, float* xbar_in,

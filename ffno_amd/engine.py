@@ -219,6 +219,8 @@ class Plan:
         self._mats = dict(w_fwd=hm["w_fwd"], h_fwd=hm["h_fwd"], h_inv=hm["h_inv"], w_inv=hm["w_inv"], v_fwd=v_fwd,
                           v_inv=v_inv)
         self._tw_adj = None
+        self._column, self._k, self._factor = column, k, factor
+        self._height = None
         # per-mode gates and low-rank tables (models/ffno_model.py:375-392, :443-447, :231-237, :285-296), float32
         sd = engine.weights.sd
         ky = torch.fft.fftfreq(H, d=float(dy), device=dev)
@@ -248,6 +250,22 @@ class Plan:
         self.nwg = max(1, min((nt + NW - 1) // NW, max(1, 2048 // B)))
         self.rows = FfnoRows(B, self.voxels, C, self.nwg)
         self.device = dev
+
+    def height_tables(self):
+        """Height derivatives of the vertical transforms for the z-gradient: the three bands of the Filon weights'
+        Jacobian, the synthesis' derivative, and for the taper (non-zero only for modes between 2 pi / (3 dz) and
+        pi / dz) the transposed unscaled synthesis and the per-mode factor * d taper / d dz."""
+        if self._height is None:
+            bands = T.filon_height_bands(self._column, self._k)
+            v_dz, v0, dfac, i_med, sgn = T.vertical_height_terms(self._column, self._k, self._factor, self.length)
+            col = np.asarray(self._column)
+            ties = np.flatnonzero(col == col.min())
+            self._height = dict(
+                bands={d: T.split_bf16(bands[d], self.device) for d in (-1, 0, 1)},
+                v_dz=T.split_bf16(v_dz, self.device),
+                v0=T.split_bf16(np.ascontiguousarray(v0.T), self.device) if np.any(dfac != 0) else None,
+                dfac=torch.from_numpy(dfac).float().to(self.device), i_med=i_med, sign=sgn, argmin=ties)
+        return self._height
 
     def adjoint_tables(self):
         """Transposed transform matrices and adjoint mix tables: conj(g), and the low-rank roles swapped
@@ -502,10 +520,12 @@ class Engine:
         gb.depth_w = _p(plan.depth_w)
         return gb
 
-    def vjp(self, x, z, dx, dy, ybar, keep):
-        """x_bar [B, 6, D, H, W] of <ybar, forward(x, z)>: the hand-written adjoint, block by block in reverse,
-        each block recomputed from its saved input (the reference's non-reentrant checkpointing,
-        models/ffno_model.py:751-780). The gradient with respect to the heights is not computed (None)."""
+    def vjp(self, x, z, dx, dy, ybar, keep, want_z=False):
+        """(x_bar [B, 6, D, H, W], z_bar [B, D, H, W] or None) of <ybar, forward(x, z)>: the hand-written adjoint,
+        block by block in reverse, each block recomputed from its saved input (the reference's non-reentrant
+        checkpointing, models/ffno_model.py:751-780). With want_z the height gradient is computed too, per column:
+        through the depth weights of every GroupNorm, z_proj, and the vertical transform's Filon weights, phases,
+        taper and per-column shift (models/ffno_model.py:28-74, :239-309)."""
         dev = x.device
         self.sync(dev)
         Wt = self.weights
@@ -516,38 +536,60 @@ class Engine:
         plan.adjoint_tables()
         C = plan.C
         N = B * plan.voxels
+        M = plan.modes
+        hwc = H * W * C
+        big = 1 << 40
         x = x.contiguous().float()
         ybar = ybar.contiguous().float()
         st = self._stream()
         rows = plan.rows
         f32 = dict(device=dev, dtype=torch.float32)
         bf = self._buffers(plan, dev, extra=("mp", "ys", "yv", "gx2", "gx1", "gs", "gv", "gh"))
+        bf["v1y"] = torch.empty_like(bf["v1"])
         part2 = torch.empty(B, plan.nwg, GROUPS, 2, **f32)
         ones = torch.ones(B, C, **f32)
         zeros = torch.zeros(B, C, **f32)
         xbar = torch.empty(B, 6, D, H, W, **f32)
         hbar = torch.empty(N, C, **f32)
+        if want_z:
+            ht = plan.height_tables()
+            zbar = torch.zeros(B, plan.voxels, **f32)        # z_proj terms
+            zbar_s = torch.zeros(B, plan.voxels, **f32)      # vertical transform, with respect to the shifted heights
+            wbar = torch.zeros(B, plan.voxels, **f32)        # depth weights
+            rbuf = torch.empty(N, C, **f32)
+            dfac_col = torch.zeros(B, H * W, **f32) if ht["v0"] is not None else None
+        else:
+            zbar = zbar_s = wbar = None
+
+        def gnback(t, rec, norm, scale, act):
+            gb = self._gnback(plan, t, rec, norm, scale, act)
+            gb.wbar = _p(wbar)
+            return gb
+
         _lib.call("ffno_head_bwd", rows, _p(keep[-1]), _p(Wt.p1), _p(Wt.p2), _p(A["p2t"]), _p(A["p1t"]), _p(Wt.p2_bias),
                   _p(x), ctypes.byref(Wt.head), _p(ybar), _p(hbar), _p(xbar), st)
         for i in reversed(range(len(Wt.blocks))):
             h = keep[i]
             blk, adj = Wt.blocks[i], A["blocks"][i]
-            # recompute the block, keeping the branch transforms' outputs for the post networks' adjoints
-            gn, _ = self._block(plan, i, FfnoAssemble(None, _p(h), _p(ones), _p(zeros), None, 0), None, bf,
+            # recompute the block; the branch transforms' outputs go to ys/yv (the post networks' adjoints read them)
+            # and the vertical spectrum after the mix stays in v1y (the height gradient reads it)
+            bfr = dict(bf)
+            bfr["v1"] = bf["v1y"]
+            gn, _ = self._block(plan, i, FfnoAssemble(None, _p(h), _p(ones), _p(zeros), None, 0), None, bfr,
                                 ys=bf["ys"], yv=bf["yv"])
             self._block_tail(plan, i, h, bf, gn)
             # mlp: h' = x2 + res_mlp GN(mp)
-            gb = self._gnback(plan, bf["mp"], gn["m"], blk["norm_mlp"], blk["res_mlp"], 0)
+            gb = gnback(bf["mp"], gn["m"], blk["norm_mlp"], blk["res_mlp"], 0)
             self._gn_sums(plan, gb, hbar, part2)
             _lib.call("ffno_mlp_bwd", rows, _p(hbar), gb, _p(bf["x2"]), _p(blk["m1"]), _p(adj["m2t"]), _p(adj["m1t"]),
                       1, _p(bf["gx2"]), st)
             # pw: x2 = x1 + res_pw GN(pp)
-            gb = self._gnback(plan, bf["pp"], gn["p"], blk["norm_pw"], blk["res_pw"], 0)
+            gb = gnback(bf["pp"], gn["p"], blk["norm_pw"], blk["res_pw"], 0)
             self._gn_sums(plan, gb, bf["gx2"], part2)
             _lib.call("ffno_mlp_bwd", rows, _p(bf["gx2"]), gb, _p(bf["x1"]), _p(blk["pw1"]), _p(adj["pw2t"]),
                       _p(adj["pw1t"]), 1, _p(bf["gx1"]), st)
             # fusion: x1 = h + res_fused gelu(GN(fp))
-            gb = self._gnback(plan, bf["fp"], gn["f"], blk["norm_fuse"], blk["res_fused"], 1)
+            gb = gnback(bf["fp"], gn["f"], blk["norm_fuse"], blk["res_fused"], 1)
             self._gn_sums(plan, gb, bf["gx1"], part2)
             sa, sb_, _ = gn["s"]
             va, vb, _ = gn["v"]
@@ -555,28 +597,78 @@ class Engine:
                       _p(va), _p(vb), _p(blk["bg1"]), _p(blk["bg2"]), _p(blk["bg2_bias"]), _p(blk["f1"]),
                       _p(adj["f2t"]), _p(adj["f1t"]), _p(adj["bg2t"]), _p(adj["bg1t"]), _p(bf["gh"]), _p(bf["gs"]),
                       _p(bf["gv"]), st)
-            # horizontal branch: spec = gelu(GN_s(post(Spec(xs))))
-            gb = self._gnback(plan, bf["sp"], gn["s"], blk["norm_spec"], None, 1)
+            # horizontal branch: spec = gelu(GN_s(post(Spec(xs)))); x_s bar lands in gs
+            gb = gnback(bf["sp"], gn["s"], blk["norm_spec"], None, 1)
             self._gn_sums(plan, gb, bf["gs"], part2)
             _lib.call("ffno_mlp_bwd", rows, _p(bf["gs"]), gb, _p(bf["ys"]), _p(blk["spec.p1"]), _p(adj["spec.p2t"]),
                       _p(adj["spec.p1t"]), 0, _p(bf["gx2"]), st)
-            self.horizontal(plan, bf["gx2"], bf["s1"], bf["s2"], bf["xs"], i, adjoint=True)
-            # vertical branch
-            gb = self._gnback(plan, bf["vp"], gn["v"], blk["norm_vert"], None, 1)
+            self.horizontal(plan, bf["gx2"], bf["s1"], bf["s2"], bf["gs"], i, adjoint=True)
+            # vertical branch; x_v bar lands in gv, the spectrum gradient stays in v1
+            gb = gnback(bf["vp"], gn["v"], blk["norm_vert"], None, 1)
             self._gn_sums(plan, gb, bf["gv"], part2)
             _lib.call("ffno_mlp_bwd", rows, _p(bf["gv"]), gb, _p(bf["yv"]), _p(blk["vertical.p1"]),
                       _p(adj["vertical.p2t"]), _p(adj["vertical.p1t"]), 0, _p(bf["gx1"]), st)
-            self.vertical(plan, bf["gx1"], bf["v1"], bf["xv"], i, adjoint=True)
-            # input gates
-            _lib.call("ffno_gates_bwd", rows, _p(h), _p(bf["xs"]), _p(bf["xv"]), _p(bf["gh"]), _p(blk["spec.g1"]),
+            if want_z and ht["v0"] is not None:
+                # taper: dL/df_m per column = sum_c,part (V0^T yv_bar) * Y, before the mix adjoint overwrites v1
+                self._tw(ht["v0"], bf["gx1"], bf["v1"], B, hwc, (big, 0, D * hwc), (0, 2 * M * hwc), (big, 0, hwc),
+                         (M, M * hwc, hwc))
+                g = (bf["v1"].view(B, 2, M, H * W, C) * bf["v1y"].view(B, 2, M, H * W, C)).sum((1, 4))   # [B, M, HW]
+                dfac_col += (g * ht["dfac"][None, :, None]).sum(1)
+            self.vertical(plan, bf["gx1"], bf["v1"], bf["gv"], i, adjoint=True)
+            if want_z:
+                # Filon weights: zbar_s[e] += sum_delta sum_c xv[e + delta] (G_delta X_bar)[e]
+                for delta in (-1, 0, 1):
+                    self._tw(ht["bands"][delta], bf["v1"], rbuf, B, hwc, (big, 0, 2 * M * hwc), (0, D * hwc),
+                             (M, M * hwc, hwc), (big, 0, hwc))
+                    _lib.call("ffno_dot_reduce", rows, D, _p(bf["xv"]), delta, _p(rbuf), _p(zbar_s), st)
+                # synthesis phases: zbar_s[d] += sum_c yv_bar[d] (V_dz Y)[d]
+                self._tw(ht["v_dz"], bf["v1y"], rbuf, B, hwc, (big, 0, 2 * M * hwc), (0, D * hwc), (M, M * hwc, hwc),
+                         (big, 0, hwc))
+                _lib.call("ffno_dot_reduce", rows, D, _p(bf["gx1"]), 0, _p(rbuf), _p(zbar_s), st)
+            # input gates (and z_proj)
+            _lib.call("ffno_gates_bwd", rows, _p(h), _p(bf["gs"]), _p(bf["gv"]), _p(bf["gh"]), _p(blk["spec.g1"]),
                       _p(blk["spec.g2"]), _p(adj["spec.g2t"]), _p(adj["spec.g1t"]), _p(blk["vertical.g1"]),
                       _p(blk["vertical.g2"]), _p(adj["vertical.g2t"]), _p(adj["vertical.g1t"]), _p(blk["z_proj"]),
-                      _p(hbar), None, st)
+                      _p(hbar), _p(zbar), st)
         # lift: h0 = gelu(GN(W_lift x))
         t0 = torch.empty(N, C, **f32)
         _lib.call("ffno_lift", rows, _p(x), _p(Wt.lift), _p(t0), FfnoStatsOut(_p(plan.depth_w), _p(bf["part"])), st)
         g0 = self._gn(plan, bf["part"], Wt.lift_gn, dev)
-        gb = self._gnback(plan, t0, g0, Wt.lift_gn, None, 1)
+        gb = gnback(t0, g0, Wt.lift_gn, None, 1)
         self._gn_sums(plan, gb, hbar, part2)
         _lib.call("ffno_lift_bwd", rows, _p(hbar), gb, _p(Wt.lift), _p(xbar), st)
-        return xbar, None
+        if not want_z:
+            return xbar, None
+        return xbar, self._height_gradient(plan, z, zbar, zbar_s, wbar, dfac_col)
+
+    @staticmethod
+    def _height_gradient(plan, z, zbar, zbar_s, wbar, dfac_col):
+        """Assemble dL/dz [B, D, H, W] from the per-voxel pieces: zbar (z_proj), zbar_s (the vertical transform,
+        taken with respect to the shifted heights z - min z of each column), wbar (the depth weights, through
+        w = t / mean_d t with t the trapezoid |dz| of models/ffno_model.py:28-34), and the taper term."""
+        B, D, H, W = plan.B, plan.D, plan.H, plan.W
+        zs = zbar_s.view(B, D, H * W)
+        ht = plan.height_tables()
+        if dfac_col is not None:
+            i, sgn = ht["i_med"], ht["sign"]
+            zs[:, i + 1] += sgn * dfac_col
+            zs[:, i] -= sgn * dfac_col
+        # the per-column shift z - amin(z): the minimum's level(s) take minus the column total, shared evenly
+        total = zs.sum(1)
+        out = zbar.view(B, D, H * W) + zs
+        for j in ht["argmin"]:
+            out[:, int(j)] -= total / len(ht["argmin"])
+        # depth weights: w_d = t_d / mean(t); t_d = |dz| halves of the intervals either side of level d
+        zc = z.reshape(B, D, H * W).float()
+        if D > 1:
+            dzv = zc[:, 1:] - zc[:, :-1]
+            t = torch.zeros_like(zc)
+            t[:, :-1] += 0.5 * dzv.abs()
+            t[:, 1:] += 0.5 * dzv.abs()
+            mt = t.mean(1, keepdim=True).clamp_min(1e-12)
+            wb = wbar.view(B, D, H * W)
+            tb = wb / mt - (wb * t).sum(1, keepdim=True) / (D * mt * mt)
+            g = 0.5 * torch.sign(dzv) * (tb[:, :-1] + tb[:, 1:])      # d/d(z_{d+1} - z_d) of both halves
+            out[:, 1:] += g
+            out[:, :-1] -= g
+        return out.view(B, D, H, W)

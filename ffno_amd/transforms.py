@@ -119,3 +119,71 @@ def split_bf16(a, device):
     hi = t.to(torch.bfloat16)
     lo = (t - hi.double()).to(torch.bfloat16)
     return (hi.view(torch.int16).to(device), lo.view(torch.int16).to(device), m, k, mp, kp)
+
+
+def filon_weights_torch(z, k):
+    """filon_weights in torch (float64, differentiable), for the height derivatives of the transform."""
+    z0, z1 = z[:-1, None], z[1:, None]
+    low = torch.minimum(z0, z1)
+    h = (z1 - z0).abs()
+    th = h * k[None, :]
+    small = th.abs() < 1e-2
+    tt = torch.where(small, torch.ones_like(th), th)
+    c, s = torch.cos(tt), torch.sin(tt)
+    up_re = torch.where(small, 0.5 - th ** 2 / 8, (c + tt * s - 1) / tt ** 2)
+    up_im = torch.where(small, -th / 3 + th ** 3 / 30, (tt * c - s) / tt ** 2)
+    lo_re = torch.where(small, 0.5 - th ** 2 / 24, s / tt - up_re)
+    lo_im = torch.where(small, -th / 6 + th ** 3 / 120, (c - 1) / tt - up_im)
+    ph = low * k[None, :]
+    e_re, e_im = h * torch.cos(ph), -h * torch.sin(ph)
+    a_re, a_im = e_re * lo_re - e_im * lo_im, e_re * lo_im + e_im * lo_re
+    b_re, b_im = e_re * up_re - e_im * up_im, e_re * up_im + e_im * up_re
+    asc = z0 <= z1
+    pad_end, pad_start = (0, 0, 0, 1), (0, 0, 1, 0)
+    F = torch.nn.functional
+    q_re = F.pad(torch.where(asc, a_re, b_re), pad_end) + F.pad(torch.where(asc, b_re, a_re), pad_start)
+    q_im = F.pad(torch.where(asc, a_im, b_im), pad_end) + F.pad(torch.where(asc, b_im, a_im), pad_start)
+    return q_re, q_im
+
+
+def filon_height_bands(column, k):
+    """G[delta] [D, 2M] for delta = -1, 0, 1: G[delta][e] = d q[e + delta] / d z_e (real parts, then imaginary), the
+    three non-zero bands of the Filon weights' height Jacobian (q of level d depends on levels d-1, d, d+1), from
+    forward-mode derivatives along three interleaved tangents."""
+    z = torch.tensor(np.asarray(column, np.float64))
+    kk = torch.tensor(np.asarray(k, np.float64))
+    D, M = len(z), len(kk)
+    bands = {d: np.zeros((D, 2 * M)) for d in (-1, 0, 1)}
+    idx = np.arange(D)
+    for color in range(3):
+        v = torch.zeros(D, dtype=torch.float64)
+        v[color::3] = 1.0
+        _, (dre, dim) = torch.func.jvp(lambda zz: filon_weights_torch(zz, kk), (z,), (v,))
+        dre, dim = dre.numpy(), dim.numpy()
+        for delta in (-1, 0, 1):
+            e = idx - delta                        # row d of the derivative belongs to level e = d - delta
+            ok = (e >= 0) & (e < D) & (e % 3 == color)
+            bands[delta][e[ok], :M] = dre[idx[ok]]
+            bands[delta][e[ok], M:] = dim[idx[ok]]
+    return bands
+
+
+def vertical_height_terms(column, k, factor, length):
+    """The height derivatives of the synthesis: V_dz [D, 2M] = d V_inv / d z_d (-f k sin, -f k cos over L), the
+    unscaled synthesis V0 [D, 2M] = (cos, -sin) / L, and per mode factor * d taper / d dz at the column's dz
+    together with the index i of the median spacing (dz = |z_{i+1} - z_i|)."""
+    col = np.asarray(column, np.float64)
+    f = factor * column_taper(col, k)
+    ph = np.outer(col, k)
+    v_dz = np.concatenate([-f * k * np.sin(ph), -f * k * np.cos(ph)], 1) / length
+    v0 = np.concatenate([np.cos(ph), -np.sin(ph)], 1) / length
+    d = np.abs(np.diff(col))
+    order = np.argsort(d, kind="stable")
+    i = int(order[(len(d) - 1) // 2])
+    dz = d[i]
+    k_nyq = math.pi / dz
+    u = (k - 2 * k_nyq / 3) / (k_nyq / 3)
+    inside = (u > 0) & (u < 1)
+    # taper = (1 + cos(pi u)) / 2 with u = (3 k dz / pi) - 2: d taper / d dz = -(pi/2) sin(pi u) * 3 k / pi
+    dtaper = np.where(inside, -0.5 * np.sin(math.pi * u) * 3.0 * k, 0.0)
+    return v_dz, v0, factor * dtaper, i, float(np.sign(col[i + 1] - col[i]))

@@ -163,6 +163,8 @@ typedef struct {
   int act_gelu;
   const float* sums;      /* [B][8][2] (S1, S2) from ffno_gn_grad_finalize; unused by ffno_gn_grad_stats */
   const float* depth_w;   /* [B, D*H*W] */
+  float* wbar;            /* [B, D*H*W] or NULL: accumulates the gradient with respect to the depth weights,
+                             -(1/W) sum_c (S1 x_hat + S2 (x_hat^2 - rstd^2 var) / 2) */
 } FfnoGnBack;
 
 /* S1, S2 partial sums of one GroupNorm's backward: partials [B][nwg][8][2]. */
@@ -198,6 +200,10 @@ int ffno_gates_bwd(FfnoRows rows, const float* h, const float* xsbar, const floa
 int ffno_head_bwd(FfnoRows rows, const float* h, const uint16_t* p1, const uint16_t* p2, const uint16_t* p2t,
                   const uint16_t* p1t, const float* p2_bias, const float* x_in, const FfnoHeadDesc* desc,
                   const float* ybar, float* hbar, float* xbar_in, hipStream_t stream);
+
+/* zbar[b, d, hw] += sum_c a[b, d + delta, hw, c] * r[b, d, hw, c] (terms with d + delta outside [0, D) skipped):
+ * the channel contraction of the height gradient of the vertical branch (models/ffno_model.py:272-308). */
+int ffno_dot_reduce(FfnoRows rows, int D, const float* a, int delta, const float* r, float* zbar, hipStream_t stream);
 
 /* Adjoint of the lift (models/ffno_model.py:782-783): xbar_in += W_lift^T (GN backward of ubar). */
 int ffno_lift_bwd(FfnoRows rows, const float* ubar, FfnoGnBack gn, const float* w_lift, float* xbar_in,

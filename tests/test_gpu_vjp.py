@@ -95,3 +95,77 @@ def test_inversion_loss_gradient_matches_oracle_and_finite_differences():
     print(f"[loss] HIP vs FD at 64 entries: rel norm {nrm_fd:.3e} rel max {mx_fd:.3e}; oracle vs FD "
           f"{_errs(oracle_at, fd)[0]:.3e}", flush=True)
     assert nrm_fd <= TOL and mx_fd <= TOL
+
+
+@pytest.mark.parametrize("name", ["case_a_shared64", "case_c_rank0", "case_e_tall"])
+def test_height_gradient_matches_per_column_autograd(name):
+    """z_bar from the HIP adjoint against the float64 oracle's autograd with the per-column vertical path (every
+    column's transform built from its own heights: the mathematically exact per-column gradient; the oracle's
+    per-column path is pinned to the reference's by golden case A'). Case C has modes inside the taper band, so its
+    height gradient includes the taper term."""
+    from oracle import ffno_oracle as O
+    data, config, state = load(name)
+    model = _model_from(config, state)
+    x = torch.from_numpy(data["x"]).float().to(DEV).requires_grad_(True)
+    z = torch.from_numpy(data["z"]).float().to(DEV).requires_grad_(True)
+    ct = torch.from_numpy(data["cotangent"]).float().to(DEV)
+    y = model(x, z, 48e3, 48e3)
+    xb, zb = torch.autograd.grad(y, (x, z), grad_outputs=ct)
+    torch.cuda.synchronize()
+    _, oxb, ozb = O.vjp(O.to_dtype(state, torch.float64), oracle_config(config), torch.from_numpy(data["x"]).double(),
+                        torch.from_numpy(data["z"]).double(), 48e3, 48e3, torch.from_numpy(data["cotangent"]).double(),
+                        force_per_column=True)
+    nrm, mx = _errs(zb.cpu().numpy(), ozb.numpy())
+    print(f"[{name}] z_bar rel norm {nrm:.3e} rel max {mx:.3e}", flush=True)
+    assert nrm <= TOL and mx <= TOL
+    assert _errs(xb.cpu().numpy(), oxb.numpy())[0] <= TOL
+
+
+def test_backend_vjp_local_matches_si_autograd():
+    """GPUBackend.vjp_local (SI features and heights in metres, linear-population cotangent) against float64
+    autograd through the oracle's SI chain (ffno_fsdp_runtime.py:48-58, :76-78, :265-281) with per-column heights;
+    predict_local against the oracle's populations (1e-3 relative in log10 n)."""
+    from ffno_amd import synthetic
+    from ffno_amd.backend import GPUBackend
+    from ffno_amd.atoms import species
+    from oracle import ffno_oracle as O
+    config = dict(in_channels=6, out_channels=6, width=32, n_layers=2, hidden_spectral=32, hidden_vertical=32,
+                  freq_mix_rank=4, z_reference_length=40.0, z_min_wavelength=1.28, z_feature_wavelength=0.3,
+                  species=species(["H"]))
+    feats, zc = synthetic.atmosphere_features(1, 20, 10, 12, seed=3)
+    mean = feats.mean(axis=(0, 2, 3, 4)).astype(np.float32)
+    std = feats.std(axis=(0, 2, 3, 4)).astype(np.float32)
+    si = feats[0].copy()
+    si[0] = 10.0 ** si[0]                       # T [K]
+    si[1:4] *= 1e3                              # m/s
+    z_m = np.broadcast_to(zc[:, None, None] * 1e6, si.shape[1:]).astype(np.float32).copy()
+    model = synthetic.random_model(4, **config)
+    model.set_normalization(mean, std, np.zeros(6), np.ones(6))
+    state = {k: v.detach().clone() for k, v in model.state_dict().items()}
+    backend = GPUBackend(None, model=model, level_names=[f"H{i}" for i in range(6)])
+    pops = backend.predict_local(si.astype(np.float32), z_m, 48e3, 48e3)
+    sd = O.to_dtype(state, torch.float64)
+    cfg = oracle_config(config)
+    f64 = torch.from_numpy(si.astype(np.float32)).double()[None].requires_grad_(True)
+    z64 = torch.from_numpy(z_m).double()[None].requires_grad_(True)
+    n_ref = O.predict_si(sd, cfg, f64, z64, 48e3, 48e3)
+    ref_log = np.log10(n_ref.detach().numpy()[0].transpose(1, 2, 3, 0))
+    rel = float((np.abs(np.log10(pops) - ref_log) / np.maximum(np.abs(ref_log), 1.0)).max())
+    print(f"[backend] predict rel log10 n {rel:.3e}", flush=True)
+    assert rel <= 1e-3
+    rng = np.random.default_rng(0)
+    cot = (rng.standard_normal(pops.shape) / pops).astype(np.float32)      # O(1) in relative terms
+    out = backend.vjp_local(si.astype(np.float32), z_m, 48e3, 48e3, cot)
+    # float64 reference through the per-column vertical path
+    f2 = torch.from_numpy(si.astype(np.float32)).double()[None].requires_grad_(True)
+    z2 = torch.from_numpy(z_m).double()[None].requires_grad_(True)
+    fm, zm = O.model_inputs_si(f2, z2)
+    mx_, sx_ = sd["mean_X"][None, :, None, None, None], sd["std_X"][None, :, None, None, None]
+    y = O.forward(sd, cfg, (fm - mx_) / sx_, zm, 48e3, 48e3, force_per_column=True)
+    n = torch.pow(10.0, (y * sd["std_Y"][None, :, None, None, None] + sd["mean_Y"][None, :, None, None, None])
+                  .clamp(-37, 37))
+    gf, gz = torch.autograd.grad(n, (f2, z2), grad_outputs=torch.from_numpy(cot).double().permute(3, 0, 1, 2)[None])
+    for key, ref in (("features", gf[0].numpy()), ("z_scale", gz[0].numpy())):
+        nrm, mx = _errs(out[key], ref)
+        print(f"[backend] vjp {key}: rel norm {nrm:.3e} rel max {mx:.3e}", flush=True)
+        assert nrm <= TOL
