@@ -108,10 +108,19 @@ class GPUBackend:
         """[1, L, nz, nx, ny] -> numpy (nz, nx, ny, L) (ffno_fsdp_runtime.py:283-292)."""
         return populations[0].permute(1, 2, 3, 0).float().detach().cpu().numpy()
 
-    def predict_tensor(self, x_si, z_si, dx, dy):
+    def predict_tensor(self, x_si, z_si, dx, dy, plan=None):
+        """Linear populations [B, L, nz, nx, ny] on the device. plan: a plan built beforehand (for graph capture,
+        where nothing may synchronise with the host)."""
         xm, zm = self.model_inputs(x_si, z_si)
         with torch.no_grad():
-            return self.linear(self.log_populations(self.model(xm, zm, float(dx), float(dy))))
+            y = self.model.engine().forward(xm, zm, float(dx), float(dy), plan=plan)
+            return self.linear(self.log_populations(y))
+
+    def plan_for(self, z_si, dx, dy):
+        B, D, H, W = z_si.shape
+        eng = self.model.engine()
+        eng.sync(self.device)
+        return eng.plan(B, D, H, W, z_si / Z_METRES_PER_MODEL_UNIT, float(dx), float(dy))
 
     def predict_local(self, features, z_scale, dx, dy):
         x, z = self.inputs(features, z_scale)

@@ -7,6 +7,7 @@ models/ffno_model.py:800-832 (eval mode, no dropout).
 """
 import ctypes
 import math
+import threading
 
 import numpy as np
 import torch
@@ -183,30 +184,24 @@ class Plan:
     """Everything that depends on the request's shape, heights and spacings but not on its data: transform
     matrices (split-bf16), per-mode gates and low-rank tables for every block, depth weights."""
 
-    def __init__(self, engine, B, D, H, W, z, dx, dy):
+    def __init__(self, engine, B, D, H, W, column, span, dx, dy):
         m = engine.module
         dev = engine.device
         C, r = m.width, m.freq_mix_rank
         self.B, self.D, self.H, self.W, self.C, self.rank = B, D, H, W, C, r
         self.wf = W // 2 + 1
         self.voxels = D * H * W
-        # depth weights (models/ffno_model.py:28-34, :67-74), float32 as the reference computes them
-        zc = z.to(dev, torch.float32).reshape(B, 1, D, H, W)
+        column = np.asarray(column, np.float64)
+        # depth weights (models/ffno_model.py:28-34, :67-74) of the shared column, in float32 as the reference has
+        # them, broadcast to every voxel
+        zc = torch.from_numpy(column).float()
         if D == 1:
-            tw = torch.ones_like(zc)
+            tw = torch.ones(1)
         else:
-            half = 0.5 * (zc[:, :, 1:] - zc[:, :, :-1]).abs()
-            tw = F.pad(half, (0, 0, 0, 0, 0, 1)) + F.pad(half, (0, 0, 0, 0, 1, 0))
-        self.depth_w = (tw / tw.mean(dim=2, keepdim=True).clamp_min(1e-12)).reshape(B, -1).contiguous()
-        self.z = zc.reshape(B, -1).contiguous()
-        # vertical grid: the reference's shared-grid decision (models/ffno_model.py:267-269), made on the host once
-        zs = zc - zc.amin(dim=2, keepdim=True)
-        shared = bool((zs == zs[..., :1, :1]).all()) and bool((zs == zs[:1]).all())
-        if not shared:
-            raise NotImplementedError("per-column vertical grids are not implemented in the HIP engine; "
-                                      "every column must share its heights (after the per-column shift)")
-        column = zs[0, 0, :, 0, 0].double().cpu().numpy()
-        span = float((zc.amax(dim=2) - zc.amin(dim=2)).max())
+            half = 0.5 * (zc[1:] - zc[:-1]).abs()
+            tw = F.pad(half, (0, 1)) + F.pad(half, (1, 0))
+        w = tw / tw.mean().clamp_min(1e-12)
+        self.depth_w = w.to(dev)[None, :, None].expand(B, D, H * W).reshape(B, -1).contiguous()
         self.length, k, factor = T.vertical_box(span, m.z_reference_length, m.z_min_wavelength,
                                                 int(math.floor(m.z_reference_length / m.z_min_wavelength)) + 1)
         self.modes = len(k)
@@ -221,6 +216,7 @@ class Plan:
         self._tw_adj = None
         self._column, self._k, self._factor = column, k, factor
         self._height = None
+        self._lock = threading.Lock()
         # per-mode gates and low-rank tables (models/ffno_model.py:375-392, :443-447, :231-237, :285-296), float32
         sd = engine.weights.sd
         ky = torch.fft.fftfreq(H, d=float(dy), device=dev)
@@ -255,6 +251,10 @@ class Plan:
         """Height derivatives of the vertical transforms for the z-gradient: the three bands of the Filon weights'
         Jacobian, the synthesis' derivative, and for the taper (non-zero only for modes between 2 pi / (3 dz) and
         pi / dz) the transposed unscaled synthesis and the per-mode factor * d taper / d dz."""
+        with self._lock:
+            return self._height_tables()
+
+    def _height_tables(self):
         if self._height is None:
             bands = T.filon_height_bands(self._column, self._k)
             v_dz, v0, dfac, i_med, sgn = T.vertical_height_terms(self._column, self._k, self._factor, self.length)
@@ -270,9 +270,13 @@ class Plan:
     def adjoint_tables(self):
         """Transposed transform matrices and adjoint mix tables: conj(g), and the low-rank roles swapped
         (V' = U^T, U' = V^T), so that the same kernels apply the transpose of every forward stage."""
+        with self._lock:
+            return self._adjoint_tables()
+
+    def _adjoint_tables(self):
         if self._tw_adj is None:
             m = self._mats
-            self._tw_adj = {name + "_T": T.split_bf16(np.ascontiguousarray(a.T), self.device) for name, a in m.items()}
+            tables = {name + "_T": T.split_bf16(np.ascontiguousarray(a.T), self.device) for name, a in m.items()}
             for entry in self.mix:
                 for e in entry.values():
                     g = e["gate"].clone()
@@ -283,6 +287,7 @@ class Plan:
                         e["u_adj"] = pack_mlp_weight(e["v_raw"].transpose(-1, -2))
                     else:
                         e["v_adj"] = e["u_adj"] = None
+            self._tw_adj = tables                      # last: another thread may test it before the loop ends
         return self._tw_adj
 
 
@@ -299,27 +304,52 @@ class Engine:
         self.weights = None
         self._version = None
         self._plans = {}
+        self._lock = threading.RLock()      # the serving workers share one engine
+
+    def footprint_bytes(self, B, D, H, W, vjp=False):
+        """HBM the engine needs for one request of this shape: float32 channel-last tensors of C channels counted
+        in units of B*D*H*W*C*4 bytes (forward: 12 activation buffers and 2 half-spectrum planes; adjoint: the
+        saved block inputs plus 24 recompute and gradient buffers), plus 10% for spectra padding and tables."""
+        C, L = self.module.width, self.module.n_layers
+        unit = B * D * H * W * C * 4
+        units = (L + 1) + 24 if vjp else 14
+        return int(unit * units * 1.1) + (64 << 20)
 
     def _param_version(self):
         return tuple((p.data_ptr(), p._version) for p in self.module.parameters()) + \
             tuple((b.data_ptr(), b._version) for b in self.module.buffers())
 
     def sync(self, device):
-        ver = self._param_version()
-        if self.weights is None or ver != self._version or device != self.device:
-            self.device = device
-            self.weights = PackedWeights(self.module, device)
-            self._version = ver
-            self._plans.clear()
+        with self._lock:
+            ver = self._param_version()
+            if self.weights is None or ver != self._version or device != self.device:
+                self.device = device
+                self.weights = PackedWeights(self.module, device)
+                self.weights.adjoint()
+                self._version = ver
+                self._plans.clear()
 
     def plan(self, B, D, H, W, z, dx, dy):
-        key = (B, D, H, W, float(dx), float(dy), z.data_ptr(), z._version, tuple(z.shape))
-        p = self._plans.get(key)
-        if p is None:
-            if len(self._plans) > 16:
-                self._plans.clear()
-            p = Plan(self, B, D, H, W, z, dx, dy)
-            self._plans[key] = p
+        """The plan for these heights, cached by content: the reference's shared-grid decision
+        (models/ffno_model.py:267-269) and its box span (:252) are read on the host here, once per call, and every
+        later decision uses them (no host synchronisation inside the launch sequence)."""
+        zc = z.reshape(B, D, H * W).float()
+        zs = zc - zc.amin(dim=1, keepdim=True)
+        flags = torch.stack([(zs == zs[:, :, :1]).all() & (zs == zs[:1]).all(),
+                             (zc.amax(dim=1) - zc.amin(dim=1)).max()])
+        shared, span = flags.tolist()
+        if not shared:
+            raise NotImplementedError("per-column vertical grids are not implemented in the HIP engine; "
+                                      "every column must share its heights (after the per-column shift)")
+        column = zs[0, :, 0].double().cpu().numpy()
+        key = (B, D, H, W, float(dx), float(dy), float(span), column.tobytes())
+        with self._lock:
+            p = self._plans.get(key)
+            if p is None:
+                if len(self._plans) > 64:
+                    self._plans.pop(next(iter(self._plans)))
+                p = Plan(self, B, D, H, W, column, span, dx, dy)
+                self._plans[key] = p
         return p
 
     # ------------------------------------------------------------------------------------------------ launches
@@ -427,7 +457,7 @@ class Engine:
         ys = xs if ys is None else ys
         yv = xv if yv is None else yv
         _lib.call("ffno_block_in", rows, prev, _p(xin), _p(blk["spec.g1"]), _p(blk["spec.g2"]),
-                  _p(blk["vertical.g1"]), _p(blk["vertical.g2"]), _p(blk["z_proj"]), _p(plan.z), _p(xs), _p(xv), st)
+                  _p(blk["vertical.g1"]), _p(blk["vertical.g2"]), _p(blk["z_proj"]), _p(bf["z"]), _p(xs), _p(xv), st)
         gn = {}
         self.horizontal(plan, xs, bf["s1"], bf["s2"], ys, i)
         _lib.call("ffno_post", rows, _p(ys), _p(blk["spec.p1"]), _p(blk["spec.p2"]), _p(bf["sp"]), stats, st)
@@ -460,7 +490,7 @@ class Engine:
         ma, mb, _ = gn["m"]
         return FfnoAssemble(_p(bf["x2"]), _p(bf["mp"]), _p(ma), _p(mb), _p(blk["res_mlp"]), 0)
 
-    def forward(self, x, z, dx, dy, keep=None):
+    def forward(self, x, z, dx, dy, keep=None, plan=None):
         """Normalised log10 populations [B, Cout, D, H, W] from x [B, 6, D, H, W] and z_scale [B, D, H, W] (Mm).
         keep: a list that receives every block input and the head input (channel-last) for the adjoint."""
         if not x.is_cuda:
@@ -472,13 +502,15 @@ class Engine:
         if cin != 6:
             raise ValueError("FFNO3D expects 6 input channels")
         z = z.reshape(B, D, H, W)
-        plan = self.plan(B, D, H, W, z, dx, dy)
+        if plan is None:
+            plan = self.plan(B, D, H, W, z, dx, dy)
         C = plan.C
         N = B * plan.voxels
         x = x.contiguous().float()
         st = self._stream()
         f32 = dict(device=dev, dtype=torch.float32)
         bf = self._buffers(plan, dev, extra=("mp",))
+        bf["z"] = z.reshape(B, -1).contiguous().float()
         stats = FfnoStatsOut(_p(plan.depth_w), _p(bf["part"]))
         rows = plan.rows
         t0 = torch.empty(N, C, **f32)
@@ -520,7 +552,7 @@ class Engine:
         gb.depth_w = _p(plan.depth_w)
         return gb
 
-    def vjp(self, x, z, dx, dy, ybar, keep, want_z=False):
+    def vjp(self, x, z, dx, dy, ybar, keep, want_z=False, plan=None):
         """(x_bar [B, 6, D, H, W], z_bar [B, D, H, W] or None) of <ybar, forward(x, z)>: the hand-written adjoint,
         block by block in reverse, each block recomputed from its saved input (the reference's non-reentrant
         checkpointing, models/ffno_model.py:751-780). With want_z the height gradient is computed too, per column:
@@ -532,7 +564,8 @@ class Engine:
         A = Wt.adjoint()
         B, _, D, H, W = x.shape
         z = z.reshape(B, D, H, W)
-        plan = self.plan(B, D, H, W, z, dx, dy)
+        if plan is None:
+            plan = self.plan(B, D, H, W, z, dx, dy)
         plan.adjoint_tables()
         C = plan.C
         N = B * plan.voxels
@@ -546,6 +579,7 @@ class Engine:
         f32 = dict(device=dev, dtype=torch.float32)
         bf = self._buffers(plan, dev, extra=("mp", "ys", "yv", "gx2", "gx1", "gs", "gv", "gh"))
         bf["v1y"] = torch.empty_like(bf["v1"])
+        bf["z"] = z.reshape(B, -1).contiguous().float()
         part2 = torch.empty(B, plan.nwg, GROUPS, 2, **f32)
         ones = torch.ones(B, C, **f32)
         zeros = torch.zeros(B, C, **f32)

@@ -1,0 +1,100 @@
+"""Continuous batching on the GPU: every request's result is the same alone and in the mix (1e-5 relative in log10 n
+for predictions, 1e-5 relative for gradients), batches of different buckets run concurrently, and the captured HIP
+graph of a bucket replays to the eager result every time, with other GPU work between replays."""
+import concurrent.futures as cf
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _backend(seed=7):
+    from ffno_amd import synthetic
+    from ffno_amd.atoms import species
+    from ffno_amd.backend import GPUBackend
+    config = dict(in_channels=6, out_channels=6, width=32, n_layers=2, hidden_spectral=32, hidden_vertical=32,
+                  freq_mix_rank=4, z_reference_length=40.0, z_min_wavelength=1.28, z_feature_wavelength=0.3,
+                  species=species(["H"]))
+    model = synthetic.random_model(seed, **config)
+    mean = np.array([3.8, 0, 0, 0, 16, -7], np.float32)
+    std = np.array([0.4, 3, 3, 3, 1.2, 2.2], np.float32)
+    model.set_normalization(mean, std, np.zeros(6), np.ones(6))
+    return GPUBackend(None, model=model, level_names=[f"H{i}" for i in range(6)])
+
+
+def _request(rng, nz, nx, ny, z0=0.0):
+    from ffno_amd import synthetic
+    feats, zc = synthetic.atmosphere_features(1, nz, nx, ny, seed=int(rng.integers(1 << 30)))
+    si = feats[0].copy()
+    si[0] = 10.0 ** si[0]
+    si[1:4] *= 1e3
+    z = np.broadcast_to((zc + z0)[:, None, None] * 1e6, si.shape[1:]).astype(np.float32).copy()
+    return si.astype(np.float32), z
+
+
+def _rel(a, b):
+    return float((np.abs(np.log10(a) - np.log10(b)) / np.maximum(np.abs(np.log10(b)), 1.0)).max())
+
+
+def test_results_alone_and_in_the_mix():
+    from ffno_amd.serving import ContinuousBatcher
+    backend = _backend()
+    rng = np.random.default_rng(0)
+    shapes = [(16, 8, 8), (16, 8, 8), (16, 8, 8), (24, 16, 12), (24, 16, 12), (12, 32, 32)]
+    reqs = []
+    for i, shp in enumerate(shapes):
+        f, z = _request(rng, *shp, z0=0.5 * (i % 2))          # same grid up to a shift: the same bucket
+        op = "VJP" if i % 3 == 2 else "PREDICT"
+        cot = rng.standard_normal(shp + (6,)).astype(np.float32) * 1e-12 if op == "VJP" else None
+        reqs.append((op, f, z, cot))
+    alone = []
+    for op, f, z, cot in reqs:
+        if op == "PREDICT":
+            alone.append(backend.predict_local(f, z, 48e3, 48e3))
+        else:
+            alone.append(backend.vjp_local(f, z, 48e3, 48e3, cot))
+    batcher = ContinuousBatcher({"H": backend}, slots=3, max_batch=4)
+    try:
+        for rep in range(2):                                   # the second pass replays captured graphs
+            with cf.ThreadPoolExecutor(8) as ex:
+                futs = [ex.submit(lambda a: batcher.submit(a[0], "H", a[1], a[2], 48e3, 48e3, a[3]).result(), r)
+                        for r in reqs]
+                mixed = [fu.result() for fu in futs]
+            for (op, *_), a, m in zip(reqs, alone, mixed):
+                if op == "PREDICT":
+                    err = _rel(m["populations"], a)
+                    print(f"pass {rep} predict rel {err:.2e}", flush=True)
+                    assert err <= 1e-5
+                else:
+                    for k in ("features", "z_scale"):
+                        err = float(np.linalg.norm(m[k] - a[k]) / np.linalg.norm(a[k]))
+                        print(f"pass {rep} vjp {k} rel {err:.2e}", flush=True)
+                        assert err <= 1e-5
+        lat = batcher.latencies_ms()
+        assert len(lat) == 2 * len(reqs)
+        print(f"peak batches in flight {batcher.peak_inflight}; graphs {len(batcher.graphs)}", flush=True)
+    finally:
+        batcher.close()
+
+
+def test_graph_replays_match_eager():
+    """Replays of one captured forward, each after unrelated eager GPU work, against the eager forward."""
+    from ffno_amd.serving import ContinuousBatcher
+    backend = _backend(seed=9)
+    rng = np.random.default_rng(5)
+    f, z = _request(rng, 20, 16, 16)
+    eager = backend.predict_local(f, z, 48e3, 48e3)
+    batcher = ContinuousBatcher({"H": backend}, slots=1, max_batch=1)
+    try:
+        for i in range(4):
+            got = batcher.submit("PREDICT", "H", f, z, 48e3, 48e3).result()["populations"]
+            a = torch.randn(1 << 20, device="cuda")
+            (a * a).sum().item()                               # eager work between replays
+            err = _rel(got, eager)
+            print(f"replay {i} rel {err:.2e}", flush=True)
+            assert err <= 1e-6
+        assert len(batcher.graphs) == 1
+    finally:
+        batcher.close()
