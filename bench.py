@@ -23,6 +23,7 @@ sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
 WORKLOAD = dict(B=8, D=128, H=64, W=64, width=64, n_layers=8, z_min_wavelength=1.28, hidden=128, rank=8)
+VOLUME = dict(B=1, D=256, H=504, W=504)       # the whole-volume case: same network, one 504 x 504 x 256 atmosphere
 CPU_WORKLOAD = dict(B=1, D=64, H=32, W=32, width=32, n_layers=4, z_min_wavelength=3.5, hidden=128, rank=8)
 
 
@@ -57,6 +58,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-baselines", action="store_true", help="skip the plain FP32 GPU and CPU legs")
     ap.add_argument("--kernel", default="k_fuse_bwd", help="kernel whose launches are timed for the roofline")
+    ap.add_argument("--volume", action="store_true", help="also run the whole-volume 504 x 504 x 256 forward + VJP")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported (requests shard whole across GPUs "
@@ -163,7 +165,82 @@ def main():
                                  "ms_per_step": round(pms, 1), "speedup": round(pms / ms, 2),
                                  "rel_err_log10n": rel, "grad_rel_err": grel}
         out["cpu_baseline"] = cpu_baseline()
+    if args.volume:
+        del y, xb
+        out["whole_volume"] = volume_leg(model, dev, plain_too=not args.no_baselines)
     print(json.dumps(out), flush=True)
+
+
+def volume_leg(model, dev, plain_too=True):
+    """One 504 x 504 x 256 atmosphere (65 M voxels) through the same network on one GPU, no slabbing: forward alone,
+    then forward + input VJP (the adjoint chooses its checkpoint stride from the free HBM), peak HBM of each. Parity
+    at this size, where the float64 oracle cannot run: the network is equivariant to circular horizontal shifts
+    (1x1 layers, column-local vertical branch, FFT-periodic horizontal branch, shift-invariant GroupNorm sums), so
+    f(roll(x)) must equal roll(f(x)) up to rounding -- a wrong twiddle or index of the 504-point transforms breaks
+    it; and the plain FP32 GPU forward on the same input, when it fits in memory."""
+    import torch
+    from ffno_amd import synthetic
+    v = VOLUME
+    voxels = v["B"] * v["D"] * v["H"] * v["W"]
+    t0 = time.perf_counter()
+    x, z, _, _ = synthetic.atmosphere(v["B"], v["D"], v["H"], v["W"], seed=0)
+    ct = synthetic.cotangent((v["B"], 12, v["D"], v["H"], v["W"]), 0)
+    x, z, ct = x.to(dev), z.to(dev), ct.to(dev)
+    print(f"volume: inputs ready in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    res = {"workload": "FFNO3D 504x504x256, B=1, width 64, 8 layers, vertical modes 32", "voxels": voxels}
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        torch.cuda.reset_peak_memory_stats(dev)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        r = fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return r, e0.elapsed_time(e1), torch.cuda.max_memory_allocated(dev)
+
+    with torch.no_grad():
+        y, ms, peak = timed(lambda: model(x, z, 48e3, 48e3))
+    res["forward_ms"] = round(ms, 1)
+    res["forward_mvox_s"] = round(voxels / (ms * 1e-3) / 1e6, 3)
+    res["forward_peak_hbm_gb"] = round(peak / 1e9, 2)
+    print(f"volume: forward {ms:.0f} ms, peak {peak / 1e9:.1f} GB", file=sys.stderr, flush=True)
+    sh, sw = 137, 311
+    with torch.no_grad():
+        yr = model(torch.roll(x, (sh, sw), (3, 4)), torch.roll(z, (sh, sw), (2, 3)), 48e3, 48e3)
+        res["shift_equivariance_rel_err"] = float((yr - torch.roll(y, (sh, sw), (3, 4))).abs().max() /
+                                                  y.abs().max().clamp_min(1.0))
+    del yr
+    print(f"volume: shift equivariance {res['shift_equivariance_rel_err']:.2e}", file=sys.stderr, flush=True)
+
+    def fwd_vjp():
+        xg = x.detach().requires_grad_(True)
+        yy = model(xg, z, 48e3, 48e3)
+        (g,) = torch.autograd.grad(yy, xg, grad_outputs=ct)
+        return g
+
+    xb, ms, peak = timed(fwd_vjp)
+    res["fwd_vjp_ms"] = round(ms, 1)
+    res["fwd_vjp_mvox_s"] = round(voxels / (ms * 1e-3) / 1e6, 3)
+    res["fwd_vjp_peak_hbm_gb"] = round(peak / 1e9, 2)
+    res["checkpoint_stride"] = model.engine().last_stride
+    res["xbar_finite"] = bool(torch.isfinite(xb).all())
+    print(f"volume: fwd+vjp {ms:.0f} ms, peak {peak / 1e9:.1f} GB", file=sys.stderr, flush=True)
+    del xb
+    if plain_too:
+        from ffno_amd.reference_torch import PlainFFNO3D
+        plain = PlainFFNO3D(model).to(dev)
+        try:
+            with torch.no_grad():
+                yp, ms, peak = timed(lambda: plain(x, z, 48e3, 48e3))
+            res["plain_fp32_forward_ms"] = round(ms, 1)
+            res["plain_fp32_forward_peak_hbm_gb"] = round(peak / 1e9, 2)
+            res["rel_err_log10n_vs_plain"] = float(((y - yp).abs() / yp.abs().clamp_min(1.0)).max())
+            del yp
+        except torch.OutOfMemoryError:
+            res["plain_fp32_forward_ms"] = "out of memory"
+        print(f"volume: plain {res['plain_fp32_forward_ms']}", file=sys.stderr, flush=True)
+    return res
 
 
 def cpu_baseline():

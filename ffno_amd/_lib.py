@@ -67,7 +67,7 @@ SIGNATURES = {
     "ffno_mix": [ctypes.POINTER(FfnoMix), c_ptr],
     "ffno_gn_grad_stats": [FfnoRows, FfnoGnBack, c_ptr, c_ptr, c_ptr],
     "ffno_gn_grad_finalize": [ctypes.c_int, ctypes.c_int, c_ptr, c_ptr, c_ptr],
-    "ffno_mlp_bwd": [FfnoRows, c_ptr, FfnoGnBack, c_ptr, c_ptr, c_ptr, c_ptr, ctypes.c_int, c_ptr, c_ptr],
+    "ffno_mlp_bwd": [FfnoRows, c_ptr, FfnoGnBack, FfnoAssemble, c_ptr, c_ptr, c_ptr, ctypes.c_int, c_ptr, c_ptr],
     "ffno_fuse_bwd": [FfnoRows, c_ptr, FfnoGnBack] + [c_ptr] * 18 + [c_ptr],
     "ffno_gates_bwd": [FfnoRows] + [c_ptr] * 15 + [c_ptr],
     "ffno_head_bwd": [FfnoRows] + [c_ptr] * 7 + [ctypes.POINTER(FfnoHeadDesc)] + [c_ptr] * 3 + [c_ptr],

@@ -11,20 +11,20 @@ class FFNOFunction(torch.autograd.Function):
     def forward(ctx, x, z, module, dx, dy):
         engine = module.engine()
         need = ctx.needs_input_grad[0] or ctx.needs_input_grad[1]
-        keep = [] if need else None
-        y = engine.forward(x.detach(), z.detach(), dx, dy, keep=keep)
-        if need:
-            ctx.save_for_backward(x, z)
-            ctx.keep = keep
-            ctx.module, ctx.dx, ctx.dy = module, dx, dy
+        if not need:
+            return engine.forward(x.detach(), z.detach(), dx, dy)
+        y, saved = engine.forward_saved(x.detach(), z.detach(), dx, dy)
+        ctx.save_for_backward(x, z)
+        ctx.saved = saved
+        ctx.module, ctx.dx, ctx.dy = module, dx, dy
         return y
 
     @staticmethod
     def backward(ctx, ybar):
         x, z = ctx.saved_tensors
         engine = ctx.module.engine()
-        xbar, zbar = engine.vjp(x, z, ctx.dx, ctx.dy, ybar.contiguous(), ctx.keep, want_z=ctx.needs_input_grad[1])
-        ctx.keep = None
+        xbar, zbar = engine.vjp(x, z, ctx.dx, ctx.dy, ybar.contiguous(), ctx.saved, want_z=ctx.needs_input_grad[1])
+        ctx.saved = None
         return xbar if ctx.needs_input_grad[0] else None, zbar, None, None, None
 
 

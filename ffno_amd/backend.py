@@ -137,13 +137,14 @@ class GPUBackend:
         10^, the output normalisation, the input normalisation and the log10 T / km/s / Mm conversions."""
         xm, zm = self.model_inputs(x_si, z_si)
         engine = self.model.engine()
-        keep = []
-        y = engine.forward(xm, zm, float(dx), float(dy), keep=keep)
+        y, saved = engine.forward_saved(xm, zm, float(dx), float(dy))
         log_n = self.log_populations(y)
         n = self.linear(log_n)
         inside = (log_n >= -LOG10_POPULATION_LIMIT) & (log_n <= LOG10_POPULATION_LIMIT)
         ybar = cotangent * n * LN10 * inside * self.std_y
-        xbar, zbar = engine.vjp(xm, zm, float(dx), float(dy), ybar, keep, want_z=True)
+        del y, log_n, inside
+        xbar, zbar = engine.vjp(xm, zm, float(dx), float(dy), ybar, saved, want_z=True)
+        del saved
         fbar = xbar / self.std_x
         feature_bar = torch.cat((fbar[:, 0:1] / (x_si[:, 0:1] * LN10), fbar[:, 1:4] / VELOCITY_M_S_PER_MODEL_UNIT,
                                  fbar[:, 4:6]), dim=1)

@@ -136,10 +136,9 @@ __global__ void k_gn_grad_finalize(int nwg, const float* __restrict__ partials, 
 // MLP behind a GroupNorm, with or without the residual path
 // ---------------------------------------------------------------------------------------------------------------
 template <int CT, int NW>
-__global__ __launch_bounds__(NW * 64) void k_mlp_bwd(FfnoRows rows, const float* __restrict__ ubar, FfnoGnBack gb,
-                                                     const float* __restrict__ xin, const uint16_t* w1,
-                                                     const uint16_t* w2t, const uint16_t* w1t, int residual,
-                                                     float* __restrict__ xbar) {
+__global__ __launch_bounds__(NW * 64) void k_mlp_bwd(FfnoRows rows, const float* ubar, FfnoGnBack gb, FfnoAssemble xin,
+                                                     const uint16_t* w1, const uint16_t* w2t, const uint16_t* w1t,
+                                                     int residual, float* xbar) {
   constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32;
   extern __shared__ __align__(16) unsigned char smem[];
   bf16* lds = reinterpret_cast<bf16*>(smem);
@@ -148,6 +147,7 @@ __global__ __launch_bounds__(NW * 64) void k_mlp_bwd(FfnoRows rows, const float*
   Lane L;
   const int b = blockIdx.y;
   const float scale = gb.scale ? *gb.scale : 1.f;
+  const float xscale = xin.scale ? *xin.scale : 1.f;
   const int64_t nt = (rows.voxels + 15) / 16;
   for (int64_t tile = (int64_t)blockIdx.x * NW + L.wave; tile < nt; tile += (int64_t)gridDim.x * NW) {
     const int64_t vox = tile * 16 + L.v;
@@ -161,7 +161,7 @@ __global__ __launch_bounds__(NW * 64) void k_mlp_bwd(FfnoRows rows, const float*
       gn_back<CT>(gb, scale, row, b, ok, L.q, ub, tb);
       to_frags<CT>(tb, of);
       f32x4 x[CT];
-      load_act<CT>(xin, row, ok, L.q, x);
+      assemble<CT>(xin, xscale, row, b, ok, L.q, x);
       to_frags<CT>(x, xf);
     }
 #pragma unroll
@@ -175,7 +175,7 @@ __global__ __launch_bounds__(NW * 64) void k_mlp_bwd(FfnoRows rows, const float*
 // branch fusion adjoint (weights read from global memory: seven matrices do not fit LDS together)
 // ---------------------------------------------------------------------------------------------------------------
 template <int CT, int NW>
-__global__ __launch_bounds__(NW * 64) void k_fuse_bwd(FfnoRows rows, const float* __restrict__ x1bar, FfnoGnBack gf,
+__global__ __launch_bounds__(NW * 64) void k_fuse_bwd(FfnoRows rows, const float* x1bar, FfnoGnBack gf,
                                                       const float* __restrict__ hin, const float* __restrict__ sp,
                                                       const float* __restrict__ vp, const float* __restrict__ s_a,
                                                       const float* __restrict__ s_b, const float* __restrict__ v_a,
@@ -183,7 +183,7 @@ __global__ __launch_bounds__(NW * 64) void k_fuse_bwd(FfnoRows rows, const float
                                                       const uint16_t* bg2, const float* __restrict__ bg2_bias,
                                                       const uint16_t* f1, const uint16_t* f2t, const uint16_t* f1t,
                                                       const uint16_t* bg2t, const uint16_t* bg1t,
-                                                      float* __restrict__ hbar, float* __restrict__ sbar,
+                                                      float* hbar, float* __restrict__ sbar,
                                                       float* __restrict__ vbar) {
   constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32;
   const Wm B1 = global_w(bg1, C * 3 * KP), B2 = global_w(bg2, 2 * C * KP);
@@ -310,11 +310,11 @@ __device__ __forceinline__ void gate_bwd(Wm A1, Wm A2, Wm A2T, Wm A1T, const Fra
 template <int CT, int NW, bool LW>
 __global__ __launch_bounds__(NW * 64) void k_gates_bwd(FfnoRows rows, const float* __restrict__ hin,
                                                        const float* __restrict__ xsbar, const float* __restrict__ xvbar,
-                                                       const float* __restrict__ hbar_in, const uint16_t* g1,
+                                                       const float* hbar_in, const uint16_t* g1,
                                                        const uint16_t* g2, const uint16_t* g2t, const uint16_t* g1t,
                                                        const uint16_t* v1, const uint16_t* v2, const uint16_t* v2t,
                                                        const uint16_t* v1t, const float* __restrict__ z_proj,
-                                                       float* __restrict__ hbar_out, float* __restrict__ zbar) {
+                                                       float* hbar_out, float* __restrict__ zbar) {
   constexpr int C = CT * 16, KS = ks_of(CT), SZ = C * KS * 32;
   extern __shared__ __align__(16) unsigned char smem[];
   bf16* lds = reinterpret_cast<bf16*>(smem);
@@ -358,11 +358,11 @@ __global__ __launch_bounds__(NW * 64) void k_gates_bwd(FfnoRows rows, const floa
 constexpr float LN10 = 2.302585092994046f;
 
 template <int CT, int NW, int NTH>
-__global__ __launch_bounds__(NW * 64) void k_head_bwd(FfnoRows rows, const float* __restrict__ hin, const uint16_t* p1,
+__global__ __launch_bounds__(NW * 64) void k_head_bwd(FfnoRows rows, const float* hin, const uint16_t* p1,
                                                       const uint16_t* p2, const uint16_t* p2t, const uint16_t* p1t,
                                                       const float* __restrict__ p2_bias, const float* __restrict__ x_in,
                                                       FfnoHeadDesc hd, const float* __restrict__ ybar,
-                                                      float* __restrict__ hbar, float* __restrict__ xbar_in) {
+                                                      float* hbar, float* __restrict__ xbar_in) {
   constexpr int C = CT * 16, KS1 = ks_of(CT);
   extern __shared__ __align__(16) unsigned char smem[];
   bf16* lds = reinterpret_cast<bf16*>(smem);
@@ -556,9 +556,10 @@ int ffno_gn_grad_finalize(int B, int nwg, const float* partials, float* sums, hi
   return check_launch("ffno_gn_grad_finalize");
 }
 
-int ffno_mlp_bwd(FfnoRows rows, const float* ubar, FfnoGnBack gn, const float* x, const uint16_t* w1,
+int ffno_mlp_bwd(FfnoRows rows, const float* ubar, FfnoGnBack gn, FfnoAssemble x, const uint16_t* w1,
                  const uint16_t* w2t, const uint16_t* w1t, int residual, float* xbar, hipStream_t stream) {
-  if (!rows_ok(rows) || !gb_ok(gn) || !gn.sums || !ubar || !x || !w1 || !w2t || !w1t || !xbar)
+  if (!rows_ok(rows) || !gb_ok(gn) || !gn.sums || !ubar || (!x.base && !x.t) || (x.t && (!x.aff_a || !x.aff_b)) ||
+      !w1 || !w2t || !w1t || !xbar)
     return fail("ffno_mlp_bwd: bad arguments");
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {

@@ -290,13 +290,16 @@ __device__ __forceinline__ void assemble(const FfnoAssemble& in, float scale, in
 #pragma unroll
   for (int t = 0; t < CT; ++t) {
     const int c0 = 16 * t + 4 * q;
-    f32x4 tv = ld4(in.t + row * C + c0, ok);
-    f32x4 a = *(const f32x4*)(in.aff_a + b * C + c0);
-    f32x4 s = *(const f32x4*)(in.aff_b + b * C + c0);
-    f32x4 u = tv * a + s;
-    if (in.act_gelu) u = gelu4(u);
     f32x4 base = in.base ? ld4(in.base + row * C + c0, ok) : zero4();
-    x[t] = ok ? base + scale * u : zero4();
+    if (in.t) {
+      f32x4 tv = ld4(in.t + row * C + c0, ok);
+      f32x4 a = *(const f32x4*)(in.aff_a + b * C + c0);
+      f32x4 s = *(const f32x4*)(in.aff_b + b * C + c0);
+      f32x4 u = tv * a + s;
+      if (in.act_gelu) u = gelu4(u);
+      base += scale * u;
+    }
+    x[t] = ok ? base : zero4();
   }
 }
 

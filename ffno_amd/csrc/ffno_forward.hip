@@ -104,11 +104,10 @@ __global__ void k_gn_finalize(int nwg, int C, const float* __restrict__ partials
 // block entry: residual assembly + both input gates
 // ---------------------------------------------------------------------------------------------------------------
 template <int CT, int NW>
-__global__ __launch_bounds__(NW * 64) void k_block_in(FfnoRows rows, FfnoAssemble in, float* __restrict__ x_out,
+__global__ __launch_bounds__(NW * 64) void k_block_in(FfnoRows rows, FfnoAssemble in, float* x_out,
                                                       const uint16_t* g1, const uint16_t* g2, const uint16_t* v1,
                                                       const uint16_t* v2, const float* __restrict__ z_proj,
-                                                      const float* __restrict__ z, float* __restrict__ xs,
-                                                      float* __restrict__ xv) {
+                                                      const float* __restrict__ z, float* xs, float* xv) {
   constexpr int C = CT * 16, KS = ks_of(CT), SZ = C * KS * 32;
   extern __shared__ __align__(16) unsigned char smem[];
   bf16* lds = reinterpret_cast<bf16*>(smem);
@@ -150,8 +149,8 @@ __global__ __launch_bounds__(NW * 64) void k_block_in(FfnoRows rows, FfnoAssembl
 // post MLP C -> 2C -> C with statistics
 // ---------------------------------------------------------------------------------------------------------------
 template <int CT, int NW>
-__global__ __launch_bounds__(NW * 64) void k_post(FfnoRows rows, const float* __restrict__ u, const uint16_t* w1,
-                                                  const uint16_t* w2, float* __restrict__ y, FfnoStatsOut st) {
+__global__ __launch_bounds__(NW * 64) void k_post(FfnoRows rows, const float* u, const uint16_t* w1,
+                                                  const uint16_t* w2, float* y, FfnoStatsOut st) {
   constexpr int C = CT * 16, KS1 = ks_of(CT);
   extern __shared__ __align__(16) unsigned char smem[];
   bf16* lds = reinterpret_cast<bf16*>(smem);
@@ -185,12 +184,12 @@ __global__ __launch_bounds__(NW * 64) void k_post(FfnoRows rows, const float* __
 // ---------------------------------------------------------------------------------------------------------------
 template <int CT, int NW, bool LW>
 __global__ __launch_bounds__(NW * 64) void k_fuse(FfnoRows rows, const float* __restrict__ xin,
-                                                  const float* __restrict__ sp, const float* __restrict__ vp,
+                                                  const float* sp, const float* __restrict__ vp,
                                                   const float* __restrict__ s_a, const float* __restrict__ s_b,
                                                   const float* __restrict__ v_a, const float* __restrict__ v_b,
                                                   const uint16_t* bg1, const uint16_t* bg2,
                                                   const float* __restrict__ bg2_bias, const uint16_t* f1,
-                                                  const uint16_t* f2, float* __restrict__ fp, FfnoStatsOut st) {
+                                                  const uint16_t* f2, float* fp, FfnoStatsOut st) {
   constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32;
   constexpr int N1 = C * 3 * KP, N2 = 2 * C * KP, N3 = 2 * C * 2 * KP, N4 = C * 2 * C;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -261,8 +260,8 @@ __global__ __launch_bounds__(NW * 64) void k_fuse(FfnoRows rows, const float* __
 // residual pointwise path
 // ---------------------------------------------------------------------------------------------------------------
 template <int CT, int NW>
-__global__ __launch_bounds__(NW * 64) void k_resmlp(FfnoRows rows, FfnoAssemble in, float* __restrict__ x_out,
-                                                    const uint16_t* w1, const uint16_t* w2, float* __restrict__ y,
+__global__ __launch_bounds__(NW * 64) void k_resmlp(FfnoRows rows, FfnoAssemble in, float* x_out,
+                                                    const uint16_t* w1, const uint16_t* w2, float* y,
                                                     FfnoStatsOut st) {
   constexpr int C = CT * 16, KS1 = ks_of(CT);
   extern __shared__ __align__(16) unsigned char smem[];
@@ -300,7 +299,7 @@ __global__ __launch_bounds__(NW * 64) void k_resmlp(FfnoRows rows, FfnoAssemble 
 constexpr float LN10 = 2.302585092994046f;
 
 template <int CT, int NW, int NTH>
-__global__ __launch_bounds__(NW * 64) void k_head(FfnoRows rows, FfnoAssemble in, float* __restrict__ x_out,
+__global__ __launch_bounds__(NW * 64) void k_head(FfnoRows rows, FfnoAssemble in, float* x_out,
                                                   const uint16_t* p1,
                                                   const uint16_t* p2, const float* __restrict__ p2_bias,
                                                   const float* __restrict__ x_in, FfnoHeadDesc hd,
@@ -325,6 +324,7 @@ __global__ __launch_bounds__(NW * 64) void k_head(FfnoRows rows, FfnoAssemble in
     f32x4 x[CT];
     assemble<CT>(in, scale, row, b, ok, L.q, x);
     if (x_out) store_act<CT>(x_out, row, ok, L.q, x);
+    if (!y) continue;                            // only the head input wanted (uniform: no barrier in this loop)
     Frag xf[KS1];
     to_frags<CT>(x, xf);
     f32x4 o[NTH];
@@ -607,7 +607,8 @@ int ffno_resmlp(FfnoRows rows, FfnoAssemble in, float* x_out, const uint16_t* w1
 
 int ffno_head(FfnoRows rows, FfnoAssemble in, float* x_out, const uint16_t* p1, const uint16_t* p2,
               const float* p2_bias, const float* x_in, const FfnoHeadDesc* desc, float* y, hipStream_t stream) {
-  if (!rows_ok(rows) || !in.t || !p1 || !p2 || !p2_bias || !x_in || !desc || !y) return fail("ffno_head: bad arguments");
+  if (!rows_ok(rows) || !in.t || !p1 || !p2 || !p2_bias || !x_in || !desc || (!y && !x_out))
+    return fail("ffno_head: bad arguments");
   int n_head = 0, n_out = 0;
   if (desc->n_species < 1 || desc->n_species > 4) return fail("ffno_head: 1 to 4 species");
   for (int s = 0; s < desc->n_species; ++s) {

@@ -20,6 +20,9 @@ from ._lib import FfnoAssemble, FfnoHeadDesc, FfnoMix, FfnoRows, FfnoStatsOut, F
 SUPPORTED_WIDTHS = (16, 32, 64, 96, 128)
 GROUPS = 8
 NW = 4                       # waves per workgroup of the voxel-row kernels
+TABLE_CACHE_BYTES = 1 << 30  # per-mode tables of all blocks at most this large stay cached in the plan
+H_CHUNK_BYTES = 2 << 30      # horizontal half-spectrum scratch per buffer: (b, d) planes are transformed in chunks
+WORKING_UNITS = 9            # activation-sized buffers of the adjoint besides the saved block inputs
 
 # MULTI3D constants in cgs (models/ffno_model.py:111-113)
 _EE, _HH, _EM, _BK, _PI = 1.602189e-12, 6.626176e-27, 9.109534e-28, 1.380662e-16, 3.14159265359
@@ -182,13 +185,15 @@ class PackedWeights:
 
 class Plan:
     """Everything that depends on the request's shape, heights and spacings but not on its data: transform
-    matrices (split-bf16), per-mode gates and low-rank tables for every block, depth weights."""
+    matrices (split-bf16), per-mode gates and low-rank tables of every block (cached when small, else rebuilt per
+    block on use), depth weights, the chunking of the horizontal transforms."""
 
     def __init__(self, engine, B, D, H, W, column, span, dx, dy):
         m = engine.module
         dev = engine.device
         C, r = m.width, m.freq_mix_rank
         self.B, self.D, self.H, self.W, self.C, self.rank = B, D, H, W, C, r
+        self.n_layers = m.n_layers
         self.wf = W // 2 + 1
         self.voxels = D * H * W
         column = np.asarray(column, np.float64)
@@ -217,8 +222,11 @@ class Plan:
         self._column, self._k, self._factor = column, k, factor
         self._height = None
         self._lock = threading.Lock()
-        # per-mode gates and low-rank tables (models/ffno_model.py:375-392, :443-447, :231-237, :285-296), float32
-        sd = engine.weights.sd
+        # horizontal transforms run over chunks of (b, d) planes so that their half-spectrum scratch stays bounded
+        self.plane = 2 * H * self.wf * C
+        self.h_chunk = max(1, min(B * D, H_CHUNK_BYTES // (4 * self.plane)))
+        # per-mode k-features (models/ffno_model.py:375-392, :443-447, :231-237), float32
+        self._sd = engine.weights.sd
         ky = torch.fft.fftfreq(H, d=float(dy), device=dev)
         kx = torch.fft.rfftfreq(W, d=float(dx), device=dev)
         ky, kx = torch.meshgrid(ky, kx, indexing="ij")
@@ -227,72 +235,106 @@ class Plan:
         kt = torch.from_numpy(k).float().to(dev)
         qf = kt * m.z_feature_wavelength / (2 * math.pi)
         tf = qf / (1 + qf)
-        vfeat = torch.stack([tf, tf * tf], -1)
-        self.mix = []
-        for i in range(m.n_layers):
-            entry = {}
-            for br, feats in (("spec", hfeat), ("vertical", vfeat)):
-                prefix = f"blocks.{i}.{br}."
-                gate = _freq_mlp(prefix, sd, feats).float().contiguous()
-                e = dict(gate=gate, v=None, u=None, u_raw=None, v_raw=None)
-                if r:
-                    u, v = _freq_lowrank(prefix, sd, feats, C, r)
-                    e["v"] = pack_mlp_weight(v, n_pad=16)                      # [modes, 2, 16, KS*32]
-                    e["u"] = pack_mlp_weight(u)                                # [modes, 2, C, 32]
-                    e["u_raw"], e["v_raw"] = u, v
-                entry[br] = e
-            self.mix.append(entry)
+        self._feats = dict(spec=hfeat, vertical=torch.stack([tf, tf * tf], -1))
+        first = self._build_tables(0)
+        self.table_bytes = sum(t.numel() * t.element_size() for e in first.values() for t in e.values()
+                               if t is not None) * m.n_layers
+        self._cache_tables = self.table_bytes <= TABLE_CACHE_BYTES
+        self._tables = {0: first} if self._cache_tables else {}
         nt = (self.voxels + 15) // 16
         self.nwg = max(1, min((nt + NW - 1) // NW, max(1, 2048 // B)))
         self.rows = FfnoRows(B, self.voxels, C, self.nwg)
         self.device = dev
+
+    def _build_tables(self, i):
+        """Per-mode tables of block i for both branches (models/ffno_model.py:394-429, :285-296): the gate g(k), and
+        for the rank-r term V(k) and U(k) packed for the mix kernel; the adjoint's versions beside them: conj(g) and
+        the low-rank roles swapped (V' = U^T, U' = V^T)."""
+        C, r = self.C, self.rank
+        out = {}
+        for br in ("spec", "vertical"):
+            prefix = f"blocks.{i}.{br}."
+            feats = self._feats[br]
+            gate = _freq_mlp(prefix, self._sd, feats).float().contiguous()
+            g_adj = gate.clone()
+            g_adj[:, 1] = -g_adj[:, 1]
+            e = dict(gate=gate, gate_adj=g_adj.contiguous(), v=None, u=None, v_adj=None, u_adj=None)
+            if r:
+                u, v = _freq_lowrank(prefix, self._sd, feats, C, r)
+                e["v"] = pack_mlp_weight(v, n_pad=16)                      # [modes, 2, 16, KS*32]
+                e["u"] = pack_mlp_weight(u)                                # [modes, 2, C, 32]
+                e["v_adj"] = pack_mlp_weight(u.transpose(-1, -2), n_pad=16)
+                e["u_adj"] = pack_mlp_weight(v.transpose(-1, -2))
+            out[br] = e
+        return out
+
+    def tables(self, i):
+        with self._lock:
+            t = self._tables.get(i)
+            if t is None:
+                t = self._build_tables(i)
+                if self._cache_tables:
+                    self._tables[i] = t
+        return t
 
     def height_tables(self):
         """Height derivatives of the vertical transforms for the z-gradient: the three bands of the Filon weights'
         Jacobian, the synthesis' derivative, and for the taper (non-zero only for modes between 2 pi / (3 dz) and
         pi / dz) the transposed unscaled synthesis and the per-mode factor * d taper / d dz."""
         with self._lock:
-            return self._height_tables()
-
-    def _height_tables(self):
-        if self._height is None:
-            bands = T.filon_height_bands(self._column, self._k)
-            v_dz, v0, dfac, i_med, sgn = T.vertical_height_terms(self._column, self._k, self._factor, self.length)
-            col = np.asarray(self._column)
-            ties = np.flatnonzero(col == col.min())
-            self._height = dict(
-                bands={d: T.split_bf16(bands[d], self.device) for d in (-1, 0, 1)},
-                v_dz=T.split_bf16(v_dz, self.device),
-                v0=T.split_bf16(np.ascontiguousarray(v0.T), self.device) if np.any(dfac != 0) else None,
-                dfac=torch.from_numpy(dfac).float().to(self.device), i_med=i_med, sign=sgn, argmin=ties)
-        return self._height
+            if self._height is None:
+                bands = T.filon_height_bands(self._column, self._k)
+                v_dz, v0, dfac, i_med, sgn = T.vertical_height_terms(self._column, self._k, self._factor, self.length)
+                col = np.asarray(self._column)
+                ties = np.flatnonzero(col == col.min())
+                self._height = dict(
+                    bands={d: T.split_bf16(bands[d], self.device) for d in (-1, 0, 1)},
+                    v_dz=T.split_bf16(v_dz, self.device),
+                    v0=T.split_bf16(np.ascontiguousarray(v0.T), self.device) if np.any(dfac != 0) else None,
+                    dfac=torch.from_numpy(dfac).float().to(self.device), i_med=i_med, sign=sgn, argmin=ties)
+            return self._height
 
     def adjoint_tables(self):
-        """Transposed transform matrices and adjoint mix tables: conj(g), and the low-rank roles swapped
-        (V' = U^T, U' = V^T), so that the same kernels apply the transpose of every forward stage."""
+        """Transposed transform matrices, so that the same GEMM kernel applies the transpose of every forward
+        transform."""
         with self._lock:
-            return self._adjoint_tables()
+            if self._tw_adj is None:
+                self._tw_adj = {name + "_T": T.split_bf16(np.ascontiguousarray(a.T), self.device)
+                                for name, a in self._mats.items()}
+            return self._tw_adj
 
-    def _adjoint_tables(self):
-        if self._tw_adj is None:
-            m = self._mats
-            tables = {name + "_T": T.split_bf16(np.ascontiguousarray(a.T), self.device) for name, a in m.items()}
-            for entry in self.mix:
-                for e in entry.values():
-                    g = e["gate"].clone()
-                    g[:, 1] = -g[:, 1]
-                    e["gate_adj"] = g.contiguous()
-                    if e["u_raw"] is not None:
-                        e["v_adj"] = pack_mlp_weight(e["u_raw"].transpose(-1, -2), n_pad=16)
-                        e["u_adj"] = pack_mlp_weight(e["v_raw"].transpose(-1, -2))
-                    else:
-                        e["v_adj"] = e["u_adj"] = None
-            self._tw_adj = tables                      # last: another thread may test it before the loop ends
-        return self._tw_adj
+    # ------------------------------------------------------------------------------------------------ memory
+    def unit_bytes(self):
+        """One float32 activation tensor [B, D, H, W, C]."""
+        return self.B * self.voxels * self.C * 4
+
+    def scratch_bytes(self):
+        """Half-spectrum chunks, both vertical spectra, GroupNorm partials, inputs and outputs."""
+        v1 = self.B * 2 * self.modes * self.H * self.W * self.C * 4
+        return 2 * self.h_chunk * self.plane * 4 + 2 * v1 + self.B * self.voxels * 4 * 32
+
+    def vjp_bytes(self, stride):
+        saved = -(-self.n_layers // stride) + (1 if stride == 1 else 0) + (stride - 1)
+        tables = 0 if self._cache_tables else 2 * self.table_bytes // self.n_layers
+        return self.unit_bytes() * (saved + WORKING_UNITS) + self.scratch_bytes() + tables
+
+
+class Saved:
+    """What the forward keeps for the adjoint: the inputs of blocks 0, s, 2s, ... (s = stride) and, with s = 1, the
+    head's input. The adjoint recomputes the inputs of the other blocks from the nearest kept one."""
+
+    def __init__(self, stride=1):
+        self.stride = int(stride)
+        self.inputs = {}
+        self.head = None
+
+
+def _asm(base, t=None, a=None, b=None, scale=None, act=0):
+    return FfnoAssemble(_p(base), _p(t), _p(a), _p(b), _p(scale), act)
 
 
 class Engine:
-    """Runs FFNO3D forward on the GPU for one module (ffno_amd.model.FFNO3D)."""
+    """Runs FFNO3D forward and input-VJP on the GPU for one module (ffno_amd.model.FFNO3D)."""
 
     def __init__(self, module):
         if module.width not in SUPPORTED_WIDTHS:
@@ -305,15 +347,21 @@ class Engine:
         self._version = None
         self._plans = {}
         self._lock = threading.RLock()      # the serving workers share one engine
+        self.checkpoint_stride = None       # None: the smallest stride whose adjoint fits the free HBM
+        self.last_stride = None
 
     def footprint_bytes(self, B, D, H, W, vjp=False):
-        """HBM the engine needs for one request of this shape: float32 channel-last tensors of C channels counted
-        in units of B*D*H*W*C*4 bytes (forward: 12 activation buffers and 2 half-spectrum planes; adjoint: the
-        saved block inputs plus 24 recompute and gradient buffers), plus 10% for spectra padding and tables."""
-        C, L = self.module.width, self.module.n_layers
+        """HBM one request of this shape needs (for admission; the tables of a cached plan not included):
+        forward 3 activation-sized buffers, adjoint (L+1) saved inputs and 9 working buffers, plus the scratch."""
+        C, L, r = self.module.width, self.module.n_layers, self.module.freq_mix_rank
         unit = B * D * H * W * C * 4
-        units = (L + 1) + 24 if vjp else 14
-        return int(unit * units * 1.1) + (64 << 20)
+        wf = W // 2 + 1
+        plane = 2 * H * wf * C * 4
+        chunk = max(1, min(B * D, H_CHUNK_BYTES // plane)) * plane
+        modes = int(math.floor(self.module.z_reference_length / self.module.z_min_wavelength)) + 1
+        v1 = B * 2 * modes * H * W * C * 4
+        units = (L + 1) + WORKING_UNITS if vjp else 3
+        return unit * units + 2 * chunk + (2 if vjp else 1) * v1 + B * D * H * W * 4 * 32 + (64 << 20)
 
     def _param_version(self):
         return tuple((p.data_ptr(), p._version) for p in self.module.parameters()) + \
@@ -351,6 +399,19 @@ class Engine:
                 p = Plan(self, B, D, H, W, column, span, dx, dy)
                 self._plans[key] = p
         return p
+
+    def choose_stride(self, plan):
+        """Checkpoint stride of the adjoint: the configured one, else the smallest whose footprint fits the HBM that
+        is free now (the caching allocator's unused reserve included)."""
+        if self.checkpoint_stride:
+            return int(self.checkpoint_stride)
+        free, _ = torch.cuda.mem_get_info(plan.device)
+        free += torch.cuda.memory_reserved(plan.device) - torch.cuda.memory_allocated(plan.device)
+        for s in range(1, plan.n_layers + 1):
+            if plan.vjp_bytes(s) <= 0.97 * free:
+                return s
+        raise MemoryError(f"the adjoint of a [{plan.B}, {plan.D}, {plan.H}, {plan.W}] request needs "
+                          f"{plan.vjp_bytes(plan.n_layers) / 2**30:.1f} GiB even with one saved input")
 
     # ------------------------------------------------------------------------------------------------ launches
     @staticmethod
@@ -390,11 +451,13 @@ class Engine:
         mx.factor = ctypes.c_void_p(0)
         _lib.call("ffno_mix", ctypes.byref(mx), self._stream())
 
-    def horizontal(self, plan, xs, s1, s2, out, i, adjoint=False):
+    def horizontal(self, plan, xs, out, i, scr, tables, adjoint=False):
         """rfft2 -> mix -> irfft2 of xs [B,D,H,W,C] into out (models/ffno_model.py:441-448); with adjoint, the
-        transpose of that linear map (W_fwd^T H_fwd^T Mix^T H_inv^T W_inv^T)."""
+        transpose of that linear map (W_fwd^T H_fwd^T Mix^T H_inv^T W_inv^T). (b, d) planes are independent: they
+        go through in chunks of plan.h_chunk, each chunk's five launches back to back. out may be xs."""
         B, D, H, W, C, wf = plan.B, plan.D, plan.H, plan.W, plan.C, plan.wf
-        plane = 2 * H * wf * C
+        plane = plan.plane
+        hwc = H * W * C
         big = 1 << 40
         if adjoint:
             ta = plan.adjoint_tables()
@@ -403,20 +466,25 @@ class Engine:
         else:
             m1, m2, m3, m4 = plan.tw["w_fwd"], plan.tw["h_fwd"], plan.tw["h_inv"], plan.tw["w_inv"]
             wc = self.weights.blocks[i]["spec.wc"]
-        # along W, real -> half spectrum: one GEMM per voxel row (b, d, h)
-        self._tw(m1, xs, s1, B * D * H, C, (H, H * W * C, W * C), (plane, wf * C), (big, 0, C), (wf, H * wf * C, C))
-        # along H, complex: one GEMM per plane (b, d) over (part, h) rows, columns (kx, c)
-        self._tw(m2, s1, s2, B * D, wf * C, (big, 0, plane), (0, plane), (H, H * wf * C, wf * C),
-                 (H, H * wf * C, wf * C))
-        self._mix(plan, s2, s2[H * wf * C:], B * D, H * wf, C, (big, 0, plane), wc, plan.mix[i]["spec"], adjoint)
-        self._tw(m3, s2, s1, B * D, wf * C, (big, 0, plane), (0, plane), (H, H * wf * C, wf * C),
-                 (H, H * wf * C, wf * C))
-        # half spectrum -> real along W
-        self._tw(m4, s1, out, B * D * H, C, (H, plane, wf * C), (H * W * C, W * C), (wf, H * wf * C, C), (big, 0, C))
+        s1, s2 = scr["s1"], scr["s2"]
+        xf, of = xs.view(-1), out.view(-1)
+        for p0 in range(0, B * D, plan.h_chunk):
+            pc = min(plan.h_chunk, B * D - p0)
+            xc, oc = xf[p0 * hwc:], of[p0 * hwc:]
+            # along W, real -> half spectrum: one GEMM per voxel row (b, d, h)
+            self._tw(m1, xc, s1, pc * H, C, (H, hwc, W * C), (plane, wf * C), (big, 0, C), (wf, H * wf * C, C))
+            # along H, complex: one GEMM per plane (b, d) over (part, h) rows, columns (kx, c)
+            self._tw(m2, s1, s2, pc, wf * C, (big, 0, plane), (0, plane), (H, H * wf * C, wf * C),
+                     (H, H * wf * C, wf * C))
+            self._mix(plan, s2, s2[H * wf * C:], pc, H * wf, C, (big, 0, plane), wc, tables["spec"], adjoint)
+            self._tw(m3, s2, s1, pc, wf * C, (big, 0, plane), (0, plane), (H, H * wf * C, wf * C),
+                     (H, H * wf * C, wf * C))
+            # half spectrum -> real along W
+            self._tw(m4, s1, oc, pc * H, C, (H, plane, wf * C), (hwc, W * C), (wf, H * wf * C, C), (big, 0, C))
 
-    def vertical(self, plan, xv, v1, out, i, adjoint=False):
+    def vertical(self, plan, xv, v1, out, i, tables, adjoint=False):
         """Filon transform along D -> mix -> synthesis at the levels, /L folded in (models/ffno_model.py:270-305);
-        with adjoint, its transpose."""
+        with adjoint, its transpose. The spectrum after the mix stays in v1. out may be xv."""
         B, D, C, M = plan.B, plan.D, plan.C, plan.modes
         hwc = plan.H * plan.W * C
         big = 1 << 40
@@ -429,110 +497,117 @@ class Engine:
             wc = self.weights.blocks[i]["vertical.wc"]
         self._tw(m1, xv, v1, B, hwc, (big, 0, D * hwc), (0, 2 * M * hwc), (big, 0, hwc), (M, M * hwc, hwc))
         hw = plan.H * plan.W
-        self._mix(plan, v1, v1[M * hwc:], B * hw, M, hwc, (hw, 2 * M * hwc, C), wc, plan.mix[i]["vertical"], adjoint)
+        self._mix(plan, v1, v1[M * hwc:], B * hw, M, hwc, (hw, 2 * M * hwc, C), wc, tables["vertical"], adjoint)
         self._tw(m2, v1, out, B, hwc, (big, 0, 2 * M * hwc), (0, D * hwc), (M, M * hwc, hwc), (big, 0, hwc))
 
-    def _buffers(self, plan, dev, extra=()):
-        B, D, H, W, C = plan.B, plan.D, plan.H, plan.W, plan.C
-        N = B * plan.voxels
+    def _scratch(self, plan, dev, adjoint=False):
         f32 = dict(device=dev, dtype=torch.float32)
-        names = ("xs", "xv", "sp", "vp", "fp", "pp", "x1", "x2") + tuple(extra)
-        bufs = {n: torch.empty(N, C, **f32) for n in names}
-        bufs["s1"] = torch.empty(B * D * 2 * H * plan.wf * C, **f32)
-        bufs["s2"] = torch.empty_like(bufs["s1"])
-        bufs["v1"] = torch.empty(B * 2 * plan.modes * H * W * C, **f32)
-        bufs["part"] = torch.empty(B, plan.nwg, GROUPS, 3, **f32)
-        return bufs
+        scr = dict(s1=torch.empty(plan.h_chunk * plan.plane, **f32), s2=torch.empty(plan.h_chunk * plan.plane, **f32),
+                   v1=torch.empty(plan.B * 2 * plan.modes * plan.H * plan.W * plan.C, **f32),
+                   part=torch.empty(plan.B, plan.nwg, GROUPS, 3, **f32))
+        if adjoint:
+            scr["v1y"] = torch.empty_like(scr["v1"])
+            scr["part2"] = torch.empty(plan.B, plan.nwg, GROUPS, 2, **f32)
+        return scr
 
-    def _block(self, plan, i, prev, xin, bf, ys=None, yv=None):
-        """One block's forward from the assembled input `prev`; writes the block input to xin (unless None) and
-        every intermediate into bf. ys/yv: where the branch transforms' outputs go (default: over xs/xv).
-        Returns the GroupNorm affines and statistics of the five norms."""
-        dev = bf["xs"].device
+    def _block(self, plan, i, prev, x_out, h, hb, s, v, scr, bz, spectra="v1"):
+        """Forward of block i on three activation buffers. prev: the assembled block input, stored to x_out (may
+        be None when h already holds it); h: the tensor holding the block input. Schedule, with every write in place
+        of a buffer whose last reader is the writing kernel itself:
+            block_in: prev -> x_out, xs -> s, xv -> v        horizontal: s -> s     post: s -> s (sp)
+            vertical: v -> v                                 post: v -> v (vp)      fuse: h, s, v -> s (fp)
+            resmlp: h + res gelu(GN(fp)) -> hb (x1), MLP -> v (pp)
+            resmlp: hb + res GN(pp) -> hb (x2), MLP -> s (mp)
+        Returns the next block's input as an assembly (x2 + res GN(mp)) and the five GroupNorms."""
+        dev = s.device
         blk = self.weights.blocks[i]
+        tables = plan.tables(i)
         st = self._stream()
         rows = plan.rows
-        stats = FfnoStatsOut(_p(plan.depth_w), _p(bf["part"]))
-        xs, xv = bf["xs"], bf["xv"]
-        ys = xs if ys is None else ys
-        yv = xv if yv is None else yv
-        _lib.call("ffno_block_in", rows, prev, _p(xin), _p(blk["spec.g1"]), _p(blk["spec.g2"]),
-                  _p(blk["vertical.g1"]), _p(blk["vertical.g2"]), _p(blk["z_proj"]), _p(bf["z"]), _p(xs), _p(xv), st)
+        stats = FfnoStatsOut(_p(plan.depth_w), _p(scr["part"]))
+        _lib.call("ffno_block_in", rows, prev, _p(x_out), _p(blk["spec.g1"]), _p(blk["spec.g2"]),
+                  _p(blk["vertical.g1"]), _p(blk["vertical.g2"]), _p(blk["z_proj"]), _p(bz), _p(s), _p(v), st)
         gn = {}
-        self.horizontal(plan, xs, bf["s1"], bf["s2"], ys, i)
-        _lib.call("ffno_post", rows, _p(ys), _p(blk["spec.p1"]), _p(blk["spec.p2"]), _p(bf["sp"]), stats, st)
-        gn["s"] = self._gn(plan, bf["part"], blk["norm_spec"], dev)
-        self.vertical(plan, xv, bf["v1"], yv, i)
-        _lib.call("ffno_post", rows, _p(yv), _p(blk["vertical.p1"]), _p(blk["vertical.p2"]), _p(bf["vp"]), stats, st)
-        gn["v"] = self._gn(plan, bf["part"], blk["norm_vert"], dev)
-        h = xin if xin is not None else None
-        return gn, h
+        self.horizontal(plan, s, s, i, scr, tables)
+        _lib.call("ffno_post", rows, _p(s), _p(blk["spec.p1"]), _p(blk["spec.p2"]), _p(s), stats, st)
+        gn["s"] = self._gn(plan, scr["part"], blk["norm_spec"], dev)
+        self.vertical(plan, v, scr[spectra], v, i, tables)
+        _lib.call("ffno_post", rows, _p(v), _p(blk["vertical.p1"]), _p(blk["vertical.p2"]), _p(v), stats, st)
+        gn["v"] = self._gn(plan, scr["part"], blk["norm_vert"], dev)
+        self._tail(plan, i, h, s, v, s, hb, v, hb, s, scr, gn)
+        ma, mb, _ = gn["m"]
+        return _asm(hb, s, ma, mb, blk["res_mlp"]), gn
 
-    def _block_tail(self, plan, i, h, bf, gn):
-        dev = bf["xs"].device
+    def _tail(self, plan, i, h, sp, vp, fp, x1, pp, x2, mp, scr, gn):
+        """fuse -> pw -> mlp of block i (models/ffno_model.py:524-551), buffers as named (x2 may be None)."""
+        dev = sp.device
         blk = self.weights.blocks[i]
         st = self._stream()
         rows = plan.rows
-        stats = FfnoStatsOut(_p(plan.depth_w), _p(bf["part"]))
+        stats = FfnoStatsOut(_p(plan.depth_w), _p(scr["part"]))
         sa, sb, _ = gn["s"]
         va, vb, _ = gn["v"]
-        _lib.call("ffno_fuse", rows, _p(h), _p(bf["sp"]), _p(bf["vp"]), _p(sa), _p(sb), _p(va), _p(vb), _p(blk["bg1"]),
-                  _p(blk["bg2"]), _p(blk["bg2_bias"]), _p(blk["f1"]), _p(blk["f2"]), _p(bf["fp"]), stats, st)
-        gn["f"] = self._gn(plan, bf["part"], blk["norm_fuse"], dev)
+        _lib.call("ffno_fuse", rows, _p(h), _p(sp), _p(vp), _p(sa), _p(sb), _p(va), _p(vb), _p(blk["bg1"]),
+                  _p(blk["bg2"]), _p(blk["bg2_bias"]), _p(blk["f1"]), _p(blk["f2"]), _p(fp), stats, st)
+        gn["f"] = self._gn(plan, scr["part"], blk["norm_fuse"], dev)
         fa, fb, _ = gn["f"]
-        _lib.call("ffno_resmlp", rows, FfnoAssemble(_p(h), _p(bf["fp"]), _p(fa), _p(fb), _p(blk["res_fused"]), 1),
-                  _p(bf["x1"]), _p(blk["pw1"]), _p(blk["pw2"]), _p(bf["pp"]), stats, st)
-        gn["p"] = self._gn(plan, bf["part"], blk["norm_pw"], dev)
+        _lib.call("ffno_resmlp", rows, _asm(h, fp, fa, fb, blk["res_fused"], 1), _p(x1), _p(blk["pw1"]),
+                  _p(blk["pw2"]), _p(pp), stats, st)
+        gn["p"] = self._gn(plan, scr["part"], blk["norm_pw"], dev)
         pa, pb, _ = gn["p"]
-        _lib.call("ffno_resmlp", rows, FfnoAssemble(_p(bf["x1"]), _p(bf["pp"]), _p(pa), _p(pb), _p(blk["res_pw"]), 0),
-                  _p(bf["x2"]), _p(blk["m1"]), _p(blk["m2"]), _p(bf["mp"]), stats, st)
-        gn["m"] = self._gn(plan, bf["part"], blk["norm_mlp"], dev)
-        ma, mb, _ = gn["m"]
-        return FfnoAssemble(_p(bf["x2"]), _p(bf["mp"]), _p(ma), _p(mb), _p(blk["res_mlp"]), 0)
+        _lib.call("ffno_resmlp", rows, _asm(x1, pp, pa, pb, blk["res_pw"]), _p(x2), _p(blk["m1"]), _p(blk["m2"]),
+                  _p(mp), stats, st)
+        gn["m"] = self._gn(plan, scr["part"], blk["norm_mlp"], dev)
 
-    def forward(self, x, z, dx, dy, keep=None, plan=None):
-        """Normalised log10 populations [B, Cout, D, H, W] from x [B, 6, D, H, W] and z_scale [B, D, H, W] (Mm).
-        keep: a list that receives every block input and the head input (channel-last) for the adjoint."""
+    def _prepare(self, x, z, dx, dy, plan):
         if not x.is_cuda:
             raise RuntimeError("the FFNO3D HIP engine runs on the GPU only; move the inputs to a cuda device")
         dev = x.device
         self.sync(dev)
-        Wt = self.weights
         B, cin, D, H, W = x.shape
         if cin != 6:
             raise ValueError("FFNO3D expects 6 input channels")
         z = z.reshape(B, D, H, W)
         if plan is None:
             plan = self.plan(B, D, H, W, z, dx, dy)
-        C = plan.C
+        return dev, plan, x.contiguous().float(), z.reshape(B, -1).contiguous().float()
+
+    def forward(self, x, z, dx, dy, plan=None, saved=None):
+        """Normalised log10 populations [B, Cout, D, H, W] from x [B, 6, D, H, W] and z_scale [B, D, H, W] (Mm).
+        saved: a Saved that receives the block inputs the adjoint needs (its stride says which)."""
+        dev, plan, x, bz = self._prepare(x, z, dx, dy, plan)
+        Wt = self.weights
+        B, D, H, W, C = plan.B, plan.D, plan.H, plan.W, plan.C
         N = B * plan.voxels
-        x = x.contiguous().float()
         st = self._stream()
         f32 = dict(device=dev, dtype=torch.float32)
-        bf = self._buffers(plan, dev, extra=("mp",))
-        bf["z"] = z.reshape(B, -1).contiguous().float()
-        stats = FfnoStatsOut(_p(plan.depth_w), _p(bf["part"]))
+        hb, s, v = (torch.empty(N, C, **f32) for _ in range(3))
+        scr = self._scratch(plan, dev)
         rows = plan.rows
-        t0 = torch.empty(N, C, **f32)
-        _lib.call("ffno_lift", rows, _p(x), _p(Wt.lift), _p(t0), stats, st)
-        a, b, _ = self._gn(plan, bf["part"], Wt.lift_gn, dev)
-        prev = FfnoAssemble(None, _p(t0), _p(a), _p(b), None, 1)
-        xin_buf = torch.empty(N, C, **f32) if keep is None else None
+        _lib.call("ffno_lift", rows, _p(x), _p(Wt.lift), _p(s), FfnoStatsOut(_p(plan.depth_w), _p(scr["part"])), st)
+        a, b, _ = self._gn(plan, scr["part"], Wt.lift_gn, dev)
+        prev = _asm(None, s, a, b, None, 1)
         for i in range(len(Wt.blocks)):
-            # the block input never aliases x2 or mp, which the assembly reads
-            xin = torch.empty(N, C, **f32) if keep is not None else xin_buf
-            gn, _ = self._block(plan, i, prev, xin, bf)
-            if keep is not None:
-                keep.append(xin)
-            prev = self._block_tail(plan, i, xin, bf, gn)
-        out_ch = self.module.out_channels
-        y = torch.empty(B, out_ch, D, H, W, **f32)
-        h_last = torch.empty(N, C, **f32) if keep is not None else None
+            h = hb
+            if saved is not None and i % saved.stride == 0:
+                h = torch.empty(N, C, **f32)
+                saved.inputs[i] = h
+            prev, _ = self._block(plan, i, prev, h, h, hb, s, v, scr, bz)
+        y = torch.empty(B, self.module.out_channels, D, H, W, **f32)
+        h_last = None
+        if saved is not None and saved.stride == 1:
+            h_last = saved.head = torch.empty(N, C, **f32)
         _lib.call("ffno_head", rows, prev, _p(h_last), _p(Wt.p1), _p(Wt.p2), _p(Wt.p2_bias), _p(x),
                   ctypes.byref(Wt.head), _p(y), st)
-        if keep is not None:
-            keep.append(h_last)
         return y
+
+    def forward_saved(self, x, z, dx, dy, plan=None, stride=None):
+        """(y, saved): the forward that keeps what the adjoint needs, at the given checkpoint stride or the one that
+        fits the free HBM."""
+        dev, plan, _, _ = self._prepare(x, z, dx, dy, plan)
+        saved = Saved(stride or self.choose_stride(plan))
+        self.last_stride = saved.stride          # informational (the stride of the latest saving forward)
+        return self.forward(x, z, dx, dy, plan=plan, saved=saved), saved
 
     # ------------------------------------------------------------------------------------------------ adjoint
     def _gn_sums(self, plan, gb, ubar, part2):
@@ -542,135 +617,160 @@ class Engine:
         gb.sums = _p(sums)
         return sums
 
-    def _gnback(self, plan, t, gnrec, norm, scale, act):
-        a, b, stats = gnrec
-        gb = _lib.FfnoGnBack()
-        gb.t, gb.stats, gb.gamma, gb.aff_a, gb.aff_b = _p(t), _p(stats), _p(norm[0]), _p(a), _p(b)
-        gb.scale = _p(scale)
-        gb.act_gelu = act
-        gb.sums = ctypes.c_void_p(0)
-        gb.depth_w = _p(plan.depth_w)
-        return gb
-
-    def vjp(self, x, z, dx, dy, ybar, keep, want_z=False, plan=None):
+    def vjp(self, x, z, dx, dy, ybar, saved, want_z=False, plan=None):
         """(x_bar [B, 6, D, H, W], z_bar [B, D, H, W] or None) of <ybar, forward(x, z)>: the hand-written adjoint,
-        block by block in reverse, each block recomputed from its saved input (the reference's non-reentrant
-        checkpointing, models/ffno_model.py:751-780). With want_z the height gradient is computed too, per column:
-        through the depth weights of every GroupNorm, z_proj, and the vertical transform's Filon weights, phases,
-        taper and per-column shift (models/ffno_model.py:28-74, :239-309)."""
-        dev = x.device
-        self.sync(dev)
+        block by block in reverse, each block recomputed from its input (the reference's non-reentrant
+        checkpointing, models/ffno_model.py:751-780); inputs that were not saved are recomputed from the nearest
+        saved one. With want_z the height gradient is computed too, per column: through the depth weights of every
+        GroupNorm, z_proj, and the vertical transform's Filon weights, phases, taper and per-column shift
+        (models/ffno_model.py:28-74, :239-309).
+
+        Buffers: nine activation-sized ones besides the saved inputs (g: the gradient flowing down; s, v: the
+        branch transforms' outputs; sp, vp, fp, pp, mp: the five GroupNorms' inputs; x1), plus t for a recomputed
+        block input when the stride is above 1. The backward writes in place wherever the writer is the last reader:
+        g carries out_bar -> x2_bar -> x1_bar -> h_bar; sp, vp, fp, pp, mp, x1 are reused once dead."""
+        dev, plan, x, bz = self._prepare(x, z, dx, dy, plan)
         Wt = self.weights
         A = Wt.adjoint()
-        B, _, D, H, W = x.shape
-        z = z.reshape(B, D, H, W)
-        if plan is None:
-            plan = self.plan(B, D, H, W, z, dx, dy)
         plan.adjoint_tables()
-        C = plan.C
+        B, D, H, W, C, M = plan.B, plan.D, plan.H, plan.W, plan.C, plan.modes
         N = B * plan.voxels
-        M = plan.modes
         hwc = H * W * C
         big = 1 << 40
-        x = x.contiguous().float()
+        L = len(Wt.blocks)
+        stride = saved.stride
         ybar = ybar.contiguous().float()
         st = self._stream()
         rows = plan.rows
         f32 = dict(device=dev, dtype=torch.float32)
-        bf = self._buffers(plan, dev, extra=("mp", "ys", "yv", "gx2", "gx1", "gs", "gv", "gh"))
-        bf["v1y"] = torch.empty_like(bf["v1"])
-        bf["z"] = z.reshape(B, -1).contiguous().float()
-        part2 = torch.empty(B, plan.nwg, GROUPS, 2, **f32)
+        g, s, v, sp, vp, fp, pp, mp, x1 = (torch.empty(N, C, **f32) for _ in range(WORKING_UNITS))
+        tbuf = torch.empty(N, C, **f32) if stride > 1 else None
+        scr = self._scratch(plan, dev, adjoint=True)
+        part2 = scr["part2"]
         ones = torch.ones(B, C, **f32)
         zeros = torch.zeros(B, C, **f32)
         xbar = torch.empty(B, 6, D, H, W, **f32)
-        hbar = torch.empty(N, C, **f32)
         if want_z:
             ht = plan.height_tables()
             zbar = torch.zeros(B, plan.voxels, **f32)        # z_proj terms
             zbar_s = torch.zeros(B, plan.voxels, **f32)      # vertical transform, with respect to the shifted heights
             wbar = torch.zeros(B, plan.voxels, **f32)        # depth weights
-            rbuf = torch.empty(N, C, **f32)
             dfac_col = torch.zeros(B, H * W, **f32) if ht["v0"] is not None else None
         else:
-            zbar = zbar_s = wbar = None
+            zbar = zbar_s = wbar = dfac_col = None
 
         def gnback(t, rec, norm, scale, act):
-            gb = self._gnback(plan, t, rec, norm, scale, act)
+            a, b, stats = rec
+            gb = _lib.FfnoGnBack()
+            gb.t, gb.stats, gb.gamma, gb.aff_a, gb.aff_b = _p(t), _p(stats), _p(norm[0]), _p(a), _p(b)
+            gb.scale = _p(scale)
+            gb.act_gelu = act
+            gb.sums = ctypes.c_void_p(0)
+            gb.depth_w = _p(plan.depth_w)
             gb.wbar = _p(wbar)
             return gb
 
-        _lib.call("ffno_head_bwd", rows, _p(keep[-1]), _p(Wt.p1), _p(Wt.p2), _p(A["p2t"]), _p(A["p1t"]), _p(Wt.p2_bias),
-                  _p(x), ctypes.byref(Wt.head), _p(ybar), _p(hbar), _p(xbar), st)
-        for i in reversed(range(len(Wt.blocks))):
-            h = keep[i]
+        def head_bwd(h):
+            _lib.call("ffno_head_bwd", rows, _p(h), _p(Wt.p1), _p(Wt.p2), _p(A["p2t"]), _p(A["p1t"]),
+                      _p(Wt.p2_bias), _p(x), ctypes.byref(Wt.head), _p(ybar), _p(g), _p(xbar), st)
+
+        if saved.head is not None:
+            head_bwd(saved.head)
+        for i in reversed(range(L)):
             blk, adj = Wt.blocks[i], A["blocks"][i]
-            # recompute the block; the branch transforms' outputs go to ys/yv (the post networks' adjoints read them)
-            # and the vertical spectrum after the mix stays in v1y (the height gradient reads it)
-            bfr = dict(bf)
-            bfr["v1"] = bf["v1y"]
-            gn, _ = self._block(plan, i, FfnoAssemble(None, _p(h), _p(ones), _p(zeros), None, 0), None, bfr,
-                                ys=bf["ys"], yv=bf["yv"])
-            self._block_tail(plan, i, h, bf, gn)
-            # mlp: h' = x2 + res_mlp GN(mp)
-            gb = gnback(bf["mp"], gn["m"], blk["norm_mlp"], blk["res_mlp"], 0)
-            self._gn_sums(plan, gb, hbar, part2)
-            _lib.call("ffno_mlp_bwd", rows, _p(hbar), gb, _p(bf["x2"]), _p(blk["m1"]), _p(adj["m2t"]), _p(adj["m1t"]),
-                      1, _p(bf["gx2"]), st)
+            tables = plan.tables(i)
+            base = (i // stride) * stride
+            prev = _asm(None, saved.inputs[base], ones, zeros)
+            if i == base:
+                h, x_out = saved.inputs[base], None
+            else:
+                # forward from the nearest saved input up to block i's input (x1 as the running buffer)
+                for j in range(base, i):
+                    hj = saved.inputs[base] if j == base else x1
+                    prev, _ = self._block(plan, j, prev, None if j == base else x1, hj, x1, s, v, scr, bz)
+                h, x_out = tbuf, tbuf
+            # recompute block i: branch outputs stay in s and v, the vertical spectrum after the mix in v1y
+            _lib.call("ffno_block_in", rows, prev, _p(x_out), _p(blk["spec.g1"]), _p(blk["spec.g2"]),
+                      _p(blk["vertical.g1"]), _p(blk["vertical.g2"]), _p(blk["z_proj"]), _p(bz), _p(s), _p(v), st)
+            stats = FfnoStatsOut(_p(plan.depth_w), _p(scr["part"]))
+            gn = {}
+            self.horizontal(plan, s, s, i, scr, tables)
+            _lib.call("ffno_post", rows, _p(s), _p(blk["spec.p1"]), _p(blk["spec.p2"]), _p(sp), stats, st)
+            gn["s"] = self._gn(plan, scr["part"], blk["norm_spec"], dev)
+            self.vertical(plan, v, scr["v1y"], v, i, tables)
+            _lib.call("ffno_post", rows, _p(v), _p(blk["vertical.p1"]), _p(blk["vertical.p2"]), _p(vp), stats, st)
+            gn["v"] = self._gn(plan, scr["part"], blk["norm_vert"], dev)
+            last_unsaved = i == L - 1 and saved.head is None
+            self._tail(plan, i, h, sp, vp, fp, x1, pp, g if last_unsaved else None, mp, scr, gn)
+            pa, pb, _ = gn["p"]
+            if last_unsaved:
+                # the head input x2 + res_mlp GN(mp), assembled in g by the forward head's own code (bit for bit what
+                # the forward fed the head), then the head's adjoint in place
+                ma, mb, _ = gn["m"]
+                _lib.call("ffno_head", rows, _asm(g, mp, ma, mb, blk["res_mlp"]), _p(g), _p(Wt.p1), _p(Wt.p2),
+                          _p(Wt.p2_bias), _p(x), ctypes.byref(Wt.head), None, st)
+                head_bwd(g)
+            # mlp: out = x2 + res_mlp GN(mp), x2 = x1 + res_pw GN(pp)
+            gb = gnback(mp, gn["m"], blk["norm_mlp"], blk["res_mlp"], 0)
+            self._gn_sums(plan, gb, g, part2)
+            _lib.call("ffno_mlp_bwd", rows, _p(g), gb, _asm(x1, pp, pa, pb, blk["res_pw"]), _p(blk["m1"]),
+                      _p(adj["m2t"]), _p(adj["m1t"]), 1, _p(g), st)
             # pw: x2 = x1 + res_pw GN(pp)
-            gb = gnback(bf["pp"], gn["p"], blk["norm_pw"], blk["res_pw"], 0)
-            self._gn_sums(plan, gb, bf["gx2"], part2)
-            _lib.call("ffno_mlp_bwd", rows, _p(bf["gx2"]), gb, _p(bf["x1"]), _p(blk["pw1"]), _p(adj["pw2t"]),
-                      _p(adj["pw1t"]), 1, _p(bf["gx1"]), st)
-            # fusion: x1 = h + res_fused gelu(GN(fp))
-            gb = gnback(bf["fp"], gn["f"], blk["norm_fuse"], blk["res_fused"], 1)
-            self._gn_sums(plan, gb, bf["gx1"], part2)
+            gb = gnback(pp, gn["p"], blk["norm_pw"], blk["res_pw"], 0)
+            self._gn_sums(plan, gb, g, part2)
+            _lib.call("ffno_mlp_bwd", rows, _p(g), gb, _asm(x1), _p(blk["pw1"]), _p(adj["pw2t"]), _p(adj["pw1t"]), 1,
+                      _p(g), st)
+            # fusion: x1 = h + res_fused gelu(GN(fp)); s_bar -> mp, v_bar -> pp
+            gb = gnback(fp, gn["f"], blk["norm_fuse"], blk["res_fused"], 1)
+            self._gn_sums(plan, gb, g, part2)
             sa, sb_, _ = gn["s"]
             va, vb, _ = gn["v"]
-            _lib.call("ffno_fuse_bwd", rows, _p(bf["gx1"]), gb, _p(h), _p(bf["sp"]), _p(bf["vp"]), _p(sa), _p(sb_),
-                      _p(va), _p(vb), _p(blk["bg1"]), _p(blk["bg2"]), _p(blk["bg2_bias"]), _p(blk["f1"]),
-                      _p(adj["f2t"]), _p(adj["f1t"]), _p(adj["bg2t"]), _p(adj["bg1t"]), _p(bf["gh"]), _p(bf["gs"]),
-                      _p(bf["gv"]), st)
-            # horizontal branch: spec = gelu(GN_s(post(Spec(xs)))); x_s bar lands in gs
-            gb = gnback(bf["sp"], gn["s"], blk["norm_spec"], None, 1)
-            self._gn_sums(plan, gb, bf["gs"], part2)
-            _lib.call("ffno_mlp_bwd", rows, _p(bf["gs"]), gb, _p(bf["ys"]), _p(blk["spec.p1"]), _p(adj["spec.p2t"]),
-                      _p(adj["spec.p1t"]), 0, _p(bf["gx2"]), st)
-            self.horizontal(plan, bf["gx2"], bf["s1"], bf["s2"], bf["gs"], i, adjoint=True)
-            # vertical branch; x_v bar lands in gv, the spectrum gradient stays in v1
-            gb = gnback(bf["vp"], gn["v"], blk["norm_vert"], None, 1)
-            self._gn_sums(plan, gb, bf["gv"], part2)
-            _lib.call("ffno_mlp_bwd", rows, _p(bf["gv"]), gb, _p(bf["yv"]), _p(blk["vertical.p1"]),
-                      _p(adj["vertical.p2t"]), _p(adj["vertical.p1t"]), 0, _p(bf["gx1"]), st)
+            _lib.call("ffno_fuse_bwd", rows, _p(g), gb, _p(h), _p(sp), _p(vp), _p(sa), _p(sb_), _p(va), _p(vb),
+                      _p(blk["bg1"]), _p(blk["bg2"]), _p(blk["bg2_bias"]), _p(blk["f1"]), _p(adj["f2t"]),
+                      _p(adj["f1t"]), _p(adj["bg2t"]), _p(adj["bg1t"]), _p(g), _p(mp), _p(pp), st)
+            # horizontal branch: spec = gelu(GN_s(post(Spec(xs)))); ys_bar -> s, xs_bar -> mp
+            gb = gnback(sp, gn["s"], blk["norm_spec"], None, 1)
+            self._gn_sums(plan, gb, mp, part2)
+            _lib.call("ffno_mlp_bwd", rows, _p(mp), gb, _asm(s), _p(blk["spec.p1"]), _p(adj["spec.p2t"]),
+                      _p(adj["spec.p1t"]), 0, _p(s), st)
+            self.horizontal(plan, s, mp, i, scr, tables, adjoint=True)
+            # vertical branch: yv_bar -> v, the spectrum gradient in v1, xv_bar -> pp
+            gb = gnback(vp, gn["v"], blk["norm_vert"], None, 1)
+            self._gn_sums(plan, gb, pp, part2)
+            _lib.call("ffno_mlp_bwd", rows, _p(pp), gb, _asm(v), _p(blk["vertical.p1"]), _p(adj["vertical.p2t"]),
+                      _p(adj["vertical.p1t"]), 0, _p(v), st)
+            v1, v1y = scr["v1"], scr["v1y"]
             if want_z and ht["v0"] is not None:
                 # taper: dL/df_m per column = sum_c,part (V0^T yv_bar) * Y, before the mix adjoint overwrites v1
-                self._tw(ht["v0"], bf["gx1"], bf["v1"], B, hwc, (big, 0, D * hwc), (0, 2 * M * hwc), (big, 0, hwc),
+                self._tw(ht["v0"], v, v1, B, hwc, (big, 0, D * hwc), (0, 2 * M * hwc), (big, 0, hwc),
                          (M, M * hwc, hwc))
-                g = (bf["v1"].view(B, 2, M, H * W, C) * bf["v1y"].view(B, 2, M, H * W, C)).sum((1, 4))   # [B, M, HW]
-                dfac_col += (g * ht["dfac"][None, :, None]).sum(1)
-            self.vertical(plan, bf["gx1"], bf["v1"], bf["gv"], i, adjoint=True)
+                gsum = (v1.view(B, 2, M, H * W, C) * v1y.view(B, 2, M, H * W, C)).sum((1, 4))   # [B, M, HW]
+                dfac_col += (gsum * ht["dfac"][None, :, None]).sum(1)
+            self.vertical(plan, v, v1, pp, i, tables, adjoint=True)
             if want_z:
-                # Filon weights: zbar_s[e] += sum_delta sum_c xv[e + delta] (G_delta X_bar)[e]
+                # the vertical branch's input again (xs -> sp, xv -> fp, both dead here)
+                _lib.call("ffno_block_in", rows, _asm(None, h, ones, zeros), None, _p(blk["spec.g1"]),
+                          _p(blk["spec.g2"]), _p(blk["vertical.g1"]), _p(blk["vertical.g2"]), _p(blk["z_proj"]),
+                          _p(bz), _p(sp), _p(fp), st)
+                # Filon weights: zbar_s[e] += sum_delta sum_c xv[e + delta] (G_delta X_bar)[e]   (x1 as scratch)
                 for delta in (-1, 0, 1):
-                    self._tw(ht["bands"][delta], bf["v1"], rbuf, B, hwc, (big, 0, 2 * M * hwc), (0, D * hwc),
+                    self._tw(ht["bands"][delta], v1, x1, B, hwc, (big, 0, 2 * M * hwc), (0, D * hwc),
                              (M, M * hwc, hwc), (big, 0, hwc))
-                    _lib.call("ffno_dot_reduce", rows, D, _p(bf["xv"]), delta, _p(rbuf), _p(zbar_s), st)
+                    _lib.call("ffno_dot_reduce", rows, D, _p(fp), delta, _p(x1), _p(zbar_s), st)
                 # synthesis phases: zbar_s[d] += sum_c yv_bar[d] (V_dz Y)[d]
-                self._tw(ht["v_dz"], bf["v1y"], rbuf, B, hwc, (big, 0, 2 * M * hwc), (0, D * hwc), (M, M * hwc, hwc),
+                self._tw(ht["v_dz"], v1y, x1, B, hwc, (big, 0, 2 * M * hwc), (0, D * hwc), (M, M * hwc, hwc),
                          (big, 0, hwc))
-                _lib.call("ffno_dot_reduce", rows, D, _p(bf["gx1"]), 0, _p(rbuf), _p(zbar_s), st)
-            # input gates (and z_proj)
-            _lib.call("ffno_gates_bwd", rows, _p(h), _p(bf["gs"]), _p(bf["gv"]), _p(bf["gh"]), _p(blk["spec.g1"]),
-                      _p(blk["spec.g2"]), _p(adj["spec.g2t"]), _p(adj["spec.g1t"]), _p(blk["vertical.g1"]),
-                      _p(blk["vertical.g2"]), _p(adj["vertical.g2t"]), _p(adj["vertical.g1t"]), _p(blk["z_proj"]),
-                      _p(hbar), _p(zbar), st)
+                _lib.call("ffno_dot_reduce", rows, D, _p(v), 0, _p(x1), _p(zbar_s), st)
+            # input gates (and z_proj): h_bar in g
+            _lib.call("ffno_gates_bwd", rows, _p(h), _p(mp), _p(pp), _p(g), _p(blk["spec.g1"]), _p(blk["spec.g2"]),
+                      _p(adj["spec.g2t"]), _p(adj["spec.g1t"]), _p(blk["vertical.g1"]), _p(blk["vertical.g2"]),
+                      _p(adj["vertical.g2t"]), _p(adj["vertical.g1t"]), _p(blk["z_proj"]), _p(g), _p(zbar), st)
         # lift: h0 = gelu(GN(W_lift x))
-        t0 = torch.empty(N, C, **f32)
-        _lib.call("ffno_lift", rows, _p(x), _p(Wt.lift), _p(t0), FfnoStatsOut(_p(plan.depth_w), _p(bf["part"])), st)
-        g0 = self._gn(plan, bf["part"], Wt.lift_gn, dev)
-        gb = gnback(t0, g0, Wt.lift_gn, None, 1)
-        self._gn_sums(plan, gb, hbar, part2)
-        _lib.call("ffno_lift_bwd", rows, _p(hbar), gb, _p(Wt.lift), _p(xbar), st)
+        _lib.call("ffno_lift", rows, _p(x), _p(Wt.lift), _p(s), FfnoStatsOut(_p(plan.depth_w), _p(scr["part"])), st)
+        g0 = self._gn(plan, scr["part"], Wt.lift_gn, dev)
+        gb = gnback(s, g0, Wt.lift_gn, None, 1)
+        self._gn_sums(plan, gb, g, part2)
+        _lib.call("ffno_lift_bwd", rows, _p(g), gb, _p(Wt.lift), _p(xbar), st)
         if not want_z:
             return xbar, None
         return xbar, self._height_gradient(plan, z, zbar, zbar_s, wbar, dfac_col)

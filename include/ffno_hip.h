@@ -51,7 +51,7 @@ typedef struct {
   const float* aff_b;     /* [B][C] */
   const float* scale;     /* device scalar or NULL (1) */
   int act_gelu;           /* 1: GELU after the affine, 0: none */
-} FfnoAssemble;
+} FfnoAssemble;           /* x = base + scale * act(aff_a * t + aff_b); base NULL: 0; t NULL: x = base */
 
 /* Lift: Conv3d 6->C then GroupNorm statistics (models/ffno_model.py:686-690, :782-783). */
 int ffno_lift(FfnoRows rows, const float* x_in, const float* w_lift /* [C][6] */, float* out, FfnoStatsOut stats,
@@ -63,24 +63,27 @@ int ffno_gn_finalize(int B, int nwg, int C, const float* partials, const float* 
                      float* stats, float* aff_a, float* aff_b, hipStream_t stream);
 
 /* Block entry (models/ffno_model.py:536-542, :266, :439-440): x = assemble(...); writes x, then both input gates:
- * xs = x * sigmoid(G2 gelu(G1 x)), xv = x * sigmoid(V2 gelu(V1 x)) + z_proj * z. */
+ * xs = x * sigmoid(G2 gelu(G1 x)), xv = x * sigmoid(V2 gelu(V1 x)) + z_proj * z. x_out may be NULL, or alias
+ * in.base; xs may alias in.t (each row is read before it is written). */
 int ffno_block_in(FfnoRows rows, FfnoAssemble in, float* x_out, const uint16_t* g1, const uint16_t* g2,
                   const uint16_t* v1, const uint16_t* v2, const float* z_proj /* [C] */, const float* z /* [B,DHW] */,
                   float* xs, float* xv, hipStream_t stream);
 
 /* Two-layer pointwise MLP y = W2 gelu(W1 u) with GroupNorm statistics of y: the post networks of both branches
- * (models/ffno_model.py:221-225, :369-373, :309, :449). */
+ * (models/ffno_model.py:221-225, :369-373, :309, :449). y may alias u. */
 int ffno_post(FfnoRows rows, const float* u, const uint16_t* w1, const uint16_t* w2, float* y, FfnoStatsOut stats,
               hipStream_t stream);
 
 /* Branch fusion (models/ffno_model.py:524-534, :541-543): s = gelu(GN(sp)), v = gelu(GN(vp)),
- * gates = sigmoid(BG2 gelu(BG1 [x,s,v]) + b), fp = F2 gelu(F1 [s*gs, v*gv]) + 0.5 (s*gs + v*gv), statistics of fp. */
+ * gates = sigmoid(BG2 gelu(BG1 [x,s,v]) + b), fp = F2 gelu(F1 [s*gs, v*gv]) + 0.5 (s*gs + v*gv), statistics of fp.
+ * fp may alias sp. */
 int ffno_fuse(FfnoRows rows, const float* x, const float* sp, const float* vp, const float* s_a, const float* s_b,
               const float* v_a, const float* v_b, const uint16_t* bg1, const uint16_t* bg2, const float* bg2_bias,
               const uint16_t* f1, const uint16_t* f2, float* fp, FfnoStatsOut stats, hipStream_t stream);
 
 /* Residual pointwise path (models/ffno_model.py:545-551): x_out = assemble(...); y = W2 gelu(W1 x_out); statistics
- * of y. Used for pw (assemble = x + res_fused gelu(GN(fp))) and mlp (assemble = x1 + res_pw GN(pp)). */
+ * of y. Used for pw (assemble = x + res_fused gelu(GN(fp))) and mlp (assemble = x1 + res_pw GN(pp)). x_out may be
+ * NULL or alias in.base. */
 int ffno_resmlp(FfnoRows rows, FfnoAssemble in, float* x_out, const uint16_t* w1, const uint16_t* w2, float* y,
                 FfnoStatsOut stats, hipStream_t stream);
 
@@ -102,6 +105,7 @@ typedef struct {
   float dxi_coef;     /* 4.98e-4 * 5040 */
 } FfnoHeadDesc;
 
+/* x_out (optional, may alias in.base) receives the assembled head input; y NULL: only that is done. */
 int ffno_head(FfnoRows rows, FfnoAssemble in, float* x_out, const uint16_t* p1, const uint16_t* p2,
               const float* p2_bias, const float* x_in /* [B,6,DHW] */, const FfnoHeadDesc* desc, float* y,
               hipStream_t stream);
@@ -174,13 +178,15 @@ int ffno_gn_grad_finalize(int B, int nwg, const float* partials, float* sums, hi
 
 /* Adjoint of y = W2 gelu(W1 x) behind a GroupNorm: t_bar = GN backward of u_bar, then
  * x_bar = [u_bar if residual] + W1^T (GELU'(W1 x) * W2^T t_bar) (models/ffno_model.py:547-551 for pw and mlp with
- * residual; :221-225, :369-373 for the branch post networks without). w2t, w1t are the transposed weights packed. */
-int ffno_mlp_bwd(FfnoRows rows, const float* ubar, FfnoGnBack gn, const float* x, const uint16_t* w1,
+ * residual; :221-225, :369-373 for the branch post networks without). w2t, w1t are the transposed weights packed.
+ * The MLP input x is given as an assembly (x = base, or base + scale act(a t + b)), so a residual sum need not be
+ * stored. x_bar may alias u_bar or x.base (each row is read before it is written). */
+int ffno_mlp_bwd(FfnoRows rows, const float* ubar, FfnoGnBack gn, FfnoAssemble x, const uint16_t* w1,
                  const uint16_t* w2t, const uint16_t* w1t, int residual, float* xbar, hipStream_t stream);
 
 /* Adjoint of the branch fusion (models/ffno_model.py:524-545): from x1_bar (gradient of x1 = x + res_fused * f)
  * to the gradients of the block input (including x1_bar's residual path), of s = gelu(GN_s(sp)) and of
- * v = gelu(GN_v(vp)); gates and hidden layers are recomputed. */
+ * v = gelu(GN_v(vp)); gates and hidden layers are recomputed. hbar may alias x1bar. */
 int ffno_fuse_bwd(FfnoRows rows, const float* x1bar, FfnoGnBack gnf, const float* h, const float* sp, const float* vp,
                   const float* s_a, const float* s_b, const float* v_a, const float* v_b, const uint16_t* bg1,
                   const uint16_t* bg2, const float* bg2_bias, const uint16_t* f1, const uint16_t* f2t,
@@ -188,7 +194,7 @@ int ffno_fuse_bwd(FfnoRows rows, const float* x1bar, FfnoGnBack gnf, const float
                   float* vbar, hipStream_t stream);
 
 /* Adjoint of both input gates (models/ffno_model.py:266, :439-440): hbar_out = hbar_in + d(xs)/dh^T xs_bar +
- * d(xv)/dh^T xv_bar; zbar (optional, accumulated) += sum_c xv_bar * z_proj. */
+ * d(xv)/dh^T xv_bar; zbar (optional, accumulated) += sum_c xv_bar * z_proj. hbar_out may alias hbar_in. */
 int ffno_gates_bwd(FfnoRows rows, const float* h, const float* xsbar, const float* xvbar, const float* hbar_in,
                    const uint16_t* g1, const uint16_t* g2, const uint16_t* g2t, const uint16_t* g1t,
                    const uint16_t* v1, const uint16_t* v2, const uint16_t* v2t, const uint16_t* v1t,
@@ -196,7 +202,7 @@ int ffno_gates_bwd(FfnoRows rows, const float* h, const float* xsbar, const floa
 
 /* Adjoint of the head (models/ffno_model.py:730-749, :790-798) from the output gradient ybar [B, Cout, DHW]:
  * writes hbar [N, C] and the direct input gradient xbar_in [B, 6, DHW] (T, ne and rho through the LTE weights
- * and the total; the other channels 0). */
+ * and the total; the other channels 0). hbar may alias h. */
 int ffno_head_bwd(FfnoRows rows, const float* h, const uint16_t* p1, const uint16_t* p2, const uint16_t* p2t,
                   const uint16_t* p1t, const float* p2_bias, const float* x_in, const FfnoHeadDesc* desc,
                   const float* ybar, float* hbar, float* xbar_in, hipStream_t stream);

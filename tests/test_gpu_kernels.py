@@ -68,7 +68,7 @@ def _small_model(width=64, rank=8, layers=1, species=("H", "CA")):
 
 def test_post_mlp_and_groupnorm_statistics():
     """ffno_post (C -> 2C -> C, split-bf16 MFMA) against float64 (5e-5 of range), and its fused weighted GroupNorm
-    statistics against a float64 weighted mean/variance (1e-4 relative)."""
+    statistics against a float64 weighted mean/variance (1e-4 relative) and total weight (1e-5)."""
     from ffno_amd import _lib
     from ffno_amd._lib import FfnoRows, FfnoStatsOut
     from ffno_amd.engine import pack_mlp_weight
@@ -89,7 +89,7 @@ def test_post_mlp_and_groupnorm_statistics():
         beta = torch.zeros(C, device=DEV)
         a = torch.empty(B, C, device=DEV)
         b = torch.empty(B, C, device=DEV)
-        stats = torch.empty(B, 8, 2, device=DEV)
+        stats = torch.empty(B, 8, 3, device=DEV)             # (mean, rstd, total weight) per (b, group)
         _lib.call("ffno_gn_finalize", B, nwg, C, _p(part), _p(gamma), _p(beta), _p(stats), _p(a), _p(b), _stream())
         torch.cuda.synchronize()
         ref = _gelu(u.double() @ w1.double().t()) @ w2.double().t()
@@ -102,6 +102,8 @@ def test_post_mlp_and_groupnorm_statistics():
         var = (ww * (yy - mean[:, None, :, None]) ** 2).sum((1, 3)) / ww.sum((1, 3))
         assert ((stats[..., 0].double() - mean).abs().max() / var.sqrt().max()) < 1e-4
         assert ((stats[..., 1].double() - (var + 1e-5).rsqrt()).abs() / (var + 1e-5).rsqrt()).max() < 1e-4
+        wtot = ww.sum((1, 3))
+        assert ((stats[..., 2].double() - wtot).abs() / wtot).max() < 1e-5
 
 
 def test_spectral_pipelines_match_fft():
@@ -118,15 +120,16 @@ def test_spectral_pipelines_match_fft():
     torch.manual_seed(2)
     xs = torch.randn(B, D, H, W, C, device=DEV)
     wf = W // 2 + 1
-    s1 = torch.empty(B * D * 2 * H * wf * C, device=DEV)
-    s2 = torch.empty_like(s1)
+    plan.h_chunk = 5                 # 24 (b, d) planes in chunks of 5, 5, 5, 5, 4
+    scr = eng._scratch(plan, torch.device(DEV))
     out = torch.empty_like(xs)
-    eng.horizontal(plan, xs.reshape(-1, C), s1, s2, out.reshape(-1, C), 0)
+    tables = plan.tables(0)
+    eng.horizontal(plan, xs.reshape(-1, C), out.reshape(-1, C), 0, scr, tables)
     sd = eng.weights.sd
     xf = torch.fft.rfft2(xs.double().cpu().permute(0, 4, 1, 2, 3), dim=(-2, -1))      # [B, C, D, H, Wf]
     wr = sd["blocks.0.spec.weight_real"].double().cpu()
     wi = sd["blocks.0.spec.weight_imag"].double().cpu()
-    gate = plan.mix[0]["spec"]["gate"].double().cpu().reshape(H, wf, 2)
+    gate = tables["spec"]["gate"].double().cpu().reshape(H, wf, 2)
     wc = torch.complex(wr, wi)
     a = torch.einsum("bidyx,io->bodyx", xf, wc.to(xf.dtype))
     y = a * torch.complex(gate[..., 0], gate[..., 1])
@@ -153,7 +156,7 @@ def test_spectral_pipelines_match_fft():
     xv = torch.randn(B, D, H, W, C, device=DEV)
     v1 = torch.empty(B * 2 * M * H * W * C, device=DEV)
     outv = torch.empty_like(xv)
-    eng.vertical(plan, xv.reshape(-1, C), v1, outv.reshape(-1, C), 0)
+    eng.vertical(plan, xv.reshape(-1, C), v1, outv.reshape(-1, C), 0, tables)
     torch.cuda.synchronize()
     from oracle import ffno_oracle as O
     cfg = O.model_config(width=C, n_layers=1, hidden_spectral=32, hidden_vertical=32, freq_mix_rank=8,
@@ -167,7 +170,7 @@ def test_spectral_pipelines_match_fft():
     xx = xv.double().cpu().permute(0, 4, 1, 2, 3)
     xr = torch.einsum("bcdhw,dm->bcmhw", xx, q_re)
     xi = torch.einsum("bcdhw,dm->bcmhw", xx, q_im)
-    gate = plan.mix[0]["vertical"]["gate"].double().cpu()
+    gate = tables["vertical"]["gate"].double().cpu()
     gr, gi = gate[:, 0, None, None], gate[:, 1, None, None]
     wr = sd64["blocks.0.vertical.weight_real"]
     wi = sd64["blocks.0.vertical.weight_imag"]

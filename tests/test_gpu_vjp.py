@@ -169,3 +169,30 @@ def test_backend_vjp_local_matches_si_autograd():
         nrm, mx = _errs(out[key], ref)
         print(f"[backend] vjp {key}: rel norm {nrm:.3e} rel max {mx:.3e}", flush=True)
         assert nrm <= TOL
+
+
+def test_checkpoint_strides_agree():
+    """The adjoint that keeps every block input (stride 1) and the ones that keep every second or third and
+    recompute the others from the nearest kept input (strides 2, 3: what whole-volume adjoints use when memory is
+    short) give the same x_bar and z_bar bit for bit: the recomputed inputs are the same kernels on the same data."""
+    data, config, state = load("case_b_jitter64")
+    config = dict(config, n_layers=4) if config.get("n_layers", 0) < 4 else config
+    from ffno_amd import synthetic
+    model = synthetic.random_model(3, **config).to(DEV).eval()
+    x, z, mx, sx = synthetic.atmosphere(2, 16, 12, 10, seed=4)
+    x, z = x.to(DEV), z.to(DEV)
+    ct = synthetic.cotangent((2, config["out_channels"], 16, 12, 10), 5).to(DEV)
+    eng = model.engine()
+    results = {}
+    for stride in (1, 2, 3):
+        print(f"stride {stride}", flush=True)
+        y, saved = eng.forward_saved(x, z, 48e3, 48e3, stride=stride)
+        assert len(saved.inputs) == -(-config["n_layers"] // stride) and (saved.head is not None) == (stride == 1)
+        xb, zb = eng.vjp(x, z, 48e3, 48e3, ct, saved, want_z=True)
+        torch.cuda.synchronize()
+        results[stride] = (y.cpu(), xb.cpu(), zb.cpu())
+    for stride in (2, 3):
+        for a, b, name in zip(results[stride], results[1], ("y", "x_bar", "z_bar")):
+            err = float((a - b).abs().max() / b.abs().max())
+            print(f"stride {stride} {name} rel {err:.2e}", flush=True)
+            assert err == 0.0
