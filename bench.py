@@ -167,26 +167,31 @@ def main():
         out["cpu_baseline"] = cpu_baseline()
     if args.volume:
         del y, xb
-        out["whole_volume"] = volume_leg(model, dev, plain_too=not args.no_baselines)
+        out["whole_volume"] = volume_leg(model, dev)
     print(json.dumps(out), flush=True)
 
 
-def volume_leg(model, dev, plain_too=True):
+def volume_leg(model, dev):
     """One 504 x 504 x 256 atmosphere (65 M voxels) through the same network on one GPU, no slabbing: forward alone,
-    then forward + input VJP (the adjoint chooses its checkpoint stride from the free HBM), peak HBM of each. Parity
-    at this size, where the float64 oracle cannot run: the network is equivariant to circular horizontal shifts
-    (1x1 layers, column-local vertical branch, FFT-periodic horizontal branch, shift-invariant GroupNorm sums), so
-    f(roll(x)) must equal roll(f(x)) up to rounding -- a wrong twiddle or index of the 504-point transforms breaks
-    it; and the plain FP32 GPU forward on the same input, when it fits in memory."""
+    then forward + input VJP (the adjoint chooses its checkpoint stride from the free HBM), peak HBM of each.
+    Checks at this size, where the float64 oracle cannot run: (1) the network is equivariant to circular horizontal
+    shifts (1x1 layers, column-local vertical branch, FFT-periodic horizontal branch, shift-invariant GroupNorm
+    sums), so f(roll(x)) must equal roll(f(x)) up to rounding -- a wrong twiddle or index of the 504-point transforms
+    breaks it; (2) the adjoint against a central difference of the forward along a random direction d (step 1e-2
+    in normalised units): <ybar, f(x + e d) - f(x - e d)> / 2e against <x_bar, d>."""
     import torch
     from ffno_amd import synthetic
+
+    def log(msg):
+        print(f"volume: {msg}", file=sys.stderr, flush=True)
+
     v = VOLUME
     voxels = v["B"] * v["D"] * v["H"] * v["W"]
     t0 = time.perf_counter()
     x, z, _, _ = synthetic.atmosphere(v["B"], v["D"], v["H"], v["W"], seed=0)
     ct = synthetic.cotangent((v["B"], 12, v["D"], v["H"], v["W"]), 0)
     x, z, ct = x.to(dev), z.to(dev), ct.to(dev)
-    print(f"volume: inputs ready in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    log(f"inputs ready in {time.perf_counter() - t0:.1f} s")
     res = {"workload": "FFNO3D 504x504x256, B=1, width 64, 8 layers, vertical modes 32", "voxels": voxels}
 
     def timed(fn):
@@ -199,19 +204,21 @@ def volume_leg(model, dev, plain_too=True):
         torch.cuda.synchronize()
         return r, e0.elapsed_time(e1), torch.cuda.max_memory_allocated(dev)
 
+    log("forward")
     with torch.no_grad():
         y, ms, peak = timed(lambda: model(x, z, 48e3, 48e3))
     res["forward_ms"] = round(ms, 1)
     res["forward_mvox_s"] = round(voxels / (ms * 1e-3) / 1e6, 3)
     res["forward_peak_hbm_gb"] = round(peak / 1e9, 2)
-    print(f"volume: forward {ms:.0f} ms, peak {peak / 1e9:.1f} GB", file=sys.stderr, flush=True)
+    res["output_finite"] = bool(torch.isfinite(y).all())
+    log(f"forward {ms:.0f} ms, peak {peak / 1e9:.1f} GB")
     sh, sw = 137, 311
     with torch.no_grad():
         yr = model(torch.roll(x, (sh, sw), (3, 4)), torch.roll(z, (sh, sw), (2, 3)), 48e3, 48e3)
         res["shift_equivariance_rel_err"] = float((yr - torch.roll(y, (sh, sw), (3, 4))).abs().max() /
                                                   y.abs().max().clamp_min(1.0))
     del yr
-    print(f"volume: shift equivariance {res['shift_equivariance_rel_err']:.2e}", file=sys.stderr, flush=True)
+    log(f"shift equivariance {res['shift_equivariance_rel_err']:.2e}")
 
     def fwd_vjp():
         xg = x.detach().requires_grad_(True)
@@ -219,27 +226,23 @@ def volume_leg(model, dev, plain_too=True):
         (g,) = torch.autograd.grad(yy, xg, grad_outputs=ct)
         return g
 
+    log("forward + vjp")
     xb, ms, peak = timed(fwd_vjp)
     res["fwd_vjp_ms"] = round(ms, 1)
     res["fwd_vjp_mvox_s"] = round(voxels / (ms * 1e-3) / 1e6, 3)
     res["fwd_vjp_peak_hbm_gb"] = round(peak / 1e9, 2)
     res["checkpoint_stride"] = model.engine().last_stride
-    res["xbar_finite"] = bool(torch.isfinite(xb).all())
-    print(f"volume: fwd+vjp {ms:.0f} ms, peak {peak / 1e9:.1f} GB", file=sys.stderr, flush=True)
-    del xb
-    if plain_too:
-        from ffno_amd.reference_torch import PlainFFNO3D
-        plain = PlainFFNO3D(model).to(dev)
-        try:
-            with torch.no_grad():
-                yp, ms, peak = timed(lambda: plain(x, z, 48e3, 48e3))
-            res["plain_fp32_forward_ms"] = round(ms, 1)
-            res["plain_fp32_forward_peak_hbm_gb"] = round(peak / 1e9, 2)
-            res["rel_err_log10n_vs_plain"] = float(((y - yp).abs() / yp.abs().clamp_min(1.0)).max())
-            del yp
-        except torch.OutOfMemoryError:
-            res["plain_fp32_forward_ms"] = "out of memory"
-        print(f"volume: plain {res['plain_fp32_forward_ms']}", file=sys.stderr, flush=True)
+    log(f"fwd+vjp {ms:.0f} ms, peak {peak / 1e9:.1f} GB, stride {res['checkpoint_stride']}")
+    gen = torch.Generator(device=dev).manual_seed(7)
+    d = torch.randn(x.shape, device=dev, generator=gen)
+    eps = 1e-2
+    with torch.no_grad():
+        lp = (ct.double() * model(x + eps * d, z, 48e3, 48e3).double()).sum()
+        lm = (ct.double() * model(x - eps * d, z, 48e3, 48e3).double()).sum()
+        fd = float((lp - lm) / (2 * eps))
+        ad = float((xb.double() * d.double()).sum())
+    res["vjp_vs_central_difference_rel_err"] = abs(fd - ad) / max(abs(fd), 1e-30)
+    log(f"vjp vs central difference: {ad:.6e} vs {fd:.6e}")
     return res
 
 

@@ -127,7 +127,7 @@ class PackedWeights:
         self.p2 = pack_mlp_weight(_conv_w(sd["proj2.weight"]))
         nh = sd["proj2.weight"].shape[0]
         self.p2_bias = F.pad(sd["proj2.bias"].float(), (0, T.round_up(nh, 16) - nh)).contiguous()
-        self.head = self._head_desc(module, sd)
+        self.head = self._head_desc(module, module.state_dict())     # host scalars, read where the module lives
 
     def adjoint(self):
         """Transposed weights for the adjoint kernels, packed on first use."""

@@ -169,7 +169,14 @@ class PlainFFNO3D:
         y = torch.cat(out, 1)
         return (y - sd["mean_Y"].view(1, -1, 1, 1, 1)) / sd["std_Y"].view(1, -1, 1, 1, 1)
 
+    MAX_ELEMENTS = 2 ** 31 - 1
+
     def __call__(self, x, z_scale, dx, dy):
+        # stock library kernels (convolutions, FFTs) are only run here on activations below 2^31 elements, the sizes
+        # this baseline is timed at; a whole 504 x 504 x 256 volume at width 64 (4.2e9 elements) is refused
+        B, _, D, H, W = x.shape
+        if B * self.m.width * D * H * (2 * (W // 2 + 1)) > self.MAX_ELEMENTS:
+            raise ValueError("the plain FP32 path is not run on activations of 2^31 elements or more")
         sd, dx, dy = self.sd, float(dx), float(dy)
         zs = z_scale.unsqueeze(1)
         tw = F.pad(0.5 * (zs[:, :, 1:] - zs[:, :, :-1]).abs(), (0, 0, 0, 0, 0, 1)) + \

@@ -1,0 +1,329 @@
+"""CPU bounds check of every HIP launch the engine issues: the engine's own forward / adjoint schedule runs on meta
+tensors (shapes, no data, no GPU), every device pointer it passes gets a synthetic address inside the tensor that
+owns it, and each entry point's call is checked against a model of the bytes its kernel reads and writes -- the
+largest offset from the kernel's index arithmetic (row-major activations, the two-level GEMM index maps, per-mode
+tables, partial-statistics slots) against the end of the owning tensor. This is how a launch at a size no GPU test
+covers (a whole 504 x 504 x 256 volume) is shown to stay in bounds before it runs on a GPU."""
+import ctypes
+import types
+
+import numpy as np
+import torch
+
+from ffno_amd import _lib, engine as E
+from ffno_amd.transforms import round_up
+
+F32, BF = 4, 2
+
+
+class Registry:
+    """Synthetic device addresses: each root tensor gets its own 1 TiB-aligned range."""
+
+    def __init__(self):
+        self.roots = {}
+        self.keep = []
+        self.next = 1 << 40
+
+    def address(self, t):
+        if t is None:
+            return 0
+        root = t._base if t._base is not None else t
+        key = id(root)
+        if key not in self.roots:
+            self.roots[key] = (self.next, root.numel() * root.element_size())
+            self.keep.append(root)
+            self.next += 1 << 40
+        base, _ = self.roots[key]
+        return base + t.storage_offset() * t.element_size()
+
+    def owner(self, addr):
+        for base, nbytes in self.roots.values():
+            if base <= addr < base + max(nbytes, 1):
+                return base, nbytes
+        raise AssertionError(f"pointer {addr:#x} is not inside any tensor")
+
+
+class Checker:
+    def __init__(self, reg):
+        self.reg = reg
+        self.calls = []
+
+    # ---- helpers
+    def span(self, name, what, ptr, nbytes, optional=False):
+        """ptr .. ptr + nbytes must lie inside one tensor."""
+        ptr = ptr.value if isinstance(ptr, ctypes.c_void_p) else ptr
+        if not ptr:
+            assert optional, f"{name}: {what} is NULL"
+            return
+        assert nbytes > 0, f"{name}: {what} has an empty access"
+        base, size = self.reg.owner(ptr)
+        assert ptr + nbytes <= base + size, (f"{name}: {what} reads/writes {ptr - base + nbytes} bytes of a "
+                                             f"{size}-byte tensor")
+
+    def act(self, name, what, ptr, rows, optional=False):
+        self.span(name, what, ptr, rows.B * rows.voxels * rows.C * F32, optional)
+
+    def asm(self, name, a, rows):
+        self.act(name, "assembly base", a.base, rows, optional=True)
+        self.act(name, "assembly t", a.t, rows, optional=True)
+        if a.t:
+            self.span(name, "assembly aff_a", a.aff_a, rows.B * rows.C * F32)
+            self.span(name, "assembly aff_b", a.aff_b, rows.B * rows.C * F32)
+        self.span(name, "assembly scale", a.scale, F32, optional=True)
+
+    def stats(self, name, s, rows):
+        self.span(name, "depth_w", s.depth_w, rows.B * rows.voxels * F32)
+        self.span(name, "partials", s.partials, rows.B * rows.nwg * 8 * 3 * F32, optional=True)
+
+    def gnback(self, name, g, rows):
+        self.act(name, "gn t", g.t, rows)
+        self.span(name, "gn stats", g.stats, rows.B * 8 * 3 * F32)
+        self.span(name, "gn gamma", g.gamma, rows.C * F32)
+        self.span(name, "gn aff_a", g.aff_a, rows.B * rows.C * F32)
+        self.span(name, "gn aff_b", g.aff_b, rows.B * rows.C * F32)
+        self.span(name, "gn scale", g.scale, F32, optional=True)
+        self.span(name, "gn sums", g.sums, rows.B * 8 * 2 * F32, optional=True)
+        self.span(name, "gn depth_w", g.depth_w, rows.B * rows.voxels * F32)
+        self.span(name, "gn wbar", g.wbar, rows.B * rows.voxels * F32, optional=True)
+
+    def weight(self, name, what, ptr, n_rows, k):
+        """A packed split-bf16 matrix [2][n_rows][round_up(k, 32)] as the kernels stage it."""
+        self.span(name, what, ptr, 2 * n_rows * round_up(k, 32) * BF)
+
+    @staticmethod
+    def imap_max(count, split, hi, lo):
+        assert split > 0 and hi >= 0 and lo >= 0 and count > 0
+        q = (count - 1) // split
+        best = q * hi + ((count - 1) % split) * lo
+        if q >= 1:
+            best = max(best, (q - 1) * hi + (split - 1) * lo)
+        return best
+
+    # ---- entry points
+    def __call__(self, name, *a):
+        self.calls.append(name)
+        getattr(self, name)(name, *a)
+        return 0
+
+    def ffno_lift(self, n, rows, x_in, w, out, st, stream):
+        self.span(n, "x_in", x_in, rows.B * 6 * rows.voxels * F32)
+        self.span(n, "w_lift", w, rows.C * 6 * F32)
+        self.act(n, "out", out, rows)
+        self.stats(n, st, rows)
+
+    def ffno_gn_finalize(self, n, B, nwg, C, part, gamma, beta, stats, a, b, stream):
+        self.span(n, "partials", part, B * nwg * 8 * 3 * F32)
+        self.span(n, "gamma", gamma, C * F32)
+        self.span(n, "beta", beta, C * F32)
+        self.span(n, "stats", stats, B * 8 * 3 * F32)
+        self.span(n, "aff_a", a, B * C * F32)
+        self.span(n, "aff_b", b, B * C * F32)
+
+    def ffno_block_in(self, n, rows, a, x_out, g1, g2, v1, v2, z_proj, z, xs, xv, stream):
+        C = rows.C
+        self.asm(n, a, rows)
+        self.act(n, "x_out", x_out, rows, optional=True)
+        for what, w in (("g1", g1), ("g2", g2), ("v1", v1), ("v2", v2)):
+            self.weight(n, what, w, C, C)
+        self.span(n, "z_proj", z_proj, C * F32)
+        self.span(n, "z", z, rows.B * rows.voxels * F32)
+        self.act(n, "xs", xs, rows)
+        self.act(n, "xv", xv, rows)
+
+    def ffno_post(self, n, rows, u, w1, w2, y, st, stream):
+        C = rows.C
+        self.act(n, "u", u, rows)
+        self.weight(n, "w1", w1, 2 * C, C)
+        self.weight(n, "w2", w2, C, 2 * C)
+        self.act(n, "y", y, rows)
+        self.stats(n, st, rows)
+
+    def ffno_fuse(self, n, rows, x, sp, vp, sa, sb, va, vb, bg1, bg2, bias, f1, f2, fp, st, stream):
+        C = rows.C
+        for what, p in (("x", x), ("sp", sp), ("vp", vp), ("fp", fp)):
+            self.act(n, what, p, rows)
+        for what, p in (("s_a", sa), ("s_b", sb), ("v_a", va), ("v_b", vb)):
+            self.span(n, what, p, rows.B * C * F32)
+        self.span(n, "bg1", bg1, 2 * C * 3 * round_up(C, 32) * BF)
+        self.weight(n, "bg2", bg2, 2 * C, C)
+        self.span(n, "bg2_bias", bias, 2 * C * F32)
+        self.span(n, "f1", f1, 2 * 2 * C * 2 * round_up(C, 32) * BF)
+        self.weight(n, "f2", f2, C, 2 * C)
+        self.stats(n, st, rows)
+
+    def ffno_resmlp(self, n, rows, a, x_out, w1, w2, y, st, stream):
+        C = rows.C
+        self.asm(n, a, rows)
+        self.act(n, "x_out", x_out, rows, optional=True)
+        self.weight(n, "w1", w1, 2 * C, C)
+        self.weight(n, "w2", w2, C, 2 * C)
+        self.act(n, "y", y, rows)
+        self.stats(n, st, rows)
+
+    def ffno_head(self, n, rows, a, x_out, p1, p2, bias, x_in, desc, y, stream):
+        d = desc._obj
+        n_head = sum(d.levels[s] + 1 for s in range(d.n_species))
+        n_out = sum(d.levels[s] for s in range(d.n_species))
+        nth = (n_head + 15) // 16
+        C = rows.C
+        self.asm(n, a, rows)
+        self.act(n, "x_out", x_out, rows, optional=True)
+        self.weight(n, "p1", p1, 2 * C, C)
+        self.span(n, "p2", p2, 2 * nth * 16 * 2 * C * BF)
+        self.span(n, "p2_bias", bias, nth * 16 * F32)
+        self.span(n, "x_in", x_in, rows.B * 6 * rows.voxels * F32)
+        self.span(n, "y", y, rows.B * n_out * rows.voxels * F32, optional=True)
+        assert x_out or y, "ffno_head: nothing to write"
+
+    def ffno_tw_gemm(self, n, gref, stream):
+        g = gref._obj
+        assert g.n % 4 == 0 and g.m_pad % 16 == 0 and g.k_pad % 32 == 0 and g.m <= g.m_pad and g.k <= g.k_pad
+        self.span(n, "a_hi", g.a_hi, g.m_pad * g.k_pad * BF)
+        self.span(n, "a_lo", g.a_lo, g.m_pad * g.k_pad * BF)
+        xb = self.imap_max(g.batch, g.b_split, g.x_b_hi, g.x_b_lo)
+        xk = self.imap_max(g.k, g.k_split, g.x_k_hi, g.x_k_lo)
+        self.span(n, "x", g.x, (xb + xk + g.n) * F32)
+        yb = self.imap_max(g.batch, g.b_split, g.y_b_hi, g.y_b_lo)
+        ym = self.imap_max(g.m, g.m_split, g.y_m_hi, g.y_m_lo)
+        self.span(n, "y", g.y, (yb + ym + g.n) * F32)
+
+    def ffno_mix(self, n, mref, stream):
+        m = mref._obj
+        C = m.C
+        KP = round_up(C, 32)
+        top = self.imap_max(m.rows, m.r_split, m.r_hi, m.r_lo) + (m.modes - 1) * m.mode_stride + C
+        self.span(n, "re", m.re, top * F32)
+        self.span(n, "im", m.im, top * F32)
+        self.span(n, "wc", m.wc, 2 * 2 * C * 2 * KP * BF)
+        self.span(n, "gate", m.gate, m.modes * 2 * F32)
+        if m.rank:
+            self.span(n, "v", m.v_p, m.modes * 2 * 16 * KP * BF)
+            self.span(n, "u", m.u_p, m.modes * 2 * C * 32 * BF)
+        self.span(n, "factor", m.factor, m.modes * F32, optional=True)
+
+    def ffno_gn_grad_stats(self, n, rows, gb, ubar, part, stream):
+        self.gnback(n, gb, rows)
+        self.act(n, "ubar", ubar, rows)
+        self.span(n, "partials", part, rows.B * rows.nwg * 8 * 2 * F32)
+
+    def ffno_gn_grad_finalize(self, n, B, nwg, part, sums, stream):
+        self.span(n, "partials", part, B * nwg * 8 * 2 * F32)
+        self.span(n, "sums", sums, B * 8 * 2 * F32)
+
+    def ffno_mlp_bwd(self, n, rows, ubar, gb, x, w1, w2t, w1t, residual, xbar, stream):
+        C = rows.C
+        self.act(n, "ubar", ubar, rows)
+        self.gnback(n, gb, rows)
+        assert gb.sums, "ffno_mlp_bwd: GroupNorm gradient sums missing"
+        self.asm(n, x, rows)
+        self.weight(n, "w1", w1, 2 * C, C)
+        self.weight(n, "w2t", w2t, 2 * C, C)
+        self.weight(n, "w1t", w1t, C, 2 * C)
+        self.act(n, "xbar", xbar, rows)
+
+    def ffno_fuse_bwd(self, n, rows, x1bar, gb, h, sp, vp, sa, sb, va, vb, bg1, bg2, bias, f1, f2t, f1t, bg2t, bg1t,
+                      hbar, sbar, vbar, stream):
+        C = rows.C
+        KP = round_up(C, 32)
+        for what, p in (("x1bar", x1bar), ("h", h), ("sp", sp), ("vp", vp), ("hbar", hbar), ("sbar", sbar),
+                        ("vbar", vbar)):
+            self.act(n, what, p, rows)
+        self.gnback(n, gb, rows)
+        for what, p in (("s_a", sa), ("s_b", sb), ("v_a", va), ("v_b", vb)):
+            self.span(n, what, p, rows.B * C * F32)
+        self.span(n, "bg1", bg1, 2 * C * 3 * KP * BF)
+        self.weight(n, "bg2", bg2, 2 * C, C)
+        self.span(n, "bg2_bias", bias, 2 * C * F32)
+        self.span(n, "f1", f1, 2 * 2 * C * 2 * KP * BF)
+        self.weight(n, "f2t", f2t, 2 * C, C)
+        self.span(n, "f1t", f1t, 2 * 2 * C * 2 * C * BF)
+        self.weight(n, "bg2t", bg2t, C, 2 * C)
+        self.span(n, "bg1t", bg1t, 2 * 3 * C * KP * BF)
+
+    def ffno_gates_bwd(self, n, rows, h, xsb, xvb, hin, g1, g2, g2t, g1t, v1, v2, v2t, v1t, z_proj, hout, zbar,
+                       stream):
+        C = rows.C
+        for what, p in (("h", h), ("xsbar", xsb), ("xvbar", xvb), ("hbar_in", hin), ("hbar_out", hout)):
+            self.act(n, what, p, rows)
+        for what, w in (("g1", g1), ("g2", g2), ("g2t", g2t), ("g1t", g1t), ("v1", v1), ("v2", v2), ("v2t", v2t),
+                        ("v1t", v1t)):
+            self.weight(n, what, w, C, C)
+        self.span(n, "z_proj", z_proj, C * F32)
+        self.span(n, "zbar", zbar, rows.B * rows.voxels * F32, optional=True)
+
+    def ffno_head_bwd(self, n, rows, h, p1, p2, p2t, p1t, bias, x_in, desc, ybar, hbar, xbar_in, stream):
+        d = desc._obj
+        n_head = sum(d.levels[s] + 1 for s in range(d.n_species))
+        n_out = sum(d.levels[s] for s in range(d.n_species))
+        nth = (n_head + 15) // 16
+        C = rows.C
+        self.act(n, "h", h, rows)
+        self.weight(n, "p1", p1, 2 * C, C)
+        self.span(n, "p2", p2, 2 * nth * 16 * 2 * C * BF)
+        self.weight(n, "p2t", p2t, 2 * C, n_head)
+        self.weight(n, "p1t", p1t, C, 2 * C)
+        self.span(n, "p2_bias", bias, nth * 16 * F32)
+        self.span(n, "x_in", x_in, rows.B * 6 * rows.voxels * F32)
+        self.span(n, "ybar", ybar, rows.B * n_out * rows.voxels * F32)
+        self.act(n, "hbar", hbar, rows)
+        self.span(n, "xbar_in", xbar_in, rows.B * 6 * rows.voxels * F32)
+
+    def ffno_dot_reduce(self, n, rows, D, a, delta, r, zbar, stream):
+        assert rows.voxels % D == 0 and delta in (-1, 0, 1)
+        self.act(n, "a", a, rows)          # a is read at row + delta * hw only when that level exists
+        self.act(n, "r", r, rows)
+        self.span(n, "zbar", zbar, rows.B * rows.voxels * F32)
+
+    def ffno_lift_bwd(self, n, rows, ubar, gb, w, xbar_in, stream):
+        self.act(n, "ubar", ubar, rows)
+        self.gnback(n, gb, rows)
+        self.span(n, "w_lift", w, rows.C * 6 * F32)
+        self.span(n, "xbar_in", xbar_in, rows.B * 6 * rows.voxels * F32)
+
+
+def dry_engine(model):
+    """An Engine over `model` (kept on the host) whose tensors live on the meta device and whose launches go to a
+    Checker instead of the library. Returns (engine, checker, restore)."""
+    reg = Registry()
+    chk = Checker(reg)
+    eng = E.Engine(model)
+    meta = torch.device("meta")
+    eng.device = meta
+    eng.weights = E.PackedWeights(model, meta)
+    eng.weights.adjoint()
+    eng.sync = lambda dev: None
+
+    def prepare(self, x, z, dx, dy, plan):
+        B = x.shape[0]
+        return meta, plan, x, z.reshape(B, -1)
+
+    saved = (E._p, _lib.call, E.Engine._stream, E.Engine._prepare)
+    E._p = lambda t: ctypes.c_void_p(reg.address(t))
+    _lib.call = chk
+    E.Engine._stream = staticmethod(lambda: ctypes.c_void_p(0))
+    E.Engine._prepare = prepare
+
+    def restore():
+        E._p, _lib.call, E.Engine._stream, E.Engine._prepare = saved
+    return eng, chk, restore
+
+
+def check_shape(model, B, D, H, W, stride=1, want_z=True, column=None):
+    """Run forward, the saving forward and the adjoint of `model` at [B, D, H, W] through the checker; returns the
+    number of launches checked."""
+    eng, chk, restore = dry_engine(model)
+    try:
+        col = np.linspace(14.0, -1.0, D) if column is None else np.asarray(column, np.float64)
+        zs = col - col.min()
+        span = float(col.max() - col.min())
+        plan = E.Plan(eng, B, D, H, W, zs, span, 48e3, 48e3)
+        meta = torch.device("meta")
+        x = torch.empty(B, 6, D, H, W, device=meta)
+        z = torch.empty(B, D, H, W, device=meta)
+        eng.forward(x, z, 48e3, 48e3, plan=plan)
+        y, saved = eng.forward_saved(x, z, 48e3, 48e3, plan=plan, stride=stride)
+        ybar = torch.empty_like(y)
+        eng.vjp(x, z, 48e3, 48e3, ybar, saved, want_z=want_z, plan=plan)
+    finally:
+        restore()
+    return len(chk.calls)

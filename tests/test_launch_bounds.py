@@ -1,0 +1,74 @@
+"""Every launch of the engine's forward, saving forward and adjoint stays inside its buffers, checked on the CPU
+(meta tensors, a model of each kernel's largest offset; tests/launch_bounds.py) at the shapes the GPU runs: the
+benchmark batch, serving tiles, odd small shapes, and the whole 504 x 504 x 256 volume at the checkpoint stride its
+adjoint uses."""
+import pytest
+
+from launch_bounds import check_shape
+
+
+def _model(width=64, layers=8, rank=8):
+    from ffno_amd import synthetic
+    from ffno_amd.atoms import species
+    return synthetic.random_model(0, in_channels=6, out_channels=12, width=width, n_layers=layers,
+                                  hidden_spectral=128, hidden_vertical=128, freq_mix_rank=rank,
+                                  z_reference_length=40.0, z_min_wavelength=1.28, z_feature_wavelength=0.3,
+                                  species=species(["H", "CA"]))
+
+
+@pytest.mark.parametrize("B,D,H,W,stride", [
+    (8, 128, 64, 64, 1),          # the benchmark step
+    (1, 256, 504, 504, 2),        # the whole volume
+    (1, 256, 504, 504, 1),
+    (4, 192, 256, 256, 1),        # largest serving tile, batched
+    (3, 96, 32, 32, 3),
+    (2, 7, 9, 10, 3),             # odd sizes: partial tiles everywhere
+])
+def test_engine_launches_stay_in_bounds(B, D, H, W, stride):
+    n = check_shape(_model(), B, D, H, W, stride=stride)
+    assert n > 100
+
+
+@pytest.mark.parametrize("width,rank", [(96, 8), (32, 0), (16, 4)])
+def test_other_widths_stay_in_bounds(width, rank):
+    assert check_shape(_model(width=width, layers=3, rank=rank), 2, 20, 12, 14, stride=2) > 50
+
+
+def test_checker_catches_a_short_buffer_and_covers_every_entry_point():
+    """A planted mistake -- the half-spectrum scratch one float short -- is reported, and the runs above exercise
+    every compute entry point the header declares."""
+    import re
+    from pathlib import Path
+    import torch
+    from ffno_amd import engine as E
+    import launch_bounds as LB
+    real = E.Engine._scratch
+
+    def short(self, plan, dev, adjoint=False):
+        scr = real(self, plan, dev, adjoint)
+        scr["s1"] = torch.empty(scr["s1"].numel() - 1, device=dev)
+        return scr
+
+    E.Engine._scratch = short
+    try:
+        with pytest.raises(AssertionError, match="s1|tensor"):
+            check_shape(_model(layers=1), 1, 8, 8, 8)
+    finally:
+        E.Engine._scratch = real
+    eng, chk, restore = LB.dry_engine(_model(layers=2))
+    restore()
+    header = (Path(__file__).resolve().parents[1] / "include" / "ffno_hip.h").read_text()
+    declared = set(re.findall(r"^int (ffno_\w+)\(", header, re.M)) - {"ffno_version"}
+    calls = set()
+    orig = LB.Checker.__call__
+
+    def record(self, name, *a):
+        calls.add(name)
+        return orig(self, name, *a)
+
+    LB.Checker.__call__ = record
+    try:
+        check_shape(_model(layers=2), 1, 12, 8, 8, stride=2)
+    finally:
+        LB.Checker.__call__ = orig
+    assert declared <= calls, f"entry points never checked: {sorted(declared - calls)}"
