@@ -192,3 +192,39 @@ def test_spectral_pipelines_match_fft():
     err = (outv.double().cpu() - ref).abs().max() / ref.abs().max()
     print("vertical rel err", float(err))
     assert err < 1e-4
+
+
+@pytest.mark.parametrize("chunk", [None, 3])
+def test_spectral_adjoints_are_transposes(chunk):
+    """<T x, y> = <x, T^T y> for the horizontal and vertical spectral maps as the engine runs them (forward and
+    adjoint launch sequences, the per-mode gates and low-rank terms included), with the horizontal planes in one
+    chunk and in chunks of 3: 1e-5 relative."""
+    from ffno_amd.engine import Engine
+    model = _small_model()
+    eng = Engine(model)
+    dev = torch.device(DEV)
+    eng.sync(dev)
+    B, D, H, W, C = 2, 10, 12, 9, model.width
+    z = torch.linspace(14.0, -1.0, D)[None, :, None, None].expand(B, D, H, W).contiguous().to(DEV)
+    plan = eng.plan(B, D, H, W, z, 48e3, 48e3)
+    if chunk:
+        plan.h_chunk = chunk
+    scr = eng._scratch(plan, dev)
+    tables = plan.tables(0)
+    g = torch.Generator(device=DEV).manual_seed(11)
+    x = torch.randn(B * D * H * W, C, device=DEV, generator=g)
+    y = torch.randn(B * D * H * W, C, device=DEV, generator=g)
+    tx = torch.empty_like(x)
+    ty = torch.empty_like(y)
+    eng.horizontal(plan, x, tx, 0, scr, tables)
+    eng.horizontal(plan, y, ty, 0, scr, tables, adjoint=True)
+    lhs, rhs = float((tx.double() * y.double()).sum()), float((x.double() * ty.double()).sum())
+    print(f"horizontal chunk {chunk}: {lhs:.8e} vs {rhs:.8e}", flush=True)
+    # a transpose that is wrong anywhere is off by about |Tx| |y| / sqrt(n) = 3e-3 |Tx| |y| here
+    assert abs(lhs - rhs) <= 1e-5 * float(tx.double().norm() * y.double().norm())
+    eng.vertical(plan, x, scr["v1"], tx, 0, tables)
+    eng.vertical(plan, y, scr["v1"], ty, 0, tables, adjoint=True)
+    lhs, rhs = float((tx.double() * y.double()).sum()), float((x.double() * ty.double()).sum())
+    print(f"vertical: {lhs:.8e} vs {rhs:.8e}", flush=True)
+    # a transpose that is wrong anywhere is off by about |Tx| |y| / sqrt(n) = 3e-3 |Tx| |y| here
+    assert abs(lhs - rhs) <= 1e-5 * float(tx.double().norm() * y.double().norm())
