@@ -546,7 +546,7 @@ extern "C" {
 int ffno_gn_grad_stats(FfnoRows rows, FfnoGnBack gn, const float* ubar, float* partials, hipStream_t stream) {
   if (!rows_ok(rows) || !gb_ok(gn) || !ubar || !partials) return fail("ffno_gn_grad_stats: bad arguments");
   dim3 grid(rows.nwg, rows.B);
-  FFNO_DISPATCH_CT(rows.C, (k_gn_grad_stats<CT, NWV><<<grid, NWV * 64, 0, stream>>>(rows, gn, ubar, partials)));
+  FFNO_DISPATCH_CT(rows.C, (k_gn_grad_stats<CT, nw_of<CT>()><<<grid, nw_of<CT>() * 64, 0, stream>>>(rows, gn, ubar, partials)));
   return check_launch("ffno_gn_grad_stats");
 }
 
@@ -565,9 +565,9 @@ int ffno_mlp_bwd(FfnoRows rows, const float* ubar, FfnoGnBack gn, FfnoAssemble x
   FFNO_DISPATCH_CT(rows.C, {
     constexpr int C = CT * 16, KP = ks_of(CT) * 32;
     const size_t lds = ((size_t)2 * C * KP * 2 + (size_t)C * 2 * C) * 2 * 2;
-    auto k = k_mlp_bwd<CT, NWV>;
+    auto k = k_mlp_bwd<CT, nw_of<CT>()>;
     if (set_lds(k, lds)) return -1;
-    k<<<grid, NWV * 64, lds, stream>>>(rows, ubar, gn, x, w1, w2t, w1t, residual, xbar);
+    k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, ubar, gn, x, w1, w2t, w1t, residual, xbar);
   });
   return check_launch("ffno_mlp_bwd");
 }
@@ -598,12 +598,12 @@ int ffno_gates_bwd(FfnoRows rows, const float* h, const float* xsbar, const floa
   FFNO_DISPATCH_CT(rows.C, {
     const size_t lds = 8 * (size_t)(CT * 16) * ks_of(CT) * 32 * 2 * 2;
     if (lds <= 160 * 1024) {
-      auto k = k_gates_bwd<CT, NWV, true>;
+      auto k = k_gates_bwd<CT, nw_of<CT>(), true>;
       if (set_lds(k, lds)) return -1;
-      k<<<grid, NWV * 64, lds, stream>>>(rows, h, xsbar, xvbar, hbar_in, g1, g2, g2t, g1t, v1, v2, v2t, v1t, z_proj,
+      k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, h, xsbar, xvbar, hbar_in, g1, g2, g2t, g1t, v1, v2, v2t, v1t, z_proj,
                                          hbar_out, zbar);
     } else {
-      k_gates_bwd<CT, NWV, false><<<grid, NWV * 64, 0, stream>>>(rows, h, xsbar, xvbar, hbar_in, g1, g2, g2t, g1t, v1,
+      k_gates_bwd<CT, nw_of<CT>(), false><<<grid, nw_of<CT>() * 64, 0, stream>>>(rows, h, xsbar, xvbar, hbar_in, g1, g2, g2t, g1t, v1,
                                                                  v2, v2t, v1t, z_proj, hbar_out, zbar);
     }
   });
@@ -628,15 +628,15 @@ int ffno_head_bwd(FfnoRows rows, const float* h, const uint16_t* p1, const uint1
     constexpr int C = CT * 16;
     const size_t w_bytes = ((size_t)2 * C * ks_of(CT) * 32 + (size_t)nth * 16 * 2 * C + (size_t)2 * C * 32 +
                             (size_t)C * 2 * C) * 2 * 2;
-    const size_t lds = w_bytes + (size_t)NWV * 16 * nth * 16 * sizeof(float);
+    const size_t lds = w_bytes + (size_t)nw_of<CT>() * 16 * nth * 16 * sizeof(float);
     if (nth == 1) {
-      auto k = k_head_bwd<CT, NWV, 1>;
+      auto k = k_head_bwd<CT, nw_of<CT>(), 1>;
       if (set_lds(k, lds)) return -1;
-      k<<<grid, NWV * 64, lds, stream>>>(rows, h, p1, p2, p2t, p1t, p2_bias, x_in, *desc, ybar, hbar, xbar_in);
+      k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, h, p1, p2, p2t, p1t, p2_bias, x_in, *desc, ybar, hbar, xbar_in);
     } else {
-      auto k = k_head_bwd<CT, NWV, 2>;
+      auto k = k_head_bwd<CT, nw_of<CT>(), 2>;
       if (set_lds(k, lds)) return -1;
-      k<<<grid, NWV * 64, lds, stream>>>(rows, h, p1, p2, p2t, p1t, p2_bias, x_in, *desc, ybar, hbar, xbar_in);
+      k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, h, p1, p2, p2t, p1t, p2_bias, x_in, *desc, ybar, hbar, xbar_in);
     }
   });
   return check_launch("ffno_head_bwd");
@@ -655,7 +655,7 @@ int ffno_lift_bwd(FfnoRows rows, const float* ubar, FfnoGnBack gn, const float* 
                   hipStream_t stream) {
   if (!rows_ok(rows) || !gb_ok(gn) || !gn.sums || !ubar || !w_lift || !xbar_in) return fail("ffno_lift_bwd: bad arguments");
   dim3 grid(rows.nwg, rows.B);
-  FFNO_DISPATCH_CT(rows.C, (k_lift_bwd<CT, NWV><<<grid, NWV * 64, 0, stream>>>(rows, ubar, gn, w_lift, xbar_in)));
+  FFNO_DISPATCH_CT(rows.C, (k_lift_bwd<CT, nw_of<CT>()><<<grid, nw_of<CT>() * 64, 0, stream>>>(rows, ubar, gn, w_lift, xbar_in)));
   return check_launch("ffno_lift_bwd");
 }
 

@@ -360,4 +360,12 @@ constexpr int NWV = 4;   // waves per workgroup of the voxel-row kernels
 template <int CT>
 constexpr size_t red_bytes() { return (size_t)NWV * 4 * CT * GnAcc<CT>::SUB * 3 * sizeof(float); }
 
+// waves per workgroup of the LDS-staged row kernels: 8 (two per SIMD sharing one copy of the staged weights) while
+// a wave's registers fit 256; wider channel counts keep 4
+template <int CT>
+constexpr int nw_of() { return CT <= 4 ? 8 : 4; }
+
+template <int CT, int NW>
+constexpr size_t red_bytes_nw() { return (size_t)NW * 4 * CT * GnAcc<CT>::SUB * 3 * sizeof(float); }
+
 }  // namespace ffno

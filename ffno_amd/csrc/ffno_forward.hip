@@ -526,7 +526,7 @@ int ffno_lift(FfnoRows rows, const float* x_in, const float* w_lift, float* out,
               hipStream_t stream) {
   if (!rows_ok(rows) || !x_in || !w_lift || !out) return fail("ffno_lift: bad arguments");
   dim3 grid(rows.nwg, rows.B);
-  FFNO_DISPATCH_CT(rows.C, (k_lift<CT, NWV><<<grid, NWV * 64, 0, stream>>>(rows, x_in, w_lift, out, stats)));
+  FFNO_DISPATCH_CT(rows.C, (k_lift<CT, nw_of<CT>()><<<grid, nw_of<CT>() * 64, 0, stream>>>(rows, x_in, w_lift, out, stats)));
   return check_launch("ffno_lift");
 }
 
@@ -546,9 +546,9 @@ int ffno_block_in(FfnoRows rows, FfnoAssemble in, float* x_out, const uint16_t* 
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {
     const size_t lds = 4 * (size_t)(CT * 16) * ks_of(CT) * 32 * 2 * 2;
-    auto k = k_block_in<CT, NWV>;
+    auto k = k_block_in<CT, nw_of<CT>()>;
     if (set_lds(k, lds)) return -1;
-    k<<<grid, NWV * 64, lds, stream>>>(rows, in, x_out, g1, g2, v1, v2, z_proj, z, xs, xv);
+    k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, in, x_out, g1, g2, v1, v2, z_proj, z, xs, xv);
   });
   return check_launch("ffno_block_in");
 }
@@ -559,10 +559,10 @@ int ffno_post(FfnoRows rows, const float* u, const uint16_t* w1, const uint16_t*
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {
     const size_t lds = ((size_t)2 * CT * 16 * ks_of(CT) * 32 + (size_t)CT * 16 * 2 * CT * 16) * 2 * 2 +
-                       red_bytes<CT>();
-    auto k = k_post<CT, NWV>;
+                       red_bytes_nw<CT, nw_of<CT>()>();
+    auto k = k_post<CT, nw_of<CT>()>;
     if (set_lds(k, lds)) return -1;
-    k<<<grid, NWV * 64, lds, stream>>>(rows, u, w1, w2, y, stats);
+    k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, u, w1, w2, y, stats);
   });
   return check_launch("ffno_post");
 }
@@ -577,11 +577,11 @@ int ffno_fuse(FfnoRows rows, const float* x, const float* sp, const float* vp, c
   FFNO_DISPATCH_CT(rows.C, {
     constexpr int C = CT * 16, KP = ks_of(CT) * 32;
     const size_t lds = ((size_t)C * 3 * KP + 2 * ((size_t)2 * C * KP + 2 * C * 2 * KP + (size_t)C * 2 * C)) * 2 +
-                       red_bytes<CT>();
+                       red_bytes_nw<CT, nw_of<CT>()>();
     if (lds <= 160 * 1024) {
-      auto k = k_fuse<CT, NWV, true>;
+      auto k = k_fuse<CT, nw_of<CT>(), true>;
       if (set_lds(k, lds)) return -1;
-      k<<<grid, NWV * 64, lds, stream>>>(rows, x, sp, vp, s_a, s_b, v_a, v_b, bg1, bg2, bg2_bias, f1, f2, fp, stats);
+      k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, x, sp, vp, s_a, s_b, v_a, v_b, bg1, bg2, bg2_bias, f1, f2, fp, stats);
     } else {
       auto k = k_fuse<CT, NWV, false>;
       k<<<grid, NWV * 64, red_bytes<CT>(), stream>>>(rows, x, sp, vp, s_a, s_b, v_a, v_b, bg1, bg2, bg2_bias, f1, f2,
@@ -597,10 +597,10 @@ int ffno_resmlp(FfnoRows rows, FfnoAssemble in, float* x_out, const uint16_t* w1
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {
     const size_t lds = ((size_t)2 * CT * 16 * ks_of(CT) * 32 + (size_t)CT * 16 * 2 * CT * 16) * 2 * 2 +
-                       red_bytes<CT>();
-    auto k = k_resmlp<CT, NWV>;
+                       red_bytes_nw<CT, nw_of<CT>()>();
+    auto k = k_resmlp<CT, nw_of<CT>()>;
     if (set_lds(k, lds)) return -1;
-    k<<<grid, NWV * 64, lds, stream>>>(rows, in, x_out, w1, w2, y, stats);
+    k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, in, x_out, w1, w2, y, stats);
   });
   return check_launch("ffno_resmlp");
 }
@@ -621,15 +621,15 @@ int ffno_head(FfnoRows rows, FfnoAssemble in, float* x_out, const uint16_t* p1, 
   const int nth = (n_head + 15) / 16;
   FFNO_DISPATCH_CT(rows.C, {
     const size_t w_bytes = ((size_t)2 * CT * 16 * ks_of(CT) * 32 + (size_t)nth * 16 * 2 * CT * 16) * 2 * 2;
-    const size_t lds = w_bytes + (size_t)NWV * 16 * nth * 16 * sizeof(float);
+    const size_t lds = w_bytes + (size_t)nw_of<CT>() * 16 * nth * 16 * sizeof(float);
     if (nth == 1) {
-      auto k = k_head<CT, NWV, 1>;
+      auto k = k_head<CT, nw_of<CT>(), 1>;
       if (set_lds(k, lds)) return -1;
-      k<<<grid, NWV * 64, lds, stream>>>(rows, in, x_out, p1, p2, p2_bias, x_in, *desc, y);
+      k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, in, x_out, p1, p2, p2_bias, x_in, *desc, y);
     } else {
-      auto k = k_head<CT, NWV, 2>;
+      auto k = k_head<CT, nw_of<CT>(), 2>;
       if (set_lds(k, lds)) return -1;
-      k<<<grid, NWV * 64, lds, stream>>>(rows, in, x_out, p1, p2, p2_bias, x_in, *desc, y);
+      k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, in, x_out, p1, p2, p2_bias, x_in, *desc, y);
     }
   });
   return check_launch("ffno_head");
