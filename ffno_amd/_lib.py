@@ -59,7 +59,7 @@ SIGNATURES = {
     "ffno_block_in": [FfnoRows, FfnoAssemble, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr],
     "ffno_post": [FfnoRows, c_ptr, c_ptr, c_ptr, c_ptr, FfnoStatsOut, c_ptr],
     "ffno_fuse": [FfnoRows, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr,
-                  FfnoStatsOut, c_ptr],
+                  FfnoStatsOut, c_ptr, c_ptr],
     "ffno_resmlp": [FfnoRows, FfnoAssemble, c_ptr, c_ptr, c_ptr, c_ptr, FfnoStatsOut, c_ptr],
     "ffno_head": [FfnoRows, FfnoAssemble, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, ctypes.POINTER(FfnoHeadDesc), c_ptr,
                   c_ptr],
@@ -69,6 +69,8 @@ SIGNATURES = {
     "ffno_gn_grad_finalize": [ctypes.c_int, ctypes.c_int, c_ptr, c_ptr, c_ptr],
     "ffno_mlp_bwd": [FfnoRows, c_ptr, FfnoGnBack, FfnoAssemble, c_ptr, c_ptr, c_ptr, ctypes.c_int, c_ptr, c_ptr],
     "ffno_fuse_bwd": [FfnoRows, c_ptr, FfnoGnBack] + [c_ptr] * 18 + [c_ptr],
+    "ffno_fuse_mlp_bwd": [FfnoRows, c_ptr, FfnoGnBack] + [c_ptr] * 12 + [c_ptr],
+    "ffno_fuse_gates_bwd": [FfnoRows] + [c_ptr] * 14 + [c_ptr],
     "ffno_gates_bwd": [FfnoRows] + [c_ptr] * 15 + [c_ptr],
     "ffno_head_bwd": [FfnoRows] + [c_ptr] * 7 + [ctypes.POINTER(FfnoHeadDesc)] + [c_ptr] * 3 + [c_ptr],
     "ffno_lift_bwd": [FfnoRows, c_ptr, FfnoGnBack, c_ptr, c_ptr, c_ptr],

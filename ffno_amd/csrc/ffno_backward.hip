@@ -287,6 +287,137 @@ __global__ __launch_bounds__(NW * 64) void k_fuse_bwd(FfnoRows rows, const float
 }
 
 // ---------------------------------------------------------------------------------------------------------------
+// branch fusion adjoint in two LDS-staged passes, with the gates the recomputed forward stored
+// ---------------------------------------------------------------------------------------------------------------
+// pass 1: fp_bar from the GroupNorm (GELU after it), then the fusion MLP's adjoint with its hidden layer recomputed:
+// [sm_bar, vm_bar] = 0.5 fp_bar (both halves) + F1^T (gelu'(F1 [s gs, v gv]) * F2^T fp_bar)
+template <int CT, int NW>
+__global__ __launch_bounds__(NW * 64) void k_fuse_mlp_bwd(FfnoRows rows, const float* __restrict__ x1bar, FfnoGnBack gf,
+                                                          const float* __restrict__ sp, const float* __restrict__ vp,
+                                                          const float* __restrict__ s_a, const float* __restrict__ s_b,
+                                                          const float* __restrict__ v_a, const float* __restrict__ v_b,
+                                                          const float* __restrict__ gates, const uint16_t* f1,
+                                                          const uint16_t* f2t, const uint16_t* f1t,
+                                                          float* __restrict__ smbar, float* __restrict__ vmbar) {
+  constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32;
+  extern __shared__ __align__(16) unsigned char smem[];
+  bf16* lds = reinterpret_cast<bf16*>(smem);
+  const Wm F1 = stage_w(lds, f1, 2 * C * 2 * KP), F1T = stage_w(lds, f1t, 2 * C * 2 * C);
+  const Wm F2T = global_w(f2t, 2 * C * KP);
+  __syncthreads();
+  Lane L;
+  const int b = blockIdx.y;
+  const float scale = gf.scale ? *gf.scale : 1.f;
+  const int64_t nt = (rows.voxels + 15) / 16;
+  for (int64_t tile = (int64_t)blockIdx.x * NW + L.wave; tile < nt; tile += (int64_t)gridDim.x * NW) {
+    const int64_t vox = tile * 16 + L.v;
+    const bool ok = vox < rows.voxels;
+    const int64_t row = (int64_t)b * rows.voxels + vox;
+    Frag mm[2 * KS];
+    {
+      f32x4 s[CT], v[CT], gt[2 * CT];
+      load_act<CT>(sp, row, ok, L.q, s);
+      load_act<CT>(vp, row, ok, L.q, v);
+      load_act<2 * CT>(gates, row, ok, L.q, gt);
+#pragma unroll
+      for (int t = 0; t < CT; ++t) {
+        const int c0 = b * C + 16 * t + 4 * L.q;
+        s[t] = ok ? gelu4(s[t] * *(const f32x4*)(s_a + c0) + *(const f32x4*)(s_b + c0)) * gt[t] : zero4();
+        v[t] = ok ? gelu4(v[t] * *(const f32x4*)(v_a + c0) + *(const f32x4*)(v_b + c0)) * gt[CT + t] : zero4();
+      }
+      to_frags<CT>(s, mm);
+      to_frags<CT>(v, mm + KS);
+    }
+    f32x4 mb[2 * CT];
+    Frag fbf[KS];
+    {
+      f32x4 x1b[CT], fb[CT];
+      load_act<CT>(x1bar, row, ok, L.q, x1b);
+      gn_back<CT>(gf, scale, row, b, ok, L.q, x1b, fb);
+#pragma unroll
+      for (int t = 0; t < CT; ++t) {
+        mb[t] = 0.5f * fb[t];
+        mb[CT + t] = 0.5f * fb[t];
+      }
+      to_frags<CT>(fb, fbf);
+    }
+    mlp2_bwd<2 * KS, 2 * CT, KS, 2 * CT>(F1, F2T, F1T, mm, fbf, mb, L.v, L.q);
+    store_act<CT>(smbar, row, ok, L.q, *reinterpret_cast<f32x4(*)[CT]>(mb));
+    store_act<CT>(vmbar, row, ok, L.q, *reinterpret_cast<f32x4(*)[CT]>(mb + CT));
+  }
+}
+
+// pass 2: through the gates and the gate network (hidden layer recomputed):
+// gp = [sm_bar s, vm_bar v] gt (1 - gt); [h_bar', s_bar', v_bar'] = B1^T (gelu'(B1 [h, s, v]) * B2^T gp);
+// h_bar = x1_bar + h_bar', s_bar = sm_bar gs + s_bar', v_bar = vm_bar gv + v_bar'. In place: hbar over x1bar,
+// sbar over smbar, vbar over vmbar.
+template <int CT, int NW>
+__global__ __launch_bounds__(NW * 64) void k_fuse_gates_bwd(FfnoRows rows, float* xh_bar, const float* __restrict__ h,
+                                                            const float* __restrict__ sp, const float* __restrict__ vp,
+                                                            const float* __restrict__ s_a, const float* __restrict__ s_b,
+                                                            const float* __restrict__ v_a, const float* __restrict__ v_b,
+                                                            const float* __restrict__ gates, const uint16_t* bg1,
+                                                            const uint16_t* bg2t, const uint16_t* bg1t, float* sv_s,
+                                                            float* sv_v) {
+  constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32;
+  extern __shared__ __align__(16) unsigned char smem[];
+  bf16* lds = reinterpret_cast<bf16*>(smem);
+  const Wm B1 = stage_w(lds, bg1, C * 3 * KP), B2T = stage_w(lds, bg2t, C * 2 * C), B1T = stage_w(lds, bg1t, 3 * C * KP);
+  __syncthreads();
+  Lane L;
+  const int b = blockIdx.y;
+  const int64_t nt = (rows.voxels + 15) / 16;
+  for (int64_t tile = (int64_t)blockIdx.x * NW + L.wave; tile < nt; tile += (int64_t)gridDim.x * NW) {
+    const int64_t vox = tile * 16 + L.v;
+    const bool ok = vox < rows.voxels;
+    const int64_t row = (int64_t)b * rows.voxels + vox;
+    Frag cat[3 * KS];
+    f32x4 s[CT], v[CT];
+    {
+      f32x4 x[CT];
+      load_act<CT>(h, row, ok, L.q, x);
+      to_frags<CT>(x, cat);
+    }
+    load_act<CT>(sp, row, ok, L.q, s);
+    load_act<CT>(vp, row, ok, L.q, v);
+#pragma unroll
+    for (int t = 0; t < CT; ++t) {
+      const int c0 = b * C + 16 * t + 4 * L.q;
+      s[t] = ok ? gelu4(s[t] * *(const f32x4*)(s_a + c0) + *(const f32x4*)(s_b + c0)) : zero4();
+      v[t] = ok ? gelu4(v[t] * *(const f32x4*)(v_a + c0) + *(const f32x4*)(v_b + c0)) : zero4();
+    }
+    to_frags<CT>(s, cat + KS);
+    to_frags<CT>(v, cat + 2 * KS);
+    // cb = [h_bar', sm_bar gs, vm_bar gv], then += B1^T (...): the gate network's adjoint accumulates into it
+    Frag gpf[CT];
+    f32x4 cb[3 * CT];
+    {
+      f32x4 gt[2 * CT], gp[2 * CT];
+      load_act<2 * CT>(gates, row, ok, L.q, gt);
+      load_act<CT>(sv_s, row, ok, L.q, *reinterpret_cast<f32x4(*)[CT]>(cb + CT));
+      load_act<CT>(sv_v, row, ok, L.q, *reinterpret_cast<f32x4(*)[CT]>(cb + 2 * CT));
+#pragma unroll
+      for (int t = 0; t < CT; ++t) {
+        gp[t] = cb[CT + t] * s[t] * gt[t] * (1.f - gt[t]);
+        gp[CT + t] = cb[2 * CT + t] * v[t] * gt[CT + t] * (1.f - gt[CT + t]);
+        cb[CT + t] *= gt[t];
+        cb[2 * CT + t] *= gt[CT + t];
+        cb[t] = zero4();
+      }
+      to_frags<2 * CT>(gp, gpf);
+    }
+    mlp2_bwd<3 * KS, CT, CT, 3 * CT>(B1, B2T, B1T, cat, gpf, cb, L.v, L.q);
+    f32x4 x1b[CT];
+    load_act<CT>(xh_bar, row, ok, L.q, x1b);
+#pragma unroll
+    for (int t = 0; t < CT; ++t) x1b[t] += cb[t];
+    store_act<CT>(xh_bar, row, ok, L.q, x1b);
+    store_act<CT>(sv_s, row, ok, L.q, *reinterpret_cast<f32x4(*)[CT]>(cb + CT));
+    store_act<CT>(sv_v, row, ok, L.q, *reinterpret_cast<f32x4(*)[CT]>(cb + 2 * CT));
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
 // input-gate adjoint
 // ---------------------------------------------------------------------------------------------------------------
 // one gate xo = h * sigmoid(A2 gelu(A1 h)): hb += xb * sig + A1^T (gelu'(A1 h) * A2^T (xb * h * sig (1 - sig)))
@@ -585,6 +716,46 @@ int ffno_fuse_bwd(FfnoRows rows, const float* x1bar, FfnoGnBack gnf, const float
                                rows, x1bar, gnf, h, sp, vp, s_a, s_b, v_a, v_b, bg1, bg2, bg2_bias, f1, f2t, f1t, bg2t,
                                bg1t, hbar, sbar, vbar)));
   return check_launch("ffno_fuse_bwd");
+}
+
+int ffno_fuse_mlp_bwd(FfnoRows rows, const float* x1bar, FfnoGnBack gnf, const float* sp, const float* vp,
+                      const float* s_a, const float* s_b, const float* v_a, const float* v_b, const float* gates,
+                      const uint16_t* f1, const uint16_t* f2t, const uint16_t* f1t, float* smbar, float* vmbar,
+                      hipStream_t stream) {
+  if (!rows_ok(rows) || !gb_ok(gnf) || !gnf.sums || !x1bar || !sp || !vp || !s_a || !s_b || !v_a || !v_b || !gates ||
+      !f1 || !f2t || !f1t || !smbar || !vmbar)
+    return fail("ffno_fuse_mlp_bwd: bad arguments");
+  dim3 grid(rows.nwg, rows.B);
+  FFNO_DISPATCH_CT(rows.C, {
+    constexpr int C = CT * 16, KP = ks_of(CT) * 32;
+    const size_t lds = ((size_t)2 * C * 2 * KP + (size_t)2 * C * 2 * C) * 2 * 2;
+    if (lds > 160 * 1024) return fail("ffno_fuse_mlp_bwd: width too large for the LDS-staged pass");
+    auto k = k_fuse_mlp_bwd<CT, nw_of<CT>()>;
+    if (set_lds(k, lds)) return -1;
+    k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, x1bar, gnf, sp, vp, s_a, s_b, v_a, v_b, gates, f1, f2t, f1t, smbar,
+                                               vmbar);
+  });
+  return check_launch("ffno_fuse_mlp_bwd");
+}
+
+int ffno_fuse_gates_bwd(FfnoRows rows, float* x1bar_hbar, const float* h, const float* sp, const float* vp,
+                        const float* s_a, const float* s_b, const float* v_a, const float* v_b, const float* gates,
+                        const uint16_t* bg1, const uint16_t* bg2t, const uint16_t* bg1t, float* sm_sbar, float* vm_vbar,
+                        hipStream_t stream) {
+  if (!rows_ok(rows) || !x1bar_hbar || !h || !sp || !vp || !s_a || !s_b || !v_a || !v_b || !gates || !bg1 || !bg2t ||
+      !bg1t || !sm_sbar || !vm_vbar)
+    return fail("ffno_fuse_gates_bwd: bad arguments");
+  dim3 grid(rows.nwg, rows.B);
+  FFNO_DISPATCH_CT(rows.C, {
+    constexpr int C = CT * 16, KP = ks_of(CT) * 32;
+    const size_t lds = ((size_t)C * 3 * KP + (size_t)C * 2 * C + (size_t)3 * C * KP) * 2 * 2;
+    if (lds > 160 * 1024) return fail("ffno_fuse_gates_bwd: width too large for the LDS-staged pass");
+    auto k = k_fuse_gates_bwd<CT, nw_of<CT>()>;
+    if (set_lds(k, lds)) return -1;
+    k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, x1bar_hbar, h, sp, vp, s_a, s_b, v_a, v_b, gates, bg1, bg2t, bg1t,
+                                               sm_sbar, vm_vbar);
+  });
+  return check_launch("ffno_fuse_gates_bwd");
 }
 
 int ffno_gates_bwd(FfnoRows rows, const float* h, const float* xsbar, const float* xvbar, const float* hbar_in,

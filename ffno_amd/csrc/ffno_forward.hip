@@ -189,7 +189,8 @@ __global__ __launch_bounds__(NW * 64) void k_fuse(FfnoRows rows, const float* __
                                                   const float* __restrict__ v_a, const float* __restrict__ v_b,
                                                   const uint16_t* bg1, const uint16_t* bg2,
                                                   const float* __restrict__ bg2_bias, const uint16_t* f1,
-                                                  const uint16_t* f2, float* fp, FfnoStatsOut st) {
+                                                  const uint16_t* f2, float* fp, FfnoStatsOut st,
+                                                  float* __restrict__ gates_out) {
   constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32;
   constexpr int N1 = C * 3 * KP, N2 = 2 * C * KP, N3 = 2 * C * 2 * KP, N4 = C * 2 * C;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -239,9 +240,12 @@ __global__ __launch_bounds__(NW * 64) void k_fuse(FfnoRows rows, const float* __
     f32x4 g[2 * CT];
     mlp2<3 * KS, CT, 2 * CT>(B1, B2, cat, g, L.v, L.q);
 #pragma unroll
+    for (int t = 0; t < 2 * CT; ++t) g[t] = sigm4(g[t] + *(const f32x4*)(bg2_bias + 16 * t + 4 * L.q));
+    if (gates_out) store_act<2 * CT>(gates_out, row, ok, L.q, g);
+#pragma unroll
     for (int t = 0; t < CT; ++t) {
-      s[t] = s[t] * sigm4(g[t] + *(const f32x4*)(bg2_bias + 16 * t + 4 * L.q));
-      v[t] = v[t] * sigm4(g[CT + t] + *(const f32x4*)(bg2_bias + C + 16 * t + 4 * L.q));
+      s[t] = s[t] * g[t];
+      v[t] = v[t] * g[CT + t];
     }
     to_frags<CT>(s, cat);
     to_frags<CT>(v, cat + KS);
@@ -569,7 +573,8 @@ int ffno_post(FfnoRows rows, const float* u, const uint16_t* w1, const uint16_t*
 
 int ffno_fuse(FfnoRows rows, const float* x, const float* sp, const float* vp, const float* s_a, const float* s_b,
               const float* v_a, const float* v_b, const uint16_t* bg1, const uint16_t* bg2, const float* bg2_bias,
-              const uint16_t* f1, const uint16_t* f2, float* fp, FfnoStatsOut stats, hipStream_t stream) {
+              const uint16_t* f1, const uint16_t* f2, float* fp, FfnoStatsOut stats, float* gates_out,
+              hipStream_t stream) {
   if (!rows_ok(rows) || !x || !sp || !vp || !s_a || !s_b || !v_a || !v_b || !bg1 || !bg2 || !bg2_bias || !f1 ||
       !f2 || !fp)
     return fail("ffno_fuse: bad arguments");
@@ -581,11 +586,12 @@ int ffno_fuse(FfnoRows rows, const float* x, const float* sp, const float* vp, c
     if (lds <= 160 * 1024) {
       auto k = k_fuse<CT, nw_of<CT>(), true>;
       if (set_lds(k, lds)) return -1;
-      k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, x, sp, vp, s_a, s_b, v_a, v_b, bg1, bg2, bg2_bias, f1, f2, fp, stats);
+      k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, x, sp, vp, s_a, s_b, v_a, v_b, bg1, bg2, bg2_bias, f1, f2, fp, stats,
+                                                 gates_out);
     } else {
       auto k = k_fuse<CT, NWV, false>;
       k<<<grid, NWV * 64, red_bytes<CT>(), stream>>>(rows, x, sp, vp, s_a, s_b, v_a, v_b, bg1, bg2, bg2_bias, f1, f2,
-                                                      fp, stats);
+                                                      fp, stats, gates_out);
     }
   });
   return check_launch("ffno_fuse");

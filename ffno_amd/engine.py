@@ -23,6 +23,7 @@ NW = 4                       # waves per workgroup of the voxel-row kernels
 TABLE_CACHE_BYTES = 1 << 30  # per-mode tables of all blocks at most this large stay cached in the plan
 H_CHUNK_BYTES = 2 << 30      # horizontal half-spectrum scratch per buffer: (b, d) planes are transformed in chunks
 WORKING_UNITS = 9            # activation-sized buffers of the adjoint besides the saved block inputs
+GATE_UNITS = 2               # the stored branch gates of the two-pass fusion adjoint
 
 # MULTI3D constants in cgs (models/ffno_model.py:111-113)
 _EE, _HH, _EM, _BK, _PI = 1.602189e-12, 6.626176e-27, 9.109534e-28, 1.380662e-16, 3.14159265359
@@ -313,18 +314,20 @@ class Plan:
         v1 = self.B * 2 * self.modes * self.H * self.W * self.C * 4
         return 2 * self.h_chunk * self.plane * 4 + 2 * v1 + self.B * self.voxels * 4 * 32
 
-    def vjp_bytes(self, stride):
+    def vjp_bytes(self, stride, two_pass=False):
         saved = -(-self.n_layers // stride) + (1 if stride == 1 else 0) + (stride - 1)
         tables = 0 if self._cache_tables else 2 * self.table_bytes // self.n_layers
-        return self.unit_bytes() * (saved + WORKING_UNITS) + self.scratch_bytes() + tables
+        units = saved + WORKING_UNITS + (GATE_UNITS if two_pass else 0)
+        return self.unit_bytes() * units + self.scratch_bytes() + tables
 
 
 class Saved:
     """What the forward keeps for the adjoint: the inputs of blocks 0, s, 2s, ... (s = stride) and, with s = 1, the
     head's input. The adjoint recomputes the inputs of the other blocks from the nearest kept one."""
 
-    def __init__(self, stride=1):
+    def __init__(self, stride=1, two_pass=False):
         self.stride = int(stride)
+        self.two_pass = bool(two_pass)     # fusion adjoint in two LDS-staged passes over stored gates
         self.inputs = {}
         self.head = None
 
@@ -348,6 +351,7 @@ class Engine:
         self._plans = {}
         self._lock = threading.RLock()      # the serving workers share one engine
         self.checkpoint_stride = None       # None: the smallest stride whose adjoint fits the free HBM
+        self.two_pass_fuse = None           # None: whenever its gate buffer fits at stride 1
         self.last_stride = None
 
     def footprint_bytes(self, B, D, H, W, vjp=False):
@@ -360,7 +364,7 @@ class Engine:
         chunk = max(1, min(B * D, H_CHUNK_BYTES // plane)) * plane
         modes = int(math.floor(self.module.z_reference_length / self.module.z_min_wavelength)) + 1
         v1 = B * 2 * modes * H * W * C * 4
-        units = (L + 1) + WORKING_UNITS if vjp else 3
+        units = (L + 1) + WORKING_UNITS + GATE_UNITS if vjp else 3
         return unit * units + 2 * chunk + (2 if vjp else 1) * v1 + B * D * H * W * 4 * 32 + (64 << 20)
 
     def _param_version(self):
@@ -400,16 +404,23 @@ class Engine:
                 self._plans[key] = p
         return p
 
-    def choose_stride(self, plan):
-        """Checkpoint stride of the adjoint: the configured one, else the smallest whose footprint fits the HBM that
-        is free now (the caching allocator's unused reserve included)."""
-        if self.checkpoint_stride:
-            return int(self.checkpoint_stride)
+    def choose_checkpoints(self, plan, stride=None):
+        """(stride, two_pass) of the adjoint: the configured (or given) stride, else the smallest whose footprint
+        fits the HBM that is free now (the caching allocator's unused reserve included); the two-pass fusion
+        adjoint at stride 1 when its gate buffer fits too."""
+        stride = stride or self.checkpoint_stride
+        if stride and self.two_pass_fuse is not None:
+            return int(stride), bool(self.two_pass_fuse) and int(stride) == 1
         free, _ = torch.cuda.mem_get_info(plan.device)
         free += torch.cuda.memory_reserved(plan.device) - torch.cuda.memory_allocated(plan.device)
+        budget = 0.97 * free
+        if stride:
+            return int(stride), int(stride) == 1 and plan.vjp_bytes(1, True) <= budget
+        if self.two_pass_fuse is not False and plan.vjp_bytes(1, True) <= budget:
+            return 1, True
         for s in range(1, plan.n_layers + 1):
-            if plan.vjp_bytes(s) <= 0.97 * free:
-                return s
+            if plan.vjp_bytes(s) <= budget:
+                return s, False
         raise MemoryError(f"the adjoint of a [{plan.B}, {plan.D}, {plan.H}, {plan.W}] request needs "
                           f"{plan.vjp_bytes(plan.n_layers) / 2**30:.1f} GiB even with one saved input")
 
@@ -538,8 +549,9 @@ class Engine:
         ma, mb, _ = gn["m"]
         return _asm(hb, s, ma, mb, blk["res_mlp"]), gn
 
-    def _tail(self, plan, i, h, sp, vp, fp, x1, pp, x2, mp, scr, gn):
-        """fuse -> pw -> mlp of block i (models/ffno_model.py:524-551), buffers as named (x2 may be None)."""
+    def _tail(self, plan, i, h, sp, vp, fp, x1, pp, x2, mp, scr, gn, gates=None):
+        """fuse -> pw -> mlp of block i (models/ffno_model.py:524-551), buffers as named (x2 may be None; gates, if
+        given, receives the branch gates)."""
         dev = sp.device
         blk = self.weights.blocks[i]
         st = self._stream()
@@ -548,7 +560,7 @@ class Engine:
         sa, sb, _ = gn["s"]
         va, vb, _ = gn["v"]
         _lib.call("ffno_fuse", rows, _p(h), _p(sp), _p(vp), _p(sa), _p(sb), _p(va), _p(vb), _p(blk["bg1"]),
-                  _p(blk["bg2"]), _p(blk["bg2_bias"]), _p(blk["f1"]), _p(blk["f2"]), _p(fp), stats, st)
+                  _p(blk["bg2"]), _p(blk["bg2_bias"]), _p(blk["f1"]), _p(blk["f2"]), _p(fp), stats, _p(gates), st)
         gn["f"] = self._gn(plan, scr["part"], blk["norm_fuse"], dev)
         fa, fb, _ = gn["f"]
         _lib.call("ffno_resmlp", rows, _asm(h, fp, fa, fb, blk["res_fused"], 1), _p(x1), _p(blk["pw1"]),
@@ -605,7 +617,7 @@ class Engine:
         """(y, saved): the forward that keeps what the adjoint needs, at the given checkpoint stride or the one that
         fits the free HBM."""
         dev, plan, _, _ = self._prepare(x, z, dx, dy, plan)
-        saved = Saved(stride or self.choose_stride(plan))
+        saved = Saved(*self.choose_checkpoints(plan, stride))
         self.last_stride = saved.stride          # informational (the stride of the latest saving forward)
         return self.forward(x, z, dx, dy, plan=plan, saved=saved), saved
 
@@ -645,6 +657,7 @@ class Engine:
         f32 = dict(device=dev, dtype=torch.float32)
         g, s, v, sp, vp, fp, pp, mp, x1 = (torch.empty(N, C, **f32) for _ in range(WORKING_UNITS))
         tbuf = torch.empty(N, C, **f32) if stride > 1 else None
+        gates = torch.empty(N, 2 * C, **f32) if saved.two_pass else None
         scr = self._scratch(plan, dev, adjoint=True)
         part2 = scr["part2"]
         ones = torch.ones(B, C, **f32)
@@ -701,7 +714,7 @@ class Engine:
             _lib.call("ffno_post", rows, _p(v), _p(blk["vertical.p1"]), _p(blk["vertical.p2"]), _p(vp), stats, st)
             gn["v"] = self._gn(plan, scr["part"], blk["norm_vert"], dev)
             last_unsaved = i == L - 1 and saved.head is None
-            self._tail(plan, i, h, sp, vp, fp, x1, pp, g if last_unsaved else None, mp, scr, gn)
+            self._tail(plan, i, h, sp, vp, fp, x1, pp, g if last_unsaved else None, mp, scr, gn, gates)
             pa, pb, _ = gn["p"]
             if last_unsaved:
                 # the head input x2 + res_mlp GN(mp), assembled in g by the forward head's own code (bit for bit what
@@ -725,9 +738,16 @@ class Engine:
             self._gn_sums(plan, gb, g, part2)
             sa, sb_, _ = gn["s"]
             va, vb, _ = gn["v"]
-            _lib.call("ffno_fuse_bwd", rows, _p(g), gb, _p(h), _p(sp), _p(vp), _p(sa), _p(sb_), _p(va), _p(vb),
-                      _p(blk["bg1"]), _p(blk["bg2"]), _p(blk["bg2_bias"]), _p(blk["f1"]), _p(adj["f2t"]),
-                      _p(adj["f1t"]), _p(adj["bg2t"]), _p(adj["bg1t"]), _p(g), _p(mp), _p(pp), st)
+            if gates is not None:
+                # two LDS-staged passes over the stored gates: [sm_bar, vm_bar] -> mp, pp; then in place
+                _lib.call("ffno_fuse_mlp_bwd", rows, _p(g), gb, _p(sp), _p(vp), _p(sa), _p(sb_), _p(va), _p(vb),
+                          _p(gates), _p(blk["f1"]), _p(adj["f2t"]), _p(adj["f1t"]), _p(mp), _p(pp), st)
+                _lib.call("ffno_fuse_gates_bwd", rows, _p(g), _p(h), _p(sp), _p(vp), _p(sa), _p(sb_), _p(va), _p(vb),
+                          _p(gates), _p(blk["bg1"]), _p(adj["bg2t"]), _p(adj["bg1t"]), _p(mp), _p(pp), st)
+            else:
+                _lib.call("ffno_fuse_bwd", rows, _p(g), gb, _p(h), _p(sp), _p(vp), _p(sa), _p(sb_), _p(va), _p(vb),
+                          _p(blk["bg1"]), _p(blk["bg2"]), _p(blk["bg2_bias"]), _p(blk["f1"]), _p(adj["f2t"]),
+                          _p(adj["f1t"]), _p(adj["bg2t"]), _p(adj["bg1t"]), _p(g), _p(mp), _p(pp), st)
             # horizontal branch: spec = gelu(GN_s(post(Spec(xs)))); ys_bar -> s, xs_bar -> mp
             gb = gnback(sp, gn["s"], blk["norm_spec"], None, 1)
             self._gn_sums(plan, gb, mp, part2)

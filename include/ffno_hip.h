@@ -79,7 +79,8 @@ int ffno_post(FfnoRows rows, const float* u, const uint16_t* w1, const uint16_t*
  * fp may alias sp. */
 int ffno_fuse(FfnoRows rows, const float* x, const float* sp, const float* vp, const float* s_a, const float* s_b,
               const float* v_a, const float* v_b, const uint16_t* bg1, const uint16_t* bg2, const float* bg2_bias,
-              const uint16_t* f1, const uint16_t* f2, float* fp, FfnoStatsOut stats, hipStream_t stream);
+              const uint16_t* f1, const uint16_t* f2, float* fp, FfnoStatsOut stats,
+              float* gates_out /* optional [N, 2C]: (gs, gv) for the two-pass adjoint */, hipStream_t stream);
 
 /* Residual pointwise path (models/ffno_model.py:545-551): x_out = assemble(...); y = W2 gelu(W1 x_out); statistics
  * of y. Used for pw (assemble = x + res_fused gelu(GN(fp))) and mlp (assemble = x1 + res_pw GN(pp)). x_out may be
@@ -192,6 +193,19 @@ int ffno_fuse_bwd(FfnoRows rows, const float* x1bar, FfnoGnBack gnf, const float
                   const uint16_t* bg2, const float* bg2_bias, const uint16_t* f1, const uint16_t* f2t,
                   const uint16_t* f1t, const uint16_t* bg2t, const uint16_t* bg1t, float* hbar, float* sbar,
                   float* vbar, hipStream_t stream);
+
+/* The same adjoint in two passes whose weights fit LDS, given the gates the recomputed ffno_fuse stored
+ * (models/ffno_model.py:524-545). Pass 1: fp_bar = GN backward of x1bar (GELU after it), then
+ * [smbar, vmbar] = 0.5 fp_bar + F1^T (gelu'(F1 [s gs, v gv]) * F2^T fp_bar). Pass 2 (in place): through the gates
+ * and the gate network, x1bar -> hbar, smbar -> sbar, vmbar -> vbar. */
+int ffno_fuse_mlp_bwd(FfnoRows rows, const float* x1bar, FfnoGnBack gnf, const float* sp, const float* vp,
+                      const float* s_a, const float* s_b, const float* v_a, const float* v_b, const float* gates,
+                      const uint16_t* f1, const uint16_t* f2t, const uint16_t* f1t, float* smbar, float* vmbar,
+                      hipStream_t stream);
+int ffno_fuse_gates_bwd(FfnoRows rows, float* x1bar_hbar, const float* h, const float* sp, const float* vp,
+                        const float* s_a, const float* s_b, const float* v_a, const float* v_b, const float* gates,
+                        const uint16_t* bg1, const uint16_t* bg2t, const uint16_t* bg1t, float* sm_sbar, float* vm_vbar,
+                        hipStream_t stream);
 
 /* Adjoint of both input gates (models/ffno_model.py:266, :439-440): hbar_out = hbar_in + d(xs)/dh^T xs_bar +
  * d(xv)/dh^T xv_bar; zbar (optional, accumulated) += sum_c xv_bar * z_proj. hbar_out may alias hbar_in. */

@@ -138,7 +138,7 @@ class Checker:
         self.act(n, "y", y, rows)
         self.stats(n, st, rows)
 
-    def ffno_fuse(self, n, rows, x, sp, vp, sa, sb, va, vb, bg1, bg2, bias, f1, f2, fp, st, stream):
+    def ffno_fuse(self, n, rows, x, sp, vp, sa, sb, va, vb, bg1, bg2, bias, f1, f2, fp, st, gates, stream):
         C = rows.C
         for what, p in (("x", x), ("sp", sp), ("vp", vp), ("fp", fp)):
             self.act(n, what, p, rows)
@@ -150,6 +150,7 @@ class Checker:
         self.span(n, "f1", f1, 2 * 2 * C * 2 * round_up(C, 32) * BF)
         self.weight(n, "f2", f2, C, 2 * C)
         self.stats(n, st, rows)
+        self.span(n, "gates_out", gates, rows.B * rows.voxels * 2 * C * F32, optional=True)
 
     def ffno_resmlp(self, n, rows, a, x_out, w1, w2, y, st, stream):
         C = rows.C
@@ -240,6 +241,33 @@ class Checker:
         self.weight(n, "bg2t", bg2t, C, 2 * C)
         self.span(n, "bg1t", bg1t, 2 * 3 * C * KP * BF)
 
+    def ffno_fuse_mlp_bwd(self, n, rows, x1bar, gb, sp, vp, sa, sb, va, vb, gates, f1, f2t, f1t, smbar, vmbar, stream):
+        C = rows.C
+        KP = round_up(C, 32)
+        for what, p in (("x1bar", x1bar), ("sp", sp), ("vp", vp), ("smbar", smbar), ("vmbar", vmbar)):
+            self.act(n, what, p, rows)
+        self.gnback(n, gb, rows)
+        assert gb.sums, "ffno_fuse_mlp_bwd: GroupNorm gradient sums missing"
+        for what, p in (("s_a", sa), ("s_b", sb), ("v_a", va), ("v_b", vb)):
+            self.span(n, what, p, rows.B * C * F32)
+        self.span(n, "gates", gates, rows.B * rows.voxels * 2 * C * F32)
+        self.span(n, "f1", f1, 2 * 2 * C * 2 * KP * BF)
+        self.weight(n, "f2t", f2t, 2 * C, C)
+        self.span(n, "f1t", f1t, 2 * 2 * C * 2 * C * BF)
+
+    def ffno_fuse_gates_bwd(self, n, rows, xh, h, sp, vp, sa, sb, va, vb, gates, bg1, bg2t, bg1t, svs, svv, stream):
+        C = rows.C
+        KP = round_up(C, 32)
+        for what, p in (("x1bar/hbar", xh), ("h", h), ("sp", sp), ("vp", vp), ("smbar/sbar", svs),
+                        ("vmbar/vbar", svv)):
+            self.act(n, what, p, rows)
+        for what, p in (("s_a", sa), ("s_b", sb), ("v_a", va), ("v_b", vb)):
+            self.span(n, what, p, rows.B * C * F32)
+        self.span(n, "gates", gates, rows.B * rows.voxels * 2 * C * F32)
+        self.span(n, "bg1", bg1, 2 * C * 3 * KP * BF)
+        self.weight(n, "bg2t", bg2t, C, 2 * C)
+        self.span(n, "bg1t", bg1t, 2 * 3 * C * KP * BF)
+
     def ffno_gates_bwd(self, n, rows, h, xsb, xvb, hin, g1, g2, g2t, g1t, v1, v2, v2t, v1t, z_proj, hout, zbar,
                        stream):
         C = rows.C
@@ -308,10 +336,11 @@ def dry_engine(model):
     return eng, chk, restore
 
 
-def check_shape(model, B, D, H, W, stride=1, want_z=True, column=None):
-    """Run forward, the saving forward and the adjoint of `model` at [B, D, H, W] through the checker; returns the
-    number of launches checked."""
+def check_shape(model, B, D, H, W, stride=1, want_z=True, column=None, two_pass=True):
+    """Run forward, the saving forward and the adjoint of `model` at [B, D, H, W] through the checker (with the
+    two-pass fusion adjoint where the stride allows it); returns the number of launches checked."""
     eng, chk, restore = dry_engine(model)
+    eng.two_pass_fuse = two_pass
     try:
         col = np.linspace(14.0, -1.0, D) if column is None else np.asarray(column, np.float64)
         zs = col - col.min()

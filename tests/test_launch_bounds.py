@@ -27,6 +27,8 @@ def _model(width=64, layers=8, rank=8):
 def test_engine_launches_stay_in_bounds(B, D, H, W, stride):
     n = check_shape(_model(), B, D, H, W, stride=stride)
     assert n > 100
+    if stride == 1:
+        assert check_shape(_model(layers=2), B, D, H, W, stride=1, two_pass=False) > 20
 
 
 @pytest.mark.parametrize("width,rank", [(96, 8), (32, 0), (16, 4)])
@@ -69,6 +71,7 @@ def test_checker_catches_a_short_buffer_and_covers_every_entry_point():
     LB.Checker.__call__ = record
     try:
         check_shape(_model(layers=2), 1, 12, 8, 8, stride=2)
+        check_shape(_model(layers=2), 1, 12, 8, 8, stride=1, two_pass=True)
     finally:
         LB.Checker.__call__ = orig
     assert declared <= calls, f"entry points never checked: {sorted(declared - calls)}"
