@@ -449,6 +449,100 @@ __global__ __launch_bounds__(256) void k_tw_gemm(FfnoTwGemm g) {
   }
 }
 
+// twiddle matrices of at most 128 x 128: A staged in LDS once per persistent workgroup (rows padded by 16 bytes so
+// the lanes' 16-byte reads fall in distinct banks), the row offsets of both index maps tabulated in LDS, all MT
+// m-tiles of a 64-column task accumulated in registers (the task's input is read once), and the next k-step's input
+// loaded while the current one's MFMAs run
+constexpr int TW_PAD = 8;
+
+template <int MT>
+__global__ __launch_bounds__(256) void k_tw_small(FfnoTwGemm g) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int kp = g.k_pad, ldk = kp + TW_PAD;
+  int64_t* koff = reinterpret_cast<int64_t*>(smem);
+  int64_t* moff = koff + kp;
+  bf16* ah = reinterpret_cast<bf16*>(moff + MT * 16);
+  bf16* al = ah + MT * 16 * ldk;
+  for (int i = threadIdx.x; i < MT * 16 * (kp / 8); i += blockDim.x) {
+    const int r = i / (kp / 8), c = (i % (kp / 8)) * 8;
+    uint4 h = {0, 0, 0, 0}, l = {0, 0, 0, 0};
+    if (r < g.m_pad) {
+      h = *reinterpret_cast<const uint4*>(g.a_hi + (int64_t)r * kp + c);
+      l = *reinterpret_cast<const uint4*>(g.a_lo + (int64_t)r * kp + c);
+    }
+    *reinterpret_cast<uint4*>(ah + r * ldk + c) = h;
+    *reinterpret_cast<uint4*>(al + r * ldk + c) = l;
+  }
+  for (int k = threadIdx.x; k < kp; k += blockDim.x) koff[k] = k < g.k ? imap(k, g.k_split, g.x_k_hi, g.x_k_lo) : -1;
+  for (int m = threadIdx.x; m < MT * 16; m += blockDim.x)
+    moff[m] = m < g.m ? imap(m, g.m_split, g.y_m_hi, g.y_m_lo) : -1;
+  __syncthreads();
+  Lane L;
+  const int64_t ntn = (g.n + 63) / 64;
+  const int64_t tasks = (int64_t)g.batch * ntn;
+  const int ks = kp / 32;
+  for (int64_t task = (int64_t)blockIdx.x * 4 + L.wave; task < tasks; task += (int64_t)gridDim.x * 4) {
+    const int64_t bi = task / ntn;
+    const int n0 = (int)(task % ntn) * 64 + 4 * L.v;
+    const bool nok = n0 < g.n;
+    const float* xb = g.x + imap(bi, g.b_split, g.x_b_hi, g.x_b_lo) + n0;
+    float* yb = g.y + imap(bi, g.b_split, g.y_b_hi, g.y_b_lo) + n0;
+    f32x4 acc[MT][4];
+#pragma unroll
+    for (int a = 0; a < MT; ++a)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc[a][i] = zero4();
+    f32x4 xv[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int64_t o = koff[8 * L.q + j];
+      xv[j] = (nok && o >= 0) ? *(const f32x4*)(xb + o) : zero4();
+    }
+    for (int s = 0; s < ks; ++s) {
+      bf16x8 bh[4], bl[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const bf16 h = (bf16)xv[j][i];
+          bh[i][j] = h;
+          bl[i][j] = (bf16)(xv[j][i] - (float)h);
+        }
+      if (s + 1 < ks) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int64_t o = koff[32 * (s + 1) + 8 * L.q + j];
+          xv[j] = (nok && o >= 0) ? *(const f32x4*)(xb + o) : zero4();
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < MT; ++a) {
+        const int ao = (a * 16 + L.v) * ldk + 32 * s + 8 * L.q;
+        const bf16x8 fh = *reinterpret_cast<const bf16x8*>(ah + ao);
+        const bf16x8 fl = *reinterpret_cast<const bf16x8*>(al + ao);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          acc[a][i] = mfma(fh, bh[i], acc[a][i]);
+          acc[a][i] = mfma(fh, bl[i], acc[a][i]);
+          acc[a][i] = mfma(fl, bh[i], acc[a][i]);
+        }
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < MT; ++a)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int64_t o = moff[a * 16 + 4 * L.q + j];
+        if (o >= 0 && nok) {
+          float* p = yb + o;
+          f32x4 r = f32x4{acc[a][0][j], acc[a][1][j], acc[a][2][j], acc[a][3][j]};
+          if (g.accumulate) r += *(const f32x4*)p;
+          *(f32x4*)p = r;
+        }
+      }
+  }
+}
+
 // ---------------------------------------------------------------------------------------------------------------
 // per-mode complex channel mix, in place
 // ---------------------------------------------------------------------------------------------------------------
@@ -648,6 +742,26 @@ int ffno_tw_gemm(const FfnoTwGemm* g, hipStream_t stream) {
     return fail("ffno_tw_gemm: bad sizes");
   const int64_t tasks = (int64_t)g->batch * ((g->n + 63) / 64);
   const int64_t blocks = (tasks + 3) / 4;
+  if (g->m_pad <= 128 && g->k_pad <= 128) {
+    const int mt = (g->m_pad + 15) / 16;
+    const size_t lds = (size_t)(g->k_pad + mt * 16) * sizeof(int64_t) +
+                       (size_t)2 * mt * 16 * (g->k_pad + TW_PAD) * sizeof(uint16_t);
+    const int grid = (int)(blocks < 512 ? blocks : 512);       // persistent: A is staged once per workgroup
+    switch (mt) {
+#define FFNO_TW_CASE(M)                                        \
+  case M: {                                                    \
+    auto k = k_tw_small<M>;                                    \
+    if (set_lds(k, lds)) return -1;                            \
+    k<<<grid, 256, lds, stream>>>(*g);                         \
+    break;                                                     \
+  }
+      FFNO_TW_CASE(1) FFNO_TW_CASE(2) FFNO_TW_CASE(3) FFNO_TW_CASE(4)
+      FFNO_TW_CASE(5) FFNO_TW_CASE(6) FFNO_TW_CASE(7) FFNO_TW_CASE(8)
+#undef FFNO_TW_CASE
+      default: return fail("ffno_tw_gemm: bad tile count");
+    }
+    return check_launch("ffno_tw_gemm");
+  }
   const int grid = (int)(blocks < 4096 ? blocks : 4096);
   k_tw_gemm<<<grid, 256, 0, stream>>>(*g);
   return check_launch("ffno_tw_gemm");

@@ -28,17 +28,23 @@ def _gelu(x):
     return 0.5 * x * (1.0 + torch.erf(x / math.sqrt(2.0)))
 
 
-def test_tw_gemm_matches_float64_matmul():
+@pytest.mark.parametrize("m,k,n,batch,accumulate", [
+    (66, 70, 96, 5, 0),            # LDS-staged kernel, padding in m and k, partial 64-column task
+    (128, 128, 64, 2, 1),          # its largest matrix, accumulating
+    (16, 32, 4, 7, 0),             # one m-tile, one k-step, 4 columns
+    (200, 150, 72, 3, 1),          # beyond 128: the streaming kernel
+])
+def test_tw_gemm_matches_float64_matmul(m, k, n, batch, accumulate):
     """Split-bf16 GEMM with padding in m and k and a two-level batch map: 3e-5 of the output range."""
     from ffno_amd import _lib
     from ffno_amd import transforms as T
     from ffno_amd._lib import FfnoTwGemm
     torch.manual_seed(0)
-    m, k, n, batch = 66, 70, 96, 5
     a = torch.randn(m, k, dtype=torch.float64)
     hi, lo, _, _, mp, kp = T.split_bf16(a.numpy(), DEV)
     x = torch.randn(batch, k, n, device=DEV)
-    y = torch.zeros(batch, m, n, device=DEV)
+    y0 = torch.randn(batch, m, n, device=DEV) if accumulate else torch.zeros(batch, m, n, device=DEV)
+    y = y0.clone()
     g = FfnoTwGemm()
     g.a_hi, g.a_lo = _p(hi), _p(lo)
     g.m, g.k, g.n, g.m_pad, g.k_pad, g.batch = m, k, n, mp, kp, batch
@@ -46,10 +52,10 @@ def test_tw_gemm_matches_float64_matmul():
     g.b_split, g.x_b_hi, g.x_b_lo, g.y_b_hi, g.y_b_lo = 2, 2 * k * n, k * n, 2 * m * n, m * n
     g.k_split, g.x_k_hi, g.x_k_lo = 1 << 40, 0, n
     g.m_split, g.y_m_hi, g.y_m_lo = 1 << 40, 0, n
-    g.accumulate = 0
+    g.accumulate = accumulate
     _lib.call("ffno_tw_gemm", ctypes.byref(g), _stream())
     torch.cuda.synchronize()
-    ref = torch.einsum("mk,bkn->bmn", a, x.double().cpu())
+    ref = torch.einsum("mk,bkn->bmn", a, x.double().cpu()) + y0.double().cpu()
     err = (y.double().cpu() - ref).abs().max() / ref.abs().max()
     print("tw_gemm rel err", float(err))
     assert err < 3e-5
