@@ -95,6 +95,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-baselines", action="store_true", help="skip the plain FP32 GPU and CPU legs")
     ap.add_argument("--volume", action="store_true", help="also run the whole-volume 504 x 504 x 256 forward + VJP")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy 1 GiB device to device three times (the known-size reference for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported (requests shard whole across GPUs "
@@ -105,6 +107,7 @@ def main():
     from ffno_amd import _lib
 
     dev = torch.device("cuda:0")
+    copy_cal = copy_calibration(dev) if args.copy_calibration else None
     w = WORKLOAD
     voxels = w["B"] * w["D"] * w["H"] * w["W"]
     model, x, z, ct = make(w, dev)
@@ -205,10 +208,31 @@ def main():
                                  "ms_per_step": round(pms, 1), "speedup": round(pms / ms, 2),
                                  "rel_err_log10n": rel, "grad_rel_err": grel}
         out["cpu_baseline"] = cpu_baseline()
+    if copy_cal:
+        out["copy_calibration"] = copy_cal
     if args.volume:
         del y, xb
         out["whole_volume"] = volume_leg(model, dev)
     print(json.dumps(out), flush=True)
+
+
+def copy_calibration(dev, nbytes=1 << 30):
+    """Three device-to-device copies of nbytes (torch's copy kernel): the reference of known size that scales the
+    HBM byte counters of a profile, and the copy bandwidth this GPU reaches."""
+    import torch
+    src = torch.ones(nbytes // 4, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(3):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 3
+    del src, dst
+    return {"bytes_per_copy": 2 * nbytes, "ms": round(ms, 4), "gb_s": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}
 
 
 def volume_leg(model, dev):

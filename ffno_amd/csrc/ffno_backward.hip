@@ -8,6 +8,11 @@ namespace {
 // ---------------------------------------------------------------------------------------------------------------
 // GroupNorm backward
 // ---------------------------------------------------------------------------------------------------------------
+// Per (lane, tile t) GroupNorm constants: with at least 4 channels per group a lane's 4 channels of a tile share one
+// group, so its statistics and gradient sums are loaded once per tile, not per channel
+template <int CT>
+__device__ __forceinline__ int gn_group(int c) { return c / ((CT * 16) / G); }
+
 // g_hat and x_hat of one lane's 4 channels of tile t
 template <int CT>
 __device__ __forceinline__ void gn_ghat(const FfnoGnBack& gb, float scale, int64_t row, int b, bool ok, int q, int t,
@@ -22,11 +27,21 @@ __device__ __forceinline__ void gn_ghat(const FfnoGnBack& gb, float scale, int64
     const f32x4 s = *(const f32x4*)(gb.aff_b + b * C + c0);
     act = gelu_grad4(tv * a + s);
   }
+  if constexpr (CPG >= 4) {
+    const float* st = gb.stats + (b * G + gn_group<CT>(c0)) * 3;
+    const float mean = st[0], rstd = st[1];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const float* st = gb.stats + (b * G + (c0 + j) / CPG) * 3;
-    xh[j] = ok ? (tv[j] - st[0]) * st[1] : 0.f;
-    gh[j] = ok ? scale * ub[j] * gm[j] * act[j] : 0.f;
+    for (int j = 0; j < 4; ++j) {
+      xh[j] = ok ? (tv[j] - mean) * rstd : 0.f;
+      gh[j] = ok ? scale * ub[j] * gm[j] * act[j] : 0.f;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const float* st = gb.stats + (b * G + gn_group<CT>(c0 + j)) * 3;
+      xh[j] = ok ? (tv[j] - st[0]) * st[1] : 0.f;
+      gh[j] = ok ? scale * ub[j] * gm[j] * act[j] : 0.f;
+    }
   }
 }
 
@@ -44,12 +59,13 @@ __device__ __forceinline__ void gn_back(const FfnoGnBack& gb, float scale, int64
     gn_ghat<CT>(gb, scale, row, b, ok, q, t, ub[t], gh, xh);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const int g = (16 * t + 4 * q + j) / CPG;
+      const int g = CPG >= 4 ? gn_group<CT>(16 * t + 4 * q) : gn_group<CT>(16 * t + 4 * q + j);
       const float* st = gb.stats + (b * G + g) * 3;
       const float* sm = gb.sums + (b * G + g) * 2;
-      tb[t][j] = ok ? st[1] * (gh[j] - (w / st[2]) * (sm[0] + xh[j] * sm[1])) : 0.f;
+      const float inv_w = __builtin_amdgcn_rcpf(st[2]);
+      tb[t][j] = ok ? st[1] * (gh[j] - (w * inv_w) * (sm[0] + xh[j] * sm[1])) : 0.f;
       const float r2var = 1.f - 1e-5f * st[1] * st[1];
-      wb -= (sm[0] * xh[j] + 0.5f * sm[1] * (xh[j] * xh[j] - r2var)) / st[2];
+      wb -= (sm[0] * xh[j] + 0.5f * sm[1] * (xh[j] * xh[j] - r2var)) * inv_w;
     }
   }
   if (gb.wbar) {

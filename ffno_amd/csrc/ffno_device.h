@@ -36,8 +36,25 @@ __device__ __forceinline__ bf16x8 pack8(f32x4 a, f32x4 b) {
   return r;
 }
 
-__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
-__device__ __forceinline__ float sigm(float x) { return 1.0f / (1.0f + __expf(-x)); }
+// The standard normal CDF Phi(x) = (1 + erf(x / sqrt 2)) / 2 by Abramowitz & Stegun, Handbook of Mathematical
+// Functions, formula 7.1.26 (Hastings): for a >= 0, erf(a) = 1 - (a1 t + a2 t^2 + a3 t^3 + a4 t^4 + a5 t^5) e^(-a^2),
+// t = 1 / (1 + p a), |error| <= 1.5e-7. Phi(-|x|) = (1 - erf(|x| / sqrt 2)) / 2 is the polynomial term itself (no
+// cancellation for negative x). Also returns E = e^(-x^2 / 2), which the GELU derivative reuses.
+__device__ __forceinline__ float norm_cdf(float x, float& e) {
+  const float a = fabsf(x) * 0.70710678118654752f;
+  const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * a);
+  const float poly = t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f +
+                                                                                        t * 1.061405429f))));
+  e = __expf(-0.5f * x * x);
+  const float tail = 0.5f * poly * e;                  // Phi(-|x|)
+  return x >= 0.0f ? 1.0f - tail : tail;
+}
+__device__ __forceinline__ float gelu(float x) {
+  float e;
+  return x * norm_cdf(x, e);
+}
+// 1 / (1 + e^-x) with the hardware reciprocal (1 ulp); e^-x = inf gives 0
+__device__ __forceinline__ float sigm(float x) { return __builtin_amdgcn_rcpf(1.0f + __expf(-x)); }
 __device__ __forceinline__ f32x4 gelu4(f32x4 x) { return f32x4{gelu(x[0]), gelu(x[1]), gelu(x[2]), gelu(x[3])}; }
 __device__ __forceinline__ f32x4 sigm4(f32x4 x) { return f32x4{sigm(x[0]), sigm(x[1]), sigm(x[2]), sigm(x[3])}; }
 
@@ -103,9 +120,11 @@ __device__ __forceinline__ void gemm(Wm W, const Frag* b, f32x4 (&acc)[NT], int 
   for (int t = 0; t < NT; ++t) acc[t] = tile<KS>(W, t, b, v, q);
 }
 
-// d/dx of the exact GELU: Phi(x) + x phi(x)
+// d/dx of the exact GELU: Phi(x) + x phi(x), phi(x) = e^(-x^2/2) / sqrt(2 pi)
 __device__ __forceinline__ float gelu_grad(float x) {
-  return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * 0.3989422804014327f * __expf(-0.5f * x * x);
+  float e;
+  const float c = norm_cdf(x, e);
+  return c + x * 0.3989422804014327f * e;
 }
 __device__ __forceinline__ f32x4 gelu_grad4(f32x4 x) {
   return f32x4{gelu_grad(x[0]), gelu_grad(x[1]), gelu_grad(x[2]), gelu_grad(x[3])};
