@@ -203,11 +203,13 @@ def main():
         pms = (time.perf_counter() - s0) * 1e3
         rel = float(((y.detach() - yp.detach()).abs() / yp.detach().abs().clamp_min(1.0)).max())
         grel = float((xb - xbp).norm() / xbp.norm())
+        y0_plain = yp[:1].detach().double().cpu()
         del yp, xbp
         out["plain_fp32_gpu"] = {"value": round(voxels / (pms * 1e-3) / 1e6, 3), "unit": "Mvoxels/s",
                                  "ms_per_step": round(pms, 1), "speedup": round(pms / ms, 2),
-                                 "rel_err_log10n": rel, "grad_rel_err": grel}
-        out["cpu_baseline"] = cpu_baseline()
+                                 "rel_err_log10n_vs_hip": rel, "grad_rel_err_vs_hip": grel}
+        out["cpu_baseline"], out["accuracy_vs_fp64"] = cpu_leg(model, x[:1].cpu(), z[:1].cpu(),
+                                                               y[:1].detach().double().cpu(), y0_plain)
     if copy_cal:
         out["copy_calibration"] = copy_cal
     if args.volume:
@@ -312,8 +314,11 @@ def volume_leg(model, dev):
     return res
 
 
-def cpu_baseline():
-    """The float32 CPU oracle (oracle/ffno_oracle.py) on the CPU workload, forward + input VJP, all host cores."""
+def cpu_leg(model, x0, z0, y0_hip, y0_plain):
+    """The CPU leg: (1) the float32 CPU oracle (oracle/ffno_oracle.py) timed on the CPU workload, forward + input
+    VJP, on all host cores; (2) the float64 oracle as the checker of the benchmark's first batch item (B=1, D=128,
+    64 x 64, same weights): the relative error in log10 n, max |y - y64| / max(|y64|, 1), of the HIP path and of the
+    plain FP32 GPU path."""
     import torch
     from ffno_amd import synthetic
     from oracle import ffno_oracle as O
@@ -322,13 +327,14 @@ def cpu_baseline():
     torch.set_num_threads(cores)
     w = CPU_WORKLOAD
     cfg = model_config(w)
+    ocfg_of = lambda c: O.model_config(**{k: c[k] for k in (  # noqa: E731
+        "width", "n_layers", "hidden_spectral", "hidden_vertical", "freq_mix_rank", "z_reference_length",
+        "z_min_wavelength", "z_feature_wavelength", "species", "out_channels")})
     x, z, mx, sx = synthetic.atmosphere(w["B"], w["D"], w["H"], w["W"], seed=0)
-    model = synthetic.random_model(0, **cfg)
-    model.set_normalization(mx, sx, np.zeros(12), np.ones(12))
-    sd = O.to_dtype({k: v.detach() for k, v in model.state_dict().items()}, torch.float32)
-    ocfg = O.model_config(**{k: cfg[k] for k in ("width", "n_layers", "hidden_spectral", "hidden_vertical",
-                                                 "freq_mix_rank", "z_reference_length", "z_min_wavelength",
-                                                 "z_feature_wavelength", "species", "out_channels")})
+    small = synthetic.random_model(0, **cfg)
+    small.set_normalization(mx, sx, np.zeros(12), np.ones(12))
+    sd = O.to_dtype({k: v.detach() for k, v in small.state_dict().items()}, torch.float32)
+    ocfg = ocfg_of(cfg)
     ct = synthetic.cotangent((w["B"], 12, w["D"], w["H"], w["W"]), 0)
     O.vjp(sd, ocfg, x, z, 48e3, 48e3, ct)
     times = []
@@ -339,10 +345,21 @@ def cpu_baseline():
         times.append(time.perf_counter() - t0)
     t = sorted(times)[len(times) // 2]
     voxels = w["B"] * w["D"] * w["H"] * w["W"]
-    return {"value": round(voxels / t / 1e6, 4), "unit": "Mvoxels/s", "cores": cores, "kind": "port",
-            "sample": f"float32 oracle forward + input VJP, B=1 D=64 H=W=32 width 32, 4 layers, 12 vertical modes; "
-                      f"median of {len(times)} after 1 warm-up, {t:.2f} s each"}
-
+    baseline = {"value": round(voxels / t / 1e6, 4), "unit": "Mvoxels/s", "cores": cores, "kind": "port",
+                "sample": f"float32 oracle forward + input VJP, B=1 D=64 H=W=32 width 32, 4 layers, 12 vertical "
+                          f"modes; median of {len(times)} after 1 warm-up, {t:.2f} s each"}
+    t0 = time.perf_counter()
+    sd64 = O.to_dtype({k: v.detach().cpu() for k, v in model.state_dict().items()}, torch.float64)
+    with torch.no_grad():
+        y64 = O.forward(sd64, ocfg_of(model_config(WORKLOAD)), x0.double(), z0.double(), 48e3, 48e3)
+    scale = y64.abs().clamp_min(1.0)
+    accuracy = {"checker": "float64 oracle, batch item 0 of the benchmark step (B=1 D=128 H=W=64 width 64, 8 layers)",
+                "hip_rel_err_log10n": float(((y0_hip - y64).abs() / scale).max()),
+                "plain_fp32_rel_err_log10n": float(((y0_plain - y64).abs() / scale).max()),
+                "hip_max_abs_dex": float((y0_hip - y64).abs().max()),
+                "plain_fp32_max_abs_dex": float((y0_plain - y64).abs().max()),
+                "checker_s": round(time.perf_counter() - t0, 1)}
+    return baseline, accuracy
 
 if __name__ == "__main__":
     main()

@@ -13,7 +13,7 @@ class FFNOFunction(torch.autograd.Function):
         need = ctx.needs_input_grad[0] or ctx.needs_input_grad[1]
         if not need:
             return engine.forward(x.detach(), z.detach(), dx, dy)
-        y, saved = engine.forward_saved(x.detach(), z.detach(), dx, dy)
+        y, saved = engine.forward_saved(x.detach(), z.detach(), dx, dy, want_z=ctx.needs_input_grad[1])
         ctx.save_for_backward(x, z)
         ctx.saved = saved
         ctx.module, ctx.dx, ctx.dy = module, dx, dy

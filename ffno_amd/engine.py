@@ -314,6 +314,15 @@ class Plan:
         v1 = self.B * 2 * self.modes * self.H * self.W * self.C * 4
         return 2 * self.h_chunk * self.plane * 4 + 2 * v1 + self.B * self.voxels * 4 * 32
 
+    def full_bytes(self, want_z=True):
+        """The adjoint keeping every block's intermediates: 11 units per block (input, both branch outputs, five
+        GroupNorm inputs, x1, the gates) and the vertical spectrum with want_z, plus the head input and two working
+        units."""
+        v1 = self.B * 2 * self.modes * self.H * self.W * self.C * 4
+        tables = 0 if self._cache_tables else 2 * self.table_bytes // self.n_layers
+        return (self.unit_bytes() * (11 * self.n_layers + 3) + (self.n_layers if want_z else 0) * v1 +
+                self.scratch_bytes() + tables)
+
     def vjp_bytes(self, stride, two_pass=False):
         saved = -(-self.n_layers // stride) + (1 if stride == 1 else 0) + (stride - 1)
         tables = 0 if self._cache_tables else 2 * self.table_bytes // self.n_layers
@@ -323,12 +332,16 @@ class Plan:
 
 class Saved:
     """What the forward keeps for the adjoint: the inputs of blocks 0, s, 2s, ... (s = stride) and, with s = 1, the
-    head's input. The adjoint recomputes the inputs of the other blocks from the nearest kept one."""
+    head's input. The adjoint recomputes the inputs of the other blocks from the nearest kept one. With full, the
+    forward keeps every intermediate the adjoint reads instead (about 11 activation-sized tensors per block), so that
+    nothing is recomputed -- the choice when HBM has room for it."""
 
-    def __init__(self, stride=1, two_pass=False):
+    def __init__(self, stride=1, two_pass=False, full=False):
         self.stride = int(stride)
-        self.two_pass = bool(two_pass)     # fusion adjoint in two LDS-staged passes over stored gates
+        self.two_pass = bool(two_pass) or bool(full)   # fusion adjoint in two LDS-staged passes over stored gates
+        self.full = bool(full)             # every block's intermediates kept: the adjoint recomputes nothing
         self.inputs = {}
+        self.blocks = []                   # full: per block, the buffers and GroupNorm records of its forward
         self.head = None
 
 
@@ -352,6 +365,7 @@ class Engine:
         self._lock = threading.RLock()      # the serving workers share one engine
         self.checkpoint_stride = None       # None: the smallest stride whose adjoint fits the free HBM
         self.two_pass_fuse = None           # None: whenever its gate buffer fits at stride 1
+        self.full_save = None               # None: keep every intermediate when that fits; False: never
         self.last_stride = None
 
     def footprint_bytes(self, B, D, H, W, vjp=False):
@@ -404,23 +418,28 @@ class Engine:
                 self._plans[key] = p
         return p
 
-    def choose_checkpoints(self, plan, stride=None):
-        """(stride, two_pass) of the adjoint: the configured (or given) stride, else the smallest whose footprint
-        fits the HBM that is free now (the caching allocator's unused reserve included); the two-pass fusion
-        adjoint at stride 1 when its gate buffer fits too."""
-        stride = stride or self.checkpoint_stride
+    def choose_checkpoints(self, plan, stride=None, want_z=True):
+        """(stride, two_pass, full) of the adjoint. stride 0 asks for every intermediate kept (full). Otherwise the
+        configured (or given) stride, else: full when it fits the HBM that is free now (the caching allocator's
+        unused reserve included), then stride 1 with the two-pass fusion adjoint, then the smallest stride that
+        fits."""
+        stride = self.checkpoint_stride if stride is None else stride
+        if stride == 0:
+            return 1, True, True
         if stride and self.two_pass_fuse is not None:
-            return int(stride), bool(self.two_pass_fuse) and int(stride) == 1
+            return int(stride), bool(self.two_pass_fuse) and int(stride) == 1, False
         free, _ = torch.cuda.mem_get_info(plan.device)
         free += torch.cuda.memory_reserved(plan.device) - torch.cuda.memory_allocated(plan.device)
         budget = 0.97 * free
         if stride:
-            return int(stride), int(stride) == 1 and plan.vjp_bytes(1, True) <= budget
+            return int(stride), int(stride) == 1 and plan.vjp_bytes(1, True) <= budget, False
+        if self.full_save is not False and self.two_pass_fuse is not False and plan.full_bytes(want_z) <= budget:
+            return 1, True, True
         if self.two_pass_fuse is not False and plan.vjp_bytes(1, True) <= budget:
-            return 1, True
+            return 1, True, False
         for s in range(1, plan.n_layers + 1):
             if plan.vjp_bytes(s) <= budget:
-                return s, False
+                return s, False, False
         raise MemoryError(f"the adjoint of a [{plan.B}, {plan.D}, {plan.H}, {plan.W}] request needs "
                           f"{plan.vjp_bytes(plan.n_layers) / 2**30:.1f} GiB even with one saved input")
 
@@ -549,6 +568,32 @@ class Engine:
         ma, mb, _ = gn["m"]
         return _asm(hb, s, ma, mb, blk["res_mlp"]), gn
 
+    def _block_keep(self, plan, i, prev, x_out, bk, scr, bz, x2=None):
+        """Forward of block i keeping what its adjoint reads: the input (assembled from prev into x_out, or already
+        in bk["h"]), the branch transforms' outputs in bk["s"], bk["v"] (the vertical spectrum after the mix in
+        bk["v1y"] or the scratch), the five GroupNorm inputs, x1 and the gates; x2 to x2 when given. Returns the
+        GroupNorm records."""
+        dev = bk["s"].device
+        blk = self.weights.blocks[i]
+        tables = plan.tables(i)
+        st = self._stream()
+        rows = plan.rows
+        s, v = bk["s"], bk["v"]
+        _lib.call("ffno_block_in", rows, prev, _p(x_out), _p(blk["spec.g1"]), _p(blk["spec.g2"]),
+                  _p(blk["vertical.g1"]), _p(blk["vertical.g2"]), _p(blk["z_proj"]), _p(bz), _p(s), _p(v), st)
+        stats = FfnoStatsOut(_p(plan.depth_w), _p(scr["part"]))
+        gn = {}
+        self.horizontal(plan, s, s, i, scr, tables)
+        _lib.call("ffno_post", rows, _p(s), _p(blk["spec.p1"]), _p(blk["spec.p2"]), _p(bk["sp"]), stats, st)
+        gn["s"] = self._gn(plan, scr["part"], blk["norm_spec"], dev)
+        v1y = bk.get("v1y")
+        self.vertical(plan, v, v1y if v1y is not None else scr["v1y" if "v1y" in scr else "v1"], v, i, tables)
+        _lib.call("ffno_post", rows, _p(v), _p(blk["vertical.p1"]), _p(blk["vertical.p2"]), _p(bk["vp"]), stats, st)
+        gn["v"] = self._gn(plan, scr["part"], blk["norm_vert"], dev)
+        self._tail(plan, i, bk["h"], bk["sp"], bk["vp"], bk["fp"], bk["x1"], bk["pp"], x2, bk["mp"], scr, gn,
+                   bk.get("gates"))
+        return gn
+
     def _tail(self, plan, i, h, sp, vp, fp, x1, pp, x2, mp, scr, gn, gates=None):
         """fuse -> pw -> mlp of block i (models/ffno_model.py:524-551), buffers as named (x2 may be None; gates, if
         given, receives the branch gates)."""
@@ -599,7 +644,18 @@ class Engine:
         _lib.call("ffno_lift", rows, _p(x), _p(Wt.lift), _p(s), FfnoStatsOut(_p(plan.depth_w), _p(scr["part"])), st)
         a, b, _ = self._gn(plan, scr["part"], Wt.lift_gn, dev)
         prev = _asm(None, s, a, b, None, 1)
+        full = saved is not None and saved.full
         for i in range(len(Wt.blocks)):
+            if full:
+                # every intermediate to a buffer of its own; x2 to a running buffer (hb), the next input from it
+                bk = {k: torch.empty(N, C, **f32) for k in ("h", "s", "v", "sp", "vp", "fp", "pp", "mp", "x1")}
+                bk["gates"] = torch.empty(N, 2 * C, **f32)
+                bk["v1y"] = torch.empty_like(scr["v1"]) if getattr(saved, "want_z", True) else None
+                bk["gn"] = self._block_keep(plan, i, prev, bk["h"], bk, scr, bz, hb)
+                saved.blocks.append(bk)
+                ma, mb, _ = bk["gn"]["m"]
+                prev = _asm(hb, bk["mp"], ma, mb, self.weights.blocks[i]["res_mlp"])
+                continue
             h = hb
             if saved is not None and i % saved.stride == 0:
                 h = torch.empty(N, C, **f32)
@@ -613,12 +669,14 @@ class Engine:
                   ctypes.byref(Wt.head), _p(y), st)
         return y
 
-    def forward_saved(self, x, z, dx, dy, plan=None, stride=None):
-        """(y, saved): the forward that keeps what the adjoint needs, at the given checkpoint stride or the one that
-        fits the free HBM."""
+    def forward_saved(self, x, z, dx, dy, plan=None, stride=None, want_z=True):
+        """(y, saved): the forward that keeps what the adjoint needs: every intermediate (stride 0, or by default
+        when it fits the free HBM), else the block inputs at the given checkpoint stride or the one that fits.
+        want_z: the adjoint will be asked for the height gradient too (full keeps the vertical spectra then)."""
         dev, plan, _, _ = self._prepare(x, z, dx, dy, plan)
-        saved = Saved(*self.choose_checkpoints(plan, stride))
-        self.last_stride = saved.stride          # informational (the stride of the latest saving forward)
+        saved = Saved(*self.choose_checkpoints(plan, stride, want_z))
+        saved.want_z = want_z
+        self.last_stride = 0 if saved.full else saved.stride     # informational (the latest saving forward's)
         return self.forward(x, z, dx, dy, plan=plan, saved=saved), saved
 
     # ------------------------------------------------------------------------------------------------ adjoint
@@ -655,9 +713,13 @@ class Engine:
         st = self._stream()
         rows = plan.rows
         f32 = dict(device=dev, dtype=torch.float32)
-        g, s, v, sp, vp, fp, pp, mp, x1 = (torch.empty(N, C, **f32) for _ in range(WORKING_UNITS))
-        tbuf = torch.empty(N, C, **f32) if stride > 1 else None
-        gates = torch.empty(N, 2 * C, **f32) if saved.two_pass else None
+        g = torch.empty(N, C, **f32)
+        if saved.full:
+            s = v = sp = vp = fp = pp = mp = x1 = tbuf = gates = None
+        else:
+            s, v, sp, vp, fp, pp, mp, x1 = (torch.empty(N, C, **f32) for _ in range(WORKING_UNITS - 1))
+            tbuf = torch.empty(N, C, **f32) if stride > 1 else None
+            gates = torch.empty(N, 2 * C, **f32) if saved.two_pass else None
         scr = self._scratch(plan, dev, adjoint=True)
         part2 = scr["part2"]
         ones = torch.ones(B, C, **f32)
@@ -692,29 +754,28 @@ class Engine:
         for i in reversed(range(L)):
             blk, adj = Wt.blocks[i], A["blocks"][i]
             tables = plan.tables(i)
-            base = (i // stride) * stride
-            prev = _asm(None, saved.inputs[base], ones, zeros)
-            if i == base:
-                h, x_out = saved.inputs[base], None
-            else:
-                # forward from the nearest saved input up to block i's input (x1 as the running buffer)
-                for j in range(base, i):
-                    hj = saved.inputs[base] if j == base else x1
-                    prev, _ = self._block(plan, j, prev, None if j == base else x1, hj, x1, s, v, scr, bz)
-                h, x_out = tbuf, tbuf
-            # recompute block i: branch outputs stay in s and v, the vertical spectrum after the mix in v1y
-            _lib.call("ffno_block_in", rows, prev, _p(x_out), _p(blk["spec.g1"]), _p(blk["spec.g2"]),
-                      _p(blk["vertical.g1"]), _p(blk["vertical.g2"]), _p(blk["z_proj"]), _p(bz), _p(s), _p(v), st)
-            stats = FfnoStatsOut(_p(plan.depth_w), _p(scr["part"]))
-            gn = {}
-            self.horizontal(plan, s, s, i, scr, tables)
-            _lib.call("ffno_post", rows, _p(s), _p(blk["spec.p1"]), _p(blk["spec.p2"]), _p(sp), stats, st)
-            gn["s"] = self._gn(plan, scr["part"], blk["norm_spec"], dev)
-            self.vertical(plan, v, scr["v1y"], v, i, tables)
-            _lib.call("ffno_post", rows, _p(v), _p(blk["vertical.p1"]), _p(blk["vertical.p2"]), _p(vp), stats, st)
-            gn["v"] = self._gn(plan, scr["part"], blk["norm_vert"], dev)
             last_unsaved = i == L - 1 and saved.head is None
-            self._tail(plan, i, h, sp, vp, fp, x1, pp, g if last_unsaved else None, mp, scr, gn, gates)
+            if saved.full:
+                bk = saved.blocks[i]
+                saved.blocks[i] = None                      # released when this block's adjoint is done
+                h, s, v, sp, vp, fp, pp, mp, x1 = (bk[k] for k in ("h", "s", "v", "sp", "vp", "fp", "pp", "mp", "x1"))
+                gates, gn = bk["gates"], bk["gn"]
+                v1y = bk["v1y"] if bk["v1y"] is not None else scr["v1y"]
+            else:
+                base = (i // stride) * stride
+                prev = _asm(None, saved.inputs[base], ones, zeros)
+                if i == base:
+                    h, x_out = saved.inputs[base], None
+                else:
+                    # forward from the nearest saved input up to block i's input (x1 as the running buffer)
+                    for j in range(base, i):
+                        hj = saved.inputs[base] if j == base else x1
+                        prev, _ = self._block(plan, j, prev, None if j == base else x1, hj, x1, s, v, scr, bz)
+                    h, x_out = tbuf, tbuf
+                # recompute block i: branch outputs stay in s and v, the vertical spectrum after the mix in v1y
+                bk = dict(h=h, s=s, v=v, sp=sp, vp=vp, fp=fp, pp=pp, mp=mp, x1=x1, gates=gates, v1y=scr["v1y"])
+                gn = self._block_keep(plan, i, prev, x_out, bk, scr, bz, g if last_unsaved else None)
+                v1y = scr["v1y"]
             pa, pb, _ = gn["p"]
             if last_unsaved:
                 # the head input x2 + res_mlp GN(mp), assembled in g by the forward head's own code (bit for bit what
@@ -759,7 +820,7 @@ class Engine:
             self._gn_sums(plan, gb, pp, part2)
             _lib.call("ffno_mlp_bwd", rows, _p(pp), gb, _asm(v), _p(blk["vertical.p1"]), _p(adj["vertical.p2t"]),
                       _p(adj["vertical.p1t"]), 0, _p(v), st)
-            v1, v1y = scr["v1"], scr["v1y"]
+            v1 = scr["v1"]
             if want_z and ht["v0"] is not None:
                 # taper: dL/df_m per column = sum_c,part (V0^T yv_bar) * Y, before the mix adjoint overwrites v1
                 self._tw(ht["v0"], v, v1, B, hwc, (big, 0, D * hwc), (0, 2 * M * hwc), (big, 0, hwc),

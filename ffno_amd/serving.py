@@ -86,6 +86,9 @@ class ContinuousBatcher:
     def __init__(self, backends, slots=3, max_batch=8, memory_fraction=0.85, graphs=True, max_graphs=32,
                  graph_bytes=1 << 30):
         self.backends = dict(backends)
+        for backend in self.backends.values():
+            # admission budgets the adjoint at checkpoint stride 1; keeping every intermediate would outgrow it
+            backend.model.engine().full_save = False
         self.slots = int(slots)
         self.max_batch = int(max_batch)
         self.graphs_enabled = bool(graphs)

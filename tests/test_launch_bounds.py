@@ -29,6 +29,7 @@ def test_engine_launches_stay_in_bounds(B, D, H, W, stride):
     assert n > 100
     if stride == 1:
         assert check_shape(_model(layers=2), B, D, H, W, stride=1, two_pass=False) > 20
+        assert check_shape(_model(layers=2), B, D, H, W, stride=0) > 20          # every intermediate kept
 
 
 @pytest.mark.parametrize("width,rank", [(96, 8), (32, 0), (16, 4)])
