@@ -37,14 +37,14 @@ class FfnoTwGemm(ctypes.Structure):
                 ("m_pad", ctypes.c_int), ("k_pad", ctypes.c_int), ("batch", ctypes.c_int), ("x", c_ptr),
                 ("y", c_ptr), ("b_split", c_i64), ("x_b_hi", c_i64), ("x_b_lo", c_i64), ("y_b_hi", c_i64),
                 ("y_b_lo", c_i64), ("k_split", c_i64), ("x_k_hi", c_i64), ("x_k_lo", c_i64), ("m_split", c_i64),
-                ("y_m_hi", c_i64), ("y_m_lo", c_i64), ("accumulate", ctypes.c_int)]
+                ("y_m_hi", c_i64), ("y_m_lo", c_i64), ("accumulate", ctypes.c_int), ("a_l3", c_ptr)]
 
 
 class FfnoMix(ctypes.Structure):
     _fields_ = [("re", c_ptr), ("im", c_ptr), ("C", ctypes.c_int), ("rows", ctypes.c_int), ("modes", ctypes.c_int),
                 ("rank", ctypes.c_int), ("mode_stride", c_i64), ("r_split", c_i64), ("r_hi", c_i64),
                 ("r_lo", c_i64), ("wc", c_ptr), ("gate", c_ptr), ("v_p", c_ptr), ("u_p", c_ptr),
-                ("factor", c_ptr)]
+                ("factor", c_ptr), ("parts", ctypes.c_int)]
 
 
 class FfnoGnBack(ctypes.Structure):

@@ -176,6 +176,84 @@ __device__ __forceinline__ void mlp2(Wm W1, Wm W2, const Frag* in, f32x4 (&out)[
   }
 }
 
+// ---------------------------------------------------------------------------------------------------------------
+// Three-way split for the forward's pointwise layers: x = h + m + l, h = bf16(x), m = bf16(x - h),
+// l = bf16(x - h - m), which holds all 24 bits of a float32. Of the nine partial products the six of order <= 2
+// (hh, hm, mh, hl, mm, lh; each split level is 2^-8 smaller) are kept, so a layer is exact to about 2^-24 relative
+// (float32 level). The two-way form above drops the order-2 terms and leaves ~2^-17 per layer, which eight blocks
+// amplify past 1e-3 in log10 n; the adjoint keeps the two-way form (its tolerance is 1e-2). A weight packed in three
+// planes [h, m, l] is read by the two-way kernels through its first two planes, which are its two-way split.
+// ---------------------------------------------------------------------------------------------------------------
+struct Frag3 {
+  bf16x8 h, m, l;
+};
+
+struct Wm3 {
+  const bf16* h;
+  const bf16* m;
+  const bf16* l;
+};
+
+__device__ __forceinline__ Frag3 split8_3(f32x4 a, f32x4 b) {
+  Frag3 f;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const float x = j < 4 ? a[j] : b[j - 4];
+    const bf16 h = (bf16)x;
+    const float r = x - (float)h;              // exact
+    const bf16 m = (bf16)r;
+    f.h[j] = h;
+    f.m[j] = m;
+    f.l[j] = (bf16)(r - (float)m);             // r - m exact
+  }
+  return f;
+}
+
+// the small terms first, so that they meet the accumulator before the large ones
+__device__ __forceinline__ f32x4 mfma6(bf16x8 ah, bf16x8 am, bf16x8 al, const Frag3& b, f32x4 c) {
+  c = mfma(al, b.h, c);
+  c = mfma(am, b.m, c);
+  c = mfma(ah, b.l, c);
+  c = mfma(am, b.h, c);
+  c = mfma(ah, b.m, c);
+  return mfma(ah, b.h, c);
+}
+
+template <int NT>
+__device__ __forceinline__ void to_frags(const f32x4 (&a)[NT], Frag3* f) {
+#pragma unroll
+  for (int s = 0; s < ks_of(NT); ++s) f[s] = split8_3(a[2 * s], (2 * s + 1 < NT) ? a[2 * s + 1] : zero4());
+}
+
+template <int KS>
+__device__ __forceinline__ f32x4 tile(Wm3 W, int t, const Frag3* b, int v, int q) {
+  f32x4 c = zero4();
+#pragma unroll
+  for (int s = 0; s < KS; ++s) {
+    const int o = (16 * t + v) * (KS * 32) + 32 * s + 8 * q;
+    c = mfma6(*(const bf16x8*)(W.h + o), *(const bf16x8*)(W.m + o), *(const bf16x8*)(W.l + o), b[s], c);
+  }
+  return c;
+}
+
+template <int KS1, int NH, int NT2>
+__device__ __forceinline__ void mlp2(Wm3 W1, Wm3 W2, const Frag3* in, f32x4 (&out)[NT2], int v, int q) {
+  constexpr int KS2 = ks_of(NH);
+#pragma unroll
+  for (int t = 0; t < NT2; ++t) out[t] = zero4();
+#pragma unroll 1
+  for (int s = 0; s < KS2; ++s) {
+    const f32x4 h0 = gelu4(tile<KS1>(W1, 2 * s, in, v, q));
+    const f32x4 h1 = (2 * s + 1 < NH) ? gelu4(tile<KS1>(W1, 2 * s + 1, in, v, q)) : zero4();
+    const Frag3 hf = split8_3(h0, h1);
+#pragma unroll
+    for (int t = 0; t < NT2; ++t) {
+      const int o = (16 * t + v) * (KS2 * 32) + 32 * s + 8 * q;
+      out[t] = mfma6(*(const bf16x8*)(W2.h + o), *(const bf16x8*)(W2.m + o), *(const bf16x8*)(W2.l + o), hf, out[t]);
+    }
+  }
+}
+
 __device__ __forceinline__ void stage(bf16* dst, const uint16_t* src, int n) {   // n bf16, multiple of 8
   const uint4* s = reinterpret_cast<const uint4*>(src);
   uint4* d = reinterpret_cast<uint4*>(dst);
@@ -341,6 +419,55 @@ __device__ __forceinline__ Wm stage_w(bf16*& lds, const uint16_t* g, int n) {   
 __device__ __forceinline__ Wm global_w(const uint16_t* g, int n) {
   return Wm{reinterpret_cast<const bf16*>(g), reinterpret_cast<const bf16*>(g) + n};
 }
+
+// a three-plane weight [3][rows][K]: its first NL planes (0, 2 or 3) staged into LDS, the rest read from global
+template <int NL>
+__device__ __forceinline__ Wm3 stage_w3(bf16*& lds, const uint16_t* g, int n) {
+  const bf16* gb = reinterpret_cast<const bf16*>(g);
+  if (NL == 0) return Wm3{gb, gb + n, gb + 2 * n};
+  stage(lds, g, NL * n);
+  const Wm3 w{lds, lds + n, NL == 3 ? lds + 2 * n : gb + 2 * n};
+  lds += NL * n;
+  return w;
+}
+
+__device__ __forceinline__ Wm3 global_w3(const uint16_t* g, int n) {
+  const bf16* gb = reinterpret_cast<const bf16*>(g);
+  return Wm3{gb, gb + n, gb + 2 * n};
+}
+
+// the two split forms behind one name, for kernels that run either (THREE: the forward's three-way products)
+template <bool THREE>
+struct SplitT {
+  using F = Frag;
+  using W = Wm;
+  static constexpr int P = 2;
+  __device__ static F split(f32x4 a, f32x4 b) { return split8(a, b); }
+  __device__ static W global(const uint16_t* g, int n) { return global_w(g, n); }
+};
+template <>
+struct SplitT<true> {
+  using F = Frag3;
+  using W = Wm3;
+  static constexpr int P = 3;
+  __device__ static F split(f32x4 a, f32x4 b) { return split8_3(a, b); }
+  __device__ static W global(const uint16_t* g, int n) { return global_w3(g, n); }
+};
+
+// LDS planes per three-plane weight for a kernel whose weights hold `elems` elements per plane next to `extra`
+// bytes of other LDS use: 3 when everything fits a CU's 160 KB, else 2, else 0
+inline int lds_planes(size_t elems, size_t extra) {
+  if (3 * elems * 2 + extra <= 160 * 1024) return 3;
+  if (2 * elems * 2 + extra <= 160 * 1024) return 2;
+  return 0;
+}
+
+#define FFNO_DISPATCH_NL(nl, ...)                          \
+  switch (nl) {                                            \
+    case 3: { constexpr int NL = 3; __VA_ARGS__; break; }  \
+    case 2: { constexpr int NL = 2; __VA_ARGS__; break; }  \
+    default: { constexpr int NL = 0; __VA_ARGS__; break; } \
+  }
 
 __device__ __forceinline__ int64_t imap(int64_t i, int64_t split, int64_t hi, int64_t lo) {
   return (i / split) * hi + (i % split) * lo;

@@ -103,7 +103,7 @@ __global__ void k_gn_finalize(int nwg, int C, const float* __restrict__ partials
 // ---------------------------------------------------------------------------------------------------------------
 // block entry: residual assembly + both input gates
 // ---------------------------------------------------------------------------------------------------------------
-template <int CT, int NW>
+template <int CT, int NW, int NL>
 __global__ __launch_bounds__(NW * 64) void k_block_in(FfnoRows rows, FfnoAssemble in, float* x_out,
                                                       const uint16_t* g1, const uint16_t* g2, const uint16_t* v1,
                                                       const uint16_t* v2, const float* __restrict__ z_proj,
@@ -111,8 +111,8 @@ __global__ __launch_bounds__(NW * 64) void k_block_in(FfnoRows rows, FfnoAssembl
   constexpr int C = CT * 16, KS = ks_of(CT), SZ = C * KS * 32;
   extern __shared__ __align__(16) unsigned char smem[];
   bf16* lds = reinterpret_cast<bf16*>(smem);
-  const Wm G1 = stage_w(lds, g1, SZ), G2 = stage_w(lds, g2, SZ);
-  const Wm V1 = stage_w(lds, v1, SZ), V2 = stage_w(lds, v2, SZ);
+  const Wm3 G1 = stage_w3<NL>(lds, g1, SZ), G2 = stage_w3<NL>(lds, g2, SZ);
+  const Wm3 V1 = stage_w3<NL>(lds, v1, SZ), V2 = stage_w3<NL>(lds, v2, SZ);
   __syncthreads();
   Lane L;
   const int b = blockIdx.y;
@@ -125,7 +125,7 @@ __global__ __launch_bounds__(NW * 64) void k_block_in(FfnoRows rows, FfnoAssembl
     f32x4 x[CT];
     assemble<CT>(in, scale, row, b, ok, L.q, x);
     if (x_out) store_act<CT>(x_out, row, ok, L.q, x);
-    Frag xf[KS];
+    Frag3 xf[KS];
     to_frags<CT>(x, xf);
     f32x4 g[CT];
     // horizontal gate (models/ffno_model.py:439-440)
@@ -148,13 +148,13 @@ __global__ __launch_bounds__(NW * 64) void k_block_in(FfnoRows rows, FfnoAssembl
 // ---------------------------------------------------------------------------------------------------------------
 // post MLP C -> 2C -> C with statistics
 // ---------------------------------------------------------------------------------------------------------------
-template <int CT, int NW>
+template <int CT, int NW, int NL>
 __global__ __launch_bounds__(NW * 64) void k_post(FfnoRows rows, const float* u, const uint16_t* w1,
                                                   const uint16_t* w2, float* y, FfnoStatsOut st) {
   constexpr int C = CT * 16, KS1 = ks_of(CT);
   extern __shared__ __align__(16) unsigned char smem[];
   bf16* lds = reinterpret_cast<bf16*>(smem);
-  const Wm W1 = stage_w(lds, w1, 2 * C * KS1 * 32), W2 = stage_w(lds, w2, C * 2 * C);
+  const Wm3 W1 = stage_w3<NL>(lds, w1, 2 * C * KS1 * 32), W2 = stage_w3<NL>(lds, w2, C * 2 * C);
   float* red = reinterpret_cast<float*>(lds);
   __syncthreads();
   Lane L;
@@ -168,7 +168,7 @@ __global__ __launch_bounds__(NW * 64) void k_post(FfnoRows rows, const float* u,
     const int64_t row = (int64_t)b * rows.voxels + vox;
     f32x4 x[CT];
     load_act<CT>(u, row, ok, L.q, x);
-    Frag xf[KS1];
+    Frag3 xf[KS1];
     to_frags<CT>(x, xf);
     f32x4 o[CT];
     mlp2<KS1, 2 * CT, CT>(W1, W2, xf, o, L.v, L.q);
@@ -263,14 +263,14 @@ __global__ __launch_bounds__(NW * 64) void k_fuse(FfnoRows rows, const float* __
 // ---------------------------------------------------------------------------------------------------------------
 // residual pointwise path
 // ---------------------------------------------------------------------------------------------------------------
-template <int CT, int NW>
+template <int CT, int NW, int NL>
 __global__ __launch_bounds__(NW * 64) void k_resmlp(FfnoRows rows, FfnoAssemble in, float* x_out,
                                                     const uint16_t* w1, const uint16_t* w2, float* y,
                                                     FfnoStatsOut st) {
   constexpr int C = CT * 16, KS1 = ks_of(CT);
   extern __shared__ __align__(16) unsigned char smem[];
   bf16* lds = reinterpret_cast<bf16*>(smem);
-  const Wm W1 = stage_w(lds, w1, 2 * C * KS1 * 32), W2 = stage_w(lds, w2, C * 2 * C);
+  const Wm3 W1 = stage_w3<NL>(lds, w1, 2 * C * KS1 * 32), W2 = stage_w3<NL>(lds, w2, C * 2 * C);
   float* red = reinterpret_cast<float*>(lds);
   __syncthreads();
   Lane L;
@@ -286,7 +286,7 @@ __global__ __launch_bounds__(NW * 64) void k_resmlp(FfnoRows rows, FfnoAssemble 
     f32x4 x[CT];
     assemble<CT>(in, scale, row, b, ok, L.q, x);
     if (x_out) store_act<CT>(x_out, row, ok, L.q, x);
-    Frag xf[KS1];
+    Frag3 xf[KS1];
     to_frags<CT>(x, xf);
     f32x4 o[CT];
     mlp2<KS1, 2 * CT, CT>(W1, W2, xf, o, L.v, L.q);
@@ -302,7 +302,7 @@ __global__ __launch_bounds__(NW * 64) void k_resmlp(FfnoRows rows, FfnoAssemble 
 // ---------------------------------------------------------------------------------------------------------------
 constexpr float LN10 = 2.302585092994046f;
 
-template <int CT, int NW, int NTH>
+template <int CT, int NW, int NTH, int NL>
 __global__ __launch_bounds__(NW * 64) void k_head(FfnoRows rows, FfnoAssemble in, float* x_out,
                                                   const uint16_t* p1,
                                                   const uint16_t* p2, const float* __restrict__ p2_bias,
@@ -311,7 +311,7 @@ __global__ __launch_bounds__(NW * 64) void k_head(FfnoRows rows, FfnoAssemble in
   constexpr int C = CT * 16, KS1 = ks_of(CT);
   extern __shared__ __align__(16) unsigned char smem[];
   bf16* lds = reinterpret_cast<bf16*>(smem);
-  const Wm P1 = stage_w(lds, p1, 2 * C * KS1 * 32), P2 = stage_w(lds, p2, NTH * 16 * 2 * C);
+  const Wm3 P1 = stage_w3<NL>(lds, p1, 2 * C * KS1 * 32), P2 = stage_w3<NL>(lds, p2, NTH * 16 * 2 * C);
   float* hs = reinterpret_cast<float*>(lds);   // [NW][16 voxels][NTH*16]
   __syncthreads();
   Lane L;
@@ -329,7 +329,7 @@ __global__ __launch_bounds__(NW * 64) void k_head(FfnoRows rows, FfnoAssemble in
     assemble<CT>(in, scale, row, b, ok, L.q, x);
     if (x_out) store_act<CT>(x_out, row, ok, L.q, x);
     if (!y) continue;                            // only the head input wanted (uniform: no barrier in this loop)
-    Frag xf[KS1];
+    Frag3 xf[KS1];
     to_frags<CT>(x, xf);
     f32x4 o[NTH];
     mlp2<KS1, 2 * CT, NTH>(P1, P2, xf, o, L.v, L.q);
@@ -382,6 +382,41 @@ __global__ __launch_bounds__(NW * 64) void k_head(FfnoRows rows, FfnoAssemble in
 // ---------------------------------------------------------------------------------------------------------------
 // split-bf16 twiddle GEMM
 // ---------------------------------------------------------------------------------------------------------------
+// split of a column group: 8 k-values of the lane's 4 columns into 2 or 3 bf16 planes
+template <bool THREE>
+__device__ __forceinline__ void split_cols(const f32x4 (&xv)[8], bf16x8 (&bh)[4], bf16x8 (&bm)[4], bf16x8 (&bl)[4]) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const float x = xv[j][i];
+      const bf16 h = (bf16)x;
+      const float r = x - (float)h;
+      const bf16 m = (bf16)r;
+      bh[i][j] = h;
+      bm[i][j] = m;
+      if (THREE) bl[i][j] = (bf16)(r - (float)m);
+    }
+}
+
+// acc += A-fragment (ah, am[, al]) x the lane's 4 column fragments
+template <bool THREE>
+__device__ __forceinline__ void tw_mac(bf16x8 ah, bf16x8 am, bf16x8 al, const bf16x8 (&bh)[4], const bf16x8 (&bm)[4],
+                                       const bf16x8 (&bl)[4], f32x4 (&acc)[4]) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (THREE) {
+      acc[i] = mfma(al, bh[i], acc[i]);
+      acc[i] = mfma(am, bm[i], acc[i]);
+      acc[i] = mfma(ah, bl[i], acc[i]);
+    }
+    acc[i] = mfma(ah, bm[i], acc[i]);
+    acc[i] = mfma(am, bh[i], acc[i]);
+    acc[i] = mfma(ah, bh[i], acc[i]);
+  }
+}
+
+template <bool THREE>
 __global__ __launch_bounds__(256) void k_tw_gemm(FfnoTwGemm g) {
   Lane L;
   const int64_t ntn = (g.n + 63) / 64;                 // 64-column tasks per batch item
@@ -406,28 +441,17 @@ __global__ __launch_bounds__(256) void k_tw_gemm(FfnoTwGemm g) {
           const int k = 32 * s + 8 * L.q + j;
           xv[j] = (nok && k < g.k) ? *(const f32x4*)(xb + imap(k, g.k_split, g.x_k_hi, g.x_k_lo)) : zero4();
         }
-        bf16x8 bh[4], bl[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int j = 0; j < 8; ++j) {
-            const bf16 h = (bf16)xv[j][i];
-            bh[i][j] = h;
-            bl[i][j] = (bf16)(xv[j][i] - (float)h);
-          }
+        bf16x8 bh[4], bm[4], bl[4];
+        split_cols<THREE>(xv, bh, bm, bl);
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
           const int mt = mb * 4 + a;
           if (mt * 16 >= g.m_pad) break;
           const int64_t ao = (int64_t)(mt * 16 + L.v) * g.k_pad + 32 * s + 8 * L.q;
           const bf16x8 ah = *(const bf16x8*)(g.a_hi + ao);
-          const bf16x8 al = *(const bf16x8*)(g.a_lo + ao);
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            acc[a][i] = mfma(ah, bh[i], acc[a][i]);
-            acc[a][i] = mfma(ah, bl[i], acc[a][i]);
-            acc[a][i] = mfma(al, bh[i], acc[a][i]);
-          }
+          const bf16x8 am = *(const bf16x8*)(g.a_lo + ao);
+          const bf16x8 al = THREE ? *(const bf16x8*)(g.a_l3 + ao) : ah;
+          tw_mac<THREE>(ah, am, al, bh, bm, bl, acc[a]);
         }
       }
 #pragma unroll
@@ -455,23 +479,26 @@ __global__ __launch_bounds__(256) void k_tw_gemm(FfnoTwGemm g) {
 // loaded while the current one's MFMAs run
 constexpr int TW_PAD = 8;
 
-template <int MT>
+template <int MT, bool THREE>
 __global__ __launch_bounds__(256) void k_tw_small(FfnoTwGemm g) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int kp = g.k_pad, ldk = kp + TW_PAD;
   int64_t* koff = reinterpret_cast<int64_t*>(smem);
   int64_t* moff = koff + kp;
   bf16* ah = reinterpret_cast<bf16*>(moff + MT * 16);
-  bf16* al = ah + MT * 16 * ldk;
+  bf16* am = ah + MT * 16 * ldk;
+  bf16* al = am + MT * 16 * ldk;          // third plane (THREE only)
   for (int i = threadIdx.x; i < MT * 16 * (kp / 8); i += blockDim.x) {
     const int r = i / (kp / 8), c = (i % (kp / 8)) * 8;
-    uint4 h = {0, 0, 0, 0}, l = {0, 0, 0, 0};
+    uint4 h = {0, 0, 0, 0}, m = {0, 0, 0, 0}, l = {0, 0, 0, 0};
     if (r < g.m_pad) {
       h = *reinterpret_cast<const uint4*>(g.a_hi + (int64_t)r * kp + c);
-      l = *reinterpret_cast<const uint4*>(g.a_lo + (int64_t)r * kp + c);
+      m = *reinterpret_cast<const uint4*>(g.a_lo + (int64_t)r * kp + c);
+      if (THREE) l = *reinterpret_cast<const uint4*>(g.a_l3 + (int64_t)r * kp + c);
     }
     *reinterpret_cast<uint4*>(ah + r * ldk + c) = h;
-    *reinterpret_cast<uint4*>(al + r * ldk + c) = l;
+    *reinterpret_cast<uint4*>(am + r * ldk + c) = m;
+    if (THREE) *reinterpret_cast<uint4*>(al + r * ldk + c) = l;
   }
   for (int k = threadIdx.x; k < kp; k += blockDim.x) koff[k] = k < g.k ? imap(k, g.k_split, g.x_k_hi, g.x_k_lo) : -1;
   for (int m = threadIdx.x; m < MT * 16; m += blockDim.x)
@@ -499,15 +526,8 @@ __global__ __launch_bounds__(256) void k_tw_small(FfnoTwGemm g) {
       xv[j] = (nok && o >= 0) ? *(const f32x4*)(xb + o) : zero4();
     }
     for (int s = 0; s < ks; ++s) {
-      bf16x8 bh[4], bl[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          const bf16 h = (bf16)xv[j][i];
-          bh[i][j] = h;
-          bl[i][j] = (bf16)(xv[j][i] - (float)h);
-        }
+      bf16x8 bh[4], bm[4], bl[4];
+      split_cols<THREE>(xv, bh, bm, bl);
       if (s + 1 < ks) {
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
@@ -519,13 +539,9 @@ __global__ __launch_bounds__(256) void k_tw_small(FfnoTwGemm g) {
       for (int a = 0; a < MT; ++a) {
         const int ao = (a * 16 + L.v) * ldk + 32 * s + 8 * L.q;
         const bf16x8 fh = *reinterpret_cast<const bf16x8*>(ah + ao);
-        const bf16x8 fl = *reinterpret_cast<const bf16x8*>(al + ao);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          acc[a][i] = mfma(fh, bh[i], acc[a][i]);
-          acc[a][i] = mfma(fh, bl[i], acc[a][i]);
-          acc[a][i] = mfma(fl, bh[i], acc[a][i]);
-        }
+        const bf16x8 fm = *reinterpret_cast<const bf16x8*>(am + ao);
+        const bf16x8 fl = THREE ? *reinterpret_cast<const bf16x8*>(al + ao) : fh;
+        tw_mac<THREE>(fh, fm, fl, bh, bm, bl, acc[a]);
       }
     }
 #pragma unroll
@@ -548,12 +564,19 @@ __global__ __launch_bounds__(256) void k_tw_small(FfnoTwGemm g) {
 // ---------------------------------------------------------------------------------------------------------------
 constexpr int MIX_TILES = 8;   // row tiles of 16 per work item of the mix
 
-template <int CT, bool LW>
+// NL: planes of wc staged in LDS (0 or 2 for the two-way form, 0, 2 or 3 for the three-way one)
+template <int CT, int NL, bool THREE>
 __global__ __launch_bounds__(256) void k_mix(FfnoMix mx) {
-  constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32, NWC = 2 * C * 2 * KP;
+  using S = SplitT<THREE>;
+  using F = typename S::F;
+  constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32, NWC = 2 * C * 2 * KP, P = S::P;
   extern __shared__ __align__(16) unsigned char smem[];
   bf16* lds = reinterpret_cast<bf16*>(smem);
-  const Wm Wc = LW ? stage_w(lds, mx.wc, NWC) : global_w(mx.wc, NWC);
+  typename S::W Wc;
+  if constexpr (THREE)
+    Wc = stage_w3<NL>(lds, mx.wc, NWC);
+  else
+    Wc = NL ? stage_w(lds, mx.wc, NWC) : global_w(mx.wc, NWC);
   __syncthreads();
   Lane L;
   // work items are (mode, chunk of MIX_TILES row tiles), mode-major so that neighbouring waves share a mode's tables
@@ -566,9 +589,9 @@ __global__ __launch_bounds__(256) void k_mix(FfnoMix mx) {
     const int rt1 = rt0 + MIX_TILES < rtiles ? rt0 + MIX_TILES : rtiles;
     const float gr = mx.gate[2 * mode], gi = mx.gate[2 * mode + 1];
     const float fac = mx.factor ? mx.factor[mode] : 1.f;
-    // per-mode low-rank tables: V [16][KP] and U [C][32], hi and lo halves of each mode's block
-    const Wm V = global_w(mx.v_p ? mx.v_p + (int64_t)mode * 2 * 16 * KP : mx.wc, 16 * KP);
-    const Wm U = global_w(mx.u_p ? mx.u_p + (int64_t)mode * 2 * C * 32 : mx.wc, C * 32);
+    // per-mode low-rank tables: V [16][KP] and U [C][32], P planes of each mode's block
+    const auto V = S::global(mx.v_p ? mx.v_p + (int64_t)mode * P * 16 * KP : mx.wc, 16 * KP);
+    const auto U = S::global(mx.u_p ? mx.u_p + (int64_t)mode * P * C * 32 : mx.wc, C * 32);
     for (int rt = rt0; rt < rt1; ++rt) {
       const int r = rt * 16 + L.v;
       const bool ok = r < mx.rows;
@@ -579,7 +602,7 @@ __global__ __launch_bounds__(256) void k_mix(FfnoMix mx) {
         yr[t] = ld4(mx.re + base + 16 * t + 4 * L.q, ok);
         yi[t] = ld4(mx.im + base + 16 * t + 4 * L.q, ok);
       }
-      Frag cat[2 * KS];
+      F cat[2 * KS];
       to_frags<CT>(yr, cat);
       to_frags<CT>(yi, cat + KS);
       f32x4 zr[CT], zi[CT];
@@ -594,7 +617,7 @@ __global__ __launch_bounds__(256) void k_mix(FfnoMix mx) {
       if (mx.rank) {
         const f32x4 lr = tile<KS>(V, 0, cat, L.v, L.q);
         const f32x4 li = tile<KS>(V, 0, cat + KS, L.v, L.q);
-        const Frag lrf = split8(lr, zero4()), lif = split8(li, zero4());
+        const F lrf = S::split(lr, zero4()), lif = S::split(li, zero4());
 #pragma unroll
         for (int t = 0; t < CT; ++t) {
           zr[t] += tile<1>(U, t, &lrf, L.v, L.q);
@@ -643,10 +666,14 @@ int ffno_block_in(FfnoRows rows, FfnoAssemble in, float* x_out, const uint16_t* 
     return fail("ffno_block_in: bad arguments");
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {
-    const size_t lds = 4 * (size_t)(CT * 16) * ks_of(CT) * 32 * 2 * 2;
-    auto k = k_block_in<CT, nw_of<CT>()>;
-    if (set_lds(k, lds)) return -1;
-    k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, in, x_out, g1, g2, v1, v2, z_proj, z, xs, xv);
+    const size_t elems = 4 * (size_t)(CT * 16) * ks_of(CT) * 32;
+    const int nl = lds_planes(elems, 0);
+    FFNO_DISPATCH_NL(nl, {
+      const size_t lds = NL * elems * 2;
+      auto k = k_block_in<CT, nw_of<CT>(), NL>;
+      if (set_lds(k, lds)) return -1;
+      k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, in, x_out, g1, g2, v1, v2, z_proj, z, xs, xv);
+    });
   });
   return check_launch("ffno_block_in");
 }
@@ -656,11 +683,15 @@ int ffno_post(FfnoRows rows, const float* u, const uint16_t* w1, const uint16_t*
   if (!rows_ok(rows) || !u || !w1 || !w2 || !y) return fail("ffno_post: bad arguments");
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {
-    const size_t lds = ((size_t)2 * CT * 16 * ks_of(CT) * 32 + (size_t)CT * 16 * 2 * CT * 16) * 2 * 2 +
-                       red_bytes_nw<CT, nw_of<CT>()>();
-    auto k = k_post<CT, nw_of<CT>()>;
-    if (set_lds(k, lds)) return -1;
-    k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, u, w1, w2, y, stats);
+    const size_t elems = (size_t)2 * CT * 16 * ks_of(CT) * 32 + (size_t)CT * 16 * 2 * CT * 16;
+    const size_t red = red_bytes_nw<CT, nw_of<CT>()>();
+    const int nl = lds_planes(elems, red);
+    FFNO_DISPATCH_NL(nl, {
+      const size_t lds = NL * elems * 2 + red;
+      auto k = k_post<CT, nw_of<CT>(), NL>;
+      if (set_lds(k, lds)) return -1;
+      k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, u, w1, w2, y, stats);
+    });
   });
   return check_launch("ffno_post");
 }
@@ -696,11 +727,15 @@ int ffno_resmlp(FfnoRows rows, FfnoAssemble in, float* x_out, const uint16_t* w1
   if (!rows_ok(rows) || !in.t || !in.aff_a || !in.aff_b || !w1 || !w2 || !y) return fail("ffno_resmlp: bad arguments");
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {
-    const size_t lds = ((size_t)2 * CT * 16 * ks_of(CT) * 32 + (size_t)CT * 16 * 2 * CT * 16) * 2 * 2 +
-                       red_bytes_nw<CT, nw_of<CT>()>();
-    auto k = k_resmlp<CT, nw_of<CT>()>;
-    if (set_lds(k, lds)) return -1;
-    k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, in, x_out, w1, w2, y, stats);
+    const size_t elems = (size_t)2 * CT * 16 * ks_of(CT) * 32 + (size_t)CT * 16 * 2 * CT * 16;
+    const size_t red = red_bytes_nw<CT, nw_of<CT>()>();
+    const int nl = lds_planes(elems, red);
+    FFNO_DISPATCH_NL(nl, {
+      const size_t lds = NL * elems * 2 + red;
+      auto k = k_resmlp<CT, nw_of<CT>(), NL>;
+      if (set_lds(k, lds)) return -1;
+      k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, in, x_out, w1, w2, y, stats);
+    });
   });
   return check_launch("ffno_resmlp");
 }
@@ -720,17 +755,21 @@ int ffno_head(FfnoRows rows, FfnoAssemble in, float* x_out, const uint16_t* p1, 
   dim3 grid(rows.nwg, rows.B);
   const int nth = (n_head + 15) / 16;
   FFNO_DISPATCH_CT(rows.C, {
-    const size_t w_bytes = ((size_t)2 * CT * 16 * ks_of(CT) * 32 + (size_t)nth * 16 * 2 * CT * 16) * 2 * 2;
-    const size_t lds = w_bytes + (size_t)nw_of<CT>() * 16 * nth * 16 * sizeof(float);
-    if (nth == 1) {
-      auto k = k_head<CT, nw_of<CT>(), 1>;
-      if (set_lds(k, lds)) return -1;
-      k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, in, x_out, p1, p2, p2_bias, x_in, *desc, y);
-    } else {
-      auto k = k_head<CT, nw_of<CT>(), 2>;
-      if (set_lds(k, lds)) return -1;
-      k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, in, x_out, p1, p2, p2_bias, x_in, *desc, y);
-    }
+    const size_t elems = (size_t)2 * CT * 16 * ks_of(CT) * 32 + (size_t)nth * 16 * 2 * CT * 16;
+    const size_t hs = (size_t)nw_of<CT>() * 16 * nth * 16 * sizeof(float);
+    const int nl = lds_planes(elems, hs);
+    FFNO_DISPATCH_NL(nl, {
+      const size_t lds = NL * elems * 2 + hs;
+      if (nth == 1) {
+        auto k = k_head<CT, nw_of<CT>(), 1, NL>;
+        if (set_lds(k, lds)) return -1;
+        k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, in, x_out, p1, p2, p2_bias, x_in, *desc, y);
+      } else {
+        auto k = k_head<CT, nw_of<CT>(), 2, NL>;
+        if (set_lds(k, lds)) return -1;
+        k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, in, x_out, p1, p2, p2_bias, x_in, *desc, y);
+      }
+    });
   });
   return check_launch("ffno_head");
 }
@@ -742,15 +781,16 @@ int ffno_tw_gemm(const FfnoTwGemm* g, hipStream_t stream) {
     return fail("ffno_tw_gemm: bad sizes");
   const int64_t tasks = (int64_t)g->batch * ((g->n + 63) / 64);
   const int64_t blocks = (tasks + 3) / 4;
+  const bool three = g->a_l3 != nullptr;
   if (g->m_pad <= 128 && g->k_pad <= 128) {
     const int mt = (g->m_pad + 15) / 16;
     const size_t lds = (size_t)(g->k_pad + mt * 16) * sizeof(int64_t) +
-                       (size_t)2 * mt * 16 * (g->k_pad + TW_PAD) * sizeof(uint16_t);
+                       (size_t)(three ? 3 : 2) * mt * 16 * (g->k_pad + TW_PAD) * sizeof(uint16_t);
     const int grid = (int)(blocks < 512 ? blocks : 512);       // persistent: A is staged once per workgroup
     switch (mt) {
 #define FFNO_TW_CASE(M)                                        \
   case M: {                                                    \
-    auto k = k_tw_small<M>;                                    \
+    auto k = three ? k_tw_small<M, true> : k_tw_small<M, false>; \
     if (set_lds(k, lds)) return -1;                            \
     k<<<grid, 256, lds, stream>>>(*g);                         \
     break;                                                     \
@@ -763,7 +803,10 @@ int ffno_tw_gemm(const FfnoTwGemm* g, hipStream_t stream) {
     return check_launch("ffno_tw_gemm");
   }
   const int grid = (int)(blocks < 4096 ? blocks : 4096);
-  k_tw_gemm<<<grid, 256, 0, stream>>>(*g);
+  if (three)
+    k_tw_gemm<true><<<grid, 256, 0, stream>>>(*g);
+  else
+    k_tw_gemm<false><<<grid, 256, 0, stream>>>(*g);
   return check_launch("ffno_tw_gemm");
 }
 
@@ -774,14 +817,21 @@ int ffno_mix(const FfnoMix* m, hipStream_t stream) {
   const int64_t items = (int64_t)m->modes * (((m->rows + 15) / 16 + MIX_TILES - 1) / MIX_TILES);
   const int64_t blocks = (items + 3) / 4;
   const int grid = (int)(blocks < 2048 ? blocks : 2048);
+  if (m->parts != 2 && m->parts != 3) return fail("ffno_mix: parts must be 2 or 3");
   FFNO_DISPATCH_CT(m->C, {
-    const size_t lds = (size_t)2 * CT * 16 * 2 * ks_of(CT) * 32 * 2 * 2;
-    if (lds <= 160 * 1024) {
-      auto k = k_mix<CT, true>;
-      if (set_lds(k, lds)) return -1;
-      k<<<grid, 256, lds, stream>>>(*m);
+    const size_t elems = (size_t)2 * CT * 16 * 2 * ks_of(CT) * 32;
+    if (m->parts == 3) {
+      FFNO_DISPATCH_NL(lds_planes(elems, 0), {
+        auto k = k_mix<CT, NL, true>;
+        if (set_lds(k, NL * elems * 2)) return -1;
+        k<<<grid, 256, NL * elems * 2, stream>>>(*m);
+      });
+    } else if (2 * elems * 2 <= 160 * 1024) {
+      auto k = k_mix<CT, 2, false>;
+      if (set_lds(k, 2 * elems * 2)) return -1;
+      k<<<grid, 256, 2 * elems * 2, stream>>>(*m);
     } else {
-      k_mix<CT, false><<<grid, 256, 0, stream>>>(*m);
+      k_mix<CT, 0, false><<<grid, 256, 0, stream>>>(*m);
     }
   });
   return check_launch("ffno_mix");

@@ -34,29 +34,34 @@ def _p(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
 
 
-def pack_mlp_weight(w, segments=None, n_pad=None):
-    """Split-bf16 [.., 2, N_pad, K_pad] (hi, lo) for the MFMA kernels from a float weight [.., N, K] (PyTorch
-    [out, in]): each input segment padded to a multiple of 32, k permuted within every 32-block as the kernels' lane
-    layout wants (k = 32s + 16h + 4q + j is stored at 32s + 8q + 4h + j), N padded to a multiple of 16;
-    hi = bf16(w), lo = bf16(w - hi)."""
+def pack_mlp_weight(w, segments=None, n_pad=None, parts=2):
+    """Split-bf16 [.., parts, N_pad, K_pad] for the MFMA kernels from a float weight [.., N, K] (PyTorch [out, in]):
+    each input segment padded to a multiple of 32, k permuted within every 32-block as the kernels' lane layout wants
+    (k = 32s + 16h + 4q + j is stored at 32s + 8q + 4h + j), N padded to a multiple of 16. Planes: hi = bf16(w),
+    lo = bf16(w - hi), and with parts=3 a third, bf16(w - hi - lo), for the forward's three-way products (the first
+    two planes are then still the two-way split the adjoint kernels read)."""
     w = w.double()
     lead = w.shape[:-2]
     N, K = w.shape[-2:]
     segs = segments or [K]
     assert sum(segs) == K
-    parts, off = [], 0
+    pieces, off = [], 0
     for s in segs:
-        part = w[..., off:off + s]
+        piece = w[..., off:off + s]
         off += s
-        parts.append(F.pad(part, (0, T.round_up(s, 32) - s)))
-    wp = torch.cat(parts, -1)
+        pieces.append(F.pad(piece, (0, T.round_up(s, 32) - s)))
+    wp = torch.cat(pieces, -1)
     npad = n_pad or T.round_up(N, 16)
     wp = F.pad(wp, (0, 0, 0, npad - N))
     kp = wp.shape[-1]
     wp = wp.reshape(*lead, npad, kp // 32, 2, 4, 4).transpose(-3, -2).reshape(*lead, npad, kp)
-    hi = wp.to(torch.bfloat16)
-    lo = (wp - hi.double()).to(torch.bfloat16)
-    return torch.stack([hi, lo], -3).contiguous().view(torch.int16)
+    planes = []
+    rest = wp
+    for _ in range(parts):
+        plane = rest.to(torch.bfloat16)
+        planes.append(plane)
+        rest = rest - plane.double()
+    return torch.stack(planes, -3).contiguous().view(torch.int16)
 
 
 def _conv_w(p):
@@ -98,14 +103,14 @@ class PackedWeights:
             p = f"blocks.{i}."
             b = {}
             for br in ("spec", "vertical"):
-                b[br + ".g1"] = pack_mlp_weight(_conv_w(sd[p + br + ".input_gate.0.weight"]))
-                b[br + ".g2"] = pack_mlp_weight(_conv_w(sd[p + br + ".input_gate.2.weight"]))
-                b[br + ".p1"] = pack_mlp_weight(_conv_w(sd[p + br + ".post.0.weight"]))
-                b[br + ".p2"] = pack_mlp_weight(_conv_w(sd[p + br + ".post.2.weight"]))
+                b[br + ".g1"] = pack_mlp_weight(_conv_w(sd[p + br + ".input_gate.0.weight"]), parts=3)
+                b[br + ".g2"] = pack_mlp_weight(_conv_w(sd[p + br + ".input_gate.2.weight"]), parts=3)
+                b[br + ".p1"] = pack_mlp_weight(_conv_w(sd[p + br + ".post.0.weight"]), parts=3)
+                b[br + ".p2"] = pack_mlp_weight(_conv_w(sd[p + br + ".post.2.weight"]), parts=3)
                 wr, wi = sd[p + br + ".weight_real"].float(), sd[p + br + ".weight_imag"].float()   # [in, out]
                 # [Ar; Ai] = [[Wr^T, -Wi^T], [Wi^T, Wr^T]] [Yr; Yi]: the complex product Y (Wr + i Wi) over channels
                 wc = torch.cat([torch.cat([wr.t(), -wi.t()], 1), torch.cat([wi.t(), wr.t()], 1)], 0)
-                b[br + ".wc"] = pack_mlp_weight(wc, segments=[C, C])
+                b[br + ".wc"] = pack_mlp_weight(wc, segments=[C, C], parts=3)
                 if br == "spec":
                     self.wc_real.append({})
                 self.wc_real[-1][br] = wc
@@ -115,17 +120,17 @@ class PackedWeights:
             b["bg2_bias"] = sd[p + "branch_gate.2.bias"].float().contiguous()
             b["f1"] = pack_mlp_weight(_conv_w(sd[p + "fuse.0.weight"]), segments=[C, C])
             b["f2"] = pack_mlp_weight(_conv_w(sd[p + "fuse.2.weight"]))
-            b["pw1"] = pack_mlp_weight(_conv_w(sd[p + "pw.0.weight"]))
-            b["pw2"] = pack_mlp_weight(_conv_w(sd[p + "pw.2.weight"]))
-            b["m1"] = pack_mlp_weight(_conv_w(sd[p + "mlp.net.0.weight"]))
-            b["m2"] = pack_mlp_weight(_conv_w(sd[p + "mlp.net.3.weight"]))
+            b["pw1"] = pack_mlp_weight(_conv_w(sd[p + "pw.0.weight"]), parts=3)
+            b["pw2"] = pack_mlp_weight(_conv_w(sd[p + "pw.2.weight"]), parts=3)
+            b["m1"] = pack_mlp_weight(_conv_w(sd[p + "mlp.net.0.weight"]), parts=3)
+            b["m2"] = pack_mlp_weight(_conv_w(sd[p + "mlp.net.3.weight"]), parts=3)
             for n in ("norm_spec", "norm_vert", "norm_fuse", "norm_pw", "norm_mlp"):
                 b[n] = (sd[p + n + ".weight"].float().contiguous(), sd[p + n + ".bias"].float().contiguous())
             for n in ("res_fused", "res_pw", "res_mlp"):
                 b[n] = sd[p + n].float().contiguous()
             self.blocks.append(b)
-        self.p1 = pack_mlp_weight(_conv_w(sd["proj1.weight"]))
-        self.p2 = pack_mlp_weight(_conv_w(sd["proj2.weight"]))
+        self.p1 = pack_mlp_weight(_conv_w(sd["proj1.weight"]), parts=3)
+        self.p2 = pack_mlp_weight(_conv_w(sd["proj2.weight"]), parts=3)
         nh = sd["proj2.weight"].shape[0]
         self.p2_bias = F.pad(sd["proj2.bias"].float(), (0, T.round_up(nh, 16) - nh)).contiguous()
         self.head = self._head_desc(module, module.state_dict())     # host scalars, read where the module lives
@@ -213,7 +218,7 @@ class Plan:
         self.modes = len(k)
         v_fwd, v_inv = T.vertical_matrices(column, k, factor, self.length)
         hm = T.horizontal_matrices(H, W)
-        self.tw = {name: T.split_bf16(a, dev) for name, a in
+        self.tw = {name: T.split_bf16(a, dev, parts=3) for name, a in
                    (("w_fwd", hm["w_fwd"]), ("h_fwd", hm["h_fwd"]), ("h_inv", hm["h_inv"]), ("w_inv", hm["w_inv"]),
                     ("v_fwd", v_fwd), ("v_inv", v_inv))}
         # the adjoint chain runs the same GEMMs with the transposed matrices in reverse order
@@ -262,8 +267,8 @@ class Plan:
             e = dict(gate=gate, gate_adj=g_adj.contiguous(), v=None, u=None, v_adj=None, u_adj=None)
             if r:
                 u, v = _freq_lowrank(prefix, self._sd, feats, C, r)
-                e["v"] = pack_mlp_weight(v, n_pad=16)                      # [modes, 2, 16, KS*32]
-                e["u"] = pack_mlp_weight(u)                                # [modes, 2, C, 32]
+                e["v"] = pack_mlp_weight(v, n_pad=16, parts=3)             # [modes, 3, 16, KS*32]
+                e["u"] = pack_mlp_weight(u, parts=3)                       # [modes, 3, C, 32]
                 e["v_adj"] = pack_mlp_weight(u.transpose(-1, -2), n_pad=16)
                 e["u_adj"] = pack_mlp_weight(v.transpose(-1, -2))
             out[br] = e
@@ -457,9 +462,10 @@ class Engine:
         return a, b, stats
 
     def _tw(self, mat, x, y, batch, n, bmap_x, bmap_y, kmap, mmap, accumulate=0):
-        hi, lo, m, k, mp, kp = mat
+        hi, lo, m, k, mp, kp = mat[:6]
         g = FfnoTwGemm()
         g.a_hi, g.a_lo = _p(hi), _p(lo)
+        g.a_l3 = _p(mat[6] if len(mat) > 6 else None)
         g.m, g.k, g.n, g.m_pad, g.k_pad, g.batch = m, k, n, mp, kp, batch
         g.x, g.y = _p(x), _p(y)
         g.b_split, g.x_b_hi, g.x_b_lo = bmap_x
@@ -479,6 +485,7 @@ class Engine:
         mx.wc, mx.gate = _p(wc), _p(e["gate" + sfx])
         mx.v_p, mx.u_p = _p(e["v" + sfx]), _p(e["u" + sfx])
         mx.factor = ctypes.c_void_p(0)
+        mx.parts = 2 if adjoint else 3
         _lib.call("ffno_mix", ctypes.byref(mx), self._stream())
 
     def horizontal(self, plan, xs, out, i, scr, tables, adjoint=False):

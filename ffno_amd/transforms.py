@@ -108,17 +108,22 @@ def vertical_matrices(column, k, factor, length):
     return v_fwd, v_inv
 
 
-def split_bf16(a, device):
-    """(hi, lo) bf16 [m_pad, k_pad] of a float64 matrix [m, k]: hi = bf16(a), lo = bf16(a - hi), zero padded."""
+def split_bf16(a, device, parts=2):
+    """(hi, lo, m, k, m_pad, k_pad[, l3]) of a float64 matrix [m, k] as bf16 planes [m_pad, k_pad], zero padded:
+    hi = bf16(a), lo = bf16(a - hi), and with parts=3 l3 = bf16(a - hi - lo) (the forward transforms' three-way
+    products)."""
     a = np.asarray(a, np.float64)
     m, k = a.shape
     mp, kp = round_up(m, 16), round_up(k, 32)
     full = np.zeros((mp, kp))
     full[:m, :k] = a
-    t = torch.from_numpy(full)
-    hi = t.to(torch.bfloat16)
-    lo = (t - hi.double()).to(torch.bfloat16)
-    return (hi.view(torch.int16).to(device), lo.view(torch.int16).to(device), m, k, mp, kp)
+    rest = torch.from_numpy(full)
+    planes = []
+    for _ in range(parts):
+        p = rest.to(torch.bfloat16)
+        planes.append(p.view(torch.int16).to(device))
+        rest = rest - p.double()
+    return (planes[0], planes[1], m, k, mp, kp) + tuple(planes[2:])
 
 
 def filon_weights_torch(z, k):

@@ -111,10 +111,12 @@ int ffno_head(FfnoRows rows, FfnoAssemble in, float* x_out, const uint16_t* p1, 
               const float* p2_bias, const float* x_in /* [B,6,DHW] */, const FfnoHeadDesc* desc, float* y,
               hipStream_t stream);
 
-/* Split-bf16 ("bf16x3") GEMM of a small precomputed real matrix A [m x k] against batched data X [k x n]:
+/* Split-bf16 GEMM of a small precomputed real matrix A [m x k] against batched data X [k x n]:
  * Y[b][m][n] = sum_k A[m][k] X[b][k][n]. It carries every DFT (rfft/ifft/irfft along W and H) and the vertical
  * Filon transform and its inverse (models/ffno_model.py:441, :448, :272-275, :303-305). Row offsets come from
- * two-level index maps: offset(i) = (i / split) * hi + (i % split) * lo. */
+ * two-level index maps: offset(i) = (i / split) * hi + (i % split) * lo. Two planes of A (a_hi, a_lo): three
+ * products per MFMA pair (~2^-17 relative); with a third plane a_l3, A and X are split three ways and the six
+ * products of order <= 2 are summed (float32 level, the forward's transforms). */
 typedef struct {
   const uint16_t* a_hi;
   const uint16_t* a_lo;
@@ -127,6 +129,7 @@ typedef struct {
   int64_t k_split, x_k_hi, x_k_lo;
   int64_t m_split, y_m_hi, y_m_lo;
   int accumulate;         /* 1: Y += A X */
+  const uint16_t* a_l3;   /* third plane of A or NULL */
 } FfnoTwGemm;
 
 int ffno_tw_gemm(const FfnoTwGemm* g, hipStream_t stream);
@@ -139,11 +142,12 @@ typedef struct {
   float* im;
   int C, rows, modes, rank;
   int64_t mode_stride, r_split, r_hi, r_lo;
-  const uint16_t* wc;     /* packed [2C][2C] */
+  const uint16_t* wc;     /* packed [parts][2C][2C] */
   const float* gate;      /* [modes][2] */
-  const uint16_t* v_p;    /* [modes][16][KS*32] or NULL */
-  const uint16_t* u_p;    /* [modes][C][32] or NULL */
+  const uint16_t* v_p;    /* [modes][parts][16][KS*32] or NULL */
+  const uint16_t* u_p;    /* [modes][parts][C][32] or NULL */
   const float* factor;    /* [modes] or NULL */
+  int parts;              /* planes of wc, v_p, u_p: 2, or 3 for the three-way products of the forward */
 } FfnoMix;
 
 int ffno_mix(const FfnoMix* m, hipStream_t stream);

@@ -86,9 +86,10 @@ class Checker:
         self.span(name, "gn depth_w", g.depth_w, rows.B * rows.voxels * F32)
         self.span(name, "gn wbar", g.wbar, rows.B * rows.voxels * F32, optional=True)
 
-    def weight(self, name, what, ptr, n_rows, k):
-        """A packed split-bf16 matrix [2][n_rows][round_up(k, 32)] as the kernels stage it."""
-        self.span(name, what, ptr, 2 * n_rows * round_up(k, 32) * BF)
+    def weight(self, name, what, ptr, n_rows, k, parts=2):
+        """A packed split-bf16 matrix [parts][n_rows][round_up(k, 32)] as the kernels stage it (parts=3: the forward's
+        three-way pointwise layers)."""
+        self.span(name, what, ptr, parts * n_rows * round_up(k, 32) * BF)
 
     @staticmethod
     def imap_max(count, split, hi, lo):
@@ -124,7 +125,7 @@ class Checker:
         self.asm(n, a, rows)
         self.act(n, "x_out", x_out, rows, optional=True)
         for what, w in (("g1", g1), ("g2", g2), ("v1", v1), ("v2", v2)):
-            self.weight(n, what, w, C, C)
+            self.weight(n, what, w, C, C, parts=3)
         self.span(n, "z_proj", z_proj, C * F32)
         self.span(n, "z", z, rows.B * rows.voxels * F32)
         self.act(n, "xs", xs, rows)
@@ -133,8 +134,8 @@ class Checker:
     def ffno_post(self, n, rows, u, w1, w2, y, st, stream):
         C = rows.C
         self.act(n, "u", u, rows)
-        self.weight(n, "w1", w1, 2 * C, C)
-        self.weight(n, "w2", w2, C, 2 * C)
+        self.weight(n, "w1", w1, 2 * C, C, parts=3)
+        self.weight(n, "w2", w2, C, 2 * C, parts=3)
         self.act(n, "y", y, rows)
         self.stats(n, st, rows)
 
@@ -156,8 +157,8 @@ class Checker:
         C = rows.C
         self.asm(n, a, rows)
         self.act(n, "x_out", x_out, rows, optional=True)
-        self.weight(n, "w1", w1, 2 * C, C)
-        self.weight(n, "w2", w2, C, 2 * C)
+        self.weight(n, "w1", w1, 2 * C, C, parts=3)
+        self.weight(n, "w2", w2, C, 2 * C, parts=3)
         self.act(n, "y", y, rows)
         self.stats(n, st, rows)
 
@@ -169,8 +170,8 @@ class Checker:
         C = rows.C
         self.asm(n, a, rows)
         self.act(n, "x_out", x_out, rows, optional=True)
-        self.weight(n, "p1", p1, 2 * C, C)
-        self.span(n, "p2", p2, 2 * nth * 16 * 2 * C * BF)
+        self.weight(n, "p1", p1, 2 * C, C, parts=3)
+        self.span(n, "p2", p2, 3 * nth * 16 * 2 * C * BF)
         self.span(n, "p2_bias", bias, nth * 16 * F32)
         self.span(n, "x_in", x_in, rows.B * 6 * rows.voxels * F32)
         self.span(n, "y", y, rows.B * n_out * rows.voxels * F32, optional=True)

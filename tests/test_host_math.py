@@ -91,6 +91,22 @@ def test_packing_matches_kernel_contraction(N, segments):
     assert not np.any(_kernel_contraction(wp, act)[N:])
 
 
+def test_three_plane_packing_is_exact_and_extends_the_two_plane_split():
+    """parts=3: the three bf16 planes add up to the float32 weight exactly, and the first two planes are the two-way
+    split (what the adjoint kernels read from the same buffer)."""
+    rng = np.random.default_rng(3)
+    w = torch.from_numpy(rng.standard_normal((48, 80)) * 10.0 ** rng.uniform(-4, 2, (48, 80))).float()
+    three = pack_mlp_weight(w, parts=3).view(torch.bfloat16).double()
+    two = pack_mlp_weight(w).view(torch.bfloat16).double()
+    assert three.shape[0] == 3 and two.shape[0] == 2
+    assert torch.equal(three[:2], two)
+    dense = three.sum(0)
+    # undo the padding and the k permutation by contracting against unit vectors
+    eye = np.eye(T.round_up(80, 32))
+    got = np.stack([_kernel_contraction(dense.numpy(), eye[k]) for k in range(80)], 1)[:48]
+    assert np.array_equal(got, w.double().numpy())
+
+
 def test_state_dict_is_the_references():
     """ffno_amd.model.FFNO3D has exactly the keys and shapes of the reference checkpoint (golden case A was loaded
     into the reference with strict=True)."""
