@@ -5,7 +5,7 @@ patches, D=128, H=W=64, width 64, vertical modes 32 (z_min_wavelength 1.28 at z_
 hidden 128, species H + Ca II -- on a synthetic stratified atmosphere (seed 0) with seeded weights (seed 0). Inputs are
 resident in HBM before timing.
 
-Beside it, in the same run: the plain FP32 GPU path (stock torch ops, ffno_amd.reference_torch) on the same inputs
+Beside it, in the same run: the plain FP32 GPU path (stock torch ops, ffnoml_amd.reference_torch) on the same inputs
 and weights (its speed and the HIP path's relative error in log10 n against it), and the CPU oracle in float32 on a
 bounded sample (the CPU workload: B=1, D=64, 32 x 32, width 32, 4 layers) timed on this host's cores.
 
@@ -28,7 +28,7 @@ CPU_WORKLOAD = dict(B=1, D=64, H=32, W=32, width=32, n_layers=4, z_min_wavelengt
 
 
 def model_config(w):
-    from ffno_amd.atoms import species
+    from ffnoml_amd.atoms import species
     return dict(in_channels=6, out_channels=12, width=w["width"], n_layers=w["n_layers"], hidden_spectral=w["hidden"],
                 hidden_vertical=w["hidden"], freq_mix_rank=w["rank"], z_reference_length=40.0,
                 z_min_wavelength=w["z_min_wavelength"], z_feature_wavelength=0.3, species=species(["H", "CA"]))
@@ -36,7 +36,7 @@ def model_config(w):
 
 def make(w, device):
     import numpy as np
-    from ffno_amd import synthetic
+    from ffnoml_amd import synthetic
     x, z, mx, sx = synthetic.atmosphere(w["B"], w["D"], w["H"], w["W"], seed=0)
     model = synthetic.random_model(0, **model_config(w))
     model.set_normalization(mx, sx, np.zeros(12), np.ones(12))
@@ -104,7 +104,7 @@ def main():
         sys.exit(2)
 
     import torch
-    from ffno_amd import _lib
+    from ffnoml_amd import _lib
 
     dev = torch.device("cuda:0")
     copy_cal = copy_calibration(dev) if args.copy_calibration else None
@@ -186,7 +186,7 @@ def main():
     }
 
     if not args.no_baselines:
-        from ffno_amd.reference_torch import PlainFFNO3D
+        from ffnoml_amd.reference_torch import PlainFFNO3D
         plain = PlainFFNO3D(model).to(dev)
 
         def plain_step():
@@ -248,7 +248,7 @@ def volume_leg(model, dev):
     coherently over the 65 M voxels while the forward's rounding (about 1e-3 in log10 n, irregular in x) averages
     out: sum(f(x + e d) - f(x - e d)) / 2e against <x_bar, d>, e = 1e-2."""
     import torch
-    from ffno_amd import synthetic
+    from ffnoml_amd import synthetic
 
     def log(msg):
         print(f"volume: {msg}", file=sys.stderr, flush=True)
@@ -320,7 +320,7 @@ def cpu_leg(model, x0, z0, y0_hip, y0_plain):
     64 x 64, same weights): the relative error in log10 n, max |y - y64| / max(|y64|, 1), of the HIP path and of the
     plain FP32 GPU path."""
     import torch
-    from ffno_amd import synthetic
+    from ffnoml_amd import synthetic
     from oracle import ffno_oracle as O
     import numpy as np
     cores = len(os.sched_getaffinity(0))

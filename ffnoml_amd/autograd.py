@@ -42,7 +42,7 @@ def _spacing(v):
 def ffno_apply(module, x, z_scale, dx, dy):
     """FFNO3D.forward(x, z_scale, dx, dy) through the HIP engine."""
     if not x.is_cuda:
-        raise RuntimeError("ffno_amd.FFNO3D runs on the GPU through libffno_hip.so; it has no CPU path")
+        raise RuntimeError("ffnoml_amd.FFNO3D runs on the GPU through libffno_hip.so; it has no CPU path")
     if z_scale.dim() == 5:
         z_scale = z_scale[:, 0]
     return FFNOFunction.apply(x, z_scale, module, _spacing(dx), _spacing(dy))

@@ -26,7 +26,7 @@ LN10 = math.log(10.0)
 class GPUBackend:
     """One model resident on one GPU (ffno_fsdp_runtime.py:81-221 for the construction it replaces)."""
 
-    def __init__(self, checkpoint_path, factory_module="ffno_amd.factory", factory_name="create_ffno3d",
+    def __init__(self, checkpoint_path, factory_module="ffnoml_amd.factory", factory_name="create_ffno3d",
                  level_names=(), factory_kwargs=None, device="cuda:0", name="model", model=None, normalization=None):
         self.device = torch.device(device)
         self.name = name

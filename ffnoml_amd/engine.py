@@ -355,7 +355,7 @@ def _asm(base, t=None, a=None, b=None, scale=None, act=0):
 
 
 class Engine:
-    """Runs FFNO3D forward and input-VJP on the GPU for one module (ffno_amd.model.FFNO3D)."""
+    """Runs FFNO3D forward and input-VJP on the GPU for one module (ffnoml_amd.model.FFNO3D)."""
 
     def __init__(self, module):
         if module.width not in SUPPORTED_WIDTHS:

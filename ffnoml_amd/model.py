@@ -1,8 +1,8 @@
 """FFNO3D as a torch.nn.Module whose parameters and buffers carry the reference's state_dict keys and shapes
 (models/ffno_model.py:147-832), so that checkpoints load with strict=True in both directions, and whose forward
-runs the HIP engine (ffno_amd.engine) on the GPU.
+runs the HIP engine (ffnoml_amd.engine) on the GPU.
 
-There is no CPU path: forward on a CPU tensor raises. Differentiation goes through ffno_amd.autograd.FFNOFunction,
+There is no CPU path: forward on a CPU tensor raises. Differentiation goes through ffnoml_amd.autograd.FFNOFunction,
 whose backward is the engine's hand-written adjoint.
 """
 import math

@@ -162,7 +162,7 @@ def load_manifest(path):
 
 
 def main(argv=None):
-    """python -m ffno_amd.service manifest.toml: the reference's manifest ([service] port, token, bind_host;
+    """python -m ffnoml_amd.service manifest.toml: the reference's manifest ([service] port, token, bind_host;
     [[models]] name, checkpoint_path, factory_module, factory_name, level_names, factory_kwargs)."""
     ap = argparse.ArgumentParser()
     ap.add_argument("manifest")
@@ -175,7 +175,7 @@ def main(argv=None):
     backends = {}
     for m in sorted(cfg["models"], key=lambda item: item["name"]):
         backends[str(m["name"])] = GPUBackend(
-            m["checkpoint_path"], factory_module=m.get("factory_module", "ffno_amd.factory"),
+            m["checkpoint_path"], factory_module=m.get("factory_module", "ffnoml_amd.factory"),
             factory_name=m.get("factory_name", "create_ffno3d"), level_names=m["level_names"],
             factory_kwargs=m.get("factory_kwargs", {}), name=str(m["name"]))
     batcher = ContinuousBatcher(backends, slots=args.slots, max_batch=args.max_batch)

@@ -90,7 +90,7 @@ def perturb_state(model, seed=0):
 
 
 def random_model(seed=0, **config):
-    """An FFNO3D (ffno_amd.model) with torch's default initialisation under manual_seed(seed), then perturb_state."""
+    """An FFNO3D (ffnoml_amd.model) with torch's default initialisation under manual_seed(seed), then perturb_state."""
     from .model import FFNO3D
     torch.manual_seed(int(seed))
     return perturb_state(FFNO3D(**config).eval(), seed)

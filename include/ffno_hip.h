@@ -2,14 +2,14 @@
  *
  * The reference has no native code: its FFNO3D forward (models/ffno_model.py:800-832) is a chain of stock torch
  * ops. Each entry point below launches one hand-written kernel that replaces a span of that chain; the comment on
- * each names the reference lines it stands for. Host code (ffno_amd/engine.py) owns every device buffer, decides the
+ * each names the reference lines it stands for. Host code (ffnoml_amd/engine.py) owns every device buffer, decides the
  * launch order and passes plain device pointers, sizes and a hipStream_t. Nothing here allocates device memory.
  *
  * Layouts (all float32 unless noted):
  *   activations    channel-last [B, D, H, W, C]  ("voxel rows" of C channels)
  *   model inputs   channel-first [B, 6, D, H, W], outputs channel-first [B, Cout, D, H, W] (the reference's)
  *   packed weights bf16 (uint16_t) [N_pad][K_pad], K permuted within every 32-block so that one 16-byte load is
- *                  one MFMA operand (see ffno_amd/engine.py: pack_mlp_weight)
+ *                  one MFMA operand (see ffnoml_amd/engine.py: pack_mlp_weight)
  *   twiddles       bf16 pairs (hi, lo) [M_pad][K_pad], the split-bf16 form of a float64 matrix
  *   GroupNorm      partial statistics [B][nwg][G][3] = (weight, mean, M2); finalized stats [B][G][3] = (mean, rstd, W);
  *                  per-channel affine [B][C] scale and [B][C] shift

@@ -1,6 +1,6 @@
 """CPU oracle for the FFNO3D forward pass and its input adjoint.
 
-TEST INFRASTRUCTURE ONLY. Nothing in the product (the ffno_amd package, its HIP library, the service) imports this
+TEST INFRASTRUCTURE ONLY. Nothing in the product (the ffnoml_amd package, its HIP library, the service) imports this
 module; only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it, and only as the checker or as
 the timed CPU baseline.
 

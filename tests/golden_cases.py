@@ -18,8 +18,8 @@ def load(name):
 
 
 def regenerate_state(config, seed, mean_x, std_x, mean_y, std_y):
-    """The weights of a case stored without them: ffno_amd.synthetic's generator, as make_golden.py used it."""
-    from ffno_amd import synthetic
+    """The weights of a case stored without them: ffnoml_amd.synthetic's generator, as make_golden.py used it."""
+    from ffnoml_amd import synthetic
     model = synthetic.random_model(seed=seed, **config)
     model.set_normalization(mean_x, std_x, mean_y, std_y)
     return {k: v.detach().clone() for k, v in model.state_dict().items()}

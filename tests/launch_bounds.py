@@ -10,8 +10,8 @@ import types
 import numpy as np
 import torch
 
-from ffno_amd import _lib, engine as E
-from ffno_amd.transforms import round_up
+from ffnoml_amd import _lib, engine as E
+from ffnoml_amd.transforms import round_up
 
 F32, BF = 4, 2
 

@@ -1,4 +1,4 @@
-"""End-to-end forward parity on the GPU: ffno_amd.FFNO3D (HIP engine) against the reference's own float64 outputs
+"""End-to-end forward parity on the GPU: ffnoml_amd.FFNO3D (HIP engine) against the reference's own float64 outputs
 (golden vectors) and against the float64 oracle.
 
 Tolerance: max over every output of |log10 n - log10 n_ref| / max(|log10 n_ref|, 1) <= 1e-3,
@@ -29,7 +29,7 @@ def log10_err(y, ref, mean_y, std_y):
 
 
 def _model_from(config, state):
-    from ffno_amd.model import FFNO3D
+    from ffnoml_amd.model import FFNO3D
     m = FFNO3D(**config)
     m.load_state_dict(state, strict=True)
     return m.eval().to(DEV)
@@ -53,8 +53,8 @@ def test_forward_matches_reference_golden(name):
 def test_forward_matches_oracle_width64():
     """Width 64, two layers, H + Ca II, B=2, D=32, 16 x 16, 32 vertical modes: the HIP engine against the float64
     oracle on the same weights and synthetic atmosphere."""
-    from ffno_amd import synthetic
-    from ffno_amd.atoms import species
+    from ffnoml_amd import synthetic
+    from ffnoml_amd.atoms import species
     from oracle import ffno_oracle as O
     config = dict(in_channels=6, out_channels=12, width=64, n_layers=2, hidden_spectral=128, hidden_vertical=128,
                   freq_mix_rank=8, z_reference_length=40.0, z_min_wavelength=1.28, z_feature_wavelength=0.3,
@@ -74,7 +74,7 @@ def test_forward_matches_oracle_width64():
 def test_forward_cpu_workload_matches_golden():
     """The CPU workload (width 32, 4 layers, 64 x 32 x 32, 12 vertical modes) against the reference's float64
     output subsample."""
-    from ffno_amd import synthetic
+    from ffnoml_amd import synthetic
     data = np.load(os.path.join(GOLDEN, "case_f_cfg0.npz"))
     config = json.loads(str(data["config_json"]))
     x, z, mx, sx = synthetic.atmosphere(1, 64, 32, 32, seed=0)

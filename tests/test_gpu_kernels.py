@@ -38,9 +38,9 @@ def _gelu(x):
 def test_tw_gemm_matches_float64_matmul(m, k, n, batch, accumulate, parts):
     """Split-bf16 GEMM with padding in m and k and a two-level batch map against float64: two planes 3e-5 of the
     output range; three planes (six products) 2e-6, float32 level (the two-plane form misses that by ~10x)."""
-    from ffno_amd import _lib
-    from ffno_amd import transforms as T
-    from ffno_amd._lib import FfnoTwGemm
+    from ffnoml_amd import _lib
+    from ffnoml_amd import transforms as T
+    from ffnoml_amd._lib import FfnoTwGemm
     torch.manual_seed(0)
     a = torch.randn(m, k, dtype=torch.float64)
     mat = T.split_bf16(a.numpy(), DEV, parts=parts)
@@ -66,8 +66,8 @@ def test_tw_gemm_matches_float64_matmul(m, k, n, batch, accumulate, parts):
 
 
 def _small_model(width=64, rank=8, layers=1, species=("H", "CA")):
-    from ffno_amd import synthetic
-    from ffno_amd.atoms import species as sp
+    from ffnoml_amd import synthetic
+    from ffnoml_amd.atoms import species as sp
     out = 6 * len(species)
     m = synthetic.random_model(3, width=width, n_layers=layers, hidden_spectral=32, hidden_vertical=32,
                                freq_mix_rank=rank, z_min_wavelength=1.28, out_channels=out, species=sp(list(species)))
@@ -80,9 +80,9 @@ def test_post_mlp_and_groupnorm_statistics():
     """ffno_post (C -> 2C -> C, three-way split-bf16 MFMA) against float64 (3e-6 of range: float32 level; the
     two-way split would miss it by several times), and its fused weighted GroupNorm
     statistics against a float64 weighted mean/variance (1e-4 relative) and total weight (1e-5)."""
-    from ffno_amd import _lib
-    from ffno_amd._lib import FfnoRows, FfnoStatsOut
-    from ffno_amd.engine import pack_mlp_weight
+    from ffnoml_amd import _lib
+    from ffnoml_amd._lib import FfnoRows, FfnoStatsOut
+    from ffnoml_amd.engine import pack_mlp_weight
     torch.manual_seed(1)
     for C in (16, 32, 64, 96):
         B, vox = 2, 1000
@@ -121,7 +121,7 @@ def test_spectral_pipelines_match_fft():
     """The horizontal rfft2 -> per-mode mix -> irfft2 and the vertical Filon -> mix -> synthesis, each as the
     engine runs them, against float64 torch.fft / einsum statements with the plan's own gates and low-rank tables:
     1e-4 of range (split-bf16 transforms and mix)."""
-    from ffno_amd.engine import Engine
+    from ffnoml_amd.engine import Engine
     model = _small_model()
     eng = Engine(model)
     eng.sync(torch.device(DEV))
@@ -144,7 +144,7 @@ def test_spectral_pipelines_match_fft():
     wc = torch.complex(wr, wi)
     a = torch.einsum("bidyx,io->bodyx", xf, wc.to(xf.dtype))
     y = a * torch.complex(gate[..., 0], gate[..., 1])
-    from ffno_amd.engine import _freq_lowrank
+    from ffnoml_amd.engine import _freq_lowrank
     feats = None
     ky = torch.fft.fftfreq(H, d=48e3)
     kx = torch.fft.rfftfreq(W, d=48e3)
@@ -210,7 +210,7 @@ def test_spectral_adjoints_are_transposes(chunk):
     """<T x, y> = <x, T^T y> for the horizontal and vertical spectral maps as the engine runs them (forward and
     adjoint launch sequences, the per-mode gates and low-rank terms included), with the horizontal planes in one
     chunk and in chunks of 3: 1e-5 relative."""
-    from ffno_amd.engine import Engine
+    from ffnoml_amd.engine import Engine
     model = _small_model()
     eng = Engine(model)
     dev = torch.device(DEV)

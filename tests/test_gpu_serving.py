@@ -11,9 +11,9 @@ pytestmark = pytest.mark.gpu
 
 
 def _backend(seed=7):
-    from ffno_amd import synthetic
-    from ffno_amd.atoms import species
-    from ffno_amd.backend import GPUBackend
+    from ffnoml_amd import synthetic
+    from ffnoml_amd.atoms import species
+    from ffnoml_amd.backend import GPUBackend
     config = dict(in_channels=6, out_channels=6, width=32, n_layers=2, hidden_spectral=32, hidden_vertical=32,
                   freq_mix_rank=4, z_reference_length=40.0, z_min_wavelength=1.28, z_feature_wavelength=0.3,
                   species=species(["H"]))
@@ -25,7 +25,7 @@ def _backend(seed=7):
 
 
 def _request(rng, nz, nx, ny, z0=0.0):
-    from ffno_amd import synthetic
+    from ffnoml_amd import synthetic
     feats, zc = synthetic.atmosphere_features(1, nz, nx, ny, seed=int(rng.integers(1 << 30)))
     si = feats[0].copy()
     si[0] = 10.0 ** si[0]
@@ -39,7 +39,7 @@ def _rel(a, b):
 
 
 def test_results_alone_and_in_the_mix():
-    from ffno_amd.serving import ContinuousBatcher
+    from ffnoml_amd.serving import ContinuousBatcher
     backend = _backend()
     rng = np.random.default_rng(0)
     shapes = [(16, 8, 8), (16, 8, 8), (16, 8, 8), (24, 16, 12), (24, 16, 12), (12, 32, 32)]
@@ -81,7 +81,7 @@ def test_results_alone_and_in_the_mix():
 
 def test_graph_replays_match_eager():
     """Replays of one captured forward, each after unrelated eager GPU work, against the eager forward."""
-    from ffno_amd.serving import ContinuousBatcher
+    from ffnoml_amd.serving import ContinuousBatcher
     backend = _backend(seed=9)
     rng = np.random.default_rng(5)
     f, z = _request(rng, 20, 16, 16)

@@ -22,7 +22,7 @@ def _errs(g, ref):
 
 
 def _model_from(config, state):
-    from ffno_amd.model import FFNO3D
+    from ffnoml_amd.model import FFNO3D
     m = FFNO3D(**config)
     m.load_state_dict(state, strict=True)
     return m.eval().to(DEV)
@@ -49,8 +49,8 @@ def test_inversion_loss_gradient_matches_oracle_and_finite_differences():
     0.02 sigma (normalised units). Width 64, 2 layers, B=1, D=24, 12 x 12, 32 vertical modes. The HIP gradient is
     compared with the float64 oracle's autograd gradient over the whole input, and with float64 central finite
     differences of the oracle (step 1e-4) at 64 random input entries."""
-    from ffno_amd import synthetic
-    from ffno_amd.atoms import species
+    from ffnoml_amd import synthetic
+    from ffnoml_amd.atoms import species
     from oracle import ffno_oracle as O
     config = dict(in_channels=6, out_channels=12, width=64, n_layers=2, hidden_spectral=64, hidden_vertical=64,
                   freq_mix_rank=8, z_reference_length=40.0, z_min_wavelength=1.28, z_feature_wavelength=0.3,
@@ -125,9 +125,9 @@ def test_backend_vjp_local_matches_si_autograd():
     """GPUBackend.vjp_local (SI features and heights in metres, linear-population cotangent) against float64
     autograd through the oracle's SI chain (ffno_fsdp_runtime.py:48-58, :76-78, :265-281) with per-column heights;
     predict_local against the oracle's populations (1e-3 relative in log10 n)."""
-    from ffno_amd import synthetic
-    from ffno_amd.backend import GPUBackend
-    from ffno_amd.atoms import species
+    from ffnoml_amd import synthetic
+    from ffnoml_amd.backend import GPUBackend
+    from ffnoml_amd.atoms import species
     from oracle import ffno_oracle as O
     config = dict(in_channels=6, out_channels=6, width=32, n_layers=2, hidden_spectral=32, hidden_vertical=32,
                   freq_mix_rank=4, z_reference_length=40.0, z_min_wavelength=1.28, z_feature_wavelength=0.3,
@@ -177,7 +177,7 @@ def test_checkpoint_strides_agree():
     short) give the same x_bar and z_bar bit for bit: the recomputed inputs are the same kernels on the same data."""
     data, config, state = load("case_b_jitter64")
     config = dict(config, n_layers=4) if config.get("n_layers", 0) < 4 else config
-    from ffno_amd import synthetic
+    from ffnoml_amd import synthetic
     model = synthetic.random_model(3, **config).to(DEV).eval()
     x, z, mx, sx = synthetic.atmosphere(2, 16, 12, 10, seed=4)
     x, z = x.to(DEV), z.to(DEV)
@@ -206,7 +206,7 @@ def test_two_pass_fusion_adjoint_matches_fused():
     shift subtracts its total) and whose own error against the float64 oracle is 3e-5 to 7e-5, to 1e-4."""
     data, config, state = load("case_b_jitter64")
     config = dict(config, n_layers=3, width=64, hidden_spectral=64, hidden_vertical=64)
-    from ffno_amd import synthetic
+    from ffnoml_amd import synthetic
     model = synthetic.random_model(4, **config).to(DEV).eval()
     x, z, mx, sx = synthetic.atmosphere(2, 16, 12, 10, seed=6)
     x, z = x.to(DEV), z.to(DEV)
@@ -231,7 +231,7 @@ def test_adjoint_keeping_every_intermediate_matches_recompute():
     same kernels on the same data, only where the data waited differs)."""
     data, config, state = load("case_b_jitter64")
     config = dict(config, n_layers=3, width=32, hidden_spectral=32, hidden_vertical=32)
-    from ffno_amd import synthetic
+    from ffnoml_amd import synthetic
     model = synthetic.random_model(8, **config).to(DEV).eval()
     x, z, mx, sx = synthetic.atmosphere(2, 12, 10, 14, seed=9)
     x, z = x.to(DEV), z.to(DEV)

@@ -10,8 +10,8 @@ import numpy as np
 import pytest
 import torch
 
-from ffno_amd import transforms as T
-from ffno_amd.engine import pack_mlp_weight
+from ffnoml_amd import transforms as T
+from ffnoml_amd.engine import pack_mlp_weight
 
 
 @pytest.mark.parametrize("H,W", [(8, 12), (11, 9), (6, 5), (64, 64)])
@@ -108,10 +108,10 @@ def test_three_plane_packing_is_exact_and_extends_the_two_plane_split():
 
 
 def test_state_dict_is_the_references():
-    """ffno_amd.model.FFNO3D has exactly the keys and shapes of the reference checkpoint (golden case A was loaded
+    """ffnoml_amd.model.FFNO3D has exactly the keys and shapes of the reference checkpoint (golden case A was loaded
     into the reference with strict=True)."""
     from golden_cases import load
-    from ffno_amd.model import FFNO3D
+    from ffnoml_amd.model import FFNO3D
     data, config, state = load("case_a_shared64")
     model = FFNO3D(**config)
     ours = model.state_dict()
@@ -123,7 +123,7 @@ def test_state_dict_is_the_references():
 
 def test_c_abi_exports_every_declared_symbol():
     """libffno_hip.so loads (no GPU needed) and exports every function include/ffno_hip.h declares."""
-    from ffno_amd import _lib
+    from ffnoml_amd import _lib
     header = open(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include",
                                "ffno_hip.h")).read()
     names = set(re.findall(r"^\s*(?:const char\*|int)\s+(ffno_\w+)\s*\(", header, re.M))
@@ -136,8 +136,8 @@ def test_c_abi_exports_every_declared_symbol():
 
 def test_cpu_tensors_are_refused():
     """No CPU path: the module raises on CPU tensors instead of computing something else."""
-    from ffno_amd import synthetic
-    from ffno_amd.atoms import species
+    from ffnoml_amd import synthetic
+    from ffnoml_amd.atoms import species
     model = synthetic.random_model(0, width=16, n_layers=1, hidden_spectral=8, hidden_vertical=8, freq_mix_rank=2,
                                    out_channels=6, species=species(["H"]))
     with pytest.raises(RuntimeError):
@@ -145,12 +145,12 @@ def test_cpu_tensors_are_refused():
 
 
 def test_plain_torch_baseline_matches_oracle():
-    """The plain FP32 comparison path (ffno_amd.reference_torch) equals the float64 oracle on golden case A to
+    """The plain FP32 comparison path (ffnoml_amd.reference_torch) equals the float64 oracle on golden case A to
     float32 precision (1e-4 of range), so the bench's baseline computes the same network."""
     from golden_cases import load, oracle_config
     from oracle import ffno_oracle as O
-    from ffno_amd.model import FFNO3D
-    from ffno_amd.reference_torch import PlainFFNO3D
+    from ffnoml_amd.model import FFNO3D
+    from ffnoml_amd.reference_torch import PlainFFNO3D
     data, config, state = load("case_a_shared64")
     model = FFNO3D(**config)
     model.load_state_dict(state, strict=True)

@@ -8,8 +8,8 @@ from launch_bounds import check_shape
 
 
 def _model(width=64, layers=8, rank=8):
-    from ffno_amd import synthetic
-    from ffno_amd.atoms import species
+    from ffnoml_amd import synthetic
+    from ffnoml_amd.atoms import species
     return synthetic.random_model(0, in_channels=6, out_channels=12, width=width, n_layers=layers,
                                   hidden_spectral=128, hidden_vertical=128, freq_mix_rank=rank,
                                   z_reference_length=40.0, z_min_wavelength=1.28, z_feature_wavelength=0.3,
@@ -43,7 +43,7 @@ def test_checker_catches_a_short_buffer_and_covers_every_entry_point():
     import re
     from pathlib import Path
     import torch
-    from ffno_amd import engine as E
+    from ffnoml_amd import engine as E
     import launch_bounds as LB
     real = E.Engine._scratch
 

@@ -68,7 +68,7 @@ def test_golden_submodules():
 def test_golden_cfg0_workload():
     """The CPU workload (width 32, 4 layers, 64x32x32, 12 vertical modes): weights and inputs regenerate
     bit-for-bit (sha256 of the state), and the oracle matches the stored subsample and the full-tensor sums."""
-    from ffno_amd import synthetic
+    from ffnoml_amd import synthetic
     data = np.load(os.path.join(GOLDEN, "case_f_cfg0.npz"))
     config = json.loads(str(data["config_json"]))
     x, z, mean_x, std_x = synthetic.atmosphere(1, 64, 32, 32, seed=0)
@@ -103,7 +103,7 @@ def test_planted_mistake_is_caught():
 
 
 def test_atom_tables_are_the_references():
-    from ffno_amd.atoms import ATOMS
+    from ffnoml_amd.atoms import ATOMS
     with open(os.path.join(GOLDEN, "atoms.json")) as f:
         ref = json.load(f)
     for a, entry in ref.items():
@@ -191,7 +191,7 @@ def _calc_lte_fractions(t, ne_cm3, atom):
 def test_lte_weights_match_calc_lte_populations():
     """delta = 0 gives MULTI3D's LTE fractions (tests/test_departure_coefficients.py:33-54, 94-109): 2e-12 dex where
     the fraction exceeds 1e-200."""
-    from ffno_amd.atoms import ATOMS
+    from ffnoml_amd.atoms import ATOMS
     rng = np.random.default_rng(4)
     log10_t = np.log10(2000.0) + (np.log10(6e4) - np.log10(2000.0)) * rng.random(500)
     log10_ne = 14.0 + 9.0 * rng.random(500)
@@ -210,7 +210,7 @@ def test_lte_weights_match_calc_lte_populations():
 
 def test_head_reproduces_any_target():
     """Any levels whose total lies within the bound are reproduced (tests/test_level_fraction_head.py:53-73), 1e-9."""
-    from ffno_amd.atoms import species
+    from ffnoml_amd.atoms import species
     cfg = O.model_config(width=4, species=species(["H", "CA"]))
     torch.manual_seed(11)
     x = torch.randn(1, 6, 5, 4, 3, dtype=torch.float64)

@@ -8,7 +8,7 @@ from concurrent.futures import Future
 import numpy as np
 import pytest
 
-from ffno_amd.service import Service, read_array, write_array
+from ffnoml_amd.service import Service, read_array, write_array
 
 
 class StandInBackend:
