@@ -95,13 +95,26 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-baselines", action="store_true", help="skip the plain FP32 GPU and CPU legs")
     ap.add_argument("--volume", action="store_true", help="also run the whole-volume 504 x 504 x 256 forward + VJP")
+    ap.add_argument("--service-requests", type=int, default=160,
+                    help="requests of the serving workload (Poisson arrivals at 40/s, seed 1; 0 skips it; the full "
+                         "workload is 2000)")
+    ap.add_argument("--plain-service-requests", type=int, default=16,
+                    help="the first requests of the same trace served one at a time by the plain FP32 GPU path")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB device to device three times (the known-size reference for HBM counters)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 --pmc child runs (FETCH_SIZE, WRITE_SIZE) that measure HBM traffic")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported (requests shard whole across GPUs "
               f"with no exchange; run one process per GPU)", file=sys.stderr)
         sys.exit(2)
+    if args.pmc_child:
+        pmc_child()
+        return
+    # the counter runs go first, as child processes, before this process touches the GPU
+    counters = None if args.no_traffic else measure_counters()
 
     import torch
     from ffnoml_amd import _lib
@@ -181,7 +194,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": name, "launches": klaunches, "avg_ms": round(kms, 4),
                      "share_of_step": round(ktot / timed_total, 3), "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None},
+                     "algorithmic_bytes_per_launch": int(kbytes / klaunches),
+                     "traffic": traffic_of(counters, name)},
         "entry_points_ms_per_step": {k: round(v[0], 2) for k, v in sorted(per.items(), key=lambda kv: -kv[1][0])},
     }
 
@@ -212,10 +226,240 @@ def main():
                                                                y[:1].detach().double().cpu(), y0_plain)
     if copy_cal:
         out["copy_calibration"] = copy_cal
-    if args.volume:
+    if args.service_requests > 0:
         del y, xb
+        out["service"] = service_leg(model, dev, args.service_requests,
+                                     0 if args.no_baselines else args.plain_service_requests)
+    if args.volume:
         out["whole_volume"] = volume_leg(model, dev)
     print(json.dumps(out), flush=True)
+
+
+def pmc_child():
+    """Run under rocprofv3 --pmc by measure_counters(): one forward + VJP step of the workload, then three 1 GiB
+    device copies (the calibration kernel of known size)."""
+    import torch
+    dev = torch.device("cuda:0")
+    model, x, z, ct = make(WORKLOAD, dev)
+    xg = x.detach().requires_grad_(True)
+    y = model(xg, z, 48e3, 48e3)
+    torch.autograd.grad(y, xg, grad_outputs=ct)
+    torch.cuda.synchronize()
+    copy_calibration(dev)
+    torch.cuda.synchronize()
+
+
+def measure_counters(timeout=240):
+    """Two child runs of this script (--pmc-child) under rocprofv3, one per counter: FETCH_SIZE, then WRITE_SIZE
+    (KiB per dispatch). Returns {counter: {kernel: [values]}} or None when rocprofv3 is missing or a run fails."""
+    import csv
+    import re
+    import shutil
+    import subprocess
+    import tempfile
+    exe = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(exe):
+        return None
+    out = {}
+    with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            d = os.path.join(tmp, counter)
+            cmd = ["timeout", "-s", "KILL", str(timeout), exe, "--pmc", counter, "--output-format", "csv", "-d", d,
+                   "-o", "run", "--", sys.executable, "-u", os.path.abspath(__file__), "--pmc-child"]
+            print(f"bench: rocprofv3 --pmc {counter}", file=sys.stderr, flush=True)
+            r = subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"), stdout=subprocess.DEVNULL,
+                               stderr=subprocess.DEVNULL)
+            path = os.path.join(d, "run_counter_collection.csv")
+            if r.returncode != 0 or not os.path.exists(path):
+                print(f"bench: counter run {counter} failed ({r.returncode}); traffic not measured", file=sys.stderr)
+                return None
+            per = {}
+            for row in csv.DictReader(open(path)):
+                if row["Counter_Name"] != counter:
+                    continue
+                k = re.sub(r"[<(].*", "", row["Kernel_Name"].replace("void ", "").replace(
+                    "ffno::(anonymous namespace)::", ""))
+                per.setdefault(k, []).append(float(row["Counter_Value"]))
+            out[counter] = per
+    return out
+
+
+def traffic_of(counters, entry):
+    """HBM bytes per launch of an entry point's kernel(s) (ffno_X -> k_X; ffno_tw_gemm -> k_tw_small / k_tw_gemm):
+    the average FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by bytes moved / counter value of the 1 GiB
+    device copy (__amd_rocclr_copyBuffer) in the same run."""
+    if not counters:
+        return None
+    gib_kib = float(1 << 20)
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        per = counters[counter]
+        cal = [v for v in per.get("__amd_rocclr_copyBuffer", []) if v > 0.25 * gib_kib]
+        names = [k for k in per if k == "k_" + entry[5:] or (entry == "ffno_tw_gemm" and k.startswith("k_tw"))]
+        vals = [v for k in names for v in per[k]]
+        if not cal or not vals:
+            return None
+        scale = gib_kib / (sum(cal) / len(cal))
+        res[counter] = (sum(vals) / len(vals), scale, len(vals))
+    (f, fs, n), (w, ws, _) = res["FETCH_SIZE"], res["WRITE_SIZE"]
+    return {"bytes_per_launch": int((f * fs + w * ws) * 1024), "fetch_kib": round(f, 1), "write_kib": round(w, 1),
+            "fetch_scale": round(fs, 4), "write_scale": round(ws, 4), "dispatches": n,
+            "method": "rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, one fwd+VJP step; scaled by the 1 GiB copy"}
+
+
+SERVICE = dict(rate=40.0, seed=1, hw=(32, 64, 128, 256), depths=(96, 128, 192), vjp_share=0.3)
+
+
+def service_trace(n, seed=SERVICE["seed"]):
+    """The serving workload's requests: Poisson arrivals at 40/s; H = W drawn uniformly from 32, 64, 128, 256 and D
+    from 96, 128, 192; 30 % forward + VJP, the rest forward only."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / SERVICE["rate"], n))
+    hw = rng.choice(SERVICE["hw"], n)
+    d = rng.choice(SERVICE["depths"], n)
+    vjp = rng.random(n) < SERVICE["vjp_share"]
+    return [dict(t=float(t[i]), D=int(d[i]), H=int(hw[i]), W=int(hw[i]), op="VJP" if vjp[i] else "PREDICT", seed=i)
+            for i in range(n)]
+
+
+def service_inputs(trace, variants=4):
+    """SI arrays per request (features (6, nz, nx, ny), z (nz, nx, ny) in m, cotangent (nz, nx, ny, 12)): one
+    stratified synthetic atmosphere per tile shape (the generator of the benchmark's inputs; the periodic random
+    fields of a 256 x 256 tile are its 128 x 128 field repeated 2 x 2), in `variants` versions per shape, each rolled
+    horizontally by its own random offset with its temperature scaled by 10^(0.01 N(0,1)); request i gets version
+    i % variants. VJP cotangents are one random field per shape. Arrays are shared between requests (read only), so
+    host memory stays bounded for the 2000-request workload."""
+    import numpy as np
+    from ffnoml_amd import synthetic
+    base, pool, cots = {}, {}, {}
+    out = []
+    for r in trace:
+        key = (r["D"], r["H"], r["W"])
+        if key not in base:
+            rep = max(1, r["H"] // 128)
+            feats, zc = synthetic.atmosphere_features(1, r["D"], r["H"] // rep, r["W"] // rep, seed=sum(key))
+            si = np.tile(feats[0], (1, 1, rep, rep))
+            si[0] = 10.0 ** si[0]
+            si[1:4] *= 1e3
+            z = np.broadcast_to(zc[:, None, None] * 1e6, si.shape[1:]).astype(np.float32).copy()
+            base[key] = (si.astype(np.float32), z)
+        si, z = base[key]
+        v = r["seed"] % variants
+        if key + (v,) not in pool:
+            rng = np.random.default_rng(1000 * sum(key) + v)
+            f = np.roll(si, (int(rng.integers(r["H"])), int(rng.integers(r["W"]))), axis=(2, 3))
+            f[0] *= np.float32(10.0 ** (0.01 * rng.standard_normal()))
+            pool[key + (v,)] = f
+        cot = None
+        if r["op"] == "VJP":
+            if key not in cots:
+                cots[key] = np.random.default_rng(sum(key)).standard_normal(key + (12,), dtype=np.float32) * 1e-15
+            cot = cots[key]
+        out.append((pool[key + (v,)], z, cot))
+    return out
+
+
+def service_leg(model, dev, n, n_plain):
+    """The serving workload through ffnoml_amd.serving.ContinuousBatcher on the benchmark's model (width 64,
+    8 layers): requests submitted at their arrival times from a client thread, batched per shape bucket, admitted by
+    HBM footprint, run on concurrent worker streams; latency = completion - arrival. Beside it, the first n_plain
+    requests of the same trace served one at a time, in arrival order, by the plain FP32 GPU path (stock torch ops,
+    SI chain and autograd VJP as the reference runtime does them)."""
+    import threading
+    import numpy as np
+    import torch
+    from ffnoml_amd.backend import GPUBackend
+    from ffnoml_amd.serving import ContinuousBatcher
+
+    def log(msg):
+        print(f"service: {msg}", file=sys.stderr, flush=True)
+
+    trace = service_trace(n)
+    t0 = time.perf_counter()
+    inputs = service_inputs(trace)
+    log(f"{n} requests generated in {time.perf_counter() - t0:.1f} s")
+    backend = GPUBackend(None, model=model, level_names=[f"L{i}" for i in range(12)], device=str(dev))
+    batcher = ContinuousBatcher({"m": backend}, slots=3, max_batch=8)
+    futures = [None] * n
+    try:
+        def client():
+            start = time.perf_counter()
+            for i, (r, (f, z, cot)) in enumerate(zip(trace, inputs)):
+                delay = start + r["t"] - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                futures[i] = batcher.submit(r["op"], "m", f, z, 48e3, 48e3, cot)
+
+        th = threading.Thread(target=client)
+        w0 = time.perf_counter()
+        th.start()
+        th.join()
+        for i, fu in enumerate(futures):
+            fu.result()
+            if (i + 1) % 20 == 0:
+                log(f"{i + 1} of {n} answered, {time.perf_counter() - w0:.1f} s")
+        wall = time.perf_counter() - w0
+        done = sorted(batcher.completed, key=lambda q: q.t_submit)
+        lat = np.array([(q.t_done - q.t_submit) * 1e3 for q in done])
+        vox = sum(q.voxels for q in done)
+        res = {"workload": f"{n} requests, Poisson 40/s (seed 1), H=W in 32..256, D in 96..192, 30% forward+VJP",
+               "requests": n, "p50_ms": round(float(np.percentile(lat, 50)), 1),
+               "p99_ms": round(float(np.percentile(lat, 99)), 1), "mean_ms": round(float(lat.mean()), 1),
+               "throughput_mvox_s": round(vox / wall / 1e6, 3), "wall_s": round(wall, 2),
+               "offered_mvox_s": round(vox / trace[-1]["t"] / 1e6, 3), "peak_batches_in_flight": batcher.peak_inflight,
+               "graphs": len(batcher.graphs), "slots": batcher.slots, "max_batch": batcher.max_batch}
+        log(f"p50 {res['p50_ms']} ms, p99 {res['p99_ms']} ms, {res['throughput_mvox_s']} Mvox/s")
+    finally:
+        batcher.close()
+    if n_plain:
+        res["plain_fp32_one_at_a_time"] = plain_service(model, dev, trace[:n_plain], inputs[:n_plain], backend)
+        log(f"plain: {res['plain_fp32_one_at_a_time']}")
+        sub = [(q.t_done - q.t_submit) * 1e3 for q in done[:n_plain]]
+        res["first_requests_p50_ms_in_the_full_run"] = round(float(np.percentile(sub, 50)), 1)
+    return res
+
+
+def plain_service(model, dev, trace, inputs, backend, budget_s=90.0):
+    """One request at a time, in arrival order, through the plain FP32 GPU path: predict = SI chain + forward under
+    no_grad; VJP = forward + torch.autograd.grad of <cotangent, populations> (ffno_fsdp_runtime.py:294-348). Stops
+    after budget_s seconds (a bounded sample: the requests answered by then)."""
+    import numpy as np
+    import torch
+    from ffnoml_amd.reference_torch import PlainFFNO3D
+    plain = PlainFFNO3D(model).to(dev)
+    lat, vox = [], 0
+    start = time.perf_counter()
+    for r, (f, z, cot) in zip(trace, inputs):
+        now = time.perf_counter() - start
+        if now < r["t"]:
+            time.sleep(r["t"] - now)
+        x_si = torch.from_numpy(f).to(dev)[None]
+        z_si = torch.from_numpy(z).to(dev)[None]
+        if r["op"] == "PREDICT":
+            with torch.no_grad():
+                xm, zm = backend.model_inputs(x_si, z_si)
+                n_pop = backend.linear(backend.log_populations(plain(xm, zm, 48e3, 48e3)))
+            n_pop.permute(0, 2, 3, 4, 1).cpu()
+        else:
+            x_si.requires_grad_(True)
+            z_si.requires_grad_(True)
+            xm, zm = backend.model_inputs(x_si, z_si)
+            n_pop = backend.linear(backend.log_populations(plain(xm, zm, 48e3, 48e3)))
+            c = torch.from_numpy(cot).to(dev).permute(3, 0, 1, 2)[None]
+            gx, gz = torch.autograd.grad(n_pop, (x_si, z_si), grad_outputs=c)
+            gx.cpu(), gz.cpu()
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - start - r["t"]) * 1e3)
+        vox += r["D"] * r["H"] * r["W"]
+        print(f"service: plain request {len(lat)} ({r['op']} {r['D']}x{r['H']}x{r['W']}) answered, "
+              f"{time.perf_counter() - start:.1f} s", file=sys.stderr, flush=True)
+        if time.perf_counter() - start > budget_s:
+            break
+    wall = time.perf_counter() - start
+    return {"requests": len(lat), "p50_ms": round(float(np.percentile(lat, 50)), 1),
+            "p99_ms": round(float(np.percentile(lat, 99)), 1), "throughput_mvox_s": round(vox / wall / 1e6, 3),
+            "wall_s": round(wall, 2)}
 
 
 def copy_calibration(dev, nbytes=1 << 30):

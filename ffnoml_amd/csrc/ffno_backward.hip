@@ -135,17 +135,30 @@ __global__ __launch_bounds__(NW * 64) void k_gn_grad_stats(FfnoRows rows, FfnoGn
   }
 }
 
-__global__ void k_gn_grad_finalize(int nwg, const float* __restrict__ partials, float* __restrict__ sums) {
-  const int b = blockIdx.x, g = threadIdx.x;
-  if (g >= G) return;
+// 256 threads: thread t sums the partials i = t/8, t/8 + 32, ... of group t%8, then one thread per group adds the 32
+// slices in order (a fixed order: the result does not depend on scheduling)
+__global__ __launch_bounds__(256) void k_gn_grad_finalize(int nwg, const float* __restrict__ partials,
+                                                           float* __restrict__ sums) {
+  __shared__ float sl[32][G][2];
+  const int b = blockIdx.x, g = threadIdx.x % G, slice = threadIdx.x / G;
   float a = 0.f, c = 0.f;
-  for (int i = 0; i < nwg; ++i) {
+  for (int i = slice; i < nwg; i += 32) {
     const float* p = partials + (((int64_t)b * nwg + i) * G + g) * 2;
     a += p[0];
     c += p[1];
   }
-  sums[(b * G + g) * 2 + 0] = a;
-  sums[(b * G + g) * 2 + 1] = c;
+  sl[slice][g][0] = a;
+  sl[slice][g][1] = c;
+  __syncthreads();
+  if (threadIdx.x < G) {
+    float ta = 0.f, tc = 0.f;
+    for (int k = 0; k < 32; ++k) {
+      ta += sl[k][threadIdx.x][0];
+      tc += sl[k][threadIdx.x][1];
+    }
+    sums[(b * G + threadIdx.x) * 2 + 0] = ta;
+    sums[(b * G + threadIdx.x) * 2 + 1] = tc;
+  }
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -699,7 +712,7 @@ int ffno_gn_grad_stats(FfnoRows rows, FfnoGnBack gn, const float* ubar, float* p
 
 int ffno_gn_grad_finalize(int B, int nwg, const float* partials, float* sums, hipStream_t stream) {
   if (B <= 0 || nwg <= 0 || !partials || !sums) return fail("ffno_gn_grad_finalize: bad arguments");
-  k_gn_grad_finalize<<<B, 64, 0, stream>>>(nwg, partials, sums);
+  k_gn_grad_finalize<<<B, 256, 0, stream>>>(nwg, partials, sums);
   return check_launch("ffno_gn_grad_finalize");
 }
 
@@ -712,9 +725,9 @@ int ffno_mlp_bwd(FfnoRows rows, const float* ubar, FfnoGnBack gn, FfnoAssemble x
   FFNO_DISPATCH_CT(rows.C, {
     constexpr int C = CT * 16, KP = ks_of(CT) * 32;
     const size_t lds = ((size_t)2 * C * KP * 2 + (size_t)C * 2 * C) * 2 * 2;
-    auto k = k_mlp_bwd<CT, nw_of<CT>()>;
+    auto k = k_mlp_bwd<CT, nw_bwd<CT>()>;
     if (set_lds(k, lds)) return -1;
-    k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, ubar, gn, x, w1, w2t, w1t, residual, xbar);
+    k<<<grid, nw_bwd<CT>() * 64, lds, stream>>>(rows, ubar, gn, x, w1, w2t, w1t, residual, xbar);
   });
   return check_launch("ffno_mlp_bwd");
 }
@@ -785,12 +798,12 @@ int ffno_gates_bwd(FfnoRows rows, const float* h, const float* xsbar, const floa
   FFNO_DISPATCH_CT(rows.C, {
     const size_t lds = 8 * (size_t)(CT * 16) * ks_of(CT) * 32 * 2 * 2;
     if (lds <= 160 * 1024) {
-      auto k = k_gates_bwd<CT, nw_of<CT>(), true>;
+      auto k = k_gates_bwd<CT, nw_bwd<CT>(), true>;
       if (set_lds(k, lds)) return -1;
-      k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, h, xsbar, xvbar, hbar_in, g1, g2, g2t, g1t, v1, v2, v2t, v1t, z_proj,
+      k<<<grid, nw_bwd<CT>() * 64, lds, stream>>>(rows, h, xsbar, xvbar, hbar_in, g1, g2, g2t, g1t, v1, v2, v2t, v1t, z_proj,
                                          hbar_out, zbar);
     } else {
-      k_gates_bwd<CT, nw_of<CT>(), false><<<grid, nw_of<CT>() * 64, 0, stream>>>(rows, h, xsbar, xvbar, hbar_in, g1, g2, g2t, g1t, v1,
+      k_gates_bwd<CT, nw_bwd<CT>(), false><<<grid, nw_bwd<CT>() * 64, 0, stream>>>(rows, h, xsbar, xvbar, hbar_in, g1, g2, g2t, g1t, v1,
                                                                  v2, v2t, v1t, z_proj, hbar_out, zbar);
     }
   });

@@ -510,6 +510,12 @@ constexpr size_t red_bytes() { return (size_t)NWV * 4 * CT * GnAcc<CT>::SUB * 3 
 // a wave's registers fit 256; wider channel counts keep 4
 template <int CT>
 constexpr int nw_of() { return CT <= 4 ? 8 : 4; }
+// the same for kernels whose waves fit 128 VGPRs (the forward's three-way row kernels: 4 waves per SIMD) or 170
+// (the MLP and input-gate adjoints: 3 per SIMD); their staged weights allow one workgroup per CU
+template <int CT>
+constexpr int nw_fwd() { return CT <= 4 ? 16 : 4; }
+template <int CT>
+constexpr int nw_bwd() { return CT <= 4 ? 12 : 4; }
 
 template <int CT, int NW>
 constexpr size_t red_bytes_nw() { return (size_t)NW * 4 * CT * GnAcc<CT>::SUB * 3 * sizeof(float); }

@@ -68,18 +68,30 @@ __global__ __launch_bounds__(NW * 64) void k_lift(FfnoRows rows, const float* __
 // ---------------------------------------------------------------------------------------------------------------
 // GroupNorm finalize
 // ---------------------------------------------------------------------------------------------------------------
-__global__ void k_gn_finalize(int nwg, int C, const float* __restrict__ partials, const float* __restrict__ gamma,
+__global__ __launch_bounds__(256) void k_gn_finalize(int nwg, int C, const float* __restrict__ partials, const float* __restrict__ gamma,
                               const float* __restrict__ beta, float* __restrict__ stats, float* __restrict__ aff_a,
                               float* __restrict__ aff_b) {
+  // 256 threads: thread t merges the partials i = t/8, t/8 + 32, ... of group t%8, then one thread per group merges
+  // the 32 slices in order (fixed order: deterministic)
   __shared__ float mean_s[G], rstd_s[G];
+  __shared__ float sl[32][G][3];
   const int b = blockIdx.x;
-  if (threadIdx.x < G) {
-    const int g = threadIdx.x;
+  {
+    const int g = threadIdx.x % G, slice = threadIdx.x / G;
     float W = 0.f, m = 0.f, M2 = 0.f;
-    for (int i = 0; i < nwg; ++i) {
+    for (int i = slice; i < nwg; i += 32) {
       const float* p = partials + (((int64_t)b * nwg + i) * G + g) * 3;
       chan(W, m, M2, p[0], p[1], p[2]);
     }
+    sl[slice][g][0] = W;
+    sl[slice][g][1] = m;
+    sl[slice][g][2] = M2;
+  }
+  __syncthreads();
+  if (threadIdx.x < G) {
+    const int g = threadIdx.x;
+    float W = 0.f, m = 0.f, M2 = 0.f;
+    for (int k = 0; k < 32; ++k) chan(W, m, M2, sl[k][g][0], sl[k][g][1], sl[k][g][2]);
     float var = W > 0.f ? fmaxf(M2 / W, 0.f) : 0.f;
     float r = rsqrtf(var + 1e-5f);
     mean_s[g] = m;
@@ -670,9 +682,9 @@ int ffno_block_in(FfnoRows rows, FfnoAssemble in, float* x_out, const uint16_t* 
     const int nl = lds_planes(elems, 0);
     FFNO_DISPATCH_NL(nl, {
       const size_t lds = NL * elems * 2;
-      auto k = k_block_in<CT, nw_of<CT>(), NL>;
+      auto k = k_block_in<CT, nw_fwd<CT>(), NL>;
       if (set_lds(k, lds)) return -1;
-      k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, in, x_out, g1, g2, v1, v2, z_proj, z, xs, xv);
+      k<<<grid, nw_fwd<CT>() * 64, lds, stream>>>(rows, in, x_out, g1, g2, v1, v2, z_proj, z, xs, xv);
     });
   });
   return check_launch("ffno_block_in");
@@ -684,13 +696,13 @@ int ffno_post(FfnoRows rows, const float* u, const uint16_t* w1, const uint16_t*
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {
     const size_t elems = (size_t)2 * CT * 16 * ks_of(CT) * 32 + (size_t)CT * 16 * 2 * CT * 16;
-    const size_t red = red_bytes_nw<CT, nw_of<CT>()>();
+    const size_t red = red_bytes_nw<CT, nw_fwd<CT>()>();
     const int nl = lds_planes(elems, red);
     FFNO_DISPATCH_NL(nl, {
       const size_t lds = NL * elems * 2 + red;
-      auto k = k_post<CT, nw_of<CT>(), NL>;
+      auto k = k_post<CT, nw_fwd<CT>(), NL>;
       if (set_lds(k, lds)) return -1;
-      k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, u, w1, w2, y, stats);
+      k<<<grid, nw_fwd<CT>() * 64, lds, stream>>>(rows, u, w1, w2, y, stats);
     });
   });
   return check_launch("ffno_post");
@@ -728,13 +740,13 @@ int ffno_resmlp(FfnoRows rows, FfnoAssemble in, float* x_out, const uint16_t* w1
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {
     const size_t elems = (size_t)2 * CT * 16 * ks_of(CT) * 32 + (size_t)CT * 16 * 2 * CT * 16;
-    const size_t red = red_bytes_nw<CT, nw_of<CT>()>();
+    const size_t red = red_bytes_nw<CT, nw_fwd<CT>()>();
     const int nl = lds_planes(elems, red);
     FFNO_DISPATCH_NL(nl, {
       const size_t lds = NL * elems * 2 + red;
-      auto k = k_resmlp<CT, nw_of<CT>(), NL>;
+      auto k = k_resmlp<CT, nw_fwd<CT>(), NL>;
       if (set_lds(k, lds)) return -1;
-      k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, in, x_out, w1, w2, y, stats);
+      k<<<grid, nw_fwd<CT>() * 64, lds, stream>>>(rows, in, x_out, w1, w2, y, stats);
     });
   });
   return check_launch("ffno_resmlp");
@@ -756,18 +768,18 @@ int ffno_head(FfnoRows rows, FfnoAssemble in, float* x_out, const uint16_t* p1, 
   const int nth = (n_head + 15) / 16;
   FFNO_DISPATCH_CT(rows.C, {
     const size_t elems = (size_t)2 * CT * 16 * ks_of(CT) * 32 + (size_t)nth * 16 * 2 * CT * 16;
-    const size_t hs = (size_t)nw_of<CT>() * 16 * nth * 16 * sizeof(float);
+    const size_t hs = (size_t)nw_fwd<CT>() * 16 * nth * 16 * sizeof(float);
     const int nl = lds_planes(elems, hs);
     FFNO_DISPATCH_NL(nl, {
       const size_t lds = NL * elems * 2 + hs;
       if (nth == 1) {
-        auto k = k_head<CT, nw_of<CT>(), 1, NL>;
+        auto k = k_head<CT, nw_fwd<CT>(), 1, NL>;
         if (set_lds(k, lds)) return -1;
-        k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, in, x_out, p1, p2, p2_bias, x_in, *desc, y);
+        k<<<grid, nw_fwd<CT>() * 64, lds, stream>>>(rows, in, x_out, p1, p2, p2_bias, x_in, *desc, y);
       } else {
-        auto k = k_head<CT, nw_of<CT>(), 2, NL>;
+        auto k = k_head<CT, nw_fwd<CT>(), 2, NL>;
         if (set_lds(k, lds)) return -1;
-        k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, in, x_out, p1, p2, p2_bias, x_in, *desc, y);
+        k<<<grid, nw_fwd<CT>() * 64, lds, stream>>>(rows, in, x_out, p1, p2, p2_bias, x_in, *desc, y);
       }
     });
   });

@@ -13,9 +13,15 @@ Vertical (models/ffno_model.py:37-64, :239-259, :270-276, :301-305) on one share
   V_inv  [D x 2M]     f_m cos(k_m z_d) / L and -f_m sin(k_m z_d) / L, f = mode factor * taper
 """
 import math
+import threading
 
 import numpy as np
 import torch
+
+
+# torch's forward-mode AD keeps its dual levels in process-global state, so concurrent torch.func.jvp calls from
+# two threads (two serving workers planning different shapes) can pop each other's level
+_FORWARD_AD_LOCK = threading.Lock()
 
 
 def round_up(n, m):
@@ -163,7 +169,8 @@ def filon_height_bands(column, k):
     for color in range(3):
         v = torch.zeros(D, dtype=torch.float64)
         v[color::3] = 1.0
-        _, (dre, dim) = torch.func.jvp(lambda zz: filon_weights_torch(zz, kk), (z,), (v,))
+        with _FORWARD_AD_LOCK:           # forward-AD levels are process-global: one plan build at a time here
+            _, (dre, dim) = torch.func.jvp(lambda zz: filon_weights_torch(zz, kk), (z,), (v,))
         dre, dim = dre.numpy(), dim.numpy()
         for delta in (-1, 0, 1):
             e = idx - delta                        # row d of the derivative belongs to level e = d - delta

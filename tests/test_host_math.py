@@ -160,3 +160,17 @@ def test_plain_torch_baseline_matches_oracle():
     with torch.no_grad():
         y = plain(x, z, 48e3, 48e3)
     assert np.abs(y.double().numpy() - data["y"]).max() / np.abs(data["y"]).max() < 1e-4
+
+
+def test_height_bands_are_thread_safe():
+    """Plans of different shapes are built concurrently by the serving workers: the Filon height Jacobian (forward-mode
+    AD inside) must give the same bands from several threads at once as alone."""
+    import concurrent.futures as cf
+    cols = [np.linspace(14.0, -1.0, d) for d in (24, 40, 56, 72)]
+    k = 2 * np.pi * np.arange(12) / 40.0
+    alone = [T.filon_height_bands(c, k) for c in cols]
+    with cf.ThreadPoolExecutor(8) as ex:
+        got = list(ex.map(lambda c: T.filon_height_bands(c, k), cols * 4))
+    for i, g in enumerate(got):
+        for d in (-1, 0, 1):
+            assert np.array_equal(g[d], alone[i % 4][d])
