@@ -164,14 +164,16 @@ __global__ __launch_bounds__(256) void k_gn_grad_finalize(int nwg, const float* 
 // ---------------------------------------------------------------------------------------------------------------
 // MLP behind a GroupNorm, with or without the residual path
 // ---------------------------------------------------------------------------------------------------------------
-template <int CT, int NW>
+template <int CT, int NW, bool LW>
 __global__ __launch_bounds__(NW * 64) void k_mlp_bwd(FfnoRows rows, const float* ubar, FfnoGnBack gb, FfnoAssemble xin,
                                                      const uint16_t* w1, const uint16_t* w2t, const uint16_t* w1t,
                                                      int residual, float* xbar) {
   constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32;
   extern __shared__ __align__(16) unsigned char smem[];
   bf16* lds = reinterpret_cast<bf16*>(smem);
-  const Wm W1 = stage_w(lds, w1, 2 * C * KP), W2T = stage_w(lds, w2t, 2 * C * KP), W1T = stage_w(lds, w1t, C * 2 * C);
+  const Wm W1 = LW ? stage_w(lds, w1, 2 * C * KP) : global_w(w1, 2 * C * KP);
+  const Wm W2T = LW ? stage_w(lds, w2t, 2 * C * KP) : global_w(w2t, 2 * C * KP);
+  const Wm W1T = LW ? stage_w(lds, w1t, C * 2 * C) : global_w(w1t, C * 2 * C);
   __syncthreads();
   Lane L;
   const int b = blockIdx.y;
@@ -320,7 +322,7 @@ __global__ __launch_bounds__(NW * 64) void k_fuse_bwd(FfnoRows rows, const float
 // ---------------------------------------------------------------------------------------------------------------
 // pass 1: fp_bar from the GroupNorm (GELU after it), then the fusion MLP's adjoint with its hidden layer recomputed:
 // [sm_bar, vm_bar] = 0.5 fp_bar (both halves) + F1^T (gelu'(F1 [s gs, v gv]) * F2^T fp_bar)
-template <int CT, int NW>
+template <int CT, int NW, bool LW>
 __global__ __launch_bounds__(NW * 64) void k_fuse_mlp_bwd(FfnoRows rows, const float* __restrict__ x1bar, FfnoGnBack gf,
                                                           const float* __restrict__ sp, const float* __restrict__ vp,
                                                           const float* __restrict__ s_a, const float* __restrict__ s_b,
@@ -331,7 +333,8 @@ __global__ __launch_bounds__(NW * 64) void k_fuse_mlp_bwd(FfnoRows rows, const f
   constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32;
   extern __shared__ __align__(16) unsigned char smem[];
   bf16* lds = reinterpret_cast<bf16*>(smem);
-  const Wm F1 = stage_w(lds, f1, 2 * C * 2 * KP), F1T = stage_w(lds, f1t, 2 * C * 2 * C);
+  const Wm F1 = LW ? stage_w(lds, f1, 2 * C * 2 * KP) : global_w(f1, 2 * C * 2 * KP);
+  const Wm F1T = LW ? stage_w(lds, f1t, 2 * C * 2 * C) : global_w(f1t, 2 * C * 2 * C);
   const Wm F2T = global_w(f2t, 2 * C * KP);
   __syncthreads();
   Lane L;
@@ -380,7 +383,7 @@ __global__ __launch_bounds__(NW * 64) void k_fuse_mlp_bwd(FfnoRows rows, const f
 // gp = [sm_bar s, vm_bar v] gt (1 - gt); [h_bar', s_bar', v_bar'] = B1^T (gelu'(B1 [h, s, v]) * B2^T gp);
 // h_bar = x1_bar + h_bar', s_bar = sm_bar gs + s_bar', v_bar = vm_bar gv + v_bar'. In place: hbar over x1bar,
 // sbar over smbar, vbar over vmbar.
-template <int CT, int NW>
+template <int CT, int NW, bool LW>
 __global__ __launch_bounds__(NW * 64) void k_fuse_gates_bwd(FfnoRows rows, float* xh_bar, const float* __restrict__ h,
                                                             const float* __restrict__ sp, const float* __restrict__ vp,
                                                             const float* __restrict__ s_a, const float* __restrict__ s_b,
@@ -391,7 +394,9 @@ __global__ __launch_bounds__(NW * 64) void k_fuse_gates_bwd(FfnoRows rows, float
   constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32;
   extern __shared__ __align__(16) unsigned char smem[];
   bf16* lds = reinterpret_cast<bf16*>(smem);
-  const Wm B1 = stage_w(lds, bg1, C * 3 * KP), B2T = stage_w(lds, bg2t, C * 2 * C), B1T = stage_w(lds, bg1t, 3 * C * KP);
+  const Wm B1 = LW ? stage_w(lds, bg1, C * 3 * KP) : global_w(bg1, C * 3 * KP);
+  const Wm B2T = LW ? stage_w(lds, bg2t, C * 2 * C) : global_w(bg2t, C * 2 * C);
+  const Wm B1T = LW ? stage_w(lds, bg1t, 3 * C * KP) : global_w(bg1t, 3 * C * KP);
   __syncthreads();
   Lane L;
   const int b = blockIdx.y;
@@ -517,7 +522,8 @@ __global__ __launch_bounds__(NW * 64) void k_gates_bwd(FfnoRows rows, const floa
 // ---------------------------------------------------------------------------------------------------------------
 constexpr float LN10 = 2.302585092994046f;
 
-template <int CT, int NW, int NTH>
+// LW: weights staged in LDS (else read from global: widths whose four matrices exceed a CU's LDS)
+template <int CT, int NW, int NTH, bool LW>
 __global__ __launch_bounds__(NW * 64) void k_head_bwd(FfnoRows rows, const float* hin, const uint16_t* p1,
                                                       const uint16_t* p2, const uint16_t* p2t, const uint16_t* p1t,
                                                       const float* __restrict__ p2_bias, const float* __restrict__ x_in,
@@ -526,8 +532,10 @@ __global__ __launch_bounds__(NW * 64) void k_head_bwd(FfnoRows rows, const float
   constexpr int C = CT * 16, KS1 = ks_of(CT);
   extern __shared__ __align__(16) unsigned char smem[];
   bf16* lds = reinterpret_cast<bf16*>(smem);
-  const Wm P1 = stage_w(lds, p1, 2 * C * KS1 * 32), P2 = stage_w(lds, p2, NTH * 16 * 2 * C);
-  const Wm P2T = stage_w(lds, p2t, 2 * C * 32), P1T = stage_w(lds, p1t, C * 2 * C);
+  const Wm P1 = LW ? stage_w(lds, p1, 2 * C * KS1 * 32) : global_w(p1, 2 * C * KS1 * 32);
+  const Wm P2 = LW ? stage_w(lds, p2, NTH * 16 * 2 * C) : global_w(p2, NTH * 16 * 2 * C);
+  const Wm P2T = LW ? stage_w(lds, p2t, 2 * C * 32) : global_w(p2t, 2 * C * 32);
+  const Wm P1T = LW ? stage_w(lds, p1t, C * 2 * C) : global_w(p1t, C * 2 * C);
   float* hs = reinterpret_cast<float*>(lds);
   __syncthreads();
   Lane L;
@@ -725,9 +733,14 @@ int ffno_mlp_bwd(FfnoRows rows, const float* ubar, FfnoGnBack gn, FfnoAssemble x
   FFNO_DISPATCH_CT(rows.C, {
     constexpr int C = CT * 16, KP = ks_of(CT) * 32;
     const size_t lds = ((size_t)2 * C * KP * 2 + (size_t)C * 2 * C) * 2 * 2;
-    auto k = k_mlp_bwd<CT, nw_bwd<CT>()>;
-    if (set_lds(k, lds)) return -1;
-    k<<<grid, nw_bwd<CT>() * 64, lds, stream>>>(rows, ubar, gn, x, w1, w2t, w1t, residual, xbar);
+    if (lds <= 160 * 1024) {
+      auto k = k_mlp_bwd<CT, nw_bwd<CT>(), true>;
+      if (set_lds(k, lds)) return -1;
+      k<<<grid, nw_bwd<CT>() * 64, lds, stream>>>(rows, ubar, gn, x, w1, w2t, w1t, residual, xbar);
+    } else {
+      k_mlp_bwd<CT, nw_bwd<CT>(), false><<<grid, nw_bwd<CT>() * 64, 0, stream>>>(rows, ubar, gn, x, w1, w2t, w1t,
+                                                                                residual, xbar);
+    }
   });
   return check_launch("ffno_mlp_bwd");
 }
@@ -758,11 +771,15 @@ int ffno_fuse_mlp_bwd(FfnoRows rows, const float* x1bar, FfnoGnBack gnf, const f
   FFNO_DISPATCH_CT(rows.C, {
     constexpr int C = CT * 16, KP = ks_of(CT) * 32;
     const size_t lds = ((size_t)2 * C * 2 * KP + (size_t)2 * C * 2 * C) * 2 * 2;
-    if (lds > 160 * 1024) return fail("ffno_fuse_mlp_bwd: width too large for the LDS-staged pass");
-    auto k = k_fuse_mlp_bwd<CT, nw_of<CT>()>;
-    if (set_lds(k, lds)) return -1;
-    k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, x1bar, gnf, sp, vp, s_a, s_b, v_a, v_b, gates, f1, f2t, f1t, smbar,
-                                               vmbar);
+    if (lds <= 160 * 1024) {
+      auto k = k_fuse_mlp_bwd<CT, nw_of<CT>(), true>;
+      if (set_lds(k, lds)) return -1;
+      k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, x1bar, gnf, sp, vp, s_a, s_b, v_a, v_b, gates, f1, f2t, f1t,
+                                                 smbar, vmbar);
+    } else {
+      k_fuse_mlp_bwd<CT, nw_of<CT>(), false><<<grid, nw_of<CT>() * 64, 0, stream>>>(
+          rows, x1bar, gnf, sp, vp, s_a, s_b, v_a, v_b, gates, f1, f2t, f1t, smbar, vmbar);
+    }
   });
   return check_launch("ffno_fuse_mlp_bwd");
 }
@@ -778,11 +795,15 @@ int ffno_fuse_gates_bwd(FfnoRows rows, float* x1bar_hbar, const float* h, const 
   FFNO_DISPATCH_CT(rows.C, {
     constexpr int C = CT * 16, KP = ks_of(CT) * 32;
     const size_t lds = ((size_t)C * 3 * KP + (size_t)C * 2 * C + (size_t)3 * C * KP) * 2 * 2;
-    if (lds > 160 * 1024) return fail("ffno_fuse_gates_bwd: width too large for the LDS-staged pass");
-    auto k = k_fuse_gates_bwd<CT, nw_of<CT>()>;
-    if (set_lds(k, lds)) return -1;
-    k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, x1bar_hbar, h, sp, vp, s_a, s_b, v_a, v_b, gates, bg1, bg2t, bg1t,
-                                               sm_sbar, vm_vbar);
+    if (lds <= 160 * 1024) {
+      auto k = k_fuse_gates_bwd<CT, nw_of<CT>(), true>;
+      if (set_lds(k, lds)) return -1;
+      k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, x1bar_hbar, h, sp, vp, s_a, s_b, v_a, v_b, gates, bg1, bg2t,
+                                                 bg1t, sm_sbar, vm_vbar);
+    } else {
+      k_fuse_gates_bwd<CT, nw_of<CT>(), false><<<grid, nw_of<CT>() * 64, 0, stream>>>(
+          rows, x1bar_hbar, h, sp, vp, s_a, s_b, v_a, v_b, gates, bg1, bg2t, bg1t, sm_sbar, vm_vbar);
+    }
   });
   return check_launch("ffno_fuse_gates_bwd");
 }
@@ -828,16 +849,21 @@ int ffno_head_bwd(FfnoRows rows, const float* h, const uint16_t* p1, const uint1
     constexpr int C = CT * 16;
     const size_t w_bytes = ((size_t)2 * C * ks_of(CT) * 32 + (size_t)nth * 16 * 2 * C + (size_t)2 * C * 32 +
                             (size_t)C * 2 * C) * 2 * 2;
-    const size_t lds = w_bytes + (size_t)nw_of<CT>() * 16 * nth * 16 * sizeof(float);
-    if (nth == 1) {
-      auto k = k_head_bwd<CT, nw_of<CT>(), 1>;
-      if (set_lds(k, lds)) return -1;
-      k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, h, p1, p2, p2t, p1t, p2_bias, x_in, *desc, ybar, hbar, xbar_in);
-    } else {
-      auto k = k_head_bwd<CT, nw_of<CT>(), 2>;
-      if (set_lds(k, lds)) return -1;
-      k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, h, p1, p2, p2t, p1t, p2_bias, x_in, *desc, ybar, hbar, xbar_in);
-    }
+    const size_t hs = (size_t)nw_of<CT>() * 16 * nth * 16 * sizeof(float);
+    const bool lw = w_bytes + hs <= 160 * 1024;
+    const size_t lds = (lw ? w_bytes : 0) + hs;
+    auto launch = [&](auto kernel) -> int {
+      if (set_lds(kernel, lds)) return -1;
+      kernel<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, h, p1, p2, p2t, p1t, p2_bias, x_in, *desc, ybar, hbar,
+                                                      xbar_in);
+      return 0;
+    };
+    int rc;
+    if (nth == 1)
+      rc = lw ? launch(k_head_bwd<CT, nw_of<CT>(), 1, true>) : launch(k_head_bwd<CT, nw_of<CT>(), 1, false>);
+    else
+      rc = lw ? launch(k_head_bwd<CT, nw_of<CT>(), 2, true>) : launch(k_head_bwd<CT, nw_of<CT>(), 2, false>);
+    if (rc) return rc;
   });
   return check_launch("ffno_head_bwd");
 }

@@ -5,6 +5,7 @@ Activations live channel-last ([B, D, H, W, C] "voxel rows") in float32 HBM; eve
 with float32 accumulation, every transform as a split-bf16 GEMM. The reference forward this replaces is
 models/ffno_model.py:800-832 (eval mode, no dropout).
 """
+import collections
 import ctypes
 import math
 import threading
@@ -20,7 +21,8 @@ from ._lib import FfnoAssemble, FfnoHeadDesc, FfnoMix, FfnoRows, FfnoStatsOut, F
 SUPPORTED_WIDTHS = (16, 32, 64, 96, 128)
 GROUPS = 8
 NW = 4                       # waves per workgroup of the voxel-row kernels
-TABLE_CACHE_BYTES = 1 << 30  # per-mode tables of all blocks at most this large stay cached in the plan
+TABLE_CACHE_BYTES = 8 << 30  # a branch's per-mode tables of all blocks stay cached when at most this large
+MODE_CACHE_BYTES = 24 << 30  # all cached per-mode tables of an engine (least recently used sets are dropped)
 H_CHUNK_BYTES = 2 << 30      # horizontal half-spectrum scratch per buffer: (b, d) planes are transformed in chunks
 WORKING_UNITS = 9            # activation-sized buffers of the adjoint besides the saved block inputs
 GATE_UNITS = 2               # the stored branch gates of the two-pass fusion adjoint
@@ -28,6 +30,13 @@ GATE_UNITS = 2               # the stored branch gates of the two-pass fusion ad
 # MULTI3D constants in cgs (models/ffno_model.py:111-113)
 _EE, _HH, _EM, _BK, _PI = 1.602189e-12, 6.626176e-27, 9.109534e-28, 1.380662e-16, 3.14159265359
 _LN_CCON = math.log(0.5 * (_HH / math.sqrt(2.0 * _PI * _EM) / math.sqrt(_BK)) ** 3)
+
+
+def _settle(device):
+    """Wait for the current stream's work: tables and weights built once are then read from other workers' streams
+    (a HIP stream does not wait for another's writes)."""
+    if torch.device(device).type == "cuda":
+        torch.cuda.current_stream(device).synchronize()
 
 
 def _p(t):
@@ -189,6 +198,59 @@ class PackedWeights:
         return d
 
 
+class ModeTables:
+    """Per-mode tables of one branch for every block (models/ffno_model.py:394-429, :285-296): the gate g(k) and the
+    rank-r factors V(k), U(k) packed for the mix kernel (three planes: the forward's three-way products), and for the
+    adjoint, built on first use, conj(g) and the low-rank roles swapped (V' = U^T, U' = V^T, two planes). They depend
+    on the mode features only -- (H, W, dx, dy) for the horizontal branch, the wavenumbers for the vertical one -- so
+    every plan with the same modes shares one set. A set whose blocks would exceed TABLE_CACHE_BYTES is rebuilt per
+    block on use instead of kept."""
+
+    def __init__(self, sd, branch, feats, C, rank, n_layers):
+        self.sd, self.branch, self.feats, self.C, self.rank, self.n_layers = sd, branch, feats, C, rank, n_layers
+        self._lock = threading.Lock()
+        self._fwd, self._adj = {}, {}
+        first = self._build(0, False)
+        per_mode = sum(t.numel() * t.element_size() for t in first.values() if t is not None)
+        per_mode_adj = per_mode * 2 // 3 if rank else per_mode       # two planes instead of three
+        self.block_bytes = per_mode + per_mode_adj
+        self.cache = self.block_bytes * n_layers <= TABLE_CACHE_BYTES
+        _settle(feats.device)                   # feats and block 0's tables are read from other streams later
+        if self.cache:
+            self._fwd[0] = first
+
+    def cached_bytes(self):
+        return sum(t.numel() * t.element_size() for d in (self._fwd, self._adj) for e in d.values()
+                   for t in e.values() if t is not None)
+
+    def _build(self, i, adjoint):
+        prefix = f"blocks.{i}.{self.branch}."
+        gate = _freq_mlp(prefix, self.sd, self.feats).float()
+        if adjoint:
+            gate[:, 1] = -gate[:, 1]
+        e = dict(gate=gate.contiguous(), v=None, u=None)
+        if self.rank:
+            u, v = _freq_lowrank(prefix, self.sd, self.feats, self.C, self.rank)
+            if adjoint:
+                e["v"] = pack_mlp_weight(u.transpose(-1, -2), n_pad=16)      # [modes, 2, 16, KS*32]
+                e["u"] = pack_mlp_weight(v.transpose(-1, -2))                # [modes, 2, C, 32]
+            else:
+                e["v"] = pack_mlp_weight(v, n_pad=16, parts=3)               # [modes, 3, 16, KS*32]
+                e["u"] = pack_mlp_weight(u, parts=3)                         # [modes, 3, C, 32]
+        return e
+
+    def get(self, i, adjoint=False):
+        store = self._adj if adjoint else self._fwd
+        with self._lock:
+            e = store.get(i)
+            if e is None:
+                e = self._build(i, adjoint)
+                if self.cache:
+                    _settle(self.feats.device)          # kept tables are read by other workers' streams
+                    store[i] = e
+        return e
+
+
 class Plan:
     """Everything that depends on the request's shape, heights and spacings but not on its data: transform
     matrices (split-bf16), per-mode gates and low-rank tables of every block (cached when small, else rebuilt per
@@ -231,57 +293,22 @@ class Plan:
         # horizontal transforms run over chunks of (b, d) planes so that their half-spectrum scratch stays bounded
         self.plane = 2 * H * self.wf * C
         self.h_chunk = max(1, min(B * D, H_CHUNK_BYTES // (4 * self.plane)))
-        # per-mode k-features (models/ffno_model.py:375-392, :443-447, :231-237), float32
-        self._sd = engine.weights.sd
-        ky = torch.fft.fftfreq(H, d=float(dy), device=dev)
-        kx = torch.fft.rfftfreq(W, d=float(dx), device=dev)
-        ky, kx = torch.meshgrid(ky, kx, indexing="ij")
-        ky, kx = ky * 1e5, kx * 1e5
-        hfeat = torch.stack([kx, ky, torch.sqrt(kx ** 2 + ky ** 2 + 1e-12), torch.sign(ky)], -1).reshape(-1, 4)
-        kt = torch.from_numpy(k).float().to(dev)
-        qf = kt * m.z_feature_wavelength / (2 * math.pi)
-        tf = qf / (1 + qf)
-        self._feats = dict(spec=hfeat, vertical=torch.stack([tf, tf * tf], -1))
-        first = self._build_tables(0)
-        self.table_bytes = sum(t.numel() * t.element_size() for e in first.values() for t in e.values()
-                               if t is not None) * m.n_layers
-        self._cache_tables = self.table_bytes <= TABLE_CACHE_BYTES
-        self._tables = {0: first} if self._cache_tables else {}
+        # per-mode tables, shared with every plan of the same modes
+        self._mode_tables = dict(spec=engine.mode_tables("spec", H, W, float(dx), float(dy)),
+                                 vertical=engine.mode_tables("vertical", self.length, tuple(k.tolist())))
         nt = (self.voxels + 15) // 16
         self.nwg = max(1, min((nt + NW - 1) // NW, max(1, 2048 // B)))
         self.rows = FfnoRows(B, self.voxels, C, self.nwg)
         self.device = dev
+        _settle(dev)                            # the plan's matrices are read from every worker's stream
 
-    def _build_tables(self, i):
-        """Per-mode tables of block i for both branches (models/ffno_model.py:394-429, :285-296): the gate g(k), and
-        for the rank-r term V(k) and U(k) packed for the mix kernel; the adjoint's versions beside them: conj(g) and
-        the low-rank roles swapped (V' = U^T, U' = V^T)."""
-        C, r = self.C, self.rank
-        out = {}
-        for br in ("spec", "vertical"):
-            prefix = f"blocks.{i}.{br}."
-            feats = self._feats[br]
-            gate = _freq_mlp(prefix, self._sd, feats).float().contiguous()
-            g_adj = gate.clone()
-            g_adj[:, 1] = -g_adj[:, 1]
-            e = dict(gate=gate, gate_adj=g_adj.contiguous(), v=None, u=None, v_adj=None, u_adj=None)
-            if r:
-                u, v = _freq_lowrank(prefix, self._sd, feats, C, r)
-                e["v"] = pack_mlp_weight(v, n_pad=16, parts=3)             # [modes, 3, 16, KS*32]
-                e["u"] = pack_mlp_weight(u, parts=3)                       # [modes, 3, C, 32]
-                e["v_adj"] = pack_mlp_weight(u.transpose(-1, -2), n_pad=16)
-                e["u_adj"] = pack_mlp_weight(v.transpose(-1, -2))
-            out[br] = e
-        return out
+    def tables(self, i, adjoint=False):
+        """{"spec": ..., "vertical": ...} per-mode tables of block i (the adjoint's with adjoint=True)."""
+        return {br: t.get(i, adjoint) for br, t in self._mode_tables.items()}
 
-    def tables(self, i):
-        with self._lock:
-            t = self._tables.get(i)
-            if t is None:
-                t = self._build_tables(i)
-                if self._cache_tables:
-                    self._tables[i] = t
-        return t
+    def uncached_table_bytes(self):
+        """Per-mode tables that are rebuilt per block on use (not kept): one block's worth is alive at a time."""
+        return sum(t.block_bytes for t in self._mode_tables.values() if not t.cache)
 
     def height_tables(self):
         """Height derivatives of the vertical transforms for the z-gradient: the three bands of the Filon weights'
@@ -298,6 +325,7 @@ class Plan:
                     v_dz=T.split_bf16(v_dz, self.device),
                     v0=T.split_bf16(np.ascontiguousarray(v0.T), self.device) if np.any(dfac != 0) else None,
                     dfac=torch.from_numpy(dfac).float().to(self.device), i_med=i_med, sign=sgn, argmin=ties)
+                _settle(self.device)
             return self._height
 
     def adjoint_tables(self):
@@ -307,6 +335,7 @@ class Plan:
             if self._tw_adj is None:
                 self._tw_adj = {name + "_T": T.split_bf16(np.ascontiguousarray(a.T), self.device)
                                 for name, a in self._mats.items()}
+                _settle(self.device)
             return self._tw_adj
 
     # ------------------------------------------------------------------------------------------------ memory
@@ -324,15 +353,13 @@ class Plan:
         GroupNorm inputs, x1, the gates) and the vertical spectrum with want_z, plus the head input and two working
         units."""
         v1 = self.B * 2 * self.modes * self.H * self.W * self.C * 4
-        tables = 0 if self._cache_tables else 2 * self.table_bytes // self.n_layers
         return (self.unit_bytes() * (11 * self.n_layers + 3) + (self.n_layers if want_z else 0) * v1 +
-                self.scratch_bytes() + tables)
+                self.scratch_bytes() + 2 * self.uncached_table_bytes())
 
     def vjp_bytes(self, stride, two_pass=False):
         saved = -(-self.n_layers // stride) + (1 if stride == 1 else 0) + (stride - 1)
-        tables = 0 if self._cache_tables else 2 * self.table_bytes // self.n_layers
         units = saved + WORKING_UNITS + (GATE_UNITS if two_pass else 0)
-        return self.unit_bytes() * units + self.scratch_bytes() + tables
+        return self.unit_bytes() * units + self.scratch_bytes() + 2 * self.uncached_table_bytes()
 
 
 class Saved:
@@ -372,6 +399,7 @@ class Engine:
         self.two_pass_fuse = None           # None: whenever its gate buffer fits at stride 1
         self.full_save = None               # None: keep every intermediate when that fits; False: never
         self.last_stride = None
+        self._modes = collections.OrderedDict()     # per-mode table sets shared by plans, least recently used first
 
     def footprint_bytes(self, B, D, H, W, vjp=False):
         """HBM one request of this shape needs (for admission; the tables of a cached plan not included):
@@ -398,7 +426,43 @@ class Engine:
                 self.weights = PackedWeights(self.module, device)
                 self.weights.adjoint()
                 self._version = ver
+                _settle(device)
                 self._plans.clear()
+                self._modes.clear()
+
+    def mode_tables(self, branch, *key):
+        """The shared ModeTables of a branch for these modes: key (H, W, dx, dy) for "spec", (L, k) for "vertical".
+        Sets are dropped least recently used first while all cached ones exceed MODE_CACHE_BYTES (a plan holding a
+        dropped set keeps it alive until the plan itself goes)."""
+        with self._lock:
+            t = self._modes.get((branch,) + key)
+            if t is not None:
+                self._modes.move_to_end((branch,) + key)
+                return t
+            m, dev = self.module, self.device
+            if branch == "spec":
+                H, W, dx, dy = key
+                ky = torch.fft.fftfreq(H, d=float(dy), device=dev)
+                kx = torch.fft.rfftfreq(W, d=float(dx), device=dev)
+                ky, kx = torch.meshgrid(ky, kx, indexing="ij")
+                ky, kx = ky * 1e5, kx * 1e5      # models/ffno_model.py:375-392, :443-447 (float32)
+                feats = torch.stack([kx, ky, torch.sqrt(kx ** 2 + ky ** 2 + 1e-12), torch.sign(ky)], -1)
+                feats = feats.reshape(-1, 4)
+            else:
+                kt = torch.tensor(key[1], dtype=torch.float32, device=dev)
+                qf = kt * m.z_feature_wavelength / (2 * math.pi)       # models/ffno_model.py:231-237
+                tf = qf / (1 + qf)
+                feats = torch.stack([tf, tf * tf], -1)
+            t = ModeTables(self.weights.sd, branch, feats, m.width, m.freq_mix_rank, m.n_layers)
+            self._modes[(branch,) + key] = t
+            while len(self._modes) > 1 and sum(v.cached_bytes() for v in self._modes.values()) > MODE_CACHE_BYTES:
+                self._modes.popitem(last=False)
+            return t
+
+    def cached_bytes(self):
+        """Device memory held by cached per-mode tables (the serving admission counts it)."""
+        with self._lock:
+            return sum(v.cached_bytes() for v in self._modes.values())
 
     def plan(self, B, D, H, W, z, dx, dy):
         """The plan for these heights, cached by content: the reference's shared-grid decision
@@ -412,7 +476,10 @@ class Engine:
         if not shared:
             raise NotImplementedError("per-column vertical grids are not implemented in the HIP engine; "
                                       "every column must share its heights (after the per-column shift)")
-        column = zs[0, :, 0].double().cpu().numpy()
+        # the shared column, shifted in float64 (the spacings then equal the float64 reference's bit for bit, so a
+        # tie between spacings at the taper's median resolves to the same interval)
+        z0 = zc[0, :, 0].double()
+        column = (z0 - z0.min()).cpu().numpy()
         key = (B, D, H, W, float(dx), float(dy), float(span), column.tobytes())
         with self._lock:
             p = self._plans.get(key)
@@ -481,9 +548,8 @@ class Engine:
         mx.C, mx.rows, mx.modes, mx.rank = plan.C, rows, modes, plan.rank
         mx.mode_stride = mode_stride
         mx.r_split, mx.r_hi, mx.r_lo = rmap
-        sfx = "_adj" if adjoint else ""
-        mx.wc, mx.gate = _p(wc), _p(e["gate" + sfx])
-        mx.v_p, mx.u_p = _p(e["v" + sfx]), _p(e["u" + sfx])
+        mx.wc, mx.gate = _p(wc), _p(e["gate"])
+        mx.v_p, mx.u_p = _p(e["v"]), _p(e["u"])
         mx.factor = ctypes.c_void_p(0)
         mx.parts = 2 if adjoint else 3
         _lib.call("ffno_mix", ctypes.byref(mx), self._stream())
@@ -500,6 +566,7 @@ class Engine:
             ta = plan.adjoint_tables()
             m1, m2, m3, m4 = ta["w_inv_T"], ta["h_inv_T"], ta["h_fwd_T"], ta["w_fwd_T"]
             wc = self.weights.adjoint()["blocks"][i]["spec.wc"]
+            tables = plan.tables(i, adjoint=True)
         else:
             m1, m2, m3, m4 = plan.tw["w_fwd"], plan.tw["h_fwd"], plan.tw["h_inv"], plan.tw["w_inv"]
             wc = self.weights.blocks[i]["spec.wc"]
@@ -529,6 +596,7 @@ class Engine:
             ta = plan.adjoint_tables()
             m1, m2 = ta["v_inv_T"], ta["v_fwd_T"]
             wc = self.weights.adjoint()["blocks"][i]["vertical.wc"]
+            tables = plan.tables(i, adjoint=True)
         else:
             m1, m2 = plan.tw["v_fwd"], plan.tw["v_inv"]
             wc = self.weights.blocks[i]["vertical.wc"]

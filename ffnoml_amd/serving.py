@@ -177,7 +177,9 @@ class ContinuousBatcher:
                 # the oldest waiting request picks the bucket; same-bucket requests join it, oldest first
                 head = self.pending[0]
                 batch = [r for r in self.pending if r.key == head.key][: self.max_batch]
-                free = self.budget - self.graph_held
+                # per-mode tables the engines cache are device memory outside any batch's footprint
+                free = self.budget - self.graph_held - sum(b.model.engine().cached_bytes()
+                                                           for b in self.backends.values())
                 while len(batch) > 1 and self.claimed + self._footprint(batch) > free:
                     batch.pop()
                 need = self._footprint(batch)
@@ -209,13 +211,20 @@ class ContinuousBatcher:
                         results, capture = self._run(batch)
                     stream.synchronize()
                 done = time.perf_counter()
-                for r, res in zip(batch, results):
+                for r in batch:
                     r.t_done = done
+                with self.cv:                           # recorded before any caller can see its answer
+                    self.completed.extend(batch)
+                for r, res in zip(batch, results):
                     r.future.set_result(res)
             except Exception as e:                      # one failed batch fails its own requests only
                 done = time.perf_counter()
+                with self.cv:
+                    for r in batch:
+                        if r.t_done is None:
+                            r.t_done = done
+                            self.completed.append(r)
                 for r in batch:
-                    r.t_done = done
                     if not r.future.done():
                         r.future.set_exception(e)
             try:
@@ -225,7 +234,6 @@ class ContinuousBatcher:
                 with self.cv:
                     self.claimed -= need
                     self.inflight -= 1
-                    self.completed.extend(batch)
                     self.cv.notify_all()
 
     def _stack(self, backend, batch):
@@ -241,13 +249,14 @@ class ContinuousBatcher:
         if r0.op == "PREDICT":
             n, capture = self._predict(backend, r0, x, z, len(batch))
             if not torch.isfinite(n).all() or torch.any(n <= 0):
-                raise RuntimeError("FFNO returned non-positive or non-finite populations")
+                raise RuntimeError(f"FFNO returned non-positive or non-finite populations (batch of {len(batch)}, "
+                                   f"tile {tuple(r0.z.shape)})")
             out = n.permute(0, 2, 3, 4, 1).float().cpu().numpy()
             return [{"populations": out[i]} for i in range(len(batch))], capture
         cot = torch.from_numpy(np.stack([r.cotangent for r in batch])).to(backend.device).permute(0, 4, 1, 2, 3)
         n, fb, zb = backend.vjp_tensor(x, z, r0.dx, r0.dy, cot.contiguous())
         if not torch.isfinite(fb).all() or not torch.isfinite(zb).all():
-            raise RuntimeError("FFNO VJP returned NaN or Inf")
+            raise RuntimeError(f"FFNO VJP returned NaN or Inf (batch of {len(batch)}, tile {tuple(r0.z.shape)})")
         fb, zb = fb.float().cpu().numpy(), zb.float().cpu().numpy()
         return [{"features": fb[i], "z_scale": zb[i]} for i in range(len(batch))], None
 

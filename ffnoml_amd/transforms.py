@@ -183,15 +183,16 @@ def filon_height_bands(column, k):
 def vertical_height_terms(column, k, factor, length):
     """The height derivatives of the synthesis: V_dz [D, 2M] = d V_inv / d z_d (-f k sin, -f k cos over L), the
     unscaled synthesis V0 [D, 2M] = (cos, -sin) / L, and per mode factor * d taper / d dz at the column's dz
-    together with the index i of the median spacing (dz = |z_{i+1} - z_i|)."""
+    together with the index i of the median spacing (dz = |z_{i+1} - z_i|). Spacings that tie at the median make the
+    median non-differentiable; the gradient goes to the lower middle of a stable sort, the interval torch.median
+    (models/ffno_model.py:241) returns on the CPU for ties."""
     col = np.asarray(column, np.float64)
     f = factor * column_taper(col, k)
     ph = np.outer(col, k)
     v_dz = np.concatenate([-f * k * np.sin(ph), -f * k * np.cos(ph)], 1) / length
     v0 = np.concatenate([np.cos(ph), -np.sin(ph)], 1) / length
     d = np.abs(np.diff(col))
-    order = np.argsort(d, kind="stable")
-    i = int(order[(len(d) - 1) // 2])
+    i = int(np.argsort(d, kind="stable")[(len(d) - 1) // 2])
     dz = d[i]
     k_nyq = math.pi / dz
     u = (k - 2 * k_nyq / 3) / (k_nyq / 3)
