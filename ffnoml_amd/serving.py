@@ -10,9 +10,10 @@ statistics are per sample, so a request batched with others gets what it would g
 Forward-only batches of small tiles (forward footprint at most `graph_bytes`), once their bucket has been seen,
 replay a captured HIP graph of the whole forward (static input/output buffers per (bucket, batch size)); there the
 launch overhead of the ~15 launches per layer is a visible share of the batch time. The memory a graph holds is
-counted against the HBM budget. A capture takes the device alone: batches run under shared holds of a gate and a
-capture waits for an exclusive one, so no other stream's host-synchronising work can invalidate it. Larger tiles and
-VJP batches run eagerly.
+counted against the HBM budget. A capture takes the device alone: batches run under shared holds of a gate, and a
+capture happens only when the gate is idle (no batch in flight), so no other stream's host-synchronising work can
+invalidate it and no batch ever waits for one; a busy device defers the capture to a later batch of the bucket.
+Larger tiles and VJP batches run eagerly.
 """
 import collections
 import contextlib
@@ -63,6 +64,21 @@ class DeviceGate:
             with self.cv:
                 self.readers -= 1
                 self.cv.notify_all()
+
+    @contextlib.contextmanager
+    def try_exclusive(self):
+        """An exclusive hold only if the device is idle right now (yields False, holding nothing, otherwise)."""
+        with self.cv:
+            ok = not self.writer and not self.readers and not self.waiting
+            if ok:
+                self.writer = True
+        try:
+            yield ok
+        finally:
+            if ok:
+                with self.cv:
+                    self.writer = False
+                    self.cv.notify_all()
 
     @contextlib.contextmanager
     def exclusive(self):
@@ -297,7 +313,9 @@ class ContinuousBatcher:
                 return
             if self.claimed + self.graph_held + held > self.budget:
                 return                                  # no room to keep another graph's memory now
-        with self.gate.exclusive(), torch.cuda.stream(stream):
+        with self.gate.try_exclusive() as idle, torch.cuda.stream(stream):
+            if not idle:
+                return                                  # other batches are running: capture on a later batch
             plan = backend.plan_for(z, dx, dy)          # the host synchronisation of planning happens here
             sx, sz = x.clone(), z.clone()
             stream.synchronize()
