@@ -22,6 +22,11 @@ class FfnoAssemble(ctypes.Structure):
                 ("act_gelu", ctypes.c_int)]
 
 
+class FfnoAssembleT(ctypes.Structure):
+    _fields_ = [("base_dot", c_ptr), ("t", c_ptr), ("t_dot", c_ptr), ("aff_a", c_ptr), ("aff_b", c_ptr),
+                ("tan_b", c_ptr), ("tan_c", c_ptr), ("scale", c_ptr), ("act_gelu", ctypes.c_int)]
+
+
 class FfnoHeadDesc(ctypes.Structure):
     _fields_ = [("n_species", ctypes.c_int), ("levels", ctypes.c_int * 4), ("log10_mass", ctypes.c_float * 4),
                 ("log_g", ctypes.c_float * 32), ("ev", ctypes.c_float * 32), ("ndxi", ctypes.c_float * 32),
@@ -75,6 +80,14 @@ SIGNATURES = {
     "ffno_head_bwd": [FfnoRows] + [c_ptr] * 7 + [ctypes.POINTER(FfnoHeadDesc)] + [c_ptr] * 3 + [c_ptr],
     "ffno_lift_bwd": [FfnoRows, c_ptr, FfnoGnBack, c_ptr, c_ptr, c_ptr],
     "ffno_dot_reduce": [FfnoRows, ctypes.c_int, c_ptr, ctypes.c_int, c_ptr, c_ptr, c_ptr],
+    "ffno_gn_tangent_stats": [FfnoRows] + [c_ptr] * 5 + [c_ptr],
+    "ffno_gn_tangent_finalize": [ctypes.c_int, ctypes.c_int, ctypes.c_int] + [c_ptr] * 5 + [c_ptr],
+    "ffno_assemble_t": [FfnoRows, FfnoAssembleT, c_ptr, c_ptr],
+    "ffno_lift_t": [FfnoRows, c_ptr, c_ptr, c_ptr, c_ptr],
+    "ffno_gates_t": [FfnoRows] + [c_ptr] * 8 + [c_ptr],
+    "ffno_mlp_t": [FfnoRows, FfnoAssemble, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr],
+    "ffno_fuse_t": [FfnoRows] + [c_ptr] * 16 + [c_ptr],
+    "ffno_head_t": [FfnoRows, FfnoAssemble] + [c_ptr] * 6 + [ctypes.POINTER(FfnoHeadDesc), c_ptr, c_ptr],
 }
 
 _lib = None

@@ -3,28 +3,86 @@ adjoint (per-block recomputation from the saved block inputs), returning x_bar a
 is the reference's checkpointed backward). z_bar is the per-column gradient of every column's own heights; on a shared
 grid the reference's autograd instead sends the vertical transform's height sensitivity of all columns to the first
 column's heights (models/ffno_model.py:270-276), so the two agree where the reference takes its per-column path."""
+import contextlib
+import threading
+
 import torch
+
+_STATE = threading.local()     # per calling thread: whether the call in progress will be differentiated
+
+
+def _plain(t):
+    """The tensor under torch.func's wrappers (a transform hands a custom Function's jvp wrapped tangents)."""
+    f = getattr(torch._C, "_functorch", None)
+    while f is not None and t is not None and f.is_functorch_wrapped_tensor(t):
+        t = f.get_unwrapped(t)
+    return t
+
+
+def _engine_mode():
+    """A context in which the engine's own tensor work is not lifted into torch.func's wrappers: inside a custom
+    Function's jvp under torch.func.jvp, even fresh tensors come out wrapped and have no data pointer to pass to the
+    HIP library."""
+    disable = getattr(torch._C, "_DisableFuncTorch", None)
+    return disable() if disable is not None else contextlib.nullcontext()
+
+
+class _Info:
+    """What forward hands to setup_context: the inputs as forward saw them and the adjoint's saved state. A plain
+    object, not a dict or tuple, so that torch.func does not wrap the tensors inside as outputs of the Function."""
+
+    def __init__(self, x, z):
+        self.x, self.z, self.saved = x, z, None
 
 
 class FFNOFunction(torch.autograd.Function):
-    @staticmethod
-    def forward(ctx, x, z, module, dx, dy):
-        engine = module.engine()
-        need = ctx.needs_input_grad[0] or ctx.needs_input_grad[1]
-        if not need:
-            return engine.forward(x.detach(), z.detach(), dx, dy)
-        y, saved = engine.forward_saved(x.detach(), z.detach(), dx, dy, want_z=ctx.needs_input_grad[1])
-        ctx.save_for_backward(x, z)
-        ctx.saved = saved
-        ctx.module, ctx.dx, ctx.dy = module, dx, dy
-        return y
+    """forward returns (y, info): y the normalised log10 n; info (_Info) the inputs as forward saw them and what the
+    adjoint needs (None when no input requires a gradient). setup_context keeps info, so the same Function serves
+    autograd (backward), torch.func transforms and forward-mode duals (jvp)."""
 
     @staticmethod
-    def backward(ctx, ybar):
-        x, z = ctx.saved_tensors
+    def forward(x, z, module, dx, dy):
+        engine = module.engine()
+        x, z = _plain(x).detach(), _plain(z).detach()
+        info = _Info(x, z)
+        need = getattr(_STATE, "needs_grad", False)
+        with _engine_mode():
+            if not need:
+                return engine.forward(x, z, dx, dy), info
+            y, info.saved = engine.forward_saved(x, z, dx, dy, want_z=need == "z")
+        return y, info
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        _, _, module, dx, dy = inputs
+        ctx.info = output[1]
+        ctx.module, ctx.dx, ctx.dy = module, dx, dy
+        ctx.set_materialize_grads(False)
+
+    @staticmethod
+    def jvp(ctx, x_dot, z_dot, *_):
+        """Forward-mode derivative along x_dot (the hand-written tangent kernels); a tangent of the heights is not
+        supported (the reference's jvp_local perturbs the features only, ffno_fsdp_runtime.py:350-372)."""
+        z_dot = _plain(z_dot)
+        if z_dot is not None and bool(torch.any(z_dot != 0)):
+            raise NotImplementedError("FFNO3D JVP along the heights is not implemented; give a feature tangent")
+        x, z = ctx.info.x, ctx.info.z
+        if x_dot is None:
+            return torch.zeros(x.shape[0], ctx.module.out_channels, *x.shape[2:], device=x.device), None
+        with _engine_mode():
+            _, y_dot = ctx.module.engine().jvp(x, z, ctx.dx, ctx.dy, _plain(x_dot))
+        return y_dot, None
+
+    @staticmethod
+    def backward(ctx, ybar, _info_bar=None):
+        info = ctx.info
         engine = ctx.module.engine()
-        xbar, zbar = engine.vjp(x, z, ctx.dx, ctx.dy, ybar.contiguous(), ctx.saved, want_z=ctx.needs_input_grad[1])
-        ctx.saved = None
+        with _engine_mode():
+            if info.saved is None:          # a transform asked for the adjoint of a forward that kept nothing
+                _, info.saved = engine.forward_saved(info.x, info.z, ctx.dx, ctx.dy, want_z=ctx.needs_input_grad[1])
+            xbar, zbar = engine.vjp(info.x, info.z, ctx.dx, ctx.dy, _plain(ybar).contiguous(), info.saved,
+                                    want_z=ctx.needs_input_grad[1])
+        info.saved = None
         return xbar if ctx.needs_input_grad[0] else None, zbar, None, None, None
 
 
@@ -45,4 +103,11 @@ def ffno_apply(module, x, z_scale, dx, dy):
         raise RuntimeError("ffnoml_amd.FFNO3D runs on the GPU through libffno_hip.so; it has no CPU path")
     if z_scale.dim() == 5:
         z_scale = z_scale[:, 0]
-    return FFNOFunction.apply(x, z_scale, module, _spacing(dx), _spacing(dy))
+    # whether the adjoint will be asked for (forward then keeps what it needs): decided here, where the autograd state
+    # of the caller's tensors is visible
+    grad = torch.is_grad_enabled() and (x.requires_grad or z_scale.requires_grad)
+    _STATE.needs_grad = ("z" if z_scale.requires_grad else "x") if grad else False
+    try:
+        return FFNOFunction.apply(x, z_scale, module, _spacing(dx), _spacing(dy))[0]
+    finally:
+        _STATE.needs_grad = False

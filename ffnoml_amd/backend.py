@@ -167,7 +167,32 @@ class GPUBackend:
         self.call_count += 1
         return {"features": fb[0].float().cpu().numpy(), "z_scale": zb[0].float().cpu().numpy()}
 
+    def jvp_tensor(self, x_si, z_si, dx, dy, tangent_si):
+        """(populations, populations_dot) [B, L, nz, nx, ny] along a tangent of the SI features [B, 6, nz, nx, ny]:
+        the SI chain written out around the engine's tangent pass -- through log10 T (T_dot / (T ln 10)), km/s, the
+        input normalisation, then the output normalisation, 10^ and its clamp (zero slope outside +-37)."""
+        xm, zm = self.model_inputs(x_si, z_si)
+        fd = torch.cat((tangent_si[:, 0:1] / (x_si[:, 0:1] * LN10), tangent_si[:, 1:4] / VELOCITY_M_S_PER_MODEL_UNIT,
+                        tangent_si[:, 4:6]), dim=1) / self.std_x
+        y, yd = self.model.engine().jvp(xm, zm, float(dx), float(dy), fd)
+        log_n = self.log_populations(y)
+        n = self.linear(log_n)
+        inside = (log_n >= -LOG10_POPULATION_LIMIT) & (log_n <= LOG10_POPULATION_LIMIT)
+        return n, n * LN10 * inside * self.std_y * yd
+
     def jvp_local(self, features, z_scale, dx, dy, feature_tangent):
-        """Forward-mode derivative along feature_tangent (ffno_fsdp_runtime.py:350-372): not provided by the HIP
-        engine, which has no tangent kernels; the request is refused rather than answered another way."""
-        raise NotImplementedError("JVP is not implemented by the single-GPU HIP backend")
+        """Forward-mode derivative of the linear populations along feature_tangent (6, nz, nx, ny), SI units
+        (ffno_fsdp_runtime.py:350-372): (nz, nx, ny, L)."""
+        x, z = self.inputs(features, z_scale)
+        td = np.asarray(feature_tangent, dtype=np.float32)
+        if tuple(td.shape) != tuple(np.asarray(features).shape):
+            raise ValueError(f"feature_tangent has shape {td.shape}; expected {np.asarray(features).shape}")
+        if not np.isfinite(td).all():
+            raise ValueError("FFNO JVP tangent contains NaN or Inf")
+        tangent = torch.from_numpy(np.ascontiguousarray(td)).to(self.device)[None]
+        _, nd = self.jvp_tensor(x, z, dx, dy, tangent)
+        if not torch.isfinite(nd).all():
+            raise RuntimeError("FFNO JVP returned NaN or Inf")
+        torch.cuda.synchronize(self.device)
+        self.call_count += 1
+        return self.canonical(nd)

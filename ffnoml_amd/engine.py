@@ -401,9 +401,10 @@ class Engine:
         self.last_stride = None
         self._modes = collections.OrderedDict()     # per-mode table sets shared by plans, least recently used first
 
-    def footprint_bytes(self, B, D, H, W, vjp=False):
+    def footprint_bytes(self, B, D, H, W, vjp=False, jvp=False):
         """HBM one request of this shape needs (for admission; the tables of a cached plan not included):
-        forward 3 activation-sized buffers, adjoint (L+1) saved inputs and 9 working buffers, plus the scratch."""
+        forward 3 activation-sized buffers, adjoint (L+1) saved inputs and 9 working buffers, JVP 16 (one block's
+        intermediates, its gates and four tangents), plus the scratch."""
         C, L, r = self.module.width, self.module.n_layers, self.module.freq_mix_rank
         unit = B * D * H * W * C * 4
         wf = W // 2 + 1
@@ -411,7 +412,7 @@ class Engine:
         chunk = max(1, min(B * D, H_CHUNK_BYTES // plane)) * plane
         modes = int(math.floor(self.module.z_reference_length / self.module.z_min_wavelength)) + 1
         v1 = B * 2 * modes * H * W * C * 4
-        units = (L + 1) + WORKING_UNITS + GATE_UNITS if vjp else 3
+        units = (L + 1) + WORKING_UNITS + GATE_UNITS if vjp else (16 if jvp else 3)
         return unit * units + 2 * chunk + (2 if vjp else 1) * v1 + B * D * H * W * 4 * 32 + (64 << 20)
 
     def _param_version(self):
@@ -743,6 +744,103 @@ class Engine:
         _lib.call("ffno_head", rows, prev, _p(h_last), _p(Wt.p1), _p(Wt.p2), _p(Wt.p2_bias), _p(x),
                   ctypes.byref(Wt.head), _p(y), st)
         return y
+
+    # ------------------------------------------------------------------------------------------------ tangent
+    def _gn_tangent(self, plan, y, yd, rec, gamma, part2):
+        """(tan_b, tan_c) [B, C] of a GroupNorm's tangent n_dot = a yd + tan_b y + tan_c (a the forward affine)."""
+        _, _, stats = rec
+        tb = torch.empty(plan.B, plan.C, device=y.device)
+        tc = torch.empty(plan.B, plan.C, device=y.device)
+        _lib.call("ffno_gn_tangent_stats", plan.rows, _p(y), _p(yd), _p(stats), _p(plan.depth_w), _p(part2),
+                  self._stream())
+        _lib.call("ffno_gn_tangent_finalize", plan.B, plan.nwg, plan.C, _p(part2), _p(stats), _p(gamma), _p(tb),
+                  _p(tc), self._stream())
+        return tb, tc
+
+    def _assemble_t(self, plan, out, t, td, rec, tan, act=0, scale=None, base_dot=None):
+        """out = base_dot + scale * act'(a t + b) * (a td + tan_b t + tan_c): the tangent of a GroupNorm output
+        (GELU after it with act) scaled and added to a residual's tangent; elementwise, out may alias td or
+        base_dot."""
+        a, b, _ = rec
+        tb, tc = tan
+        at = _lib.FfnoAssembleT(_p(base_dot), _p(t), _p(td), _p(a), _p(b), _p(tb), _p(tc), _p(scale), act)
+        _lib.call("ffno_assemble_t", plan.rows, at, _p(out), self._stream())
+
+    def jvp(self, x, z, dx, dy, xdot, plan=None):
+        """(y, y_dot): the forward and its directional derivative along xdot [B, 6, D, H, W] (normalised input
+        features; the heights carry no tangent), as the reference's forward-AD JVP (ffno_fsdp_runtime.py:350-372)
+        computes it. Block by block: the block's forward keeping its intermediates, then its tangent (gate, MLP,
+        fusion and head tangent kernels; the spectral transforms and mixes are linear and run on the tangent as they
+        are; every GroupNorm's tangent from its two tangent sums). Memory: about 16 activation-sized buffers whatever
+        the depth."""
+        dev, plan, x, bz = self._prepare(x, z, dx, dy, plan)
+        xdot = xdot.contiguous().float()
+        if tuple(xdot.shape) != tuple(x.shape):
+            raise ValueError(f"tangent shape {tuple(xdot.shape)} differs from the input's {tuple(x.shape)}")
+        Wt = self.weights
+        B, D, H, W, C = plan.B, plan.D, plan.H, plan.W, plan.C
+        N = B * plan.voxels
+        st = self._stream()
+        rows = plan.rows
+        f32 = dict(device=dev, dtype=torch.float32)
+        bk = {k: torch.empty(N, C, **f32) for k in ("h", "s", "v", "sp", "vp", "fp", "pp", "mp", "x1")}
+        bk["gates"] = torch.empty(N, 2 * C, **f32)
+        hb = torch.empty(N, C, **f32)
+        hd, t1, t2, t3 = (torch.empty(N, C, **f32) for _ in range(4))
+        scr = self._scratch(plan, dev)
+        part2 = torch.empty(B, plan.nwg, GROUPS, 2, **f32)
+        # lift: h0 = gelu(GN(W x)); its tangent from W xdot
+        lift = bk["s"]
+        _lib.call("ffno_lift", rows, _p(x), _p(Wt.lift), _p(lift), FfnoStatsOut(_p(plan.depth_w), _p(scr["part"])),
+                  st)
+        rec = self._gn(plan, scr["part"], Wt.lift_gn, dev)
+        _lib.call("ffno_lift_t", rows, _p(xdot), _p(Wt.lift), _p(t1), st)
+        self._assemble_t(plan, hd, lift, t1, rec, self._gn_tangent(plan, lift, t1, rec, Wt.lift_gn[0], part2), act=1)
+        prev = _asm(None, lift, rec[0], rec[1], None, 1)
+        for i in range(len(Wt.blocks)):
+            blk = Wt.blocks[i]
+            tables = plan.tables(i)
+            gn = self._block_keep(plan, i, prev, bk["h"], bk, scr, bz, hb)
+            # gates and the two branches' transforms and post networks
+            _lib.call("ffno_gates_t", rows, _p(bk["h"]), _p(hd), _p(blk["spec.g1"]), _p(blk["spec.g2"]),
+                      _p(blk["vertical.g1"]), _p(blk["vertical.g2"]), _p(t1), _p(t2), st)
+            self.horizontal(plan, t1, t1, i, scr, tables)
+            _lib.call("ffno_mlp_t", rows, _asm(bk["s"]), _p(t1), _p(blk["spec.p1"]), _p(blk["spec.p2"]), _p(t1), st)
+            tan_s = self._gn_tangent(plan, bk["sp"], t1, gn["s"], blk["norm_spec"][0], part2)
+            self.vertical(plan, t2, scr["v1"], t2, i, tables)
+            _lib.call("ffno_mlp_t", rows, _asm(bk["v"]), _p(t2), _p(blk["vertical.p1"]), _p(blk["vertical.p2"]),
+                      _p(t2), st)
+            tan_v = self._gn_tangent(plan, bk["vp"], t2, gn["v"], blk["norm_vert"][0], part2)
+            # s = gelu(GN_s(sp)), v = gelu(GN_v(vp)) tangents, then the fusion
+            self._assemble_t(plan, t1, bk["sp"], t1, gn["s"], tan_s, act=1)
+            self._assemble_t(plan, t2, bk["vp"], t2, gn["v"], tan_v, act=1)
+            sa, sb_, _ = gn["s"]
+            va, vb, _ = gn["v"]
+            _lib.call("ffno_fuse_t", rows, _p(bk["h"]), _p(hd), _p(bk["sp"]), _p(bk["vp"]), _p(sa), _p(sb_), _p(va),
+                      _p(vb), _p(t1), _p(t2), _p(bk["gates"]), _p(blk["bg1"]), _p(blk["bg2"]), _p(blk["f1"]),
+                      _p(blk["f2"]), _p(t3), st)
+            tan_f = self._gn_tangent(plan, bk["fp"], t3, gn["f"], blk["norm_fuse"][0], part2)
+            # x1 = h + res_fused gelu(GN_f(fp)) -> t1; pp tangent -> t2
+            self._assemble_t(plan, t1, bk["fp"], t3, gn["f"], tan_f, act=1, scale=blk["res_fused"], base_dot=hd)
+            _lib.call("ffno_mlp_t", rows, _asm(bk["x1"]), _p(t1), _p(blk["pw1"]), _p(blk["pw2"]), _p(t2), st)
+            tan_p = self._gn_tangent(plan, bk["pp"], t2, gn["p"], blk["norm_pw"][0], part2)
+            # x2 = x1 + res_pw GN_p(pp) -> t1; mp tangent -> t3
+            self._assemble_t(plan, t1, bk["pp"], t2, gn["p"], tan_p, scale=blk["res_pw"], base_dot=t1)
+            pa, pb, _ = gn["p"]
+            _lib.call("ffno_mlp_t", rows, _asm(bk["x1"], bk["pp"], pa, pb, blk["res_pw"]), _p(t1), _p(blk["m1"]),
+                      _p(blk["m2"]), _p(t3), st)
+            tan_m = self._gn_tangent(plan, bk["mp"], t3, gn["m"], blk["norm_mlp"][0], part2)
+            # the next block's input: x2 + res_mlp GN_m(mp)
+            self._assemble_t(plan, hd, bk["mp"], t3, gn["m"], tan_m, scale=blk["res_mlp"], base_dot=t1)
+            ma, mb, _ = gn["m"]
+            prev = _asm(hb, bk["mp"], ma, mb, blk["res_mlp"])
+        y = torch.empty(B, self.module.out_channels, D, H, W, **f32)
+        yd = torch.empty_like(y)
+        _lib.call("ffno_head", rows, prev, None, _p(Wt.p1), _p(Wt.p2), _p(Wt.p2_bias), _p(x), ctypes.byref(Wt.head),
+                  _p(y), st)
+        _lib.call("ffno_head_t", rows, prev, _p(hd), _p(Wt.p1), _p(Wt.p2), _p(Wt.p2_bias), _p(x), _p(xdot),
+                  ctypes.byref(Wt.head), _p(yd), st)
+        return y, yd
 
     def forward_saved(self, x, z, dx, dy, plan=None, stride=None, want_z=True):
         """(y, saved): the forward that keeps what the adjoint needs: every intermediate (stride 0, or by default

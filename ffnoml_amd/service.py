@@ -11,7 +11,8 @@ The READY line's world field carries the batcher's execution slots (at least 2):
 world >= 2 for its full-shard service and the tag it accepts for predict/VJP (FSDPService.jl:175-182). Several clients
 may connect at once; their requests meet in the batcher. A failed request is answered "ERROR <message>" and the
 connection carries on (the reference's batched service answers unsupported operations the same way,
-ffno_batched_service.py:72-77); JVP is answered that way.
+ffno_batched_service.py:72-77). A JVP answers the population tangent (nz, nx, ny, levels), as the reference's
+service does (ffno_fsdp_service.py:281-297).
 """
 import argparse
 import importlib
@@ -127,9 +128,7 @@ class Service:
                 if op == "VJP":
                     extra = read_array(conn, (nz, nx, ny, levels))
                 elif op == "JVP":
-                    read_array(conn, (6, nz, nx, ny))
-                    conn.sendall(b"ERROR JVP is not available in the single-GPU HIP service\n")
-                    continue
+                    extra = read_array(conn, (6, nz, nx, ny))
                 try:
                     backend = self.batcher.backends[name]
                     if levels != len(backend.level_names):
@@ -141,6 +140,8 @@ class Service:
                 conn.sendall(f"OK {op} {nz} {nx} {ny} {levels}\n".encode())
                 if op == "PREDICT":
                     write_array(conn, result["populations"])
+                elif op == "JVP":
+                    write_array(conn, result["population_tangent"])
                 else:
                     write_array(conn, result["features"])
                     write_array(conn, result["z_scale"])

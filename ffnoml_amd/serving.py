@@ -136,7 +136,7 @@ class ContinuousBatcher:
 
     # ------------------------------------------------------------------------------------------------ public
     def submit(self, op, model, features, z, dx, dy, cotangent=None):
-        if op not in ("PREDICT", "VJP"):
+        if op not in ("PREDICT", "VJP", "JVP"):
             raise ValueError(f"unsupported operation {op}")
         backend = self.backends[model]
         features = np.asarray(features, np.float32)
@@ -153,8 +153,14 @@ class ContinuousBatcher:
                 raise ValueError("population_cotangent has the wrong shape")
             if not np.isfinite(cotangent).all():
                 raise ValueError("FFNO VJP cotangent contains NaN or Inf")
+        if op == "JVP":
+            cotangent = np.asarray(cotangent, np.float32)          # the feature tangent rides in the same slot
+            if tuple(cotangent.shape) != tuple(features.shape):
+                raise ValueError("feature_tangent must have shape (6,nz,nx,ny)")
+            if not np.isfinite(cotangent).all():
+                raise ValueError("FFNO JVP tangent contains NaN or Inf")
         req = Request(op, model, features, z, dx, dy, cotangent)
-        need = backend.model.engine().footprint_bytes(1, *z.shape, vjp=op == "VJP")
+        need = backend.model.engine().footprint_bytes(1, *z.shape, vjp=op == "VJP", jvp=op == "JVP")
         if need > self.budget:
             req.future.set_exception(MemoryError(f"request needs {need / 2**30:.1f} GiB, more than the "
                                                  f"{self.budget / 2**30:.1f} GiB budget"))
@@ -181,7 +187,7 @@ class ContinuousBatcher:
     def _footprint(self, reqs):
         eng = self.backends[reqs[0].model].model.engine()
         nz, nx, ny = reqs[0].z.shape
-        return eng.footprint_bytes(len(reqs), nz, nx, ny, vjp=reqs[0].op == "VJP")
+        return eng.footprint_bytes(len(reqs), nz, nx, ny, vjp=reqs[0].op == "VJP", jvp=reqs[0].op == "JVP")
 
     def _schedule(self):
         while True:
@@ -269,6 +275,13 @@ class ContinuousBatcher:
                                    f"tile {tuple(r0.z.shape)})")
             out = n.permute(0, 2, 3, 4, 1).float().cpu().numpy()
             return [{"populations": out[i]} for i in range(len(batch))], capture
+        if r0.op == "JVP":
+            td = torch.from_numpy(np.stack([r.cotangent for r in batch])).to(backend.device)
+            _, nd = backend.jvp_tensor(x, z, r0.dx, r0.dy, td)
+            if not torch.isfinite(nd).all():
+                raise RuntimeError(f"FFNO JVP returned NaN or Inf (batch of {len(batch)}, tile {tuple(r0.z.shape)})")
+            out = nd.permute(0, 2, 3, 4, 1).float().cpu().numpy()
+            return [{"population_tangent": out[i]} for i in range(len(batch))], None
         cot = torch.from_numpy(np.stack([r.cotangent for r in batch])).to(backend.device).permute(0, 4, 1, 2, 3)
         n, fb, zb = backend.vjp_tensor(x, z, r0.dx, r0.dy, cot.contiguous())
         if not torch.isfinite(fb).all() or not torch.isfinite(zb).all():

@@ -233,6 +233,67 @@ int ffno_dot_reduce(FfnoRows rows, int D, const float* a, int delta, const float
 int ffno_lift_bwd(FfnoRows rows, const float* ubar, FfnoGnBack gn, const float* w_lift, float* xbar_in,
                   hipStream_t stream);
 
+/* ------------------------------------------------------------------------------------------------------------
+ * Tangent (forward-mode JVP along a tangent of the input features). The reference computes it with torch forward-AD
+ * duals through its forward with checkpointing bypassed (ffno_fsdp_runtime.py:61-73, :350-372). Here the primal
+ * forward of each block runs first (the kernels above, keeping the block's intermediates), then these kernels carry
+ * the tangent through the same block; the spectral transforms and the mix are linear and run on the tangent as they
+ * are. The heights carry no tangent (jvp_local perturbs the features only).
+ * ------------------------------------------------------------------------------------------------------------ */
+
+/* Tangent of "base + scale * act(a t + b)" where t is a depth-weighted GroupNorm's input: with the GroupNorm
+ * tangent n_dot = a t_dot + tan_b t + tan_c (ffno_gn_tangent_finalize), out = base_dot + scale * act'(a t + b) *
+ * n_dot, elementwise. */
+typedef struct {
+  const float* base_dot;  /* [N, C] or NULL */
+  const float* t;         /* primal GroupNorm input [N, C] */
+  const float* t_dot;     /* its tangent [N, C] */
+  const float* aff_a;     /* [B][C] the forward affine (gamma rstd) */
+  const float* aff_b;     /* [B][C] */
+  const float* tan_b;     /* [B][C] */
+  const float* tan_c;     /* [B][C] */
+  const float* scale;     /* device scalar or NULL (1) */
+  int act_gelu;
+} FfnoAssembleT;
+
+/* Tangent sums of a GroupNorm (models/ffno_model.py:77-95): partials [B][nwg][8][2] of S1 = sum w yd and
+ * S2 = sum w (y - mean) yd, then per channel tan_b = -gamma rstd^3 S2/W and
+ * tan_c = -gamma rstd S1/W + gamma rstd^3 (S2/W) mean (stats from ffno_gn_finalize, gamma [C]). */
+int ffno_gn_tangent_stats(FfnoRows rows, const float* y, const float* yd, const float* stats, const float* depth_w,
+                          float* partials, hipStream_t stream);
+int ffno_gn_tangent_finalize(int B, int nwg, int C, const float* partials, const float* stats, const float* gamma,
+                             float* tan_b, float* tan_c, hipStream_t stream);
+int ffno_assemble_t(FfnoRows rows, FfnoAssembleT a, float* out, hipStream_t stream);
+
+/* Lift tangent (models/ffno_model.py:782-783): yd = W_lift xd_in, xd_in channel-first [B, 6, DHW]. */
+int ffno_lift_t(FfnoRows rows, const float* xd_in, const float* w_lift, float* yd, hipStream_t stream);
+
+/* Input-gate tangents (models/ffno_model.py:266, :439-440): from the block input h and its tangent hd,
+ * xsd = hd g + h g (1 - g) a_dot with g = sigmoid(a), a = G2 gelu(G1 h), a_dot = G2 (gelu'(G1 h) * G1 hd); xvd the
+ * same with V1, V2. */
+int ffno_gates_t(FfnoRows rows, const float* h, const float* hd, const uint16_t* g1, const uint16_t* g2,
+                 const uint16_t* v1, const uint16_t* v2, float* xsd, float* xvd, hipStream_t stream);
+
+/* Tangent of y = W2 gelu(W1 x) (the post networks, pw and mlp: models/ffno_model.py:221-225, :369-373, :545-551):
+ * x assembled as the forward assembles it, xd given: yd = W2 (gelu'(W1 x) * W1 xd). */
+int ffno_mlp_t(FfnoRows rows, FfnoAssemble in, const float* xd, const uint16_t* w1, const uint16_t* w2, float* yd,
+               hipStream_t stream);
+
+/* Branch-fusion tangent (models/ffno_model.py:524-534, :541-543) from h, hd, the GroupNorm inputs sp, vp with their
+ * forward affines, the tangents sd, vd of s = gelu(GN_s(sp)) and v = gelu(GN_v(vp)) and the gates the forward stored
+ * [N, 2C]: fpd = F2 (gelu'(F1 [sm, vm]) * F1 [smd, vmd]) + 0.5 (smd + vmd). */
+int ffno_fuse_t(FfnoRows rows, const float* h, const float* hd, const float* sp, const float* vp, const float* s_a,
+                const float* s_b, const float* v_a, const float* v_b, const float* sd, const float* vd,
+                const float* gates, const uint16_t* bg1, const uint16_t* bg2, const uint16_t* f1, const uint16_t* f2,
+                float* fpd, hipStream_t stream);
+
+/* Head tangent (models/ffno_model.py:730-749, :790-798): normalised log10 n tangent yd [B, Cout, DHW] from the head
+ * input (assembled as the forward does), its tangent xd, the model input x_in and its tangent xd_in (T, ne and rho
+ * enter the LTE weights and the total directly). */
+int ffno_head_t(FfnoRows rows, FfnoAssemble in, const float* xd, const uint16_t* p1, const uint16_t* p2,
+                const float* p2_bias, const float* x_in, const float* xd_in, const FfnoHeadDesc* desc, float* yd,
+                hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

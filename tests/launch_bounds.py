@@ -310,6 +310,80 @@ class Checker:
         self.span(n, "xbar_in", xbar_in, rows.B * 6 * rows.voxels * F32)
 
 
+    # ---- tangent (JVP) entry points
+    def ffno_gn_tangent_stats(self, n, rows, y, yd, stats, depth_w, part, stream):
+        self.act(n, "y", y, rows)
+        self.act(n, "yd", yd, rows)
+        self.span(n, "stats", stats, rows.B * 8 * 3 * F32)
+        self.span(n, "depth_w", depth_w, rows.B * rows.voxels * F32)
+        self.span(n, "partials", part, rows.B * rows.nwg * 8 * 2 * F32)
+
+    def ffno_gn_tangent_finalize(self, n, B, nwg, C, part, stats, gamma, tb, tc, stream):
+        self.span(n, "partials", part, B * nwg * 8 * 2 * F32)
+        self.span(n, "stats", stats, B * 8 * 3 * F32)
+        self.span(n, "gamma", gamma, C * F32)
+        self.span(n, "tan_b", tb, B * C * F32)
+        self.span(n, "tan_c", tc, B * C * F32)
+
+    def ffno_assemble_t(self, n, rows, a, out, stream):
+        assert rows.C % 4 == 0
+        self.act(n, "base_dot", a.base_dot, rows, optional=True)
+        self.act(n, "t", a.t, rows)
+        self.act(n, "t_dot", a.t_dot, rows)
+        for what, p in (("aff_a", a.aff_a), ("aff_b", a.aff_b), ("tan_b", a.tan_b), ("tan_c", a.tan_c)):
+            self.span(n, what, p, rows.B * rows.C * F32)
+        self.span(n, "scale", a.scale, F32, optional=True)
+        self.act(n, "out", out, rows)
+
+    def ffno_lift_t(self, n, rows, xd_in, w, yd, stream):
+        self.span(n, "xd_in", xd_in, rows.B * 6 * rows.voxels * F32)
+        self.span(n, "w_lift", w, rows.C * 6 * F32)
+        self.act(n, "yd", yd, rows)
+
+    def ffno_gates_t(self, n, rows, h, hd, g1, g2, v1, v2, xsd, xvd, stream):
+        C = rows.C
+        for what, p in (("h", h), ("hd", hd), ("xsd", xsd), ("xvd", xvd)):
+            self.act(n, what, p, rows)
+        for what, w in (("g1", g1), ("g2", g2), ("v1", v1), ("v2", v2)):
+            self.weight(n, what, w, C, C)
+
+    def ffno_mlp_t(self, n, rows, a, xd, w1, w2, yd, stream):
+        C = rows.C
+        self.asm(n, a, rows)
+        self.act(n, "xd", xd, rows)
+        self.weight(n, "w1", w1, 2 * C, C)
+        self.weight(n, "w2", w2, C, 2 * C)
+        self.act(n, "yd", yd, rows)
+
+    def ffno_fuse_t(self, n, rows, h, hd, sp, vp, sa, sb, va, vb, sd, vd, gates, bg1, bg2, f1, f2, fpd, stream):
+        C = rows.C
+        KP = round_up(C, 32)
+        for what, p in (("h", h), ("hd", hd), ("sp", sp), ("vp", vp), ("sd", sd), ("vd", vd), ("fpd", fpd)):
+            self.act(n, what, p, rows)
+        for what, p in (("s_a", sa), ("s_b", sb), ("v_a", va), ("v_b", vb)):
+            self.span(n, what, p, rows.B * C * F32)
+        self.span(n, "gates", gates, rows.B * rows.voxels * 2 * C * F32)
+        self.span(n, "bg1", bg1, 2 * C * 3 * KP * BF)
+        self.weight(n, "bg2", bg2, 2 * C, C)
+        self.span(n, "f1", f1, 2 * 2 * C * 2 * KP * BF)
+        self.weight(n, "f2", f2, C, 2 * C)
+
+    def ffno_head_t(self, n, rows, a, xd, p1, p2, bias, x_in, xd_in, desc, yd, stream):
+        d = desc._obj
+        n_head = sum(d.levels[s] + 1 for s in range(d.n_species))
+        n_out = sum(d.levels[s] for s in range(d.n_species))
+        nth = (n_head + 15) // 16
+        C = rows.C
+        self.asm(n, a, rows)
+        self.act(n, "xd", xd, rows)
+        self.weight(n, "p1", p1, 2 * C, C)
+        self.span(n, "p2", p2, 2 * nth * 16 * 2 * C * BF)
+        self.span(n, "p2_bias", bias, nth * 16 * F32)
+        self.span(n, "x_in", x_in, rows.B * 6 * rows.voxels * F32)
+        self.span(n, "xd_in", xd_in, rows.B * 6 * rows.voxels * F32)
+        self.span(n, "yd", yd, rows.B * n_out * rows.voxels * F32)
+
+
 def dry_engine(model):
     """An Engine over `model` (kept on the host) whose tensors live on the meta device and whose launches go to a
     Checker instead of the library. Returns (engine, checker, restore)."""
@@ -338,8 +412,8 @@ def dry_engine(model):
 
 
 def check_shape(model, B, D, H, W, stride=1, want_z=True, column=None, two_pass=True):
-    """Run forward, the saving forward and the adjoint of `model` at [B, D, H, W] through the checker (with the
-    two-pass fusion adjoint where the stride allows it); returns the number of launches checked."""
+    """Run forward, the saving forward, the adjoint and the tangent (JVP) pass of `model` at [B, D, H, W] through the
+    checker (with the two-pass fusion adjoint where the stride allows it); returns the number of launches checked."""
     eng, chk, restore = dry_engine(model)
     eng.two_pass_fuse = two_pass
     try:
@@ -354,6 +428,7 @@ def check_shape(model, B, D, H, W, stride=1, want_z=True, column=None, two_pass=
         y, saved = eng.forward_saved(x, z, 48e3, 48e3, plan=plan, stride=stride)
         ybar = torch.empty_like(y)
         eng.vjp(x, z, 48e3, 48e3, ybar, saved, want_z=want_z, plan=plan)
+        eng.jvp(x, z, 48e3, 48e3, torch.empty_like(x), plan=plan)
     finally:
         restore()
     return len(chk.calls)

@@ -1,6 +1,6 @@
 """The socket service's wire protocol (the reference's v2: ffno_fsdp_service.py:22-45, :156-343) on the CPU, with a
 stand-in batcher, as the reference's Julia suite tests its client against a stand-in server: handshake, Fortran-order
-arrays both ways, MEMORY, errors that answer without killing the service, JVP refused, SHUTDOWN."""
+arrays both ways, MEMORY, errors that answer without killing the service, JVP, SHUTDOWN."""
 import socket
 import threading
 from concurrent.futures import Future
@@ -33,6 +33,8 @@ class StandInBatcher:
         if op == "PREDICT":
             pops = np.stack([features[0] * (i + 1) + z for i in range(3)], -1)
             f.set_result({"populations": pops})
+        elif op == "JVP":
+            f.set_result({"population_tangent": np.stack([cotangent[i] + z for i in range(3)], -1)})
         else:
             f.set_result({"features": features * 2.0, "z_scale": z + cotangent.sum(-1)})
         return f
@@ -95,12 +97,17 @@ def test_errors_answer_and_the_connection_carries_on(service):
     write_array(s, np.zeros((nz, nx, ny), np.float32))
     line = r.readline().decode()
     assert line.startswith("ERROR ValueError") and "positive" in line
-    # JVP is refused, its arrays consumed, the stream stays in step
+    # JVP: features, z, then the feature tangent (6, nz, nx, ny); answered with the population tangent
+    # (nz, nx, ny, levels), as the reference's service does (ffno_fsdp_service.py:281-297)
+    tangent = np.arange(6 * nz * nx * ny, dtype=np.float32).reshape(6, nz, nx, ny)
+    zz = np.full((nz, nx, ny), 0.5, np.float32)
     s.sendall(f"JVP H {nz} {nx} {ny} 3 1.0 1.0\n".encode())
     write_array(s, bad + 1)
-    write_array(s, np.zeros((nz, nx, ny), np.float32))
-    write_array(s, bad)
-    assert r.readline().decode().startswith("ERROR JVP")
+    write_array(s, zz)
+    write_array(s, tangent)
+    assert r.readline().decode().split() == ["OK", "JVP", str(nz), str(nx), str(ny), "3"]
+    got = read_array(s, (nz, nx, ny, 3))
+    assert np.array_equal(got, np.stack([tangent[i] + zz for i in range(3)], -1))
     good = bad + 1
     s.sendall(f"PREDICT H {nz} {nx} {ny} 3 1.0 1.0\n".encode())
     write_array(s, good)
