@@ -45,6 +45,12 @@ class FfnoTwGemm(ctypes.Structure):
                 ("y_m_hi", c_i64), ("y_m_lo", c_i64), ("accumulate", ctypes.c_int), ("a_l3", c_ptr)]
 
 
+class FfnoColGemm(ctypes.Structure):
+    _fields_ = [("a_hi", c_ptr), ("a_lo", c_ptr), ("a_l3", c_ptr), ("m", ctypes.c_int), ("k", ctypes.c_int),
+                ("m_pad", ctypes.c_int), ("k_pad", ctypes.c_int), ("B", ctypes.c_int), ("hw", c_i64),
+                ("C", ctypes.c_int), ("x", c_ptr), ("y", c_ptr)]
+
+
 class FfnoMix(ctypes.Structure):
     _fields_ = [("re", c_ptr), ("im", c_ptr), ("C", ctypes.c_int), ("rows", ctypes.c_int), ("modes", ctypes.c_int),
                 ("rank", ctypes.c_int), ("mode_stride", c_i64), ("r_split", c_i64), ("r_hi", c_i64),
@@ -70,6 +76,7 @@ SIGNATURES = {
                   c_ptr],
     "ffno_tw_gemm": [ctypes.POINTER(FfnoTwGemm), c_ptr],
     "ffno_mix": [ctypes.POINTER(FfnoMix), c_ptr],
+    "ffno_col_gemm": [ctypes.POINTER(FfnoColGemm), c_ptr],
     "ffno_gn_grad_stats": [FfnoRows, FfnoGnBack, c_ptr, c_ptr, c_ptr],
     "ffno_gn_grad_finalize": [ctypes.c_int, ctypes.c_int, c_ptr, c_ptr, c_ptr],
     "ffno_mlp_bwd": [FfnoRows, c_ptr, FfnoGnBack, FfnoAssemble, c_ptr, c_ptr, c_ptr, ctypes.c_int, c_ptr, c_ptr],

@@ -485,6 +485,73 @@ __global__ __launch_bounds__(256) void k_tw_gemm(FfnoTwGemm g) {
   }
 }
 
+// ---------------------------------------------------------------------------------------------------------------
+// per-column vertical transform: every height column has its own matrix A_j [m x k] (the Filon weights of its own
+// heights, or the synthesis at them); Y_j [m][C] = A_j X_j with X [B][k][HW][C], Y [B][m][HW][C]. A wave takes one
+// column and 64 of its channels; its A_j fragments come from global memory (each column's matrix is read once per
+// 64-row block).
+// ---------------------------------------------------------------------------------------------------------------
+template <bool THREE>
+__global__ __launch_bounds__(256) void k_col_gemm(FfnoColGemm g) {
+  Lane L;
+  const int nchunk = (g.C + 63) / 64;
+  const int64_t cols = (int64_t)g.B * g.hw;
+  const int64_t tasks = cols * nchunk;
+  const int nmb = (g.m_pad + 63) / 64;
+  const int64_t rs = (int64_t)g.hw * g.C;               // row stride of X and Y
+  const int64_t asz = (int64_t)g.m_pad * g.k_pad;
+  for (int64_t task = (int64_t)blockIdx.x * 4 + L.wave; task < tasks; task += (int64_t)gridDim.x * 4) {
+    const int64_t j = task / nchunk;
+    const int b = (int)(j / g.hw);
+    const int64_t hw = j % g.hw;
+    const int n0 = (int)(task % nchunk) * 64 + 4 * L.v;
+    const bool nok = n0 < g.C;
+    const float* xb = g.x + ((int64_t)b * g.k * g.hw + hw) * g.C + n0;
+    float* yb = g.y + ((int64_t)b * g.m * g.hw + hw) * g.C + n0;
+    const bf16* ah = reinterpret_cast<const bf16*>(g.a_hi) + j * asz;
+    const bf16* am = reinterpret_cast<const bf16*>(g.a_lo) + j * asz;
+    const bf16* al = THREE ? reinterpret_cast<const bf16*>(g.a_l3) + j * asz : ah;
+    for (int mb = 0; mb < nmb; ++mb) {
+      f32x4 acc[4][4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[a][i] = zero4();
+      for (int s = 0; s < g.k_pad / 32; ++s) {
+        f32x4 xv[8];
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) {
+          const int k = 32 * s + 8 * L.q + jj;
+          xv[jj] = (nok && k < g.k) ? *(const f32x4*)(xb + k * rs) : zero4();
+        }
+        bf16x8 bh[4], bm[4], bl[4];
+        split_cols<THREE>(xv, bh, bm, bl);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          const int mt = mb * 4 + a;
+          if (mt * 16 >= g.m_pad) break;
+          const int64_t ao = (int64_t)(mt * 16 + L.v) * g.k_pad + 32 * s + 8 * L.q;
+          const bf16x8 fh = *(const bf16x8*)(ah + ao);
+          const bf16x8 fm = *(const bf16x8*)(am + ao);
+          const bf16x8 fl = THREE ? *(const bf16x8*)(al + ao) : fh;
+          tw_mac<THREE>(fh, fm, fl, bh, bm, bl, acc[a]);
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const int mt = mb * 4 + a;
+        if (mt * 16 >= g.m_pad) break;
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+          const int m = mt * 16 + 4 * L.q + jj;
+          if (m < g.m && nok)
+            *(f32x4*)(yb + m * rs) = f32x4{acc[a][0][jj], acc[a][1][jj], acc[a][2][jj], acc[a][3][jj]};
+        }
+      }
+    }
+  }
+}
+
 // twiddle matrices of at most 128 x 128: A staged in LDS once per persistent workgroup (rows padded by 16 bytes so
 // the lanes' 16-byte reads fall in distinct banks), the row offsets of both index maps tabulated in LDS, all MT
 // m-tiles of a 64-column task accumulated in registers (the task's input is read once), and the next k-step's input
@@ -820,6 +887,21 @@ int ffno_tw_gemm(const FfnoTwGemm* g, hipStream_t stream) {
   else
     k_tw_gemm<false><<<grid, 256, 0, stream>>>(*g);
   return check_launch("ffno_tw_gemm");
+}
+
+int ffno_col_gemm(const FfnoColGemm* g, hipStream_t stream) {
+  if (!g || !g->a_hi || !g->a_lo || !g->x || !g->y) return fail("ffno_col_gemm: null pointer");
+  if (g->m <= 0 || g->k <= 0 || g->B <= 0 || g->hw <= 0 || g->C <= 0 || g->C % 4 || g->m_pad % 16 ||
+      g->k_pad % 32 || g->m_pad < g->m || g->k_pad < g->k)
+    return fail("ffno_col_gemm: bad sizes");
+  const int64_t tasks = (int64_t)g->B * g->hw * ((g->C + 63) / 64);
+  const int64_t blocks = (tasks + 3) / 4;
+  const int grid = (int)(blocks < 8192 ? blocks : 8192);
+  if (g->a_l3)
+    k_col_gemm<true><<<grid, 256, 0, stream>>>(*g);
+  else
+    k_col_gemm<false><<<grid, 256, 0, stream>>>(*g);
+  return check_launch("ffno_col_gemm");
 }
 
 int ffno_mix(const FfnoMix* m, hipStream_t stream) {

@@ -7,6 +7,7 @@ models/ffno_model.py:800-832 (eval mode, no dropout).
 """
 import collections
 import ctypes
+import hashlib
 import math
 import threading
 
@@ -256,7 +257,7 @@ class Plan:
     matrices (split-bf16), per-mode gates and low-rank tables of every block (cached when small, else rebuilt per
     block on use), depth weights, the chunking of the horizontal transforms."""
 
-    def __init__(self, engine, B, D, H, W, column, span, dx, dy):
+    def __init__(self, engine, B, D, H, W, column, span, dx, dy, columns=None):
         m = engine.module
         dev = engine.device
         C, r = m.width, m.freq_mix_rank
@@ -264,28 +265,47 @@ class Plan:
         self.n_layers = m.n_layers
         self.wf = W // 2 + 1
         self.voxels = D * H * W
-        column = np.asarray(column, np.float64)
-        # depth weights (models/ffno_model.py:28-34, :67-74) of the shared column, in float32 as the reference has
-        # them, broadcast to every voxel
-        zc = torch.from_numpy(column).float()
-        if D == 1:
-            tw = torch.ones(1)
-        else:
-            half = 0.5 * (zc[1:] - zc[:-1]).abs()
-            tw = F.pad(half, (0, 1)) + F.pad(half, (1, 0))
-        w = tw / tw.mean().clamp_min(1e-12)
-        self.depth_w = w.to(dev)[None, :, None].expand(B, D, H * W).reshape(B, -1).contiguous()
+        # columns: [B*H*W, D] float64 shifted heights of every column on the device (the reference's per-column
+        # path); else the one shared column
+        self.percol = columns is not None
         self.length, k, factor = T.vertical_box(span, m.z_reference_length, m.z_min_wavelength,
                                                 int(math.floor(m.z_reference_length / m.z_min_wavelength)) + 1)
         self.modes = len(k)
-        v_fwd, v_inv = T.vertical_matrices(column, k, factor, self.length)
         hm = T.horizontal_matrices(H, W)
-        self.tw = {name: T.split_bf16(a, dev, parts=3) for name, a in
-                   (("w_fwd", hm["w_fwd"]), ("h_fwd", hm["h_fwd"]), ("h_inv", hm["h_inv"]), ("w_inv", hm["w_inv"]),
-                    ("v_fwd", v_fwd), ("v_inv", v_inv))}
+        named = [("w_fwd", hm["w_fwd"]), ("h_fwd", hm["h_fwd"]), ("h_inv", hm["h_inv"]), ("w_inv", hm["w_inv"])]
+        if self.percol:
+            # depth weights (models/ffno_model.py:28-34, :67-74) of every column, in float32 as the reference has
+            # them
+            zc = columns.float()
+            if D == 1:
+                tw = torch.ones_like(zc)
+            else:
+                half = 0.5 * (zc[:, 1:] - zc[:, :-1]).abs()
+                tw = F.pad(half, (0, 1)) + F.pad(half, (1, 0))
+            w = tw / tw.mean(dim=1, keepdim=True).clamp_min(1e-12)                 # [B*HW, D]
+            self.depth_w = w.reshape(B, H * W, D).transpose(1, 2).reshape(B, -1).contiguous()
+            v_fwd, v_inv = T.vertical_matrices_columns(columns, k, factor, self.length)
+            self.colmats = dict(v_fwd=T.split_bf16_columns(v_fwd, parts=3), v_inv=T.split_bf16_columns(v_inv, parts=3))
+            del v_fwd, v_inv
+            self._columns = columns
+            self._col_adj = None
+            column = None
+        else:
+            column = np.asarray(column, np.float64)
+            # depth weights of the shared column, in float32 as the reference has them, broadcast to every voxel
+            zc = torch.from_numpy(column).float()
+            if D == 1:
+                tw = torch.ones(1)
+            else:
+                half = 0.5 * (zc[1:] - zc[:-1]).abs()
+                tw = F.pad(half, (0, 1)) + F.pad(half, (1, 0))
+            w = tw / tw.mean().clamp_min(1e-12)
+            self.depth_w = w.to(dev)[None, :, None].expand(B, D, H * W).reshape(B, -1).contiguous()
+            v_fwd, v_inv = T.vertical_matrices(column, k, factor, self.length)
+            named += [("v_fwd", v_fwd), ("v_inv", v_inv)]
+        self.tw = {name: T.split_bf16(a, dev, parts=3) for name, a in named}
         # the adjoint chain runs the same GEMMs with the transposed matrices in reverse order
-        self._mats = dict(w_fwd=hm["w_fwd"], h_fwd=hm["h_fwd"], h_inv=hm["h_inv"], w_inv=hm["w_inv"], v_fwd=v_fwd,
-                          v_inv=v_inv)
+        self._mats = dict(named)
         self._tw_adj = None
         self._column, self._k, self._factor = column, k, factor
         self._height = None
@@ -310,10 +330,24 @@ class Plan:
         """Per-mode tables that are rebuilt per block on use (not kept): one block's worth is alive at a time."""
         return sum(t.block_bytes for t in self._mode_tables.values() if not t.cache)
 
+    def column_adjoint(self):
+        """The transposed per-column matrices (two planes): V_inv^T [2M, D] and V_fwd^T [D, 2M] of every column."""
+        with self._lock:
+            if self._col_adj is None:
+                v_fwd, v_inv = T.vertical_matrices_columns(self._columns, self._k, self._factor, self.length)
+                self._col_adj = dict(v_inv_T=T.split_bf16_columns(v_inv.transpose(1, 2)),
+                                     v_fwd_T=T.split_bf16_columns(v_fwd.transpose(1, 2)))
+                del v_fwd, v_inv
+                _settle(self.device)
+            return self._col_adj
+
     def height_tables(self):
         """Height derivatives of the vertical transforms for the z-gradient: the three bands of the Filon weights'
         Jacobian, the synthesis' derivative, and for the taper (non-zero only for modes between 2 pi / (3 dz) and
         pi / dz) the transposed unscaled synthesis and the per-mode factor * d taper / d dz."""
+        if self.percol:
+            raise NotImplementedError("the height gradient on per-column height grids is not implemented; the "
+                                      "input-feature gradient is")
         with self._lock:
             if self._height is None:
                 bands = T.filon_height_bands(self._column, self._k)
@@ -474,20 +508,26 @@ class Engine:
         flags = torch.stack([(zs == zs[:, :, :1]).all() & (zs == zs[:1]).all(),
                              (zc.amax(dim=1) - zc.amin(dim=1)).max()])
         shared, span = flags.tolist()
-        if not shared:
-            raise NotImplementedError("per-column vertical grids are not implemented in the HIP engine; "
-                                      "every column must share its heights (after the per-column shift)")
-        # the shared column, shifted in float64 (the spacings then equal the float64 reference's bit for bit, so a
-        # tie between spacings at the taper's median resolves to the same interval)
-        z0 = zc[0, :, 0].double()
-        column = (z0 - z0.min()).cpu().numpy()
-        key = (B, D, H, W, float(dx), float(dy), float(span), column.tobytes())
+        columns = None
+        if shared:
+            # the shared column, shifted in float64 (the spacings then equal the float64 reference's bit for bit, so
+            # a tie between spacings at the taper's median resolves to the same interval)
+            z0 = zc[0, :, 0].double()
+            column = (z0 - z0.min()).cpu().numpy()
+            key = (B, D, H, W, float(dx), float(dy), float(span), column.tobytes())
+        else:
+            # the reference's per-column path (models/ffno_model.py:277-283): every column's own shifted heights
+            zd = z.reshape(B, D, H * W).double()
+            columns = (zd - zd.amin(dim=1, keepdim=True)).transpose(1, 2).reshape(B * H * W, D).contiguous()
+            column = None
+            key = (B, D, H, W, float(dx), float(dy), float(span),
+                   hashlib.sha1(columns.cpu().numpy().tobytes()).hexdigest())
         with self._lock:
             p = self._plans.get(key)
             if p is None:
                 if len(self._plans) > 64:
                     self._plans.pop(next(iter(self._plans)))
-                p = Plan(self, B, D, H, W, column, span, dx, dy)
+                p = Plan(self, B, D, H, W, column, span, dx, dy, columns=columns)
                 self._plans[key] = p
         return p
 
@@ -587,12 +627,37 @@ class Engine:
             # half spectrum -> real along W
             self._tw(m4, s1, oc, pc * H, C, (H, plane, wf * C), (hwc, W * C), (wf, H * wf * C, C), (big, 0, C))
 
+    def _colgemm(self, plan, mat, x, y):
+        hi, lo, m, k, mp, kp = mat[:6]
+        g = _lib.FfnoColGemm()
+        g.a_hi, g.a_lo = _p(hi), _p(lo)
+        g.a_l3 = _p(mat[6] if len(mat) > 6 else None)
+        g.m, g.k, g.m_pad, g.k_pad = m, k, mp, kp
+        g.B, g.hw, g.C = plan.B, plan.H * plan.W, plan.C
+        g.x, g.y = _p(x), _p(y)
+        _lib.call("ffno_col_gemm", ctypes.byref(g), self._stream())
+
     def vertical(self, plan, xv, v1, out, i, tables, adjoint=False):
         """Filon transform along D -> mix -> synthesis at the levels, /L folded in (models/ffno_model.py:270-305);
         with adjoint, its transpose. The spectrum after the mix stays in v1. out may be xv."""
         B, D, C, M = plan.B, plan.D, plan.C, plan.modes
         hwc = plan.H * plan.W * C
         big = 1 << 40
+        if plan.percol:
+            # every column its own analysis and synthesis matrices (the reference's per-column path)
+            if adjoint:
+                ca = plan.column_adjoint()
+                m1, m2 = ca["v_inv_T"], ca["v_fwd_T"]
+                wc = self.weights.adjoint()["blocks"][i]["vertical.wc"]
+                tables = plan.tables(i, adjoint=True)
+            else:
+                m1, m2 = plan.colmats["v_fwd"], plan.colmats["v_inv"]
+                wc = self.weights.blocks[i]["vertical.wc"]
+            self._colgemm(plan, m1, xv, v1)
+            hw = plan.H * plan.W
+            self._mix(plan, v1, v1[M * hwc:], B * hw, M, hwc, (hw, 2 * M * hwc, C), wc, tables["vertical"], adjoint)
+            self._colgemm(plan, m2, v1, out)
+            return
         if adjoint:
             ta = plan.adjoint_tables()
             m1, m2 = ta["v_inv_T"], ta["v_fwd_T"]

@@ -200,3 +200,70 @@ def vertical_height_terms(column, k, factor, length):
     # taper = (1 + cos(pi u)) / 2 with u = (3 k dz / pi) - 2: d taper / d dz = -(pi/2) sin(pi u) * 3 k / pi
     dtaper = np.where(inside, -0.5 * np.sin(math.pi * u) * 3.0 * k, 0.0)
     return v_dz, v0, factor * dtaper, i, float(np.sign(col[i + 1] - col[i]))
+
+
+# ----------------------------------------------------------------------------------------------------------------
+# per-column height grids (the reference's per-column path, models/ffno_model.py:277-283, :304-305): one analysis and
+# one synthesis matrix per column, built on the device in float64
+# ----------------------------------------------------------------------------------------------------------------
+def filon_weights_columns(z, k):
+    """(q_re, q_im) [n, D, M]: filon_weights for every column of z [n, D] (torch float64), k [M] torch float64."""
+    z0, z1 = z[:, :-1, None], z[:, 1:, None]
+    low = torch.minimum(z0, z1)
+    h = (z1 - z0).abs()
+    th = h * k
+    small = th.abs() < 1e-2
+    tt = torch.where(small, torch.ones_like(th), th)
+    c, s = torch.cos(tt), torch.sin(tt)
+    up_re = torch.where(small, 0.5 - th ** 2 / 8, (c + tt * s - 1) / tt ** 2)
+    up_im = torch.where(small, -th / 3 + th ** 3 / 30, (tt * c - s) / tt ** 2)
+    lo_re = torch.where(small, 0.5 - th ** 2 / 24, s / tt - up_re)
+    lo_im = torch.where(small, -th / 6 + th ** 3 / 120, (c - 1) / tt - up_im)
+    ph = low * k
+    e_re, e_im = h * torch.cos(ph), -h * torch.sin(ph)
+    a_re, a_im = e_re * lo_re - e_im * lo_im, e_re * lo_im + e_im * lo_re
+    b_re, b_im = e_re * up_re - e_im * up_im, e_re * up_im + e_im * up_re
+    asc = z0 <= z1
+    Fn = torch.nn.functional
+    q_re = Fn.pad(torch.where(asc, a_re, b_re), (0, 0, 0, 1)) + Fn.pad(torch.where(asc, b_re, a_re), (0, 0, 1, 0))
+    q_im = Fn.pad(torch.where(asc, a_im, b_im), (0, 0, 0, 1)) + Fn.pad(torch.where(asc, b_im, a_im), (0, 0, 1, 0))
+    return q_re, q_im
+
+
+def column_taper_columns(z, k):
+    """Taper [n, M] of every column (column_taper per column: the lower median of its |spacing|)."""
+    d = (z[:, 1:] - z[:, :-1]).abs()
+    dz = torch.sort(d, dim=-1, stable=True).values[:, (d.shape[1] - 1) // 2 : (d.shape[1] - 1) // 2 + 1]
+    k_nyq = math.pi / dz
+    u = ((k - 2 * k_nyq / 3) / (k_nyq / 3)).clamp(0.0, 1.0)
+    return 0.5 * (1 + torch.cos(math.pi * u))
+
+
+def vertical_matrices_columns(z, k, factor, length):
+    """V_fwd [n, 2M, D] and V_inv [n, D, 2M] for the shifted column heights z [n, D] (torch float64 on the device);
+    vertical_matrices per column."""
+    kt = torch.as_tensor(np.asarray(k, np.float64), device=z.device)
+    ft = torch.as_tensor(np.asarray(factor, np.float64), device=z.device)
+    q_re, q_im = filon_weights_columns(z, kt)
+    v_fwd = torch.cat([q_re.transpose(1, 2), q_im.transpose(1, 2)], 1)
+    if z.shape[1] > 1:
+        f = ft * column_taper_columns(z, kt)                                    # [n, M]
+    else:
+        f = ft.expand(z.shape[0], -1)
+    ph = z[:, :, None] * kt                                                     # [n, D, M]
+    v_inv = torch.cat([f[:, None, :] * torch.cos(ph), -f[:, None, :] * torch.sin(ph)], 2) / length
+    return v_fwd, v_inv
+
+
+def split_bf16_columns(a, parts=2):
+    """(hi, lo, m, k, m_pad, k_pad[, l3]) of per-column matrices a [n, m, k] (torch float64): bf16 planes
+    [n, m_pad, k_pad], zero padded, as split_bf16 per column."""
+    n, m, k = a.shape
+    mp, kp = round_up(m, 16), round_up(k, 32)
+    rest = torch.nn.functional.pad(a, (0, kp - k, 0, mp - m))
+    planes = []
+    for _ in range(parts):
+        p = rest.to(torch.bfloat16)
+        planes.append(p.contiguous().view(torch.int16))
+        rest = rest - p.double()
+    return (planes[0], planes[1], m, k, mp, kp) + tuple(planes[2:])

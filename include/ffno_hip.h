@@ -134,6 +134,23 @@ typedef struct {
 
 int ffno_tw_gemm(const FfnoTwGemm* g, hipStream_t stream);
 
+/* The vertical transform on per-column height grids (models/ffno_model.py:277-283, :304-305, the reference's
+ * per-column path): each (sample, h, w) column j has its own split-bf16 matrix A_j [m x k] (planes a_hi, a_lo and
+ * optionally a_l3, each [B*hw][m_pad][k_pad]); Y[b][m][hw][c] = sum_k A_j[m][k] X[b][k][hw][c]. */
+typedef struct {
+  const uint16_t* a_hi;
+  const uint16_t* a_lo;
+  const uint16_t* a_l3;   /* third plane or NULL (two-way) */
+  int m, k, m_pad, k_pad;
+  int B;
+  int64_t hw;             /* columns per sample (H*W) */
+  int C;                  /* channels, % 4 == 0 */
+  const float* x;         /* [B][k][hw][C] */
+  float* y;               /* [B][m][hw][C] */
+} FfnoColGemm;
+
+int ffno_col_gemm(const FfnoColGemm* g, hipStream_t stream);
+
 /* Per-mode complex channel mix of a spectrum held as separate real and imaginary planes, in place
  * (models/ffno_model.py:394-429 for the horizontal branch, :286-302 for the vertical one):
  * Z = g(k) (Y (W_r + i W_i)) + U(k) (V(k) Y), times an optional per-mode factor. */

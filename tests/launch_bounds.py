@@ -189,6 +189,16 @@ class Checker:
         ym = self.imap_max(g.m, g.m_split, g.y_m_hi, g.y_m_lo)
         self.span(n, "y", g.y, (yb + ym + g.n) * F32)
 
+    def ffno_col_gemm(self, n, gref, stream):
+        g = gref._obj
+        assert g.C % 4 == 0 and g.m_pad % 16 == 0 and g.k_pad % 32 == 0 and g.m <= g.m_pad and g.k <= g.k_pad
+        n_a = g.B * g.hw * g.m_pad * g.k_pad * BF
+        self.span(n, "a_hi", g.a_hi, n_a)
+        self.span(n, "a_lo", g.a_lo, n_a)
+        self.span(n, "a_l3", g.a_l3, n_a, optional=True)
+        self.span(n, "x", g.x, g.B * g.k * g.hw * g.C * F32)
+        self.span(n, "y", g.y, g.B * g.m * g.hw * g.C * F32)
+
     def ffno_mix(self, n, mref, stream):
         m = mref._obj
         C = m.C
@@ -411,7 +421,7 @@ def dry_engine(model):
     return eng, chk, restore
 
 
-def check_shape(model, B, D, H, W, stride=1, want_z=True, column=None, two_pass=True):
+def check_shape(model, B, D, H, W, stride=1, want_z=True, column=None, two_pass=True, percol=False):
     """Run forward, the saving forward, the adjoint and the tangent (JVP) pass of `model` at [B, D, H, W] through the
     checker (with the two-pass fusion adjoint where the stride allows it); returns the number of launches checked."""
     eng, chk, restore = dry_engine(model)
@@ -420,8 +430,13 @@ def check_shape(model, B, D, H, W, stride=1, want_z=True, column=None, two_pass=
         col = np.linspace(14.0, -1.0, D) if column is None else np.asarray(column, np.float64)
         zs = col - col.min()
         span = float(col.max() - col.min())
-        plan = E.Plan(eng, B, D, H, W, zs, span, 48e3, 48e3)
         meta = torch.device("meta")
+        if percol:              # every column its own heights: per-column matrices, no height gradient
+            cols = torch.empty(B * H * W, D, dtype=torch.float64, device=meta)
+            plan = E.Plan(eng, B, D, H, W, None, span, 48e3, 48e3, columns=cols)
+            want_z = False
+        else:
+            plan = E.Plan(eng, B, D, H, W, zs, span, 48e3, 48e3)
         x = torch.empty(B, 6, D, H, W, device=meta)
         z = torch.empty(B, D, H, W, device=meta)
         eng.forward(x, z, 48e3, 48e3, plan=plan)

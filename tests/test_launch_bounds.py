@@ -32,6 +32,11 @@ def test_engine_launches_stay_in_bounds(B, D, H, W, stride):
         assert check_shape(_model(layers=2), B, D, H, W, stride=0) > 20          # every intermediate kept
 
 
+@pytest.mark.parametrize("B,D,H,W,width", [(2, 20, 12, 14, 64), (1, 33, 9, 7, 96), (3, 16, 8, 8, 32)])
+def test_per_column_grids_stay_in_bounds(B, D, H, W, width):
+    assert check_shape(_model(width=width, layers=2), B, D, H, W, stride=1, percol=True) > 50
+
+
 @pytest.mark.parametrize("width,rank", [(96, 8), (32, 0), (16, 4)])
 def test_other_widths_stay_in_bounds(width, rank):
     assert check_shape(_model(width=width, layers=3, rank=rank), 2, 20, 12, 14, stride=2) > 50
@@ -73,6 +78,7 @@ def test_checker_catches_a_short_buffer_and_covers_every_entry_point():
     try:
         check_shape(_model(layers=2), 1, 12, 8, 8, stride=2)
         check_shape(_model(layers=2), 1, 12, 8, 8, stride=1, two_pass=True)
+        check_shape(_model(layers=2), 1, 12, 8, 8, stride=1, percol=True)
     finally:
         LB.Checker.__call__ = orig
     assert declared <= calls, f"entry points never checked: {sorted(declared - calls)}"
