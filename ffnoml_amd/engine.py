@@ -330,6 +330,26 @@ class Plan:
         """Per-mode tables that are rebuilt per block on use (not kept): one block's worth is alive at a time."""
         return sum(t.block_bytes for t in self._mode_tables.values() if not t.cache)
 
+    def _column_height_tables(self):
+        """height_tables for per-column grids: every column's Filon bands, synthesis derivative and taper terms, the
+        median interval and sign per column and the mask of each column's lowest level(s)."""
+        with self._lock:
+            if self._height is None:
+                cols = self._columns
+                bands = T.filon_height_bands_columns(cols, self._k)
+                v_dz, v0, dfac, i_med, sgn = T.vertical_height_terms_columns(cols, self._k, self._factor,
+                                                                             self.length)
+                lowest = (cols == cols.min(dim=1, keepdim=True).values).float()        # [n, D]
+                any_taper = bool((dfac != 0).any())
+                self._height = dict(
+                    bands={d: T.split_bf16_columns(bands[d]) for d in (-1, 0, 1)},
+                    v_dz=T.split_bf16_columns(v_dz),
+                    v0=T.split_bf16_columns(v0.transpose(1, 2)) if any_taper else None,
+                    dfac=dfac.float(), i_med=i_med, sign=sgn.float(), lowest=lowest / lowest.sum(1, keepdim=True))
+                del bands, v_dz, v0
+                _settle(self.device)
+            return self._height
+
     def column_adjoint(self):
         """The transposed per-column matrices (two planes): V_inv^T [2M, D] and V_fwd^T [D, 2M] of every column."""
         with self._lock:
@@ -346,8 +366,7 @@ class Plan:
         Jacobian, the synthesis' derivative, and for the taper (non-zero only for modes between 2 pi / (3 dz) and
         pi / dz) the transposed unscaled synthesis and the per-mode factor * d taper / d dz."""
         if self.percol:
-            raise NotImplementedError("the height gradient on per-column height grids is not implemented; the "
-                                      "input-feature gradient is")
+            return self._column_height_tables()
         with self._lock:
             if self._height is None:
                 bands = T.filon_height_bands(self._column, self._k)
@@ -636,6 +655,20 @@ class Engine:
         g.B, g.hw, g.C = plan.B, plan.H * plan.W, plan.C
         g.x, g.y = _p(x), _p(y)
         _lib.call("ffno_col_gemm", ctypes.byref(g), self._stream())
+
+    def _vgemm(self, plan, mat, x, y, to_spectrum):
+        """One vertical GEMM of the height gradient: levels [B, D, HW, C] -> spectrum [B, 2M, HW, C]
+        (to_spectrum) or back, with the shared matrix or every column's own."""
+        if plan.percol:
+            self._colgemm(plan, mat, x, y)
+            return
+        B, D, M = plan.B, plan.D, plan.modes
+        hwc = plan.H * plan.W * plan.C
+        big = 1 << 40
+        if to_spectrum:
+            self._tw(mat, x, y, B, hwc, (big, 0, D * hwc), (0, 2 * M * hwc), (big, 0, hwc), (M, M * hwc, hwc))
+        else:
+            self._tw(mat, x, y, B, hwc, (big, 0, 2 * M * hwc), (0, D * hwc), (M, M * hwc, hwc), (big, 0, hwc))
 
     def vertical(self, plan, xv, v1, out, i, tables, adjoint=False):
         """Filon transform along D -> mix -> synthesis at the levels, /L folded in (models/ffno_model.py:270-305);
@@ -1061,10 +1094,12 @@ class Engine:
             v1 = scr["v1"]
             if want_z and ht["v0"] is not None:
                 # taper: dL/df_m per column = sum_c,part (V0^T yv_bar) * Y, before the mix adjoint overwrites v1
-                self._tw(ht["v0"], v, v1, B, hwc, (big, 0, D * hwc), (0, 2 * M * hwc), (big, 0, hwc),
-                         (M, M * hwc, hwc))
+                self._vgemm(plan, ht["v0"], v, v1, True)
                 gsum = (v1.view(B, 2, M, H * W, C) * v1y.view(B, 2, M, H * W, C)).sum((1, 4))   # [B, M, HW]
-                dfac_col += (gsum * ht["dfac"][None, :, None]).sum(1)
+                if plan.percol:
+                    dfac_col += (gsum.transpose(1, 2) * ht["dfac"].view(B, H * W, M)).sum(2)
+                else:
+                    dfac_col += (gsum * ht["dfac"][None, :, None]).sum(1)
             self.vertical(plan, v, v1, pp, i, tables, adjoint=True)
             if want_z:
                 # the vertical branch's input again (xs -> sp, xv -> fp, both dead here)
@@ -1073,12 +1108,10 @@ class Engine:
                           _p(bz), _p(sp), _p(fp), st)
                 # Filon weights: zbar_s[e] += sum_delta sum_c xv[e + delta] (G_delta X_bar)[e]   (x1 as scratch)
                 for delta in (-1, 0, 1):
-                    self._tw(ht["bands"][delta], v1, x1, B, hwc, (big, 0, 2 * M * hwc), (0, D * hwc),
-                             (M, M * hwc, hwc), (big, 0, hwc))
+                    self._vgemm(plan, ht["bands"][delta], v1, x1, False)
                     _lib.call("ffno_dot_reduce", rows, D, _p(fp), delta, _p(x1), _p(zbar_s), st)
                 # synthesis phases: zbar_s[d] += sum_c yv_bar[d] (V_dz Y)[d]
-                self._tw(ht["v_dz"], v1y, x1, B, hwc, (big, 0, 2 * M * hwc), (0, D * hwc), (M, M * hwc, hwc),
-                         (big, 0, hwc))
+                self._vgemm(plan, ht["v_dz"], v1y, x1, False)
                 _lib.call("ffno_dot_reduce", rows, D, _p(v), 0, _p(x1), _p(zbar_s), st)
             # input gates (and z_proj): h_bar in g
             _lib.call("ffno_gates_bwd", rows, _p(h), _p(mp), _p(pp), _p(g), _p(blk["spec.g1"]), _p(blk["spec.g2"]),
@@ -1104,13 +1137,22 @@ class Engine:
         ht = plan.height_tables()
         if dfac_col is not None:
             i, sgn = ht["i_med"], ht["sign"]
-            zs[:, i + 1] += sgn * dfac_col
-            zs[:, i] -= sgn * dfac_col
+            if plan.percol:
+                i = i.view(B, 1, H * W)
+                delta = (sgn.view(B, H * W) * dfac_col)[:, None, :]
+                zs.scatter_add_(1, i + 1, delta)
+                zs.scatter_add_(1, i, -delta)
+            else:
+                zs[:, i + 1] += sgn * dfac_col
+                zs[:, i] -= sgn * dfac_col
         # the per-column shift z - amin(z): the minimum's level(s) take minus the column total, shared evenly
         total = zs.sum(1)
         out = zbar.view(B, D, H * W) + zs
-        for j in ht["argmin"]:
-            out[:, int(j)] -= total / len(ht["argmin"])
+        if plan.percol:
+            out -= ht["lowest"].view(B, H * W, D).transpose(1, 2) * total[:, None, :]
+        else:
+            for j in ht["argmin"]:
+                out[:, int(j)] -= total / len(ht["argmin"])
         # depth weights: w_d = t_d / mean(t); t_d = |dz| halves of the intervals either side of level d
         zc = z.reshape(B, D, H * W).float()
         if D > 1:

@@ -267,3 +267,47 @@ def split_bf16_columns(a, parts=2):
         planes.append(p.contiguous().view(torch.int16))
         rest = rest - p.double()
     return (planes[0], planes[1], m, k, mp, kp) + tuple(planes[2:])
+
+
+def filon_height_bands_columns(z, k):
+    """filon_height_bands for every column of z [n, D] (torch float64): G[delta] [n, D, 2M] with
+    G[delta][:, e] = d q[e + delta] / d z_e (real parts, then imaginary)."""
+    kt = torch.as_tensor(np.asarray(k, np.float64), device=z.device)
+    n, D = z.shape
+    M = kt.shape[0]
+    bands = {d: torch.zeros(n, D, 2 * M, dtype=torch.float64, device=z.device) for d in (-1, 0, 1)}
+    idx = torch.arange(D, device=z.device)
+    for color in range(3):
+        v = torch.zeros_like(z)
+        v[:, color::3] = 1.0
+        with _FORWARD_AD_LOCK:
+            _, (dre, dim) = torch.func.jvp(lambda zz: filon_weights_columns(zz, kt), (z,), (v,))
+        for delta in (-1, 0, 1):
+            e = idx - delta                        # row d of the derivative belongs to level e = d - delta
+            ok = (e >= 0) & (e < D) & (e % 3 == color)
+            rows, levels = idx[ok], e[ok]
+            bands[delta][:, levels, :M] = dre[:, rows]
+            bands[delta][:, levels, M:] = dim[:, rows]
+    return bands
+
+
+def vertical_height_terms_columns(z, k, factor, length):
+    """vertical_height_terms for every column of z [n, D] (torch float64): V_dz [n, D, 2M], V0 [n, D, 2M],
+    factor * d taper / d dz [n, M], the median interval i [n] (lower middle of a stable sort, as CPU torch.median
+    resolves ties) and the sign of its spacing [n]."""
+    kt = torch.as_tensor(np.asarray(k, np.float64), device=z.device)
+    ft = torch.as_tensor(np.asarray(factor, np.float64), device=z.device)
+    n, D = z.shape
+    ph = z[:, :, None] * kt
+    f = ft * column_taper_columns(z, kt) if D > 1 else ft.expand(n, -1)
+    v_dz = torch.cat([-(f * kt)[:, None, :] * torch.sin(ph), -(f * kt)[:, None, :] * torch.cos(ph)], 2) / length
+    v0 = torch.cat([torch.cos(ph), -torch.sin(ph)], 2) / length
+    d = (z[:, 1:] - z[:, :-1]).abs()
+    i = torch.sort(d, dim=-1, stable=True).indices[:, (D - 2) // 2]
+    dz = d.gather(1, i[:, None])
+    k_nyq = math.pi / dz
+    u = (kt - 2 * k_nyq / 3) / (k_nyq / 3)
+    inside = (u > 0) & (u < 1)
+    dtaper = torch.where(inside, -0.5 * torch.sin(math.pi * u) * 3.0 * kt, torch.zeros_like(u))
+    sgn = torch.sign(z.gather(1, (i + 1)[:, None]) - z.gather(1, i[:, None]))[:, 0]
+    return v_dz, v0, ft * dtaper, i, sgn

@@ -431,10 +431,21 @@ def check_shape(model, B, D, H, W, stride=1, want_z=True, column=None, two_pass=
         zs = col - col.min()
         span = float(col.max() - col.min())
         meta = torch.device("meta")
-        if percol:              # every column its own heights: per-column matrices, no height gradient
-            cols = torch.empty(B * H * W, D, dtype=torch.float64, device=meta)
+        if percol:              # every column its own heights: per-column matrices
+            n = B * H * W
+            cols = torch.empty(n, D, dtype=torch.float64, device=meta)
             plan = E.Plan(eng, B, D, H, W, None, span, 48e3, 48e3, columns=cols)
-            want_z = False
+            M = plan.modes
+
+            def height_tables():  # the per-column height tables at their real shapes (their values need data)
+                from ffnoml_amd.transforms import split_bf16_columns as split
+
+                def mat(m, k):
+                    return split(torch.empty(n, m, k, dtype=torch.float64, device=meta))
+                return dict(bands={d: mat(D, 2 * M) for d in (-1, 0, 1)}, v_dz=mat(D, 2 * M), v0=mat(2 * M, D),
+                            dfac=torch.empty(n, M, device=meta), i_med=torch.zeros(n, dtype=torch.long, device=meta),
+                            sign=torch.empty(n, device=meta), lowest=torch.empty(n, D, device=meta))
+            plan._column_height_tables = height_tables
         else:
             plan = E.Plan(eng, B, D, H, W, zs, span, 48e3, 48e3)
         x = torch.empty(B, 6, D, H, W, device=meta)

@@ -1,8 +1,9 @@
 """Per-column height grids on the GPU (the reference's per-column vertical path, models/ffno_model.py:277-283,
 :304-305): every column's own Filon analysis and synthesis matrices. Forward and input gradient against the
-reference's own float64 outputs for golden case B (0.01 Mm jitter per column), forward, input gradient and tangent
-against the float64 oracle at width 64 with two samples of different jittered columns; the height gradient on such
-grids is refused with NotImplementedError.
+reference's own float64 outputs for golden case B (0.01 Mm jitter per column: forward, input and height gradients),
+and forward, input and height gradients and the tangent against the float64 oracle at width 64 with two samples of
+different jittered columns (the height gradient through every column's own Filon weights, phases, taper, depth
+weights and shift).
 
 Tolerances: 1e-3 relative in log10 n, 1e-2 relative for derivatives."""
 import numpy as np
@@ -31,21 +32,22 @@ def test_jitter_golden_forward_and_input_gradient():
     data, config, state = load("case_b_jitter64")
     model = _model_from(config, state)
     x = torch.from_numpy(data["x"]).float().to(DEV).requires_grad_(True)
-    z = torch.from_numpy(data["z"]).float().to(DEV)
+    z = torch.from_numpy(data["z"]).float().to(DEV).requires_grad_(True)
     ct = torch.from_numpy(data["cotangent"]).float().to(DEV)
     print("[jitter] forward + vjp", flush=True)
     y = model(x, z, 48e3, 48e3)
-    assert model.engine().plan(*z.shape, z, 48e3, 48e3).percol
-    (xb,) = torch.autograd.grad(y, x, grad_outputs=ct)
+    assert model.engine().plan(*z.shape, z.detach(), 48e3, 48e3).percol
+    xb, zb = torch.autograd.grad(y, (x, z), grad_outputs=ct)
     torch.cuda.synchronize()
     ref = data["y"]
     sy, my = state["std_Y"].numpy().reshape(1, -1, 1, 1, 1), state["mean_Y"].numpy().reshape(1, -1, 1, 1, 1)
     a, b = y.detach().cpu().numpy() * sy + my, ref * sy + my
     rel = float((np.abs(a - b) / np.maximum(np.abs(b), 1.0)).max())
     gx = _rel(xb.cpu().numpy(), data["xbar"])
-    print(f"[jitter] forward rel {rel:.3e}, x_bar rel {gx:.3e}", flush=True)
+    gz = _rel(zb.cpu().numpy(), data["zbar"])
+    print(f"[jitter] forward rel {rel:.3e}, x_bar rel {gx:.3e}, z_bar rel {gz:.3e}", flush=True)
     assert rel <= 1e-3
-    assert gx <= 1e-2
+    assert gx <= 1e-2 and gz <= 1e-2
 
 
 def test_per_column_width64_against_the_oracle():
@@ -65,21 +67,20 @@ def test_per_column_width64_against_the_oracle():
     t = torch.randn(x.shape, generator=torch.Generator().manual_seed(3))
     print("[percol64] forward + vjp + jvp", flush=True)
     xg = x.to(DEV).requires_grad_(True)
-    y = model(xg, z.to(DEV), 48e3, 48e3)
-    (xb,) = torch.autograd.grad(y, xg, grad_outputs=ct.to(DEV))
+    zg = z.to(DEV).requires_grad_(True)
+    y = model(xg, zg, 48e3, 48e3)
+    xb, zb = torch.autograd.grad(y, (xg, zg), grad_outputs=ct.to(DEV))
     _, yd = model.engine().jvp(x.to(DEV), z.to(DEV), 48e3, 48e3, t.to(DEV))
     torch.cuda.synchronize()
     sd = O.to_dtype(state, torch.float64)
     cfg = oracle_config(config)
-    ry, rxb, _ = O.vjp(sd, cfg, x.double(), z.double(), 48e3, 48e3, ct.double())
+    ry, rxb, rzb = O.vjp(sd, cfg, x.double(), z.double(), 48e3, 48e3, ct.double())
     _, ryd = torch.func.jvp(lambda xx: O.forward(sd, cfg, xx, z.double(), 48e3, 48e3), (x.double(),),
                             (t.double(),))
     rel = float(((y.detach().cpu().double() - ry).abs() / ry.abs().clamp_min(1.0)).max())
     gx = _rel(xb.cpu().numpy(), rxb.numpy())
     gt = _rel(yd.cpu().numpy(), ryd.numpy())
-    print(f"[percol64] forward rel {rel:.3e}, x_bar {gx:.3e}, tangent {gt:.3e}", flush=True)
+    gz = _rel(zb.cpu().numpy(), rzb.numpy())
+    print(f"[percol64] forward rel {rel:.3e}, x_bar {gx:.3e}, z_bar {gz:.3e}, tangent {gt:.3e}", flush=True)
     assert rel <= 1e-3
-    assert gx <= 1e-2 and gt <= 1e-2
-    zg = z.to(DEV).requires_grad_(True)
-    with pytest.raises(NotImplementedError):
-        torch.autograd.grad(model(x.to(DEV), zg, 48e3, 48e3), zg, grad_outputs=ct.to(DEV))
+    assert gx <= 1e-2 and gz <= 1e-2 and gt <= 1e-2
