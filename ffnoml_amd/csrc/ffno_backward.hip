@@ -171,9 +171,9 @@ __global__ __launch_bounds__(NW * 64) void k_mlp_bwd(FfnoRows rows, const float*
   constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32;
   extern __shared__ __align__(16) unsigned char smem[];
   bf16* lds = reinterpret_cast<bf16*>(smem);
-  const Wm W1 = LW ? stage_w(lds, w1, 2 * C * KP) : global_w(w1, 2 * C * KP);
-  const Wm W2T = LW ? stage_w(lds, w2t, 2 * C * KP) : global_w(w2t, 2 * C * KP);
-  const Wm W1T = LW ? stage_w(lds, w1t, C * 2 * C) : global_w(w1t, C * 2 * C);
+  const Wm W1 = LW ? stage_w(lds, w1, 2 * C * wld(KP)) : global_w(w1, 2 * C * wld(KP));
+  const Wm W2T = LW ? stage_w(lds, w2t, 2 * C * wld(KP)) : global_w(w2t, 2 * C * wld(KP));
+  const Wm W1T = LW ? stage_w(lds, w1t, C * wld(2 * C)) : global_w(w1t, C * wld(2 * C));
   __syncthreads();
   Lane L;
   const int b = blockIdx.y;
@@ -217,9 +217,10 @@ __global__ __launch_bounds__(NW * 64) void k_fuse_bwd(FfnoRows rows, const float
                                                       float* hbar, float* __restrict__ sbar,
                                                       float* __restrict__ vbar) {
   constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32;
-  const Wm B1 = global_w(bg1, C * 3 * KP), B2 = global_w(bg2, 2 * C * KP);
-  const Wm F1 = global_w(f1, 2 * C * 2 * KP), F2T = global_w(f2t, 2 * C * KP), F1T = global_w(f1t, 2 * C * 2 * C);
-  const Wm B2T = global_w(bg2t, C * 2 * C), B1T = global_w(bg1t, 3 * C * KP);
+  const Wm B1 = global_w(bg1, C * wld(3 * KP)), B2 = global_w(bg2, 2 * C * wld(KP));
+  const Wm F1 = global_w(f1, 2 * C * wld(2 * KP)), F2T = global_w(f2t, 2 * C * wld(KP));
+  const Wm F1T = global_w(f1t, 2 * C * wld(2 * C));
+  const Wm B2T = global_w(bg2t, C * wld(2 * C)), B1T = global_w(bg1t, 3 * C * wld(KP));
   Lane L;
   const int b = blockIdx.y;
   const float scale = gf.scale ? *gf.scale : 1.f;
@@ -333,9 +334,9 @@ __global__ __launch_bounds__(NW * 64) void k_fuse_mlp_bwd(FfnoRows rows, const f
   constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32;
   extern __shared__ __align__(16) unsigned char smem[];
   bf16* lds = reinterpret_cast<bf16*>(smem);
-  const Wm F1 = LW ? stage_w(lds, f1, 2 * C * 2 * KP) : global_w(f1, 2 * C * 2 * KP);
-  const Wm F1T = LW ? stage_w(lds, f1t, 2 * C * 2 * C) : global_w(f1t, 2 * C * 2 * C);
-  const Wm F2T = global_w(f2t, 2 * C * KP);
+  const Wm F1 = LW ? stage_w(lds, f1, 2 * C * wld(2 * KP)) : global_w(f1, 2 * C * wld(2 * KP));
+  const Wm F1T = LW ? stage_w(lds, f1t, 2 * C * wld(2 * C)) : global_w(f1t, 2 * C * wld(2 * C));
+  const Wm F2T = global_w(f2t, 2 * C * wld(KP));
   __syncthreads();
   Lane L;
   const int b = blockIdx.y;
@@ -394,9 +395,9 @@ __global__ __launch_bounds__(NW * 64) void k_fuse_gates_bwd(FfnoRows rows, float
   constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32;
   extern __shared__ __align__(16) unsigned char smem[];
   bf16* lds = reinterpret_cast<bf16*>(smem);
-  const Wm B1 = LW ? stage_w(lds, bg1, C * 3 * KP) : global_w(bg1, C * 3 * KP);
-  const Wm B2T = LW ? stage_w(lds, bg2t, C * 2 * C) : global_w(bg2t, C * 2 * C);
-  const Wm B1T = LW ? stage_w(lds, bg1t, 3 * C * KP) : global_w(bg1t, 3 * C * KP);
+  const Wm B1 = LW ? stage_w(lds, bg1, C * wld(3 * KP)) : global_w(bg1, C * wld(3 * KP));
+  const Wm B2T = LW ? stage_w(lds, bg2t, C * wld(2 * C)) : global_w(bg2t, C * wld(2 * C));
+  const Wm B1T = LW ? stage_w(lds, bg1t, 3 * C * wld(KP)) : global_w(bg1t, 3 * C * wld(KP));
   __syncthreads();
   Lane L;
   const int b = blockIdx.y;
@@ -480,7 +481,7 @@ __global__ __launch_bounds__(NW * 64) void k_gates_bwd(FfnoRows rows, const floa
                                                        const uint16_t* v1, const uint16_t* v2, const uint16_t* v2t,
                                                        const uint16_t* v1t, const float* __restrict__ z_proj,
                                                        float* hbar_out, float* __restrict__ zbar) {
-  constexpr int C = CT * 16, KS = ks_of(CT), SZ = C * KS * 32;
+  constexpr int C = CT * 16, KS = ks_of(CT), SZ = C * wld(KS * 32);
   extern __shared__ __align__(16) unsigned char smem[];
   bf16* lds = reinterpret_cast<bf16*>(smem);
   const Wm G1 = LW ? stage_w(lds, g1, SZ) : global_w(g1, SZ), G2 = LW ? stage_w(lds, g2, SZ) : global_w(g2, SZ);
@@ -532,10 +533,10 @@ __global__ __launch_bounds__(NW * 64) void k_head_bwd(FfnoRows rows, const float
   constexpr int C = CT * 16, KS1 = ks_of(CT);
   extern __shared__ __align__(16) unsigned char smem[];
   bf16* lds = reinterpret_cast<bf16*>(smem);
-  const Wm P1 = LW ? stage_w(lds, p1, 2 * C * KS1 * 32) : global_w(p1, 2 * C * KS1 * 32);
-  const Wm P2 = LW ? stage_w(lds, p2, NTH * 16 * 2 * C) : global_w(p2, NTH * 16 * 2 * C);
-  const Wm P2T = LW ? stage_w(lds, p2t, 2 * C * 32) : global_w(p2t, 2 * C * 32);
-  const Wm P1T = LW ? stage_w(lds, p1t, C * 2 * C) : global_w(p1t, C * 2 * C);
+  const Wm P1 = LW ? stage_w(lds, p1, 2 * C * wld(KS1 * 32)) : global_w(p1, 2 * C * wld(KS1 * 32));
+  const Wm P2 = LW ? stage_w(lds, p2, NTH * 16 * wld(2 * C)) : global_w(p2, NTH * 16 * wld(2 * C));
+  const Wm P2T = LW ? stage_w(lds, p2t, 2 * C * wld(32)) : global_w(p2t, 2 * C * wld(32));
+  const Wm P1T = LW ? stage_w(lds, p1t, C * wld(2 * C)) : global_w(p1t, C * wld(2 * C));
   float* hs = reinterpret_cast<float*>(lds);
   __syncthreads();
   Lane L;
@@ -732,7 +733,7 @@ int ffno_mlp_bwd(FfnoRows rows, const float* ubar, FfnoGnBack gn, FfnoAssemble x
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {
     constexpr int C = CT * 16, KP = ks_of(CT) * 32;
-    const size_t lds = ((size_t)2 * C * KP * 2 + (size_t)C * 2 * C) * 2 * 2;
+    const size_t lds = ((size_t)2 * C * wld(KP) * 2 + (size_t)C * wld(2 * C)) * 2 * 2;
     if (lds <= 160 * 1024) {
       auto k = k_mlp_bwd<CT, nw_bwd<CT>(), true>;
       if (set_lds(k, lds)) return -1;
@@ -770,7 +771,7 @@ int ffno_fuse_mlp_bwd(FfnoRows rows, const float* x1bar, FfnoGnBack gnf, const f
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {
     constexpr int C = CT * 16, KP = ks_of(CT) * 32;
-    const size_t lds = ((size_t)2 * C * 2 * KP + (size_t)2 * C * 2 * C) * 2 * 2;
+    const size_t lds = ((size_t)2 * C * wld(2 * KP) + (size_t)2 * C * wld(2 * C)) * 2 * 2;
     if (lds <= 160 * 1024) {
       auto k = k_fuse_mlp_bwd<CT, nw_of<CT>(), true>;
       if (set_lds(k, lds)) return -1;
@@ -794,7 +795,7 @@ int ffno_fuse_gates_bwd(FfnoRows rows, float* x1bar_hbar, const float* h, const 
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {
     constexpr int C = CT * 16, KP = ks_of(CT) * 32;
-    const size_t lds = ((size_t)C * 3 * KP + (size_t)C * 2 * C + (size_t)3 * C * KP) * 2 * 2;
+    const size_t lds = ((size_t)C * wld(3 * KP) + (size_t)C * wld(2 * C) + (size_t)3 * C * wld(KP)) * 2 * 2;
     if (lds <= 160 * 1024) {
       auto k = k_fuse_gates_bwd<CT, nw_of<CT>(), true>;
       if (set_lds(k, lds)) return -1;
@@ -817,7 +818,7 @@ int ffno_gates_bwd(FfnoRows rows, const float* h, const float* xsbar, const floa
     return fail("ffno_gates_bwd: bad arguments");
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {
-    const size_t lds = 8 * (size_t)(CT * 16) * ks_of(CT) * 32 * 2 * 2;
+    const size_t lds = 8 * (size_t)(CT * 16) * wld(ks_of(CT) * 32) * 2 * 2;
     if (lds <= 160 * 1024) {
       auto k = k_gates_bwd<CT, nw_bwd<CT>(), true>;
       if (set_lds(k, lds)) return -1;
@@ -847,8 +848,8 @@ int ffno_head_bwd(FfnoRows rows, const float* h, const uint16_t* p1, const uint1
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {
     constexpr int C = CT * 16;
-    const size_t w_bytes = ((size_t)2 * C * ks_of(CT) * 32 + (size_t)nth * 16 * 2 * C + (size_t)2 * C * 32 +
-                            (size_t)C * 2 * C) * 2 * 2;
+    const size_t w_bytes = ((size_t)2 * C * wld(ks_of(CT) * 32) + (size_t)nth * 16 * wld(2 * C) + (size_t)2 * C * wld(32) +
+                            (size_t)C * wld(2 * C)) * 2 * 2;
     const size_t hs = (size_t)nw_of<CT>() * 16 * nth * 16 * sizeof(float);
     const bool lw = w_bytes + hs <= 160 * 1024;
     const size_t lds = (lw ? w_bytes : 0) + hs;

@@ -63,6 +63,12 @@ __device__ __forceinline__ void st4(float* p, f32x4 v, bool ok) { if (ok) *(f32x
 
 constexpr int ks_of(int ntiles) { return (ntiles + 1) / 2; }   // k-steps of 32 for ntiles 16-feature tiles
 
+// Every packed weight row of K bf16 (K a multiple of 32) is followed by WPAD bf16 of zeros, in global memory and in
+// LDS alike: with a row stride of K + 8 (2K + 16 bytes) the 16 rows an A-fragment read touches start in 16 distinct
+// 16-byte bank groups, where an unpadded stride (a multiple of 64 bytes) lands up to 8 of them on the same banks.
+constexpr int WPAD = 8;
+constexpr int wld(int k) { return k + WPAD; }                   // row stride of a packed weight of K columns
+
 // split-bf16 operands: x = hi + lo with hi = bf16(x), lo = bf16(x - hi); products hi*hi + hi*lo + lo*hi keep about
 // 16 mantissa bits, so every MFMA layer is accurate to ~1e-5 relative with float32 accumulation
 struct Frag {
@@ -107,7 +113,7 @@ __device__ __forceinline__ f32x4 tile(Wm W, int t, const Frag* b, int v, int q) 
   f32x4 c = zero4();
 #pragma unroll
   for (int s = 0; s < KS; ++s) {
-    const int o = (16 * t + v) * (KS * 32) + 32 * s + 8 * q;
+    const int o = (16 * t + v) * wld(KS * 32) + 32 * s + 8 * q;
     c = mfma3(*(const bf16x8*)(W.h + o), *(const bf16x8*)(W.l + o), b[s], c);
   }
   return c;
@@ -145,7 +151,7 @@ __device__ __forceinline__ void mlp2_bwd(Wm W1, Wm W2T, Wm W1T, const Frag* in, 
     const Frag df = split8(d0, d1);
 #pragma unroll
     for (int t = 0; t < NTI; ++t) {
-      const int o = (16 * t + v) * (KSH * 32) + 32 * s + 8 * q;
+      const int o = (16 * t + v) * wld(KSH * 32) + 32 * s + 8 * q;
       acc[t] = mfma3(*(const bf16x8*)(W1T.h + o), *(const bf16x8*)(W1T.l + o), df, acc[t]);
     }
   }
@@ -170,7 +176,7 @@ __device__ __forceinline__ void mlp2(Wm W1, Wm W2, const Frag* in, f32x4 (&out)[
     const Frag hf = split8(h0, h1);
 #pragma unroll
     for (int t = 0; t < NT2; ++t) {
-      const int o = (16 * t + v) * (KS2 * 32) + 32 * s + 8 * q;
+      const int o = (16 * t + v) * wld(KS2 * 32) + 32 * s + 8 * q;
       out[t] = mfma3(*(const bf16x8*)(W2.h + o), *(const bf16x8*)(W2.l + o), hf, out[t]);
     }
   }
@@ -230,7 +236,7 @@ __device__ __forceinline__ f32x4 tile(Wm3 W, int t, const Frag3* b, int v, int q
   f32x4 c = zero4();
 #pragma unroll
   for (int s = 0; s < KS; ++s) {
-    const int o = (16 * t + v) * (KS * 32) + 32 * s + 8 * q;
+    const int o = (16 * t + v) * wld(KS * 32) + 32 * s + 8 * q;
     c = mfma6(*(const bf16x8*)(W.h + o), *(const bf16x8*)(W.m + o), *(const bf16x8*)(W.l + o), b[s], c);
   }
   return c;
@@ -248,7 +254,7 @@ __device__ __forceinline__ void mlp2(Wm3 W1, Wm3 W2, const Frag3* in, f32x4 (&ou
     const Frag3 hf = split8_3(h0, h1);
 #pragma unroll
     for (int t = 0; t < NT2; ++t) {
-      const int o = (16 * t + v) * (KS2 * 32) + 32 * s + 8 * q;
+      const int o = (16 * t + v) * wld(KS2 * 32) + 32 * s + 8 * q;
       out[t] = mfma6(*(const bf16x8*)(W2.h + o), *(const bf16x8*)(W2.m + o), *(const bf16x8*)(W2.l + o), hf, out[t]);
     }
   }
@@ -407,9 +413,9 @@ __device__ __forceinline__ void gelu_tiles(f32x4 (&x)[CT]) {
 }
 
 // ---------------------------------------------------------------------------------------------------------------
-// weights: packed [2][rows][K] (hi then lo) in global memory; staged into LDS when the kernel's set fits
+// weights: packed [2][rows][wld(K)] (hi then lo) in global memory; staged into LDS when the kernel's set fits
 // ---------------------------------------------------------------------------------------------------------------
-__device__ __forceinline__ Wm stage_w(bf16*& lds, const uint16_t* g, int n) {   // n = rows * K of one half
+__device__ __forceinline__ Wm stage_w(bf16*& lds, const uint16_t* g, int n) {   // n = rows * wld(K) of one half
   stage(lds, g, 2 * n);
   Wm w{lds, lds + n};
   lds += 2 * n;
@@ -420,7 +426,7 @@ __device__ __forceinline__ Wm global_w(const uint16_t* g, int n) {
   return Wm{reinterpret_cast<const bf16*>(g), reinterpret_cast<const bf16*>(g) + n};
 }
 
-// a three-plane weight [3][rows][K]: its first NL planes (0, 2 or 3) staged into LDS, the rest read from global
+// a three-plane weight [3][rows][wld(K)]: its first NL planes (0, 2 or 3) staged into LDS, the rest read from global
 template <int NL>
 __device__ __forceinline__ Wm3 stage_w3(bf16*& lds, const uint16_t* g, int n) {
   const bf16* gb = reinterpret_cast<const bf16*>(g);

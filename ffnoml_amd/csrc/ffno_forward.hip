@@ -120,7 +120,7 @@ __global__ __launch_bounds__(NW * 64) void k_block_in(FfnoRows rows, FfnoAssembl
                                                       const uint16_t* g1, const uint16_t* g2, const uint16_t* v1,
                                                       const uint16_t* v2, const float* __restrict__ z_proj,
                                                       const float* __restrict__ z, float* xs, float* xv) {
-  constexpr int C = CT * 16, KS = ks_of(CT), SZ = C * KS * 32;
+  constexpr int C = CT * 16, KS = ks_of(CT), SZ = C * wld(KS * 32);
   extern __shared__ __align__(16) unsigned char smem[];
   bf16* lds = reinterpret_cast<bf16*>(smem);
   const Wm3 G1 = stage_w3<NL>(lds, g1, SZ), G2 = stage_w3<NL>(lds, g2, SZ);
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(NW * 64) void k_post(FfnoRows rows, const float* u,
   constexpr int C = CT * 16, KS1 = ks_of(CT);
   extern __shared__ __align__(16) unsigned char smem[];
   bf16* lds = reinterpret_cast<bf16*>(smem);
-  const Wm3 W1 = stage_w3<NL>(lds, w1, 2 * C * KS1 * 32), W2 = stage_w3<NL>(lds, w2, C * 2 * C);
+  const Wm3 W1 = stage_w3<NL>(lds, w1, 2 * C * wld(KS1 * 32)), W2 = stage_w3<NL>(lds, w2, C * wld(2 * C));
   float* red = reinterpret_cast<float*>(lds);
   __syncthreads();
   Lane L;
@@ -204,17 +204,20 @@ __global__ __launch_bounds__(NW * 64) void k_fuse(FfnoRows rows, const float* __
                                                   const uint16_t* f2, float* fp, FfnoStatsOut st,
                                                   float* __restrict__ gates_out) {
   constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32;
-  constexpr int N1 = C * 3 * KP, N2 = 2 * C * KP, N3 = 2 * C * 2 * KP, N4 = C * 2 * C;
+  constexpr int N1 = C * wld(3 * KP), N2 = 2 * C * wld(KP), N3 = 2 * C * wld(2 * KP), N4 = C * wld(2 * C);
   extern __shared__ __align__(16) unsigned char smem[];
   bf16* lds = reinterpret_cast<bf16*>(smem);
   Wm B1, B2, F1, F2;
   if (LW) {
-    stage(lds, bg1, N1);                             // hi half only
+    // B1 and F2: hi plane staged, lo plane read from global memory (everything staged would pass 160 KB)
+    stage(lds, bg1, N1);
     B1 = Wm{lds, reinterpret_cast<const bf16*>(bg1) + N1};
     lds += N1;
     B2 = stage_w(lds, bg2, N2);
     F1 = stage_w(lds, f1, N3);
-    F2 = stage_w(lds, f2, N4);
+    stage(lds, f2, N4);
+    F2 = Wm{lds, reinterpret_cast<const bf16*>(f2) + N4};
+    lds += N4;
   } else {
     B1 = global_w(bg1, N1);
     B2 = global_w(bg2, N2);
@@ -282,7 +285,7 @@ __global__ __launch_bounds__(NW * 64) void k_resmlp(FfnoRows rows, FfnoAssemble 
   constexpr int C = CT * 16, KS1 = ks_of(CT);
   extern __shared__ __align__(16) unsigned char smem[];
   bf16* lds = reinterpret_cast<bf16*>(smem);
-  const Wm3 W1 = stage_w3<NL>(lds, w1, 2 * C * KS1 * 32), W2 = stage_w3<NL>(lds, w2, C * 2 * C);
+  const Wm3 W1 = stage_w3<NL>(lds, w1, 2 * C * wld(KS1 * 32)), W2 = stage_w3<NL>(lds, w2, C * wld(2 * C));
   float* red = reinterpret_cast<float*>(lds);
   __syncthreads();
   Lane L;
@@ -323,7 +326,7 @@ __global__ __launch_bounds__(NW * 64) void k_head(FfnoRows rows, FfnoAssemble in
   constexpr int C = CT * 16, KS1 = ks_of(CT);
   extern __shared__ __align__(16) unsigned char smem[];
   bf16* lds = reinterpret_cast<bf16*>(smem);
-  const Wm3 P1 = stage_w3<NL>(lds, p1, 2 * C * KS1 * 32), P2 = stage_w3<NL>(lds, p2, NTH * 16 * 2 * C);
+  const Wm3 P1 = stage_w3<NL>(lds, p1, 2 * C * wld(KS1 * 32)), P2 = stage_w3<NL>(lds, p2, NTH * 16 * wld(2 * C));
   float* hs = reinterpret_cast<float*>(lds);   // [NW][16 voxels][NTH*16]
   __syncthreads();
   Lane L;
@@ -394,11 +397,25 @@ __global__ __launch_bounds__(NW * 64) void k_head(FfnoRows rows, FfnoAssemble in
 // ---------------------------------------------------------------------------------------------------------------
 // split-bf16 twiddle GEMM
 // ---------------------------------------------------------------------------------------------------------------
-// split of a column group: 8 k-values of the lane's 4 columns into 2 or 3 bf16 planes
-template <bool THREE>
-__device__ __forceinline__ void split_cols(const f32x4 (&xv)[8], bf16x8 (&bh)[4], bf16x8 (&bm)[4], bf16x8 (&bl)[4]) {
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+// a lane's NC consecutive columns (NC = 4 or 2) as one vector load
+template <int NC>
+struct ColVec;
+template <>
+struct ColVec<4> {
+  typedef f32x4 T;
+};
+template <>
+struct ColVec<2> {
+  typedef f32x2 T;
+};
+
+// split of a column group: 8 k-values of the lane's NC columns into 2 or 3 bf16 planes
+template <bool THREE, int NC = 4>
+__device__ __forceinline__ void split_cols(const typename ColVec<NC>::T (&xv)[8], bf16x8 (&bh)[NC], bf16x8 (&bm)[NC],
+                                           bf16x8 (&bl)[NC]) {
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < NC; ++i)
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const float x = xv[j][i];
@@ -411,12 +428,12 @@ __device__ __forceinline__ void split_cols(const f32x4 (&xv)[8], bf16x8 (&bh)[4]
     }
 }
 
-// acc += A-fragment (ah, am[, al]) x the lane's 4 column fragments
-template <bool THREE>
-__device__ __forceinline__ void tw_mac(bf16x8 ah, bf16x8 am, bf16x8 al, const bf16x8 (&bh)[4], const bf16x8 (&bm)[4],
-                                       const bf16x8 (&bl)[4], f32x4 (&acc)[4]) {
+// acc += A-fragment (ah, am[, al]) x the lane's NC column fragments
+template <bool THREE, int NC = 4>
+__device__ __forceinline__ void tw_mac(bf16x8 ah, bf16x8 am, bf16x8 al, const bf16x8 (&bh)[NC], const bf16x8 (&bm)[NC],
+                                       const bf16x8 (&bl)[NC], f32x4 (&acc)[NC]) {
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < NC; ++i) {
     if (THREE) {
       acc[i] = mfma(al, bh[i], acc[i]);
       acc[i] = mfma(am, bm[i], acc[i]);
@@ -554,12 +571,14 @@ __global__ __launch_bounds__(256) void k_col_gemm(FfnoColGemm g) {
 
 // twiddle matrices of at most 128 x 128: A staged in LDS once per persistent workgroup (rows padded by 16 bytes so
 // the lanes' 16-byte reads fall in distinct banks), the row offsets of both index maps tabulated in LDS, all MT
-// m-tiles of a 64-column task accumulated in registers (the task's input is read once), and the next k-step's input
-// loaded while the current one's MFMAs run
+// m-tiles of a task of 16 * NC columns accumulated in registers (the task's input is read once), and the next
+// k-step's input loaded while the current one's MFMAs run. NC = 2 halves the accumulators of the larger tiles so two
+// waves fit each SIMD.
 constexpr int TW_PAD = 8;
 
-template <int MT, bool THREE>
-__global__ __launch_bounds__(256) void k_tw_small(FfnoTwGemm g) {
+template <int MT, bool THREE, int NC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_tw_small(FfnoTwGemm g) {
+  typedef typename ColVec<NC>::T V;
   extern __shared__ __align__(16) unsigned char smem[];
   const int kp = g.k_pad, ldk = kp + TW_PAD;
   int64_t* koff = reinterpret_cast<int64_t*>(smem);
@@ -584,35 +603,51 @@ __global__ __launch_bounds__(256) void k_tw_small(FfnoTwGemm g) {
     moff[m] = m < g.m ? imap(m, g.m_split, g.y_m_hi, g.y_m_lo) : -1;
   __syncthreads();
   Lane L;
-  const int64_t ntn = (g.n + 63) / 64;
+  const int64_t ntn = (g.n + 16 * NC - 1) / (16 * NC);
   const int64_t tasks = (int64_t)g.batch * ntn;
   const int ks = kp / 32;
-  for (int64_t task = (int64_t)blockIdx.x * 4 + L.wave; task < tasks; task += (int64_t)gridDim.x * 4) {
-    const int64_t bi = task / ntn;
-    const int n0 = (int)(task % ntn) * 64 + 4 * L.v;
-    const bool nok = n0 < g.n;
-    const float* xb = g.x + imap(bi, g.b_split, g.x_b_hi, g.x_b_lo) + n0;
-    float* yb = g.y + imap(bi, g.b_split, g.y_b_hi, g.y_b_lo) + n0;
-    f32x4 acc[MT][4];
+  const int64_t stride = (int64_t)gridDim.x * 4;
+  // the columns of task t and the input pointer of its batch item
+  auto cols = [&](int64_t t, int& n0, bool& nok, const float*& xb) {
+    n0 = (int)(t % ntn) * (16 * NC) + NC * L.v;
+    nok = n0 < g.n;
+    xb = g.x + imap(t / ntn, g.b_split, g.x_b_hi, g.x_b_lo) + n0;
+  };
+  // k-step s of one lane's 4 columns (koff < 0: padding rows beyond K)
+  auto load = [&](const float* xb, bool nok, int s, V (&xv)[8]) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int64_t o = koff[32 * s + 8 * L.q + j];
+      xv[j] = (nok && o >= 0) ? *(const V*)(xb + o) : V{};
+    }
+  };
+  int64_t task = (int64_t)blockIdx.x * 4 + L.wave;
+  if (task >= tasks) return;
+  int n0;
+  bool nok;
+  const float* xb;
+  cols(task, n0, nok, xb);
+  V xv[8];
+  load(xb, nok, 0, xv);
+  // every wave runs its tasks with the next k-step's loads in flight across task boundaries: the first k-step of
+  // the next task is fetched during the last k-step of this one
+  while (true) {
+    float* yb = g.y + imap(task / ntn, g.b_split, g.y_b_hi, g.y_b_lo) + n0;
+    const bool out_ok = nok;
+    const int64_t next = task + stride;
+    f32x4 acc[MT][NC];
 #pragma unroll
     for (int a = 0; a < MT; ++a)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) acc[a][i] = zero4();
-    f32x4 xv[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int64_t o = koff[8 * L.q + j];
-      xv[j] = (nok && o >= 0) ? *(const f32x4*)(xb + o) : zero4();
-    }
+      for (int i = 0; i < NC; ++i) acc[a][i] = zero4();
     for (int s = 0; s < ks; ++s) {
-      bf16x8 bh[4], bm[4], bl[4];
-      split_cols<THREE>(xv, bh, bm, bl);
+      bf16x8 bh[NC], bm[NC], bl[NC];
+      split_cols<THREE, NC>(xv, bh, bm, bl);
       if (s + 1 < ks) {
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          const int64_t o = koff[32 * (s + 1) + 8 * L.q + j];
-          xv[j] = (nok && o >= 0) ? *(const f32x4*)(xb + o) : zero4();
-        }
+        load(xb, nok, s + 1, xv);
+      } else if (next < tasks) {
+        cols(next, n0, nok, xb);
+        load(xb, nok, 0, xv);
       }
 #pragma unroll
       for (int a = 0; a < MT; ++a) {
@@ -620,7 +655,7 @@ __global__ __launch_bounds__(256) void k_tw_small(FfnoTwGemm g) {
         const bf16x8 fh = *reinterpret_cast<const bf16x8*>(ah + ao);
         const bf16x8 fm = *reinterpret_cast<const bf16x8*>(am + ao);
         const bf16x8 fl = THREE ? *reinterpret_cast<const bf16x8*>(al + ao) : fh;
-        tw_mac<THREE>(fh, fm, fl, bh, bm, bl, acc[a]);
+        tw_mac<THREE, NC>(fh, fm, fl, bh, bm, bl, acc[a]);
       }
     }
 #pragma unroll
@@ -628,13 +663,17 @@ __global__ __launch_bounds__(256) void k_tw_small(FfnoTwGemm g) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int64_t o = moff[a * 16 + 4 * L.q + j];
-        if (o >= 0 && nok) {
+        if (o >= 0 && out_ok) {
           float* p = yb + o;
-          f32x4 r = f32x4{acc[a][0][j], acc[a][1][j], acc[a][2][j], acc[a][3][j]};
-          if (g.accumulate) r += *(const f32x4*)p;
-          *(f32x4*)p = r;
+          V r;
+#pragma unroll
+          for (int i = 0; i < NC; ++i) r[i] = acc[a][i][j];
+          if (g.accumulate) r += *(const V*)p;
+          *(V*)p = r;
         }
       }
+    task = next;
+    if (task >= tasks) break;
   }
 }
 
@@ -648,7 +687,7 @@ template <int CT, int NL, bool THREE>
 __global__ __launch_bounds__(256) void k_mix(FfnoMix mx) {
   using S = SplitT<THREE>;
   using F = typename S::F;
-  constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32, NWC = 2 * C * 2 * KP, P = S::P;
+  constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32, NWC = 2 * C * wld(2 * KP), P = S::P;
   extern __shared__ __align__(16) unsigned char smem[];
   bf16* lds = reinterpret_cast<bf16*>(smem);
   typename S::W Wc;
@@ -668,9 +707,9 @@ __global__ __launch_bounds__(256) void k_mix(FfnoMix mx) {
     const int rt1 = rt0 + MIX_TILES < rtiles ? rt0 + MIX_TILES : rtiles;
     const float gr = mx.gate[2 * mode], gi = mx.gate[2 * mode + 1];
     const float fac = mx.factor ? mx.factor[mode] : 1.f;
-    // per-mode low-rank tables: V [16][KP] and U [C][32], P planes of each mode's block
-    const auto V = S::global(mx.v_p ? mx.v_p + (int64_t)mode * P * 16 * KP : mx.wc, 16 * KP);
-    const auto U = S::global(mx.u_p ? mx.u_p + (int64_t)mode * P * C * 32 : mx.wc, C * 32);
+    // per-mode low-rank tables: V [16][wld(KP)] and U [C][wld(32)], P planes of each mode's block
+    const auto V = S::global(mx.v_p ? mx.v_p + (int64_t)mode * P * 16 * wld(KP) : mx.wc, 16 * wld(KP));
+    const auto U = S::global(mx.u_p ? mx.u_p + (int64_t)mode * P * C * wld(32) : mx.wc, C * wld(32));
     for (int rt = rt0; rt < rt1; ++rt) {
       const int r = rt * 16 + L.v;
       const bool ok = r < mx.rows;
@@ -745,7 +784,7 @@ int ffno_block_in(FfnoRows rows, FfnoAssemble in, float* x_out, const uint16_t* 
     return fail("ffno_block_in: bad arguments");
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {
-    const size_t elems = 4 * (size_t)(CT * 16) * ks_of(CT) * 32;
+    const size_t elems = 4 * (size_t)(CT * 16) * wld(ks_of(CT) * 32);
     const int nl = lds_planes(elems, 0);
     FFNO_DISPATCH_NL(nl, {
       const size_t lds = NL * elems * 2;
@@ -762,7 +801,7 @@ int ffno_post(FfnoRows rows, const float* u, const uint16_t* w1, const uint16_t*
   if (!rows_ok(rows) || !u || !w1 || !w2 || !y) return fail("ffno_post: bad arguments");
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {
-    const size_t elems = (size_t)2 * CT * 16 * ks_of(CT) * 32 + (size_t)CT * 16 * 2 * CT * 16;
+    const size_t elems = (size_t)2 * CT * 16 * wld(ks_of(CT) * 32) + (size_t)CT * 16 * wld(2 * CT * 16);
     const size_t red = red_bytes_nw<CT, nw_fwd<CT>()>();
     const int nl = lds_planes(elems, red);
     FFNO_DISPATCH_NL(nl, {
@@ -785,7 +824,8 @@ int ffno_fuse(FfnoRows rows, const float* x, const float* sp, const float* vp, c
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {
     constexpr int C = CT * 16, KP = ks_of(CT) * 32;
-    const size_t lds = ((size_t)C * 3 * KP + 2 * ((size_t)2 * C * KP + 2 * C * 2 * KP + (size_t)C * 2 * C)) * 2 +
+    const size_t lds = ((size_t)C * wld(3 * KP) + 2 * ((size_t)2 * C * wld(KP) + 2 * C * wld(2 * KP)) +
+                        (size_t)C * wld(2 * C)) * 2 +
                        red_bytes_nw<CT, nw_of<CT>()>();
     if (lds <= 160 * 1024) {
       auto k = k_fuse<CT, nw_of<CT>(), true>;
@@ -806,7 +846,7 @@ int ffno_resmlp(FfnoRows rows, FfnoAssemble in, float* x_out, const uint16_t* w1
   if (!rows_ok(rows) || !in.t || !in.aff_a || !in.aff_b || !w1 || !w2 || !y) return fail("ffno_resmlp: bad arguments");
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {
-    const size_t elems = (size_t)2 * CT * 16 * ks_of(CT) * 32 + (size_t)CT * 16 * 2 * CT * 16;
+    const size_t elems = (size_t)2 * CT * 16 * wld(ks_of(CT) * 32) + (size_t)CT * 16 * wld(2 * CT * 16);
     const size_t red = red_bytes_nw<CT, nw_fwd<CT>()>();
     const int nl = lds_planes(elems, red);
     FFNO_DISPATCH_NL(nl, {
@@ -834,7 +874,7 @@ int ffno_head(FfnoRows rows, FfnoAssemble in, float* x_out, const uint16_t* p1, 
   dim3 grid(rows.nwg, rows.B);
   const int nth = (n_head + 15) / 16;
   FFNO_DISPATCH_CT(rows.C, {
-    const size_t elems = (size_t)2 * CT * 16 * ks_of(CT) * 32 + (size_t)nth * 16 * 2 * CT * 16;
+    const size_t elems = (size_t)2 * CT * 16 * wld(ks_of(CT) * 32) + (size_t)nth * 16 * wld(2 * CT * 16);
     const size_t hs = (size_t)nw_fwd<CT>() * 16 * nth * 16 * sizeof(float);
     const int nl = lds_planes(elems, hs);
     FFNO_DISPATCH_NL(nl, {
@@ -865,14 +905,18 @@ int ffno_tw_gemm(const FfnoTwGemm* g, hipStream_t stream) {
     const int mt = (g->m_pad + 15) / 16;
     const size_t lds = (size_t)(g->k_pad + mt * 16) * sizeof(int64_t) +
                        (size_t)(three ? 3 : 2) * mt * 16 * (g->k_pad + TW_PAD) * sizeof(uint16_t);
-    const int grid = (int)(blocks < 512 ? blocks : 512);       // persistent: A is staged once per workgroup
+    // persistent: A is staged once per workgroup; 4 tiles and more take 32-column tasks (NC = 2)
+    const int nc = mt >= 4 ? 2 : 4;
+    const int64_t sblocks = ((int64_t)g->batch * ((g->n + 16 * nc - 1) / (16 * nc)) + 3) / 4;
+    const int grid = (int)(sblocks < 512 ? sblocks : 512);
     switch (mt) {
-#define FFNO_TW_CASE(M)                                        \
-  case M: {                                                    \
-    auto k = three ? k_tw_small<M, true> : k_tw_small<M, false>; \
-    if (set_lds(k, lds)) return -1;                            \
-    k<<<grid, 256, lds, stream>>>(*g);                         \
-    break;                                                     \
+#define FFNO_TW_CASE(M)                                                                                   \
+  case M: {                                                                                               \
+    constexpr int NC = M >= 4 ? 2 : 4;                                                                    \
+    auto k = three ? k_tw_small<M, true, NC> : k_tw_small<M, false, NC>;                                  \
+    if (set_lds(k, lds)) return -1;                                                                       \
+    k<<<grid, 256, lds, stream>>>(*g);                                                                    \
+    break;                                                                                                \
   }
       FFNO_TW_CASE(1) FFNO_TW_CASE(2) FFNO_TW_CASE(3) FFNO_TW_CASE(4)
       FFNO_TW_CASE(5) FFNO_TW_CASE(6) FFNO_TW_CASE(7) FFNO_TW_CASE(8)
@@ -913,7 +957,7 @@ int ffno_mix(const FfnoMix* m, hipStream_t stream) {
   const int grid = (int)(blocks < 2048 ? blocks : 2048);
   if (m->parts != 2 && m->parts != 3) return fail("ffno_mix: parts must be 2 or 3");
   FFNO_DISPATCH_CT(m->C, {
-    const size_t elems = (size_t)2 * CT * 16 * 2 * ks_of(CT) * 32;
+    const size_t elems = (size_t)2 * CT * 16 * wld(2 * ks_of(CT) * 32);
     if (m->parts == 3) {
       FFNO_DISPATCH_NL(lds_planes(elems, 0), {
         auto k = k_mix<CT, NL, true>;

@@ -23,7 +23,7 @@ __device__ __forceinline__ void mlp2_t(Wm W1, Wm W2, const Frag* in, const Frag*
     const Frag df = split8(d0, d1);
 #pragma unroll
     for (int t = 0; t < NT2; ++t) {
-      const int o = (16 * t + v) * (KS2 * 32) + 32 * s + 8 * q;
+      const int o = (16 * t + v) * wld(KS2 * 32) + 32 * s + 8 * q;
       out[t] = mfma3(*(const bf16x8*)(W2.h + o), *(const bf16x8*)(W2.l + o), df, out[t]);
     }
   }
@@ -199,7 +199,7 @@ __global__ __launch_bounds__(NW * 64) void k_gates_t(FfnoRows rows, const float*
                                                      const float* __restrict__ hd, const uint16_t* g1,
                                                      const uint16_t* g2, const uint16_t* v1, const uint16_t* v2,
                                                      float* __restrict__ xsd, float* __restrict__ xvd) {
-  constexpr int C = CT * 16, KS = ks_of(CT), SZ = C * KS * 32;
+  constexpr int C = CT * 16, KS = ks_of(CT), SZ = C * wld(KS * 32);
   extern __shared__ __align__(16) unsigned char smem[];
   bf16* lds = reinterpret_cast<bf16*>(smem);
   const Wm G1 = LW ? stage_w(lds, g1, SZ) : global_w(g1, SZ);
@@ -245,8 +245,8 @@ __global__ __launch_bounds__(NW * 64) void k_mlp_t(FfnoRows rows, FfnoAssemble i
   constexpr int C = CT * 16, KS1 = ks_of(CT);
   extern __shared__ __align__(16) unsigned char smem[];
   bf16* lds = reinterpret_cast<bf16*>(smem);
-  const Wm W1 = LW ? stage_w(lds, w1, 2 * C * KS1 * 32) : global_w(w1, 2 * C * KS1 * 32);
-  const Wm W2 = LW ? stage_w(lds, w2, C * 2 * C) : global_w(w2, C * 2 * C);
+  const Wm W1 = LW ? stage_w(lds, w1, 2 * C * wld(KS1 * 32)) : global_w(w1, 2 * C * wld(KS1 * 32));
+  const Wm W2 = LW ? stage_w(lds, w2, C * wld(2 * C)) : global_w(w2, C * wld(2 * C));
   __syncthreads();
   Lane L;
   const int b = blockIdx.y;
@@ -285,8 +285,8 @@ __global__ __launch_bounds__(NW * 64) void k_fuse_t(FfnoRows rows, const float* 
                                                     const uint16_t* bg1, const uint16_t* bg2, const uint16_t* f1,
                                                     const uint16_t* f2, float* __restrict__ fpd) {
   constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32;
-  const Wm B1 = global_w(bg1, C * 3 * KP), B2 = global_w(bg2, 2 * C * KP);
-  const Wm F1 = global_w(f1, 2 * C * 2 * KP), F2 = global_w(f2, C * 2 * C);
+  const Wm B1 = global_w(bg1, C * wld(3 * KP)), B2 = global_w(bg2, 2 * C * wld(KP));
+  const Wm F1 = global_w(f1, 2 * C * wld(2 * KP)), F2 = global_w(f2, C * wld(2 * C));
   Lane L;
   const int b = blockIdx.y;
   const int64_t nt = (rows.voxels + 15) / 16;
@@ -361,8 +361,8 @@ __global__ __launch_bounds__(NW * 64) void k_head_t(FfnoRows rows, FfnoAssemble 
   constexpr int C = CT * 16, KS1 = ks_of(CT);
   extern __shared__ __align__(16) unsigned char smem[];
   bf16* lds = reinterpret_cast<bf16*>(smem);
-  const Wm P1 = LW ? stage_w(lds, p1, 2 * C * KS1 * 32) : global_w(p1, 2 * C * KS1 * 32);
-  const Wm P2 = LW ? stage_w(lds, p2, NTH * 16 * 2 * C) : global_w(p2, NTH * 16 * 2 * C);
+  const Wm P1 = LW ? stage_w(lds, p1, 2 * C * wld(KS1 * 32)) : global_w(p1, 2 * C * wld(KS1 * 32));
+  const Wm P2 = LW ? stage_w(lds, p2, NTH * 16 * wld(2 * C)) : global_w(p2, NTH * 16 * wld(2 * C));
   float* hs = reinterpret_cast<float*>(lds);   // [NW][16 voxels][2][NTH*16]: head, then its tangent
   __syncthreads();
   Lane L;
@@ -488,7 +488,7 @@ int ffno_gates_t(FfnoRows rows, const float* h, const float* hd, const uint16_t*
   if (!rows_ok(rows) || !h || !hd || !g1 || !g2 || !v1 || !v2 || !xsd || !xvd) return fail("ffno_gates_t: bad arguments");
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {
-    const size_t lds = 4 * (size_t)(CT * 16) * ks_of(CT) * 32 * 2 * 2;
+    const size_t lds = 4 * (size_t)(CT * 16) * wld(ks_of(CT) * 32) * 2 * 2;
     if (lds <= 160 * 1024) {
       auto k = k_gates_t<CT, nw_of<CT>(), true>;
       if (set_lds(k, lds)) return -1;
@@ -506,7 +506,7 @@ int ffno_mlp_t(FfnoRows rows, FfnoAssemble in, const float* xd, const uint16_t* 
     return fail("ffno_mlp_t: bad arguments");
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {
-    const size_t lds = ((size_t)2 * CT * 16 * ks_of(CT) * 32 + (size_t)CT * 16 * 2 * CT * 16) * 2 * 2;
+    const size_t lds = ((size_t)2 * CT * 16 * wld(ks_of(CT) * 32) + (size_t)CT * 16 * wld(2 * CT * 16)) * 2 * 2;
     if (lds <= 160 * 1024) {
       auto k = k_mlp_t<CT, nw_of<CT>(), true>;
       if (set_lds(k, lds)) return -1;
@@ -546,7 +546,7 @@ int ffno_head_t(FfnoRows rows, FfnoAssemble in, const float* xd, const uint16_t*
   const int nth = (n_head + 15) / 16;
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {
-    const size_t w_bytes = ((size_t)2 * CT * 16 * ks_of(CT) * 32 + (size_t)nth * 16 * 2 * CT * 16) * 2 * 2;
+    const size_t w_bytes = ((size_t)2 * CT * 16 * wld(ks_of(CT) * 32) + (size_t)nth * 16 * wld(2 * CT * 16)) * 2 * 2;
     const size_t hs = (size_t)nw_of<CT>() * 16 * 2 * nth * 16 * sizeof(float);
     const bool lw = w_bytes + hs <= 160 * 1024;
     const size_t lds = (lw ? w_bytes : 0) + hs;

@@ -22,6 +22,7 @@ from ._lib import FfnoAssemble, FfnoHeadDesc, FfnoMix, FfnoRows, FfnoStatsOut, F
 SUPPORTED_WIDTHS = (16, 32, 64, 96, 128)
 GROUPS = 8
 NW = 4                       # waves per workgroup of the voxel-row kernels
+WPAD = 8                     # zero bf16 after every packed weight row (the kernels' row stride is K_pad + WPAD)
 TABLE_CACHE_BYTES = 8 << 30  # a branch's per-mode tables of all blocks stay cached when at most this large
 MODE_CACHE_BYTES = 24 << 30  # all cached per-mode tables of an engine (least recently used sets are dropped)
 H_CHUNK_BYTES = 2 << 30      # horizontal half-spectrum scratch per buffer: (b, d) planes are transformed in chunks
@@ -45,9 +46,11 @@ def _p(t):
 
 
 def pack_mlp_weight(w, segments=None, n_pad=None, parts=2):
-    """Split-bf16 [.., parts, N_pad, K_pad] for the MFMA kernels from a float weight [.., N, K] (PyTorch [out, in]):
-    each input segment padded to a multiple of 32, k permuted within every 32-block as the kernels' lane layout wants
-    (k = 32s + 16h + 4q + j is stored at 32s + 8q + 4h + j), N padded to a multiple of 16. Planes: hi = bf16(w),
+    """Split-bf16 [.., parts, N_pad, K_pad + 8] for the MFMA kernels from a float weight [.., N, K] (PyTorch
+    [out, in]): each input segment padded to a multiple of 32, k permuted within every 32-block as the kernels' lane
+    layout wants (k = 32s + 16h + 4q + j is stored at 32s + 8q + 4h + j), N padded to a multiple of 16, and every
+    row followed by 8 zeros (the kernels' row stride K_pad + 8 keeps the 16 rows of an LDS fragment read on distinct
+    banks). Planes: hi = bf16(w),
     lo = bf16(w - hi), and with parts=3 a third, bf16(w - hi - lo), for the forward's three-way products (the first
     two planes are then still the two-way split the adjoint kernels read)."""
     w = w.double()
@@ -65,6 +68,7 @@ def pack_mlp_weight(w, segments=None, n_pad=None, parts=2):
     wp = F.pad(wp, (0, 0, 0, npad - N))
     kp = wp.shape[-1]
     wp = wp.reshape(*lead, npad, kp // 32, 2, 4, 4).transpose(-3, -2).reshape(*lead, npad, kp)
+    wp = F.pad(wp, (0, WPAD))
     planes = []
     rest = wp
     for _ in range(parts):

@@ -81,7 +81,9 @@ def test_packing_matches_kernel_contraction(N, segments):
     K = sum(segments)
     w = torch.from_numpy(rng.standard_normal((N, K))).float()
     packed = pack_mlp_weight(w, segments).view(torch.bfloat16).double()
-    assert packed.shape[0] == 2
+    kp = sum(T.round_up(s, 32) for s in segments)
+    assert packed.shape == (2, T.round_up(N, 16), kp + 8)
+    assert not packed[..., kp:].any()                          # the row padding the kernels' stride skips
     wp = (packed[0] + packed[1]).numpy()                       # hi + lo
     act_parts = [rng.standard_normal(s) for s in segments]
     act = np.concatenate([np.pad(a, (0, T.round_up(len(a), 32) - len(a))) for a in act_parts])
