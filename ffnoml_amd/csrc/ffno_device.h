@@ -231,6 +231,48 @@ __device__ __forceinline__ void to_frags(const f32x4 (&a)[NT], Frag3* f) {
   for (int s = 0; s < ks_of(NT); ++s) f[s] = split8_3(a[2 * s], (2 * s + 1 < NT) ? a[2 * s + 1] : zero4());
 }
 
+// mlp2 for two voxel tiles at once: every weight fragment read from LDS feeds both tiles' MFMAs, which halves the
+// LDS traffic per voxel and gives each wait on a fragment twice the independent MFMA work
+template <int KS1, int NH, int NT2>
+__device__ __forceinline__ void mlp2x2(Wm3 W1, Wm3 W2, const Frag3* in0, const Frag3* in1, f32x4 (&out0)[NT2],
+                                       f32x4 (&out1)[NT2], int v, int q) {
+  constexpr int KS2 = ks_of(NH);
+#pragma unroll
+  for (int t = 0; t < NT2; ++t) {
+    out0[t] = zero4();
+    out1[t] = zero4();
+  }
+#pragma unroll 1
+  for (int s = 0; s < KS2; ++s) {
+    f32x4 a0 = zero4(), a1 = zero4(), b0 = zero4(), b1 = zero4();   // hidden tiles 2s (a) and 2s + 1 (b)
+#pragma unroll
+    for (int k = 0; k < KS1; ++k) {
+      const int oa = (32 * s + v) * wld(KS1 * 32) + 32 * k + 8 * q;
+      const bf16x8 ah = *(const bf16x8*)(W1.h + oa), am = *(const bf16x8*)(W1.m + oa);
+      const bf16x8 al = *(const bf16x8*)(W1.l + oa);
+      a0 = mfma6(ah, am, al, in0[k], a0);
+      a1 = mfma6(ah, am, al, in1[k], a1);
+      if (2 * s + 1 < NH) {
+        const int ob = oa + 16 * wld(KS1 * 32);
+        const bf16x8 bh = *(const bf16x8*)(W1.h + ob), bm = *(const bf16x8*)(W1.m + ob);
+        const bf16x8 bl = *(const bf16x8*)(W1.l + ob);
+        b0 = mfma6(bh, bm, bl, in0[k], b0);
+        b1 = mfma6(bh, bm, bl, in1[k], b1);
+      }
+    }
+    const Frag3 hf0 = split8_3(gelu4(a0), gelu4(b0));
+    const Frag3 hf1 = split8_3(gelu4(a1), gelu4(b1));
+#pragma unroll
+    for (int t = 0; t < NT2; ++t) {
+      const int o = (16 * t + v) * wld(KS2 * 32) + 32 * s + 8 * q;
+      const bf16x8 wh = *(const bf16x8*)(W2.h + o), wm = *(const bf16x8*)(W2.m + o);
+      const bf16x8 wl = *(const bf16x8*)(W2.l + o);
+      out0[t] = mfma6(wh, wm, wl, hf0, out0[t]);
+      out1[t] = mfma6(wh, wm, wl, hf1, out1[t]);
+    }
+  }
+}
+
 template <int KS>
 __device__ __forceinline__ f32x4 tile(Wm3 W, int t, const Frag3* b, int v, int q) {
   f32x4 c = zero4();

@@ -174,18 +174,28 @@ __global__ __launch_bounds__(NW * 64) void k_post(FfnoRows rows, const float* u,
   const int64_t nt = (rows.voxels + 15) / 16;
   GnAcc<CT> acc;
   acc.init();
-  for (int64_t tile = (int64_t)blockIdx.x * NW + L.wave; tile < nt; tile += (int64_t)gridDim.x * NW) {
-    const int64_t vox = tile * 16 + L.v;
-    const bool ok = vox < rows.voxels;
-    const int64_t row = (int64_t)b * rows.voxels + vox;
-    f32x4 x[CT];
-    load_act<CT>(u, row, ok, L.q, x);
-    Frag3 xf[KS1];
-    to_frags<CT>(x, xf);
-    f32x4 o[CT];
-    mlp2<KS1, 2 * CT, CT>(W1, W2, xf, o, L.v, L.q);
-    store_act<CT>(y, row, ok, L.q, o);
-    if (st.partials) acc.add(o, ok ? st.depth_w[row] : 0.f, ok);
+  // two voxel tiles (2p, 2p + 1) per wave and step: every weight fragment read serves both
+  const int64_t np = (nt + 1) / 2;
+  for (int64_t p = (int64_t)blockIdx.x * NW + L.wave; p < np; p += (int64_t)gridDim.x * NW) {
+    const int64_t vox0 = p * 32 + L.v, vox1 = vox0 + 16;
+    const bool ok0 = vox0 < rows.voxels, ok1 = vox1 < rows.voxels;
+    const int64_t row0 = (int64_t)b * rows.voxels + vox0, row1 = row0 + 16;
+    Frag3 xf0[KS1], xf1[KS1];
+    {
+      f32x4 x[CT];
+      load_act<CT>(u, row0, ok0, L.q, x);
+      to_frags<CT>(x, xf0);
+      load_act<CT>(u, row1, ok1, L.q, x);
+      to_frags<CT>(x, xf1);
+    }
+    f32x4 o0[CT], o1[CT];
+    mlp2x2<KS1, 2 * CT, CT>(W1, W2, xf0, xf1, o0, o1, L.v, L.q);
+    store_act<CT>(y, row0, ok0, L.q, o0);
+    store_act<CT>(y, row1, ok1, L.q, o1);
+    if (st.partials) {
+      acc.add(o0, ok0 ? st.depth_w[row0] : 0.f, ok0);
+      acc.add(o1, ok1 ? st.depth_w[row1] : 0.f, ok1);
+    }
   }
   if (st.partials)
     acc.template flush<NW>(red, st.partials + ((int64_t)b * gridDim.x + blockIdx.x) * G * 3, L.v, L.q, L.wave);
@@ -294,19 +304,30 @@ __global__ __launch_bounds__(NW * 64) void k_resmlp(FfnoRows rows, FfnoAssemble 
   const int64_t nt = (rows.voxels + 15) / 16;
   GnAcc<CT> acc;
   acc.init();
-  for (int64_t tile = (int64_t)blockIdx.x * NW + L.wave; tile < nt; tile += (int64_t)gridDim.x * NW) {
-    const int64_t vox = tile * 16 + L.v;
-    const bool ok = vox < rows.voxels;
-    const int64_t row = (int64_t)b * rows.voxels + vox;
-    f32x4 x[CT];
-    assemble<CT>(in, scale, row, b, ok, L.q, x);
-    if (x_out) store_act<CT>(x_out, row, ok, L.q, x);
-    Frag3 xf[KS1];
-    to_frags<CT>(x, xf);
-    f32x4 o[CT];
-    mlp2<KS1, 2 * CT, CT>(W1, W2, xf, o, L.v, L.q);
-    store_act<CT>(y, row, ok, L.q, o);
-    if (st.partials) acc.add(o, ok ? st.depth_w[row] : 0.f, ok);
+  // two voxel tiles (2p, 2p + 1) per wave and step: every weight fragment read serves both
+  const int64_t np = (nt + 1) / 2;
+  for (int64_t p = (int64_t)blockIdx.x * NW + L.wave; p < np; p += (int64_t)gridDim.x * NW) {
+    const int64_t vox0 = p * 32 + L.v, vox1 = vox0 + 16;
+    const bool ok0 = vox0 < rows.voxels, ok1 = vox1 < rows.voxels;
+    const int64_t row0 = (int64_t)b * rows.voxels + vox0, row1 = row0 + 16;
+    Frag3 xf0[KS1], xf1[KS1];
+    {
+      f32x4 x[CT];
+      assemble<CT>(in, scale, row0, b, ok0, L.q, x);
+      if (x_out) store_act<CT>(x_out, row0, ok0, L.q, x);
+      to_frags<CT>(x, xf0);
+      assemble<CT>(in, scale, row1, b, ok1, L.q, x);
+      if (x_out) store_act<CT>(x_out, row1, ok1, L.q, x);
+      to_frags<CT>(x, xf1);
+    }
+    f32x4 o0[CT], o1[CT];
+    mlp2x2<KS1, 2 * CT, CT>(W1, W2, xf0, xf1, o0, o1, L.v, L.q);
+    store_act<CT>(y, row0, ok0, L.q, o0);
+    store_act<CT>(y, row1, ok1, L.q, o1);
+    if (st.partials) {
+      acc.add(o0, ok0 ? st.depth_w[row0] : 0.f, ok0);
+      acc.add(o1, ok1 ? st.depth_w[row1] : 0.f, ok1);
+    }
   }
   if (st.partials)
     acc.template flush<NW>(red, st.partials + ((int64_t)b * gridDim.x + blockIdx.x) * G * 3, L.v, L.q, L.wave);
@@ -575,9 +596,13 @@ __global__ __launch_bounds__(256) void k_col_gemm(FfnoColGemm g) {
 // k-step's input loaded while the current one's MFMAs run. NC = 2 halves the accumulators of the larger tiles so two
 // waves fit each SIMD.
 constexpr int TW_PAD = 8;
+// column groups per lane of the small twiddle GEMM: 2 (32-column tasks, two waves per SIMD) from 4 m-tiles up, except
+// the three-way 8-tile case, whose 64-column tasks at one wave per SIMD measured faster (462 against 548 us on the
+// 128-row vertical inverse of a 128 x 64 x 64 x 64 batch of 8)
+constexpr int tw_cols(int mt, bool three) { return mt >= 4 && !(mt == 8 && three) ? 2 : 4; }
 
 template <int MT, bool THREE, int NC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_tw_small(FfnoTwGemm g) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NC == 2 ? 2 : 1))) void k_tw_small(FfnoTwGemm g) {
   typedef typename ColVec<NC>::T V;
   extern __shared__ __align__(16) unsigned char smem[];
   const int kp = g.k_pad, ldk = kp + TW_PAD;
@@ -801,14 +826,15 @@ int ffno_post(FfnoRows rows, const float* u, const uint16_t* w1, const uint16_t*
   if (!rows_ok(rows) || !u || !w1 || !w2 || !y) return fail("ffno_post: bad arguments");
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {
+    constexpr int NWP = nw_bwd<CT>();   // 12 waves at CT 4: two tiles per wave need a 3-wave SIMD's registers
     const size_t elems = (size_t)2 * CT * 16 * wld(ks_of(CT) * 32) + (size_t)CT * 16 * wld(2 * CT * 16);
-    const size_t red = red_bytes_nw<CT, nw_fwd<CT>()>();
+    const size_t red = red_bytes_nw<CT, NWP>();
     const int nl = lds_planes(elems, red);
     FFNO_DISPATCH_NL(nl, {
       const size_t lds = NL * elems * 2 + red;
-      auto k = k_post<CT, nw_fwd<CT>(), NL>;
+      auto k = k_post<CT, NWP, NL>;
       if (set_lds(k, lds)) return -1;
-      k<<<grid, nw_fwd<CT>() * 64, lds, stream>>>(rows, u, w1, w2, y, stats);
+      k<<<grid, NWP * 64, lds, stream>>>(rows, u, w1, w2, y, stats);
     });
   });
   return check_launch("ffno_post");
@@ -847,13 +873,15 @@ int ffno_resmlp(FfnoRows rows, FfnoAssemble in, float* x_out, const uint16_t* w1
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {
     const size_t elems = (size_t)2 * CT * 16 * wld(ks_of(CT) * 32) + (size_t)CT * 16 * wld(2 * CT * 16);
-    const size_t red = red_bytes_nw<CT, nw_fwd<CT>()>();
+    // 12 waves at CT 4: the next tile's two input rows held in flight need the registers of a 3-wave SIMD
+    constexpr int NWR = nw_bwd<CT>();
+    const size_t red = red_bytes_nw<CT, NWR>();
     const int nl = lds_planes(elems, red);
     FFNO_DISPATCH_NL(nl, {
       const size_t lds = NL * elems * 2 + red;
-      auto k = k_resmlp<CT, nw_fwd<CT>(), NL>;
+      auto k = k_resmlp<CT, NWR, NL>;
       if (set_lds(k, lds)) return -1;
-      k<<<grid, nw_fwd<CT>() * 64, lds, stream>>>(rows, in, x_out, w1, w2, y, stats);
+      k<<<grid, NWR * 64, lds, stream>>>(rows, in, x_out, w1, w2, y, stats);
     });
   });
   return check_launch("ffno_resmlp");
@@ -906,14 +934,13 @@ int ffno_tw_gemm(const FfnoTwGemm* g, hipStream_t stream) {
     const size_t lds = (size_t)(g->k_pad + mt * 16) * sizeof(int64_t) +
                        (size_t)(three ? 3 : 2) * mt * 16 * (g->k_pad + TW_PAD) * sizeof(uint16_t);
     // persistent: A is staged once per workgroup; 4 tiles and more take 32-column tasks (NC = 2)
-    const int nc = mt >= 4 ? 2 : 4;
+    const int nc = tw_cols(mt, three);
     const int64_t sblocks = ((int64_t)g->batch * ((g->n + 16 * nc - 1) / (16 * nc)) + 3) / 4;
     const int grid = (int)(sblocks < 512 ? sblocks : 512);
     switch (mt) {
 #define FFNO_TW_CASE(M)                                                                                   \
   case M: {                                                                                               \
-    constexpr int NC = M >= 4 ? 2 : 4;                                                                    \
-    auto k = three ? k_tw_small<M, true, NC> : k_tw_small<M, false, NC>;                                  \
+    auto k = three ? k_tw_small<M, true, tw_cols(M, true)> : k_tw_small<M, false, tw_cols(M, false)>;    \
     if (set_lds(k, lds)) return -1;                                                                       \
     k<<<grid, 256, lds, stream>>>(*g);                                                                    \
     break;                                                                                                \
