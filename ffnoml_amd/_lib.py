@@ -79,7 +79,8 @@ SIGNATURES = {
     "ffno_col_gemm": [ctypes.POINTER(FfnoColGemm), c_ptr],
     "ffno_gn_grad_stats": [FfnoRows, FfnoGnBack, c_ptr, c_ptr, c_ptr],
     "ffno_gn_grad_finalize": [ctypes.c_int, ctypes.c_int, c_ptr, c_ptr, c_ptr],
-    "ffno_mlp_bwd": [FfnoRows, c_ptr, FfnoGnBack, FfnoAssemble, c_ptr, c_ptr, c_ptr, ctypes.c_int, c_ptr, c_ptr],
+    "ffno_mlp_bwd": [FfnoRows, c_ptr, FfnoGnBack, FfnoAssemble, c_ptr, c_ptr, c_ptr, ctypes.c_int, c_ptr, c_ptr,
+                     c_ptr, c_ptr],
     "ffno_fuse_bwd": [FfnoRows, c_ptr, FfnoGnBack] + [c_ptr] * 18 + [c_ptr],
     "ffno_fuse_mlp_bwd": [FfnoRows, c_ptr, FfnoGnBack] + [c_ptr] * 12 + [c_ptr],
     "ffno_fuse_gates_bwd": [FfnoRows] + [c_ptr] * 14 + [c_ptr],

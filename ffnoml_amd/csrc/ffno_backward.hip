@@ -75,6 +75,90 @@ __device__ __forceinline__ void gn_back(const FfnoGnBack& gb, float scale, int64
   }
 }
 
+// sum over the 16 lanes of a row (the 16 voxels of a lane group): four DPP adds, xor 1, xor 2 (quad permutes), then
+// the half-row and row mirrors; every lane of the row ends with the row's sum
+__device__ __forceinline__ float row16_sum(float x) {
+  x += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xB1, 0xF, 0xF, true));    // quad_perm 1,0,3,2
+  x += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x4E, 0xF, 0xF, true));    // quad_perm 2,3,0,1
+  x += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x141, 0xF, 0xF, true));   // row_half_mirror
+  x += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x140, 0xF, 0xF, true));   // row_mirror
+  return x;
+}
+
+// S1 = sum g_hat and S2 = sum g_hat x_hat of one GroupNorm's backward, accumulated by a kernel that produces that
+// GroupNorm's u_bar (so no separate pass re-reads it). Each tile's sums are reduced over the 16 voxel lanes at once and
+// added to the wave's own LDS slot [wave][lane group][channel group of the quad][2] (nothing held in registers across
+// tiles); flush() merges the slots into the workgroup's partials [8][2] in the layout ffno_gn_grad_finalize reduces.
+// Same g_hat, x_hat as k_gn_grad_stats. Worth it for a GroupNorm without activation: with a GELU after it, g_hat
+// needs the GroupNorm input and a GELU derivative per channel, which cost the producer more than the separate pass.
+template <int CT>
+struct GradSums {
+  static constexpr int CPG = CT * 16 / G;
+  static constexpr int SUB = CPG >= 4 ? 1 : 4 / CPG;   // channel groups per 4-channel quad
+  static constexpr int SLOT = CT * SUB * 2;            // floats per (wave, lane group)
+  float* slot;                                         // this wave's and lane group's slot in LDS
+
+  // every thread of the workgroup calls this once, before its first tile; red holds NW * 4 * SLOT floats
+  template <int NW>
+  __device__ void init(float* red, int q, int wave) {
+    for (int i = threadIdx.x; i < NW * 4 * SLOT; i += blockDim.x) red[i] = 0.f;
+    slot = red + (wave * 4 + q) * SLOT;
+    __syncthreads();
+  }
+  __device__ __forceinline__ void add(const FfnoGnBack& gb, float scale, int64_t row, int b, bool ok, int q, int v,
+                                      const f32x4 (&ub)[CT]) {
+    float a[CT * SUB], c[CT * SUB];
+#pragma unroll
+    for (int t = 0; t < CT; ++t) {
+      f32x4 gh, xh;
+      gn_ghat<CT>(gb, scale, row, b, ok, q, t, ub[t], gh, xh);
+#pragma unroll
+      for (int u = 0; u < SUB; ++u) {
+        a[t * SUB + u] = 0.f;
+        c[t * SUB + u] = 0.f;
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        a[t * SUB + j / (4 / SUB)] += gh[j];
+        c[t * SUB + j / (4 / SUB)] += gh[j] * xh[j];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < CT * SUB; ++i) {
+      a[i] = row16_sum(a[i]);
+      c[i] = row16_sum(c[i]);
+    }
+    if (v == 0)
+#pragma unroll
+      for (int i = 0; i < CT * SUB; ++i) {
+        slot[2 * i] += a[i];
+        slot[2 * i + 1] += c[i];
+      }
+  }
+  // every thread of the workgroup calls this once, after its last tile
+  template <int NW>
+  __device__ void flush(const float* red, float* partials) {
+    __syncthreads();
+    if (threadIdx.x < G) {
+      const int g = threadIdx.x;
+      float s1 = 0.f, s2 = 0.f;
+      for (int w = 0; w < NW; ++w)
+        for (int qq = 0; qq < 4; ++qq)
+          for (int i = 0; i < CT * SUB; ++i) {
+            const int ch = 16 * (i / SUB) + 4 * qq + (i % SUB) * CPG;
+            if (ch / CPG != g) continue;
+            s1 += red[(w * 4 + qq) * SLOT + 2 * i];
+            s2 += red[(w * 4 + qq) * SLOT + 2 * i + 1];
+          }
+      partials[g * 2 + 0] = s1;
+      partials[g * 2 + 1] = s2;
+    }
+  }
+};
+
+template <int CT, int NW>
+constexpr size_t grad_sums_bytes() { return (size_t)NW * 4 * GradSums<CT>::SLOT * sizeof(float); }
+
 template <int CT, int NW>
 __global__ __launch_bounds__(NW * 64) void k_gn_grad_stats(FfnoRows rows, FfnoGnBack gb, const float* __restrict__ ubar,
                                                            float* __restrict__ partials) {
@@ -167,7 +251,8 @@ __global__ __launch_bounds__(256) void k_gn_grad_finalize(int nwg, const float* 
 template <int CT, int NW, bool LW>
 __global__ __launch_bounds__(NW * 64) void k_mlp_bwd(FfnoRows rows, const float* ubar, FfnoGnBack gb, FfnoAssemble xin,
                                                      const uint16_t* w1, const uint16_t* w2t, const uint16_t* w1t,
-                                                     int residual, float* xbar) {
+                                                     int residual, float* xbar, FfnoGnBack next,
+                                                     float* next_partials) {
   constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32;
   extern __shared__ __align__(16) unsigned char smem[];
   bf16* lds = reinterpret_cast<bf16*>(smem);
@@ -179,6 +264,10 @@ __global__ __launch_bounds__(NW * 64) void k_mlp_bwd(FfnoRows rows, const float*
   const int b = blockIdx.y;
   const float scale = gb.scale ? *gb.scale : 1.f;
   const float xscale = xin.scale ? *xin.scale : 1.f;
+  const float nscale = next_partials && next.scale ? *next.scale : 1.f;
+  GradSums<CT> sums;
+  float* red = reinterpret_cast<float*>(lds);          // after the staged weights
+  if (next_partials) sums.template init<NW>(red, L.q, L.wave);
   const int64_t nt = (rows.voxels + 15) / 16;
   for (int64_t tile = (int64_t)blockIdx.x * NW + L.wave; tile < nt; tile += (int64_t)gridDim.x * NW) {
     const int64_t vox = tile * 16 + L.v;
@@ -199,7 +288,9 @@ __global__ __launch_bounds__(NW * 64) void k_mlp_bwd(FfnoRows rows, const float*
     for (int t = 0; t < CT; ++t) acc[t] = residual ? ub[t] : zero4();
     mlp2_bwd<KS, 2 * CT, KS, CT>(W1, W2T, W1T, xf, of, acc, L.v, L.q);
     store_act<CT>(xbar, row, ok, L.q, acc);
+    if (next_partials) sums.add(next, nscale, row, b, ok, L.q, L.v, acc);
   }
+  if (next_partials) sums.template flush<NW>(red, next_partials + ((int64_t)b * gridDim.x + blockIdx.x) * G * 2);
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -726,21 +817,26 @@ int ffno_gn_grad_finalize(int B, int nwg, const float* partials, float* sums, hi
 }
 
 int ffno_mlp_bwd(FfnoRows rows, const float* ubar, FfnoGnBack gn, FfnoAssemble x, const uint16_t* w1,
-                 const uint16_t* w2t, const uint16_t* w1t, int residual, float* xbar, hipStream_t stream) {
+                 const uint16_t* w2t, const uint16_t* w1t, int residual, float* xbar, const FfnoGnBack* next,
+                 float* next_partials, hipStream_t stream) {
   if (!rows_ok(rows) || !gb_ok(gn) || !gn.sums || !ubar || (!x.base && !x.t) || (x.t && (!x.aff_a || !x.aff_b)) ||
-      !w1 || !w2t || !w1t || !xbar)
+      !w1 || !w2t || !w1t || !xbar || (next_partials && (!next || !gb_ok(*next))))
     return fail("ffno_mlp_bwd: bad arguments");
+  const FfnoGnBack nx = next_partials ? *next : FfnoGnBack{};
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {
     constexpr int C = CT * 16, KP = ks_of(CT) * 32;
+    const size_t red = next_partials ? grad_sums_bytes<CT, nw_bwd<CT>()>() : 0;
     const size_t lds = ((size_t)2 * C * wld(KP) * 2 + (size_t)C * wld(2 * C)) * 2 * 2;
-    if (lds <= 160 * 1024) {
+    if (lds + red <= 160 * 1024) {
       auto k = k_mlp_bwd<CT, nw_bwd<CT>(), true>;
-      if (set_lds(k, lds)) return -1;
-      k<<<grid, nw_bwd<CT>() * 64, lds, stream>>>(rows, ubar, gn, x, w1, w2t, w1t, residual, xbar);
+      const size_t bytes = lds + red;
+      if (set_lds(k, bytes)) return -1;
+      k<<<grid, nw_bwd<CT>() * 64, bytes, stream>>>(rows, ubar, gn, x, w1, w2t, w1t, residual, xbar, nx, next_partials);
     } else {
-      k_mlp_bwd<CT, nw_bwd<CT>(), false><<<grid, nw_bwd<CT>() * 64, 0, stream>>>(rows, ubar, gn, x, w1, w2t, w1t,
-                                                                                residual, xbar);
+      auto k = k_mlp_bwd<CT, nw_bwd<CT>(), false>;
+      if (set_lds(k, red)) return -1;
+      k<<<grid, nw_bwd<CT>() * 64, red, stream>>>(rows, ubar, gn, x, w1, w2t, w1t, residual, xbar, nx, next_partials);
     }
   });
   return check_launch("ffno_mlp_bwd");

@@ -716,6 +716,7 @@ class Engine:
         if adjoint:
             scr["v1y"] = torch.empty_like(scr["v1"])
             scr["part2"] = torch.empty(plan.B, plan.nwg, GROUPS, 2, **f32)
+            scr["part3"] = torch.empty(plan.B, plan.nwg, GROUPS, 2, **f32)   # a second GroupNorm's in flight
         return scr
 
     def _block(self, plan, i, prev, x_out, h, hb, s, v, scr, bz, spectra="v1"):
@@ -956,10 +957,18 @@ class Engine:
 
     # ------------------------------------------------------------------------------------------------ adjoint
     def _gn_sums(self, plan, gb, ubar, part2):
-        sums = torch.empty(plan.B, GROUPS, 2, device=ubar.device)
+        """A GroupNorm backward's S1, S2 sums from u_bar in a pass of their own, into gb.sums; returns the sums
+        tensor, which the caller keeps alive until gb's consumer is enqueued."""
         _lib.call("ffno_gn_grad_stats", plan.rows, gb, _p(ubar), _p(part2), self._stream())
+        return self._gn_finalize_sums(plan, gb, part2)
+
+    def _gn_finalize_sums(self, plan, gb, part2):
+        """gb.sums from the per-workgroup partials a kernel accumulated; returns the sums tensor (kept alive by the
+        caller until gb's consumer is enqueued)."""
+        sums = torch.empty(plan.B, GROUPS, 2, device=part2.device)
         _lib.call("ffno_gn_grad_finalize", plan.B, plan.nwg, _p(part2), _p(sums), self._stream())
         gb.sums = _p(sums)
+        return sums
         return sums
 
     def vjp(self, x, z, dx, dy, ybar, saved, want_z=False, plan=None):
@@ -1059,42 +1068,46 @@ class Engine:
                 _lib.call("ffno_head", rows, _asm(g, mp, ma, mb, blk["res_mlp"]), _p(g), _p(Wt.p1), _p(Wt.p2),
                           _p(Wt.p2_bias), _p(x), ctypes.byref(Wt.head), None, st)
                 head_bwd(g)
+            # The pw GroupNorm's gradient sums are accumulated by the mlp adjoint, which writes its u_bar (x2_bar);
+            # the others (behind a GELU, or fed from the block above) take a pass of their own. keep holds the sums
+            # tensors until their consumers are enqueued.
+            keep = []
+            gb_m = gnback(mp, gn["m"], blk["norm_mlp"], blk["res_mlp"], 0)
+            gb_p = gnback(pp, gn["p"], blk["norm_pw"], blk["res_pw"], 0)
+            gb_f = gnback(fp, gn["f"], blk["norm_fuse"], blk["res_fused"], 1)
+            gb_s = gnback(sp, gn["s"], blk["norm_spec"], None, 1)
+            gb_v = gnback(vp, gn["v"], blk["norm_vert"], None, 1)
             # mlp: out = x2 + res_mlp GN(mp), x2 = x1 + res_pw GN(pp)
-            gb = gnback(mp, gn["m"], blk["norm_mlp"], blk["res_mlp"], 0)
-            self._gn_sums(plan, gb, g, part2)
-            _lib.call("ffno_mlp_bwd", rows, _p(g), gb, _asm(x1, pp, pa, pb, blk["res_pw"]), _p(blk["m1"]),
-                      _p(adj["m2t"]), _p(adj["m1t"]), 1, _p(g), st)
+            keep.append(self._gn_sums(plan, gb_m, g, part2))
+            _lib.call("ffno_mlp_bwd", rows, _p(g), gb_m, _asm(x1, pp, pa, pb, blk["res_pw"]), _p(blk["m1"]),
+                      _p(adj["m2t"]), _p(adj["m1t"]), 1, _p(g), ctypes.byref(gb_p), _p(part2), st)
+            keep.append(self._gn_finalize_sums(plan, gb_p, part2))
             # pw: x2 = x1 + res_pw GN(pp)
-            gb = gnback(pp, gn["p"], blk["norm_pw"], blk["res_pw"], 0)
-            self._gn_sums(plan, gb, g, part2)
-            _lib.call("ffno_mlp_bwd", rows, _p(g), gb, _asm(x1), _p(blk["pw1"]), _p(adj["pw2t"]), _p(adj["pw1t"]), 1,
-                      _p(g), st)
+            _lib.call("ffno_mlp_bwd", rows, _p(g), gb_p, _asm(x1), _p(blk["pw1"]), _p(adj["pw2t"]), _p(adj["pw1t"]), 1,
+                      _p(g), None, None, st)
+            keep.append(self._gn_sums(plan, gb_f, g, part2))
             # fusion: x1 = h + res_fused gelu(GN(fp)); s_bar -> mp, v_bar -> pp
-            gb = gnback(fp, gn["f"], blk["norm_fuse"], blk["res_fused"], 1)
-            self._gn_sums(plan, gb, g, part2)
             sa, sb_, _ = gn["s"]
             va, vb, _ = gn["v"]
             if gates is not None:
                 # two LDS-staged passes over the stored gates: [sm_bar, vm_bar] -> mp, pp; then in place
-                _lib.call("ffno_fuse_mlp_bwd", rows, _p(g), gb, _p(sp), _p(vp), _p(sa), _p(sb_), _p(va), _p(vb),
+                _lib.call("ffno_fuse_mlp_bwd", rows, _p(g), gb_f, _p(sp), _p(vp), _p(sa), _p(sb_), _p(va), _p(vb),
                           _p(gates), _p(blk["f1"]), _p(adj["f2t"]), _p(adj["f1t"]), _p(mp), _p(pp), st)
                 _lib.call("ffno_fuse_gates_bwd", rows, _p(g), _p(h), _p(sp), _p(vp), _p(sa), _p(sb_), _p(va), _p(vb),
                           _p(gates), _p(blk["bg1"]), _p(adj["bg2t"]), _p(adj["bg1t"]), _p(mp), _p(pp), st)
             else:
-                _lib.call("ffno_fuse_bwd", rows, _p(g), gb, _p(h), _p(sp), _p(vp), _p(sa), _p(sb_), _p(va), _p(vb),
+                _lib.call("ffno_fuse_bwd", rows, _p(g), gb_f, _p(h), _p(sp), _p(vp), _p(sa), _p(sb_), _p(va), _p(vb),
                           _p(blk["bg1"]), _p(blk["bg2"]), _p(blk["bg2_bias"]), _p(blk["f1"]), _p(adj["f2t"]),
                           _p(adj["f1t"]), _p(adj["bg2t"]), _p(adj["bg1t"]), _p(g), _p(mp), _p(pp), st)
+            keep.append(self._gn_sums(plan, gb_s, mp, part2))
+            keep.append(self._gn_sums(plan, gb_v, pp, scr["part3"]))
             # horizontal branch: spec = gelu(GN_s(post(Spec(xs)))); ys_bar -> s, xs_bar -> mp
-            gb = gnback(sp, gn["s"], blk["norm_spec"], None, 1)
-            self._gn_sums(plan, gb, mp, part2)
-            _lib.call("ffno_mlp_bwd", rows, _p(mp), gb, _asm(s), _p(blk["spec.p1"]), _p(adj["spec.p2t"]),
-                      _p(adj["spec.p1t"]), 0, _p(s), st)
+            _lib.call("ffno_mlp_bwd", rows, _p(mp), gb_s, _asm(s), _p(blk["spec.p1"]), _p(adj["spec.p2t"]),
+                      _p(adj["spec.p1t"]), 0, _p(s), None, None, st)
             self.horizontal(plan, s, mp, i, scr, tables, adjoint=True)
             # vertical branch: yv_bar -> v, the spectrum gradient in v1, xv_bar -> pp
-            gb = gnback(vp, gn["v"], blk["norm_vert"], None, 1)
-            self._gn_sums(plan, gb, pp, part2)
-            _lib.call("ffno_mlp_bwd", rows, _p(pp), gb, _asm(v), _p(blk["vertical.p1"]), _p(adj["vertical.p2t"]),
-                      _p(adj["vertical.p1t"]), 0, _p(v), st)
+            _lib.call("ffno_mlp_bwd", rows, _p(pp), gb_v, _asm(v), _p(blk["vertical.p1"]), _p(adj["vertical.p2t"]),
+                      _p(adj["vertical.p1t"]), 0, _p(v), None, None, st)
             v1 = scr["v1"]
             if want_z and ht["v0"] is not None:
                 # taper: dL/df_m per column = sum_c,part (V0^T yv_bar) * Y, before the mix adjoint overwrites v1
@@ -1125,8 +1138,9 @@ class Engine:
         _lib.call("ffno_lift", rows, _p(x), _p(Wt.lift), _p(s), FfnoStatsOut(_p(plan.depth_w), _p(scr["part"])), st)
         g0 = self._gn(plan, scr["part"], Wt.lift_gn, dev)
         gb = gnback(s, g0, Wt.lift_gn, None, 1)
-        self._gn_sums(plan, gb, g, part2)
+        sums = self._gn_sums(plan, gb, g, part2)
         _lib.call("ffno_lift_bwd", rows, _p(g), gb, _p(Wt.lift), _p(xbar), st)
+        del sums
         if not want_z:
             return xbar, None
         return xbar, self._height_gradient(plan, z, zbar, zbar_s, wbar, dfac_col)

@@ -204,7 +204,12 @@ int ffno_gn_grad_finalize(int B, int nwg, const float* partials, float* sums, hi
  * The MLP input x is given as an assembly (x = base, or base + scale act(a t + b)), so a residual sum need not be
  * stored. x_bar may alias u_bar or x.base (each row is read before it is written). */
 int ffno_mlp_bwd(FfnoRows rows, const float* ubar, FfnoGnBack gn, FfnoAssemble x, const uint16_t* w1,
-                 const uint16_t* w2t, const uint16_t* w1t, int residual, float* xbar, hipStream_t stream);
+                 const uint16_t* w2t, const uint16_t* w1t, int residual, float* xbar, const FfnoGnBack* next,
+                 float* next_partials, hipStream_t stream);
+/* (next, next_partials): when next_partials is given, the kernel also accumulates the S1, S2 partial sums of the
+ * GroupNorm whose u_bar is this call's x_bar (next describes that GroupNorm; its sums field is not read) into
+ * next_partials [B][nwg][8][2], exactly what ffno_gn_grad_stats(rows, *next, xbar, next_partials) would write, so that
+ * pass is skipped (models/ffno_model.py:547-551: x2_bar of the mlp is u_bar of the pw GroupNorm). */
 
 /* Adjoint of the branch fusion (models/ffno_model.py:524-545): from x1_bar (gradient of x1 = x + res_fused * f)
  * to the gradients of the block input (including x1_bar's residual path), of s = gelu(GN_s(sp)) and of

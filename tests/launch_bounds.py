@@ -222,7 +222,15 @@ class Checker:
         self.span(n, "partials", part, B * nwg * 8 * 2 * F32)
         self.span(n, "sums", sums, B * 8 * 2 * F32)
 
-    def ffno_mlp_bwd(self, n, rows, ubar, gb, x, w1, w2t, w1t, residual, xbar, stream):
+    def next_sums(self, n, ref, part, rows):
+        """An optional fused GroupNorm-gradient-sums target: the next GroupNorm's descriptor and its partials."""
+        if part is None or not getattr(part, "value", part):
+            return
+        assert ref is not None, f"{n}: partials without a GroupNorm descriptor"
+        self.gnback(n, ref._obj, rows)
+        self.span(n, "next partials", part, rows.B * rows.nwg * 8 * 2 * F32)
+
+    def ffno_mlp_bwd(self, n, rows, ubar, gb, x, w1, w2t, w1t, residual, xbar, nxt, npart, stream):
         C = rows.C
         self.act(n, "ubar", ubar, rows)
         self.gnback(n, gb, rows)
@@ -232,6 +240,7 @@ class Checker:
         self.weight(n, "w2t", w2t, 2 * C, C)
         self.weight(n, "w1t", w1t, C, 2 * C)
         self.act(n, "xbar", xbar, rows)
+        self.next_sums(n, nxt, npart, rows)
 
     def ffno_fuse_bwd(self, n, rows, x1bar, gb, h, sp, vp, sa, sb, va, vb, bg1, bg2, bias, f1, f2t, f1t, bg2t, bg1t,
                       hbar, sbar, vbar, stream):
