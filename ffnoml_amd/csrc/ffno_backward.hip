@@ -13,13 +13,12 @@ namespace {
 template <int CT>
 __device__ __forceinline__ int gn_group(int c) { return c / ((CT * 16) / G); }
 
-// g_hat and x_hat of one lane's 4 channels of tile t
+// g_hat and x_hat of one lane's 4 channels of tile t, from u_bar and the GroupNorm input tv of those channels
 template <int CT>
-__device__ __forceinline__ void gn_ghat(const FfnoGnBack& gb, float scale, int64_t row, int b, bool ok, int q, int t,
-                                        f32x4 ub, f32x4& gh, f32x4& xh) {
+__device__ __forceinline__ void gn_ghat_of(const FfnoGnBack& gb, float scale, int b, bool ok, int q, int t, f32x4 ub,
+                                           f32x4 tv, f32x4& gh, f32x4& xh) {
   constexpr int C = CT * 16, CPG = C / G;
   const int c0 = 16 * t + 4 * q;
-  const f32x4 tv = ld4(gb.t + row * C + c0, ok);
   const f32x4 gm = *(const f32x4*)(gb.gamma + c0);
   f32x4 act = f32x4{1.f, 1.f, 1.f, 1.f};
   if (gb.act_gelu) {
@@ -43,6 +42,13 @@ __device__ __forceinline__ void gn_ghat(const FfnoGnBack& gb, float scale, int64
       gh[j] = ok ? scale * ub[j] * gm[j] * act[j] : 0.f;
     }
   }
+}
+
+// the same, loading tv from the GroupNorm input
+template <int CT>
+__device__ __forceinline__ void gn_ghat(const FfnoGnBack& gb, float scale, int64_t row, int b, bool ok, int q, int t,
+                                        f32x4 ub, f32x4& gh, f32x4& xh) {
+  gn_ghat_of<CT>(gb, scale, b, ok, q, t, ub, ld4(gb.t + row * CT * 16 + 16 * t + 4 * q, ok), gh, xh);
 }
 
 // t_bar = rstd * (g_hat - (w / W) * (S1 + x_hat * S2)); with gb.wbar, also the depth-weight gradient of this voxel,
@@ -162,61 +168,62 @@ constexpr size_t grad_sums_bytes() { return (size_t)NW * 4 * GradSums<CT>::SLOT 
 template <int CT, int NW>
 __global__ __launch_bounds__(NW * 64) void k_gn_grad_stats(FfnoRows rows, FfnoGnBack gb, const float* __restrict__ ubar,
                                                            float* __restrict__ partials) {
-  constexpr int C = CT * 16, CPG = C / G;
-  __shared__ float red[NW][4][CT * 4][2];
+  constexpr int C = CT * 16;
+  using S = GradSums<CT>;
+  __shared__ float red[NW * 4 * S::SLOT];
   Lane L;
   const int b = blockIdx.y;
   const float scale = gb.scale ? *gb.scale : 1.f;
   const int64_t nt = (rows.voxels + 15) / 16;
-  float s1[CT * 4], s2[CT * 4];
+  const int64_t stride = (int64_t)gridDim.x * NW;
+  // per lane and channel group sums; the next tile's rows are loaded while this tile's are reduced
+  float s1[CT * S::SUB], s2[CT * S::SUB];
 #pragma unroll
-  for (int i = 0; i < CT * 4; ++i) { s1[i] = 0.f; s2[i] = 0.f; }
-  for (int64_t tile = (int64_t)blockIdx.x * NW + L.wave; tile < nt; tile += (int64_t)gridDim.x * NW) {
-    const int64_t vox = tile * 16 + L.v;
+  for (int i = 0; i < CT * S::SUB; ++i) { s1[i] = 0.f; s2[i] = 0.f; }
+  f32x4 ubn[CT], tvn[CT];
+  auto fetch = [&](int64_t t) {
+    const int64_t vox = t * 16 + L.v;
     const bool ok = vox < rows.voxels;
     const int64_t row = (int64_t)b * rows.voxels + vox;
 #pragma unroll
+    for (int u = 0; u < CT; ++u) {
+      ubn[u] = ld4(ubar + row * C + 16 * u + 4 * L.q, ok);
+      tvn[u] = ld4(gb.t + row * C + 16 * u + 4 * L.q, ok);
+    }
+  };
+  int64_t tile = (int64_t)blockIdx.x * NW + L.wave;
+  if (tile < nt) fetch(tile);
+  for (; tile < nt; tile += stride) {
+    const bool ok = tile * 16 + L.v < rows.voxels;
+    f32x4 ub[CT], tv[CT];
+#pragma unroll
+    for (int u = 0; u < CT; ++u) { ub[u] = ubn[u]; tv[u] = tvn[u]; }
+    if (tile + stride < nt) fetch(tile + stride);
+#pragma unroll
     for (int t = 0; t < CT; ++t) {
-      const f32x4 ub = ld4(ubar + row * C + 16 * t + 4 * L.q, ok);
       f32x4 gh, xh;
-      gn_ghat<CT>(gb, scale, row, b, ok, L.q, t, ub, gh, xh);
+      gn_ghat_of<CT>(gb, scale, b, ok, L.q, t, ub[t], tv[t], gh, xh);
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        s1[t * 4 + j] += gh[j];
-        s2[t * 4 + j] += gh[j] * xh[j];
+        s1[t * S::SUB + j / (4 / S::SUB)] += gh[j];
+        s2[t * S::SUB + j / (4 / S::SUB)] += gh[j] * xh[j];
       }
     }
   }
   // sum over the 16 voxel lanes, then over waves and lane groups per channel group
 #pragma unroll
-  for (int i = 0; i < CT * 4; ++i)
-#pragma unroll
-    for (int off = 1; off < 16; off <<= 1) {
-      s1[i] += __shfl_xor(s1[i], off);
-      s2[i] += __shfl_xor(s2[i], off);
-    }
+  for (int i = 0; i < CT * S::SUB; ++i) {
+    s1[i] = row16_sum(s1[i]);
+    s2[i] = row16_sum(s2[i]);
+  }
   if (L.v == 0)
 #pragma unroll
-    for (int i = 0; i < CT * 4; ++i) {
-      red[L.wave][L.q][i][0] = s1[i];
-      red[L.wave][L.q][i][1] = s2[i];
+    for (int i = 0; i < CT * S::SUB; ++i) {
+      red[(L.wave * 4 + L.q) * S::SLOT + 2 * i] = s1[i];
+      red[(L.wave * 4 + L.q) * S::SLOT + 2 * i + 1] = s2[i];
     }
-  __syncthreads();
-  if (threadIdx.x < G) {
-    const int g = threadIdx.x;
-    float a = 0.f, c = 0.f;
-    for (int w = 0; w < NW; ++w)
-      for (int qq = 0; qq < 4; ++qq)
-        for (int i = 0; i < CT * 4; ++i) {
-          const int ch = 16 * (i / 4) + 4 * qq + (i % 4);
-          if (ch / CPG != g) continue;
-          a += red[w][qq][i][0];
-          c += red[w][qq][i][1];
-        }
-    float* p = partials + (((int64_t)b * gridDim.x + blockIdx.x) * G + g) * 2;
-    p[0] = a;
-    p[1] = c;
-  }
+  S sums;
+  sums.template flush<NW>(red, partials + ((int64_t)b * gridDim.x + blockIdx.x) * G * 2);
 }
 
 // 256 threads: thread t sums the partials i = t/8, t/8 + 32, ... of group t%8, then one thread per group adds the 32
@@ -806,7 +813,8 @@ extern "C" {
 int ffno_gn_grad_stats(FfnoRows rows, FfnoGnBack gn, const float* ubar, float* partials, hipStream_t stream) {
   if (!rows_ok(rows) || !gb_ok(gn) || !ubar || !partials) return fail("ffno_gn_grad_stats: bad arguments");
   dim3 grid(rows.nwg, rows.B);
-  FFNO_DISPATCH_CT(rows.C, (k_gn_grad_stats<CT, nw_of<CT>()><<<grid, nw_of<CT>() * 64, 0, stream>>>(rows, gn, ubar, partials)));
+  // 4-wave workgroups: three fit a CU at this kernel's register count (two 8-wave ones would not)
+  FFNO_DISPATCH_CT(rows.C, (k_gn_grad_stats<CT, 4><<<grid, 4 * 64, 0, stream>>>(rows, gn, ubar, partials)));
   return check_launch("ffno_gn_grad_stats");
 }
 

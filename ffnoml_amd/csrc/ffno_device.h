@@ -38,23 +38,27 @@ __device__ __forceinline__ bf16x8 pack8(f32x4 a, f32x4 b) {
 
 // The standard normal CDF Phi(x) = (1 + erf(x / sqrt 2)) / 2 by Abramowitz & Stegun, Handbook of Mathematical
 // Functions, formula 7.1.26 (Hastings): for a >= 0, erf(a) = 1 - (a1 t + a2 t^2 + a3 t^3 + a4 t^4 + a5 t^5) e^(-a^2),
-// t = 1 / (1 + p a), |error| <= 1.5e-7. Phi(-|x|) = (1 - erf(|x| / sqrt 2)) / 2 is the polynomial term itself (no
-// cancellation for negative x). Also returns E = e^(-x^2 / 2), which the GELU derivative reuses.
-__device__ __forceinline__ float norm_cdf(float x, float& e) {
-  const float a = fabsf(x) * 0.70710678118654752f;
-  const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * a);
-  const float poly = t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f +
-                                                                                        t * 1.061405429f))));
-  e = __expf(-0.5f * x * x);
-  const float tail = 0.5f * poly * e;                  // Phi(-|x|)
-  return x >= 0.0f ? 1.0f - tail : tail;
+// t = 1 / (1 + p a), |error| <= 1.5e-7. The tail Phi(-|x|) = (1 - erf(|x| / sqrt 2)) / 2 is the polynomial term itself
+// (no cancellation for negative x). Written for few instructions: with u = k x, k = sqrt(log2(e) / 2), the Gaussian
+// E = e^(-x^2 / 2) is 2^(-u^2) (one v_exp_f32), p a = (p / (sqrt 2 k)) |u|, and the 1/2 is folded into the a_i.
+// Returns the tail and E (which the GELU derivative reuses).
+__device__ __forceinline__ float norm_tail(float x, float& e) {
+  const float u = x * 0.84932180028801907f;
+  const float t = __builtin_amdgcn_rcpf(fmaf(fabsf(u), 0.27273747324943542f, 1.0f));
+  const float poly = t * (0.127414796f + t * (-0.142248368f + t * (0.7107068705f + t * (-0.7265760135f +
+                                                                                        t * 0.5307027145f))));
+  e = __builtin_amdgcn_exp2f(-(u * u));
+  return poly * e;
 }
+// GELU(x) = x Phi(x) = max(x, 0) - |x| Phi(-|x|): no compare or select
 __device__ __forceinline__ float gelu(float x) {
   float e;
-  return x * norm_cdf(x, e);
+  return fmaf(-fabsf(x), norm_tail(x, e), fmaxf(x, 0.0f));
 }
 // 1 / (1 + e^-x) with the hardware reciprocal (1 ulp); e^-x = inf gives 0
-__device__ __forceinline__ float sigm(float x) { return __builtin_amdgcn_rcpf(1.0f + __expf(-x)); }
+__device__ __forceinline__ float sigm(float x) {
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(x * -1.4426950408889634f));
+}
 __device__ __forceinline__ f32x4 gelu4(f32x4 x) { return f32x4{gelu(x[0]), gelu(x[1]), gelu(x[2]), gelu(x[3])}; }
 __device__ __forceinline__ f32x4 sigm4(f32x4 x) { return f32x4{sigm(x[0]), sigm(x[1]), sigm(x[2]), sigm(x[3])}; }
 
@@ -129,8 +133,9 @@ __device__ __forceinline__ void gemm(Wm W, const Frag* b, f32x4 (&acc)[NT], int 
 // d/dx of the exact GELU: Phi(x) + x phi(x), phi(x) = e^(-x^2/2) / sqrt(2 pi)
 __device__ __forceinline__ float gelu_grad(float x) {
   float e;
-  const float c = norm_cdf(x, e);
-  return c + x * 0.3989422804014327f * e;
+  const float tail = norm_tail(x, e);
+  const float c = x >= 0.0f ? 1.0f - tail : tail;
+  return fmaf(x * 0.3989422804014327f, e, c);
 }
 __device__ __forceinline__ f32x4 gelu_grad4(f32x4 x) {
   return f32x4{gelu_grad(x[0]), gelu_grad(x[1]), gelu_grad(x[2]), gelu_grad(x[3])};
