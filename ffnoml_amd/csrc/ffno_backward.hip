@@ -70,8 +70,10 @@ __device__ __forceinline__ void gn_back(const FfnoGnBack& gb, float scale, int64
       const float* sm = gb.sums + (b * G + g) * 2;
       const float inv_w = __builtin_amdgcn_rcpf(st[2]);
       tb[t][j] = ok ? st[1] * (gh[j] - (w * inv_w) * (sm[0] + xh[j] * sm[1])) : 0.f;
-      const float r2var = 1.f - 1e-5f * st[1] * st[1];
-      wb -= (sm[0] * xh[j] + 0.5f * sm[1] * (xh[j] * xh[j] - r2var)) * inv_w;
+      if (gb.wbar) {
+        const float r2var = 1.f - 1e-5f * st[1] * st[1];
+        wb -= (sm[0] * xh[j] + 0.5f * sm[1] * (xh[j] * xh[j] - r2var)) * inv_w;
+      }
     }
   }
   if (gb.wbar) {
@@ -162,8 +164,74 @@ struct GradSums {
   }
 };
 
+// The GroupNorm backward and an input assembly as per-channel constants of one batch item, computed once per workgroup
+// into LDS kc[8][C]: P = rstd scale gamma, Q = rstd S1 / W, R = rstd^2 S2 / W, M = mean (per channel of its group),
+// A, B = the GroupNorm's affine (for the GELU after it), XA, XB = the assembly's affine. Then
+//   t_bar = P act u_bar - w (Q + R (t - M)),  act = GELU'(A t + B) or 1,
+// which is gn_back's rstd (g_hat - (w / W) (S1 + x_hat S2)) rearranged (the depth-weight gradient is not formed).
+template <int CT>
+__device__ void gn_consts(const FfnoGnBack& gb, float scale, const FfnoAssemble& xin, int b, float* kc) {
+  constexpr int C = CT * 16, CPG = C / G;
+  for (int c = threadIdx.x; c < C; c += blockDim.x) {
+    const int g = c / CPG;
+    const float* st = gb.stats + (b * G + g) * 3;
+    const float* sm = gb.sums + (b * G + g) * 2;
+    const float rstd = st[1], inv_w = __builtin_amdgcn_rcpf(st[2]);
+    kc[c] = rstd * scale * gb.gamma[c];
+    kc[C + c] = rstd * inv_w * sm[0];
+    kc[2 * C + c] = rstd * rstd * inv_w * sm[1];
+    kc[3 * C + c] = st[0];
+    kc[4 * C + c] = gb.act_gelu ? gb.aff_a[b * C + c] : 0.f;
+    kc[5 * C + c] = gb.act_gelu ? gb.aff_b[b * C + c] : 0.f;
+    kc[6 * C + c] = xin.t ? xin.aff_a[b * C + c] : 0.f;
+    kc[7 * C + c] = xin.t ? xin.aff_b[b * C + c] : 0.f;
+  }
+}
+
+// t_bar from the constants (rows outside the batch item have u_bar = t = 0 and w = 0, so t_bar = 0 there); the
+// GroupNorm input is read per tile t, as gn_back does
+template <int CT>
+__device__ __forceinline__ void gn_back_k(const FfnoGnBack& gb, const float* kc, int64_t row, bool ok, int q,
+                                          const f32x4 (&ub)[CT], f32x4 (&tb)[CT]) {
+  constexpr int C = CT * 16;
+  const float w = ok ? gb.depth_w[row] : 0.f;
+#pragma unroll
+  for (int t = 0; t < CT; ++t) {
+    const int c0 = 16 * t + 4 * q;
+    const f32x4 tv = ld4(gb.t + row * C + c0, ok);
+    f32x4 g = *(const f32x4*)(kc + c0) * ub[t];
+    if (gb.act_gelu) g *= gelu_grad4(tv * *(const f32x4*)(kc + 4 * C + c0) + *(const f32x4*)(kc + 5 * C + c0));
+    const f32x4 qr = *(const f32x4*)(kc + C + c0) + *(const f32x4*)(kc + 2 * C + c0) *
+                                                        (tv - *(const f32x4*)(kc + 3 * C + c0));
+    tb[t] = g - w * qr;
+  }
+}
+
+// the assembly base + scale act(t XA + XB) from the constants (rows outside the batch item give finite values that
+// only reach their own, unstored, voxel column)
+template <int CT>
+__device__ __forceinline__ void assemble_k(const FfnoAssemble& in, const float* kc, float scale, int64_t row, bool ok,
+                                           int q, f32x4 (&x)[CT]) {
+  constexpr int C = CT * 16;
+#pragma unroll
+  for (int t = 0; t < CT; ++t) {
+    const int c0 = 16 * t + 4 * q;
+    f32x4 base = in.base ? ld4(in.base + row * C + c0, ok) : zero4();
+    if (in.t) {
+      f32x4 u = ld4(in.t + row * C + c0, ok) * *(const f32x4*)(kc + 6 * C + c0) + *(const f32x4*)(kc + 7 * C + c0);
+      if (in.act_gelu) u = gelu4(u);
+      base += scale * u;
+    }
+    x[t] = base;
+  }
+}
+
 template <int CT, int NW>
 constexpr size_t grad_sums_bytes() { return (size_t)NW * 4 * GradSums<CT>::SLOT * sizeof(float); }
+
+// the LDS the MLP adjoint keeps ahead of its weights: kc[8][C] and the fused sums' slots
+template <int CT, int NW>
+constexpr size_t mlp_bwd_head_bytes() { return (size_t)8 * CT * 16 * sizeof(float) + grad_sums_bytes<CT, NW>(); }
 
 template <int CT, int NW>
 __global__ __launch_bounds__(NW * 64) void k_gn_grad_stats(FfnoRows rows, FfnoGnBack gb, const float* __restrict__ ubar,
@@ -255,14 +323,19 @@ __global__ __launch_bounds__(256) void k_gn_grad_finalize(int nwg, const float* 
 // ---------------------------------------------------------------------------------------------------------------
 // MLP behind a GroupNorm, with or without the residual path
 // ---------------------------------------------------------------------------------------------------------------
-template <int CT, int NW, bool LW>
+// KC: the GroupNorm backward from per-channel constants (no depth-weight gradient), else gn_back
+template <int CT, int NW, bool LW, bool KC>
 __global__ __launch_bounds__(NW * 64) void k_mlp_bwd(FfnoRows rows, const float* ubar, FfnoGnBack gb, FfnoAssemble xin,
                                                      const uint16_t* w1, const uint16_t* w2t, const uint16_t* w1t,
                                                      int residual, float* xbar, FfnoGnBack next,
                                                      float* next_partials) {
   constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32;
   extern __shared__ __align__(16) unsigned char smem[];
-  bf16* lds = reinterpret_cast<bf16*>(smem);
+  // LDS: the per-channel constants kc[8][C] and the fused sums' slots first (within reach of the LDS instructions'
+  // 16-bit immediate offsets), then the staged weights
+  float* kc = reinterpret_cast<float*>(smem);
+  float* red = kc + 8 * C;
+  bf16* lds = reinterpret_cast<bf16*>(smem + mlp_bwd_head_bytes<CT, NW>());
   const Wm W1 = LW ? stage_w(lds, w1, 2 * C * wld(KP)) : global_w(w1, 2 * C * wld(KP));
   const Wm W2T = LW ? stage_w(lds, w2t, 2 * C * wld(KP)) : global_w(w2t, 2 * C * wld(KP));
   const Wm W1T = LW ? stage_w(lds, w1t, C * wld(2 * C)) : global_w(w1t, C * wld(2 * C));
@@ -272,9 +345,12 @@ __global__ __launch_bounds__(NW * 64) void k_mlp_bwd(FfnoRows rows, const float*
   const float scale = gb.scale ? *gb.scale : 1.f;
   const float xscale = xin.scale ? *xin.scale : 1.f;
   const float nscale = next_partials && next.scale ? *next.scale : 1.f;
+  if (KC) gn_consts<CT>(gb, scale, xin, b, kc);
   GradSums<CT> sums;
-  float* red = reinterpret_cast<float*>(lds);          // after the staged weights
-  if (next_partials) sums.template init<NW>(red, L.q, L.wave);
+  if (next_partials)
+    sums.template init<NW>(red, L.q, L.wave);           // its barrier also publishes kc
+  else
+    __syncthreads();
   const int64_t nt = (rows.voxels + 15) / 16;
   for (int64_t tile = (int64_t)blockIdx.x * NW + L.wave; tile < nt; tile += (int64_t)gridDim.x * NW) {
     const int64_t vox = tile * 16 + L.v;
@@ -284,11 +360,15 @@ __global__ __launch_bounds__(NW * 64) void k_mlp_bwd(FfnoRows rows, const float*
     load_act<CT>(ubar, row, ok, L.q, ub);
     Frag of[KS], xf[KS];
     {
-      f32x4 tb[CT];
-      gn_back<CT>(gb, scale, row, b, ok, L.q, ub, tb);
+      f32x4 tb[CT], x[CT];
+      if constexpr (KC) {
+        gn_back_k<CT>(gb, kc, row, ok, L.q, ub, tb);
+        assemble_k<CT>(xin, kc, xscale, row, ok, L.q, x);
+      } else {
+        gn_back<CT>(gb, scale, row, b, ok, L.q, ub, tb);
+        assemble<CT>(xin, xscale, row, b, ok, L.q, x);
+      }
       to_frags<CT>(tb, of);
-      f32x4 x[CT];
-      assemble<CT>(xin, xscale, row, b, ok, L.q, x);
       to_frags<CT>(x, xf);
     }
 #pragma unroll
@@ -834,18 +914,21 @@ int ffno_mlp_bwd(FfnoRows rows, const float* ubar, FfnoGnBack gn, FfnoAssemble x
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {
     constexpr int C = CT * 16, KP = ks_of(CT) * 32;
-    const size_t red = next_partials ? grad_sums_bytes<CT, nw_bwd<CT>()>() : 0;
+    const size_t red = mlp_bwd_head_bytes<CT, nw_bwd<CT>()>();
     const size_t lds = ((size_t)2 * C * wld(KP) * 2 + (size_t)C * wld(2 * C)) * 2 * 2;
-    if (lds + red <= 160 * 1024) {
-      auto k = k_mlp_bwd<CT, nw_bwd<CT>(), true>;
-      const size_t bytes = lds + red;
+    const bool lw = lds + red <= 160 * 1024;
+    const size_t bytes = (lw ? lds : 0) + red;
+    auto launch = [&](auto k) -> int {
       if (set_lds(k, bytes)) return -1;
       k<<<grid, nw_bwd<CT>() * 64, bytes, stream>>>(rows, ubar, gn, x, w1, w2t, w1t, residual, xbar, nx, next_partials);
-    } else {
-      auto k = k_mlp_bwd<CT, nw_bwd<CT>(), false>;
-      if (set_lds(k, red)) return -1;
-      k<<<grid, nw_bwd<CT>() * 64, red, stream>>>(rows, ubar, gn, x, w1, w2t, w1t, residual, xbar, nx, next_partials);
-    }
+      return 0;
+    };
+    int rc;
+    if (gn.wbar)
+      rc = lw ? launch(k_mlp_bwd<CT, nw_bwd<CT>(), true, false>) : launch(k_mlp_bwd<CT, nw_bwd<CT>(), false, false>);
+    else
+      rc = lw ? launch(k_mlp_bwd<CT, nw_bwd<CT>(), true, true>) : launch(k_mlp_bwd<CT, nw_bwd<CT>(), false, true>);
+    if (rc) return rc;
   });
   return check_launch("ffno_mlp_bwd");
 }
