@@ -501,7 +501,8 @@ __global__ __launch_bounds__(NW * 64) void k_fuse_bwd(FfnoRows rows, const float
 // ---------------------------------------------------------------------------------------------------------------
 // pass 1: fp_bar from the GroupNorm (GELU after it), then the fusion MLP's adjoint with its hidden layer recomputed:
 // [sm_bar, vm_bar] = 0.5 fp_bar (both halves) + F1^T (gelu'(F1 [s gs, v gv]) * F2^T fp_bar)
-template <int CT, int NW, bool LW>
+// KC: the fusion GroupNorm's backward from per-channel constants (no depth-weight gradient), else gn_back
+template <int CT, int NW, bool LW, bool KC>
 __global__ __launch_bounds__(NW * 64) void k_fuse_mlp_bwd(FfnoRows rows, const float* __restrict__ x1bar, FfnoGnBack gf,
                                                           const float* __restrict__ sp, const float* __restrict__ vp,
                                                           const float* __restrict__ s_a, const float* __restrict__ s_b,
@@ -511,30 +512,43 @@ __global__ __launch_bounds__(NW * 64) void k_fuse_mlp_bwd(FfnoRows rows, const f
                                                           float* __restrict__ smbar, float* __restrict__ vmbar) {
   constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32;
   extern __shared__ __align__(16) unsigned char smem[];
-  bf16* lds = reinterpret_cast<bf16*>(smem);
+  // LDS: kc[12][C] first (the GroupNorm constants, then the branch affines s_a, s_b, v_a, v_b of this batch item),
+  // within reach of 16-bit immediate offsets; then the staged weights
+  float* kc = reinterpret_cast<float*>(smem);
+  bf16* lds = reinterpret_cast<bf16*>(smem + 12 * C * sizeof(float));
   const Wm F1 = LW ? stage_w(lds, f1, 2 * C * wld(2 * KP)) : global_w(f1, 2 * C * wld(2 * KP));
   const Wm F1T = LW ? stage_w(lds, f1t, 2 * C * wld(2 * C)) : global_w(f1t, 2 * C * wld(2 * C));
   const Wm F2T = global_w(f2t, 2 * C * wld(KP));
-  __syncthreads();
   Lane L;
   const int b = blockIdx.y;
   const float scale = gf.scale ? *gf.scale : 1.f;
+  if (KC) gn_consts<CT>(gf, scale, FfnoAssemble{}, b, kc);
+  for (int c = threadIdx.x; c < C; c += blockDim.x) {
+    kc[8 * C + c] = s_a[b * C + c];
+    kc[9 * C + c] = s_b[b * C + c];
+    kc[10 * C + c] = v_a[b * C + c];
+    kc[11 * C + c] = v_b[b * C + c];
+  }
+  __syncthreads();
   const int64_t nt = (rows.voxels + 15) / 16;
   for (int64_t tile = (int64_t)blockIdx.x * NW + L.wave; tile < nt; tile += (int64_t)gridDim.x * NW) {
     const int64_t vox = tile * 16 + L.v;
     const bool ok = vox < rows.voxels;
     const int64_t row = (int64_t)b * rows.voxels + vox;
+    // the constants are read from LDS per tile: hoisted out of the loop they would not fit the registers
+    asm volatile("" ::: "memory");
     Frag mm[2 * KS];
     {
       f32x4 s[CT], v[CT], gt[2 * CT];
       load_act<CT>(sp, row, ok, L.q, s);
       load_act<CT>(vp, row, ok, L.q, v);
       load_act<2 * CT>(gates, row, ok, L.q, gt);
+      // rows outside the batch item load zero gates, so s and v are zero there
 #pragma unroll
       for (int t = 0; t < CT; ++t) {
-        const int c0 = b * C + 16 * t + 4 * L.q;
-        s[t] = ok ? gelu4(s[t] * *(const f32x4*)(s_a + c0) + *(const f32x4*)(s_b + c0)) * gt[t] : zero4();
-        v[t] = ok ? gelu4(v[t] * *(const f32x4*)(v_a + c0) + *(const f32x4*)(v_b + c0)) * gt[CT + t] : zero4();
+        const int c0 = 16 * t + 4 * L.q;
+        s[t] = gelu4(s[t] * *(const f32x4*)(kc + 8 * C + c0) + *(const f32x4*)(kc + 9 * C + c0)) * gt[t];
+        v[t] = gelu4(v[t] * *(const f32x4*)(kc + 10 * C + c0) + *(const f32x4*)(kc + 11 * C + c0)) * gt[CT + t];
       }
       to_frags<CT>(s, mm);
       to_frags<CT>(v, mm + KS);
@@ -544,7 +558,10 @@ __global__ __launch_bounds__(NW * 64) void k_fuse_mlp_bwd(FfnoRows rows, const f
     {
       f32x4 x1b[CT], fb[CT];
       load_act<CT>(x1bar, row, ok, L.q, x1b);
-      gn_back<CT>(gf, scale, row, b, ok, L.q, x1b, fb);
+      if constexpr (KC)
+        gn_back_k<CT>(gf, kc, row, ok, L.q, x1b, fb);
+      else
+        gn_back<CT>(gf, scale, row, b, ok, L.q, x1b, fb);
 #pragma unroll
       for (int t = 0; t < CT; ++t) {
         mb[t] = 0.5f * fb[t];
@@ -572,18 +589,29 @@ __global__ __launch_bounds__(NW * 64) void k_fuse_gates_bwd(FfnoRows rows, float
                                                             float* sv_v) {
   constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32;
   extern __shared__ __align__(16) unsigned char smem[];
-  bf16* lds = reinterpret_cast<bf16*>(smem);
+  // LDS: kc[4][C] first (s_a, s_b, v_a, v_b of this batch item), within reach of 16-bit immediate offsets; then the
+  // staged weights
+  float* kc = reinterpret_cast<float*>(smem);
+  bf16* lds = reinterpret_cast<bf16*>(smem + 4 * C * sizeof(float));
   const Wm B1 = LW ? stage_w(lds, bg1, C * wld(3 * KP)) : global_w(bg1, C * wld(3 * KP));
   const Wm B2T = LW ? stage_w(lds, bg2t, C * wld(2 * C)) : global_w(bg2t, C * wld(2 * C));
   const Wm B1T = LW ? stage_w(lds, bg1t, 3 * C * wld(KP)) : global_w(bg1t, 3 * C * wld(KP));
-  __syncthreads();
   Lane L;
   const int b = blockIdx.y;
+  for (int c = threadIdx.x; c < C; c += blockDim.x) {
+    kc[c] = s_a[b * C + c];
+    kc[C + c] = s_b[b * C + c];
+    kc[2 * C + c] = v_a[b * C + c];
+    kc[3 * C + c] = v_b[b * C + c];
+  }
+  __syncthreads();
   const int64_t nt = (rows.voxels + 15) / 16;
   for (int64_t tile = (int64_t)blockIdx.x * NW + L.wave; tile < nt; tile += (int64_t)gridDim.x * NW) {
     const int64_t vox = tile * 16 + L.v;
     const bool ok = vox < rows.voxels;
     const int64_t row = (int64_t)b * rows.voxels + vox;
+    // the constants are read from LDS per tile: hoisted out of the loop they would not fit the registers
+    asm volatile("" ::: "memory");
     Frag cat[3 * KS];
     f32x4 s[CT], v[CT];
     {
@@ -595,9 +623,9 @@ __global__ __launch_bounds__(NW * 64) void k_fuse_gates_bwd(FfnoRows rows, float
     load_act<CT>(vp, row, ok, L.q, v);
 #pragma unroll
     for (int t = 0; t < CT; ++t) {
-      const int c0 = b * C + 16 * t + 4 * L.q;
-      s[t] = ok ? gelu4(s[t] * *(const f32x4*)(s_a + c0) + *(const f32x4*)(s_b + c0)) : zero4();
-      v[t] = ok ? gelu4(v[t] * *(const f32x4*)(v_a + c0) + *(const f32x4*)(v_b + c0)) : zero4();
+      const int c0 = 16 * t + 4 * L.q;
+      s[t] = ok ? gelu4(s[t] * *(const f32x4*)(kc + c0) + *(const f32x4*)(kc + C + c0)) : zero4();
+      v[t] = ok ? gelu4(v[t] * *(const f32x4*)(kc + 2 * C + c0) + *(const f32x4*)(kc + 3 * C + c0)) : zero4();
     }
     to_frags<CT>(s, cat + KS);
     to_frags<CT>(v, cat + 2 * KS);
@@ -958,16 +986,22 @@ int ffno_fuse_mlp_bwd(FfnoRows rows, const float* x1bar, FfnoGnBack gnf, const f
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {
     constexpr int C = CT * 16, KP = ks_of(CT) * 32;
+    const size_t head = (size_t)12 * C * sizeof(float);
     const size_t lds = ((size_t)2 * C * wld(2 * KP) + (size_t)2 * C * wld(2 * C)) * 2 * 2;
-    if (lds <= 160 * 1024) {
-      auto k = k_fuse_mlp_bwd<CT, nw_of<CT>(), true>;
-      if (set_lds(k, lds)) return -1;
-      k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, x1bar, gnf, sp, vp, s_a, s_b, v_a, v_b, gates, f1, f2t, f1t,
-                                                 smbar, vmbar);
-    } else {
-      k_fuse_mlp_bwd<CT, nw_of<CT>(), false><<<grid, nw_of<CT>() * 64, 0, stream>>>(
-          rows, x1bar, gnf, sp, vp, s_a, s_b, v_a, v_b, gates, f1, f2t, f1t, smbar, vmbar);
-    }
+    const bool lw = lds + head <= 160 * 1024;
+    const size_t bytes = (lw ? lds : 0) + head;
+    auto launch = [&](auto k) -> int {
+      if (set_lds(k, bytes)) return -1;
+      k<<<grid, nw_of<CT>() * 64, bytes, stream>>>(rows, x1bar, gnf, sp, vp, s_a, s_b, v_a, v_b, gates, f1, f2t, f1t,
+                                                   smbar, vmbar);
+      return 0;
+    };
+    int rc;
+    if (gnf.wbar)
+      rc = lw ? launch(k_fuse_mlp_bwd<CT, nw_of<CT>(), true, false>) : launch(k_fuse_mlp_bwd<CT, nw_of<CT>(), false, false>);
+    else
+      rc = lw ? launch(k_fuse_mlp_bwd<CT, nw_of<CT>(), true, true>) : launch(k_fuse_mlp_bwd<CT, nw_of<CT>(), false, true>);
+    if (rc) return rc;
   });
   return check_launch("ffno_fuse_mlp_bwd");
 }
@@ -982,16 +1016,18 @@ int ffno_fuse_gates_bwd(FfnoRows rows, float* x1bar_hbar, const float* h, const 
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {
     constexpr int C = CT * 16, KP = ks_of(CT) * 32;
+    const size_t head = (size_t)4 * C * sizeof(float);
     const size_t lds = ((size_t)C * wld(3 * KP) + (size_t)C * wld(2 * C) + (size_t)3 * C * wld(KP)) * 2 * 2;
-    if (lds <= 160 * 1024) {
-      auto k = k_fuse_gates_bwd<CT, nw_of<CT>(), true>;
-      if (set_lds(k, lds)) return -1;
-      k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, x1bar_hbar, h, sp, vp, s_a, s_b, v_a, v_b, gates, bg1, bg2t,
-                                                 bg1t, sm_sbar, vm_vbar);
-    } else {
-      k_fuse_gates_bwd<CT, nw_of<CT>(), false><<<grid, nw_of<CT>() * 64, 0, stream>>>(
-          rows, x1bar_hbar, h, sp, vp, s_a, s_b, v_a, v_b, gates, bg1, bg2t, bg1t, sm_sbar, vm_vbar);
-    }
+    const bool lw = lds + head <= 160 * 1024;
+    const size_t bytes = (lw ? lds : 0) + head;
+    auto launch = [&](auto k) -> int {
+      if (set_lds(k, bytes)) return -1;
+      k<<<grid, nw_of<CT>() * 64, bytes, stream>>>(rows, x1bar_hbar, h, sp, vp, s_a, s_b, v_a, v_b, gates, bg1, bg2t,
+                                                   bg1t, sm_sbar, vm_vbar);
+      return 0;
+    };
+    if (lw ? launch(k_fuse_gates_bwd<CT, nw_of<CT>(), true>) : launch(k_fuse_gates_bwd<CT, nw_of<CT>(), false>))
+      return -1;
   });
   return check_launch("ffno_fuse_gates_bwd");
 }

@@ -216,7 +216,10 @@ __global__ __launch_bounds__(NW * 64) void k_fuse(FfnoRows rows, const float* __
   constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32;
   constexpr int N1 = C * wld(3 * KP), N2 = 2 * C * wld(KP), N3 = 2 * C * wld(2 * KP), N4 = C * wld(2 * C);
   extern __shared__ __align__(16) unsigned char smem[];
-  bf16* lds = reinterpret_cast<bf16*>(smem);
+  // LDS: kc[6][C] first (s_a, s_b, v_a, v_b of this batch item, then the gate bias [2C]), within reach of 16-bit
+  // immediate offsets; then the staged weights and the statistics scratch
+  float* kc = reinterpret_cast<float*>(smem);
+  bf16* lds = reinterpret_cast<bf16*>(smem + 6 * C * sizeof(float));
   Wm B1, B2, F1, F2;
   if (LW) {
     // B1 and F2: hi plane staged, lo plane read from global memory (everything staged would pass 160 KB)
@@ -235,9 +238,17 @@ __global__ __launch_bounds__(NW * 64) void k_fuse(FfnoRows rows, const float* __
     F2 = global_w(f2, N4);
   }
   float* red = reinterpret_cast<float*>(lds);
-  __syncthreads();
   Lane L;
   const int b = blockIdx.y;
+  for (int c = threadIdx.x; c < C; c += blockDim.x) {
+    kc[c] = s_a[b * C + c];
+    kc[C + c] = s_b[b * C + c];
+    kc[2 * C + c] = v_a[b * C + c];
+    kc[3 * C + c] = v_b[b * C + c];
+    kc[4 * C + c] = bg2_bias[c];
+    kc[5 * C + c] = bg2_bias[C + c];
+  }
+  __syncthreads();
   const int64_t nt = (rows.voxels + 15) / 16;
   GnAcc<CT> acc;
   acc.init();
@@ -245,6 +256,8 @@ __global__ __launch_bounds__(NW * 64) void k_fuse(FfnoRows rows, const float* __
     const int64_t vox = tile * 16 + L.v;
     const bool ok = vox < rows.voxels;
     const int64_t row = (int64_t)b * rows.voxels + vox;
+    // the constants are read from LDS per tile: hoisted out of the loop they would not fit the registers
+    asm volatile("" ::: "memory");
     f32x4 s[CT], v[CT];
     Frag cat[3 * KS];
     {
@@ -256,16 +269,16 @@ __global__ __launch_bounds__(NW * 64) void k_fuse(FfnoRows rows, const float* __
     load_act<CT>(vp, row, ok, L.q, v);
 #pragma unroll
     for (int t = 0; t < CT; ++t) {
-      const int c0 = b * C + 16 * t + 4 * L.q;
-      s[t] = ok ? gelu4(s[t] * *(const f32x4*)(s_a + c0) + *(const f32x4*)(s_b + c0)) : zero4();
-      v[t] = ok ? gelu4(v[t] * *(const f32x4*)(v_a + c0) + *(const f32x4*)(v_b + c0)) : zero4();
+      const int c0 = 16 * t + 4 * L.q;
+      s[t] = ok ? gelu4(s[t] * *(const f32x4*)(kc + c0) + *(const f32x4*)(kc + C + c0)) : zero4();
+      v[t] = ok ? gelu4(v[t] * *(const f32x4*)(kc + 2 * C + c0) + *(const f32x4*)(kc + 3 * C + c0)) : zero4();
     }
     to_frags<CT>(s, cat + KS);
     to_frags<CT>(v, cat + 2 * KS);
     f32x4 g[2 * CT];
     mlp2<3 * KS, CT, 2 * CT>(B1, B2, cat, g, L.v, L.q);
 #pragma unroll
-    for (int t = 0; t < 2 * CT; ++t) g[t] = sigm4(g[t] + *(const f32x4*)(bg2_bias + 16 * t + 4 * L.q));
+    for (int t = 0; t < 2 * CT; ++t) g[t] = sigm4(g[t] + *(const f32x4*)(kc + 4 * C + 16 * t + 4 * L.q));
     if (gates_out) store_act<2 * CT>(gates_out, row, ok, L.q, g);
 #pragma unroll
     for (int t = 0; t < CT; ++t) {
@@ -852,7 +865,7 @@ int ffno_fuse(FfnoRows rows, const float* x, const float* sp, const float* vp, c
     constexpr int C = CT * 16, KP = ks_of(CT) * 32;
     const size_t lds = ((size_t)C * wld(3 * KP) + 2 * ((size_t)2 * C * wld(KP) + 2 * C * wld(2 * KP)) +
                         (size_t)C * wld(2 * C)) * 2 +
-                       red_bytes_nw<CT, nw_of<CT>()>();
+                       red_bytes_nw<CT, nw_of<CT>()>() + 6 * C * sizeof(float);
     if (lds <= 160 * 1024) {
       auto k = k_fuse<CT, nw_of<CT>(), true>;
       if (set_lds(k, lds)) return -1;
@@ -860,8 +873,10 @@ int ffno_fuse(FfnoRows rows, const float* x, const float* sp, const float* vp, c
                                                  gates_out);
     } else {
       auto k = k_fuse<CT, NWV, false>;
-      k<<<grid, NWV * 64, red_bytes<CT>(), stream>>>(rows, x, sp, vp, s_a, s_b, v_a, v_b, bg1, bg2, bg2_bias, f1, f2,
-                                                      fp, stats, gates_out);
+      const size_t bytes = red_bytes<CT>() + 6 * C * sizeof(float);
+      if (set_lds(k, bytes)) return -1;
+      k<<<grid, NWV * 64, bytes, stream>>>(rows, x, sp, vp, s_a, s_b, v_a, v_b, bg1, bg2, bg2_bias, f1, f2, fp, stats,
+                                           gates_out);
     }
   });
   return check_launch("ffno_fuse");
