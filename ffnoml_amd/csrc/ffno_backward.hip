@@ -236,12 +236,23 @@ constexpr size_t mlp_bwd_head_bytes() { return (size_t)8 * CT * 16 * sizeof(floa
 template <int CT, int NW>
 __global__ __launch_bounds__(NW * 64) void k_gn_grad_stats(FfnoRows rows, FfnoGnBack gb, const float* __restrict__ ubar,
                                                            float* __restrict__ partials) {
-  constexpr int C = CT * 16;
+  constexpr int C = CT * 16, CPG = C / G;
   using S = GradSums<CT>;
   __shared__ float red[NW * 4 * S::SLOT];
+  // per-channel constants of this batch item: scale gamma, mean, rstd, and the affine before the GELU
+  __shared__ float kc[5 * C];
   Lane L;
   const int b = blockIdx.y;
   const float scale = gb.scale ? *gb.scale : 1.f;
+  for (int c = threadIdx.x; c < C; c += blockDim.x) {
+    const float* st = gb.stats + (b * G + c / CPG) * 3;
+    kc[c] = scale * gb.gamma[c];
+    kc[C + c] = st[0];
+    kc[2 * C + c] = st[1];
+    kc[3 * C + c] = gb.act_gelu ? gb.aff_a[b * C + c] : 0.f;
+    kc[4 * C + c] = gb.act_gelu ? gb.aff_b[b * C + c] : 0.f;
+  }
+  __syncthreads();
   const int64_t nt = (rows.voxels + 15) / 16;
   const int64_t stride = (int64_t)gridDim.x * NW;
   // per lane and channel group sums; the next tile's rows are loaded while this tile's are reduced
@@ -262,15 +273,18 @@ __global__ __launch_bounds__(NW * 64) void k_gn_grad_stats(FfnoRows rows, FfnoGn
   int64_t tile = (int64_t)blockIdx.x * NW + L.wave;
   if (tile < nt) fetch(tile);
   for (; tile < nt; tile += stride) {
-    const bool ok = tile * 16 + L.v < rows.voxels;
     f32x4 ub[CT], tv[CT];
 #pragma unroll
     for (int u = 0; u < CT; ++u) { ub[u] = ubn[u]; tv[u] = tvn[u]; }
     if (tile + stride < nt) fetch(tile + stride);
+    asm volatile("" ::: "memory");        // the constants are read from LDS per tile, not hoisted into registers
 #pragma unroll
     for (int t = 0; t < CT; ++t) {
-      f32x4 gh, xh;
-      gn_ghat_of<CT>(gb, scale, b, ok, L.q, t, ub[t], tv[t], gh, xh);
+      // rows outside the batch item load u_bar = t = 0, so g_hat = 0 there
+      const int c0 = 16 * t + 4 * L.q;
+      f32x4 gh = *(const f32x4*)(kc + c0) * ub[t];
+      if (gb.act_gelu) gh *= gelu_grad4(tv[t] * *(const f32x4*)(kc + 3 * C + c0) + *(const f32x4*)(kc + 4 * C + c0));
+      const f32x4 xh = (tv[t] - *(const f32x4*)(kc + C + c0)) * *(const f32x4*)(kc + 2 * C + c0);
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         s1[t * S::SUB + j / (4 / S::SUB)] += gh[j];

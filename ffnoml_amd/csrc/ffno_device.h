@@ -453,6 +453,35 @@ __device__ __forceinline__ void assemble(const FfnoAssemble& in, float scale, in
   }
 }
 
+// An assembly's affine (a, b of its batch item) staged into LDS kc[2][C] by the whole workgroup (the caller then
+// synchronises), and the assembly read from there: x = base + scale act(t a + b). Rows outside the batch item give
+// finite values that reach only their own, unstored, voxel column.
+template <int CT>
+__device__ __forceinline__ void stage_affine(const FfnoAssemble& in, int b, float* kc) {
+  constexpr int C = CT * 16;
+  for (int c = threadIdx.x; c < C; c += blockDim.x) {
+    kc[c] = in.t ? in.aff_a[b * C + c] : 0.f;
+    kc[C + c] = in.t ? in.aff_b[b * C + c] : 0.f;
+  }
+}
+
+template <int CT>
+__device__ __forceinline__ void assemble_c(const FfnoAssemble& in, const float* kc, float scale, int64_t row, bool ok,
+                                           int q, f32x4 (&x)[CT]) {
+  constexpr int C = CT * 16;
+#pragma unroll
+  for (int t = 0; t < CT; ++t) {
+    const int c0 = 16 * t + 4 * q;
+    f32x4 base = in.base ? ld4(in.base + row * C + c0, ok) : zero4();
+    if (in.t) {
+      f32x4 u = ld4(in.t + row * C + c0, ok) * *(const f32x4*)(kc + c0) + *(const f32x4*)(kc + C + c0);
+      if (in.act_gelu) u = gelu4(u);
+      base += scale * u;
+    }
+    x[t] = base;
+  }
+}
+
 template <int CT>
 __device__ __forceinline__ void gelu_tiles(f32x4 (&x)[CT]) {
 #pragma unroll

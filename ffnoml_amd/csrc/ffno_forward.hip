@@ -122,20 +122,25 @@ __global__ __launch_bounds__(NW * 64) void k_block_in(FfnoRows rows, FfnoAssembl
                                                       const float* __restrict__ z, float* xs, float* xv) {
   constexpr int C = CT * 16, KS = ks_of(CT), SZ = C * wld(KS * 32);
   extern __shared__ __align__(16) unsigned char smem[];
-  bf16* lds = reinterpret_cast<bf16*>(smem);
+  // LDS: the assembly's affine kc[2][C] first (within reach of 16-bit immediate offsets), then the weights
+  float* kc = reinterpret_cast<float*>(smem);
+  bf16* lds = reinterpret_cast<bf16*>(smem + 2 * C * sizeof(float));
   const Wm3 G1 = stage_w3<NL>(lds, g1, SZ), G2 = stage_w3<NL>(lds, g2, SZ);
   const Wm3 V1 = stage_w3<NL>(lds, v1, SZ), V2 = stage_w3<NL>(lds, v2, SZ);
-  __syncthreads();
   Lane L;
   const int b = blockIdx.y;
+  stage_affine<CT>(in, b, kc);
+  __syncthreads();
   const float scale = in.scale ? *in.scale : 1.f;
   const int64_t nt = (rows.voxels + 15) / 16;
   for (int64_t tile = (int64_t)blockIdx.x * NW + L.wave; tile < nt; tile += (int64_t)gridDim.x * NW) {
     const int64_t vox = tile * 16 + L.v;
     const bool ok = vox < rows.voxels;
     const int64_t row = (int64_t)b * rows.voxels + vox;
+    // the constants are read from LDS per tile: hoisted out of the loop they would not fit the registers
+    asm volatile("" ::: "memory");
     f32x4 x[CT];
-    assemble<CT>(in, scale, row, b, ok, L.q, x);
+    assemble_c<CT>(in, kc, scale, row, ok, L.q, x);
     if (x_out) store_act<CT>(x_out, row, ok, L.q, x);
     Frag3 xf[KS];
     to_frags<CT>(x, xf);
@@ -307,12 +312,16 @@ __global__ __launch_bounds__(NW * 64) void k_resmlp(FfnoRows rows, FfnoAssemble 
                                                     FfnoStatsOut st) {
   constexpr int C = CT * 16, KS1 = ks_of(CT);
   extern __shared__ __align__(16) unsigned char smem[];
-  bf16* lds = reinterpret_cast<bf16*>(smem);
+  // LDS: the assembly's affine kc[2][C] first (within reach of 16-bit immediate offsets), then the weights and the
+  // statistics scratch
+  float* kc = reinterpret_cast<float*>(smem);
+  bf16* lds = reinterpret_cast<bf16*>(smem + 2 * C * sizeof(float));
   const Wm3 W1 = stage_w3<NL>(lds, w1, 2 * C * wld(KS1 * 32)), W2 = stage_w3<NL>(lds, w2, C * wld(2 * C));
   float* red = reinterpret_cast<float*>(lds);
-  __syncthreads();
   Lane L;
   const int b = blockIdx.y;
+  stage_affine<CT>(in, b, kc);
+  __syncthreads();
   const float scale = in.scale ? *in.scale : 1.f;
   const int64_t nt = (rows.voxels + 15) / 16;
   GnAcc<CT> acc;
@@ -323,13 +332,15 @@ __global__ __launch_bounds__(NW * 64) void k_resmlp(FfnoRows rows, FfnoAssemble 
     const int64_t vox0 = p * 32 + L.v, vox1 = vox0 + 16;
     const bool ok0 = vox0 < rows.voxels, ok1 = vox1 < rows.voxels;
     const int64_t row0 = (int64_t)b * rows.voxels + vox0, row1 = row0 + 16;
+    // the constants are read from LDS per step: hoisted out of the loop they would not fit the registers
+    asm volatile("" ::: "memory");
     Frag3 xf0[KS1], xf1[KS1];
     {
       f32x4 x[CT];
-      assemble<CT>(in, scale, row0, b, ok0, L.q, x);
+      assemble_c<CT>(in, kc, scale, row0, ok0, L.q, x);
       if (x_out) store_act<CT>(x_out, row0, ok0, L.q, x);
       to_frags<CT>(x, xf0);
-      assemble<CT>(in, scale, row1, b, ok1, L.q, x);
+      assemble_c<CT>(in, kc, scale, row1, ok1, L.q, x);
       if (x_out) store_act<CT>(x_out, row1, ok1, L.q, x);
       to_frags<CT>(x, xf1);
     }
@@ -823,9 +834,10 @@ int ffno_block_in(FfnoRows rows, FfnoAssemble in, float* x_out, const uint16_t* 
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {
     const size_t elems = 4 * (size_t)(CT * 16) * wld(ks_of(CT) * 32);
-    const int nl = lds_planes(elems, 0);
+    const size_t head = (size_t)2 * CT * 16 * sizeof(float);   // the assembly's affine
+    const int nl = lds_planes(elems, head);
     FFNO_DISPATCH_NL(nl, {
-      const size_t lds = NL * elems * 2;
+      const size_t lds = NL * elems * 2 + head;
       auto k = k_block_in<CT, nw_fwd<CT>(), NL>;
       if (set_lds(k, lds)) return -1;
       k<<<grid, nw_fwd<CT>() * 64, lds, stream>>>(rows, in, x_out, g1, g2, v1, v2, z_proj, z, xs, xv);
@@ -890,7 +902,7 @@ int ffno_resmlp(FfnoRows rows, FfnoAssemble in, float* x_out, const uint16_t* w1
     const size_t elems = (size_t)2 * CT * 16 * wld(ks_of(CT) * 32) + (size_t)CT * 16 * wld(2 * CT * 16);
     // 12 waves at CT 4: the next tile's two input rows held in flight need the registers of a 3-wave SIMD
     constexpr int NWR = nw_bwd<CT>();
-    const size_t red = red_bytes_nw<CT, NWR>();
+    const size_t red = red_bytes_nw<CT, NWR>() + (size_t)2 * CT * 16 * sizeof(float);   // + the assembly's affine
     const int nl = lds_planes(elems, red);
     FFNO_DISPATCH_NL(nl, {
       const size_t lds = NL * elems * 2 + red;
