@@ -680,8 +680,41 @@ template <int CT>
 __device__ __forceinline__ void gate_bwd(Wm A1, Wm A2, Wm A2T, Wm A1T, const Frag* hf, const f32x4 (&h)[CT],
                                          const f32x4 (&xb)[CT], f32x4 (&hb)[CT], int v, int q) {
   constexpr int KS = ks_of(CT);
-  f32x4 sg[CT], og[CT];
-  mlp2<KS, CT, CT>(A1, A2, hf, sg, v, q);
+  // forward: the hidden layer's GELU and GELU' from one evaluation per element (the derivative kept for the
+  // adjoint below, so A1 h is formed once), then sg = sigmoid(A2 gelu(A1 h))
+  f32x4 sg[CT], gd[CT];
+#pragma unroll
+  for (int t = 0; t < CT; ++t) sg[t] = zero4();
+#pragma unroll 1
+  for (int s = 0; s < KS; ++s) {
+    f32x4 p0 = tile<KS>(A1, 2 * s, hf, v, q), p1 = zero4();
+    if (2 * s + 1 < CT) p1 = tile<KS>(A1, 2 * s + 1, hf, v, q);
+    f32x4 g0, g1, d0, d1;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      float a, c;
+      gelu_and_grad(p0[j], a, c);
+      g0[j] = a;
+      d0[j] = c;
+      gelu_and_grad(p1[j], a, c);
+      g1[j] = a;
+      d1[j] = c;
+    }
+    // (registers indexed by the rolled loop's s live in gd through a select, not a dynamic index)
+#pragma unroll
+    for (int u = 0; u < KS; ++u)
+      if (u == s) {
+        gd[2 * u] = d0;
+        if (2 * u + 1 < CT) gd[2 * u + 1] = d1;
+      }
+    const Frag gf = split8(g0, g1);
+#pragma unroll
+    for (int t = 0; t < CT; ++t) {
+      const int o = (16 * t + v) * wld(KS * 32) + 32 * s + 8 * q;
+      sg[t] = mfma3(*(const bf16x8*)(A2.h + o), *(const bf16x8*)(A2.l + o), gf, sg[t]);
+    }
+  }
+  f32x4 og[CT];
 #pragma unroll
   for (int t = 0; t < CT; ++t) {
     sg[t] = sigm4(sg[t]);
@@ -690,7 +723,26 @@ __device__ __forceinline__ void gate_bwd(Wm A1, Wm A2, Wm A2T, Wm A1T, const Fra
   }
   Frag ogf[KS];
   to_frags<CT>(og, ogf);
-  mlp2_bwd<KS, CT, KS, CT>(A1, A2T, A1T, hf, ogf, hb, v, q);
+  // adjoint: hb += A1^T (gelu'(A1 h) * A2^T og)
+#pragma unroll 1
+  for (int s = 0; s < KS; ++s) {
+    f32x4 d0 = zero4(), d1 = zero4();
+#pragma unroll
+    for (int u = 0; u < KS; ++u)
+      if (u == s) {
+        d0 = gd[2 * u];
+        if (2 * u + 1 < CT) d1 = gd[2 * u + 1];
+      }
+    const f32x4 e0 = tile<KS>(A2T, 2 * s, ogf, v, q) * d0;
+    f32x4 e1 = zero4();
+    if (2 * s + 1 < CT) e1 = tile<KS>(A2T, 2 * s + 1, ogf, v, q) * d1;
+    const Frag df = split8(e0, e1);
+#pragma unroll
+    for (int t = 0; t < CT; ++t) {
+      const int o = (16 * t + v) * wld(KS * 32) + 32 * s + 8 * q;
+      hb[t] = mfma3(*(const bf16x8*)(A1T.h + o), *(const bf16x8*)(A1T.l + o), df, hb[t]);
+    }
+  }
 }
 
 template <int CT, int NW, bool LW>

@@ -137,6 +137,13 @@ __device__ __forceinline__ float gelu_grad(float x) {
   const float c = x >= 0.0f ? 1.0f - tail : tail;
   return fmaf(x * 0.3989422804014327f, e, c);
 }
+// GELU and its derivative from one tail evaluation
+__device__ __forceinline__ void gelu_and_grad(float x, float& g, float& d) {
+  float e;
+  const float tail = norm_tail(x, e);
+  g = fmaf(-fabsf(x), tail, fmaxf(x, 0.0f));
+  d = fmaf(x * 0.3989422804014327f, e, x >= 0.0f ? 1.0f - tail : tail);
+}
 __device__ __forceinline__ f32x4 gelu_grad4(f32x4 x) {
   return f32x4{gelu_grad(x[0]), gelu_grad(x[1]), gelu_grad(x[2]), gelu_grad(x[3])};
 }
