@@ -21,7 +21,8 @@ from ._lib import FfnoAssemble, FfnoHeadDesc, FfnoMix, FfnoRows, FfnoStatsOut, F
 
 SUPPORTED_WIDTHS = (16, 32, 64, 96, 128)
 GROUPS = 8
-NW = 4                       # waves per workgroup of the voxel-row kernels
+ROW_TILES_PER_WG = 24         # 16-voxel tiles per workgroup of the voxel-row kernels, at least (small requests)
+ROW_WGS_PER_CU = 3            # workgroups per CU and batch item, at most (large requests: three rounds of staging)
 WPAD = 8                     # zero bf16 after every packed weight row (the kernels' row stride is K_pad + WPAD)
 TABLE_CACHE_BYTES = 8 << 30  # a branch's per-mode tables of all blocks stay cached when at most this large
 MODE_CACHE_BYTES = 24 << 30  # all cached per-mode tables of an engine (least recently used sets are dropped)
@@ -321,7 +322,11 @@ class Plan:
         self._mode_tables = dict(spec=engine.mode_tables("spec", H, W, float(dx), float(dy)),
                                  vertical=engine.mode_tables("vertical", self.length, tuple(k.tolist())))
         nt = (self.voxels + 15) // 16
-        self.nwg = max(1, min((nt + NW - 1) // NW, max(1, 2048 // B)))
+        # workgroups per batch item of the voxel-row kernels: enough tiles per workgroup that its waves all have work
+        # (small requests), at most ROW_WGS_PER_CU per CU across the batch (large ones: each extra round restages the
+        # workgroup's weights into LDS)
+        cus = torch.cuda.get_device_properties(dev).multi_processor_count if dev.type == "cuda" else 256
+        self.nwg = max(1, min(-(-nt // ROW_TILES_PER_WG), max(1, ROW_WGS_PER_CU * cus // B)))
         self.rows = FfnoRows(B, self.voxels, C, self.nwg)
         self.device = dev
         _settle(dev)                            # the plan's matrices are read from every worker's stream
