@@ -732,8 +732,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NC == 2 ? 2
 constexpr int MIX_TILES = 8;   // row tiles of 16 per work item of the mix
 
 // NL: planes of wc staged in LDS (0 or 2 for the two-way form, 0, 2 or 3 for the three-way one)
-template <int CT, int NL, bool THREE>
-__global__ __launch_bounds__(256) void k_mix(FfnoMix mx) {
+template <int CT, int NL, bool THREE, int NWM>
+__global__ __launch_bounds__(NWM * 64) void k_mix(FfnoMix mx) {
   using S = SplitT<THREE>;
   using F = typename S::F;
   constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32, NWC = 2 * C * wld(2 * KP), P = S::P;
@@ -750,7 +750,7 @@ __global__ __launch_bounds__(256) void k_mix(FfnoMix mx) {
   const int rtiles = (mx.rows + 15) / 16;
   const int nchunk = (rtiles + MIX_TILES - 1) / MIX_TILES;
   const int64_t items = (int64_t)mx.modes * nchunk;
-  for (int64_t item = (int64_t)blockIdx.x * 4 + L.wave; item < items; item += (int64_t)gridDim.x * 4) {
+  for (int64_t item = (int64_t)blockIdx.x * NWM + L.wave; item < items; item += (int64_t)gridDim.x * NWM) {
     const int mode = (int)(item / nchunk);
     const int rt0 = (int)(item % nchunk) * MIX_TILES;
     const int rt1 = rt0 + MIX_TILES < rtiles ? rt0 + MIX_TILES : rtiles;
@@ -1007,23 +1007,27 @@ int ffno_mix(const FfnoMix* m, hipStream_t stream) {
   if (m->rank && (!m->v_p || !m->u_p || m->rank > 16)) return fail("ffno_mix: low-rank tables missing or rank > 16");
   if (m->rows <= 0 || m->modes <= 0 || m->r_split <= 0) return fail("ffno_mix: bad sizes");
   const int64_t items = (int64_t)m->modes * (((m->rows + 15) / 16 + MIX_TILES - 1) / MIX_TILES);
-  const int64_t blocks = (items + 3) / 4;
-  const int grid = (int)(blocks < 2048 ? blocks : 2048);
   if (m->parts != 2 && m->parts != 3) return fail("ffno_mix: parts must be 2 or 3");
+  // the staged wc allows one workgroup per CU: its waves are the occupancy (8 for the three-way form's registers,
+  // 12 for the two-way one's, 4 at widths whose wc is read from global memory)
+  auto launch = [&](auto k, int nw, size_t lds) -> int {
+    if (set_lds(k, lds)) return -1;
+    const int64_t blocks = (items + nw - 1) / nw;
+    const int grid = (int)(blocks < 2048 ? blocks : 2048);
+    k<<<grid, nw * 64, lds, stream>>>(*m);
+    return 0;
+  };
   FFNO_DISPATCH_CT(m->C, {
+    constexpr int NW3 = CT <= 4 ? 8 : 4, NW2 = CT <= 4 ? 12 : 4;
     const size_t elems = (size_t)2 * CT * 16 * wld(2 * ks_of(CT) * 32);
     if (m->parts == 3) {
       FFNO_DISPATCH_NL(lds_planes(elems, 0), {
-        auto k = k_mix<CT, NL, true>;
-        if (set_lds(k, NL * elems * 2)) return -1;
-        k<<<grid, 256, NL * elems * 2, stream>>>(*m);
+        if (launch(k_mix<CT, NL, true, NW3>, NW3, NL * elems * 2)) return -1;
       });
     } else if (2 * elems * 2 <= 160 * 1024) {
-      auto k = k_mix<CT, 2, false>;
-      if (set_lds(k, 2 * elems * 2)) return -1;
-      k<<<grid, 256, 2 * elems * 2, stream>>>(*m);
+      if (launch(k_mix<CT, 2, false, NW2>, NW2, 2 * elems * 2)) return -1;
     } else {
-      k_mix<CT, 0, false><<<grid, 256, 0, stream>>>(*m);
+      if (launch(k_mix<CT, 0, false, 4>, 4, 0)) return -1;
     }
   });
   return check_launch("ffno_mix");
