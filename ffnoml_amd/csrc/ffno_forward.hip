@@ -625,8 +625,11 @@ constexpr int TW_PAD = 8;
 // 128-row vertical inverse of a 128 x 64 x 64 x 64 batch of 8)
 constexpr int tw_cols(int mt, bool three) { return mt >= 4 && !(mt == 8 && three) ? 2 : 4; }
 
-template <int MT, bool THREE, int NC>
+// TWID: the first stage of a factorised complex DFT; output rows k1 (real part, tile a) and m_split + k1 (imaginary
+// part, tile a + MT/2) sit in the same lane and register slot, so the twiddle multiply is lane-local.
+template <int MT, bool THREE, int NC, bool TWID = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NC == 2 ? 2 : 1))) void k_tw_small(FfnoTwGemm g) {
+  static_assert(!TWID || MT % 2 == 0, "twiddle rows come in real/imaginary tile pairs");
   typedef typename ColVec<NC>::T V;
   extern __shared__ __align__(16) unsigned char smem[];
   const int kp = g.k_pad, ldk = kp + TW_PAD;
@@ -649,7 +652,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NC == 2 ? 2
   }
   for (int k = threadIdx.x; k < kp; k += blockDim.x) koff[k] = k < g.k ? imap(k, g.k_split, g.x_k_hi, g.x_k_lo) : -1;
   for (int m = threadIdx.x; m < MT * 16; m += blockDim.x)
-    moff[m] = m < g.m ? imap(m, g.m_split, g.y_m_hi, g.y_m_lo) : -1;
+    moff[m] = m < g.m && (!TWID || m % g.m_split < g.tw_n1) ? imap(m, g.m_split, g.y_m_hi, g.y_m_lo) : -1;
   __syncthreads();
   Lane L;
   const int64_t ntn = (g.n + 16 * NC - 1) / (16 * NC);
@@ -683,6 +686,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NC == 2 ? 2
   while (true) {
     float* yb = g.y + imap(task / ntn, g.b_split, g.y_b_hi, g.y_b_lo) + n0;
     const bool out_ok = nok;
+    const int n_cur = n0;
     const int64_t next = task + stride;
     f32x4 acc[MT][NC];
 #pragma unroll
@@ -707,20 +711,44 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NC == 2 ? 2
         tw_mac<THREE, NC>(fh, fm, fl, bh, bm, bl, acc[a]);
       }
     }
+    if constexpr (TWID) {
+      constexpr int RT = MT / 2;
+      const float2* tw = reinterpret_cast<const float2*>(g.twiddle) + (out_ok ? n_cur / g.tw_cols : 0);
 #pragma unroll
-    for (int a = 0; a < MT; ++a)
+      for (int a = 0; a < RT; ++a)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int64_t o = moff[a * 16 + 4 * L.q + j];
-        if (o >= 0 && out_ok) {
-          float* p = yb + o;
-          V r;
+        for (int j = 0; j < 4; ++j) {
+          const int k1 = a * 16 + 4 * L.q + j;
+          const int64_t o = moff[k1];
+          if (o >= 0 && out_ok) {
+            const float2 w = tw[(int64_t)k1 * g.tw_n2];
+            V re, im;
 #pragma unroll
-          for (int i = 0; i < NC; ++i) r[i] = acc[a][i][j];
-          if (g.accumulate) r += *(const V*)p;
-          *(V*)p = r;
+            for (int i = 0; i < NC; ++i) {
+              const float xr = acc[a][i][j], xi = acc[a + RT][i][j];
+              re[i] = xr * w.x - xi * w.y;
+              im[i] = xr * w.y + xi * w.x;
+            }
+            *(V*)(yb + o) = re;
+            *(V*)(yb + moff[RT * 16 + k1]) = im;
+          }
         }
-      }
+    } else {
+#pragma unroll
+      for (int a = 0; a < MT; ++a)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int64_t o = moff[a * 16 + 4 * L.q + j];
+          if (o >= 0 && out_ok) {
+            float* p = yb + o;
+            V r;
+#pragma unroll
+            for (int i = 0; i < NC; ++i) r[i] = acc[a][i][j];
+            if (g.accumulate) r += *(const V*)p;
+            *(V*)p = r;
+          }
+        }
+    }
     task = next;
     if (task >= tasks) break;
   }
@@ -956,6 +984,31 @@ int ffno_tw_gemm(const FfnoTwGemm* g, hipStream_t stream) {
   const int64_t tasks = (int64_t)g->batch * ((g->n + 63) / 64);
   const int64_t blocks = (tasks + 3) / 4;
   const bool three = g->a_l3 != nullptr;
+  if (g->twiddle) {
+    const int mt = g->m_pad / 16;
+    if (g->m_pad > 128 || g->k_pad > 128 || mt % 2 || g->m_split != g->m_pad / 2 || g->accumulate ||
+        g->tw_n1 <= 0 || g->tw_n1 > g->m_split || g->m != g->m_split + g->tw_n1 || g->tw_n2 <= 0 ||
+        g->tw_cols <= 0 || g->tw_cols % 4 || (g->n + g->tw_cols - 1) / g->tw_cols > g->tw_n2)
+      return fail("ffno_tw_gemm: bad twiddle stage");
+    const size_t lds = (size_t)(g->k_pad + mt * 16) * sizeof(int64_t) +
+                       (size_t)(three ? 3 : 2) * mt * 16 * (g->k_pad + TW_PAD) * sizeof(uint16_t);
+    const int nc = tw_cols(mt, three);
+    const int64_t sblocks = ((int64_t)g->batch * ((g->n + 16 * nc - 1) / (16 * nc)) + 3) / 4;
+    const int grid = (int)(sblocks < 512 ? sblocks : 512);
+    switch (mt) {
+#define FFNO_TWID_CASE(M)                                                                                 \
+  case M: {                                                                                               \
+    auto k = three ? k_tw_small<M, true, tw_cols(M, true), true> : k_tw_small<M, false, tw_cols(M, false), true>; \
+    if (set_lds(k, lds)) return -1;                                                                       \
+    k<<<grid, 256, lds, stream>>>(*g);                                                                    \
+    break;                                                                                                \
+  }
+      FFNO_TWID_CASE(2) FFNO_TWID_CASE(4) FFNO_TWID_CASE(6) FFNO_TWID_CASE(8)
+#undef FFNO_TWID_CASE
+      default: return fail("ffno_tw_gemm: bad tile count");
+    }
+    return check_launch("ffno_tw_gemm");
+  }
   if (g->m_pad <= 128 && g->k_pad <= 128) {
     const int mt = (g->m_pad + 15) / 16;
     const size_t lds = (size_t)(g->k_pad + mt * 16) * sizeof(int64_t) +

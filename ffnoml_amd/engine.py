@@ -310,6 +310,18 @@ class Plan:
             v_fwd, v_inv = T.vertical_matrices(column, k, factor, self.length)
             named += [("v_fwd", v_fwd), ("v_inv", v_inv)]
         self.tw = {name: T.split_bf16(a, dev, parts=3) for name, a in named}
+        # along H beyond 64 rows (a [2H x 2H] matrix no longer fits the LDS-staged kernel, and its cost grows as
+        # H^2 per column) the complex DFTs and their transposes run factorised: two small GEMMs, the first with
+        # the twiddle factors applied to its output (forward ones three-way, adjoint ones two-way)
+        self.hfac = None
+        f = T.dft_factors(H) if 2 * H > 128 else None
+        if f is not None:
+            self.hfac = dict(n=f)
+            for name, sign, scale, parts in (("h_fwd", -1, 1.0, 3), ("h_inv", 1, 1.0 / H, 3),
+                                             ("h_inv_T", -1, 1.0 / H, 2), ("h_fwd_T", 1, 1.0, 2)):
+                a, tw, b = T.dft_stages(H, f[0], f[1], sign, scale)
+                self.hfac[name] = (T.split_bf16(a, dev, parts=parts), torch.from_numpy(tw).to(dev),
+                                   T.split_bf16(b, dev, parts=parts))
         # the adjoint chain runs the same GEMMs with the transposed matrices in reverse order
         self._mats = dict(named)
         self._tw_adj = None
@@ -598,7 +610,7 @@ class Engine:
                   _p(a), _p(b), self._stream())
         return a, b, stats
 
-    def _tw(self, mat, x, y, batch, n, bmap_x, bmap_y, kmap, mmap, accumulate=0):
+    def _tw(self, mat, x, y, batch, n, bmap_x, bmap_y, kmap, mmap, accumulate=0, twiddle=None):
         hi, lo, m, k, mp, kp = mat[:6]
         g = FfnoTwGemm()
         g.a_hi, g.a_lo = _p(hi), _p(lo)
@@ -610,7 +622,21 @@ class Engine:
         g.k_split, g.x_k_hi, g.x_k_lo = kmap
         g.m_split, g.y_m_hi, g.y_m_lo = mmap
         g.accumulate = accumulate
+        if twiddle is not None:
+            g.twiddle = _p(twiddle[0])
+            g.tw_n1, g.tw_n2, g.tw_cols = twiddle[1:]
         _lib.call("ffno_tw_gemm", ctypes.byref(g), self._stream())
+
+    def _hdft(self, plan, stages, s, tmp, pc):
+        """The complex DFT along H of pc (b, d) planes of s (layout [part][H][Wf][C]) in place, factorised
+        (transforms.dft_stages): stage A into tmp (layout [part][k1][j2][Wf][C]), then stage B back into s."""
+        H, row, pl = plan.H, plan.wf * plan.C, plan.plane
+        n1, n2 = plan.hfac["n"]
+        a, tw, b = stages
+        big = 1 << 40
+        self._tw(a, s, tmp, pc, n2 * row, (big, 0, pl), (0, pl), (n1, H * row, n2 * row), (a[4] // 2, H * row, n2 * row),
+                 twiddle=(tw, n1, n2, row))
+        self._tw(b, tmp, s, pc * n1, row, (n1, pl, n2 * row), (pl, row), (n2, H * row, row), (n2, H * row, n1 * row))
 
     def _mix(self, plan, re, im, rows, modes, mode_stride, rmap, wc, e, adjoint=False):
         mx = FfnoMix()
@@ -641,18 +667,29 @@ class Engine:
             m1, m2, m3, m4 = plan.tw["w_fwd"], plan.tw["h_fwd"], plan.tw["h_inv"], plan.tw["w_inv"]
             wc = self.weights.blocks[i]["spec.wc"]
         s1, s2 = scr["s1"], scr["s2"]
+        hf = plan.hfac
+        if hf is not None:
+            f2, f3 = (hf["h_inv_T"], hf["h_fwd_T"]) if adjoint else (hf["h_fwd"], hf["h_inv"])
+        spec = s1 if hf is not None else s2        # where the spectrum is mixed
         xf, of = xs.view(-1), out.view(-1)
         for p0 in range(0, B * D, plan.h_chunk):
             pc = min(plan.h_chunk, B * D - p0)
             xc, oc = xf[p0 * hwc:], of[p0 * hwc:]
             # along W, real -> half spectrum: one GEMM per voxel row (b, d, h)
             self._tw(m1, xc, s1, pc * H, C, (H, hwc, W * C), (plane, wf * C), (big, 0, C), (wf, H * wf * C, C))
-            # along H, complex: one GEMM per plane (b, d) over (part, h) rows, columns (kx, c)
-            self._tw(m2, s1, s2, pc, wf * C, (big, 0, plane), (0, plane), (H, H * wf * C, wf * C),
-                     (H, H * wf * C, wf * C))
-            self._mix(plan, s2, s2[H * wf * C:], pc, H * wf, C, (big, 0, plane), wc, tables["spec"], adjoint)
-            self._tw(m3, s2, s1, pc, wf * C, (big, 0, plane), (0, plane), (H, H * wf * C, wf * C),
-                     (H, H * wf * C, wf * C))
+            # along H, complex: one GEMM per plane (b, d) over (part, h) rows, columns (kx, c), or its factorised
+            # form in place in s1
+            if hf is not None:
+                self._hdft(plan, f2, s1, s2, pc)
+            else:
+                self._tw(m2, s1, s2, pc, wf * C, (big, 0, plane), (0, plane), (H, H * wf * C, wf * C),
+                         (H, H * wf * C, wf * C))
+            self._mix(plan, spec, spec[H * wf * C:], pc, H * wf, C, (big, 0, plane), wc, tables["spec"], adjoint)
+            if hf is not None:
+                self._hdft(plan, f3, s1, s2, pc)
+            else:
+                self._tw(m3, s2, s1, pc, wf * C, (big, 0, plane), (0, plane), (H, H * wf * C, wf * C),
+                         (H, H * wf * C, wf * C))
             # half spectrum -> real along W
             self._tw(m4, s1, oc, pc * H, C, (H, plane, wf * C), (hwc, W * C), (wf, H * wf * C, C), (big, 0, C))
 

@@ -114,6 +114,48 @@ def vertical_matrices(column, k, factor, length):
     return v_fwd, v_inv
 
 
+def dft_factors(n, largest=64):
+    """(n1, n2), n1 * n2 = n, both in [2, largest], that needs the fewest MFMA tile steps over the two stages of the
+    factorised DFT (dft_stages), or None when n has no such split."""
+    best = None
+    for n1 in range(2, largest + 1):
+        n2 = n // n1
+        if n % n1 or not 2 <= n2 <= largest:
+            continue
+        r = round_up(n1, 16)
+        cost = (2 * r // 16) * (round_up(2 * n1, 32) // 32) + (round_up(2 * n2, 16) // 16) * (round_up(2 * n2, 32) // 32)
+        if best is None or cost < best[0]:
+            best = (cost, n1, n2)
+    return None if best is None else best[1:]
+
+
+def _cmul_block(theta, sign):
+    """Real block matrix of the complex products exp(sign i theta) z for a complex vector held as (real parts,
+    imaginary parts): [[cos, -sign sin], [sign sin, cos]]."""
+    c, s = np.cos(theta), np.sin(theta)
+    return np.block([[c, -sign * s], [sign * s, c]])
+
+
+def dft_stages(n, n1, n2, sign, scale):
+    """The complex DFT X[k] = scale sum_h x[h] exp(sign 2 pi i h k / n) in two stages (Cooley-Tukey, n = n1 n2,
+    h = n2 j1 + j2, k = k1 + n1 k2):
+      stage A  y[k1, j2] = exp(sign 2 pi i j2 k1 / n) sum_j1 exp(sign 2 pi i j1 k1 / n1) x[n2 j1 + j2]
+      stage B  X[k1 + n1 k2] = scale sum_j2 exp(sign 2 pi i j2 k2 / n2) y[k1, j2]
+    Returns (A [R + n1, 2 n1] with real-part rows 0..n1-1 and imaginary-part rows R..R+n1-1, R = n1 rounded up to
+    16; twiddle [n1, n2, 2] float32 (cos, sin of the stage-A factor); B [2 n2, 2 n2]), float64 matrices."""
+    r = round_up(n1, 16)
+    j = np.arange(n1)
+    blk = _cmul_block(2 * np.pi * np.outer(j, j) / n1, sign)
+    a = np.zeros((r + n1, 2 * n1))
+    a[:n1] = blk[:n1]
+    a[r:] = blk[n1:]
+    ph = 2 * np.pi * np.outer(np.arange(n1), np.arange(n2)) / n
+    tw = np.stack([np.cos(ph), sign * np.sin(ph)], -1).astype(np.float32)
+    j2 = np.arange(n2)
+    b = scale * _cmul_block(2 * np.pi * np.outer(j2, j2) / n2, sign)
+    return a, tw, b
+
+
 def split_bf16(a, device, parts=2):
     """(hi, lo, m, k, m_pad, k_pad[, l3]) of a float64 matrix [m, k] as bf16 planes [m_pad, k_pad], zero padded:
     hi = bf16(a), lo = bf16(a - hi), and with parts=3 l3 = bf16(a - hi - lo) (the forward transforms' three-way

@@ -186,7 +186,17 @@ class Checker:
         xk = self.imap_max(g.k, g.k_split, g.x_k_hi, g.x_k_lo)
         self.span(n, "x", g.x, (xb + xk + g.n) * F32)
         yb = self.imap_max(g.batch, g.b_split, g.y_b_hi, g.y_b_lo)
-        ym = self.imap_max(g.m, g.m_split, g.y_m_hi, g.y_m_lo)
+        if g.twiddle:
+            # the first stage of a factorised DFT: rows k1 < tw_n1 of each part are written, and the twiddle table
+            # [tw_n1][tw_n2] complex is read at the column groups present
+            assert g.m_pad <= 128 and g.k_pad <= 128 and g.m_pad % 32 == 0 and g.m_split == g.m_pad // 2
+            assert g.m == g.m_split + g.tw_n1 and 0 < g.tw_n1 <= g.m_split and not g.accumulate
+            assert g.tw_cols % 4 == 0 and -(-g.n // g.tw_cols) <= g.tw_n2
+            ym = max((m // g.m_split) * g.y_m_hi + (m % g.m_split) * g.y_m_lo
+                     for m in range(g.m) if m % g.m_split < g.tw_n1)
+            self.span(n, "twiddle", g.twiddle, g.tw_n1 * g.tw_n2 * 2 * F32)
+        else:
+            ym = self.imap_max(g.m, g.m_split, g.y_m_hi, g.y_m_lo)
         self.span(n, "y", g.y, (yb + ym + g.n) * F32)
 
     def ffno_col_gemm(self, n, gref, stream):

@@ -65,6 +65,36 @@ def test_tw_gemm_matches_float64_matmul(m, k, n, batch, accumulate, parts):
     assert err < (3e-5 if parts == 2 else 2e-6)
 
 
+@pytest.mark.parametrize("H,W,parts", [(96, 8, 3), (256, 6, 3), (504, 4, 3), (256, 6, 2)])
+def test_factorised_dft_along_h_matches_fft(H, W, parts):
+    """The engine's factorised complex DFT along H (stage A with the twiddle epilogue, stage B) on planes laid
+    out [part][H][Wf][C], against numpy.fft in float64: forward (three-way products) 2e-6 of range, the adjoint's
+    two-way transposed inverse 3e-5; the columns of every plane are independent transforms."""
+    from ffnoml_amd.engine import Engine
+    model = _small_model()
+    eng = Engine(model)
+    eng.sync(torch.device(DEV))
+    B, D, C = 1, 3, model.width
+    z = torch.linspace(14.0, -1.0, D)[None, :, None, None].expand(B, D, H, W).contiguous().to(DEV)
+    plan = eng.plan(B, D, H, W, z, 48e3, 48e3)
+    wf = W // 2 + 1
+    name = "h_fwd" if parts == 3 else "h_fwd_T"
+    print(f"[hdft] H {H} factors {plan.hfac['n']} {name}", flush=True)
+    g = torch.Generator(device=DEV).manual_seed(H)
+    pc = B * D
+    s = torch.randn(pc * plan.plane, device=DEV, generator=g)
+    x = s.double().cpu().numpy().reshape(pc, 2, H, wf * C)
+    tmp = torch.empty_like(s)
+    eng._hdft(plan, plan.hfac[name], s, tmp, pc)
+    torch.cuda.synchronize()
+    got = s.double().cpu().numpy().reshape(pc, 2, H, wf * C)
+    xc = x[:, 0] + 1j * x[:, 1]
+    ref = np.fft.fft(xc, axis=1) if parts == 3 else H * np.fft.ifft(xc, axis=1)
+    err = np.abs(got[:, 0] + 1j * got[:, 1] - ref).max() / np.abs(ref).max()
+    print(f"[hdft] rel err {err:.3e}", flush=True)
+    assert err < (2e-6 if parts == 3 else 3e-5)
+
+
 def _small_model(width=64, rank=8, layers=1, species=("H", "CA")):
     from ffnoml_amd import synthetic
     from ffnoml_amd.atoms import species as sp
@@ -117,17 +147,20 @@ def test_post_mlp_and_groupnorm_statistics():
         assert ((stats[..., 2].double() - wtot).abs() / wtot).max() < 1e-5
 
 
-def test_spectral_pipelines_match_fft():
+@pytest.mark.parametrize("H,W", [(10, 8), (96, 6), (130, 5)])
+def test_spectral_pipelines_match_fft(H, W):
     """The horizontal rfft2 -> per-mode mix -> irfft2 and the vertical Filon -> mix -> synthesis, each as the
     engine runs them, against float64 torch.fft / einsum statements with the plan's own gates and low-rank tables:
-    1e-4 of range (split-bf16 transforms and mix)."""
+    1e-4 of range (split-bf16 transforms and mix). H = 96 and 130 take the factorised DFT along H (12 x 8 and
+    10 x 13)."""
     from ffnoml_amd.engine import Engine
     model = _small_model()
     eng = Engine(model)
     eng.sync(torch.device(DEV))
-    B, D, H, W, C = 2, 12, 10, 8, model.width
+    B, D, C = 2, 12, model.width
     z = torch.linspace(14.0, -1.0, D)[None, :, None, None].expand(B, D, H, W).contiguous().to(DEV)
     plan = eng.plan(B, D, H, W, z, 48e3, 48e3)
+    assert (plan.hfac is not None) == (H > 64)
     torch.manual_seed(2)
     xs = torch.randn(B, D, H, W, C, device=DEV)
     wf = W // 2 + 1
@@ -205,17 +238,17 @@ def test_spectral_pipelines_match_fft():
     assert err < 1e-4
 
 
-@pytest.mark.parametrize("chunk", [None, 3])
-def test_spectral_adjoints_are_transposes(chunk):
+@pytest.mark.parametrize("chunk,H", [(None, 12), (3, 12), (3, 72)])
+def test_spectral_adjoints_are_transposes(chunk, H):
     """<T x, y> = <x, T^T y> for the horizontal and vertical spectral maps as the engine runs them (forward and
     adjoint launch sequences, the per-mode gates and low-rank terms included), with the horizontal planes in one
-    chunk and in chunks of 3: 1e-5 relative."""
+    chunk and in chunks of 3, and along H = 72 factorised (9 x 8): 1e-5 relative."""
     from ffnoml_amd.engine import Engine
     model = _small_model()
     eng = Engine(model)
     dev = torch.device(DEV)
     eng.sync(dev)
-    B, D, H, W, C = 2, 10, 12, 9, model.width
+    B, D, W, C = 2, 10, 9, model.width
     z = torch.linspace(14.0, -1.0, D)[None, :, None, None].expand(B, D, H, W).contiguous().to(DEV)
     plan = eng.plan(B, D, H, W, z, 48e3, 48e3)
     if chunk:

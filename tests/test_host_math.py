@@ -38,6 +38,40 @@ def test_horizontal_matrices_are_rfft2_and_irfft2(H, W):
     assert np.abs(y - yt).max() < 1e-12 * np.abs(yref).max()
 
 
+@pytest.mark.parametrize("n", [72, 96, 130, 256, 504, 1000])
+def test_factorised_dft_stages_are_the_fft(n):
+    """The two stages of the factorised DFT along H (stage A with its twiddle rows laid out as the kernel writes
+    them, stage B), applied as the engine's index maps apply them, against numpy.fft for both signs and the four
+    (sign, scale) pairs the engine uses, and against the dense matrices they replace: 1e-7 of range (the twiddle
+    table is float32). Each stage stays within the LDS-staged kernel's limits."""
+    rng = np.random.default_rng(n)
+    n1, n2 = T.dft_factors(n)
+    assert n1 * n2 == n and 2 * T.round_up(n1, 16) <= 128 and 2 * n2 <= 128
+    dense = T.horizontal_matrices(n, 4)
+    x = rng.standard_normal((n, 3)) + 1j * rng.standard_normal((n, 3))            # 3 columns
+    for name, sign, scale in (("h_fwd", -1, 1.0), ("h_inv", 1, 1.0 / n), ("h_inv_T", -1, 1.0 / n),
+                              ("h_fwd_T", 1, 1.0)):
+        a, tw, b = T.dft_stages(n, n1, n2, sign, scale)
+        r = T.round_up(n1, 16)
+        assert a.shape == (r + n1, 2 * n1) and np.all(a[n1:r] == 0)
+        # stage A: batch over j2 columns; input row (part, j1) at h = n2 j1 + j2
+        xin = np.concatenate([x.real, x.imag], 0).reshape(2, n1, n2, 3)
+        ya = np.einsum("mk,kjc->mjc", a, xin.reshape(2 * n1, n2, 3))                # [R + n1, j2, c]
+        y = (ya[:n1] + 1j * ya[r:]) * (tw[..., 0] + 1j * tw[..., 1])[:, :, None]    # [k1, j2, c]
+        # stage B: batch over k1; input row (part, j2); output row (part, k2) at k = k1 + n1 k2
+        yb = np.einsum("mk,bkc->bmc", b, np.concatenate([y.real, y.imag], 1))       # [k1, 2 n2, c]
+        got = (yb[:, :n2] + 1j * yb[:, n2:]).transpose(1, 0, 2).reshape(n, 3)       # [k2, k1] -> k
+        ref = scale * (np.fft.fft(x, axis=0) if sign < 0 else n * np.fft.ifft(x, axis=0))
+        assert np.abs(got - ref).max() < 1e-7 * np.abs(ref).max(), name
+        if name in dense:
+            d = dense[name] @ np.concatenate([x.real, x.imag], 0)
+            assert np.abs(d[:n] + 1j * d[n:] - got).max() < 1e-7 * np.abs(ref).max(), name
+        else:
+            d = dense[name[:-2]].T @ np.concatenate([x.real, x.imag], 0)
+            assert np.abs(d[:n] + 1j * d[n:] - got).max() < 1e-7 * np.abs(ref).max(), name
+    assert T.dft_factors(127) is None and T.dft_factors(64 * 67) is None
+
+
 def test_filon_matrices_match_oracle():
     from oracle import ffno_oracle as O
     z = np.linspace(15.0, 0.0, 37) ** 1.1 - 1.0
