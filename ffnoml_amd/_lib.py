@@ -43,7 +43,8 @@ class FfnoTwGemm(ctypes.Structure):
                 ("y", c_ptr), ("b_split", c_i64), ("x_b_hi", c_i64), ("x_b_lo", c_i64), ("y_b_hi", c_i64),
                 ("y_b_lo", c_i64), ("k_split", c_i64), ("x_k_hi", c_i64), ("x_k_lo", c_i64), ("m_split", c_i64),
                 ("y_m_hi", c_i64), ("y_m_lo", c_i64), ("accumulate", ctypes.c_int), ("a_l3", c_ptr),
-                ("twiddle", c_ptr), ("tw_n1", ctypes.c_int), ("tw_n2", ctypes.c_int), ("tw_cols", c_i64)]
+                ("twiddle", c_ptr), ("tw_n1", ctypes.c_int), ("tw_n2", ctypes.c_int), ("tw_cols", c_i64),
+                ("tw_bsplit", c_i64)]
 
 
 class FfnoColGemm(ctypes.Structure):

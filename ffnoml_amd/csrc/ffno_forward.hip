@@ -713,7 +713,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NC == 2 ? 2
     }
     if constexpr (TWID) {
       constexpr int RT = MT / 2;
-      const float2* tw = reinterpret_cast<const float2*>(g.twiddle) + (out_ok ? n_cur / g.tw_cols : 0);
+      const int64_t tcol = g.tw_cols > 0 ? (out_ok ? n_cur / g.tw_cols : 0) : task / ntn % g.tw_bsplit;
+      const float2* tw = reinterpret_cast<const float2*>(g.twiddle) + tcol;
 #pragma unroll
       for (int a = 0; a < RT; ++a)
 #pragma unroll
@@ -988,7 +989,8 @@ int ffno_tw_gemm(const FfnoTwGemm* g, hipStream_t stream) {
     const int mt = g->m_pad / 16;
     if (g->m_pad > 128 || g->k_pad > 128 || mt % 2 || g->m_split != g->m_pad / 2 || g->accumulate ||
         g->tw_n1 <= 0 || g->tw_n1 > g->m_split || g->m != g->m_split + g->tw_n1 || g->tw_n2 <= 0 ||
-        g->tw_cols <= 0 || g->tw_cols % 4 || (g->n + g->tw_cols - 1) / g->tw_cols > g->tw_n2)
+        g->tw_cols < 0 || g->tw_cols % 4 || (g->tw_cols > 0 && (g->n + g->tw_cols - 1) / g->tw_cols > g->tw_n2) ||
+        (g->tw_cols == 0 && (g->tw_bsplit <= 0 || g->tw_bsplit > g->tw_n2)))
       return fail("ffno_tw_gemm: bad twiddle stage");
     const size_t lds = (size_t)(g->k_pad + mt * 16) * sizeof(int64_t) +
                        (size_t)(three ? 3 : 2) * mt * 16 * (g->k_pad + TW_PAD) * sizeof(uint16_t);

@@ -28,6 +28,8 @@ TABLE_CACHE_BYTES = 12 << 30  # a branch's per-mode tables of all blocks stay ca
 #                               256 x 256 tile's horizontal set, 8 blocks at width 64, is 9.4 GiB)
 MODE_CACHE_BYTES = 24 << 30  # all cached per-mode tables of an engine (least recently used sets are dropped)
 H_CHUNK_BYTES = 2 << 30      # horizontal half-spectrum scratch per buffer: (b, d) planes are transformed in chunks
+DENSE_DFT_MAX = 128          # horizontal DFT matrices up to this many rows and columns run as one LDS-staged GEMM;
+#                              larger ones run factorised (two stages) when the length splits into factors <= 64
 WORKING_UNITS = 9            # activation-sized buffers of the adjoint besides the saved block inputs
 GATE_UNITS = 2               # the stored branch gates of the two-pass fusion adjoint
 
@@ -45,6 +47,21 @@ def _settle(device):
 
 def _p(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def horizontal_factors(H, W):
+    """(factors of H or None, factors of W or None): the horizontal DFTs that run factorised."""
+    wf = W // 2 + 1
+    fh = T.dft_factors(H) if 2 * H > DENSE_DFT_MAX else None
+    fw = T.dft_factors(W) if max(T.round_up(2 * wf, 16), T.round_up(W, 32)) > DENSE_DFT_MAX else None
+    return fh, fw
+
+
+def horizontal_planes(H, W, C):
+    """(floats of one (b, d) plane's half spectrum, floats per plane of the second horizontal scratch buffer: the
+    half spectrum, or with W factorised the larger of that and its full-length complex intermediate)."""
+    plane = 2 * H * (W // 2 + 1) * C
+    return plane, (max(plane, 2 * H * W * C) if horizontal_factors(H, W)[1] is not None else plane)
 
 
 def pack_mlp_weight(w, segments=None, n_pad=None, parts=2):
@@ -313,8 +330,24 @@ class Plan:
         # along H beyond 64 rows (a [2H x 2H] matrix no longer fits the LDS-staged kernel, and its cost grows as
         # H^2 per column) the complex DFTs and their transposes run factorised: two small GEMMs, the first with
         # the twiddle factors applied to its output (forward ones three-way, adjoint ones two-way)
-        self.hfac = None
-        f = T.dft_factors(H) if 2 * H > 128 else None
+        self.hfac = self.wfac = None
+        f, fw = horizontal_factors(H, W)
+        if fw is not None:
+            # along W: real rows -> half spectrum (forward; with the irfft weights and 1/W the adjoint of the
+            # inverse) and back (inverse with the irfft weights and 1/W; unweighted the adjoint of the forward)
+            self.wfac = dict(n=fw)
+            for name, real_in, scale, weighted, parts in (("w_fwd", True, 1.0, False, 3), ("w_inv", False, 1.0 / W, True, 3),
+                                                          ("w_inv_T", True, 1.0 / W, True, 2),
+                                                          ("w_fwd_T", False, 1.0, False, 2)):
+                sp = lambda a: T.split_bf16(a, dev, parts=parts)      # noqa: E731
+                if real_in:
+                    a, tw, runs = T.rdft_stages(W, fw[0], fw[1], scale, weighted)
+                    self.wfac[name] = (True, sp(a), torch.from_numpy(tw).to(dev),
+                                       [(st, cnt, sp(b)) for st, cnt, b in runs])
+                else:
+                    runs, tw, b = T.irdft_stages(W, fw[0], fw[1], scale, weighted)
+                    self.wfac[name] = (False, [(st, cnt, sp(a)) for st, cnt, a in runs],
+                                       torch.from_numpy(tw).to(dev), sp(b))
         if f is not None:
             self.hfac = dict(n=f)
             for name, sign, scale, parts in (("h_fwd", -1, 1.0, 3), ("h_inv", 1, 1.0 / H, 3),
@@ -329,8 +362,8 @@ class Plan:
         self._height = None
         self._lock = threading.Lock()
         # horizontal transforms run over chunks of (b, d) planes so that their half-spectrum scratch stays bounded
-        self.plane = 2 * H * self.wf * C
-        self.h_chunk = max(1, min(B * D, H_CHUNK_BYTES // (4 * self.plane)))
+        self.plane, self.plane2 = horizontal_planes(H, W, C)
+        self.h_chunk = max(1, min(B * D, H_CHUNK_BYTES // (4 * self.plane2)))
         # per-mode tables, shared with every plan of the same modes
         self._mode_tables = dict(spec=engine.mode_tables("spec", H, W, float(dx), float(dy)),
                                  vertical=engine.mode_tables("vertical", self.length, tuple(k.tolist())))
@@ -421,7 +454,7 @@ class Plan:
     def scratch_bytes(self):
         """Half-spectrum chunks, both vertical spectra, GroupNorm partials, inputs and outputs."""
         v1 = self.B * 2 * self.modes * self.H * self.W * self.C * 4
-        return 2 * self.h_chunk * self.plane * 4 + 2 * v1 + self.B * self.voxels * 4 * 32
+        return self.h_chunk * (self.plane + self.plane2) * 4 + 2 * v1 + self.B * self.voxels * 4 * 32
 
     def full_bytes(self, want_z=True):
         """The adjoint keeping every block's intermediates: 11 units per block (input, both branch outputs, five
@@ -482,13 +515,12 @@ class Engine:
         intermediates, its gates and four tangents), plus the scratch."""
         C, L, r = self.module.width, self.module.n_layers, self.module.freq_mix_rank
         unit = B * D * H * W * C * 4
-        wf = W // 2 + 1
-        plane = 2 * H * wf * C * 4
-        chunk = max(1, min(B * D, H_CHUNK_BYTES // plane)) * plane
+        plane, plane2 = horizontal_planes(H, W, C)
+        chunk = max(1, min(B * D, H_CHUNK_BYTES // (4 * plane2))) * (plane + plane2) * 4
         modes = int(math.floor(self.module.z_reference_length / self.module.z_min_wavelength)) + 1
         v1 = B * 2 * modes * H * W * C * 4
         units = (L + 1) + WORKING_UNITS + GATE_UNITS if vjp else (16 if jvp else 3)
-        return unit * units + 2 * chunk + (2 if vjp else 1) * v1 + B * D * H * W * 4 * 32 + (64 << 20)
+        return unit * units + chunk + (2 if vjp else 1) * v1 + B * D * H * W * 4 * 32 + (64 << 20)
 
     def _param_version(self):
         return tuple((p.data_ptr(), p._version) for p in self.module.parameters()) + \
@@ -624,19 +656,69 @@ class Engine:
         g.accumulate = accumulate
         if twiddle is not None:
             g.twiddle = _p(twiddle[0])
-            g.tw_n1, g.tw_n2, g.tw_cols = twiddle[1:]
+            g.tw_n1, g.tw_n2, g.tw_cols, g.tw_bsplit = twiddle[1:]
         _lib.call("ffno_tw_gemm", ctypes.byref(g), self._stream())
 
-    def _hdft(self, plan, stages, s, tmp, pc):
-        """The complex DFT along H of pc (b, d) planes of s (layout [part][H][Wf][C]) in place, factorised
-        (transforms.dft_stages): stage A into tmp (layout [part][k1][j2][Wf][C]), then stage B back into s."""
+    @staticmethod
+    def _w_side(plan):
+        """(part stride, h stride) of the half spectrum where the W-direction transforms meet the H-direction ones:
+        [part][H][Wf][C] per plane, or with W factorised row-major [H][part][Wf][C] (each voxel row's spectrum in
+        one piece, so that the W stages address it with two-level maps)."""
+        row = plan.wf * plan.C
+        return (row, 2 * row) if plan.wfac is not None else (plan.H * row, row)
+
+    def _hdft(self, plan, stages, s, tmp, pc, into_spectrum):
+        """The complex DFT along H of pc (b, d) planes of s in place, factorised (transforms.dft_stages): stage A
+        into tmp (layout [part][k1][j2][Wf][C]), then stage B back into s. into_spectrum: from the W-side layout
+        into [part][H][Wf][C] (the layout the mix reads); else the reverse."""
         H, row, pl = plan.H, plan.wf * plan.C, plan.plane
+        ps, hs = self._w_side(plan)
         n1, n2 = plan.hfac["n"]
         a, tw, b = stages
         big = 1 << 40
-        self._tw(a, s, tmp, pc, n2 * row, (big, 0, pl), (0, pl), (n1, H * row, n2 * row), (a[4] // 2, H * row, n2 * row),
-                 twiddle=(tw, n1, n2, row))
-        self._tw(b, tmp, s, pc * n1, row, (n1, pl, n2 * row), (pl, row), (n2, H * row, row), (n2, H * row, n1 * row))
+        r = a[4] // 2
+        in_ps, in_hs = (ps, hs) if into_spectrum else (H * row, row)
+        out_ps, out_hs = (H * row, row) if into_spectrum else (ps, hs)
+        if in_hs == row:
+            # rows j2 of a j1 are contiguous: columns (j2, kx, c), the twiddle by column group
+            self._tw(a, s, tmp, pc, n2 * row, (big, 0, pl), (0, pl), (n1, in_ps, n2 * row), (r, H * row, n2 * row),
+                     twiddle=(tw, n1, n2, row, 0))
+        else:
+            # batch items (plane, j2), columns (kx, c), the twiddle by batch item
+            self._tw(a, s, tmp, pc * n2, row, (n2, pl, in_hs), (pl, row), (n1, in_ps, n2 * in_hs),
+                     (r, H * row, n2 * row), twiddle=(tw, n1, n2, 0, n2))
+        self._tw(b, tmp, s, pc * n1, row, (n1, pl, n2 * row), (pl, out_hs), (n2, H * row, row),
+                 (n2, out_ps, n1 * out_hs))
+
+    def _wdft_r(self, plan, st, x, s, tmp, rows):
+        """Real voxel rows (W x C, row-major) -> their half spectra in the W-side layout, factorised
+        (transforms.rdft_stages): stage A into tmp (per row [part][k1][j2][C]), stage B by runs of k1 into s."""
+        W, C, wf = plan.W, plan.C, plan.wf
+        n1, n2 = plan.wfac["n"]
+        _, a, tw, runs = st
+        big = 1 << 40
+        wc = W * C
+        self._tw(a, x, tmp, rows, n2 * C, (big, 0, wc), (0, 2 * wc), (big, 0, n2 * C), (a[4] // 2, wc, n2 * C),
+                 twiddle=(tw, n1, n2, C, 0))
+        for k0, cnt, b in runs:
+            K2 = b[2] // 2
+            self._tw(b, tmp[k0 * n2 * C:], s[k0 * C:], rows * cnt, C, (cnt, 2 * wc, n2 * C), (2 * wf * C, C),
+                     (n2, wc, C), (K2, wf * C, n1 * C))
+
+    def _wdft_i(self, plan, st, s, out, tmp, rows):
+        """Half spectra in the W-side layout -> real voxel rows (W x C), factorised (transforms.irdft_stages): stage A
+        by runs of j2 into tmp (per row [part][k1][j2][C]), stage B into out."""
+        W, C, wf = plan.W, plan.C, plan.wf
+        n1, n2 = plan.wfac["n"]
+        _, runs, tw, b = st
+        big = 1 << 40
+        wc = W * C
+        twf = tw.view(-1)
+        for j0, cnt, a in runs:
+            J1 = a[3] // 2
+            self._tw(a, s[j0 * C:], tmp[j0 * C:], rows * cnt, C, (cnt, 2 * wf * C, C), (2 * wc, C),
+                     (J1, wf * C, n2 * C), (a[4] // 2, wc, n2 * C), twiddle=(twf[2 * j0:], n1, n2, 0, cnt))
+        self._tw(b, tmp, out, rows * n1, C, (n1, 2 * wc, n2 * C), (wc, C), (n2, wc, C), (big, 0, n1 * C))
 
     def _mix(self, plan, re, im, rows, modes, mode_stride, rmap, wc, e, adjoint=False):
         mx = FfnoMix()
@@ -667,31 +749,39 @@ class Engine:
             m1, m2, m3, m4 = plan.tw["w_fwd"], plan.tw["h_fwd"], plan.tw["h_inv"], plan.tw["w_inv"]
             wc = self.weights.blocks[i]["spec.wc"]
         s1, s2 = scr["s1"], scr["s2"]
-        hf = plan.hfac
+        hf, wv = plan.hfac, plan.wfac
         if hf is not None:
             f2, f3 = (hf["h_inv_T"], hf["h_fwd_T"]) if adjoint else (hf["h_fwd"], hf["h_inv"])
+        if wv is not None:
+            w1, w4 = (wv["w_inv_T"], wv["w_fwd_T"]) if adjoint else (wv["w_fwd"], wv["w_inv"])
+        ps, hs = self._w_side(plan)
+        row = wf * C
         spec = s1 if hf is not None else s2        # where the spectrum is mixed
         xf, of = xs.view(-1), out.view(-1)
         for p0 in range(0, B * D, plan.h_chunk):
             pc = min(plan.h_chunk, B * D - p0)
             xc, oc = xf[p0 * hwc:], of[p0 * hwc:]
-            # along W, real -> half spectrum: one GEMM per voxel row (b, d, h)
-            self._tw(m1, xc, s1, pc * H, C, (H, hwc, W * C), (plane, wf * C), (big, 0, C), (wf, H * wf * C, C))
+            # along W, real -> half spectrum: one GEMM per voxel row (b, d, h), or its factorised form through s2
+            if wv is not None:
+                self._wdft_r(plan, w1, xc, s1, s2, pc * H)
+            else:
+                self._tw(m1, xc, s1, pc * H, C, (H, hwc, W * C), (plane, hs), (big, 0, C), (wf, ps, C))
             # along H, complex: one GEMM per plane (b, d) over (part, h) rows, columns (kx, c), or its factorised
             # form in place in s1
             if hf is not None:
-                self._hdft(plan, f2, s1, s2, pc)
+                self._hdft(plan, f2, s1, s2, pc, True)
             else:
-                self._tw(m2, s1, s2, pc, wf * C, (big, 0, plane), (0, plane), (H, H * wf * C, wf * C),
-                         (H, H * wf * C, wf * C))
-            self._mix(plan, spec, spec[H * wf * C:], pc, H * wf, C, (big, 0, plane), wc, tables["spec"], adjoint)
+                self._tw(m2, s1, s2, pc, row, (big, 0, plane), (0, plane), (H, ps, hs), (H, H * row, row))
+            self._mix(plan, spec, spec[H * row:], pc, H * wf, C, (big, 0, plane), wc, tables["spec"], adjoint)
             if hf is not None:
-                self._hdft(plan, f3, s1, s2, pc)
+                self._hdft(plan, f3, s1, s2, pc, False)
             else:
-                self._tw(m3, s2, s1, pc, wf * C, (big, 0, plane), (0, plane), (H, H * wf * C, wf * C),
-                         (H, H * wf * C, wf * C))
+                self._tw(m3, s2, s1, pc, row, (big, 0, plane), (0, plane), (H, H * row, row), (H, ps, hs))
             # half spectrum -> real along W
-            self._tw(m4, s1, oc, pc * H, C, (H, plane, wf * C), (hwc, W * C), (wf, H * wf * C, C), (big, 0, C))
+            if wv is not None:
+                self._wdft_i(plan, w4, s1, oc, s2, pc * H)
+            else:
+                self._tw(m4, s1, oc, pc * H, C, (H, plane, hs), (hwc, W * C), (wf, ps, C), (big, 0, C))
 
     def _colgemm(self, plan, mat, x, y):
         hi, lo, m, k, mp, kp = mat[:6]
@@ -753,7 +843,7 @@ class Engine:
 
     def _scratch(self, plan, dev, adjoint=False):
         f32 = dict(device=dev, dtype=torch.float32)
-        scr = dict(s1=torch.empty(plan.h_chunk * plan.plane, **f32), s2=torch.empty(plan.h_chunk * plan.plane, **f32),
+        scr = dict(s1=torch.empty(plan.h_chunk * plan.plane, **f32), s2=torch.empty(plan.h_chunk * plan.plane2, **f32),
                    v1=torch.empty(plan.B * 2 * plan.modes * plan.H * plan.W * plan.C, **f32),
                    part=torch.empty(plan.B, plan.nwg, GROUPS, 3, **f32))
         if adjoint:

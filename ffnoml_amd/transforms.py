@@ -156,6 +156,85 @@ def dft_stages(n, n1, n2, sign, scale):
     return a, tw, b
 
 
+def _runs(keys):
+    """[(start, count, key)] of the runs of equal consecutive keys."""
+    out = []
+    for i, key in enumerate(keys):
+        if out and out[-1][2] == key:
+            out[-1][1] += 1
+        else:
+            out.append([i, 1, key])
+    return [tuple(r) for r in out]
+
+
+def _irfft_weight(k, n):
+    return 1.0 if k == 0 or 2 * k == n else 2.0
+
+
+def rdft_stages(n, n1, n2, scale, weighted):
+    """Real rows of length n -> half spectrum (n // 2 + 1 bins): X[k] = scale w(k) sum_x x[j] exp(-2 pi i j k / n),
+    w(k) = 1, or with weighted the irfft bin weights (1 for bin 0 and, n even, bin n/2; 2 otherwise), in two
+    stages: j = n2 j1 + j2, k = k1 + n1 k2.
+      stage A  y[k1, j2] = exp(-2 pi i j2 k1 / n) sum_j1 exp(-2 pi i j1 k1 / n1) x[n2 j1 + j2]: A [R + n1, n1]
+               (real-part rows 0..n1-1, imaginary-part rows R..R+n1-1, R = n1 rounded up to 16), twiddle
+               [n1, n2, 2] float32
+      stage B  X[k1 + n1 k2] = scale w sum_j2 exp(-2 pi i j2 k2 / n2) y[k1, j2] for the bins below n // 2 + 1:
+               runs of k1 [(k1 start, count, B [2 K2, 2 n2])] with K2 the bins each k1 of the run reaches
+    Returns (A, twiddle, runs), float64 matrices."""
+    wf = n // 2 + 1
+    r = round_up(n1, 16)
+    j = np.arange(n1)
+    th = 2 * np.pi * np.outer(j, j) / n1                                   # [k1, j1]
+    a = np.zeros((r + n1, n1))
+    a[:n1], a[r:] = np.cos(th), -np.sin(th)
+    ph = 2 * np.pi * np.outer(np.arange(n1), np.arange(n2)) / n
+    tw = np.stack([np.cos(ph), -np.sin(ph)], -1).astype(np.float32)
+    j2 = np.arange(n2)
+    runs = []
+    keys = []
+    for k1 in range(n1):
+        K2 = -(-(wf - k1) // n1)
+        wts = tuple(_irfft_weight(k1 + n1 * k2, n) if weighted else 1.0 for k2 in range(K2))
+        keys.append((K2, wts))
+    for start, count, (K2, wts) in _runs(keys):
+        blk = _cmul_block(2 * np.pi * np.outer(np.arange(K2), j2) / n2, -1)   # [2 K2, 2 n2]
+        wv = scale * np.asarray(wts)
+        runs.append((start, count, blk * np.concatenate([wv, wv])[:, None]))
+    return a, tw, runs
+
+
+def irdft_stages(n, n1, n2, scale, weighted):
+    """Half spectrum (n // 2 + 1 bins) -> real rows of length n: x[j] = scale sum_k w(k) Re(Z[k] exp(2 pi i j k / n)),
+    w(k) = 1, or with weighted the irfft bin weights (so that weighted, scale 1/n is irfft for any spectrum, Hermitian
+    or not), in two stages: k = n2 j1 + j2, j = k1 + n1 k2.
+      stage A  y[k1, j2] = exp(2 pi i j2 k1 / n) sum_j1 w exp(2 pi i j1 k1 / n1) Z[n2 j1 + j2] over the bins below
+               n // 2 + 1: runs of j2 [(j2 start, count, A [R + n1, 2 J1])] with J1 the bins each j2 of the run
+               reaches (real-part rows 0..n1-1, imaginary-part rows R..R+n1-1; columns real parts, then imaginary),
+               twiddle [n1, n2, 2] float32 (indexed by the run's j2)
+      stage B  x[k1 + n1 k2] = scale Re sum_j2 exp(2 pi i j2 k2 / n2) y[k1, j2]: B [n2, 2 n2]
+    Returns (runs, twiddle, B), float64 matrices."""
+    wf = n // 2 + 1
+    r = round_up(n1, 16)
+    k1 = np.arange(n1)
+    keys = []
+    for j2 in range(n2):
+        J1 = -(-(wf - j2) // n2)
+        keys.append((J1, tuple(_irfft_weight(n2 * j1 + j2, n) if weighted else 1.0 for j1 in range(J1))))
+    runs = []
+    for start, count, (J1, wts) in _runs(keys):
+        blk = _cmul_block(2 * np.pi * np.outer(k1, np.arange(J1)) / n1, 1)   # [2 n1, 2 J1]
+        blk = blk * np.concatenate([wts, wts])[None, :]
+        a = np.zeros((r + n1, 2 * J1))
+        a[:n1], a[r:] = blk[:n1], blk[n1:]
+        runs.append((start, count, a))
+    ph = 2 * np.pi * np.outer(np.arange(n1), np.arange(n2)) / n
+    tw = np.stack([np.cos(ph), np.sin(ph)], -1).astype(np.float32)
+    j2 = np.arange(n2)
+    th = 2 * np.pi * np.outer(j2, j2) / n2                                  # [k2, j2]
+    b = scale * np.concatenate([np.cos(th), -np.sin(th)], 1)                # Re(exp(i th) y)
+    return runs, tw, b
+
+
 def split_bf16(a, device, parts=2):
     """(hi, lo, m, k, m_pad, k_pad[, l3]) of a float64 matrix [m, k] as bf16 planes [m_pad, k_pad], zero padded:
     hi = bf16(a), lo = bf16(a - hi), and with parts=3 l3 = bf16(a - hi - lo) (the forward transforms' three-way

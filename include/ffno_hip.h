@@ -130,14 +130,15 @@ typedef struct {
   int64_t m_split, y_m_hi, y_m_lo;
   int accumulate;         /* 1: Y += A X */
   const uint16_t* a_l3;   /* third plane of A or NULL */
-  /* Optional twiddle of a factorised complex DFT (the first of its two stages; NULL: none). A's rows are then
-   * [real part rows 0..tw_n1-1, zero rows up to m_split, imaginary part rows m_split..m_split+tw_n1-1], m_split =
-   * m_pad / 2 a multiple of 16, m_pad <= 128, k_pad <= 128, no accumulate. Output row pair (k1, m_split + k1) of
-   * column j is multiplied, as one complex number, by twiddle[k1 * tw_n2 + j / tw_cols] (complex float pairs);
-   * rows k1 >= tw_n1 of either part are not written. */
+  /* Optional twiddle of a factorised DFT (the first of its two stages; NULL: none). A's rows are then [real part
+   * rows 0..tw_n1-1, zero rows up to m_split, imaginary part rows m_split..m_split+tw_n1-1], m_split = m_pad / 2 a
+   * multiple of 16, m_pad <= 128, k_pad <= 128, no accumulate. Output row pair (k1, m_split + k1) of column j of
+   * batch item b is multiplied, as one complex number, by twiddle[k1 * tw_n2 + t] (complex float pairs), t =
+   * j / tw_cols when tw_cols > 0, else b % tw_bsplit; rows k1 >= tw_n1 of either part are not written. */
   const float* twiddle;
   int tw_n1, tw_n2;
-  int64_t tw_cols;        /* columns per twiddle column group, % 4 == 0 */
+  int64_t tw_cols;        /* columns per twiddle column group, % 4 == 0, or 0 */
+  int64_t tw_bsplit;      /* with tw_cols == 0: batch items per twiddle column cycle */
 } FfnoTwGemm;
 
 int ffno_tw_gemm(const FfnoTwGemm* g, hipStream_t stream);

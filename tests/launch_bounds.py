@@ -191,10 +191,16 @@ class Checker:
             # [tw_n1][tw_n2] complex is read at the column groups present
             assert g.m_pad <= 128 and g.k_pad <= 128 and g.m_pad % 32 == 0 and g.m_split == g.m_pad // 2
             assert g.m == g.m_split + g.tw_n1 and 0 < g.tw_n1 <= g.m_split and not g.accumulate
-            assert g.tw_cols % 4 == 0 and -(-g.n // g.tw_cols) <= g.tw_n2
+            assert g.tw_cols >= 0 and g.tw_cols % 4 == 0
+            if g.tw_cols:
+                tmax = -(-g.n // g.tw_cols) - 1                    # column group of the last column
+            else:
+                assert 0 < g.tw_bsplit <= g.tw_n2
+                tmax = min(g.batch, g.tw_bsplit) - 1               # batch item's place in its cycle
+            assert tmax < g.tw_n2
             ym = max((m // g.m_split) * g.y_m_hi + (m % g.m_split) * g.y_m_lo
                      for m in range(g.m) if m % g.m_split < g.tw_n1)
-            self.span(n, "twiddle", g.twiddle, g.tw_n1 * g.tw_n2 * 2 * F32)
+            self.span(n, "twiddle", g.twiddle, ((g.tw_n1 - 1) * g.tw_n2 + tmax + 1) * 2 * F32)
         else:
             ym = self.imap_max(g.m, g.m_split, g.y_m_hi, g.y_m_lo)
         self.span(n, "y", g.y, (yb + ym + g.n) * F32)

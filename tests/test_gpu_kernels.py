@@ -85,7 +85,7 @@ def test_factorised_dft_along_h_matches_fft(H, W, parts):
     s = torch.randn(pc * plan.plane, device=DEV, generator=g)
     x = s.double().cpu().numpy().reshape(pc, 2, H, wf * C)
     tmp = torch.empty_like(s)
-    eng._hdft(plan, plan.hfac[name], s, tmp, pc)
+    eng._hdft(plan, plan.hfac[name], s, tmp, pc, True)
     torch.cuda.synchronize()
     got = s.double().cpu().numpy().reshape(pc, 2, H, wf * C)
     xc = x[:, 0] + 1j * x[:, 1]
@@ -147,12 +147,12 @@ def test_post_mlp_and_groupnorm_statistics():
         assert ((stats[..., 2].double() - wtot).abs() / wtot).max() < 1e-5
 
 
-@pytest.mark.parametrize("H,W", [(10, 8), (96, 6), (130, 5)])
+@pytest.mark.parametrize("H,W", [(10, 8), (96, 6), (130, 5), (12, 130), (72, 256)])
 def test_spectral_pipelines_match_fft(H, W):
     """The horizontal rfft2 -> per-mode mix -> irfft2 and the vertical Filon -> mix -> synthesis, each as the
     engine runs them, against float64 torch.fft / einsum statements with the plan's own gates and low-rank tables:
-    1e-4 of range (split-bf16 transforms and mix). H = 96 and 130 take the factorised DFT along H (12 x 8 and
-    10 x 13)."""
+    1e-4 of range (split-bf16 transforms and mix). H = 96, 130 and 72 take the factorised DFT along H (12 x 8,
+    10 x 13, 9 x 8), W = 130 and 256 the factorised real DFTs along W."""
     from ffnoml_amd.engine import Engine
     model = _small_model()
     eng = Engine(model)
@@ -160,7 +160,7 @@ def test_spectral_pipelines_match_fft(H, W):
     B, D, C = 2, 12, model.width
     z = torch.linspace(14.0, -1.0, D)[None, :, None, None].expand(B, D, H, W).contiguous().to(DEV)
     plan = eng.plan(B, D, H, W, z, 48e3, 48e3)
-    assert (plan.hfac is not None) == (H > 64)
+    assert (plan.hfac is not None) == (H > 64) and (plan.wfac is not None) == (W > 126)
     torch.manual_seed(2)
     xs = torch.randn(B, D, H, W, C, device=DEV)
     wf = W // 2 + 1
@@ -238,17 +238,18 @@ def test_spectral_pipelines_match_fft(H, W):
     assert err < 1e-4
 
 
-@pytest.mark.parametrize("chunk,H", [(None, 12), (3, 12), (3, 72)])
-def test_spectral_adjoints_are_transposes(chunk, H):
+@pytest.mark.parametrize("chunk,H,W", [(None, 12, 9), (3, 12, 9), (3, 72, 9), (3, 12, 130), (None, 66, 200)])
+def test_spectral_adjoints_are_transposes(chunk, H, W):
     """<T x, y> = <x, T^T y> for the horizontal and vertical spectral maps as the engine runs them (forward and
     adjoint launch sequences, the per-mode gates and low-rank terms included), with the horizontal planes in one
-    chunk and in chunks of 3, and along H = 72 factorised (9 x 8): 1e-5 relative."""
+    chunk and in chunks of 3, and factorised along H (72 = 9 x 8, 66), along W (130, 200) and both: 1e-5
+    relative."""
     from ffnoml_amd.engine import Engine
     model = _small_model()
     eng = Engine(model)
     dev = torch.device(DEV)
     eng.sync(dev)
-    B, D, W, C = 2, 10, 9, model.width
+    B, D, C = 2, 10, model.width
     z = torch.linspace(14.0, -1.0, D)[None, :, None, None].expand(B, D, H, W).contiguous().to(DEV)
     plan = eng.plan(B, D, H, W, z, 48e3, 48e3)
     if chunk:
