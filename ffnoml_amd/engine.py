@@ -10,6 +10,7 @@ import ctypes
 import hashlib
 import math
 import threading
+import weakref
 
 import numpy as np
 import torch
@@ -47,6 +48,41 @@ def _settle(device):
 
 def _p(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+_K_BLOCKS = {}        # id of a split matrix's first plane -> (weak reference to it, {row block: column blocks})
+
+
+def _k_cache(hi):
+    c = _K_BLOCKS.get(id(hi))
+    if c is None or c[0]() is not hi:
+        c = (weakref.ref(hi), {})
+        _K_BLOCKS[id(hi)] = c
+        weakref.finalize(hi, _K_BLOCKS.pop, id(hi), None)
+    return c[1]
+
+
+def _k_blocks(mat, m0):
+    """[(k0, split matrix)] of rows m0 .. m0 + DENSE_DFT_MAX of a split matrix in column blocks of at most
+    DENSE_DFT_MAX (row blocks are views of the planes; column blocks are copies, made once per matrix)."""
+    hi, lo, m, k, mp, kp = mat[:6]
+    planes = (hi, lo) + tuple(mat[6:])
+    rows = min(DENSE_DFT_MAX, m - m0)
+    rp = T.round_up(rows, 16)
+    if kp <= DENSE_DFT_MAX:
+        sub = tuple(pl[m0:m0 + rp] for pl in planes)
+        return [(0, (sub[0], sub[1], rows, k, rp, kp) + sub[2:])]
+    cache = _k_cache(hi)
+    if m0 not in cache:
+        out = []
+        for k0 in range(0, k, DENSE_DFT_MAX):
+            kb = min(DENSE_DFT_MAX, k - k0)
+            kbp = T.round_up(kb, 32)
+            sub = tuple(torch.nn.functional.pad(pl[m0:m0 + rp, k0:k0 + kb], (0, kbp - kb)).contiguous()
+                        for pl in planes)
+            out.append((k0, (sub[0], sub[1], rows, kb, rp, kbp) + sub[2:]))
+        cache[m0] = out
+    return cache[m0]
 
 
 def horizontal_factors(H, W):
@@ -644,6 +680,15 @@ class Engine:
 
     def _tw(self, mat, x, y, batch, n, bmap_x, bmap_y, kmap, mmap, accumulate=0, twiddle=None):
         hi, lo, m, k, mp, kp = mat[:6]
+        if (twiddle is None and (mp > DENSE_DFT_MAX or kp > DENSE_DFT_MAX) and (mp <= DENSE_DFT_MAX or mmap[0] >= m)
+                and (kp <= DENSE_DFT_MAX or kmap[0] >= k)):
+            # the large dimensions have single-level maps: blocks of the LDS-staged size, the k blocks accumulated
+            xf, yf = x.view(-1), y.view(-1)
+            for m0 in range(0, m, DENSE_DFT_MAX):
+                for j, (k0, blk) in enumerate(_k_blocks(mat, m0)):
+                    self._tw(blk, xf[k0 * kmap[2]:], yf[m0 * mmap[2]:], batch, n, bmap_x, bmap_y, kmap, mmap,
+                             accumulate if j == 0 else 1)
+            return
         g = FfnoTwGemm()
         g.a_hi, g.a_lo = _p(hi), _p(lo)
         g.a_l3 = _p(mat[6] if len(mat) > 6 else None)

@@ -147,17 +147,19 @@ def test_post_mlp_and_groupnorm_statistics():
         assert ((stats[..., 2].double() - wtot).abs() / wtot).max() < 1e-5
 
 
-@pytest.mark.parametrize("H,W", [(10, 8), (96, 6), (130, 5), (12, 130), (72, 256)])
-def test_spectral_pipelines_match_fft(H, W):
+@pytest.mark.parametrize("H,W,D", [(10, 8, 12), (96, 6, 12), (130, 5, 12), (12, 130, 12), (72, 256, 12),
+                                   (6, 5, 200)])
+def test_spectral_pipelines_match_fft(H, W, D):
     """The horizontal rfft2 -> per-mode mix -> irfft2 and the vertical Filon -> mix -> synthesis, each as the
     engine runs them, against float64 torch.fft / einsum statements with the plan's own gates and low-rank tables:
     1e-4 of range (split-bf16 transforms and mix). H = 96, 130 and 72 take the factorised DFT along H (12 x 8,
-    10 x 13, 9 x 8), W = 130 and 256 the factorised real DFTs along W."""
+    10 x 13, 9 x 8), W = 130 and 256 the factorised real DFTs along W; 200 levels put the vertical matrices in
+    row and column blocks."""
     from ffnoml_amd.engine import Engine
     model = _small_model()
     eng = Engine(model)
     eng.sync(torch.device(DEV))
-    B, D, C = 2, 12, model.width
+    B, C = 2, model.width
     z = torch.linspace(14.0, -1.0, D)[None, :, None, None].expand(B, D, H, W).contiguous().to(DEV)
     plan = eng.plan(B, D, H, W, z, 48e3, 48e3)
     assert (plan.hfac is not None) == (H > 64) and (plan.wfac is not None) == (W > 126)
@@ -238,18 +240,19 @@ def test_spectral_pipelines_match_fft(H, W):
     assert err < 1e-4
 
 
-@pytest.mark.parametrize("chunk,H,W", [(None, 12, 9), (3, 12, 9), (3, 72, 9), (3, 12, 130), (None, 66, 200)])
-def test_spectral_adjoints_are_transposes(chunk, H, W):
+@pytest.mark.parametrize("chunk,H,W,D", [(None, 12, 9, 10), (3, 12, 9, 10), (3, 72, 9, 10), (3, 12, 130, 10),
+                                         (None, 66, 200, 10), (None, 5, 6, 160)])
+def test_spectral_adjoints_are_transposes(chunk, H, W, D):
     """<T x, y> = <x, T^T y> for the horizontal and vertical spectral maps as the engine runs them (forward and
     adjoint launch sequences, the per-mode gates and low-rank terms included), with the horizontal planes in one
-    chunk and in chunks of 3, and factorised along H (72 = 9 x 8, 66), along W (130, 200) and both: 1e-5
-    relative."""
+    chunk and in chunks of 3, factorised along H (72 = 9 x 8, 66), along W (130, 200) and both, and with 160
+    levels (vertical matrices in blocks): 1e-5 relative."""
     from ffnoml_amd.engine import Engine
     model = _small_model()
     eng = Engine(model)
     dev = torch.device(DEV)
     eng.sync(dev)
-    B, D, C = 2, 10, model.width
+    B, C = 2, model.width
     z = torch.linspace(14.0, -1.0, D)[None, :, None, None].expand(B, D, H, W).contiguous().to(DEV)
     plan = eng.plan(B, D, H, W, z, 48e3, 48e3)
     if chunk:

@@ -2,7 +2,9 @@
 and index maps as the engine issues them, run on the CPU through a float64 model of the GEMM entry point
 (tests/tw_emulator.py): dense transforms, and the factorised DFTs along H, along W and along both (the dense-size
 limit lowered so that small shapes take them), against numpy.fft: 1e-5 of range (float32 buffers between stages,
-bf16-split matrices summed exactly); the adjoint as the transpose: <T x, y> = <x, T^T y> to 1e-6."""
+bf16-split matrices summed exactly); the adjoint as the transpose: <T x, y> = <x, T^T y> to 1e-6. The vertical
+pipeline with more levels than one LDS-staged GEMM takes (its matrices in row and column blocks) against the dense
+float64 matrices."""
 import numpy as np
 import pytest
 import torch
@@ -10,7 +12,7 @@ import torch
 from tw_emulator import Emulator
 
 
-def _plan(H, W, dense_max, B=1, D=2, C=16):
+def _plan(H, W, dense_max, B=1, D=2, C=16, column=None):
     from ffnoml_amd import _lib, engine as E, synthetic
     from ffnoml_amd.atoms import species
     model = synthetic.random_model(0, width=C, n_layers=1, hidden_spectral=16, hidden_vertical=16, freq_mix_rank=4,
@@ -23,8 +25,8 @@ def _plan(H, W, dense_max, B=1, D=2, C=16):
     saved = E.DENSE_DFT_MAX
     E.DENSE_DFT_MAX = dense_max
     try:
-        col = np.linspace(14.0, -1.0, D)
-        plan = E.Plan(eng, B, D, H, W, col - col.min(), 15.0, 48e3, 48e3)
+        col = np.linspace(14.0, -1.0, D) if column is None else column
+        plan = E.Plan(eng, B, D, H, W, col - col.min(), float(col.max() - col.min()), 48e3, 48e3)
     finally:
         E.DENSE_DFT_MAX = saved
     return E, _lib, eng, plan
@@ -87,3 +89,41 @@ def test_horizontal_pipeline_matches_fft(dense_max, H, W, fact):
     lhs = float((tx.double() * y.double()).sum())
     rhs = float((x.double() * ty.double()).sum())
     assert abs(lhs - rhs) <= 1e-6 * float(tx.double().norm() * y.double().norm())
+
+
+@pytest.mark.parametrize("D,dense_max", [(200, 128), (40, 16)])
+def test_vertical_pipeline_in_blocks(D, dense_max):
+    """Filon analysis [2M x D] and synthesis [D x 2M] with D beyond the LDS-staged size (column blocks accumulated,
+    row blocks), forward and adjoint, against the dense float64 matrices with the per-mode factor between them."""
+    E, _lib, eng, plan = _plan(3, 4, 128, B=2, D=D)
+    B, C, M = plan.B, plan.C, plan.modes
+    hw = plan.H * plan.W
+    rng = np.random.default_rng(D)
+    factor = rng.standard_normal(M) + 1j * rng.standard_normal(M)
+    x = torch.from_numpy(rng.standard_normal((B, D, hw, C)).astype(np.float32))
+    out, out_t = torch.empty_like(x), torch.empty_like(x)
+    v1 = torch.empty(B * 2 * M * hw * C)
+    tables = {"vertical": {"gate": torch.zeros(M, 2), "v": None, "u": None}}
+    em = Emulator(_tensors(plan, [x, out, out_t, v1]), factor)
+    saved = _lib.call, E.Engine._stream, E.DENSE_DFT_MAX
+    _lib.call = em
+    E.Engine._stream = staticmethod(lambda: None)
+    E.DENSE_DFT_MAX = dense_max
+    wc = torch.zeros(8)
+    eng.weights.blocks[0]["vertical.wc"] = wc
+    eng.weights.adjoint()["blocks"][0]["vertical.wc"] = wc
+    plan.tables = lambda i, adjoint=False: tables
+    try:
+        eng.vertical(plan, x.view(-1, C), v1, out.view(-1, C), 0, tables)
+        eng.vertical(plan, x.view(-1, C), v1, out_t.view(-1, C), 0, tables, adjoint=True)
+    finally:
+        _lib.call, E.Engine._stream, E.DENSE_DFT_MAX = saved
+    vf, vi = plan._mats["v_fwd"], plan._mats["v_inv"]                    # [2M, D], [D, 2M]
+    blk = np.block([[np.diag(factor.real), -np.diag(factor.imag)], [np.diag(factor.imag), np.diag(factor.real)]])
+    T_ = vi @ blk @ vf                                                   # [D, D] per column and channel
+    xd = x.double().numpy()
+    for got, mat in ((out, T_), (out_t, T_.T)):
+        ref = np.einsum("ed,bdnc->benc", mat, xd)
+        err = np.abs(got.double().numpy() - ref).max() / np.abs(ref).max()
+        print(f"D {D}: rel err {err:.2e}")
+        assert err < 1e-5

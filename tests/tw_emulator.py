@@ -13,20 +13,27 @@ import torch
 class Resolver:
     """Maps a raw host pointer to (numpy view of the owning root tensor, element offset)."""
 
-    def __init__(self, tensors):
+    def __init__(self, tensors, more=None):
         self.roots = []
+        self.more = more            # called for further tensors when a pointer is in none registered so far
+        self.add(tensors)
+
+    def add(self, tensors):
         for t in tensors:
             root = t._base if t._base is not None else t
             if not any(r is root for r in self.roots):
                 self.roots.append(root)
 
-    def __call__(self, ptr, dtype):
+    def __call__(self, ptr, dtype, retry=True):
         ptr = ptr.value if isinstance(ptr, ctypes.c_void_p) else ptr
         for r in self.roots:
             base, nbytes = r.data_ptr(), r.numel() * r.element_size()
             if base <= ptr < base + nbytes:
                 arr = (r.view(torch.int16) if dtype == "bf16" else r).numpy().reshape(-1)
                 return arr, (ptr - base) // arr.itemsize
+        if retry and self.more is not None:
+            self.add(self.more())
+            return self(ptr, dtype, retry=False)
         raise AssertionError(f"pointer {ptr:#x} is in no registered tensor")
 
 
@@ -80,7 +87,12 @@ class Emulator:
     ([modes] complex, mode = ky * Wf + kx)."""
 
     def __init__(self, tensors, factor):
-        self.res = Resolver(tensors)
+        from ffnoml_amd import engine as E
+
+        def blocks():              # column blocks the engine cut from large matrices
+            return [t for _, cache in E._K_BLOCKS.values() for blks in cache.values() for _, sub in blks
+                    for t in sub if isinstance(t, torch.Tensor)]
+        self.res = Resolver(tensors, blocks)
         self.factor = factor
         self.calls = []
 
