@@ -33,6 +33,7 @@ DENSE_DFT_MAX = 128          # horizontal DFT matrices up to this many rows and 
 #                              larger ones run factorised (two stages) when the length splits into factors <= 64
 WORKING_UNITS = 9            # activation-sized buffers of the adjoint besides the saved block inputs
 GATE_UNITS = 2               # the stored branch gates of the two-pass fusion adjoint
+FUSE_CHUNK_BYTES = 2 << 30   # gates of one chunk of voxels when a whole block's do not fit (one sample)
 
 # MULTI3D constants in cgs (models/ffno_model.py:111-113)
 _EE, _HH, _EM, _BK, _PI = 1.602189e-12, 6.626176e-27, 9.109534e-28, 1.380662e-16, 3.14159265359
@@ -107,8 +108,9 @@ def pack_mlp_weight(w, segments=None, n_pad=None, parts=2):
     row followed by 8 zeros (the kernels' row stride K_pad + 8 keeps the 16 rows of an LDS fragment read on distinct
     banks). Planes: hi = bf16(w),
     lo = bf16(w - hi), and with parts=3 a third, bf16(w - hi - lo), for the forward's three-way products (the first
-    two planes are then still the two-way split the adjoint kernels read)."""
-    w = w.double()
+    two planes are then still the two-way split the adjoint kernels read). A float32 weight is split in float32:
+    each remainder (w - hi, w - hi - mid) is exact in float32, so the planes equal those of a float64 split."""
+    w = w.double() if w.dtype == torch.float64 else w.float()
     lead = w.shape[:-2]
     N, K = w.shape[-2:]
     segs = segments or [K]
@@ -129,7 +131,7 @@ def pack_mlp_weight(w, segments=None, n_pad=None, parts=2):
     for _ in range(parts):
         plane = rest.to(torch.bfloat16)
         planes.append(plane)
-        rest = rest - plane.double()
+        rest = rest - plane.to(rest.dtype)
     return torch.stack(planes, -3).contiguous().view(torch.int16)
 
 
@@ -503,7 +505,8 @@ class Plan:
     def vjp_bytes(self, stride, two_pass=False):
         saved = -(-self.n_layers // stride) + (1 if stride == 1 else 0) + (stride - 1)
         units = saved + WORKING_UNITS + (GATE_UNITS if two_pass else 0)
-        return self.unit_bytes() * units + self.scratch_bytes() + 2 * self.uncached_table_bytes()
+        chunk = 0 if two_pass else min(FUSE_CHUNK_BYTES, GATE_UNITS * self.unit_bytes())
+        return self.unit_bytes() * units + chunk + self.scratch_bytes() + 2 * self.uncached_table_bytes()
 
 
 class Saved:
@@ -542,6 +545,8 @@ class Engine:
         self.checkpoint_stride = None       # None: the smallest stride whose adjoint fits the free HBM
         self.two_pass_fuse = None           # None: whenever its gate buffer fits at stride 1
         self.full_save = None               # None: keep every intermediate when that fits; False: never
+        self.fuse_chunk = None              # voxels per chunk of the one-sample fusion adjoint without a block's gate
+        #                                     buffer (None: FUSE_CHUNK_BYTES of gates; False: the one-pass kernel)
         self.last_stride = None
         self._modes = collections.OrderedDict()     # per-mode table sets shared by plans, least recently used first
 
@@ -1182,6 +1187,8 @@ class Engine:
             s, v, sp, vp, fp, pp, mp, x1 = (torch.empty(N, C, **f32) for _ in range(WORKING_UNITS - 1))
             tbuf = torch.empty(N, C, **f32) if stride > 1 else None
             gates = torch.empty(N, 2 * C, **f32) if saved.two_pass else None
+        chunk = self._fuse_chunk(plan) if (gates is None and not saved.full and B == 1) else 0
+        gates_c = torch.empty(min(chunk, plan.voxels), 2 * C, **f32) if chunk else None
         scr = self._scratch(plan, dev, adjoint=True)
         part2 = scr["part2"]
         ones = torch.ones(B, C, **f32)
@@ -1273,6 +1280,9 @@ class Engine:
                           _p(gates), _p(blk["f1"]), _p(adj["f2t"]), _p(adj["f1t"]), _p(mp), _p(pp), st)
                 _lib.call("ffno_fuse_gates_bwd", rows, _p(g), _p(h), _p(sp), _p(vp), _p(sa), _p(sb_), _p(va), _p(vb),
                           _p(gates), _p(blk["bg1"]), _p(adj["bg2t"]), _p(adj["bg1t"]), _p(mp), _p(pp), st)
+            elif gates_c is not None:
+                self._fuse_bwd_chunked(plan, chunk, gates_c, blk, adj, gb_f, g, h, sp, vp, fp, mp, pp, x1,
+                                       (sa, sb_, va, vb), wbar, scr, st)
             else:
                 _lib.call("ffno_fuse_bwd", rows, _p(g), gb_f, _p(h), _p(sp), _p(vp), _p(sa), _p(sb_), _p(va), _p(vb),
                           _p(blk["bg1"]), _p(blk["bg2"]), _p(blk["bg2_bias"]), _p(blk["f1"]), _p(adj["f2t"]),
@@ -1322,6 +1332,40 @@ class Engine:
         if not want_z:
             return xbar, None
         return xbar, self._height_gradient(plan, z, zbar, zbar_s, wbar, dfac_col)
+
+    def _fuse_chunk(self, plan):
+        if self.fuse_chunk is False:
+            return 0
+        if self.fuse_chunk:
+            return T.round_up(int(self.fuse_chunk), 16)
+        return max(16, FUSE_CHUNK_BYTES // (8 * plan.C) // 16 * 16)
+
+    def _fuse_bwd_chunked(self, plan, chunk, gates_c, blk, adj, gb_f, g, h, sp, vp, fp, mp, pp, x1, aff, wbar, scr,
+                          st):
+        """The two-pass branch-fusion adjoint of one sample, chunk by chunk over its voxels, for when a whole block's
+        gate buffer does not fit: each chunk's gates are recomputed by the fusion forward (its output goes to x1 and
+        its statistics to the forward's partials buffer, neither read here; every voxel's result is its own, so the
+        gates equal the whole block's), then both adjoint passes run on the chunk. Row tensors are addressed by
+        offset; the GroupNorm's per-sample sums and affines are shared by every chunk."""
+        C = plan.C
+        sa, sb, va, vb = aff
+        row = lambda t, v0: t.view(-1)[v0 * C:]          # noqa: E731
+        for v0 in range(0, plan.voxels, chunk):
+            cv = min(chunk, plan.voxels - v0)
+            rc = FfnoRows(1, cv, C, max(1, min(plan.nwg, -(-((cv + 15) // 16) // ROW_TILES_PER_WG))))
+            dw = plan.depth_w.view(-1)[v0:]
+            _lib.call("ffno_fuse", rc, _p(row(h, v0)), _p(row(sp, v0)), _p(row(vp, v0)), _p(sa), _p(sb), _p(va),
+                      _p(vb), _p(blk["bg1"]), _p(blk["bg2"]), _p(blk["bg2_bias"]), _p(blk["f1"]), _p(blk["f2"]),
+                      _p(row(x1, v0)), FfnoStatsOut(_p(dw), _p(scr["part"])), _p(gates_c), st)
+            gc = _lib.FfnoGnBack.from_buffer_copy(gb_f)
+            gc.t, gc.depth_w = _p(row(fp, v0)), _p(dw)
+            gc.wbar = _p(wbar.view(-1)[v0:]) if wbar is not None else None
+            _lib.call("ffno_fuse_mlp_bwd", rc, _p(row(g, v0)), gc, _p(row(sp, v0)), _p(row(vp, v0)), _p(sa), _p(sb),
+                      _p(va), _p(vb), _p(gates_c), _p(blk["f1"]), _p(adj["f2t"]), _p(adj["f1t"]), _p(row(mp, v0)),
+                      _p(row(pp, v0)), st)
+            _lib.call("ffno_fuse_gates_bwd", rc, _p(row(g, v0)), _p(row(h, v0)), _p(row(sp, v0)), _p(row(vp, v0)),
+                      _p(sa), _p(sb), _p(va), _p(vb), _p(gates_c), _p(blk["bg1"]), _p(adj["bg2t"]), _p(adj["bg1t"]),
+                      _p(row(mp, v0)), _p(row(pp, v0)), st)
 
     @staticmethod
     def _height_gradient(plan, z, zbar, zbar_s, wbar, dfac_col):

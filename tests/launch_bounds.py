@@ -446,11 +446,12 @@ def dry_engine(model):
     return eng, chk, restore
 
 
-def check_shape(model, B, D, H, W, stride=1, want_z=True, column=None, two_pass=True, percol=False):
+def check_shape(model, B, D, H, W, stride=1, want_z=True, column=None, two_pass=True, percol=False, fuse_chunk=None):
     """Run forward, the saving forward, the adjoint and the tangent (JVP) pass of `model` at [B, D, H, W] through the
     checker (with the two-pass fusion adjoint where the stride allows it); returns the number of launches checked."""
     eng, chk, restore = dry_engine(model)
     eng.two_pass_fuse = two_pass
+    eng.fuse_chunk = fuse_chunk
     try:
         col = np.linspace(14.0, -1.0, D) if column is None else np.asarray(column, np.float64)
         zs = col - col.min()

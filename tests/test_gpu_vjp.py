@@ -228,6 +228,37 @@ def test_two_pass_fusion_adjoint_matches_fused():
         assert err <= tol
 
 
+def test_chunked_fusion_adjoint_matches_two_pass():
+    """One sample at checkpoint stride 2, where a whole block's gate buffer is not kept: the two-pass fusion adjoint
+    run chunk by chunk over the voxels (208-voxel chunks and a ragged last one, each chunk's gates recomputed by the
+    fusion forward) against the two-pass adjoint over the stored gates (stride 1) -- the same per-voxel arithmetic,
+    so x_bar agrees to 1e-6 relative -- and against the one-pass kernel at stride 2 (x_bar 1e-6, z_bar 1e-4, as the
+    two-pass against the one-pass)."""
+    data, config, state = load("case_b_jitter64")
+    config = dict(config, n_layers=3, width=64, hidden_spectral=64, hidden_vertical=64)
+    from ffnoml_amd import synthetic
+    model = synthetic.random_model(5, **config).to(DEV).eval()
+    x, z, mx, sx = synthetic.atmosphere(1, 16, 12, 10, seed=8)
+    x, z = x.to(DEV), z.to(DEV)
+    ct = synthetic.cotangent((1, config["out_channels"], 16, 12, 10), 4).to(DEV)
+    eng = model.engine()
+    out = {}
+    for name, stride, two_pass, chunk in (("two-pass", 1, True, None), ("chunked", 2, False, 208),
+                                          ("one-pass", 2, False, False)):
+        print(f"[fuse] {name}", flush=True)
+        eng.two_pass_fuse, eng.fuse_chunk = two_pass, chunk
+        y, saved = eng.forward_saved(x, z, 48e3, 48e3, stride=stride)
+        assert saved.two_pass == two_pass
+        out[name] = eng.vjp(x, z, 48e3, 48e3, ct, saved, want_z=True)
+        torch.cuda.synchronize()
+    eng.two_pass_fuse, eng.fuse_chunk = None, None
+    for other, tols in (("two-pass", (1e-6, 1e-4)), ("one-pass", (1e-6, 1e-4))):
+        for a, b, name, tol in zip(out["chunked"], out[other], ("x_bar", "z_bar"), tols):
+            err = float((a - b).abs().max() / b.abs().max())
+            print(f"chunked vs {other} {name} rel {err:.2e}", flush=True)
+            assert err <= tol
+
+
 def test_adjoint_keeping_every_intermediate_matches_recompute():
     """The adjoint that reads every intermediate the forward kept (no recompute) against the one that recomputes
     each block from its saved input, both with the two-pass fusion adjoint: bit-identical x_bar and z_bar (the

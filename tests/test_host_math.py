@@ -127,6 +127,15 @@ def test_packing_matches_kernel_contraction(N, segments):
     assert not np.any(_kernel_contraction(wp, act)[N:])
 
 
+def test_float32_packing_equals_the_float64_split():
+    """A float32 weight is split in float32 (its remainders are exact there): the planes are bit-identical to those
+    of its float64 copy, over a wide dynamic range, for two and three planes."""
+    torch.manual_seed(0)
+    for shape, parts in (((64, 96), 3), ((5, 16, 64), 3), ((3, 64, 32), 2)):
+        w = torch.randn(*shape) * torch.logspace(-6, 3, shape[-1])
+        assert torch.equal(pack_mlp_weight(w, parts=parts), pack_mlp_weight(w.double(), parts=parts))
+
+
 def test_three_plane_packing_is_exact_and_extends_the_two_plane_split():
     """parts=3: the three bf16 planes add up to the float32 weight exactly, and the first two planes are the two-way
     split (what the adjoint kernels read from the same buffer)."""

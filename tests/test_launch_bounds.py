@@ -32,6 +32,13 @@ def test_engine_launches_stay_in_bounds(B, D, H, W, stride):
         assert check_shape(_model(layers=2), B, D, H, W, stride=0) > 20          # every intermediate kept
 
 
+@pytest.mark.parametrize("fuse_chunk", [None, 96, 1000, False])
+def test_one_sample_chunked_fusion_adjoint_stays_in_bounds(fuse_chunk):
+    """The one-sample fusion adjoint without a whole block's gate buffer (stride 2 here): chunks of 96 and 1000
+    voxels (a ragged last chunk), the default chunk, and the one-pass kernel."""
+    assert check_shape(_model(layers=3), 1, 13, 11, 9, stride=2, fuse_chunk=fuse_chunk) > 50
+
+
 @pytest.mark.parametrize("B,D,H,W,width", [(2, 20, 12, 14, 64), (1, 33, 9, 7, 96), (3, 16, 8, 8, 32)])
 def test_per_column_grids_stay_in_bounds(B, D, H, W, width):
     assert check_shape(_model(width=width, layers=2), B, D, H, W, stride=1, percol=True) > 50
@@ -77,6 +84,7 @@ def test_checker_catches_a_short_buffer_and_covers_every_entry_point():
     LB.Checker.__call__ = record
     try:
         check_shape(_model(layers=2), 1, 12, 8, 8, stride=2)
+        check_shape(_model(layers=2), 1, 12, 8, 8, stride=2, fuse_chunk=False)
         check_shape(_model(layers=2), 1, 12, 8, 8, stride=1, two_pass=True)
         check_shape(_model(layers=2), 1, 12, 8, 8, stride=1, percol=True)
     finally:
