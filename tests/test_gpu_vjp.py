@@ -285,8 +285,8 @@ def test_adjoint_keeping_every_intermediate_matches_recompute():
 
 
 def test_factorised_transforms_match_oracle():
-    """The whole network with its horizontal DFTs factorised along H and W and its Filon matrices in blocks (the
-    dense-size limit lowered to 32 so that a small tile takes those paths: H = 36, W = 40, D = 40): forward, x_bar
+    """The whole network with its horizontal DFTs factorised along H and W (the dense-size limit lowered to 32 so
+    that a small tile takes those paths: H = 36 = 6 x 6, W = 40 = 5 x 8, D = 40): forward, x_bar
     and z_bar of the HIP engine against the float64 oracle on the same weights (1e-3 relative in log10 n, 1e-2 in
     the gradients; z_bar against the oracle's per-column path)."""
     from ffnoml_amd import engine as E, synthetic
