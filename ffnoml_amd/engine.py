@@ -66,7 +66,7 @@ def _k_cache(hi):
 def _k_blocks(mat, m0):
     """[(k0, split matrix)] of rows m0 .. m0 + DENSE_DFT_MAX of a split matrix in column blocks of at most
     DENSE_DFT_MAX (row blocks are views of the planes; column blocks are copies, made once per matrix)."""
-    hi, lo, m, k, mp, kp = mat[:6]
+    hi, lo, m, k, _, kp = mat[:6]
     planes = (hi, lo) + tuple(mat[6:])
     rows = min(DENSE_DFT_MAX, m - m0)
     rp = T.round_up(rows, 16)
@@ -1172,8 +1172,6 @@ class Engine:
         plan.adjoint_tables()
         B, D, H, W, C, M = plan.B, plan.D, plan.H, plan.W, plan.C, plan.modes
         N = B * plan.voxels
-        hwc = H * W * C
-        big = 1 << 40
         L = len(Wt.blocks)
         stride = saved.stride
         ybar = ybar.contiguous().float()
