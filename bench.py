@@ -98,6 +98,7 @@ def main():
     ap.add_argument("--service-requests", type=int, default=160,
                     help="requests of the serving workload (Poisson arrivals at 40/s, seed 1; 0 skips it; the full "
                          "workload is 2000)")
+    ap.add_argument("--service-slots", type=int, default=3, help="execution slots of the serving leg's batcher")
     ap.add_argument("--plain-service-requests", type=int, default=16,
                     help="the first requests of the same trace served one at a time by the plain FP32 GPU path")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -229,7 +230,7 @@ def main():
     if args.service_requests > 0:
         del y, xb
         out["service"] = service_leg(model, dev, args.service_requests,
-                                     0 if args.no_baselines else args.plain_service_requests)
+                                     0 if args.no_baselines else args.plain_service_requests, args.service_slots)
     if args.volume:
         out["whole_volume"] = volume_leg(model, dev)
     print(json.dumps(out), flush=True)
@@ -360,7 +361,7 @@ def service_inputs(trace, variants=4):
     return out
 
 
-def service_leg(model, dev, n, n_plain):
+def service_leg(model, dev, n, n_plain, slots=3):
     """The serving workload through ffnoml_amd.serving.ContinuousBatcher on the benchmark's model (width 64,
     8 layers): requests submitted at their arrival times from a client thread, batched per shape bucket, admitted by
     HBM footprint, run on concurrent worker streams; latency = completion - arrival. Beside it, the first n_plain
@@ -380,7 +381,7 @@ def service_leg(model, dev, n, n_plain):
     inputs = service_inputs(trace)
     log(f"{n} requests generated in {time.perf_counter() - t0:.1f} s")
     backend = GPUBackend(None, model=model, level_names=[f"L{i}" for i in range(12)], device=str(dev))
-    batcher = ContinuousBatcher({"m": backend}, slots=3, max_batch=8)
+    batcher = ContinuousBatcher({"m": backend}, slots=slots, max_batch=8)
     futures = [None] * n
     try:
         def client():
