@@ -79,12 +79,14 @@ def test_results_alone_and_in_the_mix():
         batcher.close()
 
 
-def test_graph_replays_match_eager():
-    """Replays of one captured forward, each after unrelated eager GPU work, against the eager forward."""
+@pytest.mark.parametrize("nz,nx,ny", [(20, 16, 16), (12, 130, 132)])
+def test_graph_replays_match_eager(nz, nx, ny):
+    """Replays of one captured forward, each after unrelated eager GPU work, against the eager forward; the second
+    tile's horizontal DFTs run factorised along both directions (130 = 10 x 13, 132 = 11 x 12)."""
     from ffnoml_amd.serving import ContinuousBatcher
     backend = _backend(seed=9)
     rng = np.random.default_rng(5)
-    f, z = _request(rng, 20, 16, 16)
+    f, z = _request(rng, nz, nx, ny)
     eager = backend.predict_local(f, z, 48e3, 48e3)
     batcher = ContinuousBatcher({"H": backend}, slots=1, max_batch=1)
     try:
