@@ -111,3 +111,26 @@ def ffno_apply(module, x, z_scale, dx, dy):
         return FFNOFunction.apply(x, z_scale, module, _spacing(dx), _spacing(dy))[0]
     finally:
         _STATE.needs_grad = False
+
+
+class HeadOutputFunction(torch.autograd.Function):
+    """FFNO3D.head_output through the HIP library: forward ffno_head_output, backward ffno_head_output_bwd."""
+
+    @staticmethod
+    def forward(ctx, head, x_in, module):
+        head, x_in = head.detach().contiguous().float(), x_in.detach().contiguous().float()
+        ctx.save_for_backward(head, x_in)
+        ctx.module = module
+        return module.engine().head_output(head, x_in)
+
+    @staticmethod
+    def backward(ctx, ybar):
+        head, x_in = ctx.saved_tensors
+        head_bar, x_bar = ctx.module.engine().head_output_bwd(head, x_in, ybar.contiguous().float())
+        return (head_bar if ctx.needs_input_grad[0] else None), (x_bar if ctx.needs_input_grad[1] else None), None
+
+
+def head_output_apply(module, head, x_in):
+    if not (head.is_cuda and x_in.is_cuda):
+        raise RuntimeError("ffnoml_amd.FFNO3D.head_output runs on the GPU through libffno_hip.so; it has no CPU path")
+    return HeadOutputFunction.apply(head, x_in, module)

@@ -1,6 +1,7 @@
 // FFNO3D adjoint kernels for gfx950: the input VJP of every forward kernel, with the forward recomputed from saved
 // block inputs. Same lane layout and split-bf16 MFMA arithmetic as the forward (ffno_device.h).
 #include "ffno_device.h"
+#include "ffno_head.h"
 
 namespace ffno {
 namespace {
@@ -793,7 +794,6 @@ __global__ __launch_bounds__(NW * 64) void k_gates_bwd(FfnoRows rows, const floa
 // ---------------------------------------------------------------------------------------------------------------
 // head adjoint
 // ---------------------------------------------------------------------------------------------------------------
-constexpr float LN10 = 2.302585092994046f;
 
 // LW: weights staged in LDS (else read from global: widths whose four matrices exceed a CU's LDS)
 template <int CT, int NW, int NTH, bool LW>
@@ -815,8 +815,11 @@ __global__ __launch_bounds__(NW * 64) void k_head_bwd(FfnoRows rows, const float
   const int b = blockIdx.y;
   const int64_t nt = (rows.voxels + 15) / 16;
   volatile float* mine = hs + L.wave * 16 * NTH * 16;
-  int n_out = 0;
-  for (int s = 0; s < hd.n_species; ++s) n_out += hd.levels[s];
+  int n_out = 0, n_head = 0;
+  for (int s = 0; s < hd.n_species; ++s) {
+    n_out += hd.levels[s];
+    n_head += hd.levels[s] + 1;
+  }
   for (int64_t tile = (int64_t)blockIdx.x * NW + L.wave; tile < nt; tile += (int64_t)gridDim.x * NW) {
     const int64_t vox = tile * 16 + L.v;
     const bool ok = vox < rows.voxels;
@@ -839,43 +842,11 @@ __global__ __launch_bounds__(NW * 64) void k_head_bwd(FfnoRows rows, const float
       float gt_ = 0.f, gne = 0.f, grho = 0.f;
       if (ok) {
         auto feat = [&](int c) { return x_in[((int64_t)b * 6 + c) * rows.voxels + vox] * hd.std_x[c] + hd.mean_x[c]; };
-        const float lt = feat(hd.t_channel) * LN10;
-        const float lne = (feat(hd.ne_channel) - 6.0f) * LN10;
-        const float inv_t = __expf(-lt);
-        const float conl = hd.ln_ccon + lne - 1.5f * lt;
-        const float dxi = hd.dxi_coef * __expf(0.5f * (lne + hd.ln_k - lt));
-        int c = 0, k = 0;
-        for (int s = 0; s < hd.n_species; ++s) {
-          const int nl = hd.levels[s];
-          float logit[15], a[15];
-          float mx = -3.0e38f, asum = 0.f;
-          for (int i = 0; i < nl; ++i) {
-            const int li = k + i;
-            logit[i] = hv[c + i] + hd.log_g[li] - hd.e_over_k * inv_t * (hd.ev[li] - dxi * hd.ndxi[li]) -
-                       hd.dion[li] * conl;
-            mx = fmaxf(mx, logit[i]);
-            a[i] = ybar[((int64_t)b * n_out + li) * rows.voxels + vox] / hd.std_y[li];
-            asum += a[i];
-          }
-          float se = 0.f;
-          for (int i = 0; i < nl; ++i) se += __expf(logit[i] - mx);
-          const float lse = mx + __logf(se);
-          for (int i = 0; i < nl; ++i) {
-            const int li = k + i;
-            const float p = __expf(logit[i] - lse);
-            const float lb = (a[i] - p * asum) / LN10;          // gradient of the logit = head + ln f*
-            hv[c + i] = lb;
-            // d ln f*_i / d ln T and d ln ne (MULTI3D weights, models/ffno_model.py:131-141)
-            gt_ += lb * (hd.e_over_k * inv_t * (hd.ev[li] - 1.5f * dxi * hd.ndxi[li]) + 1.5f * hd.dion[li]);
-            gne += lb * (0.5f * hd.e_over_k * inv_t * dxi * hd.ndxi[li] - hd.dion[li]);
-          }
-          const float th = tanhf(hv[c + nl] / hd.total_bound);
-          hv[c + nl] = asum * (1.f - th * th);
-          grho += asum;
-          c += nl + 1;
-          k += nl;
-        }
-        for (int i = c; i < NTH * 16; ++i) hv[i] = 0.f;
+        level_fractions_bwd(
+            hd, [&](int c) { return hv[c]; }, feat(hd.t_channel), feat(hd.ne_channel),
+            [&](int o) { return ybar[((int64_t)b * n_out + o) * rows.voxels + vox]; },
+            [&](int c, float val) { hv[c] = val; }, gt_, gne, grho);
+        for (int i = n_head; i < NTH * 16; ++i) hv[i] = 0.f;
       } else {
         for (int i = 0; i < NTH * 16; ++i) hv[i] = 0.f;
       }

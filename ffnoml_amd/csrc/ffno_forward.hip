@@ -2,6 +2,7 @@
 // statistics, the split-bf16 twiddle GEMM for the DFTs and the vertical Filon transform, and the per-mode
 // spectral channel mix. Conventions are in ffno_device.h.
 #include "ffno_device.h"
+#include "ffno_head.h"
 
 namespace ffno {
 
@@ -360,8 +361,6 @@ __global__ __launch_bounds__(NW * 64) void k_resmlp(FfnoRows rows, FfnoAssemble 
 // ---------------------------------------------------------------------------------------------------------------
 // head: projection MLP + level-fraction output, channel-first store
 // ---------------------------------------------------------------------------------------------------------------
-constexpr float LN10 = 2.302585092994046f;
-
 template <int CT, int NW, int NTH, int NL>
 __global__ __launch_bounds__(NW * 64) void k_head(FfnoRows rows, FfnoAssemble in, float* x_out,
                                                   const uint16_t* p1,
@@ -404,37 +403,8 @@ __global__ __launch_bounds__(NW * 64) void k_head(FfnoRows rows, FfnoAssemble in
       // voxel reads what the lanes q = 1..3 of that voxel stored just above (volatile keeps the order)
       const volatile float* hv = mine + L.v * (NTH * 16);
       auto feat = [&](int c) { return x_in[((int64_t)b * 6 + c) * rows.voxels + vox] * hd.std_x[c] + hd.mean_x[c]; };
-      const float lt = feat(hd.t_channel) * LN10;                 // ln T
-      const float lne = (feat(hd.ne_channel) - 6.0f) * LN10;      // ln ne [cm^-3]
-      const float l10rho = feat(hd.rho_channel);
-      const float inv_t = __expf(-lt);
-      // MULTI3D calc_lte_populations: conl = ln(ccon ne) - 1.5 ln T, dxi = 4.98e-4 * 5040 / T * sqrt(ne k T) [eV]
-      const float conl = hd.ln_ccon + lne - 1.5f * lt;
-      const float dxi = hd.dxi_coef * __expf(0.5f * (lne + hd.ln_k - lt));
-      int c = 0, k = 0;
-      for (int s = 0; s < hd.n_species; ++s) {
-        const int nl = hd.levels[s];
-        float logit[15];
-        float mx = -3.0e38f;
-        for (int i = 0; i < nl; ++i) {
-          const int li = k + i;
-          const float lte = hd.log_g[li] - hd.e_over_k * inv_t * (hd.ev[li] - dxi * hd.ndxi[li]) - hd.dion[li] * conl;
-          logit[i] = hv[c + i] + lte;
-          mx = fmaxf(mx, logit[i]);
-        }
-        float se = 0.f;
-        for (int i = 0; i < nl; ++i) se += __expf(logit[i] - mx);
-        const float lse = mx + __logf(se);
-        const float bound = hd.total_bound;
-        const float stot = bound * tanhf(hv[c + nl] / bound);
-        for (int i = 0; i < nl; ++i) {
-          const int o = k + i;
-          const float l10n = l10rho - hd.log10_mass[s] + stot + (logit[i] - lse) / LN10;
-          y[((int64_t)b * n_out + o) * rows.voxels + vox] = (l10n - hd.mean_y[o]) / hd.std_y[o];
-        }
-        c += nl + 1;
-        k += nl;
-      }
+      level_fractions(hd, [&](int c) { return hv[c]; }, feat(hd.t_channel), feat(hd.ne_channel), feat(hd.rho_channel),
+                      [&](int o, float val) { y[((int64_t)b * n_out + o) * rows.voxels + vox] = val; });
     }
   }
 }

@@ -4,6 +4,7 @@
 // tangent. Products are two-way split-bf16 (the tangent's tolerance is that of the gradients, 1e-2 relative).
 // Lane layout and weight packing as in ffno_device.h.
 #include "ffno_device.h"
+#include "ffno_head.h"
 
 namespace ffno {
 namespace {
@@ -350,8 +351,6 @@ __global__ __launch_bounds__(NW * 64) void k_fuse_t(FfnoRows rows, const float* 
 // species d log10 n_i = d log10 rho + (1 - tanh^2(u/6)) u_dot + (l_dot_i - sum_j p_j l_dot_j) / ln 10 with
 // l_i = head_i + ln f*_i(T, ne) the logits and p their softmax; normalised by std_y
 // ---------------------------------------------------------------------------------------------------------------
-constexpr float LN10 = 2.302585092994046f;
-
 template <int CT, int NW, int NTH, bool LW>
 __global__ __launch_bounds__(NW * 64) void k_head_t(FfnoRows rows, FfnoAssemble in, const float* __restrict__ xd,
                                                     const uint16_t* p1, const uint16_t* p2,
@@ -403,13 +402,9 @@ __global__ __launch_bounds__(NW * 64) void k_head_t(FfnoRows rows, FfnoAssemble 
       const volatile float* hdv = hv + NTH * 16;
       auto feat = [&](int c) { return x_in[((int64_t)b * 6 + c) * rows.voxels + vox] * hd.std_x[c] + hd.mean_x[c]; };
       auto dfeat = [&](int c) { return xd_in[((int64_t)b * 6 + c) * rows.voxels + vox] * hd.std_x[c]; };
-      const float lt = feat(hd.t_channel) * LN10;
-      const float lne = (feat(hd.ne_channel) - 6.0f) * LN10;
       const float dlt = dfeat(hd.t_channel) * LN10, dlne = dfeat(hd.ne_channel) * LN10;
       const float dl10rho = dfeat(hd.rho_channel);
-      const float inv_t = __expf(-lt);
-      const float conl = hd.ln_ccon + lne - 1.5f * lt;
-      const float dxi = hd.dxi_coef * __expf(0.5f * (lne + hd.ln_k - lt));
+      const Lte e = lte_of(hd, feat(hd.t_channel), feat(hd.ne_channel));
       int c = 0, k = 0;
       for (int s = 0; s < hd.n_species; ++s) {
         const int nl = hd.levels[s];
@@ -417,11 +412,8 @@ __global__ __launch_bounds__(NW * 64) void k_head_t(FfnoRows rows, FfnoAssemble 
         float mx = -3.0e38f;
         for (int i = 0; i < nl; ++i) {
           const int li = k + i;
-          logit[i] = hv[c + i] + hd.log_g[li] - hd.e_over_k * inv_t * (hd.ev[li] - dxi * hd.ndxi[li]) -
-                     hd.dion[li] * conl;
-          // d ln f*_i / d ln T and d ln ne (MULTI3D weights, models/ffno_model.py:131-141)
-          dl[i] = hdv[c + i] + dlt * (hd.e_over_k * inv_t * (hd.ev[li] - 1.5f * dxi * hd.ndxi[li]) + 1.5f * hd.dion[li]) +
-                  dlne * (0.5f * hd.e_over_k * inv_t * dxi * hd.ndxi[li] - hd.dion[li]);
+          logit[i] = hv[c + i] + lte_logw(hd, e, li);
+          dl[i] = hdv[c + i] + dlt * lte_dlt(hd, e, li) + dlne * lte_dlne(hd, e, li);
           mx = fmaxf(mx, logit[i]);
         }
         float se = 0.f;

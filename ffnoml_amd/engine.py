@@ -110,7 +110,7 @@ def pack_mlp_weight(w, segments=None, n_pad=None, parts=2):
     lo = bf16(w - hi), and with parts=3 a third, bf16(w - hi - lo), for the forward's three-way products (the first
     two planes are then still the two-way split the adjoint kernels read). A float32 weight is split in float32:
     each remainder (w - hi, w - hi - mid) is exact in float32, so the planes equal those of a float64 split."""
-    w = w.double() if w.dtype == torch.float64 else w.float()
+    w = w.double() if w.dtype == torch.float64 and w.device.type == "cpu" else w.float()
     lead = w.shape[:-2]
     N, K = w.shape[-2:]
     segs = segments or [K]
@@ -162,7 +162,9 @@ class PackedWeights:
 
     def __init__(self, module, device):
         C = module.width
-        sd = {k: v.detach().to(device) for k, v in module.state_dict().items()}
+        # float32 on the device whatever the module's dtype: no float64 arithmetic runs on the GPU
+        sd = {k: (v.detach().float() if v.is_floating_point() else v.detach()).to(device)
+              for k, v in module.state_dict().items()}
         self.sd = sd
         self.C = C
         self._adj = None
@@ -326,8 +328,8 @@ class Plan:
         self.n_layers = m.n_layers
         self.wf = W // 2 + 1
         self.voxels = D * H * W
-        # columns: [B*H*W, D] float64 shifted heights of every column on the device (the reference's per-column
-        # path); else the one shared column
+        # columns: [B*H*W, D] float64 shifted heights of every column, on the host (the reference's per-column
+        # path); else the one shared column. Every matrix below is built on the host in float64 and uploaded split.
         self.percol = columns is not None
         self.length, k, factor = T.vertical_box(span, m.z_reference_length, m.z_min_wavelength,
                                                 int(math.floor(m.z_reference_length / m.z_min_wavelength)) + 1)
@@ -344,9 +346,10 @@ class Plan:
                 half = 0.5 * (zc[:, 1:] - zc[:, :-1]).abs()
                 tw = F.pad(half, (0, 1)) + F.pad(half, (1, 0))
             w = tw / tw.mean(dim=1, keepdim=True).clamp_min(1e-12)                 # [B*HW, D]
-            self.depth_w = w.reshape(B, H * W, D).transpose(1, 2).reshape(B, -1).contiguous()
-            v_fwd, v_inv = T.vertical_matrices_columns(columns, k, factor, self.length)
-            self.colmats = dict(v_fwd=T.split_bf16_columns(v_fwd, parts=3), v_inv=T.split_bf16_columns(v_inv, parts=3))
+            self.depth_w = w.reshape(B, H * W, D).transpose(1, 2).reshape(B, -1).contiguous().to(dev)
+            v_fwd, v_inv = T.vertical_matrices_t(columns, k, factor, self.length)
+            self.colmats = dict(v_fwd=T.split_bf16_columns(v_fwd, dev, parts=3),
+                                v_inv=T.split_bf16_columns(v_inv, dev, parts=3))
             del v_fwd, v_inv
             self._columns = columns
             self._col_adj = None
@@ -428,17 +431,17 @@ class Plan:
         median interval and sign per column and the mask of each column's lowest level(s)."""
         with self._lock:
             if self._height is None:
-                cols = self._columns
-                bands = T.filon_height_bands_columns(cols, self._k)
-                v_dz, v0, dfac, i_med, sgn = T.vertical_height_terms_columns(cols, self._k, self._factor,
-                                                                             self.length)
+                cols, dev = self._columns, self.device
+                bands = T.filon_height_bands_t(cols, self._k)
+                v_dz, v0, dfac, i_med, sgn = T.vertical_height_terms_t(cols, self._k, self._factor, self.length)
                 lowest = (cols == cols.min(dim=1, keepdim=True).values).float()        # [n, D]
                 any_taper = bool((dfac != 0).any())
                 self._height = dict(
-                    bands={d: T.split_bf16_columns(bands[d]) for d in (-1, 0, 1)},
-                    v_dz=T.split_bf16_columns(v_dz),
-                    v0=T.split_bf16_columns(v0.transpose(1, 2)) if any_taper else None,
-                    dfac=dfac.float(), i_med=i_med, sign=sgn.float(), lowest=lowest / lowest.sum(1, keepdim=True))
+                    bands={d: T.split_bf16_columns(bands[d], dev) for d in (-1, 0, 1)},
+                    v_dz=T.split_bf16_columns(v_dz, dev),
+                    v0=T.split_bf16_columns(v0.transpose(1, 2), dev) if any_taper else None,
+                    dfac=dfac.float().to(dev), i_med=i_med.to(dev), sign=sgn.float().to(dev),
+                    lowest=(lowest / lowest.sum(1, keepdim=True)).to(dev))
                 del bands, v_dz, v0
                 _settle(self.device)
             return self._height
@@ -447,9 +450,9 @@ class Plan:
         """The transposed per-column matrices (two planes): V_inv^T [2M, D] and V_fwd^T [D, 2M] of every column."""
         with self._lock:
             if self._col_adj is None:
-                v_fwd, v_inv = T.vertical_matrices_columns(self._columns, self._k, self._factor, self.length)
-                self._col_adj = dict(v_inv_T=T.split_bf16_columns(v_inv.transpose(1, 2)),
-                                     v_fwd_T=T.split_bf16_columns(v_fwd.transpose(1, 2)))
+                v_fwd, v_inv = T.vertical_matrices_t(self._columns, self._k, self._factor, self.length)
+                self._col_adj = dict(v_inv_T=T.split_bf16_columns(v_inv.transpose(1, 2), self.device),
+                                     v_fwd_T=T.split_bf16_columns(v_fwd.transpose(1, 2), self.device))
                 del v_fwd, v_inv
                 _settle(self.device)
             return self._col_adj
@@ -624,18 +627,18 @@ class Engine:
         shared, span = flags.tolist()
         columns = None
         if shared:
-            # the shared column, shifted in float64 (the spacings then equal the float64 reference's bit for bit, so
-            # a tie between spacings at the taper's median resolves to the same interval)
-            z0 = zc[0, :, 0].double()
-            column = (z0 - z0.min()).cpu().numpy()
+            # the shared column, shifted in float64 on the host (the spacings then equal the float64 reference's bit
+            # for bit, so a tie between spacings at the taper's median resolves to the same interval)
+            z0 = zc[0, :, 0].cpu().double()
+            column = (z0 - z0.min()).numpy()
             key = (B, D, H, W, float(dx), float(dy), float(span), column.tobytes())
         else:
-            # the reference's per-column path (models/ffno_model.py:277-283): every column's own shifted heights
-            zd = z.reshape(B, D, H * W).double()
+            # the reference's per-column path (models/ffno_model.py:277-283): every column's own shifted heights, in
+            # float64 on the host
+            zd = zc.cpu().double()
             columns = (zd - zd.amin(dim=1, keepdim=True)).transpose(1, 2).reshape(B * H * W, D).contiguous()
             column = None
-            key = (B, D, H, W, float(dx), float(dy), float(span),
-                   hashlib.sha1(columns.cpu().numpy().tobytes()).hexdigest())
+            key = (B, D, H, W, float(dx), float(dy), float(span), hashlib.sha1(columns.numpy().tobytes()).hexdigest())
         with self._lock:
             p = self._plans.get(key)
             if p is None:
@@ -1030,6 +1033,42 @@ class Engine:
         _lib.call("ffno_head", rows, prev, _p(h_last), _p(Wt.p1), _p(Wt.p2), _p(Wt.p2_bias), _p(x),
                   ctypes.byref(Wt.head), _p(y), st)
         return y
+
+    # ------------------------------------------------------------------------------------------------ head only
+    def _head_args(self, head, x_in):
+        if head.device.type == "cpu" or x_in.device.type == "cpu":
+            raise RuntimeError("the FFNO3D HIP engine runs on the GPU only; move head and x_in to a cuda device")
+        self.sync(head.device)
+        m = self.module
+        n_head = m.out_channels + len(m.species_levels)
+        if head.dim() < 3 or head.shape[1] != n_head:
+            raise ValueError(f"head must have shape [B, {n_head}, ...]; got {tuple(head.shape)}")
+        if tuple(x_in.shape) != (head.shape[0], 6) + tuple(head.shape[2:]):
+            raise ValueError(f"x_in must have shape {(head.shape[0], 6) + tuple(head.shape[2:])}; got "
+                             f"{tuple(x_in.shape)}")
+        voxels = int(math.prod(head.shape[2:]))
+        return head.shape[0], voxels, head.contiguous().float(), x_in.contiguous().float()
+
+    def head_output(self, head, x_in):
+        """FFNO3D.head_output (models/ffno_model.py:730-749): normalised log10 n [B, Cout, ...] from the head channels
+        [B, Cout + S, ...] and the normalised inputs [B, 6, ...] (ffno_head_output)."""
+        B, voxels, head, x_in = self._head_args(head, x_in)
+        y = torch.empty((B, self.module.out_channels) + tuple(head.shape[2:]), device=head.device)
+        _lib.call("ffno_head_output", B, voxels, _p(head), _p(x_in), ctypes.byref(self.weights.head), _p(y),
+                  self._stream())
+        return y
+
+    def head_output_bwd(self, head, x_in, ybar):
+        """(head_bar, x_in_bar) of <ybar, head_output(head, x_in)> (ffno_head_output_bwd)."""
+        B, voxels, head, x_in = self._head_args(head, x_in)
+        if tuple(ybar.shape) != (B, self.module.out_channels) + tuple(head.shape[2:]):
+            raise ValueError(f"ybar has shape {tuple(ybar.shape)}; expected {(B, self.module.out_channels) + tuple(head.shape[2:])}")
+        ybar = ybar.contiguous().float()
+        head_bar = torch.empty_like(head)
+        x_bar = torch.empty_like(x_in)
+        _lib.call("ffno_head_output_bwd", B, voxels, _p(head), _p(x_in), ctypes.byref(self.weights.head), _p(ybar),
+                  _p(head_bar), _p(x_bar), self._stream())
+        return head_bar, x_bar
 
     # ------------------------------------------------------------------------------------------------ tangent
     def _gn_tangent(self, plan, y, yd, rec, gamma, part2):

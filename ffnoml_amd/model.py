@@ -221,6 +221,18 @@ class FFNO3D(nn.Module):
             self._engine = Engine(self)
         return self._engine
 
-    def forward(self, x, z_scale, dx, dy):
+    def head_output(self, head, x_in):
+        """Normalised log10 populations [B, Cout, D, H, W] from the head's channels head [B, Cout + S, D, H, W] and
+        the normalised inputs x_in [B, 6, D, H, W] (models/ffno_model.py:730-749), by the HIP library in float32;
+        differentiable in both arguments."""
+        from .autograd import head_output_apply
+        return head_output_apply(self, head, x_in)
+
+    def forward(self, x, z_scale, dx, dy, collect_stats=False, branch_mask=None):
+        """models/ffno_model.py:800-832. collect_stats and branch_mask, the reference's training-statistics and
+        branch-ablation switches, are accepted at their defaults only."""
+        if collect_stats or branch_mask is not None:
+            raise NotImplementedError("collect_stats and branch_mask (per-block statistics and branch ablation) are "
+                                      "not part of the HIP path; call forward with their defaults")
         from .autograd import ffno_apply
         return ffno_apply(self, x, z_scale, dx, dy)

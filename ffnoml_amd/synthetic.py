@@ -96,6 +96,30 @@ def random_model(seed=0, **config):
     return perturb_state(FFNO3D(**config).eval(), seed)
 
 
+def si_request(D, H, W, seed=None, levels=12, cotangent_scale=1e-15):
+    """One service request in the SI units the client sends: features (6, nz, nx, ny) float32 = (T [K], vx, vy, vz
+    [m/s], log10 ne, log10 rho) of a synthetic atmosphere, heights (nz, nx, ny) [m] and a population cotangent
+    (nz, nx, ny, levels) of N(0, 1) * cotangent_scale. seed None: D + H + W."""
+    seed = D + H + W if seed is None else int(seed)
+    feats, zc = atmosphere_features(1, D, H, W, seed=seed)
+    si = feats[0].copy()
+    si[0] = 10.0 ** si[0]
+    si[1:4] *= 1e3
+    z = np.broadcast_to(zc[:, None, None] * 1e6, si.shape[1:]).astype(np.float32).copy()
+    cot = np.random.default_rng(seed).standard_normal((D, H, W, levels), dtype=np.float32) * np.float32(cotangent_scale)
+    return si.astype(np.float32), z, cot
+
+
+def perturbation(shape, seed=0, scale=0.02):
+    """scale * N(0, 1) in normalised input units, shape [B, 6, D, H, W]: the inputs whose prediction is the inversion
+    loss's target. Item b comes from a generator of its own (seed, b), so that any batch item regenerates alone."""
+    out = torch.empty(shape, dtype=torch.float32)
+    for b in range(shape[0]):
+        g = torch.Generator().manual_seed(int(seed) * 1000 + 555 + b)
+        out[b] = scale * torch.randn(shape[1:], generator=g, dtype=torch.float32)
+    return out
+
+
 def cotangent(shape, seed=0):
     g = torch.Generator().manual_seed(int(seed) + 777)
     return torch.randn(shape, generator=g, dtype=torch.float32)

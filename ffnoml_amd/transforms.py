@@ -1,4 +1,5 @@
-"""Transform matrices of the spectral branches, built once per plan in float64 on the host and uploaded in split-bf16.
+"""Transform matrices of the spectral branches, built once per plan in float64 on the host (CPU) and uploaded in
+split-bf16: no float64 arithmetic runs on the GPU.
 
 Horizontal (models/ffno_model.py:441, :448): torch.fft.rfft2 over (H, W), then irfft2 with s=(H, W), written as four
 real matrices acting on spectra held as separate real/imaginary planes:
@@ -48,37 +49,6 @@ def horizontal_matrices(H, W):
     return dict(w_fwd=w_fwd, h_fwd=h_fwd, h_inv=h_inv, w_inv=w_inv)
 
 
-def filon_weights(z, k):
-    """Filon quadrature weights (q_re, q_im) [D, M], float64: sum_j x_j q_jm = int x(z) exp(-i k_m z) dz for x linear
-    between the levels z [D] (any order). Same quadrature and small-angle series (|theta| < 1e-2) as
-    models/ffno_model.py:37-64."""
-    z = np.asarray(z, np.float64)
-    k = np.asarray(k, np.float64)
-    z0, z1 = z[:-1, None], z[1:, None]
-    low = np.minimum(z0, z1)
-    h = np.abs(z1 - z0)
-    th = h * k[None, :]
-    small = np.abs(th) < 1e-2
-    tt = np.where(small, 1.0, th)
-    c, s = np.cos(tt), np.sin(tt)
-    up_re = np.where(small, 0.5 - th ** 2 / 8, (c + tt * s - 1) / tt ** 2)
-    up_im = np.where(small, -th / 3 + th ** 3 / 30, (tt * c - s) / tt ** 2)
-    lo_re = np.where(small, 0.5 - th ** 2 / 24, s / tt - up_re)
-    lo_im = np.where(small, -th / 6 + th ** 3 / 120, (c - 1) / tt - up_im)
-    ph = low * k[None, :]
-    e_re, e_im = h * np.cos(ph), -h * np.sin(ph)
-    a_re, a_im = e_re * lo_re - e_im * lo_im, e_re * lo_im + e_im * lo_re
-    b_re, b_im = e_re * up_re - e_im * up_im, e_re * up_im + e_im * up_re
-    asc = z0 <= z1
-    q_re = np.zeros((len(z), len(k)))
-    q_im = np.zeros((len(z), len(k)))
-    q_re[:-1] += np.where(asc, a_re, b_re)
-    q_re[1:] += np.where(asc, b_re, a_re)
-    q_im[:-1] += np.where(asc, a_im, b_im)
-    q_im[1:] += np.where(asc, b_im, a_im)
-    return q_re, q_im
-
-
 def vertical_box(span, reference_length, min_wavelength, modes):
     """(L, k [M] as the reference's float32 values in float64, mode factors [M]) (models/ffno_model.py:248-259,
     :226-229): L = reference_length unless twice the tallest column is longer."""
@@ -92,26 +62,6 @@ def vertical_box(span, reference_length, min_wavelength, modes):
     factor = np.full(modes, 2.0)
     factor[0] = 1.0
     return length, k, factor
-
-
-def column_taper(column, k):
-    """Half-cosine taper [M] from 2 pi / (3 dz) to pi / dz, dz the lower median of |spacing|
-    (models/ffno_model.py:239-246; torch.median returns the lower of the two middle values)."""
-    d = np.sort(np.abs(np.diff(np.asarray(column, np.float64))))
-    dz = d[(len(d) - 1) // 2]
-    k_nyq = math.pi / dz
-    u = np.clip((k - 2 * k_nyq / 3) / (k_nyq / 3), 0.0, 1.0)
-    return 0.5 * (1 + np.cos(math.pi * u))
-
-
-def vertical_matrices(column, k, factor, length):
-    """V_fwd [2M, D] and V_inv [D, 2M] for the shifted column heights [D] (float64)."""
-    q_re, q_im = filon_weights(column, k)
-    v_fwd = np.concatenate([q_re.T, q_im.T], 0)
-    f = factor * column_taper(column, k)
-    ph = np.outer(np.asarray(column, np.float64), k)
-    v_inv = np.concatenate([f * np.cos(ph), -f * np.sin(ph)], 1) / length
-    return v_fwd, v_inv
 
 
 def dft_factors(n, largest=64):
@@ -253,182 +203,160 @@ def split_bf16(a, device, parts=2):
     return (planes[0], planes[1], m, k, mp, kp) + tuple(planes[2:])
 
 
-def filon_weights_torch(z, k):
-    """filon_weights in torch (float64, differentiable), for the height derivatives of the transform."""
-    z0, z1 = z[:-1, None], z[1:, None]
-    low = torch.minimum(z0, z1)
-    h = (z1 - z0).abs()
-    th = h * k[None, :]
-    small = th.abs() < 1e-2
-    tt = torch.where(small, torch.ones_like(th), th)
-    c, s = torch.cos(tt), torch.sin(tt)
-    up_re = torch.where(small, 0.5 - th ** 2 / 8, (c + tt * s - 1) / tt ** 2)
-    up_im = torch.where(small, -th / 3 + th ** 3 / 30, (tt * c - s) / tt ** 2)
-    lo_re = torch.where(small, 0.5 - th ** 2 / 24, s / tt - up_re)
-    lo_im = torch.where(small, -th / 6 + th ** 3 / 120, (c - 1) / tt - up_im)
-    ph = low * k[None, :]
-    e_re, e_im = h * torch.cos(ph), -h * torch.sin(ph)
-    a_re, a_im = e_re * lo_re - e_im * lo_im, e_re * lo_im + e_im * lo_re
-    b_re, b_im = e_re * up_re - e_im * up_im, e_re * up_im + e_im * up_re
-    asc = z0 <= z1
-    pad_end, pad_start = (0, 0, 0, 1), (0, 0, 1, 0)
-    F = torch.nn.functional
-    q_re = F.pad(torch.where(asc, a_re, b_re), pad_end) + F.pad(torch.where(asc, b_re, a_re), pad_start)
-    q_im = F.pad(torch.where(asc, a_im, b_im), pad_end) + F.pad(torch.where(asc, b_im, a_im), pad_start)
-    return q_re, q_im
-
-
-def filon_height_bands(column, k):
-    """G[delta] [D, 2M] for delta = -1, 0, 1: G[delta][e] = d q[e + delta] / d z_e (real parts, then imaginary), the
-    three non-zero bands of the Filon weights' height Jacobian (q of level d depends on levels d-1, d, d+1), from
-    forward-mode derivatives along three interleaved tangents."""
-    z = torch.tensor(np.asarray(column, np.float64))
-    kk = torch.tensor(np.asarray(k, np.float64))
-    D, M = len(z), len(kk)
-    bands = {d: np.zeros((D, 2 * M)) for d in (-1, 0, 1)}
-    idx = np.arange(D)
-    for color in range(3):
-        v = torch.zeros(D, dtype=torch.float64)
-        v[color::3] = 1.0
-        with _FORWARD_AD_LOCK:           # forward-AD levels are process-global: one plan build at a time here
-            _, (dre, dim) = torch.func.jvp(lambda zz: filon_weights_torch(zz, kk), (z,), (v,))
-        dre, dim = dre.numpy(), dim.numpy()
-        for delta in (-1, 0, 1):
-            e = idx - delta                        # row d of the derivative belongs to level e = d - delta
-            ok = (e >= 0) & (e < D) & (e % 3 == color)
-            bands[delta][e[ok], :M] = dre[idx[ok]]
-            bands[delta][e[ok], M:] = dim[idx[ok]]
-    return bands
-
-
-def vertical_height_terms(column, k, factor, length):
-    """The height derivatives of the synthesis: V_dz [D, 2M] = d V_inv / d z_d (-f k sin, -f k cos over L), the
-    unscaled synthesis V0 [D, 2M] = (cos, -sin) / L, and per mode factor * d taper / d dz at the column's dz
-    together with the index i of the median spacing (dz = |z_{i+1} - z_i|). Spacings that tie at the median make the
-    median non-differentiable; the gradient goes to the lower middle of a stable sort, the interval torch.median
-    (models/ffno_model.py:241) returns on the CPU for ties."""
-    col = np.asarray(column, np.float64)
-    f = factor * column_taper(col, k)
-    ph = np.outer(col, k)
-    v_dz = np.concatenate([-f * k * np.sin(ph), -f * k * np.cos(ph)], 1) / length
-    v0 = np.concatenate([np.cos(ph), -np.sin(ph)], 1) / length
-    d = np.abs(np.diff(col))
-    i = int(np.argsort(d, kind="stable")[(len(d) - 1) // 2])
-    dz = d[i]
-    k_nyq = math.pi / dz
-    u = (k - 2 * k_nyq / 3) / (k_nyq / 3)
-    inside = (u > 0) & (u < 1)
-    # taper = (1 + cos(pi u)) / 2 with u = (3 k dz / pi) - 2: d taper / d dz = -(pi/2) sin(pi u) * 3 k / pi
-    dtaper = np.where(inside, -0.5 * np.sin(math.pi * u) * 3.0 * k, 0.0)
-    return v_dz, v0, factor * dtaper, i, float(np.sign(col[i + 1] - col[i]))
-
-
 # ----------------------------------------------------------------------------------------------------------------
-# per-column height grids (the reference's per-column path, models/ffno_model.py:277-283, :304-305): one analysis and
-# one synthesis matrix per column, built on the device in float64
+# vertical (height) transforms, on one shared column [D] or on every column of a per-column grid [n, D] (the
+# reference's per-column path, models/ffno_model.py:277-283, :304-305): one torch implementation over any leading
+# dimensions, evaluated in float64 on the host
 # ----------------------------------------------------------------------------------------------------------------
-def filon_weights_columns(z, k):
-    """(q_re, q_im) [n, D, M]: filon_weights for every column of z [n, D] (torch float64), k [M] torch float64."""
-    z0, z1 = z[:, :-1, None], z[:, 1:, None]
+def _t(a, like=None):
+    """a as a float64 tensor (on like's device when given)."""
+    t = a if torch.is_tensor(a) else torch.as_tensor(np.asarray(a, np.float64))
+    return t if like is None else t.to(like.device)
+
+
+def filon_weights_t(z, k):
+    """Filon quadrature weights (q_re, q_im) [..., D, M] of the heights z [..., D] at the wavenumbers k [M] (torch,
+    differentiable in z): sum_j x_j q_jm = int x(z) exp(-i k_m z) dz for x linear between the levels (any order).
+    Same quadrature and small-angle series (|theta| < 1e-2) as models/ffno_model.py:37-64."""
+    z0, z1 = z[..., :-1, None], z[..., 1:, None]
     low = torch.minimum(z0, z1)
     h = (z1 - z0).abs()
     th = h * k
     small = th.abs() < 1e-2
     tt = torch.where(small, torch.ones_like(th), th)
     c, s = torch.cos(tt), torch.sin(tt)
+    # shares of an interval's upper end, int_0^1 u e^{-i th u} du, and of its lower end, int_0^1 (1-u) e^{-i th u} du
     up_re = torch.where(small, 0.5 - th ** 2 / 8, (c + tt * s - 1) / tt ** 2)
     up_im = torch.where(small, -th / 3 + th ** 3 / 30, (tt * c - s) / tt ** 2)
     lo_re = torch.where(small, 0.5 - th ** 2 / 24, s / tt - up_re)
     lo_im = torch.where(small, -th / 6 + th ** 3 / 120, (c - 1) / tt - up_im)
     ph = low * k
     e_re, e_im = h * torch.cos(ph), -h * torch.sin(ph)
-    a_re, a_im = e_re * lo_re - e_im * lo_im, e_re * lo_im + e_im * lo_re
-    b_re, b_im = e_re * up_re - e_im * up_im, e_re * up_im + e_im * up_re
+    a_re, a_im = e_re * lo_re - e_im * lo_im, e_re * lo_im + e_im * lo_re      # the interval's lower level
+    b_re, b_im = e_re * up_re - e_im * up_im, e_re * up_im + e_im * up_re      # its upper level
     asc = z0 <= z1
-    Fn = torch.nn.functional
-    q_re = Fn.pad(torch.where(asc, a_re, b_re), (0, 0, 0, 1)) + Fn.pad(torch.where(asc, b_re, a_re), (0, 0, 1, 0))
-    q_im = Fn.pad(torch.where(asc, a_im, b_im), (0, 0, 0, 1)) + Fn.pad(torch.where(asc, b_im, a_im), (0, 0, 1, 0))
+    pad = torch.nn.functional.pad
+    q_re = pad(torch.where(asc, a_re, b_re), (0, 0, 0, 1)) + pad(torch.where(asc, b_re, a_re), (0, 0, 1, 0))
+    q_im = pad(torch.where(asc, a_im, b_im), (0, 0, 0, 1)) + pad(torch.where(asc, b_im, a_im), (0, 0, 1, 0))
     return q_re, q_im
 
 
-def column_taper_columns(z, k):
-    """Taper [n, M] of every column (column_taper per column: the lower median of its |spacing|)."""
-    d = (z[:, 1:] - z[:, :-1]).abs()
-    dz = torch.sort(d, dim=-1, stable=True).values[:, (d.shape[1] - 1) // 2 : (d.shape[1] - 1) // 2 + 1]
+def filon_weights(z, k):
+    """filon_weights_t of one column (numpy float64 in and out): (q_re, q_im) [D, M]."""
+    q_re, q_im = filon_weights_t(_t(z), _t(k))
+    return q_re.numpy(), q_im.numpy()
+
+
+def _median_spacing(z):
+    """(index i [...], spacing |z_{i+1} - z_i| [..., 1]) of the lower median of every column's |spacing|: the lower
+    middle of a stable sort, the interval CPU torch.median returns for ties (models/ffno_model.py:239-246)."""
+    d = (z[..., 1:] - z[..., :-1]).abs()
+    i = torch.sort(d, dim=-1, stable=True).indices[..., (d.shape[-1] - 1) // 2]
+    return i, d.gather(-1, i[..., None])
+
+
+def column_taper_t(z, k):
+    """Half-cosine taper [..., M] from 2 pi / (3 dz) to pi / dz, dz the lower median |spacing| of each column
+    (models/ffno_model.py:239-246); 1 for a single level."""
+    if z.shape[-1] < 2:
+        return torch.ones(z.shape[:-1] + k.shape, dtype=z.dtype, device=z.device)
+    _, dz = _median_spacing(z)
     k_nyq = math.pi / dz
     u = ((k - 2 * k_nyq / 3) / (k_nyq / 3)).clamp(0.0, 1.0)
     return 0.5 * (1 + torch.cos(math.pi * u))
 
 
-def vertical_matrices_columns(z, k, factor, length):
-    """V_fwd [n, 2M, D] and V_inv [n, D, 2M] for the shifted column heights z [n, D] (torch float64 on the device);
-    vertical_matrices per column."""
-    kt = torch.as_tensor(np.asarray(k, np.float64), device=z.device)
-    ft = torch.as_tensor(np.asarray(factor, np.float64), device=z.device)
-    q_re, q_im = filon_weights_columns(z, kt)
-    v_fwd = torch.cat([q_re.transpose(1, 2), q_im.transpose(1, 2)], 1)
-    if z.shape[1] > 1:
-        f = ft * column_taper_columns(z, kt)                                    # [n, M]
-    else:
-        f = ft.expand(z.shape[0], -1)
-    ph = z[:, :, None] * kt                                                     # [n, D, M]
-    v_inv = torch.cat([f[:, None, :] * torch.cos(ph), -f[:, None, :] * torch.sin(ph)], 2) / length
+def column_taper(column, k):
+    """column_taper_t of one column (numpy float64): [M]."""
+    return column_taper_t(_t(column), _t(k)).numpy()
+
+
+def vertical_matrices_t(z, k, factor, length):
+    """V_fwd [..., 2M, D] (Filon weights, real parts then imaginary) and V_inv [..., D, 2M] (f cos(k z) / L and
+    -f sin(k z) / L, f = mode factor * taper) of the shifted heights z [..., D]."""
+    k, factor = _t(k, z), _t(factor, z)
+    q_re, q_im = filon_weights_t(z, k)
+    v_fwd = torch.cat([q_re.transpose(-1, -2), q_im.transpose(-1, -2)], -2)
+    f = factor * column_taper_t(z, k)
+    ph = z[..., :, None] * k
+    v_inv = torch.cat([f[..., None, :] * torch.cos(ph), -f[..., None, :] * torch.sin(ph)], -1) / length
     return v_fwd, v_inv
 
 
-def split_bf16_columns(a, parts=2):
-    """(hi, lo, m, k, m_pad, k_pad[, l3]) of per-column matrices a [n, m, k] (torch float64): bf16 planes
-    [n, m_pad, k_pad], zero padded, as split_bf16 per column."""
+def vertical_matrices(column, k, factor, length):
+    """V_fwd [2M, D] and V_inv [D, 2M] of one shared column (numpy float64)."""
+    v_fwd, v_inv = vertical_matrices_t(_t(column), k, factor, length)
+    return v_fwd.numpy(), v_inv.numpy()
+
+
+def filon_height_bands_t(z, k):
+    """G[delta] [..., D, 2M] for delta = -1, 0, 1: G[delta][..., e, :] = d q[..., e + delta, :] / d z_e (real parts,
+    then imaginary), the three non-zero bands of the Filon weights' height Jacobian (the weights of level d depend on
+    levels d-1, d, d+1), from forward-mode derivatives along three interleaved tangents."""
+    k = _t(k, z)
+    D, M = z.shape[-1], k.shape[0]
+    bands = {d: torch.zeros(z.shape[:-1] + (D, 2 * M), dtype=z.dtype, device=z.device) for d in (-1, 0, 1)}
+    idx = torch.arange(D, device=z.device)
+    for color in range(3):
+        v = torch.zeros_like(z)
+        v[..., color::3] = 1.0
+        with _FORWARD_AD_LOCK:           # forward-AD levels are process-global: one plan build at a time here
+            _, (dre, dim) = torch.func.jvp(lambda zz: filon_weights_t(zz, k), (z,), (v,))
+        for delta in (-1, 0, 1):
+            e = idx - delta                        # row d of the derivative belongs to level e = d - delta
+            ok = (e >= 0) & (e < D) & (e % 3 == color)
+            rows, levels = idx[ok], e[ok]
+            bands[delta][..., levels, :M] = dre[..., rows, :]
+            bands[delta][..., levels, M:] = dim[..., rows, :]
+    return bands
+
+
+def filon_height_bands(column, k):
+    """filon_height_bands_t of one column (numpy float64): {delta: [D, 2M]}."""
+    return {d: b.numpy() for d, b in filon_height_bands_t(_t(column), k).items()}
+
+
+def vertical_height_terms_t(z, k, factor, length):
+    """The height derivatives of the synthesis for the heights z [..., D]: V_dz [..., D, 2M] = d V_inv / d z_d
+    (-f k sin, -f k cos over L), the unscaled synthesis V0 [..., D, 2M] = (cos, -sin) / L, factor * d taper / d dz
+    [..., M] at each column's dz, the index i [...] of its median spacing (dz = |z_{i+1} - z_i|) and the sign of
+    that spacing [...]. Spacings that tie at the median make the median non-differentiable; the gradient goes to
+    the lower middle of a stable sort, the interval torch.median (models/ffno_model.py:241) returns on the CPU."""
+    k, factor = _t(k, z), _t(factor, z)
+    ph = z[..., :, None] * k
+    f = factor * column_taper_t(z, k)
+    v_dz = torch.cat([-(f * k)[..., None, :] * torch.sin(ph), -(f * k)[..., None, :] * torch.cos(ph)], -1) / length
+    v0 = torch.cat([torch.cos(ph), -torch.sin(ph)], -1) / length
+    if z.shape[-1] < 2:
+        zero = torch.zeros(z.shape[:-1], dtype=torch.long, device=z.device)
+        return v_dz, v0, torch.zeros(z.shape[:-1] + k.shape, dtype=z.dtype, device=z.device), zero, zero.to(z.dtype)
+    i, dz = _median_spacing(z)
+    k_nyq = math.pi / dz
+    u = (k - 2 * k_nyq / 3) / (k_nyq / 3)
+    inside = (u > 0) & (u < 1)
+    # taper = (1 + cos(pi u)) / 2 with u = (3 k dz / pi) - 2: d taper / d dz = -(pi/2) sin(pi u) * 3 k / pi
+    dtaper = torch.where(inside, -0.5 * torch.sin(math.pi * u) * 3.0 * k, torch.zeros_like(u))
+    sgn = torch.sign(z.gather(-1, (i + 1)[..., None]) - z.gather(-1, i[..., None]))[..., 0]
+    return v_dz, v0, factor * dtaper, i, sgn
+
+
+def vertical_height_terms(column, k, factor, length):
+    """vertical_height_terms_t of one column (numpy float64): (V_dz, V0, dfac, i, sign) with i an int and sign a
+    float."""
+    v_dz, v0, dfac, i, sgn = vertical_height_terms_t(_t(column), k, factor, length)
+    return v_dz.numpy(), v0.numpy(), dfac.numpy(), int(i), float(sgn)
+
+
+def split_bf16_columns(a, device, parts=2):
+    """(hi, lo, m, k, m_pad, k_pad[, l3]) of per-column matrices a [n, m, k] (host torch float64): bf16 planes
+    [n, m_pad, k_pad] on the device, zero padded, as split_bf16 per column (the split is exact arithmetic in float64
+    on the host)."""
     n, m, k = a.shape
     mp, kp = round_up(m, 16), round_up(k, 32)
     rest = torch.nn.functional.pad(a, (0, kp - k, 0, mp - m))
     planes = []
     for _ in range(parts):
         p = rest.to(torch.bfloat16)
-        planes.append(p.contiguous().view(torch.int16))
+        planes.append(p.contiguous().view(torch.int16).to(device))
         rest = rest - p.double()
     return (planes[0], planes[1], m, k, mp, kp) + tuple(planes[2:])
-
-
-def filon_height_bands_columns(z, k):
-    """filon_height_bands for every column of z [n, D] (torch float64): G[delta] [n, D, 2M] with
-    G[delta][:, e] = d q[e + delta] / d z_e (real parts, then imaginary)."""
-    kt = torch.as_tensor(np.asarray(k, np.float64), device=z.device)
-    n, D = z.shape
-    M = kt.shape[0]
-    bands = {d: torch.zeros(n, D, 2 * M, dtype=torch.float64, device=z.device) for d in (-1, 0, 1)}
-    idx = torch.arange(D, device=z.device)
-    for color in range(3):
-        v = torch.zeros_like(z)
-        v[:, color::3] = 1.0
-        with _FORWARD_AD_LOCK:
-            _, (dre, dim) = torch.func.jvp(lambda zz: filon_weights_columns(zz, kt), (z,), (v,))
-        for delta in (-1, 0, 1):
-            e = idx - delta                        # row d of the derivative belongs to level e = d - delta
-            ok = (e >= 0) & (e < D) & (e % 3 == color)
-            rows, levels = idx[ok], e[ok]
-            bands[delta][:, levels, :M] = dre[:, rows]
-            bands[delta][:, levels, M:] = dim[:, rows]
-    return bands
-
-
-def vertical_height_terms_columns(z, k, factor, length):
-    """vertical_height_terms for every column of z [n, D] (torch float64): V_dz [n, D, 2M], V0 [n, D, 2M],
-    factor * d taper / d dz [n, M], the median interval i [n] (lower middle of a stable sort, as CPU torch.median
-    resolves ties) and the sign of its spacing [n]."""
-    kt = torch.as_tensor(np.asarray(k, np.float64), device=z.device)
-    ft = torch.as_tensor(np.asarray(factor, np.float64), device=z.device)
-    n, D = z.shape
-    ph = z[:, :, None] * kt
-    f = ft * column_taper_columns(z, kt) if D > 1 else ft.expand(n, -1)
-    v_dz = torch.cat([-(f * kt)[:, None, :] * torch.sin(ph), -(f * kt)[:, None, :] * torch.cos(ph)], 2) / length
-    v0 = torch.cat([torch.cos(ph), -torch.sin(ph)], 2) / length
-    d = (z[:, 1:] - z[:, :-1]).abs()
-    i = torch.sort(d, dim=-1, stable=True).indices[:, (D - 2) // 2]
-    dz = d.gather(1, i[:, None])
-    k_nyq = math.pi / dz
-    u = (kt - 2 * k_nyq / 3) / (k_nyq / 3)
-    inside = (u > 0) & (u < 1)
-    dtaper = torch.where(inside, -0.5 * torch.sin(math.pi * u) * 3.0 * kt, torch.zeros_like(u))
-    sgn = torch.sign(z.gather(1, (i + 1)[:, None]) - z.gather(1, i[:, None]))[:, 0]
-    return v_dz, v0, ft * dtaper, i, sgn

@@ -111,6 +111,19 @@ int ffno_head(FfnoRows rows, FfnoAssemble in, float* x_out, const uint16_t* p1, 
               const float* p2_bias, const float* x_in /* [B,6,DHW] */, const FfnoHeadDesc* desc, float* y,
               hipStream_t stream);
 
+/* FFNO3D.head_output(head, x_in) (models/ffno_model.py:730-749) on given head channels: per species the MULTI3D LTE
+ * log-weights of T and ne (:116-141), log10 n = log10 rho - log10 m + b tanh(u / b) + log_softmax(head + ln f*) /
+ * ln 10, normalised by mean_Y, std_Y; float32. head [B, Cout + S, DHW], x_in [B, 6, DHW] (normalised inputs), y
+ * [B, Cout, DHW], all channel-first. */
+int ffno_head_output(int B, int64_t voxels, const float* head, const float* x_in, const FfnoHeadDesc* desc, float* y,
+                     hipStream_t stream);
+
+/* Adjoint of ffno_head_output (the backward the reference's autograd runs through :730-749): head_bar [B, Cout + S,
+ * DHW] and x_bar [B, 6, DHW] (the T, ne and rho channels through the LTE weights and the total; the others 0) from
+ * ybar [B, Cout, DHW]. head_bar may alias head. */
+int ffno_head_output_bwd(int B, int64_t voxels, const float* head, const float* x_in, const FfnoHeadDesc* desc,
+                         const float* ybar, float* head_bar, float* x_bar, hipStream_t stream);
+
 /* Split-bf16 GEMM of a small precomputed real matrix A [m x k] against batched data X [k x n]:
  * Y[b][m][n] = sum_k A[m][k] X[b][k][n]. It carries every DFT (rfft/ifft/irfft along W and H) and the vertical
  * Filon transform and its inverse (models/ffno_model.py:441, :448, :272-275, :303-305). Row offsets come from

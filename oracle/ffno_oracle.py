@@ -64,14 +64,15 @@ def group_count(channels, max_groups=8):
 # elementwise pieces
 # ----------------------------------------------------------------------------------------------------------------
 def gelu(x):
-    """Exact (erf) GELU, torch.nn.GELU() default as used throughout models/ffno_model.py."""
-    return 0.5 * x * (1.0 + torch.erf(x * SQRT_HALF))
+    """Exact (erf) GELU, torch.nn.GELU() default as used throughout models/ffno_model.py: 0.5 x (1 + erf(x / sqrt 2)),
+    evaluated by torch's own kernel (one pass)."""
+    return F.gelu(x)
 
 
 def conv1x1(x, weight, bias=None):
     """A 1x1x1 Conv3d on channel-first x [B, Cin, ...] with weight [Cout, Cin(,1,1,1)]."""
     w = weight.reshape(weight.shape[0], -1)
-    y = torch.einsum("oi,bi...->bo...", w, x)
+    y = (w @ x.reshape(x.shape[0], x.shape[1], -1)).reshape((x.shape[0], w.shape[0]) + x.shape[2:])
     if bias is not None:
         y = y + bias.reshape((1, -1) + (1,) * (x.dim() - 2))
     return y
@@ -432,11 +433,13 @@ def model_inputs_si(features, z_m):
     return f, z_m / 1.0e6
 
 
-def predict_si(sd, cfg, features, z_m, dx, dy):
-    """Linear populations [B, L, D, H, W] from SI features (ffno_fsdp_runtime.py:265-281, :76-78)."""
+def predict_si(sd, cfg, features, z_m, dx, dy, force_per_column=False):
+    """Linear populations [B, L, D, H, W] from SI features (ffno_fsdp_runtime.py:265-281, :76-78). force_per_column:
+    the vertical branch's per-column path even on a shared grid (the same values; the height gradient then goes to
+    every column's own heights)."""
     f, z = model_inputs_si(features, z_m)
     mx = sd["mean_X"][None, :, None, None, None]
     sx = sd["std_X"][None, :, None, None, None]
-    y = forward(sd, cfg, (f - mx) / sx, z, dx, dy)
+    y = forward(sd, cfg, (f - mx) / sx, z, dx, dy, force_per_column)
     log = y * sd["std_Y"][None, :, None, None, None] + sd["mean_Y"][None, :, None, None, None]
     return torch.pow(10.0, log.clamp(-37.0, 37.0))

@@ -418,6 +418,24 @@ class Checker:
         self.span(n, "xd_in", xd_in, rows.B * 6 * rows.voxels * F32)
         self.span(n, "yd", yd, rows.B * n_out * rows.voxels * F32)
 
+    def _head_desc_counts(self, desc):
+        d = desc._obj
+        return sum(d.levels[s] + 1 for s in range(d.n_species)), sum(d.levels[s] for s in range(d.n_species))
+
+    def ffno_head_output(self, n, B, voxels, head, x_in, desc, y, stream):
+        n_head, n_out = self._head_desc_counts(desc)
+        self.span(n, "head", head, B * n_head * voxels * F32)
+        self.span(n, "x_in", x_in, B * 6 * voxels * F32)
+        self.span(n, "y", y, B * n_out * voxels * F32)
+
+    def ffno_head_output_bwd(self, n, B, voxels, head, x_in, desc, ybar, head_bar, x_bar, stream):
+        n_head, n_out = self._head_desc_counts(desc)
+        self.span(n, "head", head, B * n_head * voxels * F32)
+        self.span(n, "x_in", x_in, B * 6 * voxels * F32)
+        self.span(n, "ybar", ybar, B * n_out * voxels * F32)
+        self.span(n, "head_bar", head_bar, B * n_head * voxels * F32)
+        self.span(n, "x_bar", x_bar, B * 6 * voxels * F32)
+
 
 def dry_engine(model):
     """An Engine over `model` (kept on the host) whose tensors live on the meta device and whose launches go to a
@@ -457,21 +475,12 @@ def check_shape(model, B, D, H, W, stride=1, want_z=True, column=None, two_pass=
         zs = col - col.min()
         span = float(col.max() - col.min())
         meta = torch.device("meta")
-        if percol:              # every column its own heights: per-column matrices
+        if percol:              # every column its own heights (host float64, as the engine builds them)
             n = B * H * W
-            cols = torch.empty(n, D, dtype=torch.float64, device=meta)
+            jitter = 0.01 * torch.rand(n, D, generator=torch.Generator().manual_seed(D), dtype=torch.float64)
+            cols = torch.from_numpy(zs)[None] + jitter
+            cols = cols - cols.amin(dim=1, keepdim=True)
             plan = E.Plan(eng, B, D, H, W, None, span, 48e3, 48e3, columns=cols)
-            M = plan.modes
-
-            def height_tables():  # the per-column height tables at their real shapes (their values need data)
-                from ffnoml_amd.transforms import split_bf16_columns as split
-
-                def mat(m, k):
-                    return split(torch.empty(n, m, k, dtype=torch.float64, device=meta))
-                return dict(bands={d: mat(D, 2 * M) for d in (-1, 0, 1)}, v_dz=mat(D, 2 * M), v0=mat(2 * M, D),
-                            dfac=torch.empty(n, M, device=meta), i_med=torch.zeros(n, dtype=torch.long, device=meta),
-                            sign=torch.empty(n, device=meta), lowest=torch.empty(n, D, device=meta))
-            plan._column_height_tables = height_tables
         else:
             plan = E.Plan(eng, B, D, H, W, zs, span, 48e3, 48e3)
         x = torch.empty(B, 6, D, H, W, device=meta)
@@ -481,6 +490,9 @@ def check_shape(model, B, D, H, W, stride=1, want_z=True, column=None, two_pass=
         ybar = torch.empty_like(y)
         eng.vjp(x, z, 48e3, 48e3, ybar, saved, want_z=want_z, plan=plan)
         eng.jvp(x, z, 48e3, 48e3, torch.empty_like(x), plan=plan)
+        head = torch.empty(B, model.out_channels + len(model.species_levels), D, H, W, device=meta)
+        eng.head_output(head, x)
+        eng.head_output_bwd(head, x, ybar)
     finally:
         restore()
     return len(chk.calls)

@@ -100,3 +100,68 @@ def test_graph_replays_match_eager(nz, nx, ny):
         assert len(batcher.graphs) == 1
     finally:
         batcher.close()
+
+
+def test_serving_trace_tiles_with_the_benchmark_model():
+    """The serving workload's tile shapes (96 x 32^2, 128 x 64^2, 192 x 128^2; forward and VJP) with the benchmark
+    model (width 64, 8 layers, 32 vertical modes) through the batcher, every result against the backend alone
+    (1e-5); the smallest tile against the float64 oracle in SI units (tests/golden/wl_service.npz): log10 n 1e-3,
+    the feature gradient per channel and the per-level height gradient 1e-2."""
+    import hashlib
+    import json
+    import os
+    from golden_cases import GOLDEN
+    from ffnoml_amd import synthetic
+    from ffnoml_amd.atoms import species
+    from ffnoml_amd.backend import GPUBackend
+    from ffnoml_amd.serving import ContinuousBatcher
+    data = np.load(os.path.join(GOLDEN, "wl_service.npz"))
+    config = json.loads(str(data["config_json"]))
+    config["species"] = species(config["species"])
+    _, _, mx, sx = synthetic.atmosphere(8, 128, 64, 64, seed=0)
+    model = synthetic.random_model(0, **config)
+    model.set_normalization(mx, sx, np.zeros(12), np.ones(12))
+    h = hashlib.sha256()
+    for k, v in sorted(model.state_dict().items()):
+        h.update(k.encode())
+        h.update(np.ascontiguousarray(v.double().numpy()).tobytes())
+    assert h.hexdigest() == str(data["state_sha256"])
+    backend = GPUBackend(None, model=model, level_names=[f"L{i}" for i in range(12)])
+    tiles = [(96, 32, 32), (128, 64, 64), (192, 128, 128)]
+    reqs = [(op, *synthetic.si_request(*t)) for t in tiles for op in ("PREDICT", "VJP")]
+    alone = []
+    for op, f, z, cot in reqs:
+        print(f"[service] alone {op} {z.shape}", flush=True)
+        alone.append(backend.predict_local(f, z, 48e3, 48e3) if op == "PREDICT" else
+                     backend.vjp_local(f, z, 48e3, 48e3, cot))
+    batcher = ContinuousBatcher({"m": backend}, slots=3, max_batch=4)
+    try:
+        with cf.ThreadPoolExecutor(8) as ex:
+            futs = [ex.submit(lambda a: batcher.submit(a[0], "m", a[1], a[2], 48e3, 48e3,
+                                                       a[3] if a[0] == "VJP" else None).result(), r) for r in reqs]
+            mixed = [fu.result() for fu in futs]
+    finally:
+        batcher.close()
+    for (op, f, z, _), a, m in zip(reqs, alone, mixed):
+        if op == "PREDICT":
+            err = _rel(m["populations"], a)
+        else:
+            err = max(float(np.linalg.norm(m[k] - a[k]) / np.linalg.norm(a[k])) for k in ("features", "z_scale"))
+        print(f"[service] {op} {z.shape} mix vs alone {err:.2e}", flush=True)
+        assert err <= 1e-5
+    stride = int(data["stride"])
+    n = alone[0]                                                  # (nz, nx, ny, 12)
+    log_n = np.log10(n.transpose(3, 0, 1, 2)[:, ::stride, ::stride, ::stride].astype(np.float64))
+    ref = data["log10_n"].astype(np.float64)
+    err = float((np.abs(log_n - ref) / np.maximum(np.abs(ref), 1.0)).max())
+    vjp = alone[1]
+    fb = vjp["features"][:, ::stride, ::stride, ::stride].astype(np.float64)
+    fref = data["feature_bar"].astype(np.float64)
+    ferr = max(float(np.linalg.norm(fb[c] - fref[c]) / np.linalg.norm(fref[c])) for c in range(6))
+    zl = vjp["z_scale"].astype(np.float64).sum(axis=(1, 2))
+    zerr = float(np.linalg.norm(zl - data["z_bar_level"]) / np.linalg.norm(data["z_bar_level"]))
+    print(f"[service] 96x32x32 vs oracle: log10 n {err:.2e}, feature gradient {ferr:.2e}, height gradient {zerr:.2e}",
+          flush=True)
+    assert err <= 1e-3
+    assert ferr <= 1e-2
+    assert zerr <= 1e-2

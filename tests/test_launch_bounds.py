@@ -90,3 +90,51 @@ def test_checker_catches_a_short_buffer_and_covers_every_entry_point():
     finally:
         LB.Checker.__call__ = orig
     assert declared <= calls, f"entry points never checked: {sorted(declared - calls)}"
+
+
+@pytest.mark.parametrize("percol", [False, True])
+def test_no_float64_arithmetic_on_the_device(percol, monkeypatch):
+    """Every tensor op of the engine's plans, forward, adjoint (with the height gradient) and tangent that touches
+    the device (the meta device of the dry run) is float32 or integer, and so is every tensor handed to the library:
+    float64 runs on the host only. (The host's forward-mode derivatives of the Filon weights run torch.func, whose
+    own shape propagation uses meta tensors; the watch pauses inside them.)"""
+    import torch
+    from torch.utils._python_dispatch import TorchDispatchMode
+    from torch.utils._pytree import tree_leaves
+    from ffnoml_amd import engine as E, transforms as T
+    seen, paused = [], [0]
+
+    class Watch(TorchDispatchMode):
+        def __torch_dispatch__(self, func, types, args=(), kwargs=None):
+            out = func(*args, **(kwargs or {}))
+            if not paused[0]:
+                for t in tree_leaves(out):
+                    if torch.is_tensor(t) and t.device.type == "meta" and t.dtype == torch.float64:
+                        seen.append(str(func))
+            return out
+
+    real_bands = T.filon_height_bands_t
+
+    def host_bands(*a, **k):
+        paused[0] += 1
+        try:
+            return real_bands(*a, **k)
+        finally:
+            paused[0] -= 1
+
+    monkeypatch.setattr(T, "filon_height_bands_t", host_bands)
+    passed = []
+    import launch_bounds as LB
+    real_dry = LB.dry_engine
+
+    def dry(model):
+        eng, chk, restore = real_dry(model)
+        inner = E._p
+        E._p = lambda t: (passed.append(t.dtype) if t is not None else None) or inner(t)
+        return eng, chk, restore
+
+    monkeypatch.setattr(LB, "dry_engine", dry)
+    with Watch():
+        LB.check_shape(_model(layers=2), 2, 12, 8, 8, stride=1, percol=percol)
+    assert not seen, f"float64 on the device: {sorted(set(seen))}"
+    assert passed and torch.float64 not in passed
