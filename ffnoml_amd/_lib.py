@@ -71,6 +71,7 @@ SIGNATURES = {
     "ffno_gn_finalize": [ctypes.c_int, ctypes.c_int, ctypes.c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr],
     "ffno_block_in": [FfnoRows, FfnoAssemble, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr],
     "ffno_post": [FfnoRows, c_ptr, c_ptr, c_ptr, c_ptr, FfnoStatsOut, c_ptr],
+    "ffno_post_gn_noalloc": [FfnoRows, c_ptr, c_ptr, c_ptr, c_ptr] + [c_ptr] * 6 + [c_ptr],
     "ffno_fuse": [FfnoRows, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr,
                   FfnoStatsOut, c_ptr, c_ptr],
     "ffno_resmlp": [FfnoRows, FfnoAssemble, c_ptr, c_ptr, c_ptr, c_ptr, FfnoStatsOut, c_ptr],

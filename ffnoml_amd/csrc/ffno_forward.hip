@@ -1,8 +1,16 @@
 // FFNO3D forward kernels for gfx950 (MI355X): register-chained split-bf16 MFMA MLPs with fused GroupNorm
 // statistics, the split-bf16 twiddle GEMM for the DFTs and the vertical Filon transform, and the per-mode
 // spectral channel mix. Conventions are in ffno_device.h.
+#include <mutex>
+
 #include "ffno_device.h"
 #include "ffno_head.h"
+
+// Partial GroupNorm statistics of the allocation-free entry point ffno_post_gn_noalloc: [B][nwg][8][3] floats of one
+// call, at most FFNO_NOALLOC_MAX_SLOTS (B * nwg) slots (384 KiB), held by every GPU the library runs on until the
+// process ends. Host code never uses this name as an address: the entry point looks up each device's own copy with
+// hipGetSymbolAddress once and passes that pointer to the kernels.
+static __device__ __align__(256) float scratch[FFNO_NOALLOC_MAX_SLOTS * 8 * 3];
 
 namespace ffno {
 
@@ -799,6 +807,31 @@ __global__ __launch_bounds__(NWM * 64) void k_mix(FfnoMix mx) {
   }
 }
 
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[MAX_DEVICES] = {};     // the device address of `scratch` on each GPU, looked up once
+
+// The current device's copy of `scratch`: hipGetSymbolAddress on the first call on a device (with that device
+// current), the cached pointer afterwards.
+int scratch_of_current_device(float** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES)
+    return fail("ffno_post_gn_noalloc: no current HIP device (or its index is above 63)");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      snprintf(g_err, sizeof(g_err), "ffno_post_gn_noalloc: hipGetSymbolAddress(scratch) failed: %s",
+               hipGetErrorString(e));
+      return -1;
+    }
+    g_scratch_dev[dev] = p;
+  }
+  *out = static_cast<float*>(g_scratch_dev[dev]);
+  return 0;
+}
+
 }  // namespace
 }  // namespace ffno
 
@@ -862,6 +895,26 @@ int ffno_post(FfnoRows rows, const float* u, const uint16_t* w1, const uint16_t*
     });
   });
   return check_launch("ffno_post");
+}
+
+int ffno_post_gn_noalloc(FfnoRows rows, const float* u, const uint16_t* w1, const uint16_t* w2, float* y,
+                         const float* depth_w, const float* gamma, const float* beta, float* stats, float* aff_a,
+                         float* aff_b, hipStream_t stream) {
+  if (!rows_ok(rows) || !u || !w1 || !w2 || !y || !depth_w || !gamma || !beta || !aff_a || !aff_b)
+    return fail("ffno_post_gn_noalloc: bad arguments");
+  if ((int64_t)rows.B * rows.nwg > FFNO_NOALLOC_MAX_SLOTS) {
+    snprintf(g_err, sizeof(g_err),
+             "ffno_post_gn_noalloc: B * nwg = %lld partial-statistics slots exceed the static scratch's %d",
+             (long long)rows.B * rows.nwg, FFNO_NOALLOC_MAX_SLOTS);
+    return -1;
+  }
+  float* part = nullptr;
+  if (scratch_of_current_device(&part)) return -1;
+  // every workgroup of the post kernel writes all G slots of its own [b][wg] entry before the finalize reads them:
+  // nothing in the scratch has to start at zero
+  const int rc = ffno_post(rows, u, w1, w2, y, FfnoStatsOut{depth_w, part}, stream);
+  if (rc) return rc;
+  return ffno_gn_finalize(rows.B, rows.nwg, rows.C, part, gamma, beta, stats, aff_a, aff_b, stream);
 }
 
 int ffno_fuse(FfnoRows rows, const float* x, const float* sp, const float* vp, const float* s_a, const float* s_b,

@@ -74,6 +74,18 @@ int ffno_block_in(FfnoRows rows, FfnoAssemble in, float* x_out, const uint16_t* 
 int ffno_post(FfnoRows rows, const float* u, const uint16_t* w1, const uint16_t* w2, float* y, FfnoStatsOut stats,
               hipStream_t stream);
 
+/* ffno_post followed by ffno_gn_finalize of its output, allocating nothing: the partial statistics live in a static
+ * device array of the library (FFNO_NOALLOC_MAX_SLOTS = 4096 slots of [8][3] floats, 384 KiB per GPU, held until
+ * the process ends) instead of a caller's workspace. It has no counterpart in the reference; it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Writes y, stats [B][8][3] and the affine aff_a, aff_b [B][C] exactly as ffno_post + ffno_gn_finalize
+ * do. Fails when rows.B * rows.nwg exceeds FFNO_NOALLOC_MAX_SLOTS. All calls on one GPU share its array: calls that
+ * overlap on one GPU (concurrent streams or threads) race and must not be issued. */
+#define FFNO_NOALLOC_MAX_SLOTS 4096
+int ffno_post_gn_noalloc(FfnoRows rows, const float* u, const uint16_t* w1, const uint16_t* w2, float* y,
+                         const float* depth_w, const float* gamma, const float* beta, float* stats, float* aff_a,
+                         float* aff_b, hipStream_t stream);
+
 /* Branch fusion (models/ffno_model.py:524-534, :541-543): s = gelu(GN(sp)), v = gelu(GN(vp)),
  * gates = sigmoid(BG2 gelu(BG1 [x,s,v]) + b), fp = F2 gelu(F1 [s*gs, v*gv]) + 0.5 (s*gs + v*gv), statistics of fp.
  * fp may alias sp. */

@@ -139,6 +139,20 @@ class Checker:
         self.act(n, "y", y, rows)
         self.stats(n, st, rows)
 
+    def ffno_post_gn_noalloc(self, n, rows, u, w1, w2, y, depth_w, gamma, beta, stats, aff_a, aff_b, stream):
+        C = rows.C
+        self.act(n, "u", u, rows)
+        self.weight(n, "w1", w1, 2 * C, C, parts=3)
+        self.weight(n, "w2", w2, C, 2 * C, parts=3)
+        self.act(n, "y", y, rows)
+        self.span(n, "depth_w", depth_w, rows.B * rows.voxels * F32)
+        self.span(n, "gamma", gamma, C * F32)
+        self.span(n, "beta", beta, C * F32)
+        self.span(n, "stats", stats, rows.B * 8 * 3 * F32, optional=True)
+        self.span(n, "aff_a", aff_a, rows.B * C * F32)
+        self.span(n, "aff_b", aff_b, rows.B * C * F32)
+        assert rows.B * rows.nwg <= 4096, "ffno_post_gn_noalloc: more partial slots than its static scratch holds"
+
     def ffno_fuse(self, n, rows, x, sp, vp, sa, sb, va, vb, bg1, bg2, bias, f1, f2, fp, st, gates, stream):
         C = rows.C
         for what, p in (("x", x), ("sp", sp), ("vp", vp), ("fp", fp)):
@@ -493,6 +507,14 @@ def check_shape(model, B, D, H, W, stride=1, want_z=True, column=None, two_pass=
         head = torch.empty(B, model.out_channels + len(model.species_levels), D, H, W, device=meta)
         eng.head_output(head, x)
         eng.head_output_bwd(head, x, ybar)
+        # the allocation-free post + GroupNorm entry point, on the first block's post-network weights
+        rows, C = plan.rows, plan.C
+        blk = eng.weights.blocks[0]
+        u, out = torch.empty(B * plan.voxels, C, device=meta), torch.empty(B * plan.voxels, C, device=meta)
+        gn = blk["norm_spec"]
+        E._lib.call("ffno_post_gn_noalloc", rows, E._p(u), E._p(blk["spec.p1"]), E._p(blk["spec.p2"]), E._p(out),
+                    E._p(plan.depth_w), E._p(gn[0]), E._p(gn[1]), E._p(torch.empty(B, 8, 3, device=meta)),
+                    E._p(torch.empty(B, C, device=meta)), E._p(torch.empty(B, C, device=meta)), E.Engine._stream())
     finally:
         restore()
     return len(chk.calls)
