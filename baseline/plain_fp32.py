@@ -1,7 +1,7 @@
 """The plain FP32 GPU path: FFNO3D evaluated with stock PyTorch ops as the reference evaluates it (1x1 Conv3d,
 torch.fft.rfft2/irfft2, einsum, autograd through per-block non-reentrant checkpoints; models/ffno_model.py:261-309,
-:431-450, :524-553, :730-832). It is the comparison baseline of bench.py ("plain FP32 GPU path") and is never used by
-ffnoml_amd.FFNO3D, whose forward and adjoint are the HIP kernels only.
+:431-450, :524-553, :730-832). It is the comparison baseline of bench.py ("plain FP32 GPU path"), kept outside the product package: nothing in
+ffnoml_amd imports it (ffnoml_amd.FFNO3D's forward and adjoint are the HIP kernels only).
 
 Runs on a state_dict with the reference's keys, channel-first tensors, shared column grids.
 """

@@ -5,7 +5,7 @@ patches, D=128, H=W=64, width 64, vertical modes 32 (z_min_wavelength 1.28 at z_
 hidden 128, species H + Ca II -- on a synthetic stratified atmosphere (seed 0) with seeded weights (seed 0). Inputs are
 resident in HBM before timing.
 
-Beside it, in the same run: the plain FP32 GPU path (stock torch ops, ffnoml_amd.reference_torch) on the same inputs
+Beside it, in the same run: the plain FP32 GPU path (stock torch ops, baseline/plain_fp32.py) on the same inputs
 and weights (its speed and the HIP path's relative error in log10 n against it), and the CPU oracle in float32 on a
 bounded sample (the CPU workload: B=1, D=64, 32 x 32, width 32, 4 layers) timed on this host's cores.
 
@@ -88,13 +88,30 @@ def launch_bytes(name, a):
     return table[name]()
 
 
+def kernel_of(name, a):
+    """The kernel (template instance, as rocprof names it, with * for the column-group count) an entry point call
+    launches, for the entry points that launch one of several instances."""
+    if name == "ffno_tw_gemm":
+        g = a[0]._obj
+        three = "true" if g.a_l3 else "false"
+        if g.m_pad <= 128 and g.k_pad <= 128:
+            return f"k_tw_small<{g.m_pad // 16}, {three}, *, {'true' if g.twiddle else 'false'}>"
+        return f"k_tw_gemm<{three}>"
+    if name == "ffno_mix":
+        m = a[0]._obj
+        return f"k_mix<{m.C // 16}, *, {'true' if m.parts == 3 else 'false'}, *>"
+    return "k_" + name[5:]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-baselines", action="store_true", help="skip the plain FP32 GPU and CPU legs")
-    ap.add_argument("--volume", action="store_true", help="also run the whole-volume 504 x 504 x 256 forward + VJP")
+    ap.add_argument("--no-volume", action="store_true", help="skip the whole-volume 504 x 504 x 256 forward + VJP")
+    ap.add_argument("--no-production", action="store_true",
+                    help="skip the production-width leg (width 96, 1001 vertical modes, on the benchmark tile)")
     ap.add_argument("--service-requests", type=int, default=160,
                     help="requests of the serving workload (Poisson arrivals at 40/s, seed 1; 0 skips it; the full "
                          "workload is 2000)")
@@ -104,7 +121,7 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB device to device three times (the known-size reference for HBM counters)")
     ap.add_argument("--no-traffic", action="store_true",
-                    help="skip the two rocprofv3 --pmc child runs (FETCH_SIZE, WRITE_SIZE) that measure HBM traffic")
+                    help="skip the rocprofv3 --pmc child runs (FETCH_SIZE, WRITE_SIZE, MFMA and VALU instructions)")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -119,6 +136,9 @@ def main():
 
     import torch
     from ffnoml_amd import _lib
+
+    def log(msg):
+        print(f"bench: {msg}", file=sys.stderr, flush=True)
 
     dev = torch.device("cuda:0")
     copy_cal = copy_calibration(dev) if args.copy_calibration else None
@@ -145,9 +165,10 @@ def main():
     wall = time.perf_counter() - t0
     ms = start.elapsed_time(end) / args.steps
     value = voxels / (ms * 1e-3) / 1e6
+    log(f"fwd+VJP {ms:.1f} ms per step, {value:.2f} Mvox/s")
 
     # per-launch timing: HIP events around every launch of one more step, recorded on the stream the launches go
-    # to (torch's current stream, which the engine passes to every entry point); grouped by entry point, the one
+    # to (torch's current stream, which the engine passes to every entry point); grouped by kernel instance, the one
     # with the largest total time is the dominant kernel of the roofline
     events = []
     real_call = _lib.call
@@ -155,10 +176,11 @@ def main():
     def timed_call(n, *a):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         nbytes = launch_bytes(n, a)
+        kern = kernel_of(n, a)
         e0.record()
         real_call(n, *a)
         e1.record()
-        events.append((n, nbytes, e0, e1))
+        events.append((n, kern, nbytes, e0, e1))
 
     _lib.call = timed_call
     try:
@@ -166,15 +188,27 @@ def main():
         torch.cuda.synchronize()
     finally:
         _lib.call = real_call
-    per = {}
-    for n, nbytes, e0, e1 in events:
+    per, per_kernel = {}, {}
+    for n, kern, nbytes, e0, e1 in events:
+        dt = e0.elapsed_time(e1)
         t, b, c = per.get(n, (0.0, 0, 0))
-        per[n] = (t + e0.elapsed_time(e1), b + nbytes, c + 1)
-    name = max(per, key=lambda k: per[k][0])
-    ktot, kbytes, klaunches = per[name]
+        per[n] = (t + dt, b + nbytes, c + 1)
+        t, b, c, _ = per_kernel.get(kern, (0.0, 0, 0, n))
+        per_kernel[kern] = (t + dt, b + nbytes, c + 1, n)
+    timed_total = sum(v[0] for v in per.values())
+    kern = max(per_kernel, key=lambda k: per_kernel[k][0])
+    ktot, kbytes, klaunches, entry = per_kernel[kern]
     kms = ktot / klaunches
     achieved = kbytes / (ktot * 1e-3) / 1e9
-    timed_total = sum(v[0] for v in per.values())
+    ent_t, ent_b, ent_c = per[max(per, key=lambda k: per[k][0])]
+
+    def kernel_row(k):
+        t, b, c, n = per_kernel[k]
+        row = {"entry_point": n, "launches": c, "ms_per_step": round(t, 2), "avg_us": round(1e3 * t / c, 1),
+               "algorithmic_gb_per_launch": round(b / c / 1e9, 4), "hbm_frac": round(b / (t * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                                                                    4)}
+        row.update(instructions_of(counters, k, c, t))
+        return row
 
     out = {
         "metric": "FFNO3D fwd+VJP Mvoxels/s vs plain FP32 (rel err<=1e-3); service p50/p99 ms",
@@ -192,16 +226,42 @@ def main():
         "data": "synthetic (stratified random-field atmosphere, seed 0; seeded random weights)",
         "config": {"workload": "FFNO3D forward + input VJP, B=8 D=128 H=W=64 width=64 modes(D)=32 layers=8",
                    "global_batch": w["B"], "voxels_per_step": voxels, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": name, "launches": klaunches, "avg_ms": round(kms, 4),
-                     "share_of_step": round(ktot / timed_total, 3), "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "algorithmic_bytes_per_launch": int(kbytes / klaunches),
-                     "traffic": traffic_of(counters, name)},
+        "roofline": {"bound": "hbm", "kernel": kern, "entry_point": entry, "launches": klaunches,
+                     "avg_ms": round(kms, 4), "share_of_step": round(ktot / timed_total, 3),
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "algorithmic_bytes_per_launch": int(kbytes / klaunches),
+                     "traffic": traffic_of(counters, kern),
+                     **{k: v for k, v in instructions_of(counters, kern, klaunches, ktot).items()}},
         "entry_points_ms_per_step": {k: round(v[0], 2) for k, v in sorted(per.items(), key=lambda kv: -kv[1][0])},
+        "kernels": {k: kernel_row(k) for k in sorted(per_kernel, key=lambda k: -per_kernel[k][0])[:12]},
     }
+    if ent_t > ktot:
+        out["roofline"]["largest_entry_point"] = {
+            "name": max(per, key=lambda k: per[k][0]), "launches": ent_c, "ms_per_step": round(ent_t, 2),
+            "achieved_gb_s": round(ent_b / (ent_t * 1e-3) / 1e9, 1),
+            "note": "aggregates several kernel instances; the roofline above is the largest single kernel"}
+
+    # the forward alone at the same shape
+    with torch.no_grad():
+        for _ in range(2):
+            yf = model(x, z, 48e3, 48e3)
+        torch.cuda.synchronize()
+        f0, f1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        nf = max(args.steps, 5)
+        f0.record()
+        for _ in range(nf):
+            yf = model(x, z, 48e3, 48e3)
+        f1.record()
+        torch.cuda.synchronize()
+    fms = f0.elapsed_time(f1) / nf
+    out["forward_only"] = {"workload": "FFNO3D forward, B=8 D=128 H=W=64 width=64 modes(D)=32 layers=8",
+                           "ms_per_forward": round(fms, 3), "value": round(voxels / (fms * 1e-3) / 1e6, 3),
+                           "unit": "Mvoxels/s", "forwards": nf}
+    del yf
+    log(f"forward {fms:.1f} ms, {out['forward_only']['value']:.2f} Mvox/s")
 
     if not args.no_baselines:
-        from ffnoml_amd.reference_torch import PlainFFNO3D
+        from baseline.plain_fp32 import PlainFFNO3D
         plain = PlainFFNO3D(model).to(dev)
 
         def plain_step():
@@ -210,30 +270,87 @@ def main():
             (xbp,) = torch.autograd.grad(yp, xg, grad_outputs=ct)
             return yp, xbp
 
-        yp, xbp = plain_step()
-        torch.cuda.synchronize()
-        s0 = time.perf_counter()
-        yp, xbp = plain_step()
-        torch.cuda.synchronize()
-        pms = (time.perf_counter() - s0) * 1e3
+        def median_ms(fn, n=3):
+            fn()
+            torch.cuda.synchronize()
+            ts = []
+            for _ in range(n):
+                s0 = time.perf_counter()
+                r = fn()
+                torch.cuda.synchronize()
+                ts.append((time.perf_counter() - s0) * 1e3)
+            return sorted(ts)[len(ts) // 2], r
+
+        pms, (yp, xbp) = median_ms(plain_step)
         rel = float(((y.detach() - yp.detach()).abs() / yp.detach().abs().clamp_min(1.0)).max())
         grel = float((xb - xbp).norm() / xbp.norm())
         y0_plain = yp[:1].detach().double().cpu()
         del yp, xbp
         out["plain_fp32_gpu"] = {"value": round(voxels / (pms * 1e-3) / 1e6, 3), "unit": "Mvoxels/s",
                                  "ms_per_step": round(pms, 1), "speedup": round(pms / ms, 2),
-                                 "rel_err_log10n_vs_hip": rel, "grad_rel_err_vs_hip": grel}
+                                 "rel_err_log10n_vs_hip": rel, "grad_rel_err_vs_hip": grel, "timing": "median of 3"}
+        with torch.no_grad():
+            pfms, _ = median_ms(lambda: plain(x, z, 48e3, 48e3))
+        out["forward_only"]["plain_fp32_ms"] = round(pfms, 1)
+        out["forward_only"]["speedup_vs_plain_fp32"] = round(pfms / fms, 2)
+        log(f"plain FP32: fwd+VJP {pms:.0f} ms, forward {pfms:.0f} ms")
+        del plain
         out["cpu_baseline"], out["accuracy_vs_fp64"] = cpu_leg(model, x[:1].cpu(), z[:1].cpu(),
                                                                y[:1].detach().double().cpu(), y0_plain)
+        out["forward_only"]["rel_err_log10n_vs_fp64"] = out["accuracy_vs_fp64"]["hip_rel_err_log10n"]
     if copy_cal:
         out["copy_calibration"] = copy_cal
+    del y, xb
+    if not args.no_production:
+        out["production_width"] = production_leg(dev, log)
     if args.service_requests > 0:
-        del y, xb
         out["service"] = service_leg(model, dev, args.service_requests,
                                      0 if args.no_baselines else args.plain_service_requests, args.service_slots)
-    if args.volume:
+    if not args.no_volume:
+        del x, z, ct
+        model.engine().clear_caches()
+        torch.cuda.empty_cache()
         out["whole_volume"] = volume_leg(model, dev)
     print(json.dumps(out), flush=True)
+
+
+def production_leg(dev, log):
+    """The production configuration's width and vertical resolution (width 96, 1001 vertical modes at
+    z_min_wavelength 0.04, more modes than the 128 levels) on the benchmark tile B=8, D=128, 64 x 64, 8 layers:
+    forward and forward + input VJP, median of 3 after one warm-up, and the float64 oracle's error on item 0 is not
+    recomputed here (tests/test_gpu_production.py checks this configuration's parity)."""
+    import torch
+    w = dict(WORKLOAD, width=96, z_min_wavelength=0.04)
+    model, x, z, ct = make(w, dev)
+    voxels = w["B"] * w["D"] * w["H"] * w["W"]
+
+    def fwd():
+        with torch.no_grad():
+            return model(x, z, 48e3, 48e3)
+
+    def fwd_vjp():
+        xg = x.detach().requires_grad_(True)
+        return torch.autograd.grad(model(xg, z, 48e3, 48e3), xg, grad_outputs=ct)[0]
+
+    res = {"workload": "FFNO3D B=8 D=128 H=W=64, width 96, 8 layers, 1001 vertical modes (z_min_wavelength 0.04)"}
+    for name, fn in (("forward", fwd), ("fwd_vjp", fwd_vjp)):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(3):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        t = sorted(ts)[1]
+        res[name + "_ms"] = round(t, 1)
+        res[name + "_mvox_s"] = round(voxels / (t * 1e-3) / 1e6, 3)
+        log(f"production width: {name} {t:.0f} ms")
+    del model, x, z, ct
+    torch.cuda.empty_cache()
+    return res
 
 
 def pmc_child():
@@ -250,9 +367,16 @@ def pmc_child():
     torch.cuda.synchronize()
 
 
+MFMA_FLOP = 2 * 16 * 16 * 32          # one v_mfma_f32_16x16x32_bf16 (the only MFMA the library issues)
+BF16_PEAK_TFLOPS = 2500.0               # MI355X dense bf16 MFMA
+PMC_PASSES = (("FETCH_SIZE",), ("WRITE_SIZE",), ("SQ_INSTS_MFMA", "SQ_INSTS_VALU"))
+
+
 def measure_counters(timeout=240):
-    """Two child runs of this script (--pmc-child) under rocprofv3, one per counter: FETCH_SIZE, then WRITE_SIZE
-    (KiB per dispatch). Returns {counter: {kernel: [values]}} or None when rocprofv3 is missing or a run fails."""
+    """Child runs of this script (--pmc-child) under rocprofv3, one pass per counter set: FETCH_SIZE, then WRITE_SIZE
+    (KiB per dispatch), then the MFMA and VALU instruction counts. Returns {counter: {kernel: [values per
+    dispatch]}} (kernel = its name with template arguments, as rocprof reports it) or None when rocprofv3 is missing
+    or a pass fails."""
     import csv
     import re
     import shutil
@@ -263,41 +387,46 @@ def measure_counters(timeout=240):
         return None
     out = {}
     with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
-        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
-            d = os.path.join(tmp, counter)
-            cmd = ["timeout", "-s", "KILL", str(timeout), exe, "--pmc", counter, "--output-format", "csv", "-d", d,
+        for counters in PMC_PASSES:
+            d = os.path.join(tmp, counters[0])
+            cmd = ["timeout", "-s", "KILL", str(timeout), exe, "--pmc", *counters, "--output-format", "csv", "-d", d,
                    "-o", "run", "--", sys.executable, "-u", os.path.abspath(__file__), "--pmc-child"]
-            print(f"bench: rocprofv3 --pmc {counter}", file=sys.stderr, flush=True)
+            print(f"bench: rocprofv3 --pmc {' '.join(counters)}", file=sys.stderr, flush=True)
             r = subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"), stdout=subprocess.DEVNULL,
                                stderr=subprocess.DEVNULL)
             path = os.path.join(d, "run_counter_collection.csv")
             if r.returncode != 0 or not os.path.exists(path):
-                print(f"bench: counter run {counter} failed ({r.returncode}); traffic not measured", file=sys.stderr)
+                print(f"bench: counter pass {counters} failed ({r.returncode}); counters not measured", file=sys.stderr)
                 return None
-            per = {}
             for row in csv.DictReader(open(path)):
-                if row["Counter_Name"] != counter:
+                if row["Counter_Name"] not in counters:
                     continue
-                k = re.sub(r"[<(].*", "", row["Kernel_Name"].replace("void ", "").replace(
+                k = re.sub(r"\(.*", "", row["Kernel_Name"].replace("void ", "").replace(
                     "ffno::(anonymous namespace)::", ""))
-                per.setdefault(k, []).append(float(row["Counter_Value"]))
-            out[counter] = per
+                out.setdefault(row["Counter_Name"], {}).setdefault(k, []).append(float(row["Counter_Value"]))
     return out
 
 
-def traffic_of(counters, entry):
-    """HBM bytes per launch of an entry point's kernel(s) (ffno_X -> k_X; ffno_tw_gemm -> k_tw_small / k_tw_gemm):
-    the average FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by bytes moved / counter value of the 1 GiB
-    device copy (__amd_rocclr_copyBuffer) in the same run."""
-    if not counters:
+def _matching(per, kern):
+    """Values of the rocprof kernels a kernel_of() name stands for ('*' matches one template argument; a name without
+    template arguments matches every instance)."""
+    import fnmatch
+    if "<" not in kern:
+        return [v for k, vals in per.items() if k == kern or k.startswith(kern + "<") for v in vals]
+    return [v for k, vals in per.items() if fnmatch.fnmatchcase(k, kern) for v in vals]
+
+
+def traffic_of(counters, kern):
+    """HBM bytes per launch of a kernel: the average FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by bytes
+    moved / counter value of the 1 GiB device copy (__amd_rocclr_copyBuffer) in the same run."""
+    if not counters or "FETCH_SIZE" not in counters:
         return None
     gib_kib = float(1 << 20)
     res = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         per = counters[counter]
         cal = [v for v in per.get("__amd_rocclr_copyBuffer", []) if v > 0.25 * gib_kib]
-        names = [k for k in per if k == "k_" + entry[5:] or (entry == "ffno_tw_gemm" and k.startswith("k_tw"))]
-        vals = [v for k in names for v in per[k]]
+        vals = _matching(per, kern)
         if not cal or not vals:
             return None
         scale = gib_kib / (sum(cal) / len(cal))
@@ -306,6 +435,21 @@ def traffic_of(counters, entry):
     return {"bytes_per_launch": int((f * fs + w * ws) * 1024), "fetch_kib": round(f, 1), "write_kib": round(w, 1),
             "fetch_scale": round(fs, 4), "write_scale": round(ws, 4), "dispatches": n,
             "method": "rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, one fwd+VJP step; scaled by the 1 GiB copy"}
+
+
+def instructions_of(counters, kern, launches, total_ms):
+    """Executed MFMA and VALU wave-instructions per launch of a kernel (rocprofv3 --pmc SQ_INSTS_MFMA SQ_INSTS_VALU,
+    one fwd+VJP step) and the MFMA utilisation they give at the launch duration measured here: executed MFMA FLOP
+    (2 x 16 x 16 x 32 per v_mfma_f32_16x16x32_bf16) / duration / the 2.5 PFLOP/s dense bf16 peak."""
+    if not counters or "SQ_INSTS_MFMA" not in counters:
+        return {}
+    mf, va = _matching(counters["SQ_INSTS_MFMA"], kern), _matching(counters["SQ_INSTS_VALU"], kern)
+    if not mf or not va:
+        return {}
+    m, v = sum(mf) / len(mf), sum(va) / len(va)
+    sec = total_ms * 1e-3 / launches
+    return {"mfma_insts_per_launch": int(m), "valu_insts_per_launch": int(v),
+            "mfma_frac": round(m * MFMA_FLOP / sec / (BF16_PEAK_TFLOPS * 1e12), 4)}
 
 
 SERVICE = dict(rate=40.0, seed=1, hw=(32, 64, 128, 256), depths=(96, 128, 192), vjp_share=0.3)
@@ -409,7 +553,9 @@ def service_leg(model, dev, n, n_plain, slots=3):
                "p99_ms": round(float(np.percentile(lat, 99)), 1), "mean_ms": round(float(lat.mean()), 1),
                "throughput_mvox_s": round(vox / wall / 1e6, 3), "wall_s": round(wall, 2),
                "offered_mvox_s": round(vox / trace[-1]["t"] / 1e6, 3), "peak_batches_in_flight": batcher.peak_inflight,
-               "graphs": len(batcher.graphs), "slots": batcher.slots, "max_batch": batcher.max_batch}
+               "graphs": len(batcher.graphs), "graph_captures": batcher.captures,
+               "graph_replays": batcher.replays, "slots": batcher.slots, "max_batch": batcher.max_batch,
+               "queue": queue_summary(batcher.queue_samples, w0)}
         log(f"p50 {res['p50_ms']} ms, p99 {res['p99_ms']} ms, {res['throughput_mvox_s']} Mvox/s")
     finally:
         batcher.close()
@@ -421,13 +567,27 @@ def service_leg(model, dev, n, n_plain, slots=3):
     return res
 
 
+def queue_summary(samples, t0, points=20):
+    """Requests waiting at each admission: mean, max and a coarse series over the serving window."""
+    import numpy as np
+    if not samples:
+        return None
+    t = np.array([s[0] - t0 for s in samples])
+    q = np.array([s[1] for s in samples])
+    edges = np.linspace(0.0, max(float(t.max()), 1e-9), points + 1)
+    series = [int(q[(t >= a) & (t <= b)].max()) if np.any((t >= a) & (t <= b)) else None
+              for a, b in zip(edges[:-1], edges[1:])]
+    return {"mean_waiting": round(float(q.mean()), 1), "max_waiting": int(q.max()),
+            "max_waiting_per_interval": series, "interval_s": round(float(edges[1] - edges[0]), 2)}
+
+
 def plain_service(model, dev, trace, inputs, backend, budget_s=90.0):
     """One request at a time, in arrival order, through the plain FP32 GPU path: predict = SI chain + forward under
     no_grad; VJP = forward + torch.autograd.grad of <cotangent, populations> (ffno_fsdp_runtime.py:294-348). Stops
     after budget_s seconds (a bounded sample: the requests answered by then)."""
     import numpy as np
     import torch
-    from ffnoml_amd.reference_torch import PlainFFNO3D
+    from baseline.plain_fp32 import PlainFFNO3D
     plain = PlainFFNO3D(model).to(dev)
     lat, vox = [], 0
     start = time.perf_counter()

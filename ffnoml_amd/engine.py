@@ -284,8 +284,9 @@ class ModeTables:
             self._fwd[0] = first
 
     def cached_bytes(self):
-        return sum(t.numel() * t.element_size() for d in (self._fwd, self._adj) for e in d.values()
-                   for t in e.values() if t is not None)
+        with self._lock:            # worker threads insert into the stores under this lock
+            return sum(t.numel() * t.element_size() for d in (self._fwd, self._adj) for e in d.values()
+                       for t in e.values() if t is not None)
 
     def _build(self, i, adjoint):
         prefix = f"blocks.{i}.{self.branch}."
@@ -417,6 +418,22 @@ class Plan:
         self.rows = FfnoRows(B, self.voxels, C, self.nwg)
         self.device = dev
         _settle(dev)                            # the plan's matrices are read from every worker's stream
+
+    def device_bytes(self):
+        """Device memory the plan's own matrices and tables hold (per-mode table sets not included)."""
+        seen, total = set(), 0
+        stack = [v for k, v in vars(self).items() if k != "_mode_tables"]
+        while stack:
+            v = stack.pop()
+            if torch.is_tensor(v):
+                if v.device.type != "cpu" and id(v) not in seen:
+                    seen.add(id(v))
+                    total += v.numel() * v.element_size()
+            elif isinstance(v, dict):
+                stack.extend(v.values())
+            elif isinstance(v, (list, tuple)):
+                stack.extend(v)
+        return total
 
     def tables(self, i, adjoint=False):
         """{"spec": ..., "vertical": ...} per-mode tables of block i (the adjoint's with adjoint=True)."""
@@ -553,15 +570,23 @@ class Engine:
         self.last_stride = None
         self._modes = collections.OrderedDict()     # per-mode table sets shared by plans, least recently used first
 
-    def footprint_bytes(self, B, D, H, W, vjp=False, jvp=False):
+    def vertical_modes(self, span=0.0):
+        """Vertical modes of a request whose tallest column spans `span` (Mm): the reference's box rule
+        (models/ffno_model.py:248-259) gives more modes when twice the span exceeds the reference length."""
+        m = self.module
+        return len(T.vertical_box(float(span), m.z_reference_length, m.z_min_wavelength,
+                                  int(math.floor(m.z_reference_length / m.z_min_wavelength)) + 1)[1])
+
+    def footprint_bytes(self, B, D, H, W, vjp=False, jvp=False, span=0.0):
         """HBM one request of this shape needs (for admission; the tables of a cached plan not included):
         forward 3 activation-sized buffers, adjoint (L+1) saved inputs and 9 working buffers, JVP 16 (one block's
-        intermediates, its gates and four tangents), plus the scratch."""
+        intermediates, its gates and four tangents), plus the scratch; span (Mm): the tallest column's height range,
+        which sets the vertical mode count."""
         C, L, r = self.module.width, self.module.n_layers, self.module.freq_mix_rank
         unit = B * D * H * W * C * 4
         plane, plane2 = horizontal_planes(H, W, C)
         chunk = max(1, min(B * D, H_CHUNK_BYTES // (4 * plane2))) * (plane + plane2) * 4
-        modes = int(math.floor(self.module.z_reference_length / self.module.z_min_wavelength)) + 1
+        modes = self.vertical_modes(span)
         v1 = B * 2 * modes * H * W * C * 4
         units = (L + 1) + WORKING_UNITS + GATE_UNITS if vjp else (16 if jvp else 3)
         return unit * units + chunk + (2 if vjp else 1) * v1 + B * D * H * W * 4 * 32 + (64 << 20)
@@ -612,9 +637,32 @@ class Engine:
             return t
 
     def cached_bytes(self):
-        """Device memory held by cached per-mode tables (the serving admission counts it)."""
+        """Device memory held by the engine's caches (the serving admission counts it): the cached plans' matrices
+        and tables, and every per-mode table set alive in the engine's LRU or in a cached plan."""
         with self._lock:
-            return sum(v.cached_bytes() for v in self._modes.values())
+            plans = list(self._plans.values())
+            sets = {id(v): v for v in self._modes.values()}
+        for p in plans:
+            for t in p._mode_tables.values():
+                sets.setdefault(id(t), t)
+        return sum(p.device_bytes() for p in plans) + sum(v.cached_bytes() for v in sets.values())
+
+    def clear_caches(self):
+        """Drop every cached plan and per-mode table set (the memory returns to torch's caching allocator once no
+        running call holds them)."""
+        with self._lock:
+            self._plans.clear()
+            self._modes.clear()
+
+    def drop_unused_tables(self):
+        """Drop the per-mode table sets no cached plan uses, and the plans beyond the most recent few: what the
+        serving admission frees when a request waits for memory with nothing in flight."""
+        with self._lock:
+            while len(self._plans) > 4:
+                self._plans.pop(next(iter(self._plans)))
+            used = {id(t) for p in self._plans.values() for t in p._mode_tables.values()}
+            for key in [k for k, v in self._modes.items() if id(v) not in used]:
+                del self._modes[key]
 
     def plan(self, B, D, H, W, z, dx, dy):
         """The plan for these heights, cached by content: the reference's shared-grid decision

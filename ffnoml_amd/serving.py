@@ -9,14 +9,17 @@ statistics are per sample, so a request batched with others gets what it would g
 
 Forward-only batches of small tiles (forward footprint at most `graph_bytes`), once their bucket has been seen,
 replay a captured HIP graph of the whole forward (static input/output buffers per (bucket, batch size)); there the
-launch overhead of the ~15 launches per layer is a visible share of the batch time. The memory a graph holds is
-counted against the HBM budget. A capture takes the device alone: batches run under shared holds of a gate, and a
-capture happens only when the gate is idle (no batch in flight), so no other stream's host-synchronising work can
-invalidate it and no batch ever waits for one; a busy device defers the capture to a later batch of the bucket.
-Larger tiles and VJP batches run eagerly.
+launch overhead of the ~15 launches per layer is a visible share of the batch time. The worker that ran a bucket's
+first batch captures it right after answering, on its own stream, while other workers keep running: the capture is
+thread-local (only the capturing thread may not synchronise, and it does not: the plan and the per-mode tables were
+built by the eager batch), so it needs no idle device. The memory a graph holds is counted against the HBM budget;
+graphs are evicted least recently replayed first, and an evicted graph's memory is released from the count only
+when its last replay has finished. Larger tiles and VJP batches run eagerly.
+
+Admission never stalls: when the oldest request does not fit the unclaimed budget while nothing is in flight, the
+graphs and the engines' unused tables are dropped; a request that still does not fit is failed with MemoryError.
 """
 import collections
-import contextlib
 import hashlib
 import queue
 import threading
@@ -43,64 +46,11 @@ class Request:
         self.key = (model, op, z.shape, self.dx, self.dy, zh)
 
 
-class DeviceGate:
-    """Shared holds for batches, exclusive holds for graph captures; a waiting capture blocks new shared holds."""
-
-    def __init__(self):
-        self.cv = threading.Condition()
-        self.readers = 0
-        self.writer = False
-        self.waiting = 0
-
-    @contextlib.contextmanager
-    def shared(self):
-        with self.cv:
-            while self.writer or self.waiting:
-                self.cv.wait()
-            self.readers += 1
-        try:
-            yield
-        finally:
-            with self.cv:
-                self.readers -= 1
-                self.cv.notify_all()
-
-    @contextlib.contextmanager
-    def try_exclusive(self):
-        """An exclusive hold only if the device is idle right now (yields False, holding nothing, otherwise)."""
-        with self.cv:
-            ok = not self.writer and not self.readers and not self.waiting
-            if ok:
-                self.writer = True
-        try:
-            yield ok
-        finally:
-            if ok:
-                with self.cv:
-                    self.writer = False
-                    self.cv.notify_all()
-
-    @contextlib.contextmanager
-    def exclusive(self):
-        with self.cv:
-            self.waiting += 1
-            while self.writer or self.readers:
-                self.cv.wait()
-            self.waiting -= 1
-            self.writer = True
-        try:
-            yield
-        finally:
-            with self.cv:
-                self.writer = False
-                self.cv.notify_all()
-
-
 class ContinuousBatcher:
     """submit() -> Future of the backend's result ({"populations"} or {"populations", "features", "z_scale"})."""
 
     def __init__(self, backends, slots=3, max_batch=8, memory_fraction=0.85, graphs=True, max_graphs=32,
-                 graph_bytes=1 << 30):
+                 graph_bytes=1 << 30, budget=None):
         self.backends = dict(backends)
         for backend in self.backends.values():
             # admission budgets the adjoint at checkpoint stride 1; keeping every intermediate would outgrow it
@@ -111,11 +61,12 @@ class ContinuousBatcher:
         self.max_graphs = int(max_graphs)
         self.graph_bytes = int(graph_bytes)
         self.graph_held = 0                             # bytes held by captured graphs' memory pools
-        self.gate = DeviceGate()
         dev = next(iter(self.backends.values())).device
         self.device = dev
-        free, total = torch.cuda.mem_get_info(dev)
-        self.budget = int(free * memory_fraction)
+        if budget is None:
+            free, _ = torch.cuda.mem_get_info(dev)
+            budget = int(free * memory_fraction)
+        self.budget = int(budget)
         self.claimed = 0
         self.inflight = 0
         self.pending = collections.deque()
@@ -127,6 +78,9 @@ class ContinuousBatcher:
         self.graph_lock = threading.Lock()
         self.completed = []
         self.peak_inflight = 0
+        self.queue_samples = []                          # (time, requests waiting, batches in flight) per admission
+        self.captures = 0
+        self.replays = 0
         self.workers = [threading.Thread(target=self._worker, daemon=True, name=f"ffno-worker-{i}")
                         for i in range(self.slots)]
         for w in self.workers:
@@ -160,7 +114,7 @@ class ContinuousBatcher:
             if not np.isfinite(cotangent).all():
                 raise ValueError("FFNO JVP tangent contains NaN or Inf")
         req = Request(op, model, features, z, dx, dy, cotangent)
-        need = backend.model.engine().footprint_bytes(1, *z.shape, vjp=op == "VJP", jvp=op == "JVP")
+        need = self._footprint([req])
         if need > self.budget:
             req.future.set_exception(MemoryError(f"request needs {need / 2**30:.1f} GiB, more than the "
                                                  f"{self.budget / 2**30:.1f} GiB budget"))
@@ -187,37 +141,79 @@ class ContinuousBatcher:
     def _footprint(self, reqs):
         eng = self.backends[reqs[0].model].model.engine()
         nz, nx, ny = reqs[0].z.shape
-        return eng.footprint_bytes(len(reqs), nz, nx, ny, vjp=reqs[0].op == "VJP", jvp=reqs[0].op == "JVP")
+        span = float(np.max(reqs[0].z.max(axis=0) - reqs[0].z.min(axis=0))) / 1e6       # metres -> Mm
+        return eng.footprint_bytes(len(reqs), nz, nx, ny, vjp=reqs[0].op == "VJP", jvp=reqs[0].op == "JVP",
+                                   span=span)
+
+    def _free(self):
+        """The budget not held by captured graphs or by the engines' cached plans and tables."""
+        return self.budget - self.graph_held - sum(b.model.engine().cached_bytes() for b in self.backends.values())
+
+    def _release_caches(self):
+        """With nothing in flight: evict every captured graph and the engines' unused plans and tables."""
+        with self.graph_lock:
+            while self.graphs:
+                self._evict_oldest()
+        for b in self.backends.values():
+            b.model.engine().drop_unused_tables()
 
     def _schedule(self):
         while True:
-            with self.cv:
-                while not self.stop and (not self.pending or self.inflight >= self.slots):
-                    self.cv.wait()
-                if self.stop and not self.pending:
+            try:
+                if self._schedule_once():
                     return
-                # the oldest waiting request picks the bucket; same-bucket requests join it, oldest first
-                head = self.pending[0]
-                batch = [r for r in self.pending if r.key == head.key][: self.max_batch]
-                # per-mode tables the engines cache are device memory outside any batch's footprint
-                free = self.budget - self.graph_held - sum(b.model.engine().cached_bytes()
-                                                           for b in self.backends.values())
-                while len(batch) > 1 and self.claimed + self._footprint(batch) > free:
-                    batch.pop()
-                need = self._footprint(batch)
-                if self.claimed + need > free:
-                    self.cv.wait(timeout=0.05)          # wait for memory to be released by a finishing batch
-                    continue
-                for r in batch:
-                    self.pending.remove(r)
-                self.claimed += need
-                self.inflight += 1
-                self.peak_inflight = max(self.peak_inflight, self.inflight)
-            self.work.put((batch, need))
+            except Exception as e:                      # never let the scheduler die silently: fail what waits
+                import sys
+                import traceback
+                print(f"ffno scheduler: {e!r}\n{traceback.format_exc()}", file=sys.stderr, flush=True)
+                with self.cv:
+                    failed, self.pending = list(self.pending), collections.deque()
+                for r in failed:
+                    if not r.future.done():
+                        r.future.set_exception(e)
+
+    def _schedule_once(self):
+        """One scheduling decision; True when the batcher is closed and drained."""
+        with self.cv:
+            while not self.stop and (not self.pending or self.inflight >= self.slots):
+                self.cv.wait()
+            if self.stop and not self.pending:
+                return True
+            # the oldest waiting request picks the bucket; same-bucket requests join it, oldest first
+            head = self.pending[0]
+            batch = [r for r in self.pending if r.key == head.key][: self.max_batch]
+            free = self._free()
+            while len(batch) > 1 and self.claimed + self._footprint(batch) > free:
+                batch.pop()
+            need = self._footprint(batch)
+            if self.claimed + need > free:
+                if self.inflight == 0:
+                    # nothing will release memory by itself: drop the caches, and fail a request that still
+                    # does not fit instead of waiting for good
+                    self._release_caches()
+                    if need > self._free():
+                        self.pending.remove(head)
+                        head.t_done = time.perf_counter()
+                        self.completed.append(head)
+                        head.future.set_exception(MemoryError(
+                            f"request needs {need / 2**30:.1f} GiB; {max(self._free(), 0) / 2**30:.1f} GiB of the "
+                            f"{self.budget / 2**30:.1f} GiB budget are free with nothing in flight"))
+                    return False
+                self.cv.wait(timeout=0.05)              # wait for memory to be released by a finishing batch
+                return False
+            for r in batch:
+                self.pending.remove(r)
+            self.claimed += need
+            self.inflight += 1
+            self.peak_inflight = max(self.peak_inflight, self.inflight)
+            self.queue_samples.append((time.perf_counter(), len(self.pending), self.inflight))
+        self.work.put((batch, need))
+        return False
 
     # ------------------------------------------------------------------------------------------------ workers
     def _worker(self):
-        stream = torch.cuda.Stream(device=self.device)
+        cuda = torch.device(self.device).type == "cuda"
+        stream = torch.cuda.Stream(device=self.device) if cuda else None
         while True:
             item = self.work.get()
             if item is None:
@@ -228,10 +224,12 @@ class ContinuousBatcher:
                 r.t_start = t0
             capture = None
             try:
-                with self.gate.shared():
+                if cuda:
                     with torch.cuda.stream(stream):
                         results, capture = self._run(batch)
                     stream.synchronize()
+                else:
+                    results, capture = self._run(batch)
                 done = time.perf_counter()
                 for r in batch:
                     r.t_done = done
@@ -299,26 +297,46 @@ class ContinuousBatcher:
         with self.graph_lock:
             g = self.graphs.get(key)
             uncapturable = key in self.no_graph
+            if g is not None:
+                self.graphs.move_to_end(key)            # least recently replayed graphs are evicted first
+                g.users += 1
         if g is None:
-            # first batch of this shape runs eagerly (building the plan and warming the allocator); the worker
-            # captures afterwards, alone on the device
+            # first batch of this shape runs eagerly (building the plan and its tables and warming the allocator);
+            # the worker captures afterwards
             n = backend.predict_tensor(x, z, r0.dx, r0.dy)
             return n, (None if uncapturable else (backend, key, x, z, r0.dx, r0.dy, held))
-        graph, sx, sz, sn, _, lock, _ = g
-        # one graph's static buffers serve one batch at a time: a replay on this worker's stream waits until the
-        # previous replay (on whichever stream) has copied its output out
-        with lock:
-            stream = torch.cuda.current_stream()
-            if g[6] is not None:
-                stream.wait_event(g[6])
-            sx.copy_(x)
-            sz.copy_(z)
-            graph.replay()
-            out = sn.clone()
-            done = torch.cuda.Event()
-            done.record(stream)
-            g[6] = done
+        try:
+            # one graph's static buffers serve one batch at a time: a replay on this worker's stream waits until the
+            # previous replay (on whichever stream) has copied its output out
+            with g.lock:
+                stream = torch.cuda.current_stream()
+                if g.done is not None:
+                    stream.wait_event(g.done)
+                g.sx.copy_(x)
+                g.sz.copy_(z)
+                g.graph.replay()
+                self.replays += 1
+                out = g.sn.clone()
+                g.done = torch.cuda.Event()
+                g.done.record(stream)
+        finally:
+            self._release(g)
         return out, None
+
+    def _release(self, g):
+        """A replay of g is over (enqueued): the last user of an evicted graph frees its count of held memory."""
+        with self.graph_lock:
+            g.users -= 1
+            if g.evicted and g.users == 0:
+                self.graph_held -= g.held
+
+    def _evict_oldest(self):
+        """Evict the least recently replayed graph (graph_lock held). Its held memory stays counted until its last
+        replay, running now or enqueued, has released it."""
+        _, old = self.graphs.popitem(last=False)
+        old.evicted = True
+        if old.users == 0:
+            self.graph_held -= old.held
 
     def _capture(self, stream, backend, key, x, z, dx, dy, held):
         with self.graph_lock:
@@ -326,14 +344,13 @@ class ContinuousBatcher:
                 return
             if self.claimed + self.graph_held + held > self.budget:
                 return                                  # no room to keep another graph's memory now
-        with self.gate.try_exclusive() as idle, torch.cuda.stream(stream):
-            if not idle:
-                return                                  # other batches are running: capture on a later batch
-            plan = backend.plan_for(z, dx, dy)          # the host synchronisation of planning happens here
+        with torch.cuda.stream(stream):
+            plan = backend.plan_for(z, dx, dy)          # cached by the eager batch: no host synchronisation here
             sx, sz = x.clone(), z.clone()
             stream.synchronize()
             graph = torch.cuda.CUDAGraph()
             try:
+                # thread-local capture: other workers keep launching on their own streams meanwhile
                 with torch.cuda.graph(graph, stream=stream, capture_error_mode="thread_local"):
                     sn = backend.predict_tensor(sx, sz, dx, dy, plan=plan)
             except Exception:
@@ -342,11 +359,22 @@ class ContinuousBatcher:
                 return
             stream.synchronize()
         with self.graph_lock:
-            self.graphs[key] = [graph, sx, sz, sn, held, threading.Lock(), None]
+            self.graphs[key] = _Graph(graph, sx, sz, sn, held)
             self.graph_held += held
+            self.captures += 1
             while len(self.graphs) > self.max_graphs:
-                _, old = self.graphs.popitem(last=False)
-                with old[5]:                            # its last replay has finished before its pool is freed
-                    if old[6] is not None:
-                        old[6].synchronize()
-                self.graph_held -= old[4]
+                self._evict_oldest()
+
+
+class _Graph:
+    """A captured forward of one (bucket, batch size): its static buffers, the memory it holds, the event of its
+    last replay, the replays using it now, and whether it has been evicted."""
+
+    __slots__ = ("graph", "sx", "sz", "sn", "held", "lock", "done", "users", "evicted")
+
+    def __init__(self, graph, sx, sz, sn, held):
+        self.graph, self.sx, self.sz, self.sn, self.held = graph, sx, sz, sn, held
+        self.lock = threading.Lock()
+        self.done = None
+        self.users = 0
+        self.evicted = False

@@ -190,12 +190,12 @@ def test_cpu_tensors_are_refused():
 
 
 def test_plain_torch_baseline_matches_oracle():
-    """The plain FP32 comparison path (ffnoml_amd.reference_torch) equals the float64 oracle on golden case A to
+    """The plain FP32 comparison path (baseline/plain_fp32.py) equals the float64 oracle on golden case A to
     float32 precision (1e-4 of range), so the bench's baseline computes the same network."""
     from golden_cases import load, oracle_config
     from oracle import ffno_oracle as O
     from ffnoml_amd.model import FFNO3D
-    from ffnoml_amd.reference_torch import PlainFFNO3D
+    from baseline.plain_fp32 import PlainFFNO3D
     data, config, state = load("case_a_shared64")
     model = FFNO3D(**config)
     model.load_state_dict(state, strict=True)

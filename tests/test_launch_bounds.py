@@ -138,3 +138,24 @@ def test_no_float64_arithmetic_on_the_device(percol, monkeypatch):
         LB.check_shape(_model(layers=2), 2, 12, 8, 8, stride=1, percol=percol)
     assert not seen, f"float64 on the device: {sorted(set(seen))}"
     assert passed and torch.float64 not in passed
+
+
+def test_footprint_counts_the_vertical_modes_of_tall_columns():
+    """Admission's footprint of a VJP covers the adjoint's own count (Plan.vjp_bytes at stride 1, two-pass fusion)
+    also when twice the column span exceeds the reference length and the box holds more vertical modes."""
+    import numpy as np
+    import launch_bounds as LB
+    from ffnoml_amd import engine as E
+    model = _model(layers=2)
+    eng, chk, restore = LB.dry_engine(model)
+    try:
+        for span in (15.0, 30.0):                        # 2 x 30 > 40: a longer box, more modes
+            D, H, W = 20, 8, 8
+            col = np.linspace(span, 0.0, D)
+            plan = E.Plan(eng, 1, D, H, W, col, span, 48e3, 48e3)
+            need = eng.footprint_bytes(1, D, H, W, vjp=True, span=span)
+            assert plan.modes == eng.vertical_modes(span)
+            assert need >= plan.vjp_bytes(1, two_pass=True), (span, need, plan.vjp_bytes(1, two_pass=True))
+        assert eng.vertical_modes(30.0) > eng.vertical_modes(15.0)
+    finally:
+        restore()

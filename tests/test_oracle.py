@@ -37,6 +37,17 @@ def test_golden_forward_and_gradients(name):
     assert _rel(zb.numpy(), data["zbar"]) < 1e-8
 
 
+@pytest.mark.parametrize("name", ["case_a_shared64", "case_c_rank0", "case_e_tall"])
+def test_per_column_height_gradient_sums_to_the_references_shared_one(name):
+    """On a shared grid the reference's autograd gives the vertical transform's height sensitivity of every column to
+    the heights of column (0, 0) of sample 0 (models/ffno_model.py:270-276); the per-column path (the engine's
+    attribution) gives each column its own. Per level, summed over the columns of each sample, the two are the same
+    (1e-10 of the range) when summed over the columns of every sample (the reference's goes to sample 0)."""
+    data, y, xb, zb = _run(name, per_column=True)
+    assert _rel(zb.numpy().sum(axis=(0, 2, 3)), data["zbar"].sum(axis=(0, 2, 3))) < 1e-10
+    assert _rel(xb.numpy(), data["xbar"]) < 1e-8
+
+
 def test_golden_float32():
     """float32 oracle vs float32 reference, same weights and inputs: 2e-5 of the output range."""
     data, config, state = load("case_a_shared32")
