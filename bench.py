@@ -25,6 +25,8 @@ HBM_PEAK_GBS = 8000.0
 WORKLOAD = dict(B=8, D=128, H=64, W=64, width=64, n_layers=8, z_min_wavelength=1.28, hidden=128, rank=8)
 VOLUME = dict(B=1, D=256, H=504, W=504)       # the whole-volume case: same network, one 504 x 504 x 256 atmosphere
 CPU_WORKLOAD = dict(B=1, D=64, H=32, W=32, width=32, n_layers=4, z_min_wavelength=3.5, hidden=128, rank=8)
+# the production configuration's width and vertical resolution (1001 vertical modes) on the same tile
+PRODUCTION = dict(WORKLOAD, width=96, z_min_wavelength=0.04)
 
 
 def model_config(w):
@@ -110,6 +112,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-baselines", action="store_true", help="skip the plain FP32 GPU and CPU legs")
     ap.add_argument("--no-volume", action="store_true", help="skip the whole-volume 504 x 504 x 256 forward + VJP")
+    ap.add_argument("--no-forward-only", action="store_true", help="skip the forward-only leg (configs[1])")
     ap.add_argument("--no-production", action="store_true",
                     help="skip the production-width leg (width 96, 1001 vertical modes, on the benchmark tile)")
     ap.add_argument("--service-requests", type=int, default=160,
@@ -122,6 +125,9 @@ def main():
                     help="first copy 1 GiB device to device three times (the known-size reference for HBM counters)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 --pmc child runs (FETCH_SIZE, WRITE_SIZE, MFMA and VALU instructions)")
+    ap.add_argument("--workload", choices=("benchmark", "production"), default="benchmark",
+                    help="the timed step's configuration: the benchmark's (width 64, 32 vertical modes; the line's "
+                         "metric) or, for profiling, the production width (96, 1001 vertical modes) on the same tile")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -142,7 +148,7 @@ def main():
 
     dev = torch.device("cuda:0")
     copy_cal = copy_calibration(dev) if args.copy_calibration else None
-    w = WORKLOAD
+    w = WORKLOAD if args.workload == "benchmark" else PRODUCTION
     voxels = w["B"] * w["D"] * w["H"] * w["W"]
     model, x, z, ct = make(w, dev)
 
@@ -224,7 +230,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 activations, split-bf16 MFMA (fp32 accumulate)",
         "data": "synthetic (stratified random-field atmosphere, seed 0; seeded random weights)",
-        "config": {"workload": "FFNO3D forward + input VJP, B=8 D=128 H=W=64 width=64 modes(D)=32 layers=8",
+        "config": {"workload": f"FFNO3D forward + input VJP, B=8 D=128 H=W=64 width={w['width']} "
+                               f"modes(D)={int(40.0 / w['z_min_wavelength']) + 1} layers=8",
                    "global_batch": w["B"], "voxels_per_step": voxels, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": kern, "entry_point": entry, "launches": klaunches,
                      "avg_ms": round(kms, 4), "share_of_step": round(ktot / timed_total, 3),
@@ -241,24 +248,27 @@ def main():
             "achieved_gb_s": round(ent_b / (ent_t * 1e-3) / 1e9, 1),
             "note": "aggregates several kernel instances; the roofline above is the largest single kernel"}
 
-    # configs[1]: the forward alone at the same shape
-    with torch.no_grad():
-        for _ in range(2):
-            yf = model(x, z, 48e3, 48e3)
-        torch.cuda.synchronize()
-        f0, f1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        nf = max(args.steps, 5)
-        f0.record()
-        for _ in range(nf):
-            yf = model(x, z, 48e3, 48e3)
-        f1.record()
-        torch.cuda.synchronize()
-    fms = f0.elapsed_time(f1) / nf
-    out["forward_only"] = {"workload": "FFNO3D forward, B=8 D=128 H=W=64 width=64 modes(D)=32 layers=8",
-                           "ms_per_forward": round(fms, 3), "value": round(voxels / (fms * 1e-3) / 1e6, 3),
-                           "unit": "Mvoxels/s", "forwards": nf}
-    del yf
-    log(f"forward {fms:.1f} ms, {out['forward_only']['value']:.2f} Mvox/s")
+    # the forward alone at the same shape
+    fms = None
+    if not args.no_forward_only:
+        with torch.no_grad():
+            for _ in range(2):
+                yf = model(x, z, 48e3, 48e3)
+            torch.cuda.synchronize()
+            f0, f1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            nf = max(args.steps, 5)
+            f0.record()
+            for _ in range(nf):
+                yf = model(x, z, 48e3, 48e3)
+            f1.record()
+            torch.cuda.synchronize()
+        fms = f0.elapsed_time(f1) / nf
+        out["forward_only"] = {"workload": f"FFNO3D forward, B=8 D=128 H=W=64 width={w['width']} "
+                                           f"modes(D)={int(40.0 / w['z_min_wavelength']) + 1} layers=8",
+                               "ms_per_forward": round(fms, 3), "value": round(voxels / (fms * 1e-3) / 1e6, 3),
+                               "unit": "Mvoxels/s", "forwards": nf}
+        del yf
+        log(f"forward {fms:.1f} ms, {out['forward_only']['value']:.2f} Mvox/s")
 
     if not args.no_baselines:
         from baseline.plain_fp32 import PlainFFNO3D
@@ -289,15 +299,17 @@ def main():
         out["plain_fp32_gpu"] = {"value": round(voxels / (pms * 1e-3) / 1e6, 3), "unit": "Mvoxels/s",
                                  "ms_per_step": round(pms, 1), "speedup": round(pms / ms, 2),
                                  "rel_err_log10n_vs_hip": rel, "grad_rel_err_vs_hip": grel, "timing": "median of 3"}
-        with torch.no_grad():
-            pfms, _ = median_ms(lambda: plain(x, z, 48e3, 48e3))
-        out["forward_only"]["plain_fp32_ms"] = round(pfms, 1)
-        out["forward_only"]["speedup_vs_plain_fp32"] = round(pfms / fms, 2)
-        log(f"plain FP32: fwd+VJP {pms:.0f} ms, forward {pfms:.0f} ms")
+        if fms is not None:
+            with torch.no_grad():
+                pfms, _ = median_ms(lambda: plain(x, z, 48e3, 48e3))
+            out["forward_only"]["plain_fp32_ms"] = round(pfms, 1)
+            out["forward_only"]["speedup_vs_plain_fp32"] = round(pfms / fms, 2)
+        log(f"plain FP32: fwd+VJP {pms:.0f} ms")
         del plain
         out["cpu_baseline"], out["accuracy_vs_fp64"] = cpu_leg(model, x[:1].cpu(), z[:1].cpu(),
                                                                y[:1].detach().double().cpu(), y0_plain)
-        out["forward_only"]["rel_err_log10n_vs_fp64"] = out["accuracy_vs_fp64"]["hip_rel_err_log10n"]
+        if fms is not None:
+            out["forward_only"]["rel_err_log10n_vs_fp64"] = out["accuracy_vs_fp64"]["hip_rel_err_log10n"]
     if copy_cal:
         out["copy_calibration"] = copy_cal
     del y, xb
@@ -320,7 +332,7 @@ def production_leg(dev, log):
     forward and forward + input VJP, median of 3 after one warm-up, and the float64 oracle's error on item 0 is not
     recomputed here (tests/test_gpu_production.py checks this configuration's parity)."""
     import torch
-    w = dict(WORKLOAD, width=96, z_min_wavelength=0.04)
+    w = PRODUCTION
     model, x, z, ct = make(w, dev)
     voxels = w["B"] * w["D"] * w["H"] * w["W"]
 

@@ -734,16 +734,16 @@ class Engine:
                   _p(a), _p(b), self._stream())
         return a, b, stats
 
-    def _tw(self, mat, x, y, batch, n, bmap_x, bmap_y, kmap, mmap, accumulate=0, twiddle=None):
+    def _tw(self, mat, x, y, batch, n, bmap_x, bmap_y, kmap, mmap, accumulate=0, twiddle=None, block=True):
         hi, lo, m, k, mp, kp = mat[:6]
-        if (twiddle is None and (mp > DENSE_DFT_MAX or kp > DENSE_DFT_MAX) and (mp <= DENSE_DFT_MAX or mmap[0] >= m)
-                and (kp <= DENSE_DFT_MAX or kmap[0] >= k)):
+        if (block and twiddle is None and (mp > DENSE_DFT_MAX or kp > DENSE_DFT_MAX) and
+                (mp <= DENSE_DFT_MAX or mmap[0] >= m) and (kp <= DENSE_DFT_MAX or kmap[0] >= k)):
             # the large dimensions have single-level maps: blocks of the LDS-staged size, the k blocks accumulated
             xf, yf = x.view(-1), y.view(-1)
             for m0 in range(0, m, DENSE_DFT_MAX):
                 for j, (k0, blk) in enumerate(_k_blocks(mat, m0)):
                     self._tw(blk, xf[k0 * kmap[2]:], yf[m0 * mmap[2]:], batch, n, bmap_x, bmap_y, kmap, mmap,
-                             accumulate if j == 0 else 1)
+                             accumulate if j == 0 else 1, block=False)
             return
         g = FfnoTwGemm()
         g.a_hi, g.a_lo = _p(hi), _p(lo)
@@ -904,9 +904,9 @@ class Engine:
         hwc = plan.H * plan.W * plan.C
         big = 1 << 40
         if to_spectrum:
-            self._tw(mat, x, y, B, hwc, (big, 0, D * hwc), (0, 2 * M * hwc), (big, 0, hwc), (M, M * hwc, hwc))
+            self._tw(mat, x, y, B, hwc, (big, 0, D * hwc), (0, 2 * M * hwc), (big, 0, hwc), (big, 0, hwc))
         else:
-            self._tw(mat, x, y, B, hwc, (big, 0, 2 * M * hwc), (0, D * hwc), (M, M * hwc, hwc), (big, 0, hwc))
+            self._tw(mat, x, y, B, hwc, (big, 0, 2 * M * hwc), (0, D * hwc), (big, 0, hwc), (big, 0, hwc))
 
     def vertical(self, plan, xv, v1, out, i, tables, adjoint=False):
         """Filon transform along D -> mix -> synthesis at the levels, /L folded in (models/ffno_model.py:270-305);
@@ -937,10 +937,12 @@ class Engine:
         else:
             m1, m2 = plan.tw["v_fwd"], plan.tw["v_inv"]
             wc = self.weights.blocks[i]["vertical.wc"]
-        self._tw(m1, xv, v1, B, hwc, (big, 0, D * hwc), (0, 2 * M * hwc), (big, 0, hwc), (M, M * hwc, hwc))
+        # spectra [B, 2, M, H, W, C]: row i of the [2M, D] analysis (real parts, then imaginary) sits at i * hwc, a
+        # single-level map, so that 2M > 128 rows run as LDS-staged 128-row blocks
+        self._tw(m1, xv, v1, B, hwc, (big, 0, D * hwc), (0, 2 * M * hwc), (big, 0, hwc), (big, 0, hwc))
         hw = plan.H * plan.W
         self._mix(plan, v1, v1[M * hwc:], B * hw, M, hwc, (hw, 2 * M * hwc, C), wc, tables["vertical"], adjoint)
-        self._tw(m2, v1, out, B, hwc, (big, 0, 2 * M * hwc), (0, D * hwc), (M, M * hwc, hwc), (big, 0, hwc))
+        self._tw(m2, v1, out, B, hwc, (big, 0, 2 * M * hwc), (0, D * hwc), (big, 0, hwc), (big, 0, hwc))
 
     def _scratch(self, plan, dev, adjoint=False):
         f32 = dict(device=dev, dtype=torch.float32)

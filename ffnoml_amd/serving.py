@@ -49,8 +49,8 @@ class Request:
 class ContinuousBatcher:
     """submit() -> Future of the backend's result ({"populations"} or {"populations", "features", "z_scale"})."""
 
-    def __init__(self, backends, slots=3, max_batch=8, memory_fraction=0.85, graphs=True, max_graphs=32,
-                 graph_bytes=1 << 30, budget=None):
+    def __init__(self, backends, slots=3, max_batch=8, memory_fraction=0.85, graphs=True, max_graphs=24,
+                 graph_bytes=4 << 30, budget=None):
         self.backends = dict(backends)
         for backend in self.backends.values():
             # admission budgets the adjoint at checkpoint stride 1; keeping every intermediate would outgrow it
