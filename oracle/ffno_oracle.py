@@ -139,9 +139,11 @@ def height_fourier_weights(z, k):
     return q_re, q_im
 
 
-def vertical_box(z_scale, cfg, dtype):
-    """(L, k [M], mode factors [M]) for these columns (models/ffno_model.py:248-259). The span is read on the host."""
-    span = float((z_scale.amax(dim=-3) - z_scale.amin(dim=-3)).max())
+def vertical_box(z_scale, cfg, dtype, span=None):
+    """(L, k [M], mode factors [M]) for these columns (models/ffno_model.py:248-259). The span is read on the host
+    (span: that of a whole volume, when z_scale holds only some of its columns)."""
+    if span is None:
+        span = float((z_scale.detach().amax(dim=-3) - z_scale.detach().amin(dim=-3)).max())
     length = cfg["z_reference_length"]
     modes = cfg["modes"]
     if 2 * span > length:
@@ -265,12 +267,13 @@ def grid_is_shared(z):
     return bool((z == z[..., :1, :1]).all()) and bool((z == z[:1]).all())
 
 
-def vertical_branch(sd, prefix, x, z_scale, cfg, force_per_column=False):
-    """HeightSpectralConv.forward (models/ffno_model.py:261-309)."""
+def vertical_branch(sd, prefix, x, z_scale, cfg, force_per_column=False, span=None):
+    """HeightSpectralConv.forward (models/ffno_model.py:261-309). span: the whole volume's, when x holds some of its
+    columns (every column is transformed on its own; only the box length is shared)."""
     if z_scale.dim() == 4:
         z_scale = z_scale.unsqueeze(1)
     C = x.shape[1]
-    length, k, factor = vertical_box(z_scale, cfg, x.dtype)
+    length, k, factor = vertical_box(z_scale, cfg, x.dtype, span)
     x = x * input_gate(sd, prefix, x) + conv1x1(z_scale, sd[prefix + "z_proj.weight"])
     z = z_scale - z_scale.amin(dim=2, keepdim=True)
     shared = grid_is_shared(z) and not force_per_column
@@ -314,6 +317,27 @@ def vertical_branch(sd, prefix, x, z_scale, cfg, force_per_column=False):
 # ----------------------------------------------------------------------------------------------------------------
 # block, lift, head
 # ----------------------------------------------------------------------------------------------------------------
+def fuse_branches(sd, p, x, spec, vert):
+    """The branch gates and the fusion MLP before its GroupNorm: fuse([s g_s, v g_v]) + (s g_s + v g_v) / 2
+    (models/ffno_model.py:524-532)."""
+    gates = torch.sigmoid(conv1x1(gelu(conv1x1(torch.cat([x, spec, vert], 1), sd[p + "branch_gate.0.weight"])),
+                                  sd[p + "branch_gate.2.weight"], sd[p + "branch_gate.2.bias"]))
+    sg, vg = gates.chunk(2, dim=1)
+    sm, vm = spec * sg, vert * vg
+    fused = conv1x1(gelu(conv1x1(torch.cat([sm, vm], 1), sd[p + "fuse.0.weight"])), sd[p + "fuse.2.weight"])
+    return fused + 0.5 * (sm + vm)
+
+
+def pw_mlp(sd, p, x1):
+    """pw: Conv C->2C, GELU, Conv 2C->C (models/ffno_model.py:489-495, :545)."""
+    return conv1x1(gelu(conv1x1(x1, sd[p + "pw.0.weight"])), sd[p + "pw.2.weight"])
+
+
+def point_mlp(sd, p, x2):
+    """mlp: PointwiseMLP C->2C->C (models/ffno_model.py:147-160, :548)."""
+    return conv1x1(gelu(conv1x1(x2, sd[p + "mlp.net.0.weight"])), sd[p + "mlp.net.3.weight"])
+
+
 def block(sd, i, x, z_scale, dx, dy, cfg, force_per_column=False):
     """FFNOBlock3dBalanced.forward in eval mode (models/ffno_model.py:536-553, :524-534)."""
     p = f"blocks.{i}."
@@ -325,17 +349,10 @@ def block(sd, i, x, z_scale, dx, dy, cfg, force_per_column=False):
 
     spec = gelu(norm("norm_spec", horizontal_branch(sd, p + "spec.", x, dx, dy, cfg)))
     vert = gelu(norm("norm_vert", vertical_branch(sd, p + "vertical.", x, z_scale, cfg, force_per_column)))
-    gates = torch.sigmoid(conv1x1(gelu(conv1x1(torch.cat([x, spec, vert], 1), sd[p + "branch_gate.0.weight"])),
-                                  sd[p + "branch_gate.2.weight"], sd[p + "branch_gate.2.bias"]))
-    sg, vg = gates.chunk(2, dim=1)
-    sm, vm = spec * sg, vert * vg
-    fused = conv1x1(gelu(conv1x1(torch.cat([sm, vm], 1), sd[p + "fuse.0.weight"])), sd[p + "fuse.2.weight"])
-    fused = gelu(norm("norm_fuse", fused + 0.5 * (sm + vm)))
+    fused = gelu(norm("norm_fuse", fuse_branches(sd, p, x, spec, vert)))
     x1 = x + sd[p + "res_fused"] * fused
-    pw = norm("norm_pw", conv1x1(gelu(conv1x1(x1, sd[p + "pw.0.weight"])), sd[p + "pw.2.weight"]))
-    x2 = x1 + sd[p + "res_pw"] * pw
-    ml = norm("norm_mlp", conv1x1(gelu(conv1x1(x2, sd[p + "mlp.net.0.weight"])), sd[p + "mlp.net.3.weight"]))
-    return x2 + sd[p + "res_mlp"] * ml
+    x2 = x1 + sd[p + "res_pw"] * norm("norm_pw", pw_mlp(sd, p, x1))
+    return x2 + sd[p + "res_mlp"] * norm("norm_mlp", point_mlp(sd, p, x2))
 
 
 # MULTI3D constants in cgs (models/ffno_model.py:111-113)

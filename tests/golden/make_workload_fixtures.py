@@ -19,6 +19,9 @@ Cases:
                     z-gradient for a fixed cotangent (the factorised 504-point DFTs along both axes).
   wl_service        the cfg1 model in SI units (the service path) on the smallest serving tile, 96 x 32 x 32:
                     linear populations and the feature and height gradients of a fixed population cotangent.
+  wl_vol252         the whole-volume network (width 64, 8 layers, 32 vertical modes) on a quarter-area proxy of the
+                    504 x 504 x 256 volume, B=1, D=256, 252 x 252: the forward, by the streamed float64 oracle
+                    (oracle/ffno_streamed.py; the whole-volume float64 intermediates would not fit host RAM).
 
 Only the oracle (oracle/ffno_oracle.py) runs here; nothing is imported from the reference.
 """
@@ -50,6 +53,7 @@ SHAPE1 = (8, 128, 64, 64)
 TILE504 = dict(CFG1, n_layers=2, z_min_wavelength=8.0)      # 6 vertical modes: D=4 levels resolve 4
 SHAPE504 = (1, 4, 504, 504)
 SERVICE_TILE = (96, 32, 32)
+SHAPE252 = (1, 256, 252, 252)
 FD_ENTRIES, FD_STEP = 64, 1e-4
 STRIDE = 4
 
@@ -214,6 +218,20 @@ def make_service():
          config_json=np.array(json.dumps(CFG1)), tile=np.array(SERVICE_TILE), stride=np.array(STRIDE))
 
 
+def make_vol252():
+    from oracle import ffno_streamed as S
+    B, D, H, W = SHAPE252
+    x, z, mx, sx = synthetic.atmosphere(B, D, H, W, seed=0)
+    state = weights(model_config(CFG1), mx, sx)
+    sd, cfg = O.to_dtype(state, torch.float64), oracle_cfg(model_config(CFG1))
+    t0 = time.time()
+    y = S.forward(sd, cfg, x.double(), z.double(), DX, DY, plane_chunk=8, row_chunk=16,
+                  log=lambda m: print(f"vol252 {m} ({time.time() - t0:.0f} s)", flush=True)).numpy()
+    save("wl_vol252", y=sub(y), **sums("y", y), state_sha256=np.array(state_digest(state)),
+         config_json=np.array(json.dumps(CFG1)), shape=np.array(SHAPE252), stride=np.array(STRIDE), mean_x=mx,
+         std_x=sx)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--only", nargs="*", default=["service", "cfg1", "tile504", "cfg2"])
@@ -223,7 +241,7 @@ def main():
     torch.set_num_threads(args.threads)
     for name in args.only:
         {"cfg1": make_cfg1, "cfg2": lambda: make_cfg2(args.scratch), "tile504": make_tile504,
-         "service": make_service}[name]()
+         "service": make_service, "vol252": make_vol252}[name]()
 
 
 if __name__ == "__main__":

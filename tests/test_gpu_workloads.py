@@ -140,3 +140,25 @@ def test_tile504_forward_and_adjoint():
     assert y_err <= LOG_TOL
     assert x_err <= GRAD_TOL
     assert z_err <= GRAD_TOL
+
+
+def test_volume_proxy_252():
+    """The whole-volume network (width 64, 8 layers, 32 vertical modes) on a quarter-area proxy of the 504 x 504 x 256
+    volume, B=1, D=256, 252 x 252 (factorised 252-point DFTs along both axes, 128-level blocks of the Filon
+    matrices), against the streamed float64 oracle: full-size accuracy rests on this and on the 504-point tile."""
+    from ffnoml_amd import synthetic
+    data = _fixture("wl_vol252")
+    B, D, H, W = (int(v) for v in data["shape"])
+    x, z, mx, sx = synthetic.atmosphere(B, D, H, W, seed=0)
+    assert np.array_equal(mx, data["mean_x"]) and np.array_equal(sx, data["std_x"])
+    model = _model(data, mx, sx)
+    print(f"[vol252] forward {B}x{D}x{H}x{W}", flush=True)
+    with torch.no_grad():
+        y = model(x.to(DEV), z.to(DEV), 48e3, 48e3)
+    torch.cuda.synchronize()
+    y0 = y.double().cpu().numpy()
+    err = _log_err(_sub(y0, int(data["stride"])), data["y"])
+    dsum = abs(float(y0.sum()) - float(data["y:sum"])) / float(data["y:abssum"])
+    print(f"[vol252] rel log10 n {err:.3e}; sum {dsum:.2e}", flush=True)
+    assert err <= LOG_TOL
+    assert dsum <= 1e-5

@@ -243,3 +243,22 @@ def test_head_reproduces_any_target():
         head[:, 7 * sp:7 * sp + 6] = ratios * math.log(10.0) - lte[:, 6 * sp:6 * sp + 6]
         head[:, 7 * sp + 6] = 6.0 * torch.atanh(s / 6.0)
     assert (O.head_output(sd, head, x, cfg) - target).abs().max() < 1e-9
+
+
+@pytest.mark.parametrize("name", ["case_a_shared64", "case_b_jitter64", "case_e_tall"])
+def test_streamed_forward_equals_the_oracle(name):
+    """The streamed float64 forward (oracle/ffno_streamed.py: depth-plane and column-row chunks, two-pass GroupNorm
+    statistics over the chunks; the reference's streamed backend pattern, ffno_batched_backend.py:159-199, :253-289)
+    equals the whole-volume oracle to 1e-12 of the range, and so the reference's own outputs (the goldens) to 1e-10.
+    Case B's columns differ in span, so a row chunk must take the volume's box, not its own."""
+    from oracle import ffno_streamed as S
+    data, config, state = load(name)
+    cfg = oracle_config(config)
+    sd = O.to_dtype(state, torch.float64)
+    x = torch.from_numpy(data["x"]).double()
+    z = torch.from_numpy(data["z"]).double()
+    with torch.no_grad():
+        whole = O.forward(sd, cfg, x, z, 48e3, 48e3)
+    got = S.forward(sd, cfg, x, z, 48e3, 48e3, plane_chunk=5, row_chunk=3)
+    assert _rel(got.numpy(), whole.numpy()) < 1e-12
+    assert _rel(got.numpy(), data["y"]) < 1e-10

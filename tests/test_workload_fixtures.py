@@ -10,7 +10,7 @@ import pytest
 
 from golden_cases import GOLDEN
 
-NAMES = ["wl_cfg1_forward", "wl_cfg2_gradient", "wl_tile504", "wl_service"]
+NAMES = ["wl_cfg1_forward", "wl_cfg2_gradient", "wl_tile504", "wl_service", "wl_vol252"]
 
 
 def _load(name):
@@ -27,7 +27,7 @@ def test_weights_regenerate(name):
     data = _load(name)
     config = json.loads(str(data["config_json"]))
     config["species"] = species(config["species"])
-    if name == "wl_tile504":
+    if name in ("wl_tile504", "wl_vol252"):
         B, D, H, W = (int(v) for v in data["shape"])
         _, _, mx, sx = synthetic.atmosphere(B, D, H, W, seed=0)
     else:
