@@ -9,10 +9,11 @@ statistics are per sample, so a request batched with others gets what it would g
 
 Forward-only batches of small tiles (forward footprint at most `graph_bytes`), once their bucket has been seen,
 replay a captured HIP graph of the whole forward (static input/output buffers per (bucket, batch size)); there the
-launch overhead of the ~15 launches per layer is a visible share of the batch time. The worker that ran a bucket's
-first batch captures it right after answering, on its own stream, while other workers keep running: the capture is
-thread-local (only the capturing thread may not synchronise, and it does not: the plan and the per-mode tables were
-built by the eager batch), so it needs no idle device. The memory a graph holds is counted against the HBM budget;
+launch overhead of the ~15 launches per layer is a visible share of the batch time. When a (bucket, batch size)
+recurs, the worker that ran it eagerly for the second time captures it right after answering, holding the device
+alone for the capture (HIP refuses another thread's copies while a stream is capturing): new batches wait for it and
+the ones in flight drain first, a stall of about one batch per captured shape. The memory a graph holds is counted
+against the HBM budget;
 graphs are evicted least recently replayed first, and an evicted graph's memory is released from the count only
 when its last replay has finished. Larger tiles and VJP batches run eagerly.
 
@@ -20,6 +21,7 @@ Admission never stalls: when the oldest request does not fit the unclaimed budge
 graphs and the engines' unused tables are dropped; a request that still does not fit is failed with MemoryError.
 """
 import collections
+import contextlib
 import hashlib
 import queue
 import threading
@@ -46,6 +48,45 @@ class Request:
         self.key = (model, op, z.shape, self.dx, self.dy, zh)
 
 
+class DeviceGate:
+    """Shared holds for batches, an exclusive hold for a graph capture: a waiting capture blocks new shared holds, so
+    the batches in flight drain and no other thread touches the device while a stream is capturing."""
+
+    def __init__(self):
+        self.cv = threading.Condition()
+        self.readers = 0
+        self.writer = False
+        self.waiting = 0
+
+    @contextlib.contextmanager
+    def shared(self):
+        with self.cv:
+            while self.writer or self.waiting:
+                self.cv.wait()
+            self.readers += 1
+        try:
+            yield
+        finally:
+            with self.cv:
+                self.readers -= 1
+                self.cv.notify_all()
+
+    @contextlib.contextmanager
+    def exclusive(self):
+        with self.cv:
+            self.waiting += 1
+            while self.writer or self.readers:
+                self.cv.wait()
+            self.waiting -= 1
+            self.writer = True
+        try:
+            yield
+        finally:
+            with self.cv:
+                self.writer = False
+                self.cv.notify_all()
+
+
 class ContinuousBatcher:
     """submit() -> Future of the backend's result ({"populations"} or {"populations", "features", "z_scale"})."""
 
@@ -61,6 +102,8 @@ class ContinuousBatcher:
         self.max_graphs = int(max_graphs)
         self.graph_bytes = int(graph_bytes)
         self.graph_held = 0                             # bytes held by captured graphs' memory pools
+        self.gate = DeviceGate()
+        self.eager_runs = collections.Counter()         # forward batches run eagerly, per (bucket, batch size)
         dev = next(iter(self.backends.values())).device
         self.device = dev
         if budget is None:
@@ -224,12 +267,13 @@ class ContinuousBatcher:
                 r.t_start = t0
             capture = None
             try:
-                if cuda:
-                    with torch.cuda.stream(stream):
+                with self.gate.shared():
+                    if cuda:
+                        with torch.cuda.stream(stream):
+                            results, capture = self._run(batch)
+                        stream.synchronize()
+                    else:
                         results, capture = self._run(batch)
-                    stream.synchronize()
-                else:
-                    results, capture = self._run(batch)
                 done = time.perf_counter()
                 for r in batch:
                     r.t_done = done
@@ -301,10 +345,13 @@ class ContinuousBatcher:
                 self.graphs.move_to_end(key)            # least recently replayed graphs are evicted first
                 g.users += 1
         if g is None:
-            # first batch of this shape runs eagerly (building the plan and its tables and warming the allocator);
-            # the worker captures afterwards
+            # batches of this (bucket, batch size) run eagerly until it recurs (the first builds the plan and its tables
+            # and warms the allocator); at the second the worker captures, afterwards
             n = backend.predict_tensor(x, z, r0.dx, r0.dy)
-            return n, (None if uncapturable else (backend, key, x, z, r0.dx, r0.dy, held))
+            with self.graph_lock:
+                self.eager_runs[key] += 1
+                recurring = self.eager_runs[key] >= 2
+            return n, (None if uncapturable or not recurring else (backend, key, x, z, r0.dx, r0.dy, held))
         try:
             # one graph's static buffers serve one batch at a time: a replay on this worker's stream waits until the
             # previous replay (on whichever stream) has copied its output out
@@ -344,13 +391,14 @@ class ContinuousBatcher:
                 return
             if self.claimed + self.graph_held + held > self.budget:
                 return                                  # no room to keep another graph's memory now
-        with torch.cuda.stream(stream):
-            plan = backend.plan_for(z, dx, dy)          # cached by the eager batch: no host synchronisation here
+        # the device to this thread alone while it captures (HIP refuses other threads' copies to a capturing
+        # device): new batches wait, the ones in flight drain
+        with self.gate.exclusive(), torch.cuda.stream(stream):
+            plan = backend.plan_for(z, dx, dy)          # cached by the eager batches: no host synchronisation here
             sx, sz = x.clone(), z.clone()
             stream.synchronize()
             graph = torch.cuda.CUDAGraph()
             try:
-                # thread-local capture: other workers keep launching on their own streams meanwhile
                 with torch.cuda.graph(graph, stream=stream, capture_error_mode="thread_local"):
                     sn = backend.predict_tensor(sx, sz, dx, dy, plan=plan)
             except Exception:
