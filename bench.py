@@ -119,6 +119,8 @@ def main():
                     help="requests of the serving workload (Poisson arrivals at 40/s, seed 1; 0 skips it; the full "
                          "workload is 2000)")
     ap.add_argument("--service-slots", type=int, default=3, help="execution slots of the serving leg's batcher")
+    ap.add_argument("--service-log", default=None,
+                    help="write the serving leg's per-batch phase times, admissions and request times to this JSON file")
     ap.add_argument("--plain-service-requests", type=int, default=16,
                     help="the first requests of the same trace served one at a time by the plain FP32 GPU path")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -317,7 +319,8 @@ def main():
         out["production_width"] = production_leg(dev, log)
     if args.service_requests > 0:
         out["service"] = service_leg(model, dev, args.service_requests,
-                                     0 if args.no_baselines else args.plain_service_requests, args.service_slots)
+                                     0 if args.no_baselines else args.plain_service_requests, args.service_slots,
+                                     args.service_log)
     if not args.no_volume:
         del x, z, ct
         model.engine().clear_caches()
@@ -517,7 +520,7 @@ def service_inputs(trace, variants=4):
     return out
 
 
-def service_leg(model, dev, n, n_plain, slots=3):
+def service_leg(model, dev, n, n_plain, slots=3, log_path=None):
     """The serving workload through ffnoml_amd.serving.ContinuousBatcher on the benchmark's model (width 64,
     8 layers): requests submitted at their arrival times from a client thread, batched per shape bucket, admitted by
     HBM footprint, run on concurrent worker streams; latency = completion - arrival. Beside it, the first n_plain
@@ -538,6 +541,7 @@ def service_leg(model, dev, n, n_plain, slots=3):
     log(f"{n} requests generated in {time.perf_counter() - t0:.1f} s")
     backend = GPUBackend(None, model=model, level_names=[f"L{i}" for i in range(12)], device=str(dev))
     batcher = ContinuousBatcher({"m": backend}, slots=slots, max_batch=8)
+    stats0 = torch.cuda.memory_stats(dev)
     futures = [None] * n
     try:
         def client():
@@ -567,7 +571,16 @@ def service_leg(model, dev, n, n_plain, slots=3):
                "offered_mvox_s": round(vox / trace[-1]["t"] / 1e6, 3), "peak_batches_in_flight": batcher.peak_inflight,
                "graphs": len(batcher.graphs), "graph_captures": batcher.captures,
                "graph_replays": batcher.replays, "slots": batcher.slots, "max_batch": batcher.max_batch,
-               "queue": queue_summary(batcher.queue_samples, w0)}
+               "queue": queue_summary(batcher.queue_samples, w0), "phases": phase_summary(batcher.batch_log, wall)}
+        # the caching allocator's events over the window: a failed hipMalloc frees the cache and retries, and a
+        # flush synchronises every stream, so every worker waits for the longest batch in flight
+        stats1 = torch.cuda.memory_stats(dev)
+        res["allocator"] = {k: int(stats1.get(k, 0) - stats0.get(k, 0)) for k in
+                            ("num_alloc_retries", "num_sync_all_streams", "num_device_alloc", "num_device_free")}
+        if log_path:
+            with open(log_path, "w") as fh:
+                json.dump({"t0": w0, "batches": batcher.batch_log, "queue": batcher.queue_samples,
+                           "requests": [(q.t_submit, q.t_start, q.t_done, q.op, q.voxels) for q in done]}, fh)
         log(f"p50 {res['p50_ms']} ms, p99 {res['p99_ms']} ms, {res['throughput_mvox_s']} Mvox/s")
     finally:
         batcher.close()
@@ -577,6 +590,26 @@ def service_leg(model, dev, n, n_plain, slots=3):
         sub = [(q.t_done - q.t_submit) * 1e3 for q in done[:n_plain]]
         res["first_requests_p50_ms_in_the_full_run"] = round(float(np.percentile(sub, 50)), 1)
     return res
+
+
+def phase_summary(log, wall):
+    """Per operation: batches, mean batch size, Mvoxels, and the seconds the workers spent staging inputs (stack and
+    copy in), waiting for the device work and copying out, summed over batches (the workers overlap, so the sums
+    can exceed the serving window)."""
+    out = {}
+    for t, op, nb, vox, stage, compute, output in log:
+        o = out.setdefault(op, dict(batches=0, requests=0, mvox=0.0, staging_s=0.0, compute_s=0.0, output_s=0.0))
+        o["batches"] += 1
+        o["requests"] += nb
+        o["mvox"] += vox / 1e6
+        o["staging_s"] += stage
+        o["compute_s"] += compute
+        o["output_s"] += output
+    for o in out.values():
+        o["mean_batch"] = round(o["requests"] / o["batches"], 2)
+        for k in ("mvox", "staging_s", "compute_s", "output_s"):
+            o[k] = round(o[k], 3)
+    return out
 
 
 def queue_summary(samples, t0, points=20):

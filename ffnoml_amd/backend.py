@@ -18,6 +18,10 @@ from .factory import get_checkpoint_io_metadata, get_checkpoint_normalization, l
 
 INPUT_CHANNELS = ("temperature", "vx", "vy", "vz", "log10_ne", "log10_rho")
 Z_METRES_PER_MODEL_UNIT = 1.0e6                 # ffno_fsdp_runtime.py:43-45
+# metres -> Mm as one float32 multiply, which rounds alike on the host and on the device (the device's division by a
+# scalar multiplies by its reciprocal, the host's divides): a plan built from heights in host memory then sees the
+# very heights the device computes with
+MM_PER_METRE = 1.0 / Z_METRES_PER_MODEL_UNIT
 VELOCITY_M_S_PER_MODEL_UNIT = 1.0e3
 LOG10_POPULATION_LIMIT = 37.0
 LN10 = math.log(10.0)
@@ -93,7 +97,7 @@ class GPUBackend:
     def model_inputs(self, x_si, z_si):
         """SI -> normalised training features and z in Mm (ffno_fsdp_runtime.py:48-58)."""
         f = torch.cat((torch.log10(x_si[:, 0:1]), x_si[:, 1:4] / VELOCITY_M_S_PER_MODEL_UNIT, x_si[:, 4:6]), dim=1)
-        return (f - self.mean_x) / self.std_x, z_si / Z_METRES_PER_MODEL_UNIT
+        return (f - self.mean_x) / self.std_x, z_si * MM_PER_METRE
 
     def log_populations(self, y):
         return y * self.std_y + self.mean_y
@@ -117,10 +121,12 @@ class GPUBackend:
             return self.linear(self.log_populations(y))
 
     def plan_for(self, z_si, dx, dy):
+        """The engine's plan for SI heights z_si [B, nz, nx, ny] (m). Heights in host memory build it without
+        waiting for the device; on the device they are read back first."""
         B, D, H, W = z_si.shape
         eng = self.model.engine()
         eng.sync(self.device)
-        return eng.plan(B, D, H, W, z_si / Z_METRES_PER_MODEL_UNIT, float(dx), float(dy))
+        return eng.plan(B, D, H, W, z_si * MM_PER_METRE, float(dx), float(dy))
 
     def predict_local(self, features, z_scale, dx, dy):
         x, z = self.inputs(features, z_scale)
@@ -131,19 +137,22 @@ class GPUBackend:
         self.call_count += 1
         return self.canonical(populations)
 
-    def vjp_tensor(self, x_si, z_si, dx, dy, cotangent):
-        """(feature_bar [1,6,...] SI, z_bar [1,...] SI) of <cotangent, populations(x_si, z_si)>; cotangent
-        [1, L, nz, nx, ny]. The SI chain rule around the engine's adjoint is written out here: through the clamp and
-        10^, the output normalisation, the input normalisation and the log10 T / km/s / Mm conversions."""
+    def vjp_tensor(self, x_si, z_si, dx, dy, cotangent, stride=None, plan=None, keep=0):
+        """(populations, feature_bar [B,6,...] SI, z_bar [B,...] SI) of <cotangent, populations(x_si, z_si)>;
+        cotangent [B, L, nz, nx, ny]. The SI chain rule around the engine's adjoint is written out here: through the
+        clamp and 10^, the output normalisation, the input normalisation and the log10 T / km/s / Mm conversions.
+        stride: the adjoint's checkpoint stride (None: the engine's choice for the HBM free now); keep: with stride
+        1, the last `keep` blocks kept whole; plan: one built beforehand (plan_for), so that nothing here waits for
+        the device."""
         xm, zm = self.model_inputs(x_si, z_si)
         engine = self.model.engine()
-        y, saved = engine.forward_saved(xm, zm, float(dx), float(dy))
+        y, saved = engine.forward_saved(xm, zm, float(dx), float(dy), plan=plan, stride=stride, keep=keep)
         log_n = self.log_populations(y)
         n = self.linear(log_n)
         inside = (log_n >= -LOG10_POPULATION_LIMIT) & (log_n <= LOG10_POPULATION_LIMIT)
         ybar = cotangent * n * LN10 * inside * self.std_y
         del y, log_n, inside
-        xbar, zbar = engine.vjp(xm, zm, float(dx), float(dy), ybar, saved, want_z=True)
+        xbar, zbar = engine.vjp(xm, zm, float(dx), float(dy), ybar, saved, want_z=True, plan=plan)
         del saved
         fbar = xbar / self.std_x
         feature_bar = torch.cat((fbar[:, 0:1] / (x_si[:, 0:1] * LN10), fbar[:, 1:4] / VELOCITY_M_S_PER_MODEL_UNIT,
@@ -167,14 +176,14 @@ class GPUBackend:
         self.call_count += 1
         return {"features": fb[0].float().cpu().numpy(), "z_scale": zb[0].float().cpu().numpy()}
 
-    def jvp_tensor(self, x_si, z_si, dx, dy, tangent_si):
+    def jvp_tensor(self, x_si, z_si, dx, dy, tangent_si, plan=None):
         """(populations, populations_dot) [B, L, nz, nx, ny] along a tangent of the SI features [B, 6, nz, nx, ny]:
         the SI chain written out around the engine's tangent pass -- through log10 T (T_dot / (T ln 10)), km/s, the
         input normalisation, then the output normalisation, 10^ and its clamp (zero slope outside +-37)."""
         xm, zm = self.model_inputs(x_si, z_si)
         fd = torch.cat((tangent_si[:, 0:1] / (x_si[:, 0:1] * LN10), tangent_si[:, 1:4] / VELOCITY_M_S_PER_MODEL_UNIT,
                         tangent_si[:, 4:6]), dim=1) / self.std_x
-        y, yd = self.model.engine().jvp(xm, zm, float(dx), float(dy), fd)
+        y, yd = self.model.engine().jvp(xm, zm, float(dx), float(dy), fd, plan=plan)
         log_n = self.log_populations(y)
         n = self.linear(log_n)
         inside = (log_n >= -LOG10_POPULATION_LIMIT) & (log_n <= LOG10_POPULATION_LIMIT)

@@ -494,6 +494,23 @@ class Plan:
                 _settle(self.device)
             return self._height
 
+    def prepare(self, adjoint=False):
+        """Build now what a call would otherwise build on first use (the kept per-mode tables of every block, and for
+        the adjoint its transposed matrices and height tables), so that a caller can make a plan ready on one thread
+        while another launches work on the device."""
+        for t in self._mode_tables.values():
+            if t.cache:
+                for i in range(self.n_layers):
+                    t.get(i, False)
+                    if adjoint:
+                        t.get(i, True)
+        if adjoint:
+            self.adjoint_tables()
+            self.height_tables()
+            if self.percol:
+                self.column_adjoint()
+        return self
+
     def adjoint_tables(self):
         """Transposed transform matrices, so that the same GEMM kernel applies the transpose of every forward
         transform."""
@@ -533,14 +550,18 @@ class Saved:
     """What the forward keeps for the adjoint: the inputs of blocks 0, s, 2s, ... (s = stride) and, with s = 1, the
     head's input. The adjoint recomputes the inputs of the other blocks from the nearest kept one. With full, the
     forward keeps every intermediate the adjoint reads instead (about 11 activation-sized tensors per block), so that
-    nothing is recomputed -- the choice when HBM has room for it."""
+    nothing is recomputed -- the choice when HBM has room for it. In between, keep = k (stride 1) keeps the last k
+    blocks whole and the inputs of the others: the adjoint recomputes L - k blocks, in the memory of the stride-1
+    adjoint plus about 10 activation-sized tensors per kept block."""
 
-    def __init__(self, stride=1, two_pass=False, full=False):
+    def __init__(self, stride=1, two_pass=False, full=False, keep=0, n_layers=0):
         self.stride = int(stride)
         self.two_pass = bool(two_pass) or bool(full)   # fusion adjoint in two LDS-staged passes over stored gates
         self.full = bool(full)             # every block's intermediates kept: the adjoint recomputes nothing
+        # blocks from full_from on are kept whole (None: none)
+        self.full_from = 0 if full else (n_layers - int(keep) if keep and self.stride == 1 else None)
         self.inputs = {}
-        self.blocks = []                   # full: per block, the buffers and GroupNorm records of its forward
+        self.blocks = {}                   # block -> the buffers and GroupNorm records of its forward, if kept whole
         self.head = None
 
 
@@ -561,7 +582,8 @@ class Engine:
         self.weights = None
         self._version = None
         self._plans = {}
-        self._lock = threading.RLock()      # the serving workers share one engine
+        self._building = {}                 # plan key -> event set when the thread building that plan is done
+        self._lock = threading.RLock()      # the serving threads share one engine
         self.checkpoint_stride = None       # None: the smallest stride whose adjoint fits the free HBM
         self.two_pass_fuse = None           # None: whenever its gate buffer fits at stride 1
         self.full_save = None               # None: keep every intermediate when that fits; False: never
@@ -577,19 +599,29 @@ class Engine:
         return len(T.vertical_box(float(span), m.z_reference_length, m.z_min_wavelength,
                                   int(math.floor(m.z_reference_length / m.z_min_wavelength)) + 1)[1])
 
-    def footprint_bytes(self, B, D, H, W, vjp=False, jvp=False, span=0.0):
+    def footprint_bytes(self, B, D, H, W, vjp=False, jvp=False, span=0.0, full=False, keep=0):
         """HBM one request of this shape needs (for admission; the tables of a cached plan not included):
-        forward 3 activation-sized buffers, adjoint (L+1) saved inputs and 9 working buffers, JVP 16 (one block's
-        intermediates, its gates and four tangents), plus the scratch; span (Mm): the tallest column's height range,
-        which sets the vertical mode count."""
+        forward 3 activation-sized buffers, adjoint (L+1) saved inputs and 9 working buffers (full: 11 per block
+        and its vertical spectrum, as Plan.full_bytes counts them; keep = k blocks kept whole: the larger of the
+        stride-1 count and L - k inputs, 11 per kept block and 4, and k spectra), JVP 16 (one block's intermediates,
+        its gates and four tangents), plus the scratch; span (Mm): the tallest column's height range, which sets the
+        vertical mode count."""
         C, L, r = self.module.width, self.module.n_layers, self.module.freq_mix_rank
         unit = B * D * H * W * C * 4
         plane, plane2 = horizontal_planes(H, W, C)
         chunk = max(1, min(B * D, H_CHUNK_BYTES // (4 * plane2))) * (plane + plane2) * 4
         modes = self.vertical_modes(span)
         v1 = B * 2 * modes * H * W * C * 4
-        units = (L + 1) + WORKING_UNITS + GATE_UNITS if vjp else (16 if jvp else 3)
-        return unit * units + chunk + (2 if vjp else 1) * v1 + B * D * H * W * 4 * 32 + (64 << 20)
+        if vjp and (full or keep >= L):
+            units, spectra = 11 * L + 3, L + 2
+        elif vjp and keep:
+            units = max((L + 1) + WORKING_UNITS + GATE_UNITS, (L - keep) + 11 * keep + 4)
+            spectra = keep + 2
+        elif vjp:
+            units, spectra = (L + 1) + WORKING_UNITS + GATE_UNITS, 2
+        else:
+            units, spectra = (16 if jvp else 3), 1
+        return unit * units + chunk + spectra * v1 + B * D * H * W * 4 * 32 + (64 << 20)
 
     def _param_version(self):
         return tuple((p.data_ptr(), p._version) for p in self.module.parameters()) + \
@@ -687,13 +719,28 @@ class Engine:
             columns = (zd - zd.amin(dim=1, keepdim=True)).transpose(1, 2).reshape(B * H * W, D).contiguous()
             column = None
             key = (B, D, H, W, float(dx), float(dy), float(span), hashlib.sha1(columns.numpy().tobytes()).hexdigest())
-        with self._lock:
-            p = self._plans.get(key)
-            if p is None:
+        # built outside the engine's lock (a plan takes up to a few hundred ms of host work, while other threads
+        # launch calls that take the lock); a second caller of the same key waits for the first one's plan
+        while True:
+            with self._lock:
+                p = self._plans.get(key)
+                if p is not None:
+                    return p
+                building = self._building.get(key)
+                if building is None:
+                    building = self._building[key] = threading.Event()
+                    break
+            building.wait()
+        try:
+            p = Plan(self, B, D, H, W, column, span, dx, dy, columns=columns)
+            with self._lock:
                 if len(self._plans) > 64:
                     self._plans.pop(next(iter(self._plans)))
-                p = Plan(self, B, D, H, W, column, span, dx, dy, columns=columns)
                 self._plans[key] = p
+        finally:
+            with self._lock:
+                del self._building[key]
+            building.set()
         return p
 
     def choose_checkpoints(self, plan, stride=None, want_z=True):
@@ -1059,15 +1106,15 @@ class Engine:
         _lib.call("ffno_lift", rows, _p(x), _p(Wt.lift), _p(s), FfnoStatsOut(_p(plan.depth_w), _p(scr["part"])), st)
         a, b, _ = self._gn(plan, scr["part"], Wt.lift_gn, dev)
         prev = _asm(None, s, a, b, None, 1)
-        full = saved is not None and saved.full
+        keep_from = saved.full_from if saved is not None and saved.full_from is not None else len(Wt.blocks)
         for i in range(len(Wt.blocks)):
-            if full:
+            if i >= keep_from:
                 # every intermediate to a buffer of its own; x2 to a running buffer (hb), the next input from it
                 bk = {k: torch.empty(N, C, **f32) for k in ("h", "s", "v", "sp", "vp", "fp", "pp", "mp", "x1")}
                 bk["gates"] = torch.empty(N, 2 * C, **f32)
                 bk["v1y"] = torch.empty_like(scr["v1"]) if getattr(saved, "want_z", True) else None
                 bk["gn"] = self._block_keep(plan, i, prev, bk["h"], bk, scr, bz, hb)
-                saved.blocks.append(bk)
+                saved.blocks[i] = bk
                 ma, mb, _ = bk["gn"]["m"]
                 prev = _asm(hb, bk["mp"], ma, mb, self.weights.blocks[i]["res_mlp"])
                 continue
@@ -1217,12 +1264,17 @@ class Engine:
                   ctypes.byref(Wt.head), _p(yd), st)
         return y, yd
 
-    def forward_saved(self, x, z, dx, dy, plan=None, stride=None, want_z=True):
+    def forward_saved(self, x, z, dx, dy, plan=None, stride=None, want_z=True, keep=0):
         """(y, saved): the forward that keeps what the adjoint needs: every intermediate (stride 0, or by default
         when it fits the free HBM), else the block inputs at the given checkpoint stride or the one that fits.
-        want_z: the adjoint will be asked for the height gradient too (full keeps the vertical spectra then)."""
+        want_z: the adjoint will be asked for the height gradient too (full keeps the vertical spectra then).
+        keep (with stride 1): the last `keep` blocks kept whole, the others recomputed (two-pass fusion adjoint)."""
         dev, plan, _, _ = self._prepare(x, z, dx, dy, plan)
-        saved = Saved(*self.choose_checkpoints(plan, stride, want_z))
+        if keep and stride == 1:
+            keep = min(int(keep), plan.n_layers)
+            saved = Saved(1, True, keep == plan.n_layers, keep=keep, n_layers=plan.n_layers)
+        else:
+            saved = Saved(*self.choose_checkpoints(plan, stride, want_z))
         saved.want_z = want_z
         self.last_stride = 0 if saved.full else saved.stride     # informational (the latest saving forward's)
         return self.forward(x, z, dx, dy, plan=plan, saved=saved), saved
@@ -1268,14 +1320,22 @@ class Engine:
         rows = plan.rows
         f32 = dict(device=dev, dtype=torch.float32)
         g = torch.empty(N, C, **f32)
-        if saved.full:
-            s = v = sp = vp = fp = pp = mp = x1 = tbuf = gates = None
-        else:
-            s, v, sp, vp, fp, pp, mp, x1 = (torch.empty(N, C, **f32) for _ in range(WORKING_UNITS - 1))
-            tbuf = torch.empty(N, C, **f32) if stride > 1 else None
-            gates = torch.empty(N, 2 * C, **f32) if saved.two_pass else None
-        chunk = self._fuse_chunk(plan) if (gates is None and not saved.full and B == 1) else 0
-        gates_c = torch.empty(min(chunk, plan.voxels), 2 * C, **f32) if chunk else None
+        # the working buffers of the recomputed blocks, allocated at the first one (the blocks kept whole, whose
+        # adjoints run first, have been released by then)
+        work = {}
+
+        def working():
+            if not work:
+                work.update(zip(("s", "v", "sp", "vp", "fp", "pp", "mp", "x1"),
+                                (torch.empty(N, C, **f32) for _ in range(WORKING_UNITS - 1))))
+                work["tbuf"] = torch.empty(N, C, **f32) if stride > 1 else None
+                work["gates"] = torch.empty(N, 2 * C, **f32) if saved.two_pass else None
+                work["chunk"] = self._fuse_chunk(plan) if (work["gates"] is None and B == 1) else 0
+                work["gates_c"] = (torch.empty(min(work["chunk"], plan.voxels), 2 * C, **f32) if work["chunk"]
+                                   else None)
+            return work
+
+        s = chunk = gates_c = None
         scr = self._scratch(plan, dev, adjoint=True)
         part2 = scr["part2"]
         ones = torch.ones(B, C, **f32)
@@ -1311,13 +1371,16 @@ class Engine:
             blk, adj = Wt.blocks[i], A["blocks"][i]
             tables = plan.tables(i)
             last_unsaved = i == L - 1 and saved.head is None
-            if saved.full:
-                bk = saved.blocks[i]
-                saved.blocks[i] = None                      # released when this block's adjoint is done
+            bk = saved.blocks.pop(i, None)                  # kept whole: released when this block's adjoint is done
+            if bk is not None:
                 h, s, v, sp, vp, fp, pp, mp, x1 = (bk[k] for k in ("h", "s", "v", "sp", "vp", "fp", "pp", "mp", "x1"))
                 gates, gn = bk["gates"], bk["gn"]
                 v1y = bk["v1y"] if bk["v1y"] is not None else scr["v1y"]
+                chunk, gates_c = 0, None
             else:
+                wk = working()
+                s, v, sp, vp, fp, pp, mp, x1 = (wk[k] for k in ("s", "v", "sp", "vp", "fp", "pp", "mp", "x1"))
+                tbuf, gates, chunk, gates_c = wk["tbuf"], wk["gates"], wk["chunk"], wk["gates_c"]
                 base = (i // stride) * stride
                 prev = _asm(None, saved.inputs[base], ones, zeros)
                 if i == base:

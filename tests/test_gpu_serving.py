@@ -165,3 +165,20 @@ def test_serving_trace_tiles_with_the_benchmark_model():
     assert err <= 1e-3
     assert ferr <= 1e-2
     assert zerr <= 1e-2
+
+
+def test_pinned_stage_round_trips_in_chunks():
+    """The workers' pinned double buffer moves stacked arrays to the device and results back bit for bit, for arrays
+    several chunks long and not a multiple of the chunk, and for a non-contiguous device tensor."""
+    from ffnoml_amd.staging import PinnedStage
+    rng = np.random.default_rng(3)
+    stage = PinnedStage(chunk_bytes=1 << 20)
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        for shape in [(3, 5, 7), (6, 37, 41, 29), (12, 64, 33)]:
+            arrays = [rng.standard_normal(shape).astype(np.float32) for _ in range(3)]
+            t = stage.to_device(arrays, torch.device("cuda:0"), stream)
+            assert torch.equal(t.cpu(), torch.from_numpy(np.stack(arrays)))
+            back = stage.to_host(t.permute(0, *range(t.dim() - 1, 0, -1)), stream)
+            assert np.array_equal(back, np.stack(arrays).transpose(0, *range(len(shape), 0, -1)))
+    stream.synchronize()

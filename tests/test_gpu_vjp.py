@@ -264,9 +264,10 @@ def test_chunked_fusion_adjoint_matches_two_pass():
 
 
 def test_adjoint_keeping_every_intermediate_matches_recompute():
-    """The adjoint that reads every intermediate the forward kept (no recompute) against the one that recomputes
-    each block from its saved input, both with the two-pass fusion adjoint: bit-identical x_bar and z_bar (the
-    same kernels on the same data, only where the data waited differs)."""
+    """The adjoint that reads every intermediate the forward kept (no recompute), and the ones that keep the last one
+    or two blocks whole and recompute the others, against the one that recomputes each block from its saved input,
+    all with the two-pass fusion adjoint: bit-identical x_bar and z_bar (the same kernels on the same data, only
+    where the data waited differs)."""
     data, config, state = load("case_b_jitter64")
     config = dict(config, n_layers=3, width=32, hidden_spectral=32, hidden_vertical=32)
     from ffnoml_amd import synthetic
@@ -277,15 +278,50 @@ def test_adjoint_keeping_every_intermediate_matches_recompute():
     eng = model.engine()
     eng.two_pass_fuse = True
     out = {}
-    for stride in (0, 1):
-        y, saved = eng.forward_saved(x, z, 48e3, 48e3, stride=stride)
-        assert saved.full == (stride == 0)
-        out[stride] = (y,) + tuple(eng.vjp(x, z, 48e3, 48e3, ct, saved, want_z=True))
+    for mode in ("full", "stride1", "keep1", "keep2"):
+        stride, keep = (0, 0) if mode == "full" else (1, int(mode[4:]) if mode.startswith("keep") else 0)
+        y, saved = eng.forward_saved(x, z, 48e3, 48e3, stride=stride, keep=keep)
+        assert saved.full == (mode == "full") and len(saved.blocks) == (3 if mode == "full" else keep)
+        out[mode] = (y,) + tuple(eng.vjp(x, z, 48e3, 48e3, ct, saved, want_z=True))
+        assert not saved.blocks                          # each kept block released by its own adjoint
         torch.cuda.synchronize()
-    for a, b, name in zip(out[0], out[1], ("y", "x_bar", "z_bar")):
-        err = float((a - b).abs().max() / b.abs().max())
-        print(f"full vs recompute {name} rel {err:.2e}", flush=True)
-        assert err == 0.0
+    for mode in ("full", "keep1", "keep2"):
+        for a, b, name in zip(out[mode], out["stride1"], ("y", "x_bar", "z_bar")):
+            err = float((a - b).abs().max() / b.abs().max())
+            print(f"{mode} vs recompute {name} rel {err:.2e}", flush=True)
+            assert err == 0.0
+
+
+def test_adjoint_memory_within_admission_footprint():
+    """The device memory one forward-and-adjoint allocates beyond what was allocated before it (plans and tables
+    already built by a first call) stays within Engine.footprint_bytes, the serving admission's count, for the
+    stride-1 adjoint, for 3 of 8 blocks kept whole and for every intermediate kept."""
+    config = dict(in_channels=6, out_channels=12, width=64, n_layers=8, hidden_spectral=128, hidden_vertical=128,
+                  freq_mix_rank=8, z_reference_length=40.0, z_min_wavelength=1.28, z_feature_wavelength=0.3)
+    from ffnoml_amd import synthetic
+    from ffnoml_amd.atoms import species
+    model = synthetic.random_model(4, species=species(["H", "CA"]), **config).to(DEV).eval()
+    B, D, H, W = 1, 48, 64, 64
+    x, z, mx, sx = synthetic.atmosphere(B, D, H, W, seed=2)
+    x, z = x.to(DEV), z.to(DEV)
+    ct = synthetic.cotangent((B, 12, D, H, W), 1).to(DEV)
+    eng = model.engine()
+    span = float((z.amax(1) - z.amin(1)).max())
+    for stride, keep in ((1, 0), (1, 3), (0, 0)):
+        for rep in range(2):                             # the first call builds the plan's tables
+            torch.cuda.synchronize()
+            base = torch.cuda.memory_allocated()
+            torch.cuda.reset_peak_memory_stats()
+            y, saved = eng.forward_saved(x, z, 48e3, 48e3, stride=stride, keep=keep)
+            xb, zb = eng.vjp(x, z, 48e3, 48e3, ct, saved, want_z=True)
+            del saved
+            torch.cuda.synchronize()
+            peak = torch.cuda.max_memory_allocated() - base
+            del y, xb, zb
+        need = eng.footprint_bytes(B, D, H, W, vjp=True, span=span, full=stride == 0, keep=keep)
+        print(f"stride {stride} keep {keep}: peak {peak / 2**20:.0f} MiB, footprint {need / 2**20:.0f} MiB",
+              flush=True)
+        assert peak <= need
 
 
 def test_factorised_transforms_match_oracle():

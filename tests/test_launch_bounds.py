@@ -141,8 +141,9 @@ def test_no_float64_arithmetic_on_the_device(percol, monkeypatch):
 
 
 def test_footprint_counts_the_vertical_modes_of_tall_columns():
-    """Admission's footprint of a VJP covers the adjoint's own count (Plan.vjp_bytes at stride 1, two-pass fusion)
-    also when twice the column span exceeds the reference length and the box holds more vertical modes."""
+    """Admission's footprint of a VJP covers the adjoint's own count (Plan.vjp_bytes at stride 1, two-pass fusion;
+    Plan.full_bytes for the adjoint that keeps every intermediate, less the plan's cached tables) also when twice the
+    column span exceeds the reference length and the box holds more vertical modes."""
     import numpy as np
     import launch_bounds as LB
     from ffnoml_amd import engine as E
@@ -156,6 +157,12 @@ def test_footprint_counts_the_vertical_modes_of_tall_columns():
             need = eng.footprint_bytes(1, D, H, W, vjp=True, span=span)
             assert plan.modes == eng.vertical_modes(span)
             assert need >= plan.vjp_bytes(1, two_pass=True), (span, need, plan.vjp_bytes(1, two_pass=True))
+            # the adjoint that keeps every intermediate, which admission grants when it fits
+            full = eng.footprint_bytes(1, D, H, W, vjp=True, span=span, full=True)
+            assert eng.footprint_bytes(1, D, H, W, vjp=True, span=span, keep=2) == full     # 2 layers: all kept
+            assert need < eng.footprint_bytes(1, D, H, W, vjp=True, span=span, keep=1) < full
+            tables = 2 * plan.uncached_table_bytes()
+            assert full >= plan.full_bytes(want_z=True) - tables, (span, full, plan.full_bytes(want_z=True))
         assert eng.vertical_modes(30.0) > eng.vertical_modes(15.0)
     finally:
         restore()

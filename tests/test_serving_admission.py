@@ -16,8 +16,8 @@ class _Engine:
         self.full_save = None
         self.dropped = 0
 
-    def footprint_bytes(self, B, nz, nx, ny, vjp=False, jvp=False, span=0.0):
-        return self.need * B
+    def footprint_bytes(self, B, nz, nx, ny, vjp=False, jvp=False, span=0.0, keep=0):
+        return self.need * B * (1 + keep)
 
     def cached_bytes(self):
         return self.cached
@@ -55,8 +55,8 @@ def _request():
 
 @pytest.mark.parametrize("droppable", [True, False])
 def test_a_lone_request_is_not_left_waiting(droppable):
-    engine = _Engine(need=80, cached=50, droppable=droppable)
-    batcher = ContinuousBatcher({"m": _Backend(engine)}, slots=2, graphs=False, budget=100)
+    engine = _Engine(need=80_000, cached=50_000, droppable=droppable)
+    batcher = ContinuousBatcher({"m": _Backend(engine)}, slots=2, graphs=False, budget=100_000)
     try:
         f, z = _request()
         fut = batcher.submit("PREDICT", "m", f, z, 48e3, 48e3)
@@ -68,7 +68,7 @@ def test_a_lone_request_is_not_left_waiting(droppable):
                 fut.result(timeout=10)
         assert engine.dropped >= 1
         # a request larger than the whole budget is refused at once
-        engine.need = 200
+        engine.need = 200_000
         with pytest.raises(MemoryError):
             batcher.submit("PREDICT", "m", f, z, 48e3, 48e3).result(timeout=10)
     finally:
@@ -76,8 +76,8 @@ def test_a_lone_request_is_not_left_waiting(droppable):
 
 
 def test_scheduler_error_fails_waiting_requests():
-    engine = _Engine(need=10, cached=0, droppable=True)
-    batcher = ContinuousBatcher({"m": _Backend(engine)}, slots=1, graphs=False, budget=100)
+    engine = _Engine(need=10_000, cached=0, droppable=True)
+    batcher = ContinuousBatcher({"m": _Backend(engine)}, slots=1, graphs=False, budget=100_000)
     try:
         def broken():
             raise RuntimeError("cache accounting failed")

@@ -97,15 +97,15 @@ class _OracleEngine:
     def forward(self, x, z, dx, dy, plan=None):
         return self._f(x, z, dx, dy)
 
-    def forward_saved(self, x, z, dx, dy, want_z=True):
+    def forward_saved(self, x, z, dx, dy, plan=None, stride=None, want_z=True, keep=0):
         return self._f(x, z, dx, dy), None
 
-    def vjp(self, x, z, dx, dy, ybar, saved, want_z=False):
+    def vjp(self, x, z, dx, dy, ybar, saved, want_z=False, plan=None):
         xg, zg = x.detach().requires_grad_(True), z.detach().requires_grad_(True)
         xb, zb = torch.autograd.grad(self._f(xg, zg, dx, dy), (xg, zg), grad_outputs=ybar)
         return xb, (zb if want_z else None)
 
-    def jvp(self, x, z, dx, dy, xdot):
+    def jvp(self, x, z, dx, dy, xdot, plan=None):
         return torch.func.jvp(lambda t: self._f(t, z, dx, dy), (x,), (xdot,))
 
 
