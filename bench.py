@@ -363,6 +363,8 @@ def production_leg(dev, log):
         res[name + "_ms"] = round(t, 1)
         res[name + "_mvox_s"] = round(voxels / (t * 1e-3) / 1e6, 3)
         log(f"production width: {name} {t:.0f} ms")
+    res["checkpoint_stride"] = model.engine().last_stride
+    res["blocks_kept_whole"] = model.engine().last_keep
     del model, x, z, ct
     torch.cuda.empty_cache()
     return res
@@ -750,6 +752,7 @@ def volume_leg(model, dev):
     res["fwd_vjp_mvox_s"] = round(voxels / (ms * 1e-3) / 1e6, 3)
     res["fwd_vjp_peak_hbm_gb"] = round(peak / 1e9, 2)
     res["checkpoint_stride"] = model.engine().last_stride
+    res["blocks_kept_whole"] = model.engine().last_keep
     log(f"fwd+vjp {ms:.0f} ms, peak {peak / 1e9:.1f} GB, stride {res['checkpoint_stride']}")
     # again, as an inversion loop calls it: the caching allocator now holds the adjoint's buffers (the first call
     # also pays for allocating them from nearly full HBM)

@@ -539,6 +539,16 @@ class Plan:
         return (self.unit_bytes() * (11 * self.n_layers + 3) + (self.n_layers if want_z else 0) * v1 +
                 self.scratch_bytes() + 2 * self.uncached_table_bytes())
 
+    def keep_bytes(self, keep, want_z=True):
+        """The stride-1 two-pass adjoint with the last `keep` blocks kept whole: the larger of its own count and the
+        L - keep saved inputs, 11 units per kept block (and its vertical spectrum with want_z) and four more."""
+        if keep >= self.n_layers:
+            return self.full_bytes(want_z)
+        v1 = self.B * 2 * self.modes * self.H * self.W * self.C * 4
+        units = max(self.n_layers + 1 + WORKING_UNITS + GATE_UNITS, self.n_layers - keep + 11 * keep + 4)
+        return (self.unit_bytes() * units + (keep if want_z else 0) * v1 + self.scratch_bytes() +
+                2 * self.uncached_table_bytes())
+
     def vjp_bytes(self, stride, two_pass=False):
         saved = -(-self.n_layers // stride) + (1 if stride == 1 else 0) + (stride - 1)
         units = saved + WORKING_UNITS + (GATE_UNITS if two_pass else 0)
@@ -590,6 +600,7 @@ class Engine:
         self.fuse_chunk = None              # voxels per chunk of the one-sample fusion adjoint without a block's gate
         #                                     buffer (None: FUSE_CHUNK_BYTES of gates; False: the one-pass kernel)
         self.last_stride = None
+        self.last_keep = None
         self._modes = collections.OrderedDict()     # per-mode table sets shared by plans, least recently used first
 
     def vertical_modes(self, span=0.0):
@@ -744,27 +755,30 @@ class Engine:
         return p
 
     def choose_checkpoints(self, plan, stride=None, want_z=True):
-        """(stride, two_pass, full) of the adjoint. stride 0 asks for every intermediate kept (full). Otherwise the
-        configured (or given) stride, else: full when it fits the HBM that is free now (the caching allocator's
-        unused reserve included), then stride 1 with the two-pass fusion adjoint, then the smallest stride that
-        fits."""
+        """(stride, two_pass, full, keep) of the adjoint. stride 0 asks for every intermediate kept (full).
+        Otherwise the configured (or given) stride, else: full when it fits the HBM that is free now (the caching
+        allocator's unused reserve included), then stride 1 with the two-pass fusion adjoint and as many of the last
+        blocks kept whole as fit, then the smallest stride that fits."""
         stride = self.checkpoint_stride if stride is None else stride
         if stride == 0:
-            return 1, True, True
+            return 1, True, True, 0
         if stride and self.two_pass_fuse is not None:
-            return int(stride), bool(self.two_pass_fuse) and int(stride) == 1, False
+            return int(stride), bool(self.two_pass_fuse) and int(stride) == 1, False, 0
         free, _ = torch.cuda.mem_get_info(plan.device)
         free += torch.cuda.memory_reserved(plan.device) - torch.cuda.memory_allocated(plan.device)
         budget = 0.97 * free
         if stride:
-            return int(stride), int(stride) == 1 and plan.vjp_bytes(1, True) <= budget, False
+            return int(stride), int(stride) == 1 and plan.vjp_bytes(1, True) <= budget, False, 0
         if self.full_save is not False and self.two_pass_fuse is not False and plan.full_bytes(want_z) <= budget:
-            return 1, True, True
+            return 1, True, True, 0
         if self.two_pass_fuse is not False and plan.vjp_bytes(1, True) <= budget:
-            return 1, True, False
+            keep = 0
+            if self.full_save is not False:
+                keep = next((k for k in range(plan.n_layers - 1, 0, -1) if plan.keep_bytes(k, want_z) <= budget), 0)
+            return 1, True, False, keep
         for s in range(1, plan.n_layers + 1):
             if plan.vjp_bytes(s) <= budget:
-                return s, False, False
+                return s, False, False, 0
         raise MemoryError(f"the adjoint of a [{plan.B}, {plan.D}, {plan.H}, {plan.W}] request needs "
                           f"{plan.vjp_bytes(plan.n_layers) / 2**30:.1f} GiB even with one saved input")
 
@@ -1274,9 +1288,11 @@ class Engine:
             keep = min(int(keep), plan.n_layers)
             saved = Saved(1, True, keep == plan.n_layers, keep=keep, n_layers=plan.n_layers)
         else:
-            saved = Saved(*self.choose_checkpoints(plan, stride, want_z))
+            stride, two_pass, full, keep = self.choose_checkpoints(plan, stride, want_z)
+            saved = Saved(stride, two_pass, full, keep=keep, n_layers=plan.n_layers)
         saved.want_z = want_z
         self.last_stride = 0 if saved.full else saved.stride     # informational (the latest saving forward's)
+        self.last_keep = 0 if saved.full_from is None else plan.n_layers - saved.full_from   # blocks kept whole
         return self.forward(x, z, dx, dy, plan=plan, saved=saved), saved
 
     # ------------------------------------------------------------------------------------------------ adjoint
