@@ -130,6 +130,8 @@ def main():
     ap.add_argument("--workload", choices=("benchmark", "production"), default="benchmark",
                     help="the timed step's configuration: the benchmark's (width 64, 32 vertical modes; the line's "
                          "metric) or, for profiling, the production width (96, 1001 vertical modes) on the same tile")
+    ap.add_argument("--only-volume", action="store_true",
+                    help="run the whole-volume leg alone (for profiling it) and print its record")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -140,7 +142,7 @@ def main():
         pmc_child()
         return
     # the counter runs go first, as child processes, before this process touches the GPU
-    counters = None if args.no_traffic else measure_counters()
+    counters = None if args.no_traffic or args.only_volume else measure_counters()
 
     import torch
     from ffnoml_amd import _lib
@@ -149,6 +151,10 @@ def main():
         print(f"bench: {msg}", file=sys.stderr, flush=True)
 
     dev = torch.device("cuda:0")
+    if args.only_volume:
+        model = make(WORKLOAD, dev)[0]
+        print(json.dumps({"whole_volume": volume_leg(model, dev)}), flush=True)
+        return
     copy_cal = copy_calibration(dev) if args.copy_calibration else None
     w = WORKLOAD if args.workload == "benchmark" else PRODUCTION
     voxels = w["B"] * w["D"] * w["H"] * w["W"]
