@@ -86,7 +86,7 @@ SIGNATURES = {
                      c_ptr, c_ptr],
     "ffno_fuse_bwd": [FfnoRows, c_ptr, FfnoGnBack] + [c_ptr] * 18 + [c_ptr],
     "ffno_fuse_mlp_bwd": [FfnoRows, c_ptr, FfnoGnBack] + [c_ptr] * 12 + [c_ptr],
-    "ffno_fuse_gates_bwd": [FfnoRows] + [c_ptr] * 14 + [c_ptr],
+    "ffno_fuse_gates_bwd": [FfnoRows] + [c_ptr] * 14 + [c_ptr, c_ptr, c_ptr, c_ptr] + [c_ptr],
     "ffno_gates_bwd": [FfnoRows] + [c_ptr] * 15 + [c_ptr],
     "ffno_head_bwd": [FfnoRows] + [c_ptr] * 7 + [ctypes.POINTER(FfnoHeadDesc)] + [c_ptr] * 3 + [c_ptr],
     "ffno_lift_bwd": [FfnoRows, c_ptr, FfnoGnBack, c_ptr, c_ptr, c_ptr],

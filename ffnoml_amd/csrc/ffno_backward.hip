@@ -593,7 +593,8 @@ __global__ __launch_bounds__(NW * 64) void k_fuse_mlp_bwd(FfnoRows rows, const f
 // pass 2: through the gates and the gate network (hidden layer recomputed):
 // gp = [sm_bar s, vm_bar v] gt (1 - gt); [h_bar', s_bar', v_bar'] = B1^T (gelu'(B1 [h, s, v]) * B2^T gp);
 // h_bar = x1_bar + h_bar', s_bar = sm_bar gs + s_bar', v_bar = vm_bar gv + v_bar'. In place: hbar over x1bar,
-// sbar over smbar, vbar over vmbar.
+// sbar over smbar, vbar over vmbar. With part_s and part_v, also the gradient sums of the two branch GroupNorms
+// (descriptors gs, gv; their u_bar are the s_bar and v_bar written here), so that no pass of their own re-reads them.
 template <int CT, int NW, bool LW>
 __global__ __launch_bounds__(NW * 64) void k_fuse_gates_bwd(FfnoRows rows, float* xh_bar, const float* __restrict__ h,
                                                             const float* __restrict__ sp, const float* __restrict__ vp,
@@ -601,7 +602,8 @@ __global__ __launch_bounds__(NW * 64) void k_fuse_gates_bwd(FfnoRows rows, float
                                                             const float* __restrict__ v_a, const float* __restrict__ v_b,
                                                             const float* __restrict__ gates, const uint16_t* bg1,
                                                             const uint16_t* bg2t, const uint16_t* bg1t, float* sv_s,
-                                                            float* sv_v) {
+                                                            float* sv_v, FfnoGnBack gs, float* part_s, FfnoGnBack gv,
+                                                            float* part_v) {
   constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32;
   extern __shared__ __align__(16) unsigned char smem[];
   // LDS: kc[4][C] first (s_a, s_b, v_a, v_b of this batch item), within reach of 16-bit immediate offsets; then the
@@ -611,6 +613,9 @@ __global__ __launch_bounds__(NW * 64) void k_fuse_gates_bwd(FfnoRows rows, float
   const Wm B1 = LW ? stage_w(lds, bg1, C * wld(3 * KP)) : global_w(bg1, C * wld(3 * KP));
   const Wm B2T = LW ? stage_w(lds, bg2t, C * wld(2 * C)) : global_w(bg2t, C * wld(2 * C));
   const Wm B1T = LW ? stage_w(lds, bg1t, 3 * C * wld(KP)) : global_w(bg1t, 3 * C * wld(KP));
+  // the fused sums' slots after the weights (two GroupNorms)
+  float* red_s = reinterpret_cast<float*>(lds);
+  float* red_v = red_s + NW * 4 * GradSums<CT>::SLOT;
   Lane L;
   const int b = blockIdx.y;
   for (int c = threadIdx.x; c < C; c += blockDim.x) {
@@ -619,7 +624,15 @@ __global__ __launch_bounds__(NW * 64) void k_fuse_gates_bwd(FfnoRows rows, float
     kc[2 * C + c] = v_a[b * C + c];
     kc[3 * C + c] = v_b[b * C + c];
   }
-  __syncthreads();
+  const bool sums = part_s != nullptr;
+  const float sscale = sums && gs.scale ? *gs.scale : 1.f, vscale = sums && gv.scale ? *gv.scale : 1.f;
+  GradSums<CT> sum_s, sum_v;
+  if (sums) {
+    sum_s.template init<NW>(red_s, L.q, L.wave);
+    sum_v.template init<NW>(red_v, L.q, L.wave);      // its barrier also publishes kc
+  } else {
+    __syncthreads();
+  }
   const int64_t nt = (rows.voxels + 15) / 16;
   for (int64_t tile = (int64_t)blockIdx.x * NW + L.wave; tile < nt; tile += (int64_t)gridDim.x * NW) {
     const int64_t vox = tile * 16 + L.v;
@@ -670,6 +683,16 @@ __global__ __launch_bounds__(NW * 64) void k_fuse_gates_bwd(FfnoRows rows, float
     store_act<CT>(xh_bar, row, ok, L.q, x1b);
     store_act<CT>(sv_s, row, ok, L.q, *reinterpret_cast<f32x4(*)[CT]>(cb + CT));
     store_act<CT>(sv_v, row, ok, L.q, *reinterpret_cast<f32x4(*)[CT]>(cb + 2 * CT));
+    if (sums) {
+      // the GroupNorm inputs sp, vp are re-read from cache (this tile loaded them above)
+      sum_s.add(gs, sscale, row, b, ok, L.q, L.v, *reinterpret_cast<f32x4(*)[CT]>(cb + CT));
+      sum_v.add(gv, vscale, row, b, ok, L.q, L.v, *reinterpret_cast<f32x4(*)[CT]>(cb + 2 * CT));
+    }
+  }
+  if (sums) {
+    const int64_t slot = ((int64_t)b * gridDim.x + blockIdx.x) * G * 2;
+    sum_s.template flush<NW>(red_s, part_s + slot);
+    sum_v.template flush<NW>(red_v, part_v + slot);
   }
 }
 
@@ -1046,21 +1069,25 @@ int ffno_fuse_mlp_bwd(FfnoRows rows, const float* x1bar, FfnoGnBack gnf, const f
 int ffno_fuse_gates_bwd(FfnoRows rows, float* x1bar_hbar, const float* h, const float* sp, const float* vp,
                         const float* s_a, const float* s_b, const float* v_a, const float* v_b, const float* gates,
                         const uint16_t* bg1, const uint16_t* bg2t, const uint16_t* bg1t, float* sm_sbar, float* vm_vbar,
+                        const FfnoGnBack* gn_s, float* partials_s, const FfnoGnBack* gn_v, float* partials_v,
                         hipStream_t stream) {
   if (!rows_ok(rows) || !x1bar_hbar || !h || !sp || !vp || !s_a || !s_b || !v_a || !v_b || !gates || !bg1 || !bg2t ||
-      !bg1t || !sm_sbar || !vm_vbar)
+      !bg1t || !sm_sbar || !vm_vbar || (!partials_s != !partials_v) ||
+      (partials_s && (!gn_s || !gn_v || !gb_ok(*gn_s) || !gb_ok(*gn_v))))
     return fail("ffno_fuse_gates_bwd: bad arguments");
+  const FfnoGnBack gs = partials_s ? *gn_s : FfnoGnBack{}, gv = partials_s ? *gn_v : FfnoGnBack{};
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {
     constexpr int C = CT * 16, KP = ks_of(CT) * 32;
     const size_t head = (size_t)4 * C * sizeof(float);
+    const size_t red = partials_s ? 2 * grad_sums_bytes<CT, nw_of<CT>()>() : 0;
     const size_t lds = ((size_t)C * wld(3 * KP) + (size_t)C * wld(2 * C) + (size_t)3 * C * wld(KP)) * 2 * 2;
-    const bool lw = lds + head <= 160 * 1024;
-    const size_t bytes = (lw ? lds : 0) + head;
+    const bool lw = lds + head + red <= 160 * 1024;
+    const size_t bytes = (lw ? lds : 0) + head + red;
     auto launch = [&](auto k) -> int {
       if (set_lds(k, bytes)) return -1;
       k<<<grid, nw_of<CT>() * 64, bytes, stream>>>(rows, x1bar_hbar, h, sp, vp, s_a, s_b, v_a, v_b, gates, bg1, bg2t,
-                                                   bg1t, sm_sbar, vm_vbar);
+                                                   bg1t, sm_sbar, vm_vbar, gs, partials_s, gv, partials_v);
       return 0;
     };
     if (lw ? launch(k_fuse_gates_bwd<CT, nw_of<CT>(), true>) : launch(k_fuse_gates_bwd<CT, nw_of<CT>(), false>))

@@ -896,8 +896,9 @@ class Engine:
 
     def horizontal(self, plan, xs, out, i, scr, tables, adjoint=False):
         """rfft2 -> mix -> irfft2 of xs [B,D,H,W,C] into out (models/ffno_model.py:441-448); with adjoint, the
-        transpose of that linear map (W_fwd^T H_fwd^T Mix^T H_inv^T W_inv^T). (b, d) planes are independent: they
-        go through in chunks of plan.h_chunk, each chunk's five launches back to back. out may be xs."""
+        transpose of that linear map (W_fwd^T H_fwd^T Mix^T H_inv^T W_inv^T), tables then being the adjoint's.
+        (b, d) planes are independent: they go through in chunks of plan.h_chunk, each chunk's five launches back to
+        back. out may be xs."""
         B, D, H, W, C, wf = plan.B, plan.D, plan.H, plan.W, plan.C, plan.wf
         plane = plan.plane
         hwc = H * W * C
@@ -906,7 +907,6 @@ class Engine:
             ta = plan.adjoint_tables()
             m1, m2, m3, m4 = ta["w_inv_T"], ta["h_inv_T"], ta["h_fwd_T"], ta["w_fwd_T"]
             wc = self.weights.adjoint()["blocks"][i]["spec.wc"]
-            tables = plan.tables(i, adjoint=True)
         else:
             m1, m2, m3, m4 = plan.tw["w_fwd"], plan.tw["h_fwd"], plan.tw["h_inv"], plan.tw["w_inv"]
             wc = self.weights.blocks[i]["spec.wc"]
@@ -971,7 +971,8 @@ class Engine:
 
     def vertical(self, plan, xv, v1, out, i, tables, adjoint=False):
         """Filon transform along D -> mix -> synthesis at the levels, /L folded in (models/ffno_model.py:270-305);
-        with adjoint, its transpose. The spectrum after the mix stays in v1. out may be xv."""
+        with adjoint, its transpose (tables then being the adjoint's). The spectrum after the mix stays in v1. out may
+        be xv."""
         B, D, C, M = plan.B, plan.D, plan.C, plan.modes
         hwc = plan.H * plan.W * C
         big = 1 << 40
@@ -981,7 +982,6 @@ class Engine:
                 ca = plan.column_adjoint()
                 m1, m2 = ca["v_inv_T"], ca["v_fwd_T"]
                 wc = self.weights.adjoint()["blocks"][i]["vertical.wc"]
-                tables = plan.tables(i, adjoint=True)
             else:
                 m1, m2 = plan.colmats["v_fwd"], plan.colmats["v_inv"]
                 wc = self.weights.blocks[i]["vertical.wc"]
@@ -994,7 +994,6 @@ class Engine:
             ta = plan.adjoint_tables()
             m1, m2 = ta["v_inv_T"], ta["v_fwd_T"]
             wc = self.weights.adjoint()["blocks"][i]["vertical.wc"]
-            tables = plan.tables(i, adjoint=True)
         else:
             m1, m2 = plan.tw["v_fwd"], plan.tw["v_inv"]
             wc = self.weights.blocks[i]["vertical.wc"]
@@ -1385,7 +1384,6 @@ class Engine:
             head_bwd(saved.head)
         for i in reversed(range(L)):
             blk, adj = Wt.blocks[i], A["blocks"][i]
-            tables = plan.tables(i)
             last_unsaved = i == L - 1 and saved.head is None
             bk = saved.blocks.pop(i, None)                  # kept whole: released when this block's adjoint is done
             if bk is not None:
@@ -1441,20 +1439,27 @@ class Engine:
             sa, sb_, _ = gn["s"]
             va, vb, _ = gn["v"]
             if gates is not None:
-                # two LDS-staged passes over the stored gates: [sm_bar, vm_bar] -> mp, pp; then in place
+                # two LDS-staged passes over the stored gates: [sm_bar, vm_bar] -> mp, pp; then in place, the second
+                # also accumulating the branch GroupNorms' gradient sums (their u_bar are the s_bar, v_bar it writes)
                 _lib.call("ffno_fuse_mlp_bwd", rows, _p(g), gb_f, _p(sp), _p(vp), _p(sa), _p(sb_), _p(va), _p(vb),
                           _p(gates), _p(blk["f1"]), _p(adj["f2t"]), _p(adj["f1t"]), _p(mp), _p(pp), st)
                 _lib.call("ffno_fuse_gates_bwd", rows, _p(g), _p(h), _p(sp), _p(vp), _p(sa), _p(sb_), _p(va), _p(vb),
-                          _p(gates), _p(blk["bg1"]), _p(adj["bg2t"]), _p(adj["bg1t"]), _p(mp), _p(pp), st)
-            elif gates_c is not None:
-                self._fuse_bwd_chunked(plan, chunk, gates_c, blk, adj, gb_f, g, h, sp, vp, fp, mp, pp, x1,
-                                       (sa, sb_, va, vb), wbar, scr, st)
+                          _p(gates), _p(blk["bg1"]), _p(adj["bg2t"]), _p(adj["bg1t"]), _p(mp), _p(pp),
+                          ctypes.byref(gb_s), _p(part2), ctypes.byref(gb_v), _p(scr["part3"]), st)
+                keep.append(self._gn_finalize_sums(plan, gb_s, part2))
+                keep.append(self._gn_finalize_sums(plan, gb_v, scr["part3"]))
             else:
-                _lib.call("ffno_fuse_bwd", rows, _p(g), gb_f, _p(h), _p(sp), _p(vp), _p(sa), _p(sb_), _p(va), _p(vb),
-                          _p(blk["bg1"]), _p(blk["bg2"]), _p(blk["bg2_bias"]), _p(blk["f1"]), _p(adj["f2t"]),
-                          _p(adj["f1t"]), _p(adj["bg2t"]), _p(adj["bg1t"]), _p(g), _p(mp), _p(pp), st)
-            keep.append(self._gn_sums(plan, gb_s, mp, part2))
-            keep.append(self._gn_sums(plan, gb_v, pp, scr["part3"]))
+                if gates_c is not None:
+                    self._fuse_bwd_chunked(plan, chunk, gates_c, blk, adj, gb_f, g, h, sp, vp, fp, mp, pp, x1,
+                                           (sa, sb_, va, vb), wbar, scr, st)
+                else:
+                    _lib.call("ffno_fuse_bwd", rows, _p(g), gb_f, _p(h), _p(sp), _p(vp), _p(sa), _p(sb_), _p(va),
+                              _p(vb), _p(blk["bg1"]), _p(blk["bg2"]), _p(blk["bg2_bias"]), _p(blk["f1"]),
+                              _p(adj["f2t"]), _p(adj["f1t"]), _p(adj["bg2t"]), _p(adj["bg1t"]), _p(g), _p(mp), _p(pp),
+                              st)
+                keep.append(self._gn_sums(plan, gb_s, mp, part2))
+                keep.append(self._gn_sums(plan, gb_v, pp, scr["part3"]))
+            tables = plan.tables(i, adjoint=True)         # both branches' adjoint tables, built once per block
             # horizontal branch: spec = gelu(GN_s(post(Spec(xs)))); ys_bar -> s, xs_bar -> mp
             _lib.call("ffno_mlp_bwd", rows, _p(mp), gb_s, _asm(s), _p(blk["spec.p1"]), _p(adj["spec.p2t"]),
                       _p(adj["spec.p1t"]), 0, _p(s), None, None, st)
@@ -1531,7 +1536,7 @@ class Engine:
                       _p(row(pp, v0)), st)
             _lib.call("ffno_fuse_gates_bwd", rc, _p(row(g, v0)), _p(row(h, v0)), _p(row(sp, v0)), _p(row(vp, v0)),
                       _p(sa), _p(sb), _p(va), _p(vb), _p(gates_c), _p(blk["bg1"]), _p(adj["bg2t"]), _p(adj["bg1t"]),
-                      _p(row(mp, v0)), _p(row(pp, v0)), st)
+                      _p(row(mp, v0)), _p(row(pp, v0)), None, None, None, None, st)
 
     @staticmethod
     def _height_gradient(plan, z, zbar, zbar_s, wbar, dfac_col):

@@ -257,7 +257,9 @@ int ffno_fuse_bwd(FfnoRows rows, const float* x1bar, FfnoGnBack gnf, const float
 /* The same adjoint in two passes whose weights fit LDS, given the gates the recomputed ffno_fuse stored
  * (models/ffno_model.py:524-545). Pass 1: fp_bar = GN backward of x1bar (GELU after it), then
  * [smbar, vmbar] = 0.5 fp_bar + F1^T (gelu'(F1 [s gs, v gv]) * F2^T fp_bar). Pass 2 (in place): through the gates
- * and the gate network, x1bar -> hbar, smbar -> sbar, vmbar -> vbar. */
+ * and the gate network, x1bar -> hbar, smbar -> sbar, vmbar -> vbar; with partials_s and partials_v (both or
+ * neither), also the gradient sums of the two branch GroupNorms gn_s, gn_v whose u_bar are sbar and vbar, as
+ * ffno_gn_grad_stats would write them (reduce with ffno_gn_grad_finalize). */
 int ffno_fuse_mlp_bwd(FfnoRows rows, const float* x1bar, FfnoGnBack gnf, const float* sp, const float* vp,
                       const float* s_a, const float* s_b, const float* v_a, const float* v_b, const float* gates,
                       const uint16_t* f1, const uint16_t* f2t, const uint16_t* f1t, float* smbar, float* vmbar,
@@ -265,6 +267,7 @@ int ffno_fuse_mlp_bwd(FfnoRows rows, const float* x1bar, FfnoGnBack gnf, const f
 int ffno_fuse_gates_bwd(FfnoRows rows, float* x1bar_hbar, const float* h, const float* sp, const float* vp,
                         const float* s_a, const float* s_b, const float* v_a, const float* v_b, const float* gates,
                         const uint16_t* bg1, const uint16_t* bg2t, const uint16_t* bg1t, float* sm_sbar, float* vm_vbar,
+                        const FfnoGnBack* gn_s, float* partials_s, const FfnoGnBack* gn_v, float* partials_v,
                         hipStream_t stream);
 
 /* Adjoint of both input gates (models/ffno_model.py:266, :439-440): hbar_out = hbar_in + d(xs)/dh^T xs_bar +

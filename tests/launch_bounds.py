@@ -305,7 +305,8 @@ class Checker:
         self.weight(n, "f2t", f2t, 2 * C, C)
         self.span(n, "f1t", f1t, 2 * 2 * C * (2 * C + 8) * BF)
 
-    def ffno_fuse_gates_bwd(self, n, rows, xh, h, sp, vp, sa, sb, va, vb, gates, bg1, bg2t, bg1t, svs, svv, stream):
+    def ffno_fuse_gates_bwd(self, n, rows, xh, h, sp, vp, sa, sb, va, vb, gates, bg1, bg2t, bg1t, svs, svv, gs, ps,
+                            gv, pv, stream):
         C = rows.C
         KP = round_up(C, 32)
         for what, p in (("x1bar/hbar", xh), ("h", h), ("sp", sp), ("vp", vp), ("smbar/sbar", svs),
@@ -317,6 +318,9 @@ class Checker:
         self.span(n, "bg1", bg1, 2 * C * (3 * KP + 8) * BF)
         self.weight(n, "bg2t", bg2t, C, 2 * C)
         self.span(n, "bg1t", bg1t, 2 * 3 * C * (KP + 8) * BF)
+        assert bool(getattr(ps, "value", ps)) == bool(getattr(pv, "value", pv)), f"{n}: one of two sums targets"
+        self.next_sums(n, gs, ps, rows)
+        self.next_sums(n, gv, pv, rows)
 
     def ffno_gates_bwd(self, n, rows, h, xsb, xvb, hin, g1, g2, g2t, g1t, v1, v2, v2t, v1t, z_proj, hout, zbar,
                        stream):

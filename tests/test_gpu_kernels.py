@@ -265,13 +265,13 @@ def test_spectral_adjoints_are_transposes(chunk, H, W, D):
     tx = torch.empty_like(x)
     ty = torch.empty_like(y)
     eng.horizontal(plan, x, tx, 0, scr, tables)
-    eng.horizontal(plan, y, ty, 0, scr, tables, adjoint=True)
+    eng.horizontal(plan, y, ty, 0, scr, plan.tables(0, adjoint=True), adjoint=True)
     lhs, rhs = float((tx.double() * y.double()).sum()), float((x.double() * ty.double()).sum())
     print(f"horizontal chunk {chunk}: {lhs:.8e} vs {rhs:.8e}", flush=True)
     # a transpose that is wrong anywhere is off by about |Tx| |y| / sqrt(n) = 3e-3 |Tx| |y| here
     assert abs(lhs - rhs) <= 1e-5 * float(tx.double().norm() * y.double().norm())
     eng.vertical(plan, x, scr["v1"], tx, 0, tables)
-    eng.vertical(plan, y, scr["v1"], ty, 0, tables, adjoint=True)
+    eng.vertical(plan, y, scr["v1"], ty, 0, plan.tables(0, adjoint=True), adjoint=True)
     lhs, rhs = float((tx.double() * y.double()).sum()), float((x.double() * ty.double()).sum())
     print(f"vertical: {lhs:.8e} vs {rhs:.8e}", flush=True)
     # a transpose that is wrong anywhere is off by about |Tx| |y| / sqrt(n) = 3e-3 |Tx| |y| here
