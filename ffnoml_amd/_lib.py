@@ -99,6 +99,8 @@ SIGNATURES = {
     "ffno_mlp_t": [FfnoRows, FfnoAssemble, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr],
     "ffno_fuse_t": [FfnoRows] + [c_ptr] * 16 + [c_ptr],
     "ffno_head_t": [FfnoRows, FfnoAssemble] + [c_ptr] * 6 + [ctypes.POINTER(FfnoHeadDesc), c_ptr, c_ptr],
+    "ffno_lowrank_tables": [c_i64, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [c_ptr] * 5 +
+                           [ctypes.c_int, ctypes.c_int, c_ptr, c_ptr, c_ptr],
     "ffno_head_output": [ctypes.c_int, c_i64, c_ptr, c_ptr, ctypes.POINTER(FfnoHeadDesc), c_ptr, c_ptr],
     "ffno_head_output_bwd": [ctypes.c_int, c_i64, c_ptr, c_ptr, ctypes.POINTER(FfnoHeadDesc), c_ptr, c_ptr, c_ptr,
                              c_ptr],

@@ -149,12 +149,46 @@ def _freq_mlp(prefix, sd, feats):
     return F.linear(h, sd[prefix + "freq_mlp.4.weight"], sd[prefix + "freq_mlp.4.bias"])
 
 
-def _freq_lowrank(prefix, sd, feats, C, r):
+def _lowrank_hidden(prefix, sd, feats):
     h = _gelu(F.linear(feats, sd[prefix + "freq_lowrank.0.weight"], sd[prefix + "freq_lowrank.0.bias"]))
-    h = _gelu(F.linear(h, sd[prefix + "freq_lowrank.2.weight"], sd[prefix + "freq_lowrank.2.bias"]))
+    return _gelu(F.linear(h, sd[prefix + "freq_lowrank.2.weight"], sd[prefix + "freq_lowrank.2.bias"]))
+
+
+def _freq_lowrank(prefix, sd, feats, C, r):
+    h = _lowrank_hidden(prefix, sd, feats)
     u = F.linear(h, sd[prefix + "freq_u.weight"], sd[prefix + "freq_u.bias"]).reshape(-1, C, r)
     v = F.linear(h, sd[prefix + "freq_v.weight"], sd[prefix + "freq_v.bias"]).reshape(-1, r, C)
     return u, v
+
+
+def lowrank_tables_torch(prefix, sd, feats, C, r, adjoint):
+    """The packed low-rank tables (v, u) of every mode by torch ops (host-side plans; on the GPU the check of
+    ffno_lowrank_tables): u = freq_u(h) [C x r], v = freq_v(h) [r x C] through pack_mlp_weight."""
+    u, v = _freq_lowrank(prefix, sd, feats, C, r)
+    if adjoint:
+        return pack_mlp_weight(u.transpose(-1, -2), n_pad=16), pack_mlp_weight(v.transpose(-1, -2))
+    return pack_mlp_weight(v, n_pad=16, parts=3), pack_mlp_weight(u, parts=3)
+
+
+def _lowrank_tables(prefix, sd, feats, C, r, adjoint):
+    """The packed low-rank tables (v, u) of every mode: the hidden features h(k) of the frequency low-rank network
+    (models/ffno_model.py:404-429) by torch, then u = freq_u(h), v = freq_v(h) and their split-bf16 packing by
+    ffno_lowrank_tables (forward: v [r x C] and u [C x r], three planes; adjoint: u^T and v^T, two planes), the same
+    layouts pack_mlp_weight writes. Plans built in host memory (tests) take the torch form."""
+    if feats.device.type == "cpu":
+        return lowrank_tables_torch(prefix, sd, feats, C, r, adjoint)
+    h = _lowrank_hidden(prefix, sd, feats).float().contiguous()
+    modes, hidden = h.shape
+    parts = 2 if adjoint else 3
+    kc = T.round_up(C, 32)
+    v = torch.zeros(modes, parts, 16, kc + WPAD, dtype=torch.int16, device=h.device)
+    u = torch.zeros(modes, parts, C, 32 + WPAD, dtype=torch.int16, device=h.device)
+    wu = sd[prefix + "freq_u.weight"].float().t().contiguous()          # [hidden, C r]
+    wv = sd[prefix + "freq_v.weight"].float().t().contiguous()          # [hidden, r C]
+    bu, bv = sd[prefix + "freq_u.bias"].float().contiguous(), sd[prefix + "freq_v.bias"].float().contiguous()
+    _lib.call("ffno_lowrank_tables", modes, C, r, hidden, _p(h), _p(wu), _p(bu), _p(wv), _p(bv), parts,
+              1 if adjoint else 0, _p(v), _p(u), Engine._stream())
+    return v, u
 
 
 class PackedWeights:
@@ -295,13 +329,8 @@ class ModeTables:
             gate[:, 1] = -gate[:, 1]
         e = dict(gate=gate.contiguous(), v=None, u=None)
         if self.rank:
-            u, v = _freq_lowrank(prefix, self.sd, self.feats, self.C, self.rank)
-            if adjoint:
-                e["v"] = pack_mlp_weight(u.transpose(-1, -2), n_pad=16)      # [modes, 2, 16, KS*32]
-                e["u"] = pack_mlp_weight(v.transpose(-1, -2))                # [modes, 2, C, 32]
-            else:
-                e["v"] = pack_mlp_weight(v, n_pad=16, parts=3)               # [modes, 3, 16, KS*32]
-                e["u"] = pack_mlp_weight(u, parts=3)                         # [modes, 3, C, 32]
+            # [modes, parts, 16, KS*32 + 8] and [modes, parts, C, 32 + 8]
+            e["v"], e["u"] = _lowrank_tables(prefix, self.sd, self.feats, self.C, self.rank, adjoint)
         return e
 
     def get(self, i, adjoint=False):

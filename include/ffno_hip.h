@@ -203,6 +203,17 @@ typedef struct {
 
 int ffno_mix(const FfnoMix* m, hipStream_t stream);
 
+/* The per-mode low-rank tables ffno_mix reads, from the hidden features hid [modes][hidden] of the frequency low-rank
+ * network (models/ffno_model.py:404-429): u(k) = freq_u(h) as [C][rank] and v(k) = freq_v(h) as [rank][C] in
+ * float32 (wu_t, wv_t: the two Linear weights transposed, [hidden][C rank]; bu, bv their biases), split into
+ * `parts` bf16 planes and packed: forward (adjoint = 0, parts 3) v_out [modes][parts][16][KC + 8] from v and
+ * u_out [modes][parts][C][40] from u (KC = C rounded up to 32); adjoint (adjoint = 1, parts 2) the same shapes from
+ * u^T and v^T. Padding entries are left as they are (the caller zeroes the outputs). hidden <= 256, a multiple
+ * of 4; rank <= 16. Replaces the table build of SpectralConv2dFull / HeightSpectralConv's mix (:394-429, :285-296). */
+int ffno_lowrank_tables(int64_t modes, int C, int rank, int hidden, const float* hid, const float* wu_t,
+                        const float* bu, const float* wv_t, const float* bv, int parts, int adjoint, uint16_t* v_out,
+                        uint16_t* u_out, hipStream_t stream);
+
 /* ------------------------------------------------------------------------------------------------------------
  * Adjoint (input VJP). The reference differentiates its forward with torch autograd through non-reentrant
  * checkpoints (models/ffno_model.py:751-798; ffno_fsdp_runtime.py:306-348). Here each block is recomputed from its

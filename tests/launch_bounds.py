@@ -322,6 +322,16 @@ class Checker:
         self.next_sums(n, gs, ps, rows)
         self.next_sums(n, gv, pv, rows)
 
+    def ffno_lowrank_tables(self, n, modes, C, r, hidden, hid, wu, bu, wv, bv, parts, adjoint, v, u, stream):
+        kc = round_up(C, 32)
+        self.span(n, "hid", hid, modes * hidden * F32)
+        for what, p in (("wu_t", wu), ("wv_t", wv)):
+            self.span(n, what, p, hidden * C * r * F32)
+        for what, p in (("bu", bu), ("bv", bv)):
+            self.span(n, what, p, C * r * F32)
+        self.span(n, "v_out", v, modes * parts * 16 * (kc + 8) * BF)
+        self.span(n, "u_out", u, modes * parts * C * (32 + 8) * BF)
+
     def ffno_gates_bwd(self, n, rows, h, xsb, xvb, hin, g1, g2, g2t, g1t, v1, v2, v2t, v1t, z_proj, hout, zbar,
                        stream):
         C = rows.C

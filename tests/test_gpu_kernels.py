@@ -276,3 +276,41 @@ def test_spectral_adjoints_are_transposes(chunk, H, W, D):
     print(f"vertical: {lhs:.8e} vs {rhs:.8e}", flush=True)
     # a transpose that is wrong anywhere is off by about |Tx| |y| / sqrt(n) = 3e-3 |Tx| |y| here
     assert abs(lhs - rhs) <= 1e-5 * float(tx.double().norm() * y.double().norm())
+
+
+@pytest.mark.parametrize("adjoint", [False, True])
+@pytest.mark.parametrize("C,r,modes", [(64, 8, 2112), (96, 8, 70), (32, 4, 33)])
+def test_lowrank_tables_match_torch_packing(C, r, modes, adjoint):
+    """ffno_lowrank_tables (u = freq_u(h), v = freq_v(h), split and packed on the device) against the same tables by
+    torch ops (F.linear, then pack_mlp_weight): the planes' sum within 2e-6 of the table's range with three planes
+    (float32 rounding of a 128-term dot product in another order), 2^-15 with two (one step of the second plane,
+    where the two float values straddle its rounding point), with three planes (an exact split) every plane equal
+    wherever the two float values are equal, every padding entry zero."""
+    from ffnoml_amd import engine as E
+    torch.manual_seed(C + r + modes)
+    hidden = 128
+    prefix = "blocks.0.spec."
+    sd = {prefix + "freq_lowrank.0.weight": torch.randn(hidden, 4, device=DEV) * 0.5,
+          prefix + "freq_lowrank.0.bias": torch.randn(hidden, device=DEV) * 0.1,
+          prefix + "freq_lowrank.2.weight": torch.randn(hidden, hidden, device=DEV) / math.sqrt(hidden),
+          prefix + "freq_lowrank.2.bias": torch.randn(hidden, device=DEV) * 0.1,
+          prefix + "freq_u.weight": torch.randn(C * r, hidden, device=DEV) / math.sqrt(hidden),
+          prefix + "freq_u.bias": torch.randn(C * r, device=DEV) * 0.1,
+          prefix + "freq_v.weight": torch.randn(r * C, hidden, device=DEV) / math.sqrt(hidden),
+          prefix + "freq_v.bias": torch.randn(r * C, device=DEV) * 0.1}
+    feats = torch.randn(modes, 4, device=DEV)
+    v, u = E._lowrank_tables(prefix, sd, feats, C, r, adjoint)
+    v_ref, u_ref = E.lowrank_tables_torch(prefix, sd, feats, C, r, adjoint)
+    torch.cuda.synchronize()
+    for got, ref in ((v, v_ref), (u, u_ref)):
+        assert got.shape == ref.shape and got.dtype == ref.dtype
+        planes = got.view(torch.bfloat16).float()
+        planes_ref = ref.view(torch.bfloat16).float()
+        val, val_ref = planes.sum(1), planes_ref.sum(1)
+        scale = float(val_ref.abs().max())
+        assert float((val - val_ref).abs().max()) <= (2.0 ** -15 if adjoint else 2e-6) * scale
+        assert bool(((planes_ref == 0).all(1) <= (planes == 0).all(1)).all())       # padding stays zero
+        if not adjoint:
+            same = (val == val_ref)[:, None].expand_as(planes)
+            assert float(same.float().mean()) > 0.5
+            assert bool((planes == planes_ref)[same].all())
