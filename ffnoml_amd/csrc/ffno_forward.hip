@@ -766,19 +766,35 @@ __global__ __launch_bounds__(NWM * 64) void k_mix(FfnoMix mx) {
     // per-mode low-rank tables: V [16][wld(KP)] and U [C][wld(32)], P planes of each mode's block
     const auto V = S::global(mx.v_p ? mx.v_p + (int64_t)mode * P * 16 * wld(KP) : mx.wc, 16 * wld(KP));
     const auto U = S::global(mx.u_p ? mx.u_p + (int64_t)mode * P * C * wld(32) : mx.wc, C * wld(32));
-    for (int rt = rt0; rt < rt1; ++rt) {
+    // row tile rt's offset; its loads are issued one tile ahead, into registers the split has just freed, so that
+    // they are in flight while the previous tile's GEMMs run
+    auto row_of = [&](int rt, bool& ok) {
       const int r = rt * 16 + L.v;
-      const bool ok = r < mx.rows;
-      const int64_t base = imap(ok ? r : 0, mx.r_split, mx.r_hi, mx.r_lo) + (int64_t)mode * mx.mode_stride;
-      f32x4 yr[CT], yi[CT];
+      ok = r < mx.rows;
+      return imap(ok ? r : 0, mx.r_split, mx.r_hi, mx.r_lo) + (int64_t)mode * mx.mode_stride;
+    };
+    f32x4 yr[CT], yi[CT];
+    bool nok;
+    int64_t nbase = row_of(rt0, nok);
 #pragma unroll
-      for (int t = 0; t < CT; ++t) {
-        yr[t] = ld4(mx.re + base + 16 * t + 4 * L.q, ok);
-        yi[t] = ld4(mx.im + base + 16 * t + 4 * L.q, ok);
-      }
+    for (int t = 0; t < CT; ++t) {
+      yr[t] = ld4(mx.re + nbase + 16 * t + 4 * L.q, nok);
+      yi[t] = ld4(mx.im + nbase + 16 * t + 4 * L.q, nok);
+    }
+    for (int rt = rt0; rt < rt1; ++rt) {
+      const bool ok = nok;
+      const int64_t base = nbase;
       F cat[2 * KS];
       to_frags<CT>(yr, cat);
       to_frags<CT>(yi, cat + KS);
+      if (rt + 1 < rt1) {
+        nbase = row_of(rt + 1, nok);
+#pragma unroll
+        for (int t = 0; t < CT; ++t) {
+          yr[t] = ld4(mx.re + nbase + 16 * t + 4 * L.q, nok);
+          yi[t] = ld4(mx.im + nbase + 16 * t + 4 * L.q, nok);
+        }
+      }
       f32x4 zr[CT], zi[CT];
 #pragma unroll
       for (int t = 0; t < CT; ++t) {
