@@ -182,3 +182,82 @@ def test_pinned_stage_round_trips_in_chunks():
             back = stage.to_host(t.permute(0, *range(t.dim() - 1, 0, -1)), stream)
             assert np.array_equal(back, np.stack(arrays).transpose(0, *range(len(shape), 0, -1)))
     stream.synchronize()
+
+
+def test_socket_service_end_to_end():
+    """The protocol-v2 socket front end over the continuous batcher on the GPU: a client that speaks the wire format
+    (HELLO, READY with world >= 2, PREDICT / VJP / JVP with Fortran-order float32 arrays, MEMORY, SHUTDOWN) gets the
+    backend's own answers (1e-5 relative in log10 n; 1e-5 relative for the feature and height gradients and the
+    tangent), two clients at once."""
+    import socket
+    import threading
+    from ffnoml_amd.service import Service, read_array, write_array
+    from ffnoml_amd.serving import ContinuousBatcher
+    backend = _backend()
+    batcher = ContinuousBatcher({"H": backend}, slots=2, max_batch=4)
+    svc = Service(batcher, "tok", host="127.0.0.1", port=0, log=lambda s: None)
+    server = threading.Thread(target=svc.serve_forever, daemon=True)
+    server.start()
+    rng = np.random.default_rng(5)
+    nz, nx, ny = 16, 12, 10
+    L = len(backend.level_names)
+    errors = []
+
+    def client(seed):
+        try:
+            r = np.random.default_rng(seed)
+            f, z = _request(r, nz, nx, ny)
+            cot = r.standard_normal((nz, nx, ny, L)).astype(np.float32) * 1e-12
+            tan = r.standard_normal((6, nz, nx, ny)).astype(np.float32) * f.std(axis=(1, 2, 3))[:, None, None, None]
+            s = socket.create_connection(("127.0.0.1", svc.port), timeout=60)
+            rd = s.makefile("rb", buffering=0)
+            s.sendall(b"HELLO tok\n")
+            ready = rd.readline().decode().split()
+            assert ready[0] == "READY" and int(ready[2]) >= 2
+            s.sendall(f"PREDICT H {nz} {nx} {ny} {L} 48000.0 48000.0\n".encode())
+            write_array(s, f)
+            write_array(s, z)
+            assert rd.readline().decode().split() == ["OK", "PREDICT", str(nz), str(nx), str(ny), str(L)]
+            pops = read_array(s, (nz, nx, ny, L))
+            assert _rel(pops, backend.predict_local(f, z, 48e3, 48e3)) <= 1e-5
+            s.sendall(f"VJP H {nz} {nx} {ny} {L} 48000.0 48000.0\n".encode())
+            for a in (f, z, cot):
+                write_array(s, a)
+            assert rd.readline().decode().split()[:2] == ["OK", "VJP"]
+            fb, zb = read_array(s, (6, nz, nx, ny)), read_array(s, (nz, nx, ny))
+            ref = backend.vjp_local(f, z, 48e3, 48e3, cot)
+            for got, want in ((fb, ref["features"]), (zb, ref["z_scale"])):
+                assert float(np.linalg.norm(got - want) / np.linalg.norm(want)) <= 1e-5
+            s.sendall(f"JVP H {nz} {nx} {ny} {L} 48000.0 48000.0\n".encode())
+            for a in (f, z, tan):
+                write_array(s, a)
+            assert rd.readline().decode().split()[:2] == ["OK", "JVP"]
+            jt = read_array(s, (nz, nx, ny, L))
+            want = backend.jvp_local(f, z, 48e3, 48e3, tan)
+            assert float(np.linalg.norm(jt - want) / np.linalg.norm(want)) <= 1e-5
+            s.sendall(b"MEMORY\n")
+            assert rd.readline().decode().split() == ["OK", "MEMORY", "1"]
+            rd.readline()
+            s.close()
+        except Exception as e:          # reported by the main thread
+            errors.append(e)
+
+    try:
+        threads = [threading.Thread(target=client, args=(seed,)) for seed in (11, 12)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join(timeout=180)
+        assert not errors, errors
+        s = socket.create_connection(("127.0.0.1", svc.port), timeout=60)
+        rd = s.makefile("rb", buffering=0)
+        s.sendall(b"HELLO tok\n")
+        rd.readline()
+        s.sendall(b"SHUTDOWN\n")
+        assert rd.readline() == b"BYE\n"
+        s.close()
+        server.join(timeout=10)
+        assert not server.is_alive()
+    finally:
+        svc.stopping.set()
+        batcher.close()
