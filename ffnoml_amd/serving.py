@@ -28,7 +28,9 @@ Inputs and outputs cross PCIe through pinned double buffers (ffnoml_amd.staging)
 A VJP batch keeps as many blocks whole (every intermediate its adjoint reads, so the block is not recomputed) as
 fit within `full_share` of the budget and within what is unclaimed now: all of them (about a third less device time
 than the checkpointed adjoint), some, or none (checkpoint stride 1: the block inputs kept, each block recomputed
-once).
+once). A VJP batch is cut to the largest size that keeps as many blocks whole as its first request alone would: a
+recomputed block costs a block's forward for every voxel of the batch, more than batching saves once tiles fill
+the GPU.
 
 Forward-only batches of small tiles (forward working set at most `graph_bytes`) replay a captured HIP graph of the
 whole forward: a batch of n requests runs on the graph of its bucket for the next power of two at or above n (the
@@ -322,13 +324,25 @@ class ContinuousBatcher:
                 return False
             stride, keep = None, 0
             if head.op == "VJP":
-                # the most blocks kept whole that fit a share of the budget and what is unclaimed now
                 layers = self.backends[head.model].model.n_layers
-                for k in range(layers, 0, -1):
-                    need = self._footprint(batch, keep=k)
-                    if need <= self.full_share * self.budget and self._claimed(need, io) <= free:
-                        keep, work = k, need
-                        break
+
+                def kept(reqs):
+                    # the most blocks kept whole that fit a share of the budget and what is unclaimed now
+                    for k in range(layers, 0, -1):
+                        need = self._footprint(reqs, keep=k)
+                        if need <= self.full_share * self.budget and self._claimed(need, self._io(reqs)) <= free:
+                            return k, need
+                    return 0, self._footprint(reqs)
+
+                keep, work = kept(batch)
+                if len(batch) > 1 and keep < layers:
+                    # every block not kept whole is recomputed, a forward of that block for every voxel of the batch:
+                    # a batch keeping fewer blocks than the head alone would costs more per voxel than it saves
+                    alone = kept(batch[:1])[0]
+                    while len(batch) > 1 and keep < alone:
+                        batch.pop()
+                        keep, work = kept(batch)
+                    io = self._io(batch)
                 stride = 0 if keep == layers else 1
             for r in batch:
                 self.pending.remove(r)

@@ -2,7 +2,8 @@
 memory the engines' caches leave free, with nothing in flight, is admitted once the caches are dropped -- or failed
 with MemoryError when they cannot be -- instead of waiting for good; a scheduler error fails the waiting requests
 instead of ending the scheduler silently; captured graphs are evicted least recently replayed first, and an evicted
-graph's memory stays counted until its last replay releases it."""
+graph's memory stays counted until its last replay releases it; a VJP batch is no larger than keeps as many blocks
+whole as its first request alone."""
 import numpy as np
 import pytest
 import torch
@@ -110,4 +111,45 @@ def test_graph_eviction_is_lru_and_counts_until_released():
         batcher._release(a_graph)                        # its replay ends: now released
         assert batcher.graph_held == 0
     finally:
+        batcher.close()
+
+
+def test_vjp_batch_keeps_as_many_blocks_whole_as_one_request():
+    """Three same-bucket VJP requests: together they would keep one of four blocks whole (the other three recomputed
+    for every voxel of the batch), one alone keeps all four, two keep three: the scheduler admits the first alone,
+    with every intermediate kept (stride 0). Forward requests of the bucket still batch together."""
+    import queue
+    engine = _Engine(need=10_000, cached=0, droppable=True)
+    backend = _Backend(engine)
+    backend.model.n_layers = 4
+    batcher = ContinuousBatcher({"m": backend}, slots=2, graphs=False, budget=100_000)
+    planner_queue = batcher.plans
+    try:
+        admitted = queue.Queue()
+        with batcher.cv:
+            batcher.plans = admitted                    # the planner stays on the old queue: batches stop here
+            batcher.inflight = batcher.slots            # hold the scheduler until all requests wait
+        f, z = _request()
+        cot = np.zeros(z.shape + (2,), np.float32)
+        for _ in range(3):
+            batcher.submit("VJP", "m", f, z, 48e3, 48e3, cot)
+        with batcher.cv:
+            batcher.inflight = 0
+            batcher.cv.notify_all()
+        b = admitted.get(timeout=10)
+        assert len(b.reqs) == 1 and b.keep == 4 and b.stride == 0
+        with batcher.cv:
+            batcher.inflight = batcher.slots
+        for _ in range(3):
+            batcher.submit("PREDICT", "m", f, z, 48e3, 48e3)
+        with batcher.cv:
+            batcher.inflight = 1                        # one slot free: the next VJP is admitted alone, then no more
+            batcher.cv.notify_all()
+        b = admitted.get(timeout=10)
+        assert len(b.reqs) == 1 and b.reqs[0].op == "VJP"
+    finally:
+        with batcher.cv:                                # what was admitted here never runs: forget it, then close
+            batcher.inflight = 0
+            batcher.pending.clear()
+            batcher.plans = planner_queue
         batcher.close()
