@@ -277,6 +277,28 @@ def main():
                                "unit": "Mvoxels/s", "forwards": nf}
         del yf
         log(f"forward {fms:.1f} ms, {out['forward_only']['value']:.2f} Mvox/s")
+        # the tangent (JVP) at the same shape: forward and directional derivative along a random input tangent, as
+        # the inversion's JVP requests run it (Engine.jvp, the hand-written tangent kernels)
+        from ffnoml_amd import synthetic
+        eng = model.engine()
+        tan = synthetic.perturbation(tuple(x.shape), seed=1).to(dev)
+        with torch.no_grad():
+            for _ in range(2):
+                eng.jvp(x, z, 48e3, 48e3, tan)
+            torch.cuda.synchronize()
+            j0, j1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            nj = max(args.steps, 3)
+            j0.record()
+            for _ in range(nj):
+                eng.jvp(x, z, 48e3, 48e3, tan)
+            j1.record()
+            torch.cuda.synchronize()
+        jms = j0.elapsed_time(j1) / nj
+        out["jvp"] = {"workload": f"FFNO3D forward + JVP (tangent), B=8 D=128 H=W=64 width={w['width']} layers=8",
+                      "ms_per_jvp": round(jms, 3), "value": round(voxels / (jms * 1e-3) / 1e6, 3),
+                      "unit": "Mvoxels/s", "calls": nj}
+        del tan
+        log(f"jvp {jms:.1f} ms, {out['jvp']['value']:.2f} Mvox/s")
 
     if not args.no_baselines:
         from baseline.plain_fp32 import PlainFFNO3D
