@@ -79,15 +79,43 @@ struct Frag {
   bf16x8 h, l;
 };
 
+typedef bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// The bf16 pairs (-1, 0) and (0, -1), made in scalar registers by an instruction the compiler cannot fold: given as
+// an inline constant -1.0, v_dot2c_f32_bf16 applies it to the other half of the pair on gfx950
+struct NegPair {
+  bf16x2 lo, hi;
+  __device__ __forceinline__ NegPair() {
+    unsigned a, b;
+    asm("s_mov_b32 %0, 0xbf80" : "=s"(a));
+    asm("s_mov_b32 %0, 0xbf800000" : "=s"(b));
+    lo = __builtin_bit_cast(bf16x2, a);
+    hi = __builtin_bit_cast(bf16x2, b);
+  }
+};
+
+// p = (bf16(x0), bf16(x1)) (one v_cvt_pk_bf16_f32), and x0, x1 replaced by their remainders x - p, exact in float32,
+// one v_dot2 each: x0 + p . (-1, 0) (the pair's other half enters with weight 0)
+__device__ __forceinline__ bf16x2 split_pair(float& x0, float& x1, const NegPair& n) {
+  bf16x2 p;
+  p[0] = (bf16)x0;
+  p[1] = (bf16)x1;
+  x0 = __builtin_amdgcn_fdot2_f32_bf16(p, n.lo, x0, false);
+  x1 = __builtin_amdgcn_fdot2_f32_bf16(p, n.hi, x1, false);
+  return p;
+}
+
 __device__ __forceinline__ Frag split8(f32x4 a, f32x4 b) {
+  const NegPair n;
+  float x[8] = {a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]};
   Frag f;
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const bf16 ha = (bf16)a[j], hb = (bf16)b[j];
-    f.h[j] = ha;
-    f.h[4 + j] = hb;
-    f.l[j] = (bf16)(a[j] - (float)ha);
-    f.l[4 + j] = (bf16)(b[j] - (float)hb);
+  for (int j = 0; j < 8; j += 2) {
+    const bf16x2 p = split_pair(x[j], x[j + 1], n);
+    f.h[j] = p[0];
+    f.h[j + 1] = p[1];
+    f.l[j] = (bf16)x[j];
+    f.l[j + 1] = (bf16)x[j + 1];
   }
   return f;
 }
@@ -213,16 +241,19 @@ struct Wm3 {
 };
 
 __device__ __forceinline__ Frag3 split8_3(f32x4 a, f32x4 b) {
+  const NegPair n;
+  float x[8] = {a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]};
   Frag3 f;
 #pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const float x = j < 4 ? a[j] : b[j - 4];
-    const bf16 h = (bf16)x;
-    const float r = x - (float)h;              // exact
-    const bf16 m = (bf16)r;
-    f.h[j] = h;
-    f.m[j] = m;
-    f.l[j] = (bf16)(r - (float)m);             // r - m exact
+  for (int j = 0; j < 8; j += 2) {
+    const bf16x2 h = split_pair(x[j], x[j + 1], n);     // x - h, exact
+    const bf16x2 m = split_pair(x[j], x[j + 1], n);     // x - h - m, exact
+    f.h[j] = h[0];
+    f.h[j + 1] = h[1];
+    f.m[j] = m[0];
+    f.m[j + 1] = m[1];
+    f.l[j] = (bf16)x[j];
+    f.l[j + 1] = (bf16)x[j + 1];
   }
   return f;
 }

@@ -437,17 +437,25 @@ struct ColVec<2> {
 template <bool THREE, int NC = 4>
 __device__ __forceinline__ void split_cols(const typename ColVec<NC>::T (&xv)[8], bf16x8 (&bh)[NC], bf16x8 (&bm)[NC],
                                            bf16x8 (&bl)[NC]) {
+  const NegPair n;
 #pragma unroll
   for (int i = 0; i < NC; ++i)
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const float x = xv[j][i];
-      const bf16 h = (bf16)x;
-      const float r = x - (float)h;
-      const bf16 m = (bf16)r;
-      bh[i][j] = h;
-      bm[i][j] = m;
-      if (THREE) bl[i][j] = (bf16)(r - (float)m);
+    for (int j = 0; j < 8; j += 2) {
+      float x0 = xv[j][i], x1 = xv[j + 1][i];
+      const bf16x2 h = split_pair(x0, x1, n);           // remainders x - h, exact
+      bh[i][j] = h[0];
+      bh[i][j + 1] = h[1];
+      if (THREE) {
+        const bf16x2 m = split_pair(x0, x1, n);         // x - h - m, exact
+        bm[i][j] = m[0];
+        bm[i][j + 1] = m[1];
+        bl[i][j] = (bf16)x0;
+        bl[i][j + 1] = (bf16)x1;
+      } else {
+        bm[i][j] = (bf16)x0;
+        bm[i][j + 1] = (bf16)x1;
+      }
     }
 }
 
