@@ -27,6 +27,10 @@ class FfnoAssembleT(ctypes.Structure):
                 ("tan_b", c_ptr), ("tan_c", c_ptr), ("scale", c_ptr), ("act_gelu", ctypes.c_int)]
 
 
+class FfnoTanStatsOut(ctypes.Structure):
+    _fields_ = [("y", c_ptr), ("stats", c_ptr), ("depth_w", c_ptr), ("partials", c_ptr)]
+
+
 class FfnoHeadDesc(ctypes.Structure):
     _fields_ = [("n_species", ctypes.c_int), ("levels", ctypes.c_int * 4), ("log10_mass", ctypes.c_float * 4),
                 ("log_g", ctypes.c_float * 32), ("ev", ctypes.c_float * 32), ("ndxi", ctypes.c_float * 32),
@@ -96,8 +100,8 @@ SIGNATURES = {
     "ffno_assemble_t": [FfnoRows, FfnoAssembleT, c_ptr, c_ptr],
     "ffno_lift_t": [FfnoRows, c_ptr, c_ptr, c_ptr, c_ptr],
     "ffno_gates_t": [FfnoRows] + [c_ptr] * 8 + [c_ptr],
-    "ffno_mlp_t": [FfnoRows, FfnoAssemble, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr],
-    "ffno_fuse_t": [FfnoRows] + [c_ptr] * 16 + [c_ptr],
+    "ffno_mlp_t": [FfnoRows, FfnoAssemble, c_ptr, c_ptr, c_ptr, c_ptr, FfnoTanStatsOut, c_ptr],
+    "ffno_fuse_t": [FfnoRows, c_ptr, c_ptr, FfnoAssembleT, FfnoAssembleT] + [c_ptr] * 6 + [FfnoTanStatsOut, c_ptr],
     "ffno_head_t": [FfnoRows, FfnoAssemble] + [c_ptr] * 6 + [ctypes.POINTER(FfnoHeadDesc), c_ptr, c_ptr],
     "ffno_lowrank_tables": [c_i64, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [c_ptr] * 5 +
                            [ctypes.c_int, ctypes.c_int, c_ptr, c_ptr, c_ptr],

@@ -459,6 +459,69 @@ struct Lane {
   }
 };
 
+// sum over the 16 lanes of a row (the 16 voxels of a lane group): four DPP adds, xor 1, xor 2 (quad permutes), then
+// the half-row and row mirrors; every lane of the row ends with the row's sum
+__device__ __forceinline__ float row16_sum(float x) {
+  x += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xB1, 0xF, 0xF, true));    // quad_perm 1,0,3,2
+  x += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x4E, 0xF, 0xF, true));    // quad_perm 2,3,0,1
+  x += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x141, 0xF, 0xF, true));   // row_half_mirror
+  x += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x140, 0xF, 0xF, true));   // row_mirror
+  return x;
+}
+
+// A pair of per-(sample, GroupNorm group) sums (S1, S2) accumulated by the kernel that produces one of their factors,
+// so that no separate pass re-reads it. Each tile's per-lane sums are reduced over the 16 voxel lanes at once and
+// added to the wave's own LDS slot [wave][lane group][channel group of the quad][2] (nothing held in registers across
+// tiles); flush() merges the slots into the workgroup's partials [8][2], the layout the finalize kernels reduce.
+template <int CT>
+struct GroupSums {
+  static constexpr int CPG = CT * 16 / G;
+  static constexpr int SUB = CPG >= 4 ? 1 : 4 / CPG;   // channel groups per 4-channel quad
+  static constexpr int SLOT = CT * SUB * 2;            // floats per (wave, lane group)
+  float* slot;                                         // this wave's and lane group's slot in LDS
+
+  // every thread of the workgroup calls this once, before its first tile; red holds NW * 4 * SLOT floats
+  template <int NW>
+  __device__ void init(float* red, int q, int wave) {
+    for (int i = threadIdx.x; i < NW * 4 * SLOT; i += blockDim.x) red[i] = 0.f;
+    slot = red + (wave * 4 + q) * SLOT;
+    __syncthreads();
+  }
+  // a, c: this lane's sums over its own 4 channels of each tile, index t * SUB + (channel group within the quad)
+  __device__ __forceinline__ void put(float (&a)[CT * SUB], float (&c)[CT * SUB], int v) {
+#pragma unroll
+    for (int i = 0; i < CT * SUB; ++i) {
+      a[i] = row16_sum(a[i]);
+      c[i] = row16_sum(c[i]);
+    }
+    if (v == 0)
+#pragma unroll
+      for (int i = 0; i < CT * SUB; ++i) {
+        slot[2 * i] += a[i];
+        slot[2 * i + 1] += c[i];
+      }
+  }
+  // every thread of the workgroup calls this once, after its last tile
+  template <int NW>
+  __device__ void flush(const float* red, float* partials) {
+    __syncthreads();
+    if (threadIdx.x < G) {
+      const int g = threadIdx.x;
+      float s1 = 0.f, s2 = 0.f;
+      for (int w = 0; w < NW; ++w)
+        for (int qq = 0; qq < 4; ++qq)
+          for (int i = 0; i < CT * SUB; ++i) {
+            const int ch = 16 * (i / SUB) + 4 * qq + (i % SUB) * CPG;
+            if (ch / CPG != g) continue;
+            s1 += red[(w * 4 + qq) * SLOT + 2 * i];
+            s2 += red[(w * 4 + qq) * SLOT + 2 * i + 1];
+          }
+      partials[g * 2 + 0] = s1;
+      partials[g * 2 + 1] = s2;
+    }
+  }
+};
+
 template <int CT>
 __device__ __forceinline__ void load_act(const float* p, int64_t row, bool ok, int q, f32x4 (&x)[CT]) {
 #pragma unroll

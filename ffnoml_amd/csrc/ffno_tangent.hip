@@ -94,6 +94,50 @@ __global__ __launch_bounds__(NW * 64) void k_gn_tstats(FfnoRows rows, const floa
   }
 }
 
+// The same sums accumulated by the kernel that produces yd (its output tiles in lane layout), with y read per tile
+// and the group means of this batch item copied per channel into LDS (read per tile, not held in registers)
+template <int CT>
+struct TanSums : GroupSums<CT> {
+  using GroupSums<CT>::SUB;
+  using GroupSums<CT>::SLOT;
+  const float* mean;                                   // [C] in LDS
+  // every thread calls this once, before its first tile; red holds tan_sums_floats<CT, NW>() floats
+  template <int NW>
+  __device__ void init(float* red, const FfnoTanStatsOut& ts, int b, int q, int wave) {
+    constexpr int C = CT * 16, CPG = C / G;
+    float* m = red + NW * 4 * SLOT;
+    for (int c = threadIdx.x; c < C; c += blockDim.x) m[c] = ts.stats[(b * G + c / CPG) * 3];
+    mean = m;
+    GroupSums<CT>::template init<NW>(red, q, wave);   // its barrier also publishes the means
+  }
+  __device__ __forceinline__ void add(const FfnoTanStatsOut& ts, int64_t row, bool ok, int q, int v,
+                                      const f32x4 (&yd)[CT]) {
+    constexpr int C = CT * 16;
+    const float w = ok ? ts.depth_w[row] : 0.f;
+    float a[CT * SUB], c[CT * SUB];
+#pragma unroll
+    for (int t = 0; t < CT; ++t) {
+      const int c0 = 16 * t + 4 * q;
+      const f32x4 dy = ld4(ts.y + row * C + c0, ok) - *(const f32x4*)(mean + c0);
+#pragma unroll
+      for (int u = 0; u < SUB; ++u) {
+        a[t * SUB + u] = 0.f;
+        c[t * SUB + u] = 0.f;
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const float wd = ok ? w * yd[t][j] : 0.f;
+        a[t * SUB + j / (4 / SUB)] += wd;
+        c[t * SUB + j / (4 / SUB)] += dy[j] * wd;
+      }
+    }
+    this->put(a, c, v);
+  }
+};
+
+template <int CT, int NW>
+constexpr int tan_sums_floats() { return NW * 4 * GroupSums<CT>::SLOT + CT * 16; }
+
 // mean_dot = S1 / W, c = S2 / W, rstd_dot = -rstd^3 c:
 // n_dot = gamma (rstd (yd - mean_dot) - (y - mean) rstd^3 c) = a yd + tan_b y + tan_c
 __global__ __launch_bounds__(256) void k_gn_tfinalize(int nwg, int C, const float* __restrict__ partials,
@@ -242,15 +286,21 @@ __global__ __launch_bounds__(NW * 64) void k_gates_t(FfnoRows rows, const float*
 // ---------------------------------------------------------------------------------------------------------------
 template <int CT, int NW, bool LW>
 __global__ __launch_bounds__(NW * 64) void k_mlp_t(FfnoRows rows, FfnoAssemble in, const float* __restrict__ xd,
-                                                   const uint16_t* w1, const uint16_t* w2, float* yd) {
+                                                   const uint16_t* w1, const uint16_t* w2, float* yd,
+                                                   FfnoTanStatsOut ts) {
   constexpr int C = CT * 16, KS1 = ks_of(CT);
+  __shared__ float red[tan_sums_floats<CT, NW>()];
   extern __shared__ __align__(16) unsigned char smem[];
   bf16* lds = reinterpret_cast<bf16*>(smem);
   const Wm W1 = LW ? stage_w(lds, w1, 2 * C * wld(KS1 * 32)) : global_w(w1, 2 * C * wld(KS1 * 32));
   const Wm W2 = LW ? stage_w(lds, w2, C * wld(2 * C)) : global_w(w2, C * wld(2 * C));
-  __syncthreads();
   Lane L;
   const int b = blockIdx.y;
+  TanSums<CT> sums;
+  if (ts.partials)
+    sums.template init<NW>(red, ts, b, L.q, L.wave);    // its barrier also publishes the staged weights
+  else
+    __syncthreads();
   const float scale = in.scale ? *in.scale : 1.f;
   const int64_t nt = (rows.voxels + 15) / 16;
   for (int64_t tile = (int64_t)blockIdx.x * NW + L.wave; tile < nt; tile += (int64_t)gridDim.x * NW) {
@@ -268,44 +318,81 @@ __global__ __launch_bounds__(NW * 64) void k_mlp_t(FfnoRows rows, FfnoAssemble i
     f32x4 o[CT];
     mlp2_t<KS1, 2 * CT, CT>(W1, W2, xf, xdf, o, L.v, L.q);
     store_act<CT>(yd, row, ok, L.q, o);
+    if (ts.partials) sums.add(ts, row, ok, L.q, L.v, o);
   }
+  if (ts.partials) sums.template flush<NW>(red, ts.partials + ((int64_t)b * gridDim.x + blockIdx.x) * G * 2);
 }
 
 // ---------------------------------------------------------------------------------------------------------------
 // branch fusion along a tangent (models/ffno_model.py:524-534, :541-543): s = gelu(GN_s(sp)), v = gelu(GN_v(vp)),
 // g = sigmoid(BG2 gelu(BG1 [h, s, v]) + b) (read from the forward), sm = s gs, vm = v gv,
-// fp = F2 gelu(F1 [sm, vm]) + 0.5 (sm + vm); tangents hd, sd, vd given; weights from global (L2)
+// fp = F2 gelu(F1 [sm, vm]) + 0.5 (sm + vm); tangents hd given, sd and vd formed here from the branch GroupNorms'
+// tangent affines (s and gelu' from one tail evaluation); weights from global (L2)
 // ---------------------------------------------------------------------------------------------------------------
+// x = gelu(a t + b) and its tangent gelu'(a t + b) (a t_dot + tan_b t + tan_c) for one lane's 4 channels of tile t
+// (on entry x = t, xd = t_dot); k = the assembly's per-channel constants [a, b, tan_b, tan_c][C] in LDS
+template <int CT>
+__device__ __forceinline__ void gn_gelu_t(const float* k, bool ok, int q, int t, f32x4& x, f32x4& xd) {
+  constexpr int C = CT * 16;
+  const int c0 = 16 * t + 4 * q;
+  const f32x4 ka = *(const f32x4*)(k + c0), kb = *(const f32x4*)(k + C + c0);
+  const f32x4 n = ka * xd + *(const f32x4*)(k + 2 * C + c0) * x + *(const f32x4*)(k + 3 * C + c0);
+  const f32x4 u = x * ka + kb;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    float g, d;
+    gelu_and_grad(u[j], g, d);
+    x[j] = ok ? g : 0.f;
+    xd[j] = ok ? n[j] * d : 0.f;
+  }
+}
+
 template <int CT, int NW>
 __global__ __launch_bounds__(NW * 64) void k_fuse_t(FfnoRows rows, const float* __restrict__ h,
-                                                    const float* __restrict__ hd, const float* __restrict__ sp,
-                                                    const float* __restrict__ vp, const float* __restrict__ s_a,
-                                                    const float* __restrict__ s_b, const float* __restrict__ v_a,
-                                                    const float* __restrict__ v_b, const float* __restrict__ sd,
-                                                    const float* __restrict__ vd, const float* __restrict__ gates,
-                                                    const uint16_t* bg1, const uint16_t* bg2, const uint16_t* f1,
-                                                    const uint16_t* f2, float* __restrict__ fpd) {
+                                                    const float* __restrict__ hd, FfnoAssembleT st, FfnoAssembleT vt,
+                                                    const float* __restrict__ gates, const uint16_t* bg1,
+                                                    const uint16_t* bg2, const uint16_t* f1, const uint16_t* f2,
+                                                    float* __restrict__ fpd, FfnoTanStatsOut ts) {
   constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32;
+  // the two branch assemblies' per-channel constants of this batch item, then the tangent sums' slots
+  __shared__ __align__(16) float kc[8 * C];
+  __shared__ __align__(16) float red[tan_sums_floats<CT, NW>()];
   const Wm B1 = global_w(bg1, C * wld(3 * KP)), B2 = global_w(bg2, 2 * C * wld(KP));
   const Wm F1 = global_w(f1, 2 * C * wld(2 * KP)), F2 = global_w(f2, C * wld(2 * C));
   Lane L;
   const int b = blockIdx.y;
+  for (int c = threadIdx.x; c < C; c += blockDim.x) {
+    const int e = b * C + c;
+    kc[c] = st.aff_a[e];
+    kc[C + c] = st.aff_b[e];
+    kc[2 * C + c] = st.tan_b[e];
+    kc[3 * C + c] = st.tan_c[e];
+    kc[4 * C + c] = vt.aff_a[e];
+    kc[5 * C + c] = vt.aff_b[e];
+    kc[6 * C + c] = vt.tan_b[e];
+    kc[7 * C + c] = vt.tan_c[e];
+  }
+  TanSums<CT> sums;
+  if (ts.partials)
+    sums.template init<NW>(red, ts, b, L.q, L.wave);    // its barrier also publishes kc
+  else
+    __syncthreads();
   const int64_t nt = (rows.voxels + 15) / 16;
   for (int64_t tile = (int64_t)blockIdx.x * NW + L.wave; tile < nt; tile += (int64_t)gridDim.x * NW) {
     const int64_t vox = tile * 16 + L.v;
     const bool ok = vox < rows.voxels;
     const int64_t row = (int64_t)b * rows.voxels + vox;
+    asm volatile("" ::: "memory");        // the constants are read from LDS per tile, not hoisted into registers
     f32x4 s[CT], v[CT], ds[CT], dv[CT];
-    load_act<CT>(sp, row, ok, L.q, s);
-    load_act<CT>(vp, row, ok, L.q, v);
+    load_act<CT>(st.t, row, ok, L.q, s);
+    load_act<CT>(vt.t, row, ok, L.q, v);
+    load_act<CT>(st.t_dot, row, ok, L.q, ds);
+    load_act<CT>(vt.t_dot, row, ok, L.q, dv);
 #pragma unroll
     for (int t = 0; t < CT; ++t) {
-      const int c0 = b * C + 16 * t + 4 * L.q;
-      s[t] = ok ? gelu4(s[t] * *(const f32x4*)(s_a + c0) + *(const f32x4*)(s_b + c0)) : zero4();
-      v[t] = ok ? gelu4(v[t] * *(const f32x4*)(v_a + c0) + *(const f32x4*)(v_b + c0)) : zero4();
+      gn_gelu_t<CT>(kc, ok, L.q, t, s[t], ds[t]);
+      gn_gelu_t<CT>(kc + 4 * C, ok, L.q, t, v[t], dv[t]);
     }
-    load_act<CT>(sd, row, ok, L.q, ds);
-    load_act<CT>(vd, row, ok, L.q, dv);
     // gate pre-activation tangent: BG2 (gelu'(BG1 [h, s, v]) * BG1 [hd, sd, vd])
     f32x4 gd[2 * CT];
     {
@@ -343,7 +430,9 @@ __global__ __launch_bounds__(NW * 64) void k_fuse_t(FfnoRows rows, const float* 
 #pragma unroll
     for (int t = 0; t < CT; ++t) o[t] = o[t] + 0.5f * (ds[t] + dv[t]);
     store_act<CT>(fpd, row, ok, L.q, o);
+    if (ts.partials) sums.add(ts, row, ok, L.q, L.v, o);
   }
+  if (ts.partials) sums.template flush<NW>(red, ts.partials + ((int64_t)b * gridDim.x + blockIdx.x) * G * 2);
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -493,33 +582,38 @@ int ffno_gates_t(FfnoRows rows, const float* h, const float* hd, const uint16_t*
 }
 
 int ffno_mlp_t(FfnoRows rows, FfnoAssemble in, const float* xd, const uint16_t* w1, const uint16_t* w2, float* yd,
-               hipStream_t stream) {
-  if (!rows_ok(rows) || (!in.base && !in.t) || (in.t && (!in.aff_a || !in.aff_b)) || !xd || !w1 || !w2 || !yd)
+               FfnoTanStatsOut ts, hipStream_t stream) {
+  if (!rows_ok(rows) || (!in.base && !in.t) || (in.t && (!in.aff_a || !in.aff_b)) || !xd || !w1 || !w2 || !yd ||
+      (ts.partials && (!ts.y || !ts.stats || !ts.depth_w)))
     return fail("ffno_mlp_t: bad arguments");
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {
+    constexpr int NW = nw_of<CT>();
     const size_t lds = ((size_t)2 * CT * 16 * wld(ks_of(CT) * 32) + (size_t)CT * 16 * wld(2 * CT * 16)) * 2 * 2;
-    if (lds <= 160 * 1024) {
-      auto k = k_mlp_t<CT, nw_of<CT>(), true>;
+    if (lds + tan_sums_floats<CT, NW>() * sizeof(float) <= 160 * 1024) {
+      auto k = k_mlp_t<CT, NW, true>;
       if (set_lds(k, lds)) return -1;
-      k<<<grid, nw_of<CT>() * 64, lds, stream>>>(rows, in, xd, w1, w2, yd);
+      k<<<grid, NW * 64, lds, stream>>>(rows, in, xd, w1, w2, yd, ts);
     } else {
-      k_mlp_t<CT, nw_of<CT>(), false><<<grid, nw_of<CT>() * 64, 0, stream>>>(rows, in, xd, w1, w2, yd);
+      k_mlp_t<CT, NW, false><<<grid, NW * 64, 0, stream>>>(rows, in, xd, w1, w2, yd, ts);
     }
   });
   return check_launch("ffno_mlp_t");
 }
 
-int ffno_fuse_t(FfnoRows rows, const float* h, const float* hd, const float* sp, const float* vp, const float* s_a,
-                const float* s_b, const float* v_a, const float* v_b, const float* sd, const float* vd,
-                const float* gates, const uint16_t* bg1, const uint16_t* bg2, const uint16_t* f1, const uint16_t* f2,
-                float* fpd, hipStream_t stream) {
-  if (!rows_ok(rows) || !h || !hd || !sp || !vp || !s_a || !s_b || !v_a || !v_b || !sd || !vd || !gates || !bg1 ||
-      !bg2 || !f1 || !f2 || !fpd)
+static bool assemble_t_ok(const FfnoAssembleT& e) {
+  return e.t && e.t_dot && e.aff_a && e.aff_b && e.tan_b && e.tan_c && e.act_gelu == 1 && !e.base_dot && !e.scale;
+}
+
+int ffno_fuse_t(FfnoRows rows, const float* h, const float* hd, FfnoAssembleT st, FfnoAssembleT vt, const float* gates,
+                const uint16_t* bg1, const uint16_t* bg2, const uint16_t* f1, const uint16_t* f2, float* fpd,
+                FfnoTanStatsOut ts, hipStream_t stream) {
+  if (!rows_ok(rows) || !h || !hd || !assemble_t_ok(st) || !assemble_t_ok(vt) || !gates || !bg1 || !bg2 || !f1 ||
+      !f2 || !fpd || (ts.partials && (!ts.y || !ts.stats || !ts.depth_w)))
     return fail("ffno_fuse_t: bad arguments");
   dim3 grid(rows.nwg, rows.B);
-  FFNO_DISPATCH_CT(rows.C, (k_fuse_t<CT, NWV><<<grid, NWV * 64, 0, stream>>>(rows, h, hd, sp, vp, s_a, s_b, v_a, v_b,
-                                                                           sd, vd, gates, bg1, bg2, f1, f2, fpd)));
+  FFNO_DISPATCH_CT(rows.C, (k_fuse_t<CT, NWV><<<grid, NWV * 64, 0, stream>>>(rows, h, hd, st, vt, gates, bg1, bg2, f1,
+                                                                           f2, fpd, ts)));
   return check_launch("ffno_fuse_t");
 }
 

@@ -1213,11 +1213,20 @@ class Engine:
     def _gn_tangent(self, plan, y, yd, rec, gamma, part2):
         """(tan_b, tan_c) [B, C] of a GroupNorm's tangent n_dot = a yd + tan_b y + tan_c (a the forward affine)."""
         _, _, stats = rec
-        tb = torch.empty(plan.B, plan.C, device=y.device)
-        tc = torch.empty(plan.B, plan.C, device=y.device)
         _lib.call("ffno_gn_tangent_stats", plan.rows, _p(y), _p(yd), _p(stats), _p(plan.depth_w), _p(part2),
                   self._stream())
-        _lib.call("ffno_gn_tangent_finalize", plan.B, plan.nwg, plan.C, _p(part2), _p(stats), _p(gamma), _p(tb),
+        return self._gn_tangent_finalize(plan, rec, gamma, part2)
+
+    @staticmethod
+    def _tan_sums(plan, y, rec, part2):
+        """The producer of a GroupNorm input's tangent accumulates that GroupNorm's tangent sums (y its primal
+        input) into part2."""
+        return _lib.FfnoTanStatsOut(_p(y), _p(rec[2]), _p(plan.depth_w), _p(part2))
+
+    def _gn_tangent_finalize(self, plan, rec, gamma, part2):
+        tb = torch.empty(plan.B, plan.C, device=part2.device)
+        tc = torch.empty(plan.B, plan.C, device=part2.device)
+        _lib.call("ffno_gn_tangent_finalize", plan.B, plan.nwg, plan.C, _p(part2), _p(rec[2]), _p(gamma), _p(tb),
                   _p(tc), self._stream())
         return tb, tc
 
@@ -1265,35 +1274,38 @@ class Engine:
             blk = Wt.blocks[i]
             tables = plan.tables(i)
             gn = self._block_keep(plan, i, prev, bk["h"], bk, scr, bz, hb)
-            # gates and the two branches' transforms and post networks
+            # gates and the two branches' transforms and post networks; each post network's tangent kernel also
+            # accumulates its GroupNorm's tangent sums
             _lib.call("ffno_gates_t", rows, _p(bk["h"]), _p(hd), _p(blk["spec.g1"]), _p(blk["spec.g2"]),
                       _p(blk["vertical.g1"]), _p(blk["vertical.g2"]), _p(t1), _p(t2), st)
             self.horizontal(plan, t1, t1, i, scr, tables)
-            _lib.call("ffno_mlp_t", rows, _asm(bk["s"]), _p(t1), _p(blk["spec.p1"]), _p(blk["spec.p2"]), _p(t1), st)
-            tan_s = self._gn_tangent(plan, bk["sp"], t1, gn["s"], blk["norm_spec"][0], part2)
+            _lib.call("ffno_mlp_t", rows, _asm(bk["s"]), _p(t1), _p(blk["spec.p1"]), _p(blk["spec.p2"]), _p(t1),
+                      self._tan_sums(plan, bk["sp"], gn["s"], part2), st)
+            tan_s = self._gn_tangent_finalize(plan, gn["s"], blk["norm_spec"][0], part2)
             self.vertical(plan, t2, scr["v1"], t2, i, tables)
             _lib.call("ffno_mlp_t", rows, _asm(bk["v"]), _p(t2), _p(blk["vertical.p1"]), _p(blk["vertical.p2"]),
-                      _p(t2), st)
-            tan_v = self._gn_tangent(plan, bk["vp"], t2, gn["v"], blk["norm_vert"][0], part2)
-            # s = gelu(GN_s(sp)), v = gelu(GN_v(vp)) tangents, then the fusion
-            self._assemble_t(plan, t1, bk["sp"], t1, gn["s"], tan_s, act=1)
-            self._assemble_t(plan, t2, bk["vp"], t2, gn["v"], tan_v, act=1)
+                      _p(t2), self._tan_sums(plan, bk["vp"], gn["v"], part2), st)
+            tan_v = self._gn_tangent_finalize(plan, gn["v"], blk["norm_vert"][0], part2)
+            # the fusion forms s = gelu(GN_s(sp)), v = gelu(GN_v(vp)) and their tangents itself
             sa, sb_, _ = gn["s"]
             va, vb, _ = gn["v"]
-            _lib.call("ffno_fuse_t", rows, _p(bk["h"]), _p(hd), _p(bk["sp"]), _p(bk["vp"]), _p(sa), _p(sb_), _p(va),
-                      _p(vb), _p(t1), _p(t2), _p(bk["gates"]), _p(blk["bg1"]), _p(blk["bg2"]), _p(blk["f1"]),
-                      _p(blk["f2"]), _p(t3), st)
-            tan_f = self._gn_tangent(plan, bk["fp"], t3, gn["f"], blk["norm_fuse"][0], part2)
+            st_s = _lib.FfnoAssembleT(None, _p(bk["sp"]), _p(t1), _p(sa), _p(sb_), _p(tan_s[0]), _p(tan_s[1]), None, 1)
+            st_v = _lib.FfnoAssembleT(None, _p(bk["vp"]), _p(t2), _p(va), _p(vb), _p(tan_v[0]), _p(tan_v[1]), None, 1)
+            _lib.call("ffno_fuse_t", rows, _p(bk["h"]), _p(hd), st_s, st_v, _p(bk["gates"]), _p(blk["bg1"]),
+                      _p(blk["bg2"]), _p(blk["f1"]), _p(blk["f2"]), _p(t3),
+                      self._tan_sums(plan, bk["fp"], gn["f"], part2), st)
+            tan_f = self._gn_tangent_finalize(plan, gn["f"], blk["norm_fuse"][0], part2)
             # x1 = h + res_fused gelu(GN_f(fp)) -> t1; pp tangent -> t2
             self._assemble_t(plan, t1, bk["fp"], t3, gn["f"], tan_f, act=1, scale=blk["res_fused"], base_dot=hd)
-            _lib.call("ffno_mlp_t", rows, _asm(bk["x1"]), _p(t1), _p(blk["pw1"]), _p(blk["pw2"]), _p(t2), st)
-            tan_p = self._gn_tangent(plan, bk["pp"], t2, gn["p"], blk["norm_pw"][0], part2)
+            _lib.call("ffno_mlp_t", rows, _asm(bk["x1"]), _p(t1), _p(blk["pw1"]), _p(blk["pw2"]), _p(t2),
+                      self._tan_sums(plan, bk["pp"], gn["p"], part2), st)
+            tan_p = self._gn_tangent_finalize(plan, gn["p"], blk["norm_pw"][0], part2)
             # x2 = x1 + res_pw GN_p(pp) -> t1; mp tangent -> t3
             self._assemble_t(plan, t1, bk["pp"], t2, gn["p"], tan_p, scale=blk["res_pw"], base_dot=t1)
             pa, pb, _ = gn["p"]
             _lib.call("ffno_mlp_t", rows, _asm(bk["x1"], bk["pp"], pa, pb, blk["res_pw"]), _p(t1), _p(blk["m1"]),
-                      _p(blk["m2"]), _p(t3), st)
-            tan_m = self._gn_tangent(plan, bk["mp"], t3, gn["m"], blk["norm_mlp"][0], part2)
+                      _p(blk["m2"]), _p(t3), self._tan_sums(plan, bk["mp"], gn["m"], part2), st)
+            tan_m = self._gn_tangent_finalize(plan, gn["m"], blk["norm_mlp"][0], part2)
             # the next block's input: x2 + res_mlp GN_m(mp)
             self._assemble_t(plan, hd, bk["mp"], t3, gn["m"], tan_m, scale=blk["res_mlp"], base_dot=t1)
             ma, mb, _ = gn["m"]

@@ -331,6 +331,16 @@ typedef struct {
  * tan_c = -gamma rstd S1/W + gamma rstd^3 (S2/W) mean (stats from ffno_gn_finalize, gamma [C]). */
 int ffno_gn_tangent_stats(FfnoRows rows, const float* y, const float* yd, const float* stats, const float* depth_w,
                           float* partials, hipStream_t stream);
+
+/* The same tangent sums accumulated by the kernel that produces yd (ffno_mlp_t, ffno_fuse_t), so that no separate
+ * pass re-reads it: y the GroupNorm's primal input, partials laid out as ffno_gn_tangent_stats writes them (NULL: no
+ * sums). */
+typedef struct {
+  const float* y;         /* [N, C] */
+  const float* stats;     /* [B][8][3] from ffno_gn_finalize */
+  const float* depth_w;   /* [B, D*H*W] */
+  float* partials;        /* [B][nwg][8][2] or NULL */
+} FfnoTanStatsOut;
 int ffno_gn_tangent_finalize(int B, int nwg, int C, const float* partials, const float* stats, const float* gamma,
                              float* tan_b, float* tan_c, hipStream_t stream);
 int ffno_assemble_t(FfnoRows rows, FfnoAssembleT a, float* out, hipStream_t stream);
@@ -345,17 +355,19 @@ int ffno_gates_t(FfnoRows rows, const float* h, const float* hd, const uint16_t*
                  const uint16_t* v1, const uint16_t* v2, float* xsd, float* xvd, hipStream_t stream);
 
 /* Tangent of y = W2 gelu(W1 x) (the post networks, pw and mlp: models/ffno_model.py:221-225, :369-373, :545-551):
- * x assembled as the forward assembles it, xd given: yd = W2 (gelu'(W1 x) * W1 xd). */
+ * x assembled as the forward assembles it, xd given: yd = W2 (gelu'(W1 x) * W1 xd); with ts.partials, also the
+ * tangent sums of the GroupNorm that y feeds. */
 int ffno_mlp_t(FfnoRows rows, FfnoAssemble in, const float* xd, const uint16_t* w1, const uint16_t* w2, float* yd,
-               hipStream_t stream);
+               FfnoTanStatsOut ts, hipStream_t stream);
 
-/* Branch-fusion tangent (models/ffno_model.py:524-534, :541-543) from h, hd, the GroupNorm inputs sp, vp with their
- * forward affines, the tangents sd, vd of s = gelu(GN_s(sp)) and v = gelu(GN_v(vp)) and the gates the forward stored
- * [N, 2C]: fpd = F2 (gelu'(F1 [sm, vm]) * F1 [smd, vmd]) + 0.5 (smd + vmd). */
-int ffno_fuse_t(FfnoRows rows, const float* h, const float* hd, const float* sp, const float* vp, const float* s_a,
-                const float* s_b, const float* v_a, const float* v_b, const float* sd, const float* vd,
-                const float* gates, const uint16_t* bg1, const uint16_t* bg2, const uint16_t* f1, const uint16_t* f2,
-                float* fpd, hipStream_t stream);
+/* Branch-fusion tangent (models/ffno_model.py:524-534, :541-543) from h, hd, the gates the forward stored [N, 2C]
+ * and the two branch GroupNorms as tangent assemblies: s = gelu(GN_s(sp)) with its tangent
+ * sd = gelu'(a sp + b) (a sp_dot + tan_b sp + tan_c) (st.t = sp, st.t_dot = sp_dot, st.act_gelu = 1, no base or
+ * scale), likewise v from vt: fpd = F2 (gelu'(F1 [sm, vm]) * F1 [smd, vmd]) + 0.5 (smd + vmd); with ts.partials, also
+ * the tangent sums of the fusion GroupNorm (ts.y = fp). */
+int ffno_fuse_t(FfnoRows rows, const float* h, const float* hd, FfnoAssembleT st, FfnoAssembleT vt, const float* gates,
+                const uint16_t* bg1, const uint16_t* bg2, const uint16_t* f1, const uint16_t* f2, float* fpd,
+                FfnoTanStatsOut ts, hipStream_t stream);
 
 /* Head tangent (models/ffno_model.py:730-749, :790-798): normalised log10 n tangent yd [B, Cout, DHW] from the head
  * input (assembled as the forward does), its tangent xd, the model input x_in and its tangent xd_in (T, ne and rho

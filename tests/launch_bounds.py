@@ -410,26 +410,40 @@ class Checker:
         for what, w in (("g1", g1), ("g2", g2), ("v1", v1), ("v2", v2)):
             self.weight(n, what, w, C, C)
 
-    def ffno_mlp_t(self, n, rows, a, xd, w1, w2, yd, stream):
+    def tan_sums(self, n, ts, rows):
+        if not ts.partials:
+            return
+        self.act(n, "ts.y", ts.y, rows)
+        self.span(n, "ts.stats", ts.stats, rows.B * 8 * 3 * F32)
+        self.span(n, "ts.depth_w", ts.depth_w, rows.B * rows.voxels * F32)
+        self.span(n, "ts.partials", ts.partials, rows.B * rows.nwg * 8 * 2 * F32)
+
+    def ffno_mlp_t(self, n, rows, a, xd, w1, w2, yd, ts, stream):
         C = rows.C
         self.asm(n, a, rows)
         self.act(n, "xd", xd, rows)
         self.weight(n, "w1", w1, 2 * C, C)
         self.weight(n, "w2", w2, C, 2 * C)
         self.act(n, "yd", yd, rows)
+        self.tan_sums(n, ts, rows)
 
-    def ffno_fuse_t(self, n, rows, h, hd, sp, vp, sa, sb, va, vb, sd, vd, gates, bg1, bg2, f1, f2, fpd, stream):
+    def ffno_fuse_t(self, n, rows, h, hd, st, vt, gates, bg1, bg2, f1, f2, fpd, ts, stream):
         C = rows.C
         KP = round_up(C, 32)
-        for what, p in (("h", h), ("hd", hd), ("sp", sp), ("vp", vp), ("sd", sd), ("vd", vd), ("fpd", fpd)):
+        for what, p in (("h", h), ("hd", hd), ("fpd", fpd)):
             self.act(n, what, p, rows)
-        for what, p in (("s_a", sa), ("s_b", sb), ("v_a", va), ("v_b", vb)):
-            self.span(n, what, p, rows.B * C * F32)
+        for tag, e in (("s", st), ("v", vt)):
+            assert e.act_gelu == 1 and not e.base_dot and not e.scale, f"{n}: {tag} assembly"
+            self.act(n, f"{tag}.t", e.t, rows)
+            self.act(n, f"{tag}.t_dot", e.t_dot, rows)
+            for what, p in (("aff_a", e.aff_a), ("aff_b", e.aff_b), ("tan_b", e.tan_b), ("tan_c", e.tan_c)):
+                self.span(n, f"{tag}.{what}", p, rows.B * C * F32)
         self.span(n, "gates", gates, rows.B * rows.voxels * 2 * C * F32)
         self.span(n, "bg1", bg1, 2 * C * (3 * KP + 8) * BF)
         self.weight(n, "bg2", bg2, 2 * C, C)
         self.span(n, "f1", f1, 2 * 2 * C * (2 * KP + 8) * BF)
         self.weight(n, "f2", f2, C, 2 * C)
+        self.tan_sums(n, ts, rows)
 
     def ffno_head_t(self, n, rows, a, xd, p1, p2, bias, x_in, xd_in, desc, yd, stream):
         d = desc._obj
