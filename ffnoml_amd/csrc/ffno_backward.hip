@@ -117,9 +117,12 @@ struct GradSums : GroupSums<CT> {
 // into LDS kc[8][C]: P = rstd scale gamma, Q = rstd S1 / W, R = rstd^2 S2 / W, M = mean (per channel of its group),
 // A, B = the GroupNorm's affine (for the GELU after it), XA, XB = the assembly's affine. Then
 //   t_bar = P act u_bar - w (Q + R (t - M)),  act = GELU'(A t + B) or 1,
-// which is gn_back's rstd (g_hat - (w / W) (S1 + x_hat S2)) rearranged (the depth-weight gradient is not formed).
+// which is gn_back's rstd (g_hat - (w / W) (S1 + x_hat S2)) rearranged. gn_back's depth-weight gradient of a voxel,
+// -(1/W) sum_c (S1 x_hat + S2 (x_hat^2 - rstd^2 var) / 2), is in the same constants with d = t - M
+//   w_bar = K - sum_c d (Q + R d / 2),  K = sum_c (S2 / 2W) (1 - eps rstd^2)  (one scalar, written to *ksum).
 template <int CT>
-__device__ void gn_consts(const FfnoGnBack& gb, float scale, const FfnoAssemble& xin, int b, float* kc) {
+__device__ void gn_consts(const FfnoGnBack& gb, float scale, const FfnoAssemble& xin, int b, float* kc,
+                          float* ksum = nullptr) {
   constexpr int C = CT * 16, CPG = C / G;
   for (int c = threadIdx.x; c < C; c += blockDim.x) {
     const int g = c / CPG;
@@ -135,24 +138,49 @@ __device__ void gn_consts(const FfnoGnBack& gb, float scale, const FfnoAssemble&
     kc[6 * C + c] = xin.t ? xin.aff_a[b * C + c] : 0.f;
     kc[7 * C + c] = xin.t ? xin.aff_b[b * C + c] : 0.f;
   }
+  if (ksum && threadIdx.x == 0) {
+    float k = 0.f;
+    for (int g = 0; g < G; ++g) {
+      const float* st = gb.stats + (b * G + g) * 3;
+      const float* sm = gb.sums + (b * G + g) * 2;
+      k += (float)CPG * 0.5f * sm[1] * __builtin_amdgcn_rcpf(st[2]) * (1.f - 1e-5f * st[1] * st[1]);
+    }
+    *ksum = k;
+  }
 }
 
 // t_bar from the constants (rows outside the batch item have u_bar = t = 0 and w = 0, so t_bar = 0 there); the
-// GroupNorm input is read per tile t, as gn_back does
-template <int CT>
+// GroupNorm input is read per tile t, as gn_back does. WB: also the voxel's depth-weight gradient, added to
+// gb.wbar (ksum: the scalar K gn_consts wrote)
+template <int CT, bool WB = false>
 __device__ __forceinline__ void gn_back_k(const FfnoGnBack& gb, const float* kc, int64_t row, bool ok, int q,
-                                          const f32x4 (&ub)[CT], f32x4 (&tb)[CT]) {
+                                          const f32x4 (&ub)[CT], f32x4 (&tb)[CT], const float* ksum = nullptr) {
   constexpr int C = CT * 16;
   const float w = ok ? gb.depth_w[row] : 0.f;
+  float wb = 0.f;
 #pragma unroll
   for (int t = 0; t < CT; ++t) {
     const int c0 = 16 * t + 4 * q;
     const f32x4 tv = ld4(gb.t + row * C + c0, ok);
     f32x4 g = *(const f32x4*)(kc + c0) * ub[t];
     if (gb.act_gelu) g *= gelu_grad4(tv * *(const f32x4*)(kc + 4 * C + c0) + *(const f32x4*)(kc + 5 * C + c0));
-    const f32x4 qr = *(const f32x4*)(kc + C + c0) + *(const f32x4*)(kc + 2 * C + c0) *
-                                                        (tv - *(const f32x4*)(kc + 3 * C + c0));
-    tb[t] = g - w * qr;
+    if constexpr (WB) {
+      const f32x4 Q = *(const f32x4*)(kc + C + c0);
+      const f32x4 d = tv - *(const f32x4*)(kc + 3 * C + c0);
+      const f32x4 rd = *(const f32x4*)(kc + 2 * C + c0) * d;
+      tb[t] = g - w * (Q + rd);
+      const f32x4 e = d * (Q + 0.5f * rd);
+      wb -= (e[0] + e[1]) + (e[2] + e[3]);
+    } else {
+      const f32x4 qr = *(const f32x4*)(kc + C + c0) + *(const f32x4*)(kc + 2 * C + c0) *
+                                                          (tv - *(const f32x4*)(kc + 3 * C + c0));
+      tb[t] = g - w * qr;
+    }
+  }
+  if constexpr (WB) {
+    wb += __shfl_xor(wb, 16);
+    wb += __shfl_xor(wb, 32);
+    if (ok && q == 0) gb.wbar[row] += wb + *ksum;
   }
 }
 
@@ -178,9 +206,9 @@ __device__ __forceinline__ void assemble_k(const FfnoAssemble& in, const float* 
 template <int CT, int NW>
 constexpr size_t grad_sums_bytes() { return (size_t)NW * 4 * GradSums<CT>::SLOT * sizeof(float); }
 
-// the LDS the MLP adjoint keeps ahead of its weights: kc[8][C] and the fused sums' slots
+// the LDS the MLP adjoint keeps ahead of its weights: kc[8][C], K (padded to 4 floats) and the fused sums' slots
 template <int CT, int NW>
-constexpr size_t mlp_bwd_head_bytes() { return (size_t)8 * CT * 16 * sizeof(float) + grad_sums_bytes<CT, NW>(); }
+constexpr size_t mlp_bwd_head_bytes() { return ((size_t)8 * CT * 16 + 4) * sizeof(float) + grad_sums_bytes<CT, NW>(); }
 
 template <int CT, int NW>
 __global__ __launch_bounds__(NW * 64) void k_gn_grad_stats(FfnoRows rows, FfnoGnBack gb, const float* __restrict__ ubar,
@@ -286,18 +314,19 @@ __global__ __launch_bounds__(256) void k_gn_grad_finalize(int nwg, const float* 
 // ---------------------------------------------------------------------------------------------------------------
 // MLP behind a GroupNorm, with or without the residual path
 // ---------------------------------------------------------------------------------------------------------------
-// KC: the GroupNorm backward from per-channel constants (no depth-weight gradient), else gn_back
-template <int CT, int NW, bool LW, bool KC>
+// WB: the GroupNorm's depth-weight gradient too (the height gradient is wanted)
+template <int CT, int NW, bool LW, bool WB>
 __global__ __launch_bounds__(NW * 64) void k_mlp_bwd(FfnoRows rows, const float* ubar, FfnoGnBack gb, FfnoAssemble xin,
                                                      const uint16_t* w1, const uint16_t* w2t, const uint16_t* w1t,
                                                      int residual, float* xbar, FfnoGnBack next,
                                                      float* next_partials) {
   constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32;
   extern __shared__ __align__(16) unsigned char smem[];
-  // LDS: the per-channel constants kc[8][C] and the fused sums' slots first (within reach of the LDS instructions'
+  // LDS: the per-channel constants kc[8][C], K and the fused sums' slots first (within reach of the LDS instructions'
   // 16-bit immediate offsets), then the staged weights
   float* kc = reinterpret_cast<float*>(smem);
-  float* red = kc + 8 * C;
+  float* ksum = kc + 8 * C;
+  float* red = ksum + 4;
   bf16* lds = reinterpret_cast<bf16*>(smem + mlp_bwd_head_bytes<CT, NW>());
   const Wm W1 = LW ? stage_w(lds, w1, 2 * C * wld(KP)) : global_w(w1, 2 * C * wld(KP));
   const Wm W2T = LW ? stage_w(lds, w2t, 2 * C * wld(KP)) : global_w(w2t, 2 * C * wld(KP));
@@ -308,7 +337,7 @@ __global__ __launch_bounds__(NW * 64) void k_mlp_bwd(FfnoRows rows, const float*
   const float scale = gb.scale ? *gb.scale : 1.f;
   const float xscale = xin.scale ? *xin.scale : 1.f;
   const float nscale = next_partials && next.scale ? *next.scale : 1.f;
-  if (KC) gn_consts<CT>(gb, scale, xin, b, kc);
+  gn_consts<CT>(gb, scale, xin, b, kc, WB ? ksum : nullptr);
   GradSums<CT> sums;
   if (next_partials)
     sums.template init<NW>(red, L.q, L.wave);           // its barrier also publishes kc
@@ -324,13 +353,8 @@ __global__ __launch_bounds__(NW * 64) void k_mlp_bwd(FfnoRows rows, const float*
     Frag of[KS], xf[KS];
     {
       f32x4 tb[CT], x[CT];
-      if constexpr (KC) {
-        gn_back_k<CT>(gb, kc, row, ok, L.q, ub, tb);
-        assemble_k<CT>(xin, kc, xscale, row, ok, L.q, x);
-      } else {
-        gn_back<CT>(gb, scale, row, b, ok, L.q, ub, tb);
-        assemble<CT>(xin, xscale, row, b, ok, L.q, x);
-      }
+      gn_back_k<CT, WB>(gb, kc, row, ok, L.q, ub, tb, ksum);
+      assemble_k<CT>(xin, kc, xscale, row, ok, L.q, x);
       to_frags<CT>(tb, of);
       to_frags<CT>(x, xf);
     }
@@ -464,8 +488,8 @@ __global__ __launch_bounds__(NW * 64) void k_fuse_bwd(FfnoRows rows, const float
 // ---------------------------------------------------------------------------------------------------------------
 // pass 1: fp_bar from the GroupNorm (GELU after it), then the fusion MLP's adjoint with its hidden layer recomputed:
 // [sm_bar, vm_bar] = 0.5 fp_bar (both halves) + F1^T (gelu'(F1 [s gs, v gv]) * F2^T fp_bar)
-// KC: the fusion GroupNorm's backward from per-channel constants (no depth-weight gradient), else gn_back
-template <int CT, int NW, bool LW, bool KC>
+// WB: the fusion GroupNorm's depth-weight gradient too (the height gradient is wanted)
+template <int CT, int NW, bool LW, bool WB>
 __global__ __launch_bounds__(NW * 64) void k_fuse_mlp_bwd(FfnoRows rows, const float* __restrict__ x1bar, FfnoGnBack gf,
                                                           const float* __restrict__ sp, const float* __restrict__ vp,
                                                           const float* __restrict__ s_a, const float* __restrict__ s_b,
@@ -476,16 +500,17 @@ __global__ __launch_bounds__(NW * 64) void k_fuse_mlp_bwd(FfnoRows rows, const f
   constexpr int C = CT * 16, KS = ks_of(CT), KP = KS * 32;
   extern __shared__ __align__(16) unsigned char smem[];
   // LDS: kc[12][C] first (the GroupNorm constants, then the branch affines s_a, s_b, v_a, v_b of this batch item),
-  // within reach of 16-bit immediate offsets; then the staged weights
+  // within reach of 16-bit immediate offsets, and K (padded to 4 floats); then the staged weights
   float* kc = reinterpret_cast<float*>(smem);
-  bf16* lds = reinterpret_cast<bf16*>(smem + 12 * C * sizeof(float));
+  float* ksum = kc + 12 * C;
+  bf16* lds = reinterpret_cast<bf16*>(smem + (12 * C + 4) * sizeof(float));
   const Wm F1 = LW ? stage_w(lds, f1, 2 * C * wld(2 * KP)) : global_w(f1, 2 * C * wld(2 * KP));
   const Wm F1T = LW ? stage_w(lds, f1t, 2 * C * wld(2 * C)) : global_w(f1t, 2 * C * wld(2 * C));
   const Wm F2T = global_w(f2t, 2 * C * wld(KP));
   Lane L;
   const int b = blockIdx.y;
   const float scale = gf.scale ? *gf.scale : 1.f;
-  if (KC) gn_consts<CT>(gf, scale, FfnoAssemble{}, b, kc);
+  gn_consts<CT>(gf, scale, FfnoAssemble{}, b, kc, WB ? ksum : nullptr);
   for (int c = threadIdx.x; c < C; c += blockDim.x) {
     kc[8 * C + c] = s_a[b * C + c];
     kc[9 * C + c] = s_b[b * C + c];
@@ -521,10 +546,7 @@ __global__ __launch_bounds__(NW * 64) void k_fuse_mlp_bwd(FfnoRows rows, const f
     {
       f32x4 x1b[CT], fb[CT];
       load_act<CT>(x1bar, row, ok, L.q, x1b);
-      if constexpr (KC)
-        gn_back_k<CT>(gf, kc, row, ok, L.q, x1b, fb);
-      else
-        gn_back<CT>(gf, scale, row, b, ok, L.q, x1b, fb);
+      gn_back_k<CT, WB>(gf, kc, row, ok, L.q, x1b, fb, ksum);
 #pragma unroll
       for (int t = 0; t < CT; ++t) {
         mb[t] = 0.5f * fb[t];
@@ -961,9 +983,9 @@ int ffno_mlp_bwd(FfnoRows rows, const float* ubar, FfnoGnBack gn, FfnoAssemble x
     };
     int rc;
     if (gn.wbar)
-      rc = lw ? launch(k_mlp_bwd<CT, nw_bwd<CT>(), true, false>) : launch(k_mlp_bwd<CT, nw_bwd<CT>(), false, false>);
-    else
       rc = lw ? launch(k_mlp_bwd<CT, nw_bwd<CT>(), true, true>) : launch(k_mlp_bwd<CT, nw_bwd<CT>(), false, true>);
+    else
+      rc = lw ? launch(k_mlp_bwd<CT, nw_bwd<CT>(), true, false>) : launch(k_mlp_bwd<CT, nw_bwd<CT>(), false, false>);
     if (rc) return rc;
   });
   return check_launch("ffno_mlp_bwd");
@@ -994,7 +1016,7 @@ int ffno_fuse_mlp_bwd(FfnoRows rows, const float* x1bar, FfnoGnBack gnf, const f
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {
     constexpr int C = CT * 16, KP = ks_of(CT) * 32;
-    const size_t head = (size_t)12 * C * sizeof(float);
+    const size_t head = ((size_t)12 * C + 4) * sizeof(float);
     const size_t lds = ((size_t)2 * C * wld(2 * KP) + (size_t)2 * C * wld(2 * C)) * 2 * 2;
     const bool lw = lds + head <= 160 * 1024;
     const size_t bytes = (lw ? lds : 0) + head;
@@ -1006,9 +1028,9 @@ int ffno_fuse_mlp_bwd(FfnoRows rows, const float* x1bar, FfnoGnBack gnf, const f
     };
     int rc;
     if (gnf.wbar)
-      rc = lw ? launch(k_fuse_mlp_bwd<CT, nw_of<CT>(), true, false>) : launch(k_fuse_mlp_bwd<CT, nw_of<CT>(), false, false>);
-    else
       rc = lw ? launch(k_fuse_mlp_bwd<CT, nw_of<CT>(), true, true>) : launch(k_fuse_mlp_bwd<CT, nw_of<CT>(), false, true>);
+    else
+      rc = lw ? launch(k_fuse_mlp_bwd<CT, nw_of<CT>(), true, false>) : launch(k_fuse_mlp_bwd<CT, nw_of<CT>(), false, false>);
     if (rc) return rc;
   });
   return check_launch("ffno_fuse_mlp_bwd");
