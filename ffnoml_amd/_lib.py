@@ -64,6 +64,12 @@ class FfnoMix(ctypes.Structure):
                 ("factor", c_ptr), ("parts", ctypes.c_int)]
 
 
+class FfnoZGrad(ctypes.Structure):
+    _fields_ = [("B", ctypes.c_int), ("hw", c_i64), ("D", ctypes.c_int), ("R", ctypes.c_int), ("C", ctypes.c_int),
+                ("r_pad", ctypes.c_int), ("d_pad", ctypes.c_int), ("xv", c_ptr), ("xbar_spec", c_ptr),
+                ("yv_bar", c_ptr), ("y_spec", c_ptr), ("tables", c_ptr), ("zbar", c_ptr)]
+
+
 class FfnoGnBack(ctypes.Structure):
     _fields_ = [("t", c_ptr), ("stats", c_ptr), ("gamma", c_ptr), ("aff_a", c_ptr), ("aff_b", c_ptr),
                 ("scale", c_ptr), ("act_gelu", ctypes.c_int), ("sums", c_ptr), ("depth_w", c_ptr), ("wbar", c_ptr)]
@@ -95,6 +101,7 @@ SIGNATURES = {
     "ffno_head_bwd": [FfnoRows] + [c_ptr] * 7 + [ctypes.POINTER(FfnoHeadDesc)] + [c_ptr] * 3 + [c_ptr],
     "ffno_lift_bwd": [FfnoRows, c_ptr, FfnoGnBack, c_ptr, c_ptr, c_ptr],
     "ffno_dot_reduce": [FfnoRows, ctypes.c_int, c_ptr, ctypes.c_int, c_ptr, c_ptr, c_ptr],
+    "ffno_zgrad_columns": [ctypes.POINTER(FfnoZGrad), c_ptr],
     "ffno_gn_tangent_stats": [FfnoRows] + [c_ptr] * 5 + [c_ptr],
     "ffno_gn_tangent_finalize": [ctypes.c_int, ctypes.c_int, ctypes.c_int] + [c_ptr] * 5 + [c_ptr],
     "ffno_assemble_t": [FfnoRows, FfnoAssembleT, c_ptr, c_ptr],

@@ -908,6 +908,97 @@ __global__ __launch_bounds__(NW * 64) void k_lift_bwd(FfnoRows rows, const float
 }
 
 // ---------------------------------------------------------------------------------------------------------------
+// height gradient of the vertical branch per column of a shared height grid
+// ---------------------------------------------------------------------------------------------------------------
+// For one height column (b, hw), with P[d', r] = sum_c xv[d', c] X_bar[r, c] and Q[d, r] = sum_c yv_bar[d, c] Y[r, c]
+// (r: the 2M rows of the vertical spectrum),
+//   zbar[d] += sum_{delta = -1, 0, 1} sum_r G_delta[d, r] P[d + delta, r] + sum_r V_dz[d, r] Q[d, r],
+// what the three Filon-band GEMMs and the synthesis-phase GEMM followed by their channel contractions form, without
+// writing the GEMMs' D x HW x C outputs. P and Q are two-way split-bf16 MFMA products over the channels (A: 16 levels,
+// B: 16 spectrum rows), each 16 x 16 tile contracted in float32 with the tables tab[k][r][d'] = G_delta[d' - delta, r]
+// (k = delta + 1; zero where d' - delta leaves the column) and tab[3][r][d] = V_dz[d, r]: a lane holds rows
+// d' = 4q + j of spectrum row r = v, so its table reads are one float4 each. The 16 lanes of a row are summed (DPP) and
+// the sums added to the wave's LDS row for its column: for one (delta, j) the four lane groups add to four distinct
+// levels, so no two lanes add to one entry at once. One wave per column.
+template <int NDELTA, int CT, int RTC>
+__device__ __forceinline__ void zgrad_pass(const FfnoZGrad& g, int b, int64_t hw, const float* A, const float* Bm,
+                                           const float* tab, float* zw, const Lane& L) {
+  constexpr int C = CT * 16, KS = ks_of(CT);
+  const int DT = g.d_pad / 16, RT = g.r_pad / 16;
+  for (int r0 = 0; r0 < RT; r0 += RTC) {
+    // B fragments of this chunk of spectrum rows: column v = row r, k = channels 32 s + 8 q + j
+    Frag bf[RTC][KS];
+#pragma unroll
+    for (int u = 0; u < RTC; ++u) {
+      const int r = (r0 + u) * 16 + L.v;
+      const bool rok = r0 + u < RT && r < g.R;
+      const float* p = Bm + (((int64_t)b * g.R + (rok ? r : 0)) * g.hw + hw) * C;
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        const int c0 = 32 * s + 8 * L.q;
+        const bool ok = rok && c0 < C;
+        bf[u][s] = split8(ld4(p + c0, ok), ld4(p + c0 + 4, ok));
+      }
+    }
+    for (int dt = 0; dt < DT; ++dt) {
+      const int d = dt * 16 + L.v;
+      const bool dok = d < g.D;
+      const float* pa = A + (((int64_t)b * g.D + (dok ? d : 0)) * g.hw + hw) * C;
+      Frag af[KS];
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        const int c0 = 32 * s + 8 * L.q;
+        const bool ok = dok && c0 < C;
+        af[s] = split8(ld4(pa + c0, ok), ld4(pa + c0 + 4, ok));
+      }
+      const int d1 = dt * 16 + 4 * L.q;             // this lane's first level d' of the tile
+      f32x4 zp[NDELTA];
+#pragma unroll
+      for (int k = 0; k < NDELTA; ++k) zp[k] = zero4();
+#pragma unroll
+      for (int u = 0; u < RTC; ++u) {
+        if (r0 + u < RT) {
+          f32x4 acc = zero4();
+#pragma unroll
+          for (int s = 0; s < KS; ++s) acc = mfma3(af[s].h, af[s].l, bf[u][s], acc);
+          const int r = (r0 + u) * 16 + L.v;
+#pragma unroll
+          for (int k = 0; k < NDELTA; ++k)
+            zp[k] += *(const f32x4*)(tab + ((int64_t)k * g.r_pad + r) * g.d_pad + d1) * acc;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < NDELTA; ++k) {
+        const int delta = NDELTA == 1 ? 0 : k - 1;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const float sum = row16_sum(zp[k][j]);
+          if (L.v == 0) zw[d1 + j - delta] += sum;
+        }
+      }
+    }
+  }
+}
+
+template <int CT>
+__global__ __launch_bounds__(256) void k_zgrad_cols(FfnoZGrad g) {
+  constexpr int KS = ks_of(CT);
+  constexpr int RTC = KS >= 3 ? 2 : (KS == 2 ? 4 : 8);   // spectrum-row tiles per chunk: RTC * KS fragments held
+  extern __shared__ float zl[];                           // per wave: levels -1 .. d_pad
+  Lane L;
+  float* zw = zl + L.wave * (g.d_pad + 2) + 1;
+  const int64_t cols = (int64_t)g.B * g.hw;
+  for (int64_t col = (int64_t)blockIdx.x * 4 + L.wave; col < cols; col += (int64_t)gridDim.x * 4) {
+    const int b = (int)(col / g.hw);
+    const int64_t hw = col % g.hw;
+    for (int i = L.lane; i < g.d_pad + 2; i += 64) zw[i - 1] = 0.f;
+    zgrad_pass<3, CT, RTC>(g, b, hw, g.xv, g.xbar_spec, g.tables, zw, L);
+    zgrad_pass<1, CT, RTC>(g, b, hw, g.yv_bar, g.y_spec, g.tables + (int64_t)3 * g.r_pad * g.d_pad, zw, L);
+    for (int d = L.lane; d < g.D; d += 64) g.zbar[((int64_t)b * g.D + d) * g.hw + hw] += zw[d];
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
 // channel contraction for the height gradient
 // ---------------------------------------------------------------------------------------------------------------
 template <int CT>
@@ -1124,6 +1215,23 @@ int ffno_head_bwd(FfnoRows rows, const float* h, const uint16_t* p1, const uint1
     if (rc) return rc;
   });
   return check_launch("ffno_head_bwd");
+}
+
+int ffno_zgrad_columns(const FfnoZGrad* g, hipStream_t stream) {
+  if (!g || g->B <= 0 || g->hw <= 0 || g->D <= 0 || g->R <= 0 || g->C % 16 || g->d_pad < g->D || g->d_pad % 16 ||
+      g->r_pad < g->R || g->r_pad % 16 || !g->xv || !g->xbar_spec || !g->yv_bar || !g->y_spec || !g->tables ||
+      !g->zbar)
+    return fail("ffno_zgrad_columns: bad arguments");
+  const int64_t cols = (int64_t)g->B * g->hw;
+  const int64_t bx = (cols + 3) / 4;
+  const size_t lds = (size_t)4 * (g->d_pad + 2) * sizeof(float);
+  if (lds > 160 * 1024) return fail("ffno_zgrad_columns: column too deep");
+  FFNO_DISPATCH_CT(g->C, {
+    auto k = k_zgrad_cols<CT>;
+    if (set_lds(k, lds)) return -1;
+    k<<<(unsigned)(bx < 8192 ? bx : 8192), 256, lds, stream>>>(*g);
+  });
+  return check_launch("ffno_zgrad_columns");
 }
 
 int ffno_dot_reduce(FfnoRows rows, int D, const float* a, int delta, const float* r, float* zbar, hipStream_t stream) {

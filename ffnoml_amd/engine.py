@@ -518,10 +518,24 @@ class Plan:
                 self._height = dict(
                     bands={d: T.split_bf16(bands[d], self.device) for d in (-1, 0, 1)},
                     v_dz=T.split_bf16(v_dz, self.device),
+                    columns=self._zgrad_tables(bands, v_dz),
                     v0=T.split_bf16(np.ascontiguousarray(v0.T), self.device) if np.any(dfac != 0) else None,
                     dfac=torch.from_numpy(dfac).float().to(self.device), i_med=i_med, sign=sgn, argmin=ties)
                 _settle(self.device)
             return self._height
+
+    def _zgrad_tables(self, bands, v_dz):
+        """The tables of ffno_zgrad_columns, float32 [4][R_pad][D_pad] on the device: [delta + 1][r][d'] =
+        G_delta[d' - delta, r] (zero where d' - delta leaves the column), [3][r][d] = V_dz[d, r]; (tables, R_pad,
+        D_pad)."""
+        D, R = v_dz.shape
+        rp, dp = T.round_up(R, 16), T.round_up(D, 16)
+        tab = np.zeros((4, rp, dp))
+        for k, delta in enumerate((-1, 0, 1)):
+            lo, hi = max(0, delta), min(D, D + delta)          # levels d' whose d' - delta lies in the column
+            tab[k, :R, lo:hi] = bands[delta][lo - delta:hi - delta, :].T
+        tab[3, :R, :D] = v_dz.T
+        return torch.from_numpy(tab).float().to(self.device), rp, dp
 
     def prepare(self, adjoint=False):
         """Build now what a call would otherwise build on first use (the kept per-mode tables of every block, and for
@@ -1523,13 +1537,21 @@ class Engine:
                 _lib.call("ffno_block_in", rows, _asm(None, h, ones, zeros), None, _p(blk["spec.g1"]),
                           _p(blk["spec.g2"]), _p(blk["vertical.g1"]), _p(blk["vertical.g2"]), _p(blk["z_proj"]),
                           _p(bz), _p(sp), _p(fp), st)
-                # Filon weights: zbar_s[e] += sum_delta sum_c xv[e + delta] (G_delta X_bar)[e]   (x1 as scratch)
-                for delta in (-1, 0, 1):
-                    self._vgemm(plan, ht["bands"][delta], v1, x1, False)
-                    _lib.call("ffno_dot_reduce", rows, D, _p(fp), delta, _p(x1), _p(zbar_s), st)
-                # synthesis phases: zbar_s[d] += sum_c yv_bar[d] (V_dz Y)[d]
-                self._vgemm(plan, ht["v_dz"], v1y, x1, False)
-                _lib.call("ffno_dot_reduce", rows, D, _p(v), 0, _p(x1), _p(zbar_s), st)
+                if "columns" in ht:
+                    # Filon weights and synthesis phases per column: zbar_s[d] += sum_delta sum_r G_delta[d, r]
+                    # (xv X_bar^T)[d + delta, r] + sum_r V_dz[d, r] (yv_bar Y^T)[d, r]
+                    tab, rp, dp = ht["columns"]
+                    zg = _lib.FfnoZGrad(B, H * W, D, 2 * M, C, rp, dp, _p(fp), _p(v1), _p(v), _p(v1y), _p(tab),
+                                        _p(zbar_s))
+                    _lib.call("ffno_zgrad_columns", ctypes.byref(zg), st)
+                else:
+                    # Filon weights: zbar_s[e] += sum_delta sum_c xv[e + delta] (G_delta X_bar)[e]   (x1 as scratch)
+                    for delta in (-1, 0, 1):
+                        self._vgemm(plan, ht["bands"][delta], v1, x1, False)
+                        _lib.call("ffno_dot_reduce", rows, D, _p(fp), delta, _p(x1), _p(zbar_s), st)
+                    # synthesis phases: zbar_s[d] += sum_c yv_bar[d] (V_dz Y)[d]
+                    self._vgemm(plan, ht["v_dz"], v1y, x1, False)
+                    _lib.call("ffno_dot_reduce", rows, D, _p(v), 0, _p(x1), _p(zbar_s), st)
             # input gates (and z_proj): h_bar in g
             _lib.call("ffno_gates_bwd", rows, _p(h), _p(mp), _p(pp), _p(g), _p(blk["spec.g1"]), _p(blk["spec.g2"]),
                       _p(adj["spec.g2t"]), _p(adj["spec.g1t"]), _p(blk["vertical.g1"]), _p(blk["vertical.g2"]),

@@ -299,6 +299,28 @@ int ffno_head_bwd(FfnoRows rows, const float* h, const uint16_t* p1, const uint1
  * the channel contraction of the height gradient of the vertical branch (models/ffno_model.py:272-308). */
 int ffno_dot_reduce(FfnoRows rows, int D, const float* a, int delta, const float* r, float* zbar, hipStream_t stream);
 
+/* The same contractions for the three Filon bands and the synthesis phases at once, per column of a shared height
+ * grid (models/ffno_model.py:272-308), without the band GEMMs' outputs:
+ *   zbar[b, d, hw] += sum_{delta=-1,0,1} sum_r G_delta[d, r] P[d + delta, r] + sum_r V_dz[d, r] Q[d, r],
+ *   P[d', r] = sum_c xv[b, d', hw, c] xbar_spec[b, r, hw, c],  Q[d, r] = sum_c yv_bar[b, d, hw, c] y_spec[b, r, hw, c],
+ * with tables [4][r_pad][d_pad] float32: tables[delta + 1][r][d'] = G_delta[d' - delta, r] (zero where d' - delta
+ * is outside [0, D)), tables[3][r][d] = V_dz[d, r]; r_pad, d_pad multiples of 16, zero beyond R, D. */
+typedef struct {
+  int B;
+  int64_t hw;             /* columns per batch item (H * W) */
+  int D;                  /* levels */
+  int R;                  /* spectrum rows, 2M */
+  int C;
+  int r_pad, d_pad;
+  const float* xv;        /* [B, D, HW, C] the vertical branch input */
+  const float* xbar_spec; /* [B, R, HW, C] the gradient of its Filon spectrum */
+  const float* yv_bar;    /* [B, D, HW, C] the gradient of the branch output */
+  const float* y_spec;    /* [B, R, HW, C] the spectrum after the mix */
+  const float* tables;
+  float* zbar;            /* [B, D, HW], accumulated */
+} FfnoZGrad;
+int ffno_zgrad_columns(const FfnoZGrad* g, hipStream_t stream);
+
 /* Adjoint of the lift (models/ffno_model.py:782-783): xbar_in += W_lift^T (GN backward of ubar). */
 int ffno_lift_bwd(FfnoRows rows, const float* ubar, FfnoGnBack gn, const float* w_lift, float* xbar_in,
                   hipStream_t stream);

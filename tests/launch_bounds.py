@@ -366,6 +366,20 @@ class Checker:
         self.act(n, "r", r, rows)
         self.span(n, "zbar", zbar, rows.B * rows.voxels * F32)
 
+    def ffno_zgrad_columns(self, n, g, stream):
+        g = g._obj
+        assert g.d_pad % 16 == 0 and g.r_pad % 16 == 0 and g.d_pad >= g.D and g.r_pad >= g.R and g.C % 16 == 0
+        KP = round_up(g.C, 32)
+        # per column the kernel reads channels up to the last 32-block's end only where they exist (c0 < C)
+        lev = g.B * g.D * g.hw * g.C * F32
+        spec = g.B * g.R * g.hw * g.C * F32
+        assert KP - 32 < g.C
+        for what, p, nb in (("xv", g.xv, lev), ("xbar_spec", g.xbar_spec, spec), ("yv_bar", g.yv_bar, lev),
+                            ("y_spec", g.y_spec, spec)):
+            self.span(n, what, p, nb)
+        self.span(n, "tables", g.tables, 4 * g.r_pad * g.d_pad * F32)
+        self.span(n, "zbar", g.zbar, g.B * g.D * g.hw * F32)
+
     def ffno_lift_bwd(self, n, rows, ubar, gb, w, xbar_in, stream):
         self.act(n, "ubar", ubar, rows)
         self.gnback(n, gb, rows)
