@@ -154,11 +154,13 @@ __global__ __launch_bounds__(NW * 64) void k_block_in(FfnoRows rows, FfnoAssembl
     Frag3 xf[KS];
     to_frags<CT>(x, xf);
     f32x4 g[CT];
-    // horizontal gate (models/ffno_model.py:439-440)
-    mlp2<KS, CT, CT>(G1, G2, xf, g, L.v, L.q);
+    // horizontal gate (models/ffno_model.py:439-440); xs NULL: the vertical branch's input alone
+    if (xs) {
+      mlp2<KS, CT, CT>(G1, G2, xf, g, L.v, L.q);
 #pragma unroll
-    for (int t = 0; t < CT; ++t) g[t] = x[t] * sigm4(g[t]);
-    store_act<CT>(xs, row, ok, L.q, g);
+      for (int t = 0; t < CT; ++t) g[t] = x[t] * sigm4(g[t]);
+      store_act<CT>(xs, row, ok, L.q, g);
+    }
     // vertical gate and height projection (models/ffno_model.py:266)
     mlp2<KS, CT, CT>(V1, V2, xf, g, L.v, L.q);
     const float zv = ok ? z[row] : 0.f;
@@ -885,7 +887,7 @@ int ffno_gn_finalize(int B, int nwg, int C, const float* partials, const float* 
 int ffno_block_in(FfnoRows rows, FfnoAssemble in, float* x_out, const uint16_t* g1, const uint16_t* g2,
                   const uint16_t* v1, const uint16_t* v2, const float* z_proj, const float* z, float* xs, float* xv,
                   hipStream_t stream) {
-  if (!rows_ok(rows) || !in.t || !in.aff_a || !in.aff_b || !g1 || !g2 || !v1 || !v2 || !z_proj || !z || !xs || !xv)
+  if (!rows_ok(rows) || !in.t || !in.aff_a || !in.aff_b || !g1 || !g2 || !v1 || !v2 || !z_proj || !z || !xv)
     return fail("ffno_block_in: bad arguments");
   dim3 grid(rows.nwg, rows.B);
   FFNO_DISPATCH_CT(rows.C, {

@@ -1533,10 +1533,10 @@ class Engine:
                     dfac_col += (gsum * ht["dfac"][None, :, None]).sum(1)
             self.vertical(plan, v, v1, pp, i, tables, adjoint=True)
             if want_z:
-                # the vertical branch's input again (xs -> sp, xv -> fp, both dead here)
+                # the vertical branch's input again, into fp (dead here)
                 _lib.call("ffno_block_in", rows, _asm(None, h, ones, zeros), None, _p(blk["spec.g1"]),
                           _p(blk["spec.g2"]), _p(blk["vertical.g1"]), _p(blk["vertical.g2"]), _p(blk["z_proj"]),
-                          _p(bz), _p(sp), _p(fp), st)
+                          _p(bz), None, _p(fp), st)
                 if "columns" in ht:
                     # Filon weights and synthesis phases per column: zbar_s[d] += sum_delta sum_r G_delta[d, r]
                     # (xv X_bar^T)[d + delta, r] + sum_r V_dz[d, r] (yv_bar Y^T)[d, r]

@@ -64,7 +64,7 @@ int ffno_gn_finalize(int B, int nwg, int C, const float* partials, const float* 
 
 /* Block entry (models/ffno_model.py:536-542, :266, :439-440): x = assemble(...); writes x, then both input gates:
  * xs = x * sigmoid(G2 gelu(G1 x)), xv = x * sigmoid(V2 gelu(V1 x)) + z_proj * z. x_out may be NULL, or alias
- * in.base; xs may alias in.t (each row is read before it is written). */
+ * in.base; xs may alias in.t (each row is read before it is written), or be NULL (xv alone). */
 int ffno_block_in(FfnoRows rows, FfnoAssemble in, float* x_out, const uint16_t* g1, const uint16_t* g2,
                   const uint16_t* v1, const uint16_t* v2, const float* z_proj /* [C] */, const float* z /* [B,DHW] */,
                   float* xs, float* xv, hipStream_t stream);

@@ -128,7 +128,7 @@ class Checker:
             self.weight(n, what, w, C, C, parts=3)
         self.span(n, "z_proj", z_proj, C * F32)
         self.span(n, "z", z, rows.B * rows.voxels * F32)
-        self.act(n, "xs", xs, rows)
+        self.act(n, "xs", xs, rows, optional=True)
         self.act(n, "xv", xv, rows)
 
     def ffno_post(self, n, rows, u, w1, w2, y, st, stream):
