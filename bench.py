@@ -105,6 +105,30 @@ def kernel_of(name, a):
     return "k_" + name[5:]
 
 
+def jvp_leg(model, x, z, dev, w, voxels, steps):
+    """The tangent (JVP) at the benchmark shape: forward and directional derivative along a random input tangent, as
+    the inversion's JVP requests run it (Engine.jvp, the hand-written tangent kernels)."""
+    import torch
+    from ffnoml_amd import synthetic
+    eng = model.engine()
+    tan = synthetic.perturbation(tuple(x.shape), seed=1).to(dev)
+    with torch.no_grad():
+        for _ in range(2):
+            eng.jvp(x, z, 48e3, 48e3, tan)
+        torch.cuda.synchronize()
+        j0, j1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        nj = max(steps, 3)
+        j0.record()
+        for _ in range(nj):
+            eng.jvp(x, z, 48e3, 48e3, tan)
+        j1.record()
+        torch.cuda.synchronize()
+    jms = j0.elapsed_time(j1) / nj
+    return {"workload": f"FFNO3D forward + JVP (tangent), B=8 D=128 H=W=64 width={w['width']} layers=8",
+            "ms_per_jvp": round(jms, 3), "value": round(voxels / (jms * 1e-3) / 1e6, 3), "unit": "Mvoxels/s",
+            "calls": nj}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,6 +156,8 @@ def main():
                          "metric) or, for profiling, the production width (96, 1001 vertical modes) on the same tile")
     ap.add_argument("--only-volume", action="store_true",
                     help="run the whole-volume leg alone (for profiling it) and print its record")
+    ap.add_argument("--only-jvp", action="store_true",
+                    help="run the forward + JVP leg alone at the benchmark shape (for profiling it) and print its record")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -142,7 +168,7 @@ def main():
         pmc_child()
         return
     # the counter runs go first, as child processes, before this process touches the GPU
-    counters = None if args.no_traffic or args.only_volume else measure_counters()
+    counters = None if args.no_traffic or args.only_volume or args.only_jvp else measure_counters()
 
     import torch
     from ffnoml_amd import _lib
@@ -159,6 +185,9 @@ def main():
     w = WORKLOAD if args.workload == "benchmark" else PRODUCTION
     voxels = w["B"] * w["D"] * w["H"] * w["W"]
     model, x, z, ct = make(w, dev)
+    if args.only_jvp:
+        print(json.dumps({"jvp": jvp_leg(model, x, z, dev, w, voxels, args.steps)}), flush=True)
+        return
 
     def step():
         xg = x.detach().requires_grad_(True)
@@ -277,28 +306,8 @@ def main():
                                "unit": "Mvoxels/s", "forwards": nf}
         del yf
         log(f"forward {fms:.1f} ms, {out['forward_only']['value']:.2f} Mvox/s")
-        # the tangent (JVP) at the same shape: forward and directional derivative along a random input tangent, as
-        # the inversion's JVP requests run it (Engine.jvp, the hand-written tangent kernels)
-        from ffnoml_amd import synthetic
-        eng = model.engine()
-        tan = synthetic.perturbation(tuple(x.shape), seed=1).to(dev)
-        with torch.no_grad():
-            for _ in range(2):
-                eng.jvp(x, z, 48e3, 48e3, tan)
-            torch.cuda.synchronize()
-            j0, j1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            nj = max(args.steps, 3)
-            j0.record()
-            for _ in range(nj):
-                eng.jvp(x, z, 48e3, 48e3, tan)
-            j1.record()
-            torch.cuda.synchronize()
-        jms = j0.elapsed_time(j1) / nj
-        out["jvp"] = {"workload": f"FFNO3D forward + JVP (tangent), B=8 D=128 H=W=64 width={w['width']} layers=8",
-                      "ms_per_jvp": round(jms, 3), "value": round(voxels / (jms * 1e-3) / 1e6, 3),
-                      "unit": "Mvoxels/s", "calls": nj}
-        del tan
-        log(f"jvp {jms:.1f} ms, {out['jvp']['value']:.2f} Mvox/s")
+        out["jvp"] = jvp_leg(model, x, z, dev, w, voxels, args.steps)
+        log(f"jvp {out['jvp']['ms_per_jvp']:.1f} ms, {out['jvp']['value']:.2f} Mvox/s")
 
     if not args.no_baselines:
         from baseline.plain_fp32 import PlainFFNO3D

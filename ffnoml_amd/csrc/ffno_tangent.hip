@@ -348,7 +348,7 @@ __device__ __forceinline__ void gn_gelu_t(const float* k, bool ok, int q, int t,
 }
 
 template <int CT, int NW>
-__global__ __launch_bounds__(NW * 64) void k_fuse_t(FfnoRows rows, const float* __restrict__ h,
+__global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(CT <= 4 ? 2 : 1))) void k_fuse_t(FfnoRows rows, const float* __restrict__ h,
                                                     const float* __restrict__ hd, FfnoAssembleT st, FfnoAssembleT vt,
                                                     const float* __restrict__ gates, const uint16_t* bg1,
                                                     const uint16_t* bg2, const uint16_t* f1, const uint16_t* f2,
@@ -357,9 +357,11 @@ __global__ __launch_bounds__(NW * 64) void k_fuse_t(FfnoRows rows, const float* 
   // the two branch assemblies' per-channel constants of this batch item, then the tangent sums' slots
   __shared__ __align__(16) float kc[8 * C];
   __shared__ __align__(16) float red[tan_sums_floats<CT, NW>()];
+  extern __shared__ __align__(16) unsigned char smem[];   // the waves' stashes: [NW][4 CT][64 lanes] float4
   const Wm B1 = global_w(bg1, C * wld(3 * KP)), B2 = global_w(bg2, 2 * C * wld(KP));
   const Wm F1 = global_w(f1, 2 * C * wld(2 * KP)), F2 = global_w(f2, C * wld(2 * C));
   Lane L;
+  f32x4* stash = reinterpret_cast<f32x4*>(smem) + (int64_t)L.wave * 4 * CT * 64 + L.lane;
   const int b = blockIdx.y;
   for (int c = threadIdx.x; c < C; c += blockDim.x) {
     const int e = b * C + c;
@@ -393,7 +395,8 @@ __global__ __launch_bounds__(NW * 64) void k_fuse_t(FfnoRows rows, const float* 
       gn_gelu_t<CT>(kc, ok, L.q, t, s[t], ds[t]);
       gn_gelu_t<CT>(kc + 4 * C, ok, L.q, t, v[t], dv[t]);
     }
-    // gate pre-activation tangent: BG2 (gelu'(BG1 [h, s, v]) * BG1 [hd, sd, vd])
+    // gate pre-activation tangent: BG2 (gelu'(BG1 [h, s, v]) * BG1 [hd, sd, vd]); s, v, sd, vd wait in the wave's
+    // LDS stash meanwhile (held in registers beside the six input fragments they would leave one wave per SIMD)
     f32x4 gd[2 * CT];
     {
       Frag cat[3 * KS], catd[3 * KS];
@@ -406,7 +409,23 @@ __global__ __launch_bounds__(NW * 64) void k_fuse_t(FfnoRows rows, const float* 
       to_frags<CT>(ds, catd + KS);
       to_frags<CT>(v, cat + 2 * KS);
       to_frags<CT>(dv, catd + 2 * KS);
+#pragma unroll
+      for (int t = 0; t < CT; ++t) {
+        stash[t * 64] = s[t];
+        stash[(CT + t) * 64] = v[t];
+        stash[(2 * CT + t) * 64] = ds[t];
+        stash[(3 * CT + t) * 64] = dv[t];
+      }
+      asm volatile("" ::: "memory");
       mlp2_t<3 * KS, CT, 2 * CT>(B1, B2, cat, catd, gd, L.v, L.q);
+    }
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int t = 0; t < CT; ++t) {
+      s[t] = stash[t * 64];
+      v[t] = stash[(CT + t) * 64];
+      ds[t] = stash[(2 * CT + t) * 64];
+      dv[t] = stash[(3 * CT + t) * 64];
     }
     // sm_dot = sd gs + s gs (1 - gs) gd_s, likewise vm; then the primal sm, vm
 #pragma unroll
@@ -612,8 +631,12 @@ int ffno_fuse_t(FfnoRows rows, const float* h, const float* hd, FfnoAssembleT st
       !f2 || !fpd || (ts.partials && (!ts.y || !ts.stats || !ts.depth_w)))
     return fail("ffno_fuse_t: bad arguments");
   dim3 grid(rows.nwg, rows.B);
-  FFNO_DISPATCH_CT(rows.C, (k_fuse_t<CT, NWV><<<grid, NWV * 64, 0, stream>>>(rows, h, hd, st, vt, gates, bg1, bg2, f1,
-                                                                           f2, fpd, ts)));
+  FFNO_DISPATCH_CT(rows.C, {
+    const size_t lds = (size_t)NWV * 4 * CT * 64 * sizeof(f32x4);
+    auto k = k_fuse_t<CT, NWV>;
+    if (set_lds(k, lds)) return -1;
+    k<<<grid, NWV * 64, lds, stream>>>(rows, h, hd, st, vt, gates, bg1, bg2, f1, f2, fpd, ts);
+  });
   return check_launch("ffno_fuse_t");
 }
 
