@@ -617,6 +617,7 @@ def service_leg(model, dev, n, n_plain, slots=3, log_path=None):
         res["allocator"] = {k: int(stats1.get(k, 0) - stats0.get(k, 0)) for k in
                             ("num_alloc_retries", "num_sync_all_streams", "num_device_alloc", "num_device_free")}
         if log_path:
+            os.makedirs(os.path.dirname(os.path.abspath(log_path)), exist_ok=True)
             with open(log_path, "w") as fh:
                 json.dump({"t0": w0, "batches": batcher.batch_log, "queue": batcher.queue_samples,
                            "requests": [(q.t_submit, q.t_start, q.t_done, q.op, q.voxels) for q in done]}, fh)
