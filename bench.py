@@ -139,7 +139,7 @@ def main():
     ap.add_argument("--no-forward-only", action="store_true", help="skip the forward-only leg (configs[1])")
     ap.add_argument("--no-production", action="store_true",
                     help="skip the production-width leg (width 96, 1001 vertical modes, on the benchmark tile)")
-    ap.add_argument("--service-requests", type=int, default=160,
+    ap.add_argument("--service-requests", type=int, default=400,
                     help="requests of the serving workload (Poisson arrivals at 40/s, seed 1; 0 skips it; the full "
                          "workload is 2000)")
     ap.add_argument("--service-slots", type=int, default=3, help="execution slots of the serving leg's batcher")
