@@ -94,6 +94,20 @@ def horizontal_factors(H, W):
     return fh, fw
 
 
+def zgrad_tables(bands, v_dz):
+    """The tables of ffno_zgrad_columns from one column's Filon bands {delta: G_delta [D, 2M]} and synthesis
+    derivative V_dz [D, 2M] (host float64): [4][R_pad][D_pad] with [delta + 1][r][d'] = G_delta[d' - delta, r] (zero
+    where d' - delta leaves the column) and [3][r][d] = V_dz[d, r]; R_pad, D_pad = 2M, D rounded up to 16."""
+    D, R = v_dz.shape
+    rp, dp = T.round_up(R, 16), T.round_up(D, 16)
+    tab = np.zeros((4, rp, dp))
+    for k, delta in enumerate((-1, 0, 1)):
+        lo, hi = max(0, delta), min(D, D + delta)          # levels d' whose d' - delta lies in the column
+        tab[k, :R, lo:hi] = bands[delta][lo - delta:hi - delta, :].T
+    tab[3, :R, :D] = v_dz.T
+    return tab
+
+
 def horizontal_planes(H, W, C):
     """(floats of one (b, d) plane's half spectrum, floats per plane of the second horizontal scratch buffer: the
     half spectrum, or with W factorised the larger of that and its full-length complex intermediate)."""
@@ -525,17 +539,9 @@ class Plan:
             return self._height
 
     def _zgrad_tables(self, bands, v_dz):
-        """The tables of ffno_zgrad_columns, float32 [4][R_pad][D_pad] on the device: [delta + 1][r][d'] =
-        G_delta[d' - delta, r] (zero where d' - delta leaves the column), [3][r][d] = V_dz[d, r]; (tables, R_pad,
-        D_pad)."""
-        D, R = v_dz.shape
-        rp, dp = T.round_up(R, 16), T.round_up(D, 16)
-        tab = np.zeros((4, rp, dp))
-        for k, delta in enumerate((-1, 0, 1)):
-            lo, hi = max(0, delta), min(D, D + delta)          # levels d' whose d' - delta lies in the column
-            tab[k, :R, lo:hi] = bands[delta][lo - delta:hi - delta, :].T
-        tab[3, :R, :D] = v_dz.T
-        return torch.from_numpy(tab).float().to(self.device), rp, dp
+        """zgrad_tables on the device in float32: (tables, R_pad, D_pad)."""
+        tab = zgrad_tables(bands, v_dz)
+        return torch.from_numpy(tab).float().to(self.device), tab.shape[1], tab.shape[2]
 
     def prepare(self, adjoint=False):
         """Build now what a call would otherwise build on first use (the kept per-mode tables of every block, and for
