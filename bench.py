@@ -625,6 +625,11 @@ def service_leg(model, dev, n, n_plain, slots=3, log_path=None):
     finally:
         batcher.close()
     if n_plain:
+        # the engine's plans and per-mode tables of the trace's shapes go too: the plain path needs the memory
+        import gc
+        model.engine().clear_caches()
+        gc.collect()
+        torch.cuda.empty_cache()
         res["plain_fp32_one_at_a_time"] = plain_service(model, dev, trace[:n_plain], inputs[:n_plain], backend)
         log(f"plain: {res['plain_fp32_one_at_a_time']}")
         sub = [(q.t_done - q.t_submit) * 1e3 for q in done[:n_plain]]
@@ -669,17 +674,28 @@ def queue_summary(samples, t0, points=20):
 def plain_service(model, dev, trace, inputs, backend, budget_s=90.0):
     """One request at a time, in arrival order, through the plain FP32 GPU path: predict = SI chain + forward under
     no_grad; VJP = forward + torch.autograd.grad of <cotangent, populations> (ffno_fsdp_runtime.py:294-348). Stops
-    after budget_s seconds (a bounded sample: the requests answered by then)."""
+    after budget_s seconds, or before a request that the rate of the same operation's requests so far says would end
+    past it (a bounded sample: the requests answered by then)."""
     import numpy as np
     import torch
     from baseline.plain_fp32 import PlainFFNO3D
     plain = PlainFFNO3D(model).to(dev)
     lat, vox = [], 0
+    rate = {}                                   # op -> (seconds, voxels) of its answered requests
     start = time.perf_counter()
     for r, (f, z, cot) in zip(trace, inputs):
         now = time.perf_counter() - start
+        n_vox = r["D"] * r["H"] * r["W"]
+        if r["op"] in rate:
+            sec, done_vox = rate[r["op"]]
+            if max(now, r["t"]) + sec / done_vox * n_vox > budget_s:
+                print(f"service: plain sample stops before request {len(lat) + 1} ({r['op']} {r['D']}x{r['H']}x"
+                      f"{r['W']}: about {sec / done_vox * n_vox:.0f} s at this path's rate)", file=sys.stderr,
+                      flush=True)
+                break
         if now < r["t"]:
             time.sleep(r["t"] - now)
+        t_req = time.perf_counter()
         x_si = torch.from_numpy(f).to(dev)[None]
         z_si = torch.from_numpy(z).to(dev)[None]
         if r["op"] == "PREDICT":
@@ -697,7 +713,9 @@ def plain_service(model, dev, trace, inputs, backend, budget_s=90.0):
             gx.cpu(), gz.cpu()
         torch.cuda.synchronize()
         lat.append((time.perf_counter() - start - r["t"]) * 1e3)
-        vox += r["D"] * r["H"] * r["W"]
+        vox += n_vox
+        sec, done_vox = rate.get(r["op"], (0.0, 0))
+        rate[r["op"]] = (sec + time.perf_counter() - t_req, done_vox + n_vox)
         print(f"service: plain request {len(lat)} ({r['op']} {r['D']}x{r['H']}x{r['W']}) answered, "
               f"{time.perf_counter() - start:.1f} s", file=sys.stderr, flush=True)
         if time.perf_counter() - start > budget_s:

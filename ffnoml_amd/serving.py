@@ -241,6 +241,14 @@ class ContinuousBatcher:
         self.plans.put(None)                            # each later thread ends after the batches before it
         for t in (self.planner, self.executor, self.finisher):
             t.join()
+        # release the device memory the batcher holds: its captured graphs (their pools and static buffers), then the
+        # reserved segment and every other cached block
+        with self.graph_lock:
+            self.graphs.clear()
+            self.graph_held = 0
+        if self.cuda:
+            torch.cuda.synchronize(self.device)
+            torch.cuda.empty_cache()
 
     def latencies_ms(self):
         return np.array([(r.t_done - r.t_submit) * 1e3 for r in self.completed])
