@@ -136,7 +136,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-baselines", action="store_true", help="skip the plain FP32 GPU and CPU legs")
     ap.add_argument("--no-volume", action="store_true", help="skip the whole-volume 504 x 504 x 256 forward + VJP")
-    ap.add_argument("--no-forward-only", action="store_true", help="skip the forward-only leg (configs[1])")
+    ap.add_argument("--no-forward-only", action="store_true", help="skip the forward-only leg at the benchmark shape (and the tangent leg with it)")
     ap.add_argument("--no-production", action="store_true",
                     help="skip the production-width leg (width 96, 1001 vertical modes, on the benchmark tile)")
     ap.add_argument("--service-requests", type=int, default=400,

@@ -1,7 +1,7 @@
 """End-to-end forward parity on the GPU: ffnoml_amd.FFNO3D (HIP engine) against the reference's own float64 outputs
 (golden vectors) and against the float64 oracle.
 
-Tolerance: max over every output of |log10 n - log10 n_ref| / max(|log10 n_ref|, 1) <= 1e-3,
+Tolerance (the accuracy target): max over every output of |log10 n - log10 n_ref| / max(|log10 n_ref|, 1) <= 1e-3,
 on de-normalised log10 populations."""
 import json
 import os

@@ -3,7 +3,7 @@ ffnoml_amd.FFNO3D, against forward-mode autograd through the float64 oracle; the
 dot-product identity <u, J t> = <J^T u, t> at the benchmark's width and depth; and GPUBackend.jvp_local (SI units)
 against the oracle's SI chain.
 
-Tolerance (the north-star bar for derivatives): 1e-2 relative, ||g - g_ref|| / ||g_ref||."""
+Tolerance (the accuracy target for derivatives): 1e-2 relative, ||g - g_ref|| / ||g_ref||."""
 import numpy as np
 import pytest
 import torch

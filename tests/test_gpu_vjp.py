@@ -2,7 +2,7 @@
 float64 autograd gradients (golden vectors), against the float64 oracle's autograd, and against float64 central
 finite differences of the oracle.
 
-Tolerance (the north-star bar for gradients): relative error <= 1e-2, measured as ||g - g_ref|| / ||g_ref|| over the
+Tolerance (the accuracy target for gradients): relative error <= 1e-2, measured as ||g - g_ref|| / ||g_ref|| over the
 whole gradient and as max |g - g_ref| / max |g_ref|."""
 import numpy as np
 import pytest

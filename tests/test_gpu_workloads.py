@@ -2,7 +2,7 @@
 tests/golden/make_workload_fixtures.py (inputs and weights regenerate from their seeds; each fixture's weight digest
 is checked first).
 
-Bars (north star): <= 1e-3 relative in log10 n, max |d log10 n| / max(|log10 n|, 1) over the stored subsample;
+Bars (the accuracy targets): <= 1e-3 relative in log10 n, max |d log10 n| / max(|log10 n|, 1) over the stored subsample;
 <= 1e-2 relative in gradients, ||g - g_ref|| / ||g_ref|| over the stored subsample, and against the float64 central
 differences max_i |g_i - fd_i| / max_i |fd_i|. Sums over the whole output check the entries between the subsample."""
 import hashlib
