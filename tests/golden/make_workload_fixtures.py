@@ -175,7 +175,7 @@ def make_cfg2(scratch):
         print(f"fd {j + 1}/{FD_ENTRIES} {tuple(int(v) for v in idx[j])}: fd {fd[j]:.8e} autograd {ad:.8e} "
               f"({time.time() - t1:.0f} s)", flush=True)
     ad = np.array([grad[(0,) + tuple(i)] for i in idx])
-    save("wl_cfg2_gradient", grad=sub(grad), **sums("grad", grad), fd_index=idx, fd_value=fd, fd_autograd=ad,
+    save("wl_cfg2_gradient", grad=sub(grad[0]), **sums("grad", grad), fd_index=idx, fd_value=fd, fd_autograd=ad,
          fd_step=np.array(FD_STEP), loss=np.array(float(loss)), state_sha256=np.array(state_digest(state)),
          config_json=np.array(json.dumps(CFG1)), shape=np.array(SHAPE1), item=np.array(0),
          stride=np.array(STRIDE))
