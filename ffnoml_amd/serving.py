@@ -35,10 +35,13 @@ the GPU.
 Forward-only batches of small tiles (forward working set at most `graph_bytes`) replay a captured HIP graph of the
 whole forward: a batch of n requests runs on the graph of its bucket for the next power of two at or above n (the
 other samples of the static input hold earlier requests' inputs; GroupNorm statistics are per sample, so they change
-nothing), which the executor captures after running the first such batch eagerly. The capture holds the device alone (HIP refuses another thread's copies while a stream is capturing): the
-planner's and the finisher's device work waits for it. The memory a graph holds is counted against the budget;
-graphs are evicted least recently replayed first, and an evicted graph's memory is released from the count once its
-last replay is enqueued.
+nothing), which the executor captures after running the first such batch eagerly and before that batch's results are
+handed out. The capture holds the device alone (HIP refuses another thread's copies while a stream is capturing): the
+planner's and the finisher's device work waits for it. A caller in the same process that launches GPU work from a
+thread of its own while a capture runs (a request's results arrive only after its batch's capture, so a client waiting
+on its own requests never does) can meet torch's capture-state errors, e.g. from its RNG. The memory a graph holds is
+counted against the budget; graphs are evicted least recently replayed first, and an evicted graph's memory is
+released from the count once its last replay is enqueued.
 
 Admission never stalls: when the oldest request does not fit the unclaimed budget while nothing is in the pipeline,
 the graphs and the engines' unused tables are dropped; a request that still does not fit is failed with MemoryError.
@@ -436,9 +439,11 @@ class ContinuousBatcher:
                     self._fail(b, e)
                     continue
                 capture, b.capture = b.capture, None
-                self.finish.put(b)
                 if capture is not None:
+                    # before the batch's results go out: a client woken by them may launch GPU work of its own
+                    # (torch's RNG refuses to advance while a stream of the process is capturing)
                     self._capture(*capture)
+                self.finish.put(b)
         finally:
             self.finish.put(None)
 
