@@ -68,9 +68,10 @@ __device__ __forceinline__ void st4(float* p, f32x4 v, bool ok) { if (ok) *(f32x
 constexpr int ks_of(int ntiles) { return (ntiles + 1) / 2; }   // k-steps of 32 for ntiles 16-feature tiles
 
 // Every packed weight row of K bf16 (K a multiple of 32) is followed by WPAD bf16 of zeros, in global memory and in
-// LDS alike: with a row stride of K + 8 (2K + 16 bytes) the 16 rows an A-fragment read touches start in 16 distinct
-// 16-byte bank groups, where an unpadded stride (a multiple of 64 bytes) lands up to 8 of them on the same banks.
-constexpr int WPAD = 8;
+// LDS alike. The row stride K + 16 (2K + 32 bytes) is the one that reads A-fragments at the full LDS rate on gfx950:
+// timed over 16-row ds_read_b128 fragment reads, pads of 0, 4, 8, 16 and 24 bf16 take 3.5, 14, 1.8, 1 and 1.8 times
+// the time of a pad of 16, at K = 64 and K = 128 alike.
+constexpr int WPAD = 16;
 constexpr int wld(int k) { return k + WPAD; }                   // row stride of a packed weight of K columns
 
 // split-bf16 operands: x = hi + lo with hi = bf16(x), lo = bf16(x - hi); products hi*hi + hi*lo + lo*hi keep about

@@ -24,7 +24,7 @@ SUPPORTED_WIDTHS = (16, 32, 64, 96, 128)
 GROUPS = 8
 ROW_TILES_PER_WG = 24         # 16-voxel tiles per workgroup of the voxel-row kernels, at least (small requests)
 ROW_WGS_PER_CU = 3            # workgroups per CU and batch item, at most (large requests: three rounds of staging)
-WPAD = 8                     # zero bf16 after every packed weight row (the kernels' row stride is K_pad + WPAD)
+WPAD = 16                    # zero bf16 after every packed weight row (the kernels' row stride is K_pad + WPAD)
 TABLE_CACHE_BYTES = 12 << 30  # a branch's per-mode tables of all blocks stay cached when at most this large (a
 #                               256 x 256 tile's horizontal set, 8 blocks at width 64, is 9.4 GiB)
 MODE_CACHE_BYTES = 24 << 30  # all cached per-mode tables of an engine (least recently used sets are dropped)

@@ -87,9 +87,9 @@ class Checker:
         self.span(name, "gn wbar", g.wbar, rows.B * rows.voxels * F32, optional=True)
 
     def weight(self, name, what, ptr, n_rows, k, parts=2):
-        """A packed split-bf16 matrix [parts][n_rows][round_up(k, 32) + 8] as the kernels stage it (rows padded by 8
+        """A packed split-bf16 matrix [parts][n_rows][round_up(k, 32) + WPAD] as the kernels stage it (rows padded by WPAD
         bf16; parts=3: the forward's three-way pointwise layers)."""
-        self.span(name, what, ptr, parts * n_rows * (round_up(k, 32) + 8) * BF)
+        self.span(name, what, ptr, parts * n_rows * (round_up(k, 32) + E.WPAD) * BF)
 
     @staticmethod
     def imap_max(count, split, hi, lo):
@@ -159,10 +159,10 @@ class Checker:
             self.act(n, what, p, rows)
         for what, p in (("s_a", sa), ("s_b", sb), ("v_a", va), ("v_b", vb)):
             self.span(n, what, p, rows.B * C * F32)
-        self.span(n, "bg1", bg1, 2 * C * (3 * round_up(C, 32) + 8) * BF)
+        self.span(n, "bg1", bg1, 2 * C * (3 * round_up(C, 32) + E.WPAD) * BF)
         self.weight(n, "bg2", bg2, 2 * C, C)
         self.span(n, "bg2_bias", bias, 2 * C * F32)
-        self.span(n, "f1", f1, 2 * 2 * C * (2 * round_up(C, 32) + 8) * BF)
+        self.span(n, "f1", f1, 2 * 2 * C * (2 * round_up(C, 32) + E.WPAD) * BF)
         self.weight(n, "f2", f2, C, 2 * C)
         self.stats(n, st, rows)
         self.span(n, "gates_out", gates, rows.B * rows.voxels * 2 * C * F32, optional=True)
@@ -185,7 +185,7 @@ class Checker:
         self.asm(n, a, rows)
         self.act(n, "x_out", x_out, rows, optional=True)
         self.weight(n, "p1", p1, 2 * C, C, parts=3)
-        self.span(n, "p2", p2, 3 * nth * 16 * (2 * C + 8) * BF)
+        self.span(n, "p2", p2, 3 * nth * 16 * (2 * C + E.WPAD) * BF)
         self.span(n, "p2_bias", bias, nth * 16 * F32)
         self.span(n, "x_in", x_in, rows.B * 6 * rows.voxels * F32)
         self.span(n, "y", y, rows.B * n_out * rows.voxels * F32, optional=True)
@@ -236,11 +236,11 @@ class Checker:
         top = self.imap_max(m.rows, m.r_split, m.r_hi, m.r_lo) + (m.modes - 1) * m.mode_stride + C
         self.span(n, "re", m.re, top * F32)
         self.span(n, "im", m.im, top * F32)
-        self.span(n, "wc", m.wc, m.parts * 2 * C * (2 * KP + 8) * BF)
+        self.span(n, "wc", m.wc, m.parts * 2 * C * (2 * KP + E.WPAD) * BF)
         self.span(n, "gate", m.gate, m.modes * 2 * F32)
         if m.rank:
-            self.span(n, "v", m.v_p, m.modes * m.parts * 16 * (KP + 8) * BF)
-            self.span(n, "u", m.u_p, m.modes * m.parts * C * (32 + 8) * BF)
+            self.span(n, "v", m.v_p, m.modes * m.parts * 16 * (KP + E.WPAD) * BF)
+            self.span(n, "u", m.u_p, m.modes * m.parts * C * (32 + E.WPAD) * BF)
         self.span(n, "factor", m.factor, m.modes * F32, optional=True)
 
     def ffno_gn_grad_stats(self, n, rows, gb, ubar, part, stream):
@@ -282,14 +282,14 @@ class Checker:
         self.gnback(n, gb, rows)
         for what, p in (("s_a", sa), ("s_b", sb), ("v_a", va), ("v_b", vb)):
             self.span(n, what, p, rows.B * C * F32)
-        self.span(n, "bg1", bg1, 2 * C * (3 * KP + 8) * BF)
+        self.span(n, "bg1", bg1, 2 * C * (3 * KP + E.WPAD) * BF)
         self.weight(n, "bg2", bg2, 2 * C, C)
         self.span(n, "bg2_bias", bias, 2 * C * F32)
-        self.span(n, "f1", f1, 2 * 2 * C * (2 * KP + 8) * BF)
+        self.span(n, "f1", f1, 2 * 2 * C * (2 * KP + E.WPAD) * BF)
         self.weight(n, "f2t", f2t, 2 * C, C)
-        self.span(n, "f1t", f1t, 2 * 2 * C * (2 * C + 8) * BF)
+        self.span(n, "f1t", f1t, 2 * 2 * C * (2 * C + E.WPAD) * BF)
         self.weight(n, "bg2t", bg2t, C, 2 * C)
-        self.span(n, "bg1t", bg1t, 2 * 3 * C * (KP + 8) * BF)
+        self.span(n, "bg1t", bg1t, 2 * 3 * C * (KP + E.WPAD) * BF)
 
     def ffno_fuse_mlp_bwd(self, n, rows, x1bar, gb, sp, vp, sa, sb, va, vb, gates, f1, f2t, f1t, smbar, vmbar, stream):
         C = rows.C
@@ -301,9 +301,9 @@ class Checker:
         for what, p in (("s_a", sa), ("s_b", sb), ("v_a", va), ("v_b", vb)):
             self.span(n, what, p, rows.B * C * F32)
         self.span(n, "gates", gates, rows.B * rows.voxels * 2 * C * F32)
-        self.span(n, "f1", f1, 2 * 2 * C * (2 * KP + 8) * BF)
+        self.span(n, "f1", f1, 2 * 2 * C * (2 * KP + E.WPAD) * BF)
         self.weight(n, "f2t", f2t, 2 * C, C)
-        self.span(n, "f1t", f1t, 2 * 2 * C * (2 * C + 8) * BF)
+        self.span(n, "f1t", f1t, 2 * 2 * C * (2 * C + E.WPAD) * BF)
 
     def ffno_fuse_gates_bwd(self, n, rows, xh, h, sp, vp, sa, sb, va, vb, gates, bg1, bg2t, bg1t, svs, svv, gs, ps,
                             gv, pv, stream):
@@ -315,9 +315,9 @@ class Checker:
         for what, p in (("s_a", sa), ("s_b", sb), ("v_a", va), ("v_b", vb)):
             self.span(n, what, p, rows.B * C * F32)
         self.span(n, "gates", gates, rows.B * rows.voxels * 2 * C * F32)
-        self.span(n, "bg1", bg1, 2 * C * (3 * KP + 8) * BF)
+        self.span(n, "bg1", bg1, 2 * C * (3 * KP + E.WPAD) * BF)
         self.weight(n, "bg2t", bg2t, C, 2 * C)
-        self.span(n, "bg1t", bg1t, 2 * 3 * C * (KP + 8) * BF)
+        self.span(n, "bg1t", bg1t, 2 * 3 * C * (KP + E.WPAD) * BF)
         assert bool(getattr(ps, "value", ps)) == bool(getattr(pv, "value", pv)), f"{n}: one of two sums targets"
         self.next_sums(n, gs, ps, rows)
         self.next_sums(n, gv, pv, rows)
@@ -329,8 +329,8 @@ class Checker:
             self.span(n, what, p, hidden * C * r * F32)
         for what, p in (("bu", bu), ("bv", bv)):
             self.span(n, what, p, C * r * F32)
-        self.span(n, "v_out", v, modes * parts * 16 * (kc + 8) * BF)
-        self.span(n, "u_out", u, modes * parts * C * (32 + 8) * BF)
+        self.span(n, "v_out", v, modes * parts * 16 * (kc + E.WPAD) * BF)
+        self.span(n, "u_out", u, modes * parts * C * (32 + E.WPAD) * BF)
 
     def ffno_gates_bwd(self, n, rows, h, xsb, xvb, hin, g1, g2, g2t, g1t, v1, v2, v2t, v1t, z_proj, hout, zbar,
                        stream):
@@ -351,7 +351,7 @@ class Checker:
         C = rows.C
         self.act(n, "h", h, rows)
         self.weight(n, "p1", p1, 2 * C, C)
-        self.span(n, "p2", p2, 2 * nth * 16 * (2 * C + 8) * BF)
+        self.span(n, "p2", p2, 2 * nth * 16 * (2 * C + E.WPAD) * BF)
         self.weight(n, "p2t", p2t, 2 * C, n_head)
         self.weight(n, "p1t", p1t, C, 2 * C)
         self.span(n, "p2_bias", bias, nth * 16 * F32)
@@ -453,9 +453,9 @@ class Checker:
             for what, p in (("aff_a", e.aff_a), ("aff_b", e.aff_b), ("tan_b", e.tan_b), ("tan_c", e.tan_c)):
                 self.span(n, f"{tag}.{what}", p, rows.B * C * F32)
         self.span(n, "gates", gates, rows.B * rows.voxels * 2 * C * F32)
-        self.span(n, "bg1", bg1, 2 * C * (3 * KP + 8) * BF)
+        self.span(n, "bg1", bg1, 2 * C * (3 * KP + E.WPAD) * BF)
         self.weight(n, "bg2", bg2, 2 * C, C)
-        self.span(n, "f1", f1, 2 * 2 * C * (2 * KP + 8) * BF)
+        self.span(n, "f1", f1, 2 * 2 * C * (2 * KP + E.WPAD) * BF)
         self.weight(n, "f2", f2, C, 2 * C)
         self.tan_sums(n, ts, rows)
 
@@ -468,7 +468,7 @@ class Checker:
         self.asm(n, a, rows)
         self.act(n, "xd", xd, rows)
         self.weight(n, "p1", p1, 2 * C, C)
-        self.span(n, "p2", p2, 2 * nth * 16 * (2 * C + 8) * BF)
+        self.span(n, "p2", p2, 2 * nth * 16 * (2 * C + E.WPAD) * BF)
         self.span(n, "p2_bias", bias, nth * 16 * F32)
         self.span(n, "x_in", x_in, rows.B * 6 * rows.voxels * F32)
         self.span(n, "xd_in", xd_in, rows.B * 6 * rows.voxels * F32)

@@ -8,6 +8,7 @@ import re
 
 import numpy as np
 import pytest
+from ffnoml_amd import engine as E
 import torch
 
 from ffnoml_amd import transforms as T
@@ -116,7 +117,7 @@ def test_packing_matches_kernel_contraction(N, segments):
     w = torch.from_numpy(rng.standard_normal((N, K))).float()
     packed = pack_mlp_weight(w, segments).view(torch.bfloat16).double()
     kp = sum(T.round_up(s, 32) for s in segments)
-    assert packed.shape == (2, T.round_up(N, 16), kp + 8)
+    assert packed.shape == (2, T.round_up(N, 16), kp + E.WPAD)
     assert not packed[..., kp:].any()                          # the row padding the kernels' stride skips
     wp = (packed[0] + packed[1]).numpy()                       # hi + lo
     act_parts = [rng.standard_normal(s) for s in segments]
